@@ -1,0 +1,165 @@
+/*
+ * les_amd.h -- C ABI of the MI355X (gfx950) LES hot-path library
+ * (les_amd/libles_amd.so).
+ *
+ * Each routine-level entry point is a drop-in for one external subroutine of
+ * the KhaledGhannam/LES reference (implicit-interface Fortran, arrays passed
+ * by base address).  Arrays are DEVICE pointers in the reference's layout:
+ * real (ld, ny, nlev) column-major, x fastest, ld = 2*(nx/2+1); arrays
+ * dimensioned (ld,ny,1:nz) hold nz levels, p holds nz+1 levels (0:nz).
+ * The module globals the Fortran routines read (grid, kx/ky, G_test,
+ * G_test_test, dt, jt, jt_total, the lagrange_Sdep SAVE flags) live in the
+ * les_ctx.  All calls enqueue on the context's stream and return 0 on
+ * success; a non-zero status means les_last_error() holds the message (the
+ * Fortran shim maps it to `error stop`, where the reference used `stop`).
+ *
+ * Suffix _f32: the same routine on float arrays (FP32-bulk mode).
+ */
+#ifndef LES_AMD_H
+#define LES_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct les_ctx les_ctx;
+
+/* Grid + run constants: param.f90:56-65 (grid), :100 (cs_count), :144-145
+   (mean pressure forcing), :79 (initu, inilag), param.f90:219 (dt),
+   bottombc.f90:53 (zo = zom/z_i). */
+typedef struct les_grid {
+  int nx, ny, nz;
+  double Lx, Ly, Lz; /* nondimensional (L/z_i) */
+  double dt, z0, force;
+  int inilag, initu, cs_count;
+} les_grid;
+
+/* Lagrangian SGS state, sgsmodule.f90:302-326 / sim_param.f90:289 (device
+   pointers; (ld,ny,nz) in the call's precision, Beta_avg/Betaclip_avg (nz)
+   always double). */
+typedef struct les_sgs_fields {
+  void *F_LM, *F_MM, *F_QN, *F_NN, *Beta, *Cs_opt2, *Nu_t, *u_lag, *v_lag, *w_lag;
+  void *TnLMMM, *TnQNNN, *L11t, *L22t, *L33t, *Q11t, *Q22t, *Q33t;
+  double *Beta_avg, *Betaclip_avg;
+} les_sgs_fields;
+
+/* ---- context (replaces init_fft fft.f90:35-51, init_wavenumber
+   fft.f90:54-81 and the two test_filter_init calls main.f90:166-170) ---- */
+int les_create(const les_grid *g, les_ctx **out);
+void les_destroy(les_ctx *c);
+const char *les_last_error(les_ctx *c);
+int les_set_stream(les_ctx *c, void *hip_stream);
+int les_sync(les_ctx *c);
+int les_set_time(les_ctx *c, int jt, int jt_total);        /* param.f90:147-148 */
+int les_get_time(les_ctx *c, int *jt, int *jt_total);
+int les_set_lag_init(les_ctx *c, int flm_init, int fqn_init); /* lagrange_Sdep.f90:67-68 */
+/* host copies of kx, ky, k2 (fft.f90:19), G_test, G_test_test
+   (test_filtermodule.f90:9), each (lh, ny); any pointer may be NULL */
+int les_get_tables(les_ctx *c, double *kx, double *ky, double *k2, double *G_test, double *G_test_test);
+
+/* ---- horizontal spectral operators ---- */
+/* filt_da.f90:3-42  filt_da(f, dfdx, dfdy): f is overwritten (Nyquist killed) */
+int les_filt_da(les_ctx *c, double *f, double *dfdx, double *dfdy, int nlev);
+/* ddx.f90:1-31, ddy.f90:1-31: f untouched */
+int les_ddx(les_ctx *c, double *dfdx, const double *f, int nlev);
+int les_ddy(les_ctx *c, double *dfdy, const double *f, int nlev);
+/* ddxy.f90:1-29 */
+int les_ddxy(les_ctx *c, double *dfdx, double *dfdy, const double *f, int nlev);
+/* test_filtermodule.f90:32-63: G (lh,ny) device array */
+int les_test_filter_init(les_ctx *c, double alpha, double *G);
+/* test_filtermodule.f90:13-30: one (ld,ny) plane filtered in place */
+int les_test_filter(les_ctx *c, double *plane, const double *G);
+/* padd.f90:1-26 / unpadd.f90:1-22: complex (lh,ny) <-> (lh_big,ny2) planes */
+int les_padd(les_ctx *c, double *u_big, const double *u);
+int les_unpadd(les_ctx *c, double *cc, const double *cc_big);
+/* dealias1.f90:1-31 (u untouched) / dealias2.f90:1-33 */
+int les_dealias1(les_ctx *c, const double *u, double *u_big, int nlev);
+int les_dealias2(les_ctx *c, double *u, double *u_big, int nlev);
+
+/* ---- vertical FD: ddz_uv.f90:5-57, ddz_w.f90:6-51 (nz levels) ---- */
+int les_ddz_uv(les_ctx *c, double *dfdz, const double *f);
+int les_ddz_w(les_ctx *c, double *dfdz, const double *f);
+
+/* ---- convec.f90:1-248  convec(cx,cy,cz) with its sim_param inputs ---- */
+int les_convec(les_ctx *c, const double *u, const double *v, const double *w, const double *dudy,
+               const double *dudz, const double *dvdx, const double *dvdz, const double *dwdx,
+               const double *dwdy, double *cx, double *cy, double *cz);
+
+/* ---- wallstress.f90:5-69 ('wall'); psi_m, psi_m0, phi_m, ustar_avg (nx,ny) ---- */
+int les_wallstress(les_ctx *c, const double *u, const double *v, double *dudz, double *dvdz, double *txz,
+                   double *tyz, const double *psi_m, const double *psi_m0, const double *phi_m,
+                   double *ustar_avg);
+
+/* ---- SGS: sgs_stag.f90:3-500 (model 5), lagrange_Sdep.f90:4-478,
+   interpolag_Sdep.f90:2-380.  grad: dudx,dudy,dudz,dvdx,dvdy,dvdz,dwdx,dwdy,dwdz;
+   tau: txx,txy,txz,tyy,tyz,tzz; S: S11,S12,S13,S22,S23,S33 ---- */
+int les_calc_sij(les_ctx *c, double *const grad[9], double *const S[6]);
+int les_sgs_stag(les_ctx *c, double *const grad[9], const double *u, const double *v, const double *w,
+                 double *const tau[6], const les_sgs_fields *s);
+int les_lagrange_sdep(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                      const les_sgs_fields *s);
+int les_interpolag_sdep(les_ctx *c, const les_sgs_fields *s);
+
+/* ---- divstress_uv.f90:1-63, divstress_w.f90:4-86 ---- */
+int les_divstress_uv(les_ctx *c, double *divt, const double *tx, const double *ty, const double *tz);
+int les_divstress_w(les_ctx *c, double *divt, const double *tx, const double *ty, const double *tz);
+
+/* ---- press_stag_array.f90:1-294 (p is (ld,ny,0:nz)); tridag_array.f90:1-109
+   (a,b,c real (lh,ny,nz+1), r,u complex (lh,ny,nz+1)) ---- */
+int les_press_stag_array(les_ctx *c, const double *u, const double *v, const double *w,
+                         const double *divtz, double *p, double *dpdx, double *dpdy);
+int les_tridag_array(les_ctx *c, const double *a, const double *b, const double *cc, const double *r,
+                     double *u);
+
+/* ---- forcing.f90:128-219 project; rmsdiv.f90:1-113; scalars_module2.f90:22-70 ---- */
+int les_project(les_ctx *c, double *u, double *v, double *w, const double *dpdx, const double *dpdy,
+                const double *dpdz);
+int les_rmsdiv(les_ctx *c, const double *dudx, const double *dvdy, const double *dwdz, double *rms);
+int les_timestep_conditions(les_ctx *c, const double *u, const double *v, const double *w,
+                            const double *Nu_t, double out[4]);
+
+/* FP32 variants (FP32-bulk mode) */
+int les_filt_da_f32(les_ctx *c, float *f, float *dfdx, float *dfdy, int nlev);
+int les_ddx_f32(les_ctx *c, float *dfdx, const float *f, int nlev);
+int les_ddy_f32(les_ctx *c, float *dfdy, const float *f, int nlev);
+int les_test_filter_f32(les_ctx *c, float *plane, const float *G);
+int les_dealias1_f32(les_ctx *c, const float *u, float *u_big, int nlev);
+int les_dealias2_f32(les_ctx *c, float *u, float *u_big, int nlev);
+int les_convec_f32(les_ctx *c, const float *u, const float *v, const float *w, const float *dudy,
+                   const float *dudz, const float *dvdx, const float *dvdz, const float *dwdx,
+                   const float *dwdy, float *cx, float *cy, float *cz);
+int les_divstress_uv_f32(les_ctx *c, float *divt, const float *tx, const float *ty, const float *tz);
+int les_divstress_w_f32(les_ctx *c, float *divt, const float *tx, const float *ty, const float *tz);
+int les_press_stag_array_f32(les_ctx *c, const float *u, const float *v, const float *w,
+                             const float *divtz, float *p, float *dpdx, float *dpdy);
+
+/* ---- whole time step (main.f90:202-415, neutral model=5) on
+   context-owned device state.  precision: 0 = FP64, 1 = FP32-bulk (FP64 kept
+   for the tridiagonal solve, wall stress and reductions) ---- */
+int les_state_alloc(les_ctx *c, int precision);
+/* device pointer of a state field by reference name (u, v, w, dudx ..., RHSx,
+   RHSx_f, txx ..., divtx ..., p, dpdx ..., F_LM ..., Cs_opt2, Nu_t, Beta,
+   u_lag ..., psi_m, ustar_avg, ...); nlev and elsize (may be NULL) receive
+   its level count and element size.  NULL if unknown. */
+void *les_state_field(les_ctx *c, const char *name, int *nlev, int *elsize);
+/* seeded initial condition: u = (ustar/kappa_ic) ln(z/z0) at z=(k-1/2)dz,
+   v = w = 0, plus amp*ustar*(2U-1) on u, v, w where z < 0.5, U = splitmix64
+   uniform of (seed, ((field*nz + k)*ny + y)*nx + x); w(1)=w(nz)=0,
+   u(nz)=u(nz-1), v(nz)=v(nz-1), RHS=RHS_f=0.  Resets jt, jt_total and the
+   SGS state. */
+int les_state_init_ic(les_ctx *c, uint64_t seed, double amp, double ustar, double kappa_ic);
+int les_step(les_ctx *c, int nsteps);
+/* rmsdiv of the state's current derivatives and timestep_conditions */
+int les_state_diag(les_ctx *c, double *rms, double out4[4]);
+
+/* Per-launch HIP-event timing on the context stream (no reference
+   counterpart; measurement support).  les_profile_report writes a JSON object
+   {"routine/kernel": [launches, total_ms, algorithmic_bytes], ...} and clears
+   the records. */
+int les_profile(les_ctx *c, int enable);
+int les_profile_report(les_ctx *c, char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+"""ctypes binding of les_amd/libles_amd.so (the C ABI in include/les_amd.h).
+
+The product path has no CPU fallback: if the HIP library is missing or a GPU
+is absent, using it raises.
+"""
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libles_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "les_amd.h")
+
+_lib = None
+
+
+class Grid(C.Structure):
+    """les_grid (param.f90:56-65,100,144-145,219; bottombc.f90:53)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("dt", C.c_double), ("z0", C.c_double), ("force", C.c_double),
+                ("inilag", C.c_int), ("initu", C.c_int), ("cs_count", C.c_int)]
+
+
+SGS_NAMES = ("F_LM", "F_MM", "F_QN", "F_NN", "Beta", "Cs_opt2", "Nu_t", "u_lag", "v_lag", "w_lag",
+             "TnLMMM", "TnQNNN", "L11t", "L22t", "L33t", "Q11t", "Q22t", "Q33t")
+
+
+class SgsFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in SGS_NAMES] + [("Beta_avg", C.c_void_p), ("Betaclip_avg", C.c_void_p)]
+
+
+def declared_symbols():
+    """Names of the functions include/les_amd.h declares."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(les_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load():
+    """Load the HIP library (does not touch the GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"les_amd: {LIB_PATH} is not built; run __graft_entry__.build() "
+                           "(the HIP path has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.les_create.argtypes = [C.POINTER(Grid), C.POINTER(vp)]
+    L.les_create.restype = C.c_int
+    L.les_destroy.argtypes = [vp]
+    L.les_destroy.restype = None
+    L.les_last_error.argtypes = [vp]
+    L.les_last_error.restype = C.c_char_p
+    L.les_state_field.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.les_state_field.restype = vp
+    L.les_state_init_ic.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, C.c_double]
+    L.les_test_filter_init.argtypes = [vp, C.c_double, vp]
+    for name in declared_symbols():
+        if name not in ("les_destroy", "les_last_error", "les_state_field"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L

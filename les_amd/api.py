@@ -1,0 +1,256 @@
+"""Host-side mirror of the reference's routine interface over the HIP C ABI.
+
+Fields are torch tensors on the GPU with shape (nlev, ny, ld), C-contiguous,
+float64 (or float32 for the FP32-bulk variants): byte-for-byte the
+reference's Fortran (ld, ny, nlev) column-major arrays.  Method names and
+argument order follow the reference subroutines (filt_da(f, dfdx, dfdy),
+ddx(dfdx, f), convec(cx, cy, cz) ...); module globals they read (u, v, w,
+gradients, F_LM ...) are passed explicitly.  Errors the reference turned into
+`stop` raise LesError.
+"""
+import ctypes as C
+import math
+
+import torch
+
+from . import _lib
+from ._lib import Grid, SgsFields, SGS_NAMES
+
+
+class LesError(RuntimeError):
+    pass
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise LesError("les_amd works on GPU tensors only (no CPU fallback)")
+    if not t.is_contiguous():
+        raise LesError("les_amd needs contiguous tensors in the reference layout")
+    return C.c_void_p(t.data_ptr())
+
+
+class Les:
+    """One grid: the les_ctx (wavenumbers, test filters, FFT twiddles, scratch)."""
+
+    def __init__(self, nx, ny, nz, Lx=2 * math.pi, Ly=2 * math.pi, Lz=1.0, dt=2e-4, z0=1e-4,
+                 force=1.0, inilag=1, initu=1, cs_count=5, stream=None):
+        if not torch.cuda.is_available():
+            raise LesError("les_amd needs a GPU (HIP); none is visible")
+        self.L = _lib.load()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.lh = nx // 2 + 1
+        self.ld = 2 * self.lh
+        self.nx2, self.ny2 = 3 * nx // 2, 3 * ny // 2
+        self.lh_big = self.nx2 // 2 + 1
+        self.ld_big = 2 * self.lh_big
+        self.dz = Lz / (nz - 1)
+        self.grid = Grid(nx, ny, nz, Lx, Ly, Lz, dt, z0, force, inilag, initu, cs_count)
+        h = C.c_void_p()
+        rc = self.L.les_create(C.byref(self.grid), C.byref(h))
+        if rc != 0:
+            raise LesError(f"les_create failed ({rc})")
+        self.ctx = h
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._chk(self.L.les_set_stream(self.ctx, C.c_void_p(s.cuda_stream)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "ctx", None):
+                self.L.les_destroy(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self.L.les_last_error(self.ctx)
+            raise LesError((msg or b"les_amd error").decode())
+
+    @staticmethod
+    def _sfx(t):
+        return "_f32" if t.dtype == torch.float32 else ""
+
+    # ---- allocation helpers (reference layout) ----
+    def field(self, nlev=None, dtype=torch.float64, big=False):
+        nlev = self.nz if nlev is None else nlev
+        if big:
+            return torch.zeros((nlev, self.ny2, self.ld_big), dtype=dtype, device="cuda")
+        return torch.zeros((nlev, self.ny, self.ld), dtype=dtype, device="cuda")
+
+    def set_time(self, jt, jt_total, flm_init=None, fqn_init=None):
+        self._chk(self.L.les_set_time(self.ctx, jt, jt_total))
+        if flm_init is not None:
+            self._chk(self.L.les_set_lag_init(self.ctx, int(flm_init), int(fqn_init)))
+
+    def tables(self):
+        import numpy as np
+        out = [np.zeros((self.ny, self.lh)) for _ in range(5)]
+        p = [o.ctypes.data_as(C.c_void_p) for o in out]
+        self._chk(self.L.les_get_tables(self.ctx, *p))
+        return dict(zip(("kx", "ky", "k2", "G_test", "G_test_test"), out))
+
+    def sync(self):
+        self._chk(self.L.les_sync(self.ctx))
+
+    # ---- routines (reference argument order) ----
+    def filt_da(self, f, dfdx, dfdy):
+        self._chk(getattr(self.L, "les_filt_da" + self._sfx(f))(self.ctx, _ptr(f), _ptr(dfdx), _ptr(dfdy), f.shape[0]))
+
+    def ddx(self, dfdx, f):
+        self._chk(getattr(self.L, "les_ddx" + self._sfx(f))(self.ctx, _ptr(dfdx), _ptr(f), f.shape[0]))
+
+    def ddy(self, dfdy, f):
+        self._chk(getattr(self.L, "les_ddy" + self._sfx(f))(self.ctx, _ptr(dfdy), _ptr(f), f.shape[0]))
+
+    def ddxy(self, dfdx, dfdy, f):
+        self._chk(self.L.les_ddxy(self.ctx, _ptr(dfdx), _ptr(dfdy), _ptr(f), f.shape[0]))
+
+    def test_filter_init(self, alpha):
+        G = torch.zeros((self.ny, self.lh), dtype=torch.float64, device="cuda")
+        self._chk(self.L.les_test_filter_init(self.ctx, C.c_double(alpha), _ptr(G)))
+        return G
+
+    def test_filter(self, plane, G):
+        self._chk(getattr(self.L, "les_test_filter" + self._sfx(plane))(self.ctx, _ptr(plane), _ptr(G)))
+
+    def padd(self, u_big, u):
+        self._chk(self.L.les_padd(self.ctx, _ptr(u_big), _ptr(u)))
+
+    def unpadd(self, cc, cc_big):
+        self._chk(self.L.les_unpadd(self.ctx, _ptr(cc), _ptr(cc_big)))
+
+    def dealias1(self, u, u_big):
+        self._chk(getattr(self.L, "les_dealias1" + self._sfx(u))(self.ctx, _ptr(u), _ptr(u_big), u.shape[0]))
+
+    def dealias2(self, u, u_big):
+        self._chk(getattr(self.L, "les_dealias2" + self._sfx(u))(self.ctx, _ptr(u), _ptr(u_big), u.shape[0]))
+
+    def ddz_uv(self, dfdz, f):
+        self._chk(self.L.les_ddz_uv(self.ctx, _ptr(dfdz), _ptr(f)))
+
+    def ddz_w(self, dfdz, f):
+        self._chk(self.L.les_ddz_w(self.ctx, _ptr(dfdz), _ptr(f)))
+
+    def convec(self, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz):
+        fn = getattr(self.L, "les_convec" + self._sfx(u))
+        self._chk(fn(self.ctx, *[_ptr(t) for t in (u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz)]))
+
+    def wallstress(self, u, v, dudz, dvdz, txz, tyz, psi_m=None, psi_m0=None, phi_m=None):
+        n2 = (self.ny, self.nx)
+        dev = dict(dtype=torch.float64, device="cuda")
+        psi_m = torch.zeros(n2, **dev) if psi_m is None else psi_m
+        psi_m0 = torch.zeros(n2, **dev) if psi_m0 is None else psi_m0
+        phi_m = torch.ones(n2, **dev) if phi_m is None else phi_m
+        ustar = torch.zeros(n2, **dev)
+        self._chk(self.L.les_wallstress(self.ctx, *[_ptr(t) for t in (u, v, dudz, dvdz, txz, tyz, psi_m, psi_m0,
+                                                                      phi_m, ustar)]))
+        return ustar
+
+    def sgs_fields(self, dtype=torch.float64):
+        d = {n: self.field(dtype=dtype) for n in SGS_NAMES}
+        d["Beta_avg"] = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        d["Betaclip_avg"] = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        return d
+
+    @staticmethod
+    def _sgs_struct(d):
+        return SgsFields(*[C.c_void_p(d[n].data_ptr()) for n in SGS_NAMES],
+                         C.c_void_p(d["Beta_avg"].data_ptr()), C.c_void_p(d["Betaclip_avg"].data_ptr()))
+
+    def calc_sij(self, grads):
+        S = [self.field() for _ in range(6)]
+        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
+        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        self._chk(self.L.les_calc_sij(self.ctx, g, s))
+        return S
+
+    def sgs_stag(self, grads, u, v, w, taus, sgs):
+        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
+        t = (C.c_void_p * 6)(*[x.data_ptr() for x in taus])
+        st = self._sgs_struct(sgs)
+        self._chk(self.L.les_sgs_stag(self.ctx, g, _ptr(u), _ptr(v), _ptr(w), t, C.byref(st)))
+
+    def lagrange_sdep(self, S, u, v, w, sgs):
+        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        st = self._sgs_struct(sgs)
+        self._chk(self.L.les_lagrange_sdep(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), C.byref(st)))
+
+    def interpolag_sdep(self, sgs):
+        st = self._sgs_struct(sgs)
+        self._chk(self.L.les_interpolag_sdep(self.ctx, C.byref(st)))
+
+    def divstress_uv(self, divt, tx, ty, tz):
+        self._chk(getattr(self.L, "les_divstress_uv" + self._sfx(divt))(self.ctx, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)))
+
+    def divstress_w(self, divt, tx, ty, tz):
+        self._chk(getattr(self.L, "les_divstress_w" + self._sfx(divt))(self.ctx, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)))
+
+    def press_stag_array(self, u, v, w, divtz, p, dpdx, dpdy):
+        fn = getattr(self.L, "les_press_stag_array" + self._sfx(u))
+        self._chk(fn(self.ctx, *[_ptr(t) for t in (u, v, w, divtz, p, dpdx, dpdy)]))
+
+    def tridag_array(self, a, b, c, r, u):
+        self._chk(self.L.les_tridag_array(self.ctx, *[_ptr(t) for t in (a, b, c, r, u)]))
+
+    def project(self, u, v, w, dpdx, dpdy, dpdz):
+        self._chk(self.L.les_project(self.ctx, *[_ptr(t) for t in (u, v, w, dpdx, dpdy, dpdz)]))
+
+    def rmsdiv(self, dudx, dvdy, dwdz):
+        r = C.c_double()
+        self._chk(self.L.les_rmsdiv(self.ctx, _ptr(dudx), _ptr(dvdy), _ptr(dwdz), C.byref(r)))
+        return r.value
+
+    def timestep_conditions(self, u, v, w, nu_t):
+        out = (C.c_double * 4)()
+        self._chk(self.L.les_timestep_conditions(self.ctx, _ptr(u), _ptr(v), _ptr(w), _ptr(nu_t), out))
+        return list(out)
+
+
+class LesSim(Les):
+    """Whole-step driver on context-owned device state (main.f90:202-415)."""
+
+    def __init__(self, nx, ny, nz, precision="fp64", **kw):
+        super().__init__(nx, ny, nz, **kw)
+        self.precision = precision
+        self._chk(self.L.les_state_alloc(self.ctx, 1 if precision == "fp32" else 0))
+
+    def init_ic(self, seed=42, amp=0.1, ustar=1.0, kappa_ic=0.4):
+        self._chk(self.L.les_state_init_ic(self.ctx, seed, amp, ustar, kappa_ic))
+
+    def step(self, n=1):
+        self._chk(self.L.les_step(self.ctx, n))
+
+    def ptr(self, name):
+        nlev, es = C.c_int(), C.c_int()
+        p = self.L.les_state_field(self.ctx, name.encode(), C.byref(nlev), C.byref(es))
+        if not p:
+            raise KeyError(name)
+        return p, nlev.value, es.value
+
+    def view(self, name):
+        """Zero-copy torch view of a state field (reference layout)."""
+        p, nlev, es = self.ptr(name)
+        if name in ("psi_m", "psi_m0", "phi_m", "ustar_avg"):
+            shape, typestr = (self.ny, self.nx), "<f8"
+        elif name in ("Beta_avg", "Betaclip_avg"):
+            shape, typestr = (self.nz,), "<f8"
+        else:
+            shape, typestr = (nlev, self.ny, self.ld), ("<f4" if es == 4 else "<f8")
+
+        class _Dev:
+            __cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (p, False),
+                                        "version": 2, "strides": None}
+        return torch.as_tensor(_Dev(), device="cuda")
+
+    def get(self, name):
+        """Copy a state field to a host numpy array (float64)."""
+        self.sync()
+        return self.view(name).double().cpu().numpy()
+
+    @property
+    def jt(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(self.L.les_get_time(self.ctx, C.byref(a), C.byref(b)))
+        return a.value

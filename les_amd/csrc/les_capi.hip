@@ -1,0 +1,743 @@
+// les_capi.hip -- C ABI (include/les_amd.h): context set-up, device scratch,
+// routine-level drop-in entry points and the whole-step driver.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/les_amd.h"
+#include "les_internal.h"
+
+using namespace les;
+
+namespace les {
+
+constexpr double PI = 3.1415926535897932384626433;  // param.f90:51
+
+void* scratch(les_ctx* c, int slot, size_t bytes) {
+  Arena& a = c->arena;
+  if ((int)a.ptr.size() < S_NSLOTS) {
+    a.ptr.assign(S_NSLOTS, nullptr);
+    a.cap.assign(S_NSLOTS, 0);
+  }
+  if (bytes == 0) bytes = 16;
+  if (a.cap[slot] < bytes) {
+    // growth happens on first use of a size; in-flight work may still read
+    // the old buffer, so drain the stream before releasing it
+    check(c, hipStreamSynchronize(c->stream), "scratch sync");
+    if (a.ptr[slot]) check(c, hipFree(a.ptr[slot]), "scratch free");
+    a.ptr[slot] = nullptr;
+    size_t want = bytes + bytes / 8;
+    check(c, hipMalloc(&a.ptr[slot], want), "scratch alloc");
+    a.cap[slot] = a.ptr[slot] ? want : 0;
+  }
+  return a.ptr[slot];
+}
+
+static FftPlan make_plan(int n) {
+  FftPlan p;
+  std::memset(&p, 0, sizeof(p));
+  p.n = n;
+  int m = n;
+  while (m % 4 == 0) { p.rad[p.ns++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.rad[p.ns++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.ns++] = 3; m /= 3; }
+  if (m != 1 || n % 4 != 0) p.n = -1;
+  return p;
+}
+
+template <typename T>
+static cx<T>* upload_twiddles(les_ctx* c, int n) {
+  std::vector<cx<T>> h(n);
+  for (int k = 0; k < n; ++k) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    h[k].x = (T)cosl(a);
+    h[k].y = (T)sinl(a);
+  }
+  cx<T>* d = nullptr;
+  check(c, hipMalloc(&d, n * sizeof(cx<T>)), "twiddle alloc");
+  check(c, hipMemcpy(d, h.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice), "twiddle upload");
+  return d;
+}
+template <typename T>
+static T* upload(les_ctx* c, const std::vector<double>& v) {
+  std::vector<T> h(v.size());
+  for (size_t i = 0; i < v.size(); ++i) h[i] = (T)v[i];
+  T* d = nullptr;
+  check(c, hipMalloc(&d, h.size() * sizeof(T)), "table alloc");
+  check(c, hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "table upload");
+  return d;
+}
+
+// fft.f90:54-81 init_wavenumber (incl. the ky(lh,:)=0 quirk) and
+// test_filtermodule.f90:32-63 test_filter_init (ifilter=1)
+struct HostTables {
+  std::vector<double> kx, ky, k2, G2, G4;
+};
+static void host_tables(const Grid& g, HostTables& t) {
+  const int lh = g.lh, ny = g.ny;
+  t.kx.assign((size_t)lh * ny, 0.0);
+  t.ky.assign((size_t)lh * ny, 0.0);
+  t.k2.assign((size_t)lh * ny, 0.0);
+  for (int jy = 0; jy < ny; ++jy)
+    for (int jx = 0; jx < lh; ++jx) {
+      double kx = jx < lh - 1 ? (double)jx : 0.0;
+      double ky = (double)(((jy + ny / 2) % ny) - ny / 2);
+      if (jx == lh - 1) { kx = 0.0; ky = 0.0; }
+      if (jy == ny / 2) { kx = 0.0; ky = 0.0; }
+      kx = 2.0 * PI / g.Lx * kx;
+      ky = 2.0 * PI / g.Ly * ky;
+      t.kx[jx + (size_t)lh * jy] = kx;
+      t.ky[jx + (size_t)lh * jy] = ky;
+      t.k2[jx + (size_t)lh * jy] = kx * kx + ky * ky;
+    }
+  auto filt = [&](double alpha, std::vector<double>& G) {
+    G.assign((size_t)lh * ny, 0.0);
+    const double delta = alpha * sqrt(g.dx * g.dy);
+    const double q = PI / delta, kc2 = q * q;
+    for (size_t i = 0; i < G.size(); ++i) G[i] = t.k2[i] >= kc2 ? 0.0 : 1.0 / ((double)g.nx * g.ny);
+    for (int y = 0; y < ny; ++y) G[lh - 1 + (size_t)lh * y] = 0.0;
+    for (int x = 0; x < lh; ++x) G[x + (size_t)lh * (ny / 2)] = 0.0;
+  };
+  filt(2.0, t.G2);
+  filt(4.0, t.G4);
+}
+static int keep_cols(const Grid& g, const std::vector<double>& G) {
+  int keep = 1;
+  for (int y = 0; y < g.ny; ++y)
+    for (int x = 0; x < g.nxh; ++x)
+      if (G[x + (size_t)g.lh * y] != 0.0 && x + 1 > keep) keep = x + 1;
+  return keep;
+}
+
+template <typename T>
+static void make_tables(les_ctx* c, Tables<T>& t, const HostTables& h) {
+  const Grid& g = c->g;
+  t.tw_nx = upload_twiddles<T>(c, g.nx);
+  t.tw_ny = upload_twiddles<T>(c, g.ny);
+  t.tw_nx2 = upload_twiddles<T>(c, g.nx2);
+  t.tw_ny2 = upload_twiddles<T>(c, g.ny2);
+  std::vector<double> kx(g.nxh), ky(g.ny);
+  for (int x = 0; x < g.nxh; ++x) kx[x] = h.kx[x];                   // row jy=1 (ky=0)
+  for (int y = 0; y < g.ny; ++y) ky[y] = h.ky[(size_t)g.lh * y];     // column jx=1
+  t.kx = upload<T>(c, kx);
+  t.ky = upload<T>(c, ky);
+  t.G2 = upload<T>(c, h.G2);
+  t.G4 = upload<T>(c, h.G4);
+}
+template <typename T>
+static void free_tables(Tables<T>& t) {
+  void* ps[] = {t.tw_nx, t.tw_ny, t.tw_nx2, t.tw_ny2, t.kx, t.ky, t.G2, t.G4};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+}
+
+// ------------------------------------------------------------------------
+// whole-step state
+// ------------------------------------------------------------------------
+struct FieldInfo {
+  void* p;
+  int nlev;
+  int elsize;
+};
+struct State {
+  int prec = 0;  // 0 fp64, 1 fp32-bulk
+  size_t es = 8;
+  std::map<std::string, FieldInfo> f;
+  int rhs_swap = 0;  // RHS <-> RHS_f buffer parity
+  std::vector<void*> owned;
+};
+
+static const char* F3[] = {"u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+                           "RHSx", "RHSy", "RHSz", "RHSx_f", "RHSy_f", "RHSz_f", "txx", "txy", "txz", "tyy", "tyz",
+                           "tzz", "divtx", "divty", "divtz", "dpdx", "dpdy", "dpdz", "F_LM", "F_MM", "F_QN", "F_NN",
+                           "Beta", "Cs_opt2", "Nu_t", "u_lag", "v_lag", "w_lag", "TnLMMM", "TnQNNN", "L11t", "L22t",
+                           "L33t", "Q11t", "Q22t", "Q33t"};
+
+static void* state_ptr(les_ctx* c, const char* name) {
+  State* s = c->st;
+  std::string n(name);
+  if (s->rhs_swap) {
+    if (n.size() >= 4 && n.compare(0, 3, "RHS") == 0) {
+      if (n.size() == 4) n += "_f";
+      else if (n.size() == 6) n = n.substr(0, 4);
+    }
+  }
+  auto it = s->f.find(n);
+  return it == s->f.end() ? nullptr : it->second.p;
+}
+
+template <typename T>
+static T* F(les_ctx* c, const char* name) {
+  return (T*)state_ptr(c, name);
+}
+
+template <typename T>
+static SgsPtrs sgs_ptrs(les_ctx* c) {
+  SgsPtrs s;
+  s.F_LM = F<T>(c, "F_LM"); s.F_MM = F<T>(c, "F_MM"); s.F_QN = F<T>(c, "F_QN"); s.F_NN = F<T>(c, "F_NN");
+  s.Beta = F<T>(c, "Beta"); s.Cs_opt2 = F<T>(c, "Cs_opt2"); s.Nu_t = F<T>(c, "Nu_t");
+  s.u_lag = F<T>(c, "u_lag"); s.v_lag = F<T>(c, "v_lag"); s.w_lag = F<T>(c, "w_lag");
+  s.TnLMMM = F<T>(c, "TnLMMM"); s.TnQNNN = F<T>(c, "TnQNNN");
+  s.L11t = F<T>(c, "L11t"); s.L22t = F<T>(c, "L22t"); s.L33t = F<T>(c, "L33t");
+  s.Q11t = F<T>(c, "Q11t"); s.Q22t = F<T>(c, "Q22t"); s.Q33t = F<T>(c, "Q33t");
+  s.Beta_avg = state_ptr(c, "Beta_avg");
+  s.Betaclip_avg = state_ptr(c, "Betaclip_avg");
+  return s;
+}
+
+template <typename T>
+static void step_once(les_ctx* c) {
+  const Grid& g = c->g;
+  State* s = c->st;
+  c->jt += 1;
+  c->jt_total += 1;
+  s->rhs_swap ^= 1;  // RHS_f = RHS (main.f90:212-214) as a buffer swap
+  T *u = F<T>(c, "u"), *v = F<T>(c, "v"), *w = F<T>(c, "w");
+  T* gr[9] = {F<T>(c, "dudx"), F<T>(c, "dudy"), F<T>(c, "dudz"), F<T>(c, "dvdx"), F<T>(c, "dvdy"),
+              F<T>(c, "dvdz"), F<T>(c, "dwdx"), F<T>(c, "dwdy"), F<T>(c, "dwdz")};
+  T* tau[6] = {F<T>(c, "txx"), F<T>(c, "txy"), F<T>(c, "txz"), F<T>(c, "tyy"), F<T>(c, "tyz"), F<T>(c, "tzz")};
+  // obukhov (neutral): psi = 0, phi = 1 are the state's constant contents
+  op_filt_da<T>(c, u, gr[0], gr[1], g.nz);
+  op_filt_da<T>(c, v, gr[3], gr[4], g.nz);
+  op_filt_da<T>(c, w, gr[6], gr[7], g.nz);
+  op_ddz_uv<T>(c, gr[2], u);
+  op_ddz_uv<T>(c, gr[5], v);
+  op_ddz_w<T>(c, gr[8], w);
+  op_wallstress<T>(c, u, v, gr[2], gr[5], tau[2], tau[4], (const double*)state_ptr(c, "psi_m"),
+                   (const double*)state_ptr(c, "psi_m0"), (const double*)state_ptr(c, "phi_m"),
+                   (double*)state_ptr(c, "ustar_avg"));
+  const T* grc[9] = {gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]};
+  op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c));
+  T *dx = F<T>(c, "divtx"), *dy = F<T>(c, "divty"), *dz = F<T>(c, "divtz");
+  op_divstress<T>(c, dx, tau[0], tau[1], tau[2], 0);
+  op_divstress<T>(c, dy, tau[1], tau[3], tau[4], 0);
+  op_divstress<T>(c, dz, tau[2], tau[4], tau[5], 1);
+  T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
+  op_convec<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz);
+  op_ab2<T>(c, u, v, w, Rx, Ry, Rz, F<T>(c, "RHSx_f"), F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz,
+            (c->jt == 1 && !g.initu) ? 1 : 0);
+  T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
+  op_press_stag<T>(c, u, v, w, dz, p, px, py);
+  op_pressure_update<T>(c, u, v, w, Rx, Ry, Rz, p, px, py, pz);
+}
+
+}  // namespace les
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+#define GUARD(c)                     \
+  if (!(c)) return -1;               \
+  (c)->err.clear();
+#define DONE(c) return (c)->err.empty() ? 0 : 1;
+
+extern "C" {
+
+int les_create(const les_grid* gi, les_ctx** out) {
+  if (!gi || !out) return -1;
+  *out = nullptr;
+  les_ctx* c = new les_ctx();
+  Grid& g = c->g;
+  g.nx = gi->nx; g.ny = gi->ny; g.nz = gi->nz;
+  g.lh = g.nx / 2 + 1; g.ld = 2 * g.lh; g.nxh = g.nx / 2;
+  g.nx2 = 3 * g.nx / 2; g.ny2 = 3 * g.ny / 2;
+  g.Lx = gi->Lx; g.Ly = gi->Ly; g.Lz = gi->Lz;
+  g.dx = g.Lx / g.nx; g.dy = g.Ly / g.ny; g.dz = g.Lz / (g.nz - 1);
+  g.dt = gi->dt; g.z0 = gi->z0; g.force = gi->force;
+  g.delta = sqrt(g.dx * g.dy);
+  g.cs_count = gi->cs_count > 0 ? gi->cs_count : 5;
+  g.inilag = gi->inilag; g.initu = gi->initu;
+  c->p_nx = make_plan(g.nx); c->p_ny = make_plan(g.ny);
+  c->p_nx2 = make_plan(g.nx2); c->p_ny2 = make_plan(g.ny2);
+  if (c->p_nx.n < 0 || c->p_ny.n < 0 || c->p_nx2.n < 0 || c->p_ny2.n < 0 || g.nx > 1024 || g.ny > 1024 ||
+      g.nz < 3 || g.nx % 8 || g.ny % 8) {
+    fprintf(stderr, "les_create: nx, ny must be multiples of 8 of the form 2^a 3^b (<= 1024), nz >= 3\n");
+    delete c;
+    return 2;
+  }
+  check(c, hipGetDevice(&c->device), "hipGetDevice");
+  HostTables h;
+  host_tables(g, h);
+  c->keep2 = keep_cols(g, h.G2);
+  c->keep4 = keep_cols(g, h.G4);
+  make_tables<double>(c, c->td, h);
+  make_tables<float>(c, c->tf, h);
+  c->G2_host = new double[h.G2.size()];
+  std::memcpy(c->G2_host, h.G2.data(), h.G2.size() * sizeof(double));
+  if (!c->err.empty()) {
+    fprintf(stderr, "les_create: %s\n", c->err.c_str());
+    les_destroy(c);
+    return 1;
+  }
+  *out = c;
+  return 0;
+}
+
+void les_destroy(les_ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  free_tables(c->td);
+  free_tables(c->tf);
+  for (void* p : c->arena.ptr)
+    if (p) (void)hipFree(p);
+  if (c->st) {
+    for (void* p : c->st->owned) (void)hipFree(p);
+    delete c->st;
+  }
+  delete[] c->G2_host;
+  delete c;
+}
+
+const char* les_last_error(les_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int les_set_stream(les_ctx* c, void* s) {
+  GUARD(c);
+  c->stream = (hipStream_t)s;
+  DONE(c);
+}
+int les_sync(les_ctx* c) {
+  GUARD(c);
+  check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  DONE(c);
+}
+int les_set_time(les_ctx* c, int jt, int jt_total) {
+  GUARD(c);
+  c->jt = jt;
+  c->jt_total = jt_total;
+  DONE(c);
+}
+int les_get_time(les_ctx* c, int* jt, int* jt_total) {
+  GUARD(c);
+  if (jt) *jt = c->jt;
+  if (jt_total) *jt_total = c->jt_total;
+  DONE(c);
+}
+int les_set_lag_init(les_ctx* c, int a, int b) {
+  GUARD(c);
+  c->flm_init = a;
+  c->fqn_init = b;
+  DONE(c);
+}
+int les_get_tables(les_ctx* c, double* kx, double* ky, double* k2, double* G, double* GG) {
+  GUARD(c);
+  HostTables h;
+  host_tables(c->g, h);
+  const size_t n = h.kx.size() * sizeof(double);
+  if (kx) std::memcpy(kx, h.kx.data(), n);
+  if (ky) std::memcpy(ky, h.ky.data(), n);
+  if (k2) std::memcpy(k2, h.k2.data(), n);
+  if (G) std::memcpy(G, h.G2.data(), n);
+  if (GG) std::memcpy(GG, h.G4.data(), n);
+  DONE(c);
+}
+
+// ---- routine-level entry points ----
+int les_filt_da(les_ctx* c, double* f, double* dx, double* dy, int nlev) {
+  GUARD(c);
+  op_filt_da<double>(c, f, dx, dy, nlev);
+  DONE(c);
+}
+int les_filt_da_f32(les_ctx* c, float* f, float* dx, float* dy, int nlev) {
+  GUARD(c);
+  op_filt_da<float>(c, f, dx, dy, nlev);
+  DONE(c);
+}
+int les_ddx(les_ctx* c, double* dfdx, const double* f, int nlev) {
+  GUARD(c);
+  op_ddxy<double>(c, dfdx, nullptr, f, nlev);
+  DONE(c);
+}
+int les_ddy(les_ctx* c, double* dfdy, const double* f, int nlev) {
+  GUARD(c);
+  op_ddxy<double>(c, nullptr, dfdy, f, nlev);
+  DONE(c);
+}
+int les_ddx_f32(les_ctx* c, float* dfdx, const float* f, int nlev) {
+  GUARD(c);
+  op_ddxy<float>(c, dfdx, nullptr, f, nlev);
+  DONE(c);
+}
+int les_ddy_f32(les_ctx* c, float* dfdy, const float* f, int nlev) {
+  GUARD(c);
+  op_ddxy<float>(c, nullptr, dfdy, f, nlev);
+  DONE(c);
+}
+int les_ddxy(les_ctx* c, double* dfdx, double* dfdy, const double* f, int nlev) {
+  GUARD(c);
+  op_ddxy<double>(c, dfdx, dfdy, f, nlev);
+  DONE(c);
+}
+int les_test_filter_init(les_ctx* c, double alpha, double* G) {
+  GUARD(c);
+  HostTables h;
+  host_tables(c->g, h);
+  std::vector<double> out((size_t)c->g.lh * c->g.ny);
+  const double delta = alpha * sqrt(c->g.dx * c->g.dy);
+  const double q = PI / delta, kc2 = q * q;
+  for (size_t i = 0; i < out.size(); ++i) out[i] = h.k2[i] >= kc2 ? 0.0 : 1.0 / ((double)c->g.nx * c->g.ny);
+  for (int y = 0; y < c->g.ny; ++y) out[c->g.lh - 1 + (size_t)c->g.lh * y] = 0.0;
+  for (int x = 0; x < c->g.lh; ++x) out[x + (size_t)c->g.lh * (c->g.ny / 2)] = 0.0;
+  check(c, hipMemcpyAsync(G, out.data(), out.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
+        "test_filter_init upload");
+  check(c, hipStreamSynchronize(c->stream), "sync");
+  DONE(c);
+}
+int les_test_filter(les_ctx* c, double* plane, const double* G) {
+  GUARD(c);
+  op_test_filter<double>(c, plane, 1, G, c->g.nxh);
+  DONE(c);
+}
+int les_test_filter_f32(les_ctx* c, float* plane, const float* G) {
+  GUARD(c);
+  op_test_filter<float>(c, plane, 1, G, c->g.nxh);
+  DONE(c);
+}
+int les_padd(les_ctx* c, double* u_big, const double* u) {
+  GUARD(c);
+  op_padd<double>(c, u_big, u, 1);
+  DONE(c);
+}
+int les_unpadd(les_ctx* c, double* cc, const double* cc_big) {
+  GUARD(c);
+  op_unpadd<double>(c, cc, cc_big, 1);
+  DONE(c);
+}
+int les_dealias1(les_ctx* c, const double* u, double* u_big, int nlev) {
+  GUARD(c);
+  op_dealias1<double>(c, u, u_big, nlev);
+  DONE(c);
+}
+int les_dealias2(les_ctx* c, double* u, double* u_big, int nlev) {
+  GUARD(c);
+  op_dealias2<double>(c, u, u_big, nlev);
+  DONE(c);
+}
+int les_dealias1_f32(les_ctx* c, const float* u, float* u_big, int nlev) {
+  GUARD(c);
+  op_dealias1<float>(c, u, u_big, nlev);
+  DONE(c);
+}
+int les_dealias2_f32(les_ctx* c, float* u, float* u_big, int nlev) {
+  GUARD(c);
+  op_dealias2<float>(c, u, u_big, nlev);
+  DONE(c);
+}
+int les_ddz_uv(les_ctx* c, double* dfdz, const double* f) {
+  GUARD(c);
+  op_ddz_uv<double>(c, dfdz, f);
+  DONE(c);
+}
+int les_ddz_w(les_ctx* c, double* dfdz, const double* f) {
+  GUARD(c);
+  op_ddz_w<double>(c, dfdz, f);
+  DONE(c);
+}
+int les_convec(les_ctx* c, const double* u, const double* v, const double* w, const double* dudy,
+               const double* dudz, const double* dvdx, const double* dvdz, const double* dwdx, const double* dwdy,
+               double* cx, double* cy, double* cz) {
+  GUARD(c);
+  op_convec<double>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz);
+  DONE(c);
+}
+int les_convec_f32(les_ctx* c, const float* u, const float* v, const float* w, const float* dudy,
+                   const float* dudz, const float* dvdx, const float* dvdz, const float* dwdx, const float* dwdy,
+                   float* cx, float* cy, float* cz) {
+  GUARD(c);
+  op_convec<float>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz);
+  DONE(c);
+}
+int les_wallstress(les_ctx* c, const double* u, const double* v, double* dudz, double* dvdz, double* txz,
+                   double* tyz, const double* psi_m, const double* psi_m0, const double* phi_m, double* ustar) {
+  GUARD(c);
+  op_wallstress<double>(c, u, v, dudz, dvdz, txz, tyz, psi_m, psi_m0, phi_m, ustar);
+  DONE(c);
+}
+static SgsPtrs to_ptrs(const les_sgs_fields* s) {
+  SgsPtrs p;
+  p.F_LM = s->F_LM; p.F_MM = s->F_MM; p.F_QN = s->F_QN; p.F_NN = s->F_NN; p.Beta = s->Beta;
+  p.Cs_opt2 = s->Cs_opt2; p.Nu_t = s->Nu_t; p.u_lag = s->u_lag; p.v_lag = s->v_lag; p.w_lag = s->w_lag;
+  p.TnLMMM = s->TnLMMM; p.TnQNNN = s->TnQNNN; p.L11t = s->L11t; p.L22t = s->L22t; p.L33t = s->L33t;
+  p.Q11t = s->Q11t; p.Q22t = s->Q22t; p.Q33t = s->Q33t; p.Beta_avg = s->Beta_avg; p.Betaclip_avg = s->Betaclip_avg;
+  return p;
+}
+int les_calc_sij(les_ctx* c, double* const grad[9], double* const S[6]) {
+  GUARD(c);
+  const double* g[9];
+  for (int i = 0; i < 9; ++i) g[i] = grad[i];
+  op_calc_sij<double>(c, g, S);
+  DONE(c);
+}
+int les_sgs_stag(les_ctx* c, double* const grad[9], const double* u, const double* v, const double* w,
+                 double* const tau[6], const les_sgs_fields* s) {
+  GUARD(c);
+  if (!s) return -1;
+  const double* g[9];
+  for (int i = 0; i < 9; ++i) g[i] = grad[i];
+  op_sgs_stag<double>(c, g, u, v, w, tau, to_ptrs(s));
+  DONE(c);
+}
+int les_lagrange_sdep(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                      const les_sgs_fields* s) {
+  GUARD(c);
+  if (!s) return -1;
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_lagrange<double>(c, Sc, u, v, w, to_ptrs(s));
+  DONE(c);
+}
+int les_interpolag_sdep(les_ctx* c, const les_sgs_fields* s) {
+  GUARD(c);
+  if (!s) return -1;
+  op_interpolag<double>(c, to_ptrs(s));
+  DONE(c);
+}
+int les_divstress_uv(les_ctx* c, double* divt, const double* tx, const double* ty, const double* tz) {
+  GUARD(c);
+  op_divstress<double>(c, divt, tx, ty, tz, 0);
+  DONE(c);
+}
+int les_divstress_w(les_ctx* c, double* divt, const double* tx, const double* ty, const double* tz) {
+  GUARD(c);
+  op_divstress<double>(c, divt, tx, ty, tz, 1);
+  DONE(c);
+}
+int les_divstress_uv_f32(les_ctx* c, float* divt, const float* tx, const float* ty, const float* tz) {
+  GUARD(c);
+  op_divstress<float>(c, divt, tx, ty, tz, 0);
+  DONE(c);
+}
+int les_divstress_w_f32(les_ctx* c, float* divt, const float* tx, const float* ty, const float* tz) {
+  GUARD(c);
+  op_divstress<float>(c, divt, tx, ty, tz, 1);
+  DONE(c);
+}
+int les_press_stag_array(les_ctx* c, const double* u, const double* v, const double* w, const double* divtz,
+                         double* p, double* dpdx, double* dpdy) {
+  GUARD(c);
+  op_press_stag<double>(c, u, v, w, divtz, p, dpdx, dpdy);
+  DONE(c);
+}
+int les_press_stag_array_f32(les_ctx* c, const float* u, const float* v, const float* w, const float* divtz,
+                             float* p, float* dpdx, float* dpdy) {
+  GUARD(c);
+  op_press_stag<float>(c, u, v, w, divtz, p, dpdx, dpdy);
+  DONE(c);
+}
+int les_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u) {
+  GUARD(c);
+  int e = op_tridag_array(c, a, b, cc, r, u);
+  if (e && c->err.empty()) c->err = e == 1 ? "tridag_array: rewrite eqs (b(1) == 0)" : "tridag_array failed (zero pivot)";
+  DONE(c);
+}
+int les_project(les_ctx* c, double* u, double* v, double* w, const double* dpdx, const double* dpdy,
+                const double* dpdz) {
+  GUARD(c);
+  op_project<double>(c, u, v, w, dpdx, dpdy, dpdz);
+  DONE(c);
+}
+int les_rmsdiv(les_ctx* c, const double* dudx, const double* dvdy, const double* dwdz, double* rms) {
+  GUARD(c);
+  double r = op_rmsdiv<double>(c, dudx, dvdy, dwdz);
+  if (rms) *rms = r;
+  DONE(c);
+}
+int les_timestep_conditions(les_ctx* c, const double* u, const double* v, const double* w, const double* nu,
+                            double out[4]) {
+  GUARD(c);
+  op_timestep_conditions<double>(c, u, v, w, nu, out);
+  DONE(c);
+}
+
+// ---- whole-step state ----
+int les_state_alloc(les_ctx* c, int precision) {
+  GUARD(c);
+  if (c->st) {
+    for (void* p : c->st->owned) (void)hipFree(p);
+    delete c->st;
+  }
+  State* s = new State();
+  s->prec = precision ? 1 : 0;
+  s->es = s->prec ? 4 : 8;
+  c->st = s;
+  const Grid& g = c->g;
+  const size_t pl = (size_t)g.ld * g.ny;
+  auto add = [&](const char* name, int nlev, size_t es) {
+    void* p = nullptr;
+    check(c, hipMalloc(&p, pl * nlev * es), "state alloc");
+    if (p) check(c, hipMemset(p, 0, pl * nlev * es), "state zero");
+    s->owned.push_back(p);
+    s->f[name] = FieldInfo{p, nlev, (int)es};
+  };
+  for (const char* n : F3) add(n, g.nz, s->es);
+  add("p", g.nz + 1, s->es);
+  auto add2 = [&](const char* name, size_t n, double val) {
+    void* p = nullptr;
+    check(c, hipMalloc(&p, n * sizeof(double)), "state alloc");
+    std::vector<double> h(n, val);
+    if (p) check(c, hipMemcpy(p, h.data(), n * sizeof(double), hipMemcpyHostToDevice), "state init");
+    s->owned.push_back(p);
+    s->f[name] = FieldInfo{p, 1, 8};
+  };
+  const size_t n2 = (size_t)g.nx * g.ny;
+  add2("psi_m", n2, 0.0);
+  add2("psi_m0", n2, 0.0);
+  add2("phi_m", n2, 1.0);
+  add2("ustar_avg", n2, 0.0);
+  add2("Beta_avg", g.nz, 0.0);
+  add2("Betaclip_avg", g.nz, 0.0);
+  DONE(c);
+}
+
+void* les_state_field(les_ctx* c, const char* name, int* nlev, int* elsize) {
+  if (!c || !c->st || !name) return nullptr;
+  void* p = state_ptr(c, name);
+  if (!p) return nullptr;
+  std::string n(name);
+  auto it = c->st->f.begin();
+  for (; it != c->st->f.end(); ++it)
+    if (it->second.p == p) break;
+  if (nlev) *nlev = it->second.nlev;
+  if (elsize) *elsize = it->second.elsize;
+  return p;
+}
+
+static uint64_t splitmix(uint64_t seed, uint64_t index) {
+  uint64_t z = seed * 0x9E3779B97F4A7C15ULL + index + 0x632BE59BD9B4E019ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int les_state_init_ic(les_ctx* c, uint64_t seed, double amp, double ustar, double kappa_ic) {
+  GUARD(c);
+  if (!c->st) return -1;
+  State* s = c->st;
+  const Grid& g = c->g;
+  const size_t pl = (size_t)g.ld * g.ny, n3 = pl * g.nz;
+  std::vector<double> f[3];
+  for (int q = 0; q < 3; ++q) f[q].assign(n3, 0.0);
+  for (int k = 0; k < g.nz - 1; ++k) {
+    const double z = (k + 0.5) * g.dz;
+    const double ulog = (ustar / kappa_ic) * log(z / g.z0);
+    for (int y = 0; y < g.ny; ++y)
+      for (int x = 0; x < g.nx; ++x) {
+        const size_t i = pl * k + x + (size_t)g.ld * y;
+        f[0][i] = ulog;
+        if (z < 0.5)
+          for (int q = 0; q < 3; ++q) {
+            const uint64_t idx = (((uint64_t)q * g.nz + k) * g.ny + y) * g.nx + x;
+            const double U = (double)(splitmix(seed, idx) >> 11) * (1.0 / 9007199254740992.0);
+            f[q][i] += amp * ustar * (2.0 * U - 1.0);
+          }
+      }
+  }
+  for (size_t i = 0; i < pl; ++i) {
+    f[2][i] = 0.0;
+    f[0][pl * (g.nz - 1) + i] = f[0][pl * (g.nz - 2) + i];
+    f[1][pl * (g.nz - 1) + i] = f[1][pl * (g.nz - 2) + i];
+    f[2][pl * (g.nz - 1) + i] = 0.0;
+  }
+  for (auto& kv : s->f) {
+    if (kv.first == "phi_m" || kv.first == "psi_m" || kv.first == "psi_m0") continue;
+    const size_t bytes = (kv.first == "ustar_avg") ? (size_t)g.nx * g.ny * 8
+                         : (kv.first == "Beta_avg" || kv.first == "Betaclip_avg") ? (size_t)g.nz * 8
+                         : pl * kv.second.nlev * s->es;
+    check(c, hipMemsetAsync(kv.second.p, 0, bytes, c->stream), "state reset");
+  }
+  s->rhs_swap = 0;
+  const char* names[3] = {"u", "v", "w"};
+  for (int q = 0; q < 3; ++q) {
+    void* d = state_ptr(c, names[q]);
+    if (s->prec) {
+      std::vector<float> h(n3);
+      for (size_t i = 0; i < n3; ++i) h[i] = (float)f[q][i];
+      check(c, hipMemcpyAsync(d, h.data(), n3 * 4, hipMemcpyHostToDevice, c->stream), "ic upload");
+      check(c, hipStreamSynchronize(c->stream), "ic sync");
+    } else {
+      check(c, hipMemcpyAsync(d, f[q].data(), n3 * 8, hipMemcpyHostToDevice, c->stream), "ic upload");
+      check(c, hipStreamSynchronize(c->stream), "ic sync");
+    }
+  }
+  c->jt = 0;
+  c->jt_total = 0;
+  c->flm_init = 0;
+  c->fqn_init = 0;
+  DONE(c);
+}
+
+int les_step(les_ctx* c, int nsteps) {
+  GUARD(c);
+  if (!c->st) {
+    c->err = "les_step: no state (call les_state_alloc)";
+    return 1;
+  }
+  for (int i = 0; i < nsteps && c->err.empty(); ++i) {
+    if (c->st->prec)
+      step_once<float>(c);
+    else
+      step_once<double>(c);
+  }
+  DONE(c);
+}
+
+int les_profile(les_ctx* c, int enable) {
+  GUARD(c);
+  c->prof = enable != 0;
+  DONE(c);
+}
+
+int les_profile_report(les_ctx* c, char* buf, int buflen) {
+  GUARD(c);
+  check(c, hipStreamSynchronize(c->stream), "profile sync");
+  struct Acc {
+    int n = 0;
+    double ms = 0, bytes = 0;
+  };
+  std::map<std::string, Acc> m;
+  for (auto& r : c->prof_recs) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    Acc& a = m[r.tag];
+    a.n += 1;
+    a.ms += ms;
+    a.bytes += r.bytes;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  c->prof_recs.clear();
+  std::string out = "{";
+  bool first = true;
+  for (auto& kv : m) {
+    char tmp[256];
+    snprintf(tmp, sizeof(tmp), "%s\"%s\": [%d, %.6f, %.1f]", first ? "" : ", ", kv.first.c_str(), kv.second.n,
+             kv.second.ms, kv.second.bytes);
+    out += tmp;
+    first = false;
+  }
+  out += "}";
+  if (buf && buflen > 0) {
+    std::strncpy(buf, out.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = 0;
+  }
+  if ((int)out.size() >= buflen) c->err = "les_profile_report: buffer too small";
+  DONE(c);
+}
+
+int les_state_diag(les_ctx* c, double* rms, double out4[4]) {
+  GUARD(c);
+  if (!c->st) return -1;
+  if (c->st->prec) {
+    if (rms) *rms = op_rmsdiv<float>(c, F<float>(c, "dudx"), F<float>(c, "dvdy"), F<float>(c, "dwdz"));
+    if (out4) op_timestep_conditions<float>(c, F<float>(c, "u"), F<float>(c, "v"), F<float>(c, "w"),
+                                            F<float>(c, "Nu_t"), out4);
+  } else {
+    if (rms) *rms = op_rmsdiv<double>(c, F<double>(c, "dudx"), F<double>(c, "dvdy"), F<double>(c, "dwdz"));
+    if (out4) op_timestep_conditions<double>(c, F<double>(c, "u"), F<double>(c, "v"), F<double>(c, "w"),
+                                             F<double>(c, "Nu_t"), out4);
+  }
+  DONE(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// les_fft.h -- LDS-resident mixed-radix (4/2/3) FFT building blocks for gfx950.
+//
+// All horizontal transforms of the LES hot path (2^a 3^b lengths: 16..384)
+// run through one in-place Stockham routine that works on a batch of
+// transforms staged in LDS.  Element e of transform t lives at
+// buf[t*ts + e*es]; the work-item -> (transform, butterfly) mapping is a
+// template flag so that both layouts used here are bank-conflict free:
+//   * row transforms   (es = 1,  ts = n): lanes walk butterflies of one row,
+//   * column tiles     (es = TX, ts = 1): lanes walk adjacent columns, so a
+//     64-wide wavefront reads 64 consecutive complex words per access.
+// Twiddles come from a per-length table exp(-2*pi*i*k/n) copied to LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace les {
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+struct FftPlan {
+  int n;        // transform length
+  int ns;       // number of stages
+  int rad[12];  // radices, product == n
+};
+
+// R-point DFT in registers, sign -1 (forward) or +1 (backward).
+template <typename T, int R>
+__device__ __forceinline__ void butterfly(cx<T> (&v)[R], int sign) {
+  if constexpr (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  } else if constexpr (R == 4) {
+    cx<T> a0 = {v[0].x + v[2].x, v[0].y + v[2].y};
+    cx<T> a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    cx<T> a2 = {v[1].x + v[3].x, v[1].y + v[3].y};
+    cx<T> d = {v[1].x - v[3].x, v[1].y - v[3].y};
+    cx<T> a3 = sign < 0 ? cx<T>{d.y, -d.x} : cx<T>{-d.y, d.x};
+    v[0] = {a0.x + a2.x, a0.y + a2.y};
+    v[1] = {a1.x + a3.x, a1.y + a3.y};
+    v[2] = {a0.x - a2.x, a0.y - a2.y};
+    v[3] = {a1.x - a3.x, a1.y - a3.y};
+  } else {  // R == 3
+    const T s3 = T(0.86602540378443864676372317075294);
+    cx<T> sm = {v[1].x + v[2].x, v[1].y + v[2].y};
+    cx<T> df = {v[1].x - v[2].x, v[1].y - v[2].y};
+    cx<T> t1 = {v[0].x - T(0.5) * sm.x, v[0].y - T(0.5) * sm.y};
+    T f = sign < 0 ? -s3 : s3;
+    cx<T> t2 = {-f * df.y, f * df.x};
+    v[0] = {v[0].x + sm.x, v[0].y + sm.y};
+    v[1] = {t1.x + t2.x, t1.y + t2.y};
+    v[2] = {t1.x - t2.x, t1.y - t2.y};
+  }
+}
+
+// One Stockham stage over `ntr` transforms.  Every thread of the block must
+// call it (it contains block barriers).  Requires ntr*(n/R) <= MAXB*blockDim.
+template <typename T, int R, bool LANE_T, int MAXB>
+__device__ __forceinline__ void fft_stage(cx<T>* buf, int n, int es, int ts, int ntr, int Ns,
+                                          const cx<T>* tw, int sign) {
+  const int nb = n / R;
+  const int items = ntr * nb;
+  const int tws = n / (Ns * R);
+  cx<T> v[MAXB][R];
+  int tt[MAXB], dd[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int w = threadIdx.x + i * blockDim.x;
+    tt[i] = -1;
+    dd[i] = 0;
+    if (w < items) {
+      const int t = LANE_T ? (w % ntr) : (w / nb);
+      const int b = LANE_T ? (w / ntr) : (w % nb);
+      const int k = b % Ns;
+      cx<T>* base = buf + (long)t * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> a = base[(b + r * nb) * es];
+        if (r > 0 && k > 0) {
+          cx<T> wv = tw[r * k * tws];
+          if (sign > 0) wv.y = -wv.y;
+          a = cmul(a, wv);
+        }
+        v[i][r] = a;
+      }
+      butterfly<T, R>(v[i], sign);
+      tt[i] = t;
+      dd[i] = (b / Ns) * Ns * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    if (tt[i] >= 0) {
+      cx<T>* base = buf + (long)tt[i] * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[(dd[i] + r * Ns) * es] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-place transform of `ntr` transforms held in LDS.
+template <typename T, bool LANE_T, int MAXB>
+__device__ void fft_lds(cx<T>* buf, const FftPlan& p, int es, int ts, int ntr, const cx<T>* tw,
+                        int sign) {
+  int Ns = 1;
+  for (int s = 0; s < p.ns; ++s) {
+    const int R = p.rad[s];
+    if (R == 4)
+      fft_stage<T, 4, LANE_T, MAXB>(buf, p.n, es, ts, ntr, Ns, tw, sign);
+    else if (R == 2)
+      fft_stage<T, 2, LANE_T, MAXB>(buf, p.n, es, ts, ntr, Ns, tw, sign);
+    else
+      fft_stage<T, 3, LANE_T, MAXB>(buf, p.n, es, ts, ntr, Ns, tw, sign);
+    Ns *= R;
+  }
+}
+
+// Copy a global twiddle table of length n into LDS (all threads call).
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace les

@@ -1,0 +1,166 @@
+// les_internal.h -- device context, constant tables and launcher declarations
+// shared by the HIP kernels (les_kernels.hip) and the C-ABI (les_capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "les_fft.h"
+
+namespace les {
+
+// Grid and run constants (param.f90:56-65,98,100,144-151 of the reference),
+// plus the derived sizes used by the kernels.
+struct Grid {
+  int nx, ny, nz;       // nz = w-levels incl. lid (arrays (ld,ny,1:nz))
+  int lh, ld, nxh;      // lh = nx/2+1, ld = 2*lh, nxh = nx/2 (kept kx count)
+  int nx2, ny2;         // 3/2-padded sizes
+  double Lx, Ly, Lz, dx, dy, dz, dt, z0, force, delta;
+  int cs_count, inilag, initu;
+};
+
+// Per-precision device tables.
+template <typename T>
+struct Tables {
+  cx<T>* tw_nx = nullptr;   // exp(-2 pi i k/n) tables
+  cx<T>* tw_ny = nullptr;
+  cx<T>* tw_nx2 = nullptr;
+  cx<T>* tw_ny2 = nullptr;
+  T* kx = nullptr;          // (nxh)   kx(jx) for jx < nx/2  (fft.f90:60-76)
+  T* ky = nullptr;          // (ny)    ky(jy), Nyquist row 0
+  T* G2 = nullptr;          // (lh*ny) G_test      (2 Delta)
+  T* G4 = nullptr;          // (lh*ny) G_test_test (4 Delta)
+};
+
+// Device scratch: named slots that grow on demand and live with the context.
+struct Arena {
+  std::vector<void*> ptr;
+  std::vector<size_t> cap;
+};
+
+struct State;  // whole-step device state (les_capi.hip)
+
+struct ProfRec {
+  std::string tag;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+}  // namespace les
+
+struct les_ctx {
+  les::Grid g;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int jt = 0, jt_total = 0;        // param.f90:147-148
+  int flm_init = 0, fqn_init = 0;  // lagrange_Sdep.f90:67-68 SAVE flags
+  int keep2 = 0, keep4 = 0;        // kx columns with non-zero G_test / G_test_test
+  les::FftPlan p_nx, p_ny, p_nx2, p_ny2;
+  les::Tables<double> td;
+  les::Tables<float> tf;
+  double* G2_host = nullptr;       // host copies used for pruning decisions
+  les::Arena arena;
+  les::State* st = nullptr;
+  std::string err;
+  // optional per-launch HIP-event timing (les_profile)
+  bool prof = false;
+  std::vector<les::ProfRec> prof_recs;
+  std::string op_tag;
+};
+
+namespace les {
+
+// scratch slots
+enum Slot : int {
+  S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
+  S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE, S_NSLOTS
+};
+void* scratch(les_ctx* c, int slot, size_t bytes);
+
+template <typename T> const Tables<T>& tables(const les_ctx* c);
+
+// ---- launchers (les_kernels.hip); all enqueue on c->stream ----
+// horizontal spectral operators, fields (ld,ny,nlev)
+template <typename T> void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev);
+template <typename T> void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev);  // null outputs skipped
+template <typename T> void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep);
+template <typename T> void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev);
+template <typename T> void op_dealias2(les_ctx* c, T* u, T* u_big, int nlev);
+template <typename T> void op_padd(les_ctx* c, T* big_c, const T* small_c, int nplanes);
+template <typename T> void op_unpadd(les_ctx* c, T* small_c, const T* big_c, int nplanes);
+template <typename T> void op_ddz_uv(les_ctx* c, T* dfdz, const T* f);
+template <typename T> void op_ddz_w(les_ctx* c, T* dfdz, const T* f);
+template <typename T> void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, int wnode);
+template <typename T>
+void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,
+               const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy, T* cx, T* cy, T* cz);
+template <typename T>
+void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz, T* tyz,
+                   const double* psi_m, const double* psi_m0, const double* phi_m, double* ustar_avg);
+
+struct SgsPtrs {  // device pointers, (ld,ny,nz) unless noted
+  void *F_LM, *F_MM, *F_QN, *F_NN, *Beta, *Cs_opt2, *Nu_t, *u_lag, *v_lag, *w_lag;
+  void *TnLMMM, *TnQNNN, *L11t, *L22t, *L33t, *Q11t, *Q22t, *Q33t;
+  void *Beta_avg, *Betaclip_avg;  // (nz), double in every precision
+};
+template <typename T> void op_calc_sij(les_ctx* c, const T* const g[9], T* const S[6]);
+template <typename T> void op_interpolag(les_ctx* c, const SgsPtrs& s);
+template <typename T>
+void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s);
+template <typename T>
+void op_sgs_stag(les_ctx* c, const T* const g[9], const T* u, const T* v, const T* w, T* const tau[6],
+                 const SgsPtrs& s);
+
+template <typename T>
+void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy);
+int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u);
+template <typename T>
+void op_project(les_ctx* c, T* u, T* v, T* w, const T* dpdx, const T* dpdy, const T* dpdz);
+template <typename T> double op_rmsdiv(les_ctx* c, const T* dudx, const T* dvdy, const T* dwdz);
+template <typename T>
+void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, const T* nu, double out[4]);
+
+// step pieces
+template <typename T>
+void op_ab2(les_ctx* c, T* u, T* v, T* w, T* RHSx, T* RHSy, T* RHSz, const T* RHSx_f, const T* RHSy_f,
+            const T* RHSz_f, const T* divtx, const T* divty, const T* divtz, int euler_first);
+template <typename T>
+void op_pressure_update(les_ctx* c, T* u, T* v, T* w, T* RHSx, T* RHSy, T* RHSz, const T* p,
+                        const T* dpdx, const T* dpdy, T* dpdz);
+template <typename T> void op_fill(les_ctx* c, T* a, size_t n, T val);
+template <typename T> void op_copy(les_ctx* c, T* dst, const T* src, size_t n);
+
+void check(les_ctx* c, hipError_t e, const char* what);
+
+// Records HIP events around one launch on the context stream when profiling
+// is enabled; `bytes` is the launch's algorithmic HBM traffic (in + out).
+struct ProfScope {
+  les_ctx* c;
+  int idx = -1;
+  ProfScope(les_ctx* cc, const char* kind, double bytes) : c(cc) {
+    if (!c->prof) return;
+    ProfRec r;
+    r.tag = c->op_tag + "/" + kind;
+    r.bytes = bytes;
+    (void)hipEventCreate(&r.a);
+    (void)hipEventCreate(&r.b);
+    (void)hipEventRecord(r.a, c->stream);
+    c->prof_recs.push_back(r);
+    idx = (int)c->prof_recs.size() - 1;
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(c->prof_recs[idx].b, c->stream);
+  }
+};
+// Names the routine whose launches follow (profiling tags).
+struct OpTag {
+  les_ctx* c;
+  std::string old;
+  OpTag(les_ctx* cc, const char* name) : c(cc), old(cc->op_tag) { c->op_tag = name; }
+  ~OpTag() { c->op_tag = old; }
+};
+
+}  // namespace les

@@ -1,0 +1,1672 @@
+// les_kernels.hip -- gfx950 kernels and host launchers for the LES hot path.
+//
+// Horizontal operators are built from three transform kernels:
+//   k_row_fwd : real rows (x) -> first `kxo` Fourier coefficients, two real
+//               rows packed into one complex FFT (x-spectral rows)
+//   k_col     : LDS tile of TX adjacent kx columns x all ky of one plane;
+//               forward y-FFT, a fused spectral operation (i kx, i ky, test
+//               filter G, 3/2 zero-pad / truncate, divergence), inverse y-FFT
+//   k_row_inv : x-spectral rows -> real rows (c2r), with a fused epilogue
+// Every spectral operation of the reference zeroes the kx = nx/2 column and
+// the ky = ny/2 row, so the x-spectral intermediates keep only kx < nx/2
+// (and fewer for the sharp test filters, whose cutoff disc prunes columns).
+// Loaders / storers are functors so that pointwise work before and after a
+// transform (vorticity, staggered products, d/dz terms) is fused into the
+// transform kernels instead of making extra passes over HBM.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "les_internal.h"
+
+namespace les {
+
+constexpr double BOGUS = -1234567890.0;  // param.f90:50
+constexpr double VONK = 0.41;            // param.f90:98
+constexpr double TADV1 = 1.5, TADV2 = -0.5;
+
+void check(les_ctx* c, hipError_t e, const char* what) {
+  if (e != hipSuccess && c->err.empty()) {
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+  }
+}
+#define LCHK(c, call) check((c), (call), #call)
+
+template <>
+const Tables<double>& tables<double>(const les_ctx* c) { return c->td; }
+template <>
+const Tables<float>& tables<float>(const les_ctx* c) { return c->tf; }
+
+template <typename T>
+static const cx<T>* tw_for(const les_ctx* c, int n) {
+  const Tables<T>& t = tables<T>(c);
+  if (n == c->g.nx) return t.tw_nx;
+  if (n == c->g.ny) return t.tw_ny;
+  if (n == c->g.nx2) return t.tw_nx2;
+  return t.tw_ny2;
+}
+static const FftPlan& plan_for(const les_ctx* c, int n) {
+  if (n == c->g.nx) return c->p_nx;
+  if (n == c->g.ny) return c->p_ny;
+  if (n == c->g.nx2) return c->p_nx2;
+  return c->p_ny2;
+}
+
+// ------------------------------------------------------------------------
+// row transforms
+// ------------------------------------------------------------------------
+template <typename T, class Loader>
+__global__ void __launch_bounds__(1024) k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int npairs,
+                                                  FftPlan p, const cx<T>* __restrict__ twg, int kxo) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n = p.n;
+  cx<T>* buf = tw + n;
+  const int P = (blockDim.x * 4) / n;
+  load_twiddles(tw, twg, n);
+  const int pair0 = blockIdx.x * P;
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    const int t = e / n, x = e - t * n, pg = pair0 + t;
+    cx<T> z = {T(0), T(0)};
+    if (pg < npairs) {
+      z.x = L(2 * pg, x);
+      z.y = L(2 * pg + 1, x);
+    }
+    buf[e] = z;
+  }
+  __syncthreads();
+  fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
+  for (int e = threadIdx.x; e < P * kxo; e += blockDim.x) {
+    const int t = e / kxo, k = e - t * kxo, pg = pair0 + t;
+    if (pg >= npairs) continue;
+    const cx<T> zk = buf[t * n + k], zn = buf[t * n + (k ? n - k : 0)];
+    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+    out[(long)(2 * pg) * orow + k] = A;
+    out[(long)(2 * pg + 1) * orow + k] = B;
+  }
+}
+
+template <typename T, class Storer>
+__global__ void __launch_bounds__(1024) k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S,
+                                                  int npairs, FftPlan p, const cx<T>* __restrict__ twg, int ldo) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n = p.n;
+  cx<T>* buf = tw + n;
+  const int P = (blockDim.x * 4) / n;
+  load_twiddles(tw, twg, n);
+  const int pair0 = blockIdx.x * P;
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    const int t = e / n, j = e - t * n, pg = pair0 + t;
+    cx<T> z = {T(0), T(0)};
+    if (pg < npairs) {
+      const cx<T>* a = in + (long)(2 * pg) * irow;
+      const cx<T>* b = a + irow;
+      if (j == 0) {
+        z = {a[0].x, b[0].x};  // Im of the DC entries is ignored (halfcomplex c2r)
+      } else if (j < kxi) {
+        const cx<T> A = a[j], B = b[j];
+        z = {A.x - B.y, A.y + B.x};
+      } else if (n - j < kxi) {
+        const cx<T> A = a[n - j], B = b[n - j];
+        z = {A.x + B.y, B.x - A.y};
+      }
+    }
+    buf[e] = z;
+  }
+  __syncthreads();
+  fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
+  for (int e = threadIdx.x; e < P * ldo; e += blockDim.x) {
+    const int t = e / ldo, x = e - t * ldo, pg = pair0 + t;
+    if (pg >= npairs) continue;
+    T v0 = T(0), v1 = T(0);
+    if (x < n) {
+      v0 = buf[t * n + x].x;
+      v1 = buf[t * n + x].y;
+    }
+    S(2 * pg, x, v0);
+    S(2 * pg + 1, x, v1);
+  }
+}
+
+template <typename T, class L>
+static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo) {
+  const FftPlan& p = plan_for(c, n);
+  const int tpf = n / 4;
+  int P = 256 / tpf;
+  if (P < 1) P = 1;
+  const int threads = P * tpf;
+  const long npairs = nrows / 2;
+  const long blocks = (npairs + P - 1) / P;
+  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  ProfScope ps(c, "row_fwd", (double)nrows * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
+  hipLaunchKernelGGL((k_row_fwd<T, L>), dim3((unsigned)blocks), dim3(threads), sm, c->stream, ld, out, orow,
+                     (int)npairs, p, tw_for<T>(c, n), kxo);
+  LCHK(c, hipGetLastError());
+}
+
+template <typename T, class S>
+static void launch_row_inv(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nrows, int n,
+                           int ldo) {
+  const FftPlan& p = plan_for(c, n);
+  const int tpf = n / 4;
+  int P = 256 / tpf;
+  if (P < 1) P = 1;
+  const int threads = P * tpf;
+  const long npairs = nrows / 2;
+  const long blocks = (npairs + P - 1) / P;
+  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  ProfScope ps(c, "row_inv", (double)nrows * (kxi * sizeof(cx<T>) + ldo * sizeof(T) * st.writes()));
+  hipLaunchKernelGGL((k_row_inv<T, S>), dim3((unsigned)blocks), dim3(threads), sm, c->stream, in, irow, kxi, st,
+                     (int)npairs, p, tw_for<T>(c, n), ldo);
+  LCHK(c, hipGetLastError());
+}
+
+// loaders / storers ------------------------------------------------------
+// row index = (field * nlev + level) * ny + y; fields are (ld, ny, nlev).
+template <typename T>
+struct LoadFields {
+  const T* p[4];
+  int rows_per_field;
+  int ld;
+  T scale;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int row, int x) const {
+    const int f = row / rows_per_field, r = row - f * rows_per_field;
+    return scale * p[f][(long)r * ld + x];
+  }
+};
+template <typename T>
+struct StoreFields {
+  T* p[4];
+  int rows_per_field;
+  int ld, nx;  // pads (x >= nx) are written 0
+  static constexpr double writes() { return 1.0; }
+  __device__ void operator()(int row, int x, T v) const {
+    const int f = row / rows_per_field, r = row - f * rows_per_field;
+    p[f][(long)r * ld + x] = x < nx ? v : T(0);
+  }
+};
+
+// ------------------------------------------------------------------------
+// column (y) kernel with fused spectral operation
+// ------------------------------------------------------------------------
+enum Kind : int { K_ID = 0, K_IKX = 1, K_IKY = 2, K_G = 3 };
+
+struct ColArgs {
+  int ny_in, ny_out, do_fwd, do_inv;
+  int nki;               // input kx columns present
+  int nout;              // 1..3 outputs
+  int kind[3];
+  int nko[3];            // output kx columns written
+  long in_row, in_plane, in2_plane;
+  long out_row[3], out_plane[3];
+  int lhG;               // row length of the G tables (lh)
+};
+
+__device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
+  if (nyi == nyo) return (yo == nyo / 2) ? -1 : yo;
+  if (nyo > nyi) {  // zero-pad (padd.f90)
+    if (yo < nyi / 2) return yo;
+    if (yo >= nyo - nyi / 2 + 1) return yo - nyo + nyi;
+    return -1;
+  }
+  if (yo < nyo / 2) return yo;  // truncate (unpadd.f90)
+  if (yo == nyo / 2) return -1;
+  return yo + nyi - nyo;
+}
+
+template <typename T, int TX>
+__global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2,
+                                             cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a, FftPlan pin,
+                                             FftPlan pout, const cx<T>* __restrict__ twin,
+                                             const cx<T>* __restrict__ twout, const T* __restrict__ kxt,
+                                             const T* __restrict__ kyt, const T* __restrict__ G0,
+                                             const T* __restrict__ G1, const T* __restrict__ G2) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* twi = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* two = twi + a.ny_in;
+  cx<T>* A = two + a.ny_out;
+  cx<T>* B = A + a.ny_in * TX;
+  load_twiddles(twi, twin, a.ny_in);
+  load_twiddles(two, twout, a.ny_out);
+  const int kx0 = blockIdx.x * TX;
+  const long pl = blockIdx.y;
+  {
+    const cx<T>* src = in + pl * a.in_plane;
+    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      A[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+    }
+  }
+  __syncthreads();
+  if (a.do_fwd) fft_lds<T, true, 4>(A, pin, TX, 1, TX, twi, -1);
+  if (in2) {
+    // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
+    const cx<T>* src = in2 + pl * a.in2_plane;
+    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      B[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<T, true, 4>(B, pin, TX, 1, TX, twi, -1);
+    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      cx<T> r = {T(0), T(0)};
+      if (y != a.ny_in / 2 && kk < a.nki) {
+        const T kx = kxt[kk], ky = kyt[y];
+        const cx<T> X = A[e], Y = B[e];
+        r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
+      }
+      A[e] = r;
+    }
+    __syncthreads();
+    if (a.do_inv) fft_lds<T, true, 4>(A, pout, TX, 1, TX, two, +1);
+    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = A[e];
+    }
+    return;
+  }
+  for (int j = 0; j < a.nout; ++j) {
+    if (kx0 >= a.nko[j]) continue;  // uniform over the block
+    const int kind = a.kind[j];
+    const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
+    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+      const int yo = e / TX, c = e - yo * TX, kk = kx0 + c;
+      const int yi = map_row(yo, a.ny_in, a.ny_out);
+      cx<T> v = {T(0), T(0)};
+      if (yi >= 0 && kk < a.nki) {
+        v = A[yi * TX + c];
+        if (kind == K_IKX) {
+          const T k = kxt[kk];
+          v = {-k * v.y, k * v.x};
+        } else if (kind == K_IKY) {
+          const T k = kyt[yo];
+          v = {-k * v.y, k * v.x};
+        } else if (kind == K_G) {
+          const T g = G[(long)yo * a.lhG + kk];
+          v = {g * v.x, g * v.y};
+        }
+      }
+      B[e] = v;
+    }
+    __syncthreads();
+    if (a.do_inv) fft_lds<T, true, 4>(B, pout, TX, 1, TX, two, +1);
+    cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
+    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = B[e];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                       const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
+                       const T* G2 = nullptr) {
+  constexpr int TX = sizeof(T) == 8 ? 8 : 16;
+  const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
+  // fft_lds<.., MAXB=4> needs blockDim >= TX*n/8 for its radix-2 stages
+  int threads = (TX * nmax / 8 + 63) / 64 * 64;
+  if (threads < 64) threads = 64;
+  if (threads > 1024) {
+    c->err = "k_col: column tile too large";
+    return;
+  }
+  const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
+  if (sm > 65536) LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const Tables<T>& t = tables<T>(c);
+  const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
+  double bytes = 0;
+  {
+    const double es = 2.0 * sizeof(T);
+    bytes += (double)nplanes * a.ny_in * (ncols < a.nki ? ncols : a.nki) * es * (in2 ? 2 : 1);
+    for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
+  }
+  ProfScope ps(c, "col", bytes);
+  hipLaunchKernelGGL((k_col<T, TX>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2, o0, o1, o2,
+                     a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in), tw_for<T>(c, a.ny_out),
+                     t.kx, t.ky, G0, G1, G2);
+  LCHK(c, hipGetLastError());
+}
+
+static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {
+  ColArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.ny_in = ny_in;
+  a.ny_out = ny_out;
+  a.do_fwd = 1;
+  a.do_inv = 1;
+  a.nki = nki;
+  a.in_row = in_row;
+  a.in_plane = in_plane;
+  return a;
+}
+static void col_out(ColArgs& a, int j, int kind, int nko, long out_row, long out_plane) {
+  a.kind[j] = kind;
+  a.nko[j] = nko;
+  a.out_row[j] = out_row;
+  a.out_plane[j] = out_plane;
+  if (a.nout < j + 1) a.nout = j + 1;
+}
+
+// ------------------------------------------------------------------------
+// simple elementwise kernels
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* a, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+template <typename T>
+void op_fill(les_ctx* c, T* a, size_t n, T val) {
+  if (!n) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  {
+    ProfScope _ps(c, "fill", (double)n * sizeof(T));
+    hipLaunchKernelGGL((k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, a, n, val);
+  }
+  LCHK(c, hipGetLastError());
+}
+template <typename T>
+__global__ void k_copy(T* d, const T* s, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+template <typename T>
+void op_copy(les_ctx* c, T* dst, const T* src, size_t n) {
+  if (!n) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  {
+    ProfScope _ps(c, "copy", 2.0 * n * sizeof(T));
+    hipLaunchKernelGGL((k_copy<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, dst, src, n);
+  }
+  LCHK(c, hipGetLastError());
+}
+// complex-(BOGUS,0) pattern written by press_stag_array.f90:288-290
+template <typename T>
+__global__ void k_fill_bogus_cplx(T* a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = (i & 1) ? T(0) : T(BOGUS);
+}
+
+static dim3 grid1(size_t n, int bs = 256) {
+  size_t b = (n + bs - 1) / bs;
+  if (b > 1048576) b = 1048576;
+  return dim3((unsigned)b);
+}
+
+// ------------------------------------------------------------------------
+// filt_da.f90 / ddx.f90 / ddy.f90 / ddxy.f90
+// ------------------------------------------------------------------------
+template <typename T>
+static void spectral_deriv(les_ctx* c, const T* f, int nlev, T* const outs[3], const int kinds[3], int nout) {
+  const Grid& g = c->g;
+  const long rows = (long)nlev * g.ny;
+  const size_t xs = (size_t)rows * g.nxh * sizeof(cx<T>);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, xs);
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs * nout);
+  LoadFields<T> L{{f, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows, g.nx, g.nxh);
+  ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  for (int j = 0; j < nout; ++j) col_out(a, j, kinds[j], g.nxh, g.nxh, (long)g.ny * g.nxh);
+  const size_t fo = (size_t)rows * g.nxh;
+  launch_col<T>(c, X0, nullptr, X1, X1 + fo, X1 + 2 * fo, a, nlev, g.nxh);
+  StoreFields<T> S{{outs[0], outs[1], outs[2], nullptr}, (int)rows, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nout, g.nx, g.ld);
+}
+
+template <typename T>
+void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev) {
+  OpTag _tag(c, "filt_da");
+  T* outs[3] = {f, dfdx, dfdy};
+  const int kinds[3] = {K_ID, K_IKX, K_IKY};
+  spectral_deriv<T>(c, f, nlev, outs, kinds, 3);
+}
+
+template <typename T>
+void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev) {
+  OpTag _tag(c, "ddxy");
+  T* outs[3] = {nullptr, nullptr, nullptr};
+  int kinds[3] = {0, 0, 0};
+  int n = 0;
+  if (dfdx) { outs[n] = dfdx; kinds[n++] = K_IKX; }
+  if (dfdy) { outs[n] = dfdy; kinds[n++] = K_IKY; }
+  if (n) spectral_deriv<T>(c, f, nlev, outs, kinds, n);
+}
+
+// test_filtermodule.f90:13-30 applied to `nplanes` (ld,ny) planes in place
+template <typename T>
+void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep) {
+  OpTag _tag(c, "test_filter");
+  const Grid& g = c->g;
+  if (keep <= 0 || keep > g.nxh) keep = g.nxh;
+  const long rows = (long)nplanes * g.ny;
+  const size_t xs = (size_t)rows * keep * sizeof(cx<T>);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, xs);
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
+  LoadFields<T> L{{planes, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1)};
+  launch_row_fwd<T>(c, L, X0, keep, rows, g.nx, keep);
+  ColArgs a = col_args(g.ny, g.ny, keep, keep, (long)g.ny * keep);
+  a.lhG = g.lh;
+  col_out(a, 0, K_G, keep, keep, (long)g.ny * keep);
+  launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nplanes, keep, G);
+  StoreFields<T> S{{planes, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, keep, keep, S, rows, g.nx, g.ld);
+}
+
+// dealias1.f90: u (ld,ny,nlev) -> u_big (ld_big,ny2,nlev); u untouched
+template <typename T>
+void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev) {
+  OpTag _tag(c, "dealias1");
+  const Grid& g = c->g;
+  const long rows = (long)nlev * g.ny, rows2 = (long)nlev * g.ny2;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)rows2 * g.nxh * sizeof(cx<T>));
+  LoadFields<T> L{{u, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows, g.nx, g.nxh);
+  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nlev, g.nxh);
+  const int ldb = 2 * (g.nx2 / 2 + 1);
+  StoreFields<T> S{{u_big, nullptr, nullptr, nullptr}, (int)rows2, ldb, g.nx2};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows2, g.nx2, ldb);
+}
+
+// dealias2.f90: u_big (ld_big,ny2,nlev) -> u (ld,ny,nlev).  The reference
+// leaves the scaled spectrum of u_big behind in u_big; here u_big is read only.
+template <typename T>
+void op_dealias2(les_ctx* c, T* u, T* u_big, int nlev) {
+  OpTag _tag(c, "dealias2");
+  const Grid& g = c->g;
+  const long rows = (long)nlev * g.ny, rows2 = (long)nlev * g.ny2;
+  const int ldb = 2 * (g.nx2 / 2 + 1);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows2 * g.nxh * sizeof(cx<T>));
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * g.nxh * sizeof(cx<T>));
+  LoadFields<T> L{{u_big, nullptr, nullptr, nullptr}, (int)rows2, ldb, T(1.0 / ((double)g.nx2 * g.ny2))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows2, g.nx2, g.nxh);
+  ColArgs a = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nlev, g.nxh);
+  StoreFields<T> S{{u, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld);
+}
+
+// padd.f90 / unpadd.f90 on complex (lh,ny) <-> (lh_big,ny2) planes
+template <typename T>
+__global__ void k_padd(cx<T>* big, const cx<T>* sm, int nx, int ny, int ny2, int lh, int lhb, int nplanes) {
+  const long n = (long)lhb * ny2 * nplanes;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long pl = i / ((long)lhb * ny2);
+    const int r = (int)(i - pl * lhb * ny2), yb = r / lhb, x = r - yb * lhb;
+    const int ys = map_row(yb, ny, ny2);
+    cx<T> v = {T(0), T(0)};
+    if (ys >= 0 && x < nx / 2) v = sm[pl * lh * ny + (long)ys * lh + x];
+    big[i] = v;
+  }
+}
+template <typename T>
+__global__ void k_unpadd(cx<T>* sm, const cx<T>* big, int nx, int ny, int ny2, int lh, int lhb, int nplanes) {
+  const long n = (long)lh * ny * nplanes;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long pl = i / ((long)lh * ny);
+    const int r = (int)(i - pl * lh * ny), ys = r / lh, x = r - ys * lh;
+    const int yb = map_row(ys, ny2, ny);
+    cx<T> v = {T(0), T(0)};
+    if (yb >= 0 && x < nx / 2) v = big[pl * lhb * ny2 + (long)yb * lhb + x];
+    sm[i] = v;
+  }
+}
+template <typename T>
+void op_padd(les_ctx* c, T* big_c, const T* small_c, int nplanes) {
+  const Grid& g = c->g;
+  const int lhb = g.nx2 / 2 + 1;
+  hipLaunchKernelGGL((k_padd<T>), grid1((size_t)lhb * g.ny2 * nplanes), dim3(256), 0, c->stream, (cx<T>*)big_c,
+                     (const cx<T>*)small_c, g.nx, g.ny, g.ny2, g.lh, lhb, nplanes);
+  LCHK(c, hipGetLastError());
+}
+template <typename T>
+void op_unpadd(les_ctx* c, T* small_c, const T* big_c, int nplanes) {
+  const Grid& g = c->g;
+  const int lhb = g.nx2 / 2 + 1;
+  hipLaunchKernelGGL((k_unpadd<T>), grid1((size_t)g.lh * g.ny * nplanes), dim3(256), 0, c->stream, (cx<T>*)small_c,
+                     (const cx<T>*)big_c, g.nx, g.ny, g.ny2, g.lh, lhb, nplanes);
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// ddz_uv.f90 / ddz_w.f90
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ddz(T* __restrict__ dfdz, const T* __restrict__ f, int nx, int ld, int ny, int nz, T cst,
+                      int wnode) {
+  const long pl = (long)ld * ny, n = pl * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    if (k == nz - 1) {
+      dfdz[i] = T(0);
+    } else if (x < nx) {
+      if (wnode)
+        dfdz[i] = cst * (f[i + pl] - f[i]);  // ddz_w: k = 1..nz-1
+      else if (k >= 1)
+        dfdz[i] = cst * (f[i] - f[i - pl]);  // ddz_uv: k = 2..nz-1
+    }
+  }
+}
+template <typename T>
+void op_ddz_uv(les_ctx* c, T* dfdz, const T* f) {
+  const Grid& g = c->g;
+  {
+    ProfScope _ps(c, "ddz", 2.0 * g.ld * g.ny * g.nz * sizeof(T));
+    hipLaunchKernelGGL((k_ddz<T>), grid1((size_t)g.ld * g.ny * g.nz), dim3(256), 0, c->stream, dfdz, f, g.nx, g.ld,
+                       g.ny, g.nz, T(1.0 / g.dz), 0);
+  }
+  LCHK(c, hipGetLastError());
+}
+template <typename T>
+void op_ddz_w(les_ctx* c, T* dfdz, const T* f) {
+  const Grid& g = c->g;
+  {
+    ProfScope _ps(c, "ddz", 2.0 * g.ld * g.ny * g.nz * sizeof(T));
+    hipLaunchKernelGGL((k_ddz<T>), grid1((size_t)g.ld * g.ny * g.nz), dim3(256), 0, c->stream, dfdz, f, g.nx, g.ld,
+                       g.ny, g.nz, T(1.0 / g.dz), 1);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// divstress_uv.f90 / divstress_w.f90:  d/dx tx + d/dy ty (one spectral
+// pass) + d/dz tz fused into the c2r epilogue
+// ------------------------------------------------------------------------
+template <typename T>
+struct StoreDiv {
+  T* divt;
+  const T* tz;
+  int ny, ld, nx, nz, wnode;
+  T rdz;
+  static constexpr double writes() { return 2.0; }  // divt + tz read
+  __device__ void operator()(int row, int x, T v) const {
+    const int k = row / ny;
+    const long i = (long)row * ld + x;
+    const long pl = (long)ld * ny;
+    if (!wnode && k == nz - 1) { divt[i] = T(BOGUS); return; }
+    if (x >= nx) { divt[i] = T(0); return; }
+    T d;
+    if (wnode) {
+      if (k == 0) d = v;                                            // divstress_w.f90:107-115
+      else if (k < nz - 1) d = v + rdz * (tz[i] - tz[i - pl]);      // ddz_uv
+      else d = v + T(0);                                            // ddz_uv: dfdz(nz) = 0
+    } else {
+      d = v + rdz * (tz[i + pl] - tz[i]);                           // ddz_w
+    }
+    divt[i] = d;
+  }
+};
+
+template <typename T>
+void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, int wnode) {
+  OpTag _tag(c, "divstress");
+  const Grid& g = c->g;
+  const int nlev = g.nz;
+  const long rows = (long)nlev * g.ny;
+  const size_t xs = (size_t)rows * g.nxh * sizeof(cx<T>);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, 2 * xs);
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
+  LoadFields<T> L{{tx, ty, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, 2 * rows, g.nx, g.nxh);
+  ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  a.in2_plane = a.in_plane;
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, X0, X0 + (size_t)rows * g.nxh, X1, nullptr, nullptr, a, nlev, g.nxh);
+  StoreDiv<T> S{divt, tz, g.ny, g.ld, g.nx, g.nz, wnode, T(1.0 / g.dz)};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld);
+}
+
+// ------------------------------------------------------------------------
+// convec.f90: -(u x omega) with 3/2 dealiasing.  Fields never exist on the
+// padded grid in physical form except inside `U6` (one pass); vorticity is
+// formed in the forward-transform loader, the staggered products in the
+// big-grid forward-transform loader.
+// ------------------------------------------------------------------------
+template <typename T>
+struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
+  const T *u1, *u2, *u3, *du1d2, *du1d3, *du2d1, *du2d3, *du3d1, *du3d2;
+  int ny, nz, ld;
+  T cst;
+  static constexpr double reads() { return 1.5; }  // 9 fields feed 6 rows
+  __device__ T operator()(int row, int x) const {
+    const int rpf = nz * ny;
+    const int f = row / rpf, r = row - f * rpf, k = r / ny;
+    const long i = (long)r * ld + x, pl = (long)ld * ny;
+    switch (f) {
+      case 0: return cst * u1[i];
+      case 1: return cst * u2[i];
+      case 2: return cst * u3[i];
+      case 3:
+        if (k == 0) return cst * (T(0.5) * (du3d2[i] + du3d2[i + pl]) - du2d3[i]);
+        return cst * (du3d2[i] - du2d3[i]);
+      case 4:
+        if (k == 0) return cst * (du1d3[i] - T(0.5) * (du3d1[i] + du3d1[i + pl]));
+        return cst * (du1d3[i] - du3d1[i]);
+      default: return cst * (du2d1[i] - du1d2[i]);
+    }
+  }
+};
+
+template <typename T>
+struct LoadProducts {  // fields cx, cy, cz on the padded grid; levels 0..nz-2
+  const T* U;          // 6 big fields, (nx2, ny2, nz), row stride nx2
+  int ny2, nz, nx2;
+  T cst;
+  static constexpr double reads() { return 2.0; }  // 6 big fields feed 3 rows
+  __device__ T operator()(int row, int x) const {
+    const int nl = nz - 1;
+    const int rpf = nl * ny2;
+    const int f = row / rpf, r = row - f * rpf, k = r / ny2, y = r - k * ny2;
+    const long fs = (long)nx2 * ny2 * nz, pl = (long)nx2 * ny2;
+    const long i = (long)k * pl + (long)y * nx2 + x;
+    const T* u1 = U;
+    const T* u2 = U + fs;
+    const T* u3 = U + 2 * fs;
+    const T* v1 = U + 3 * fs;
+    const T* v2 = U + 4 * fs;
+    const T* v3 = U + 5 * fs;
+    if (f == 0) {
+      if (k == 0) return cst * (u2[i] * (-v3[i]) + T(0.5) * u3[i + pl] * (v2[i + pl]));
+      return cst * (u2[i] * (-v3[i]) + T(0.5) * (u3[i + pl] * (v2[i + pl]) + u3[i] * (v2[i])));
+    } else if (f == 1) {
+      if (k == 0) return cst * (u1[i] * (v3[i]) + T(0.5) * u3[i + pl] * (-v1[i + pl]));
+      return cst * (u1[i] * (v3[i]) + T(0.5) * (u3[i + pl] * (-v1[i + pl]) + u3[i] * (-v1[i])));
+    }
+    if (k == 0) return T(0);
+    return cst * T(0.5) * ((u1[i] + u1[i - pl]) * (-v2[i]) + (u2[i] + u2[i - pl]) * (v1[i]));
+  }
+};
+
+template <typename T>
+struct StoreBig {
+  T* p;
+  int nx2;
+  static constexpr double writes() { return 1.0; }
+  __device__ void operator()(int row, int x, T v) const {
+    if (x < nx2) p[(long)row * nx2 + x] = v;
+  }
+};
+
+template <typename T>
+struct StoreConvec {  // 3 outputs, levels 0..nz-2 ; level nz-1 BOGUS handled apart
+  T *ox, *oy, *oz;
+  int rows_per_field, ld, nx;
+  static constexpr double writes() { return 1.0; }
+  __device__ void operator()(int row, int x, T v) const {
+    const int f = row / rows_per_field, r = row - f * rows_per_field;
+    T* p = f == 0 ? ox : (f == 1 ? oy : oz);
+    p[(long)r * ld + x] = x < nx ? v : T(0);
+  }
+};
+
+template <typename T>
+void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz, const T* dvdx,
+               const T* dvdz, const T* dwdx, const T* dwdy, T* ox, T* oy, T* oz) {
+  OpTag _tag(c, "convec");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const long rows6 = 6L * nz * g.ny;
+  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
+  LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh);
+  // pad in y: 6*nz planes
+  const long rowsB = 6L * nz * g.ny2;
+  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
+  // big physical fields
+  T* U = (T*)scratch(c, S_BIG0, (size_t)rowsB * g.nx2 * sizeof(T));
+  StoreBig<T> SB{U, g.nx2};
+  launch_row_inv<T>(c, XB, g.nxh, g.nxh, SB, rowsB, g.nx2, g.nx2);
+  // products -> x-spectral (kx < nx/2) on the padded y grid
+  const long rowsP = 3L * (nz - 1) * g.ny2;
+  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)rowsP * g.nxh * sizeof(cx<T>));
+  LoadProducts<T> LP{U, g.ny2, nz, g.nx2, T(1.0 / ((double)g.nx2 * g.ny2))};
+  launch_row_fwd<T>(c, LP, XP, g.nxh, rowsP, g.nx2, g.nxh);
+  // truncate in y
+  const long rows3 = 3L * (nz - 1) * g.ny;
+  cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)rows3 * g.nxh * sizeof(cx<T>));
+  ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, 3L * (nz - 1), g.nxh);
+  StoreConvec<T> SC{ox, oy, oz, (int)((nz - 1) * g.ny), g.ld, g.nx};
+  launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, rows3, g.nx, g.ld);
+  const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
+  op_fill<T>(c, ox + top, pl, T(BOGUS));
+  op_fill<T>(c, oy + top, pl, T(BOGUS));
+  op_fill<T>(c, oz + top, pl, T(BOGUS));
+}
+
+// ------------------------------------------------------------------------
+// wallstress.f90 ('wall'): 4-Delta test filter of u, v at k=1 and the log
+// law, FP64 throughout
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_wall(const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ u1f,
+                       const T* __restrict__ v1f, T* dudz, T* dvdz, T* txz, T* tyz, const double* psi_m,
+                       const double* psi_m0, const double* phi_m, double* ustar_avg, int nx, int ny, int ld,
+                       double dz, double z0) {
+  const int n = nx * ny;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    const long i = x + (long)ld * y;
+    const double u1 = (double)u1f[i], v1 = (double)v1f[i];
+    const double denom = log(0.5 * dz / z0) - psi_m[j] + psi_m0[j];
+    const double u_avg = sqrt(u1 * u1 + v1 * v1);
+    const double us = u_avg * VONK / denom;
+    ustar_avg[j] = us;
+    const double cst = -(us * us) / u_avg;
+    txz[i] = (T)(cst * u1);
+    tyz[i] = (T)(cst * v1);
+    const double du = us / (0.5 * dz * VONK) * u1 / u_avg * phi_m[j];
+    const double dv = us / (0.5 * dz * VONK) * v1 / u_avg * phi_m[j];
+    dudz[i] = u[i] == T(0) ? T(0) : (T)du;
+    dvdz[i] = v[i] == T(0) ? T(0) : (T)dv;
+  }
+}
+
+template <typename T>
+void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz, T* tyz, const double* psi_m,
+                   const double* psi_m0, const double* phi_m, double* ustar_avg) {
+  OpTag _tag(c, "wallstress");
+  const Grid& g = c->g;
+  const size_t pl = (size_t)g.ld * g.ny;
+  T* planes = (T*)scratch(c, S_PLANE, 2 * pl * sizeof(T));
+  LCHK(c, hipMemcpyAsync(planes, u, pl * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+  LCHK(c, hipMemcpyAsync(planes + pl, v, pl * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+  op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);
+  {
+    ProfScope _ps(c, "wall", 8.0 * g.nx * g.ny * sizeof(T));
+    hipLaunchKernelGGL((k_wall<T>), grid1((size_t)g.nx * g.ny), dim3(256), 0, c->stream, u, v, planes, planes + pl,
+                       dudz, dvdz, txz, tyz, psi_m, psi_m0, phi_m, ustar_avg, g.nx, g.ny, g.ld, g.dz, g.z0);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// sgs_stag.f90: calc_Sij (:371-498), Nu_t and tau (:176-364)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_calc_sij(const T* __restrict__ ux_, const T* __restrict__ uy_, const T* __restrict__ uz_,
+                           const T* __restrict__ vx_, const T* __restrict__ vy_, const T* __restrict__ vz_,
+                           const T* __restrict__ wx_, const T* __restrict__ wy_, const T* __restrict__ wz_,
+                           T* S11, T* S12, T* S13, T* S22, T* S23, T* S33, int nx, int ld, int ny, int nz,
+                           const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w, T* u_lag,
+                           T* v_lag, T* w_lag) {
+  const long pl = (long)ld * ny, n = pl * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    if (u_lag) {  // sgs_stag.f90:62-64, whole arrays
+      u_lag[i] = u_lag[i] + u[i];
+      v_lag[i] = v_lag[i] + v[i];
+      w_lag[i] = w_lag[i] + w[i];
+    }
+    T s11 = T(0), s12 = T(0), s13 = T(0), s22 = T(0), s23 = T(0), s33 = T(0);
+    if (x < nx) {
+      T ux, uy, uz, vx, vy, vz, wx, wy, wz;
+      if (k == 0) {
+        ux = ux_[i]; uy = uy_[i]; uz = uz_[i];
+        vx = vx_[i]; vy = vy_[i]; vz = vz_[i];
+        wx = T(0.5) * (wx_[i] + wx_[i + pl]);
+        wy = T(0.5) * (wy_[i] + wy_[i + pl]);
+        wz = wz_[i];
+      } else {
+        const long m = i - pl;
+        ux = T(0.5) * (ux_[i] + ux_[m]);
+        uy = T(0.5) * (uy_[i] + uy_[m]);
+        uz = uz_[i];
+        vx = T(0.5) * (vx_[i] + vx_[m]);
+        vy = T(0.5) * (vy_[i] + vy_[m]);
+        vz = vz_[i];
+        wx = wx_[i];
+        wy = wy_[i];
+        wz = T(0.5) * (wz_[i] + wz_[m]);
+      }
+      s11 = ux; s12 = T(0.5) * (uy + vx); s13 = T(0.5) * (uz + wx);
+      s22 = vy; s23 = T(0.5) * (vz + wy); s33 = wz;
+    }
+    S11[i] = s11; S12[i] = s12; S13[i] = s13; S22[i] = s22; S23[i] = s23; S33[i] = s33;
+  }
+}
+
+template <typename T>
+void op_calc_sij(les_ctx* c, const T* const gr[9], T* const S[6]) {
+  const Grid& g = c->g;
+  hipLaunchKernelGGL((k_calc_sij<T>), grid1((size_t)g.ld * g.ny * g.nz), dim3(256), 0, c->stream, gr[0], gr[1],
+                     gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx,
+                     g.ld, g.ny, g.nz, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr,
+                     (T*)nullptr, (T*)nullptr);
+  LCHK(c, hipGetLastError());
+}
+
+template <typename T>
+__device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
+  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
+template <typename T>
+__global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, const T* __restrict__ S13,
+                          const T* __restrict__ S22, const T* __restrict__ S23, const T* __restrict__ S33,
+                          const T* __restrict__ Cs, T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy,
+                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2) {
+  const long pl = (long)ld * ny, n = pl * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    if (use_const) Cs_out[i] = cs_const;  // sgs_stag.f90:166-173, whole arrays
+    if (k == nz - 1) {                    // sgs_stag.f90:356-364
+      txx[i] = T(BOGUS); txy[i] = T(BOGUS); tyy[i] = T(BOGUS); tzz[i] = T(BOGUS);
+      txz[i] = T(0); tyz[i] = T(0);
+    }
+    if (x >= nx) continue;
+    const T c0 = use_const ? cs_const : Cs[i];
+    const T nu0 = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]) * c0 * l2;
+    Nu_t[i] = nu0;
+    if (k == nz - 1) continue;
+    if (k == 0) {  // wall level, sgs_stag.f90:223-241 (txz, tyz(1) kept from wallstress)
+      const T cst = nu0;
+      txx[i] = T(-2) * (cst + T(0)) * S11[i];
+      txy[i] = T(-2) * (cst + T(0)) * S12[i];
+      tyy[i] = T(-2) * (cst + T(0)) * S22[i];
+      tzz[i] = T(-2) * (cst + T(0)) * S33[i];
+    } else {
+      const long p = i + pl;
+      const T c1 = use_const ? cs_const : Cs[p];
+      const T nu1 = smag(S11[p], S12[p], S13[p], S22[p], S23[p], S33[p]) * c1 * l2;
+      const T cst = T(0.5) * (nu0 + nu1);
+      txx[i] = T(-2) * (cst + T(0)) * T(0.5) * (S11[i] + S11[p]);
+      txy[i] = T(-2) * (cst + T(0)) * T(0.5) * (S12[i] + S12[p]);
+      tyy[i] = T(-2) * (cst + T(0)) * T(0.5) * (S22[i] + S22[p]);
+      tzz[i] = T(-2) * (cst + T(0)) * T(0.5) * (S33[i] + S33[p]);
+      txz[i] = T(-2) * (nu0 + T(0)) * S13[i];
+      tyz[i] = T(-2) * (nu0 + T(0)) * S23[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// interpolag_Sdep.f90: Lagrangian back-trajectory + trilinear interpolation
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl,
+                             const T* __restrict__ FF, T* FLM, T* FMM, T* FQN, T* FNN, int nx, int ld, int ny,
+                             int nz, T cs, T dt, T dx, T dy, T dz) {
+  const long pl = (long)ld * ny, n3 = pl * nz;
+  const long n = (long)nx * ny * nz;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(q / ((long)nx * ny));
+    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const long i = (long)k * pl + (long)y * ld + x;
+    // u_lag, v_lag to w-nodes (interpolag_Sdep.f90:228-241)
+    T uu = ul[i] / cs, vv = vl[i] / cs;
+    if (k >= 1) {
+      uu = T(0.5) * (uu + ul[i - pl] / cs);
+      vv = T(0.5) * (vv + vl[i - pl] / cs);
+    }
+    const T xp = -uu * dt * cs / dx;
+    const T yp = -vv * dt * cs / dy;
+    T zp;
+    if (k <= 1) {  // :243-295: level 1 = 0.25 w_lag(2); then wall doubling + clamps
+      const long i2 = (long)pl + (long)y * ld + x;
+      T z2 = -(wl[i2] / cs) * dt * cs / dz;
+      z2 = z2 + (z2 < T(0) ? z2 : T(0));
+      z2 = z2 < T(1) ? z2 : T(1);
+      z2 = z2 > T(-1) ? z2 : T(-1);
+      zp = z2;
+    } else {
+      zp = -(wl[i] / cs) * dt * cs / dz;
+    }
+    const int ax = signbit(xp) ? -1 : 1, ay = signbit(yp) ? -1 : 1, az = signbit(zp) ? -1 : 1;
+    const int xa = (x + ax + nx) % nx, ya = (y + ay + ny) % ny;
+    int za = k + az;
+    za = za < 0 ? 0 : (za > nz - 1 ? nz - 1 : za);
+    const T cxv = fabs(xp), cyv = fabs(yp), czv = fabs(zp);
+    const T fx = T(1) - cxv, fy = T(1) - cyv, fz = T(1) - czv;
+    const long i000 = i, i00z = (long)za * pl + (long)y * ld + x;
+    const long i0y0 = (long)k * pl + (long)ya * ld + x, i0yz = (long)za * pl + (long)ya * ld + x;
+    const long ix00 = (long)k * pl + (long)y * ld + xa, ix0z = (long)za * pl + (long)y * ld + xa;
+    const long ixy0 = (long)k * pl + (long)ya * ld + xa, ixyz = (long)za * pl + (long)ya * ld + xa;
+    T* outs[4] = {FLM, FMM, FQN, FNN};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const T* G = FF + f * n3;
+      outs[f][i] = fx * fy * (G[i000] * fz + G[i00z] * czv) + fx * cyv * (G[i0y0] * fz + G[i0yz] * czv) +
+                   cxv * fy * (G[ix00] * fz + G[ix0z] * czv) + cxv * cyv * (G[ixy0] * fz + G[ixyz] * czv);
+    }
+  }
+}
+
+template <typename T>
+void op_interpolag(les_ctx* c, const SgsPtrs& s) {
+  OpTag _tag(c, "interpolag");
+  const Grid& g = c->g;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  T* FF = (T*)scratch(c, S_FF, 4 * n3 * sizeof(T));
+  op_copy<T>(c, FF, (const T*)s.F_LM, n3);
+  op_copy<T>(c, FF + n3, (const T*)s.F_MM, n3);
+  op_copy<T>(c, FF + 2 * n3, (const T*)s.F_QN, n3);
+  op_copy<T>(c, FF + 3 * n3, (const T*)s.F_NN, n3);
+  {
+    ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
+                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, FF, (T*)s.F_LM, (T*)s.F_MM,
+                       (T*)s.F_QN, (T*)s.F_NN, g.nx, g.ld, g.ny, g.nz, T(g.cs_count), T(g.dt), T(g.dx), T(g.dy),
+                       T(g.dz));
+  }
+  LCHK(c, hipGetLastError());
+  op_fill<T>(c, (T*)s.u_lag, n3, T(0));
+  op_fill<T>(c, (T*)s.v_lag, n3, T(0));
+  op_fill<T>(c, (T*)s.w_lag, n3, T(0));
+}
+
+// ------------------------------------------------------------------------
+// lagrange_Sdep.f90: 21 fields, each filtered at 2 Delta and 4 Delta
+// ------------------------------------------------------------------------
+template <typename T>
+struct LoadLag {  // field order: ub vb wb | uu uv uw vv vw ww | S11 S12 S13 S22 S23 S33 | |S|S_ij
+  const T *u, *v, *w, *S11, *S12, *S13, *S22, *S23, *S33;
+  int ny, nz, ld;
+  static constexpr double reads() { return 9.0 / 21.0; }  // u,v,w,S_ij feed 21 rows
+  __device__ T operator()(int row, int x) const {
+    const int rpf = nz * ny;
+    const int f = row / rpf, r = row - f * rpf, k = r / ny;
+    const long i = (long)r * ld + x, pl = (long)ld * ny;
+    if (f < 9) {
+      T ub, vb, wb;
+      if (k == 0) { ub = u[i]; vb = v[i]; wb = T(0.25) * w[i + pl]; }
+      else { ub = T(0.5) * (u[i] + u[i - pl]); vb = T(0.5) * (v[i] + v[i - pl]); wb = w[i]; }
+      switch (f) {
+        case 0: return ub;
+        case 1: return vb;
+        case 2: return wb;
+        case 3: return ub * ub;
+        case 4: return ub * vb;
+        case 5: return ub * wb;
+        case 6: return vb * vb;
+        case 7: return vb * wb;
+        default: return wb * wb;
+      }
+    }
+    const T* Sp[6] = {S11, S12, S13, S22, S23, S33};
+    if (f < 15) return Sp[f - 9][i];
+    const T s = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);
+    return s * Sp[f - 15][i];
+  }
+};
+
+__device__ __forceinline__ double fq(double x) { return (double)(float)x; }  // Fortran real(x)
+
+template <typename T>
+__global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F4, const T* __restrict__ S11,
+                               const T* __restrict__ S12, const T* __restrict__ S13, const T* __restrict__ S22,
+                               const T* __restrict__ S23, const T* __restrict__ S33, T* FLM, T* FMM, T* FQN, T* FNN,
+                               T* TnLM, T* TnQN, T* Beta, T* Cs, T* L11t, T* L22t, T* L33t, T* Q11t, T* Q22t,
+                               T* Q33t, double* red, int nx, int ld, int ny, int nz, double cst, double opftdelta,
+                               double lagran_dt, double expo, int init_lm, int init_qn) {
+  const long pl = (long)ld * ny, n3 = pl * nz;
+  const long n = (long)nx * ny * nz;
+  const double e24 = (double)1e-24f;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(q / ((long)nx * ny));
+    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const long i = (long)k * pl + (long)y * ld + x;
+    auto f2 = [&](int f) { return (double)F2[f * n3 + i]; };
+    auto f4 = [&](int f) { return (double)F4[f * n3 + i]; };
+    const double ub = f2(0), vb = f2(1), wb = f2(2), uh = f4(0), vh = f4(1), wh = f4(2);
+    double Lv[6], Qv[6];
+    Lv[0] = f2(3) - ub * ub; Lv[1] = f2(4) - ub * vb; Lv[2] = f2(5) - ub * wb;
+    Lv[3] = f2(6) - vb * vb; Lv[4] = f2(7) - vb * wb; Lv[5] = f2(8) - wb * wb;
+    Qv[0] = f4(3) - uh * uh; Qv[1] = f4(4) - uh * vh; Qv[2] = f4(5) - uh * wh;
+    Qv[3] = f4(6) - vh * vh; Qv[4] = f4(7) - vh * wh; Qv[5] = f4(8) - wh * wh;
+    L11t[i] = (T)Lv[0]; L22t[i] = (T)Lv[3]; L33t[i] = (T)Lv[5];
+    Q11t[i] = (T)Qv[0]; Q22t[i] = (T)Qv[3]; Q33t[i] = (T)Qv[5];
+    double sb[6], sh[6], ssb[6], ssh[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { sb[m] = f2(9 + m); sh[m] = f4(9 + m); ssb[m] = f2(15 + m); ssh[m] = f4(15 + m); }
+    const double Sb = sqrt(2.0 * (sb[0] * sb[0] + sb[3] * sb[3] + sb[5] * sb[5] +
+                                  2.0 * (sb[1] * sb[1] + sb[2] * sb[2] + sb[4] * sb[4])));
+    const double Sh = sqrt(2.0 * (sh[0] * sh[0] + sh[3] * sh[3] + sh[5] * sh[5] +
+                                  2.0 * (sh[1] * sh[1] + sh[2] * sh[2] + sh[4] * sh[4])));
+    double M[6], N[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      M[m] = cst * (ssb[m] - 4.0 * Sb * sb[m]);
+      N[m] = cst * (ssh[m] - 16.0 * Sh * sh[m]);
+    }
+    const double LM = Lv[0] * M[0] + Lv[3] * M[3] + Lv[5] * M[5] + 2.0 * (Lv[1] * M[1] + Lv[2] * M[2] + Lv[4] * M[4]);
+    const double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+    const double QN = Qv[0] * N[0] + Qv[3] * N[3] + Qv[5] * N[5] + 2.0 * (Qv[1] * N[1] + Qv[2] * N[2] + Qv[4] * N[4]);
+    const double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+    double flm = FLM[i], fmm = FMM[i], fqn = FQN[i], fnn = FNN[i];
+    if (init_lm) { fmm = MM; flm = 0.03 * MM; }
+    double Tn = fmax(fq(flm * fmm), e24);
+    TnLM[i] = (T)Tn;
+    Tn = opftdelta * pow(Tn, -0.125);
+    Tn = fmax(e24, fq(Tn));
+    double dumfac = lagran_dt / Tn;
+    double epsi = dumfac / (1.0 + dumfac);
+    flm = (epsi * LM + (1.0 - epsi) * flm);
+    fmm = (epsi * MM + (1.0 - epsi) * fmm);
+    flm = fmax(e24, fq(flm));
+    const double cs2d = fmax(e24, fq(flm / fmm));
+    if (init_qn) { fnn = NN; fqn = 0.03 * NN; }
+    Tn = fmax(fq(fqn * fnn), e24);
+    TnQN[i] = (T)Tn;
+    Tn = opftdelta * pow(Tn, -0.125);
+    Tn = fmax(e24, fq(Tn));
+    dumfac = lagran_dt / Tn;
+    epsi = dumfac / (1.0 + dumfac);
+    fqn = (epsi * QN + (1.0 - epsi) * fqn);
+    fnn = (epsi * NN + (1.0 - epsi) * fnn);
+    fqn = fmax(e24, fq(fqn));
+    const double cs4d = fmax(e24, fq(fqn / fnn));
+    double beta = pow(cs4d / cs2d, expo);
+    if (k == nz - 1) beta = 1.0;
+    const double bclip = fmax(fq(beta), fq(0.125));
+    const double cso = fmax(e24, fq(cs2d / bclip));
+    FLM[i] = (T)flm; FMM[i] = (T)fmm; FQN[i] = (T)fqn; FNN[i] = (T)fnn;
+    Beta[i] = (T)beta;
+    Cs[i] = (T)cso;
+    atomicAdd(red + 3 * k + 0, cs2d);
+    atomicAdd(red + 3 * k + 1, cs4d);
+    atomicAdd(red + 3 * k + 2, cso);
+  }
+}
+
+// pads of the lagrange outputs: lagrange_Sdep.f90:295-296,344-345,353-354,383-384,404,419-420
+template <typename T>
+__global__ void k_lag_pads(T* FLM, T* FMM, T* FQN, T* FNN, T* Beta, T* Cs, int nx, int ld, int ny, int nz,
+                           int init_lm, int init_qn) {
+  const long pl = (long)ld * ny;
+  const int np = (ld - nx) * ny * nz;
+  const double e24 = (double)1e-24f;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < np; q += gridDim.x * blockDim.x) {
+    const int k = q / ((ld - nx) * ny);
+    const int r = q - k * (ld - nx) * ny, y = r / (ld - nx), x = nx + (r - y * (ld - nx));
+    const long i = (long)k * pl + (long)y * ld + x;
+    if (init_lm) { FLM[i] = T(1); FMM[i] = T(1); }
+    if (init_qn) { FQN[i] = T(1); FNN[i] = T(1); }
+    Beta[i] = T(1);
+    Cs[i] = (T)e24;
+  }
+}
+
+__global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_avg, int nz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nz) {
+    Beta_avg[k] = red[3 * k] / red[3 * k + 2];
+    Betaclip_avg[k] = red[3 * k + 1] / red[3 * k];
+  }
+}
+
+template <typename T>
+void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s) {
+  OpTag _tag(c, "lagrange");
+  const Grid& g = c->g;
+  op_interpolag<T>(c, s);
+  const int nz = g.nz, k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4;
+  const long rows = 21L * nz * g.ny;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
+  LoadLag<T> L{u, v, w, S[0], S[1], S[2], S[3], S[4], S[5], g.ny, nz, g.ld};
+  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf);
+  cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
+  cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);
+  a.lhG = g.lh;
+  col_out(a, 0, K_G, k2, k2, (long)g.ny * k2);
+  col_out(a, 1, K_G, k4, k4, (long)g.ny * k4);
+  const Tables<T>& t = tables<T>(c);
+  launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, 21L * nz, kf, t.G2, t.G4);
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  T* F = (T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
+  // row_inv of 21 fields: StoreFields handles up to 4 fields, so use one
+  // "field" spanning all 21 (rows are contiguous field-major)
+  StoreFields<T> S2{{F, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
+  launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld);
+  StoreFields<T> S4{{F + 21 * n3, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
+  launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld);
+  double* red = (double*)scratch(c, S_RED, 3 * nz * sizeof(double));
+  op_fill<double>(c, red, 3 * (size_t)nz, 0.0);
+  const int init_lm = g.inilag && !c->flm_init && (c->jt == g.cs_count || c->jt == 1);
+  const int init_qn = g.inilag && !c->fqn_init && (c->jt == g.cs_count || c->jt == 1);
+  const double delta = g.delta;
+  {
+    ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_lag_epilogue<T>), grid1((size_t)g.nx * g.ny * nz), dim3(256), 0, c->stream, F, F + 21 * n3,
+                       S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
+                       (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
+                       (T*)s.Q11t, (T*)s.Q22t, (T*)s.Q33t, red, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
+                       1.5 * delta, g.dt * (double)g.cs_count, log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
+  }
+  LCHK(c, hipGetLastError());
+  hipLaunchKernelGGL((k_lag_pads<T>), grid1((size_t)(g.ld - g.nx) * g.ny * nz), dim3(256), 0, c->stream,
+                     (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN, (T*)s.Beta, (T*)s.Cs_opt2, g.nx, g.ld, g.ny,
+                     nz, init_lm, init_qn);
+  LCHK(c, hipGetLastError());
+  if (s.Beta_avg && s.Betaclip_avg) {
+    hipLaunchKernelGGL(k_lag_avg, dim3((nz + 63) / 64), dim3(64), 0, c->stream, red, (double*)s.Beta_avg,
+                       (double*)s.Betaclip_avg, nz);
+    LCHK(c, hipGetLastError());
+  }
+  if (init_lm) c->flm_init = 1;
+  if (init_qn) c->fqn_init = 1;
+}
+
+template <typename T>
+void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const T* w, T* const tau[6],
+                 const SgsPtrs& s) {
+  OpTag _tag(c, "sgs_stag");
+  const Grid& g = c->g;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
+  T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
+  {
+    ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + 3) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_calc_sij<T>), grid1(n3), dim3(256), 0, c->stream, gr[0], gr[1], gr[2], gr[3], gr[4], gr[5],
+                       gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx, g.ld, g.ny, g.nz, u, v, w,
+                       (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag);
+  }
+  LCHK(c, hipGetLastError());
+  const int jt = c->jt;
+  if (jt == 1 && g.inilag) {
+    op_fill<T>(c, (T*)s.Cs_opt2, n3, T(0.03));
+  } else if ((jt >= 1 || g.initu) && (jt % g.cs_count == 0)) {
+    const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+    op_lagrange<T>(c, Sc, u, v, w, s);
+  }
+  const int use_const = c->jt_total < 500;
+  const T cs0 = (T)(double)(0.16f * 0.16f);
+  const T l2 = (T)(g.delta * g.delta);
+  {
+    ProfScope _ps2(c, "sgs_tau", (6.0 + 1 + 1 + 6) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_sgs_tau<T>), grid1(n3), dim3(256), 0, c->stream, S[0], S[1], S[2], S[3], S[4], S[5],
+                       (const T*)s.Cs_opt2, cs0, use_const, (T*)s.Cs_opt2, (T*)s.Nu_t, tau[0], tau[1], tau[2], tau[3],
+                       tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// press_stag_array.f90 + tridag_array.f90
+// ------------------------------------------------------------------------
+template <typename T>
+struct LoadPress {  // fields Hx, Hy, Hz on levels 0..nz-2, scaled (const/tadv1)*(u/dt)
+  const T *u, *v, *w;
+  int ny, nl, ld;
+  T cst, dt;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int row, int x) const {
+    const int rpf = nl * ny;
+    const int f = row / rpf, r = row - f * rpf;
+    const long i = (long)r * ld + x;
+    const T* p = f == 0 ? u : (f == 1 ? v : w);
+    return cst * (p[i] / dt);
+  }
+};
+template <typename T>
+struct LoadPlanes2 {  // two planes of one field: level 0 and level kt
+  const T* f;
+  int ny, ld, kt;
+  T cst;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int row, int x) const {
+    const int j = row / ny, y = row - j * ny;
+    const int k = j == 0 ? 0 : kt;
+    return cst * f[((long)k * ny + y) * ld + x];
+  }
+};
+
+// One thread per horizontal mode (kx < nx/2, all ky); Thomas algorithm in
+// FP64 along z with the boundary rows of press_stag_array.f90:142-165 and the
+// zero-mode integration of :238-247.  H: 3 fields x (nz-1) planes of
+// (ny, nxh) complex; BT: bottom/top planes; P: (nz) planes of p_hat.
+template <typename T>
+__global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__ BT, cx<T>* __restrict__ P,
+                          double* __restrict__ gam, cx<double>* __restrict__ U, const T* __restrict__ kxt,
+                          const T* __restrict__ kyt, const double* __restrict__ kxd, const double* __restrict__ kyd,
+                          int nxh, int ny, int nz, double dz, int* err) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nm = nxh * ny;
+  if (m >= nm) return;
+  const int y = m / nxh, x = m - y * nxh;
+  const long ps = (long)nm;               // plane stride (complex)
+  const long fs = ps * (nz - 1);          // field stride in H
+  const cx<T>* Hx = H;
+  const cx<T>* Hy = H + fs;
+  const cx<T>* Hz = H + 2 * fs;
+  const cx<double> bot = {(double)BT[m].x, (double)BT[m].y};
+  const cx<double> top = {(double)BT[ps + m].x, (double)BT[ps + m].y};
+  if (y == ny / 2) {
+    for (int j = 0; j < nz; ++j) P[j * ps + m] = {T(0), T(0)};
+    return;
+  }
+  if (m == 0) {
+    cx<double> pv = {0.0, 0.0};
+    P[0] = {T(0), T(0)};
+    pv = {pv.x - dz * bot.x, pv.y - dz * bot.y};
+    P[ps] = {(T)pv.x, (T)pv.y};
+    for (int j = 2; j < nz; ++j) {
+      const cx<T> h = Hz[(long)(j - 1) * ps];
+      pv = {pv.x + (double)h.x * dz, pv.y + (double)h.y * dz};
+      P[j * ps + m] = {(T)pv.x, (T)pv.y};
+    }
+    return;
+  }
+  const double kx = kxd[x], ky = kyd[y];
+  const double a_in = 1.0 / (dz * dz), c_in = 1.0 / (dz * dz);
+  const double b_in = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+  // row 0: b=-1, c=1, r=-dz*bottom
+  double bet = -1.0;
+  cx<double> u0 = {(-dz * bot.x) / bet, (-dz * bot.y) / bet};
+  U[m] = u0;
+  double cprev = 1.0;
+  cx<double> uprev = u0;
+  cx<double> hxm = {(double)Hx[m].x, (double)Hx[m].y};
+  cx<double> hym = {(double)Hy[m].x, (double)Hy[m].y};
+  cx<double> hzm = {(double)Hz[m].x, (double)Hz[m].y};
+  for (int j = 1; j <= nz; ++j) {
+    double aj, bj;
+    cx<double> rj;
+    if (j < nz) {
+      aj = a_in;
+      bj = b_in;
+      cx<double> hz = {0.0, 0.0};
+      if (j < nz - 1) hz = {(double)Hz[(long)j * ps + m].x, (double)Hz[(long)j * ps + m].y};
+      const double sr = kx * hxm.x + ky * hym.x, si = kx * hxm.y + ky * hym.y;
+      rj = {-si + (hz.x - hzm.x) / dz, sr + (hz.y - hzm.y) / dz};
+      if (j < nz - 1) {
+        hxm = {(double)Hx[(long)j * ps + m].x, (double)Hx[(long)j * ps + m].y};
+        hym = {(double)Hy[(long)j * ps + m].x, (double)Hy[(long)j * ps + m].y};
+        hzm = hz;
+      }
+    } else {
+      aj = -1.0;
+      bj = 1.0;
+      rj = {-top.x * dz, -top.y * dz};
+    }
+    const double gj = cprev / bet;
+    bet = bj - aj * gj;
+    if (bet == 0.0) atomicExch(err, 2);
+    gam[(long)j * nm + m] = gj;
+    uprev = {(rj.x - aj * uprev.x) / bet, (rj.y - aj * uprev.y) / bet};
+    U[(long)j * nm + m] = uprev;
+    cprev = j < nz ? c_in : 0.0;
+  }
+  // back substitution; p_hat levels 0..nz-1 are kept (level nz is BOGUS later)
+  cx<double> un = uprev;
+  for (int j = nz - 1; j >= 0; --j) {
+    cx<double> uj = U[(long)j * nm + m];
+    const double gn = gam[(long)(j + 1) * nm + m];
+    uj = {uj.x - gn * un.x, uj.y - gn * un.y};
+    P[(long)j * ps + m] = {(T)uj.x, (T)uj.y};
+    un = uj;
+  }
+}
+
+template <typename T>
+struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: dpdy
+  T *p, *dpdx, *dpdy;
+  int rows_per_field, ld, nx, ny;
+  static constexpr double writes() { return 1.0; }
+  __device__ void operator()(int row, int x, T v) const {
+    const int f = row / rows_per_field, r = row - f * rows_per_field;
+    const T val = x < nx ? v : T(0);
+    if (f == 0) p[(long)(r + ny) * ld + x] = val;  // p(:,:,1:nz-1) of p(ld,ny,0:nz)
+    else if (f == 1) dpdx[(long)r * ld + x] = val;
+    else dpdy[(long)r * ld + x] = val;
+  }
+};
+
+template <typename T>
+void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy) {
+  OpTag _tag(c, "press_stag");
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1, nxh = g.nxh;
+  const long ps = (long)g.ny * nxh;
+  const long rows = 3L * nl * g.ny;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)(rows + 2 * g.ny) * nxh * sizeof(cx<T>));
+  const double cstd = 1.0 / ((double)g.nx * g.ny);
+  LoadPress<T> L{u, v, w, g.ny, nl, g.ld, (T)(cstd / TADV1), (T)g.dt};
+  launch_row_fwd<T>(c, L, X0, nxh, rows, g.nx, nxh);
+  LoadPlanes2<T> LB{divtz, g.ny, g.ld, nz - 1, (T)cstd};
+  cx<T>* XB = X0 + (size_t)rows * nxh;
+  launch_row_fwd<T>(c, LB, XB, nxh, 2L * g.ny, g.nx, nxh);
+  // forward y transforms only -> spectral H (3*nl planes) and bottom/top (2)
+  cx<T>* H = (cx<T>*)scratch(c, S_PSPEC, (size_t)(3 * nl + 2) * ps * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny, nxh, nxh, ps);
+  a.do_inv = 0;
+  col_out(a, 0, K_ID, nxh, nxh, ps);
+  launch_col<T>(c, X0, nullptr, H, nullptr, nullptr, a, 3L * nl + 2, nxh);
+  // tridiagonal solves (FP64)
+  cx<T>* PH = (cx<T>*)scratch(c, S_PHAT, (size_t)nz * ps * sizeof(cx<T>));
+  char* tri = (char*)scratch(c, S_TRI, (size_t)(nz + 1) * ps * (sizeof(double) + sizeof(cx<double>)) + 256);
+  double* gam = (double*)tri;
+  cx<double>* Ub = (cx<double>*)(tri + (size_t)(nz + 1) * ps * sizeof(double));
+  int* err = (int*)scratch(c, S_TMP2, 256);
+  const Tables<T>& t = tables<T>(c);
+  {
+    ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) + (nz + 1) * ps * 2.0 * (sizeof(double) + sizeof(cx<double>)));
+    hipLaunchKernelGGL((k_tridiag<T>), dim3((unsigned)((ps + 127) / 128)), dim3(128), 0, c->stream, H,
+                       H + (size_t)3 * nl * ps, PH, gam, Ub, t.kx, t.ky, c->td.kx, c->td.ky, nxh, g.ny, nz, g.dz, err);
+  }
+  LCHK(c, hipGetLastError());
+  // inverse y transforms: levels 1..nz-1 -> p, i kx p, i ky p ; level 0 -> p
+  const long rowsP = 3L * nl * g.ny;
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)(rowsP + g.ny) * nxh * sizeof(cx<T>));
+  ColArgs b = col_args(g.ny, g.ny, nxh, nxh, ps);
+  b.do_fwd = 0;
+  const size_t fo = (size_t)nl * ps;
+  col_out(b, 0, K_ID, nxh, nxh, ps);
+  col_out(b, 1, K_IKX, nxh, nxh, ps);
+  col_out(b, 2, K_IKY, nxh, nxh, ps);
+  launch_col<T>(c, PH + ps, nullptr, X1, X1 + fo, X1 + 2 * fo, b, nl, nxh);
+  ColArgs b0 = col_args(g.ny, g.ny, nxh, nxh, ps);
+  b0.do_fwd = 0;
+  col_out(b0, 0, K_ID, nxh, nxh, ps);
+  cx<T>* X1b = X1 + 3 * fo;
+  launch_col<T>(c, PH, nullptr, X1b, nullptr, nullptr, b0, 1, nxh);
+  StorePress<T> SP{p, dpdx, dpdy, (int)(nl * g.ny), g.ld, g.nx, g.ny};
+  launch_row_inv<T>(c, X1, nxh, nxh, SP, rowsP, g.nx, g.ld);
+  StoreFields<T> S0{{p, nullptr, nullptr, nullptr}, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X1b, nxh, nxh, S0, g.ny, g.nx, g.ld);
+  const size_t pl = (size_t)g.ld * g.ny;
+  hipLaunchKernelGGL((k_fill_bogus_cplx<T>), grid1(pl), dim3(256), 0, c->stream, p + pl * nz, pl);
+  hipLaunchKernelGGL((k_fill_bogus_cplx<T>), grid1(pl), dim3(256), 0, c->stream, dpdx + pl * (nz - 1), pl);
+  hipLaunchKernelGGL((k_fill_bogus_cplx<T>), grid1(pl), dim3(256), 0, c->stream, dpdy + pl * (nz - 1), pl);
+  LCHK(c, hipGetLastError());
+}
+
+// tridag_array.f90 on caller arrays (lh,ny,nz+1): one thread per mode
+__global__ void k_tridag_array(const double* a, const double* b, const double* cc, const cx<double>* r,
+                               cx<double>* u, double* gam, int lh, int ny, int n, int* err) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nm = lh * ny;
+  if (m >= nm) return;
+  const int y = m / lh, x = m - y * lh;
+  if (x < lh - 1 && b[m] == 0.0) atomicExch(err, 1);
+  double bet = b[m];
+  u[m] = {r[m].x / bet, r[m].y / bet};
+  if (y == ny / 2 || x == lh - 1 || m == 0) return;
+  for (int j = 1; j < n; ++j) {
+    const long i = (long)j * nm + m;
+    const double gj = cc[i - nm] / bet;
+    gam[i] = gj;
+    bet = b[i] - a[i] * gj;
+    if (bet == 0.0) atomicExch(err, 2);
+    u[i] = {(r[i].x - a[i] * u[i - nm].x) / bet, (r[i].y - a[i] * u[i - nm].y) / bet};
+  }
+  for (int j = n - 2; j >= 0; --j) {
+    const long i = (long)j * nm + m;
+    u[i] = {u[i].x - gam[i + nm] * u[i + nm].x, u[i].y - gam[i + nm] * u[i + nm].y};
+  }
+}
+
+int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u) {
+  const Grid& g = c->g;
+  const int n = g.nz + 1, nm = g.lh * g.ny;
+  double* gam = (double*)scratch(c, S_TMP0, (size_t)n * nm * sizeof(double));
+  int* err = (int*)scratch(c, S_TMP2, 256);
+  LCHK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_tridag_array, dim3((nm + 127) / 128), dim3(128), 0, c->stream, a, b, cc,
+                     (const cx<double>*)r, (cx<double>*)u, gam, g.lh, g.ny, n, err);
+  LCHK(c, hipGetLastError());
+  int h = 0;
+  LCHK(c, hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  LCHK(c, hipStreamSynchronize(c->stream));
+  return h;
+}
+
+// ------------------------------------------------------------------------
+// forcing.f90:128-219 project
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_project(T* u, T* v, T* w, const T* dpdx, const T* dpdy, const T* dpdz, int nx, int ld, int ny,
+                          int nz, T dt) {
+  const long pl = (long)ld * ny;
+  const long n = pl * (nz - 1);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    T uu = u[i], vv = v[i], ww = w[i];
+    if (x < nx) {
+      uu = (uu + dt * (-T(TADV1) * dpdx[i] + T(0)));
+      vv = (vv + dt * (-T(TADV1) * dpdy[i] + T(0)));
+      if (k >= 1) ww = (ww + dt * (-T(TADV1) * dpdz[i] + T(0)));
+      u[i] = uu;
+      v[i] = vv;
+    }
+    if (k == 0) ww = T(0);
+    if (k >= 1 || x >= nx) w[i] = ww;
+    else w[i] = T(0);
+    if (k == nz - 2) {  // lid: u(nz) = u(nz-1), w(nz) = 0
+      u[i + pl] = uu;
+      v[i + pl] = vv;
+      w[i + pl] = T(0);
+    }
+  }
+}
+template <typename T>
+void op_project(les_ctx* c, T* u, T* v, T* w, const T* dpdx, const T* dpdy, const T* dpdz) {
+  const Grid& g = c->g;
+  hipLaunchKernelGGL((k_project<T>), grid1((size_t)g.ld * g.ny * (g.nz - 1)), dim3(256), 0, c->stream, u, v, w,
+                     dpdx, dpdy, dpdz, g.nx, g.ld, g.ny, g.nz, (T)g.dt);
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// main.f90:309-390: RHS = -conv - divt ; AB2 predictor ; BOGUS lid
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ab2(T* u, T* v, T* w, T* Rx, T* Ry, T* Rz, const T* Rxf, const T* Ryf, const T* Rzf,
+                      const T* dx, const T* dy, const T* dz, long nv, long pl, T dt, T force, int euler) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nv + pl; i += (long)gridDim.x * blockDim.x) {
+    if (i >= nv) {
+      u[i] = T(BOGUS); v[i] = T(BOGUS); w[i] = T(BOGUS);
+      continue;
+    }
+    const T rx = -Rx[i] - dx[i], ry = -Ry[i] - dy[i], rz = -Rz[i] - dz[i];
+    Rx[i] = rx; Ry[i] = ry; Rz[i] = rz;
+    const T fx = euler ? rx : Rxf[i], fy = euler ? ry : Ryf[i], fz = euler ? rz : Rzf[i];
+    u[i] = u[i] + dt * (T(TADV1) * rx + T(TADV2) * fx + force);
+    v[i] = v[i] + dt * (T(TADV1) * ry + T(TADV2) * fy);
+    w[i] = w[i] + dt * (T(TADV1) * rz + T(TADV2) * fz);
+  }
+}
+template <typename T>
+void op_ab2(les_ctx* c, T* u, T* v, T* w, T* RHSx, T* RHSy, T* RHSz, const T* RHSx_f, const T* RHSy_f,
+            const T* RHSz_f, const T* divtx, const T* divty, const T* divtz, int euler_first) {
+  const Grid& g = c->g;
+  const long pl = (long)g.ld * g.ny, nv = pl * (g.nz - 1);
+  {
+    ProfScope _ps(c, "ab2", 15.0 * (nv + pl) * sizeof(T));
+    hipLaunchKernelGGL((k_ab2<T>), grid1((size_t)(nv + pl)), dim3(256), 0, c->stream, u, v, w, RHSx, RHSy, RHSz,
+                       RHSx_f, RHSy_f, RHSz_f, divtx, divty, divtz, nv, pl, (T)g.dt, (T)g.force, euler_first);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// main.f90:403-415: dpdz, RHS -= grad p, then project (forcing.f90:128-219)
+template <typename T>
+__global__ void k_pressure_update(T* u, T* v, T* w, T* Rx, T* Ry, T* Rz, const T* p, const T* dpdx,
+                                  const T* dpdy, T* dpdz, int nx, int ld, int ny, int nz, T rdz, T dt) {
+  const long pl = (long)ld * ny;
+  const long nv = pl * (nz - 1);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nv + pl; i += (long)gridDim.x * blockDim.x) {
+    if (i >= nv) {
+      dpdz[i] = T(BOGUS);
+      continue;
+    }
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    T dz = dpdz[i];
+    if (x < nx) {
+      dz = (p[i + pl] - p[i]) * rdz;  // p is (ld,ny,0:nz): p(k) at i+pl
+      dpdz[i] = dz;
+    }
+    const T px = dpdx[i], py = dpdy[i];
+    Rx[i] = Rx[i] - px;
+    Ry[i] = Ry[i] - py;
+    Rz[i] = Rz[i] - dz;
+    T uu = u[i], vv = v[i], ww = w[i];
+    if (x < nx) {
+      uu = (uu + dt * (-T(TADV1) * px + T(0)));
+      vv = (vv + dt * (-T(TADV1) * py + T(0)));
+      if (k >= 1) ww = (ww + dt * (-T(TADV1) * dz + T(0)));
+      u[i] = uu;
+      v[i] = vv;
+    }
+    if (k == 0) ww = T(0);
+    w[i] = ww;
+    if (k == nz - 2) {
+      u[i + pl] = uu;
+      v[i + pl] = vv;
+      w[i + pl] = T(0);
+    }
+  }
+}
+template <typename T>
+void op_pressure_update(les_ctx* c, T* u, T* v, T* w, T* RHSx, T* RHSy, T* RHSz, const T* p, const T* dpdx,
+                        const T* dpdy, T* dpdz) {
+  const Grid& g = c->g;
+  const long pl = (long)g.ld * g.ny, nv = pl * (g.nz - 1);
+  {
+    ProfScope _ps(c, "pressure_update", 16.0 * (nv + pl) * sizeof(T));
+    hipLaunchKernelGGL((k_pressure_update<T>), grid1((size_t)(nv + pl)), dim3(256), 0, c->stream, u, v, w, RHSx,
+                       RHSy, RHSz, p, dpdx, dpdy, dpdz, g.nx, g.ld, g.ny, g.nz, (T)(1.0 / g.dz), (T)g.dt);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// rmsdiv.f90 and scalars_module2.f90:22-70 (FP64 reductions)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_rmsdiv(const T* du, const T* dv, const T* dw, int nx, int ld, int ny, int nl, double* part) {
+  __shared__ double sh[256];
+  const long n = (long)nx * ny * nl;
+  double acc = 0.0;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(q / ((long)nx * ny));
+    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const long i = ((long)k * ny + y) * ld + x;
+    acc += fabs((double)du[i] + (double)dv[i] + (double)dw[i]);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+template <typename T>
+double op_rmsdiv(les_ctx* c, const T* dudx, const T* dvdy, const T* dwdz) {
+  const Grid& g = c->g;
+  const int nb = 512;
+  double* part = (double*)scratch(c, S_RED, nb * sizeof(double));
+  hipLaunchKernelGGL((k_rmsdiv<T>), dim3(nb), dim3(256), 0, c->stream, dudx, dvdy, dwdz, g.nx, g.ld, g.ny, g.nz - 1,
+                     part);
+  LCHK(c, hipGetLastError());
+  std::vector<double> h(nb);
+  LCHK(c, hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  LCHK(c, hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (double x : h) s += x;
+  return s / ((double)g.nx * g.ny * (g.nz - 1));
+}
+
+template <typename T>
+__global__ void k_tsc(const T* u, const T* v, const T* w, const T* nu, int nx, int ld, int ny, int nl,
+                      double* part) {
+  __shared__ double sh[4][256];
+  const long n = (long)nx * ny * nl;
+  double m0 = -1.0, m1 = 0.0, m2 = 0.0, m3 = -1e300;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(q / ((long)nx * ny));
+    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const long i = ((long)k * ny + y) * ld + x;
+    const double a = u[i], b = v[i], d = w[i];
+    m0 = fmax(m0, a * a + b * b + d * d);
+    m1 = fmax(m1, fabs(b));
+    m2 = fmax(m2, fabs(d));
+    m3 = fmax(m3, (double)nu[i]);
+  }
+  sh[0][threadIdx.x] = m0; sh[1][threadIdx.x] = m1; sh[2][threadIdx.x] = m2; sh[3][threadIdx.x] = m3;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int j = 0; j < 4; ++j) sh[j][threadIdx.x] = fmax(sh[j][threadIdx.x], sh[j][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 4; ++j) part[4 * blockIdx.x + j] = sh[j][0];
+}
+template <typename T>
+void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, const T* nu, double out[4]) {
+  const Grid& g = c->g;
+  const int nb = 256;
+  double* part = (double*)scratch(c, S_RED, 4 * nb * sizeof(double));
+  hipLaunchKernelGGL((k_tsc<T>), dim3(nb), dim3(256), 0, c->stream, u, v, w, nu, g.nx, g.ld, g.ny, g.nz - 1, part);
+  LCHK(c, hipGetLastError());
+  std::vector<double> h(4 * nb);
+  LCHK(c, hipMemcpyAsync(h.data(), part, 4 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  LCHK(c, hipStreamSynchronize(c->stream));
+  double m[4] = {-1.0, 0.0, 0.0, -1e300};
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < 4; ++j) m[j] = fmax(m[j], h[4 * b + j]);
+  const double delta = fmin(g.dx, g.dy);
+  out[0] = sqrt(m[0]) * g.dt / g.dx;
+  out[1] = m[1] * g.dt / g.dy;
+  out[2] = m[2] * g.dt / g.dz;
+  out[3] = g.dt * m[3] / (delta * delta);
+}
+
+// ------------------------------------------------------------------------
+// explicit instantiations
+// ------------------------------------------------------------------------
+#define LES_INST(T)                                                                                              \
+  template void op_filt_da<T>(les_ctx*, T*, T*, T*, int);                                                        \
+  template void op_ddxy<T>(les_ctx*, T*, T*, const T*, int);                                                     \
+  template void op_test_filter<T>(les_ctx*, T*, int, const T*, int);                                             \
+  template void op_dealias1<T>(les_ctx*, const T*, T*, int);                                                     \
+  template void op_dealias2<T>(les_ctx*, T*, T*, int);                                                           \
+  template void op_padd<T>(les_ctx*, T*, const T*, int);                                                         \
+  template void op_unpadd<T>(les_ctx*, T*, const T*, int);                                                       \
+  template void op_ddz_uv<T>(les_ctx*, T*, const T*);                                                            \
+  template void op_ddz_w<T>(les_ctx*, T*, const T*);                                                             \
+  template void op_divstress<T>(les_ctx*, T*, const T*, const T*, const T*, int);                                \
+  template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
+                             const T*, const T*, T*, T*, T*);                                                    \
+  template void op_wallstress<T>(les_ctx*, const T*, const T*, T*, T*, T*, T*, const double*, const double*,     \
+                                 const double*, double*);                                                        \
+  template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
+  template void op_interpolag<T>(les_ctx*, const SgsPtrs&);                                                      \
+  template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \
+  template void op_sgs_stag<T>(les_ctx*, const T* const[9], const T*, const T*, const T*, T* const[6],           \
+                               const SgsPtrs&);                                                                  \
+  template void op_press_stag<T>(les_ctx*, const T*, const T*, const T*, const T*, T*, T*, T*);                  \
+  template void op_project<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*);                               \
+  template double op_rmsdiv<T>(les_ctx*, const T*, const T*, const T*);                                          \
+  template void op_timestep_conditions<T>(les_ctx*, const T*, const T*, const T*, const T*, double*);            \
+  template void op_ab2<T>(les_ctx*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,    \
+                          const T*, int);                                                                        \
+  template void op_pressure_update<T>(les_ctx*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, T*);       \
+  template void op_fill<T>(les_ctx*, T*, size_t, T);                                                             \
+  template void op_copy<T>(les_ctx*, T*, const T*, size_t);
+
+LES_INST(double)
+LES_INST(float)
+
+}  // namespace les

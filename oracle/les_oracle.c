@@ -1,0 +1,1270 @@
+/*
+ * les_oracle.c -- CPU FP64 restatement of the LES per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE: this is the checker for the HIP product path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Each routine cites the reference routine it restates (file:line in the
+ * KhaledGhannam/LES sources).  Evaluation order of floating-point
+ * expressions follows the Fortran text; the FP32 round trips of
+ * lagrange_Sdep.f90 are reproduced as (double)(float)x.
+ *
+ * The reference calls FFTW 2.1.5 (not vendored); the transforms here are a
+ * project-written mixed-radix Stockham FFT with FFTW-2 rfftwnd in-place
+ * semantics (unnormalised, forward sign -1, halfcomplex-style c2r that
+ * ignores Im of the kx=0 and kx=nx/2 entries), with pad columns defined = 0
+ * after every inverse transform.
+ */
+#include "les_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BOGUS (-1234567890.0)
+static const double PI = 3.1415926535897932384626433;
+static const double VONK = 0.41;   /* param.f90:98 */
+static const double TADV1 = 1.5;   /* param.f90:151 */
+static const double TADV2 = -0.5;  /* 1 - tadv1 */
+
+typedef struct { double re, im; } cpx;
+
+/* ------------------------------------------------------------------ */
+/* 1-D complex FFT: Stockham autosort, radices 4,2,3,5                 */
+/* ------------------------------------------------------------------ */
+typedef struct { int n, ns, rad[48]; cpx *tw; } fplan;
+
+static void fplan_init(fplan *p, int n) {
+  p->n = n; p->ns = 0;
+  int m = n;
+  while (m % 4 == 0) { p->rad[p->ns++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->rad[p->ns++] = 2; m /= 2; }
+  while (m % 3 == 0) { p->rad[p->ns++] = 3; m /= 3; }
+  while (m % 5 == 0) { p->rad[p->ns++] = 5; m /= 5; }
+  if (m != 1) { fprintf(stderr, "les_oracle: FFT length %d not 2^a3^b5^c\n", n); abort(); }
+  p->tw = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    p->tw[k].re = (double)cosl(a);
+    p->tw[k].im = (double)sinl(a);
+  }
+}
+static void fplan_free(fplan *p) { free(p->tw); p->tw = NULL; }
+
+/* x: data (length n), t: scratch (length n).  sign = -1 forward, +1 backward. */
+static void fft_exec(const fplan *p, cpx *x, cpx *t, int sign) {
+  const int n = p->n;
+  cpx *src = x, *dst = t;
+  int Ns = 1;
+  const double s3 = 0.86602540378443864676372317075294; /* sqrt(3)/2 */
+  const double c51 = 0.30901699437494742410229341718282, s51 = 0.95105651629515357211643933337938;
+  const double c52 = -0.80901699437494742410229341718282, s52 = 0.58778525229247312916870595463907;
+  for (int s = 0; s < p->ns; s++) {
+    const int R = p->rad[s], nR = n / R, tws = n / (Ns * R);
+    for (int j = 0; j < nR; j++) {
+      const int k = j % Ns;
+      cpx v[5] = { { 0, 0 } };
+      for (int r = 0; r < R; r++) {
+        cpx a = src[j + r * nR];
+        if (r && k) {
+          int idx = (r * k * tws) % n;
+          double wr = p->tw[idx].re, wi = sign < 0 ? p->tw[idx].im : -p->tw[idx].im;
+          cpx b = { a.re * wr - a.im * wi, a.re * wi + a.im * wr };
+          a = b;
+        }
+        v[r] = a;
+      }
+      cpx y[5] = { { 0, 0 } };
+      if (R == 2) {
+        y[0].re = v[0].re + v[1].re; y[0].im = v[0].im + v[1].im;
+        y[1].re = v[0].re - v[1].re; y[1].im = v[0].im - v[1].im;
+      } else if (R == 4) {
+        cpx a0 = { v[0].re + v[2].re, v[0].im + v[2].im };
+        cpx a1 = { v[0].re - v[2].re, v[0].im - v[2].im };
+        cpx a2 = { v[1].re + v[3].re, v[1].im + v[3].im };
+        cpx d  = { v[1].re - v[3].re, v[1].im - v[3].im };
+        /* a3 = d * W, W = exp(sign*i*pi/2) = sign*i */
+        cpx a3 = sign < 0 ? (cpx){ d.im, -d.re } : (cpx){ -d.im, d.re };
+        y[0].re = a0.re + a2.re; y[0].im = a0.im + a2.im;
+        y[1].re = a1.re + a3.re; y[1].im = a1.im + a3.im;
+        y[2].re = a0.re - a2.re; y[2].im = a0.im - a2.im;
+        y[3].re = a1.re - a3.re; y[3].im = a1.im - a3.im;
+      } else if (R == 3) {
+        cpx sm = { v[1].re + v[2].re, v[1].im + v[2].im };
+        cpx df = { v[1].re - v[2].re, v[1].im - v[2].im };
+        cpx t1 = { v[0].re - 0.5 * sm.re, v[0].im - 0.5 * sm.im };
+        double f = sign < 0 ? -s3 : s3;          /* t2 = i*f*df */
+        cpx t2 = { -f * df.im, f * df.re };
+        y[0].re = v[0].re + sm.re; y[0].im = v[0].im + sm.im;
+        y[1].re = t1.re + t2.re; y[1].im = t1.im + t2.im;
+        y[2].re = t1.re - t2.re; y[2].im = t1.im - t2.im;
+      } else { /* R == 5 */
+        double sg = sign < 0 ? -1.0 : 1.0;
+        cpx a1 = { v[1].re + v[4].re, v[1].im + v[4].im }, b1 = { v[1].re - v[4].re, v[1].im - v[4].im };
+        cpx a2 = { v[2].re + v[3].re, v[2].im + v[3].im }, b2 = { v[2].re - v[3].re, v[2].im - v[3].im };
+        y[0].re = v[0].re + a1.re + a2.re; y[0].im = v[0].im + a1.im + a2.im;
+        cpx c1 = { v[0].re + c51 * a1.re + c52 * a2.re, v[0].im + c51 * a1.im + c52 * a2.im };
+        cpx c2 = { v[0].re + c52 * a1.re + c51 * a2.re, v[0].im + c52 * a1.im + c51 * a2.im };
+        cpx d1 = { sg * (s51 * b1.re + s52 * b2.re), sg * (s51 * b1.im + s52 * b2.im) };
+        cpx d2 = { sg * (s52 * b1.re - s51 * b2.re), sg * (s52 * b1.im - s51 * b2.im) };
+        /* y1 = c1 + i d1, y4 = c1 - i d1, y2 = c2 + i d2, y3 = c2 - i d2 */
+        y[1].re = c1.re - d1.im; y[1].im = c1.im + d1.re;
+        y[4].re = c1.re + d1.im; y[4].im = c1.im - d1.re;
+        y[2].re = c2.re - d2.im; y[2].im = c2.im + d2.re;
+        y[3].re = c2.re + d2.im; y[3].im = c2.im - d2.re;
+      }
+      const int base = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; r++) dst[base + r * Ns] = y[r];
+    }
+    cpx *tmp = src; src = dst; dst = tmp;
+    Ns *= R;
+  }
+  if (src != x) memcpy(x, src, sizeof(cpx) * (size_t)n);
+}
+
+void or_cfft1(int n, double *re_im, int sign) {
+  fplan p; fplan_init(&p, n);
+  cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)n);
+  fft_exec(&p, (cpx *)re_im, t, sign);
+  free(t); fplan_free(&p);
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct { int nx, ny, lh, ld; fplan px, py; } plan2;
+
+struct or_ctx {
+  or_params P;
+  int nx, ny, nz, lh, ld, nx2, ny2, lh_big, ld_big;
+  double dx, dy, dz, delta;
+  double *kx, *ky, *k2, *Gt, *Gtt;
+  plan2 sm, bg;
+  int flm_init, fqn_init;   /* SAVE flags, lagrange_Sdep.f90:67-68 */
+  int jt, jt_total;         /* mirrors param jt/jt_total for routine calls */
+};
+
+static void plan2_init(plan2 *p, int nx, int ny) {
+  p->nx = nx; p->ny = ny; p->lh = nx / 2 + 1; p->ld = 2 * p->lh;
+  fplan_init(&p->px, nx); fplan_init(&p->py, ny);
+}
+
+/* fft.f90:54-81 init_wavenumber, incl. the ky(lh,:)=0 quirk */
+static void init_wavenumber(or_ctx *c) {
+  const int lh = c->lh, ny = c->ny;
+  for (int jy = 0; jy < ny; jy++)
+    for (int jx = 0; jx < lh; jx++) {
+      double kx = (jx < lh - 1) ? (double)jx : 0.0;
+      int m = ((jy + ny / 2) % ny) - ny / 2;
+      double ky = (double)m;
+      if (jx == lh - 1) { kx = 0.0; ky = 0.0; }
+      if (jy == ny / 2) { kx = 0.0; ky = 0.0; }
+      kx = 2.0 * PI / c->P.Lx * kx;
+      ky = 2.0 * PI / c->P.Ly * ky;
+      c->kx[jx + lh * jy] = kx;
+      c->ky[jx + lh * jy] = ky;
+      c->k2[jx + lh * jy] = kx * kx + ky * ky;
+    }
+}
+
+or_ctx *or_create(const or_params *p) {
+  or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+  c->P = *p;
+  if (c->P.cs_count <= 0) c->P.cs_count = 5;
+  c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
+  c->lh = p->nx / 2 + 1; c->ld = 2 * c->lh;
+  c->nx2 = 3 * p->nx / 2; c->ny2 = 3 * p->ny / 2;
+  c->lh_big = c->nx2 / 2 + 1; c->ld_big = 2 * c->lh_big;
+  c->dx = p->Lx / p->nx; c->dy = p->Ly / p->ny;
+  c->dz = p->Lz / (p->nz - 1);                 /* param.f90:64 */
+  c->delta = sqrt(c->dx * c->dy);              /* sgs_stag.f90:43, filter_size=1 */
+  size_t n2 = (size_t)c->lh * c->ny;
+  c->kx = (double *)malloc(n2 * sizeof(double));
+  c->ky = (double *)malloc(n2 * sizeof(double));
+  c->k2 = (double *)malloc(n2 * sizeof(double));
+  c->Gt = (double *)malloc(n2 * sizeof(double));
+  c->Gtt = (double *)malloc(n2 * sizeof(double));
+  plan2_init(&c->sm, c->nx, c->ny);
+  plan2_init(&c->bg, c->nx2, c->ny2);
+  init_wavenumber(c);
+  or_test_filter_init(c, 2.0, c->Gt);    /* main.f90:166 */
+  or_test_filter_init(c, 4.0, c->Gtt);   /* main.f90:169 */
+  return c;
+}
+
+void or_destroy(or_ctx *c) {
+  if (!c) return;
+  free(c->kx); free(c->ky); free(c->k2); free(c->Gt); free(c->Gtt);
+  fplan_free(&c->sm.px); fplan_free(&c->sm.py); fplan_free(&c->bg.px); fplan_free(&c->bg.py);
+  free(c);
+}
+
+void or_get_tables(const or_ctx *c, double *kx, double *ky, double *k2, double *G, double *GG) {
+  size_t n2 = (size_t)c->lh * c->ny * sizeof(double);
+  if (kx) memcpy(kx, c->kx, n2);
+  if (ky) memcpy(ky, c->ky, n2);
+  if (k2) memcpy(k2, c->k2, n2);
+  if (G) memcpy(G, c->Gt, n2);
+  if (GG) memcpy(GG, c->Gtt, n2);
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-D real transforms, FFTW-2 rfftwnd in-place semantics              */
+/* ------------------------------------------------------------------ */
+static void rfft2_fwd(const plan2 *P, double *a) {
+  const int nx = P->nx, ny = P->ny, lh = P->lh, ld = P->ld;
+  cpx *z = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
+  cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
+  cpx *out = (cpx *)malloc(sizeof(cpx) * (size_t)lh * ny);
+  for (int y = 0; y < ny; y += 2) {          /* two real rows per complex FFT */
+    for (int x = 0; x < nx; x++) { z[x].re = a[x + (size_t)ld * y]; z[x].im = a[x + (size_t)ld * (y + 1)]; }
+    fft_exec(&P->px, z, t, -1);
+    for (int k = 0; k < lh; k++) {
+      cpx zk = z[k % nx], zn = z[(nx - k) % nx];
+      /* A = (zk + conj zn)/2, B = (zk - conj zn)/(2i) */
+      cpx A = { 0.5 * (zk.re + zn.re), 0.5 * (zk.im - zn.im) };
+      double dr = zk.re - zn.re, di = zk.im + zn.im;
+      cpx B = { 0.5 * di, -0.5 * dr };
+      out[k + (size_t)lh * y] = A;
+      out[k + (size_t)lh * (y + 1)] = B;
+    }
+  }
+  for (int k = 0; k < lh; k++) {
+    for (int y = 0; y < ny; y++) z[y] = out[k + (size_t)lh * y];
+    fft_exec(&P->py, z, t, -1);
+    for (int y = 0; y < ny; y++) out[k + (size_t)lh * y] = z[y];
+  }
+  memcpy(a, out, sizeof(cpx) * (size_t)lh * ny);
+  free(z); free(t); free(out);
+}
+
+static void rfft2_inv(const plan2 *P, double *a) {
+  const int nx = P->nx, ny = P->ny, lh = P->lh, ld = P->ld;
+  cpx *X = (cpx *)malloc(sizeof(cpx) * (size_t)lh * ny);
+  memcpy(X, a, sizeof(cpx) * (size_t)lh * ny);
+  cpx *z = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
+  cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
+  for (int k = 0; k < lh; k++) {
+    for (int y = 0; y < ny; y++) z[y] = X[k + (size_t)lh * y];
+    fft_exec(&P->py, z, t, +1);
+    for (int y = 0; y < ny; y++) X[k + (size_t)lh * y] = z[y];
+  }
+  for (int y = 0; y < ny; y += 2) {
+    cpx *A = X + (size_t)lh * y, *B = X + (size_t)lh * (y + 1);
+    A[0].im = 0.0; B[0].im = 0.0; A[nx / 2].im = 0.0; B[nx / 2].im = 0.0;
+    for (int k = 0; k <= nx / 2; k++) {
+      z[k].re = A[k].re - B[k].im; z[k].im = A[k].im + B[k].re;          /* A + iB */
+      if (k > 0 && k < nx / 2) {
+        z[nx - k].re = A[k].re + B[k].im; z[nx - k].im = B[k].re - A[k].im; /* conj A + i conj B */
+      }
+    }
+    fft_exec(&P->px, z, t, +1);
+    for (int x = 0; x < nx; x++) { a[x + (size_t)ld * y] = z[x].re; a[x + (size_t)ld * (y + 1)] = z[x].im; }
+    for (int x = nx; x < ld; x++) { a[x + (size_t)ld * y] = 0.0; a[x + (size_t)ld * (y + 1)] = 0.0; }
+  }
+  free(X); free(z); free(t);
+}
+
+void or_rfft2_fwd(or_ctx *c, double *plane, int big) { rfft2_fwd(big ? &c->bg : &c->sm, plane); }
+void or_rfft2_inv(or_ctx *c, double *plane, int big) { rfft2_inv(big ? &c->bg : &c->sm, plane); }
+
+/* helpers on complex (lh,ny) planes stored interleaved in double arrays */
+#define CRE(a, i) ((a)[2 * (size_t)(i)])
+#define CIM(a, i) ((a)[2 * (size_t)(i) + 1])
+
+static void kill_oddballs(const or_ctx *c, double *pl) {
+  const int lh = c->lh, ny = c->ny;
+  for (int y = 0; y < ny; y++) { CRE(pl, lh - 1 + lh * y) = 0.0; CIM(pl, lh - 1 + lh * y) = 0.0; }
+  for (int x = 0; x < lh; x++) { CRE(pl, x + lh * (ny / 2)) = 0.0; CIM(pl, x + lh * (ny / 2)) = 0.0; }
+}
+/* out = (i*k) * in, as Fortran's eye*k*f (eye*k first) */
+static void mul_ik(const or_ctx *c, const double *k, const double *in, double *out) {
+  const size_t n = (size_t)c->lh * c->ny;
+  for (size_t i = 0; i < n; i++) {
+    double ar = CRE(in, i), ai = CIM(in, i), kk = k[i];
+    double wr = 0.0 * kk, wi = 1.0 * kk;           /* eye*k */
+    CRE(out, i) = wr * ar - wi * ai;
+    CIM(out, i) = wr * ai + wi * ar;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* filt_da.f90:3-42, ddx.f90:1-31, ddy.f90:1-31, ddxy.f90:1-29          */
+/* ------------------------------------------------------------------ */
+void or_filt_da(or_ctx *c, double *f, double *dfdx, double *dfdy, int nlev) {
+  const size_t pl = (size_t)c->ld * c->ny;
+  const double cst = 1.0 / (c->nx * c->ny);
+  for (int k = 0; k < nlev; k++) {
+    double *F = f + pl * k, *DX = dfdx + pl * k, *DY = dfdy + pl * k;
+    for (size_t i = 0; i < pl; i++) F[i] = cst * F[i];
+    rfft2_fwd(&c->sm, F);
+    kill_oddballs(c, F);
+    mul_ik(c, c->ky, F, DY);
+    mul_ik(c, c->kx, F, DX);
+    rfft2_inv(&c->sm, F);
+    rfft2_inv(&c->sm, DX);
+    rfft2_inv(&c->sm, DY);
+  }
+}
+
+static void ddxy_one(or_ctx *c, double *out, const double *f, const double *k, int nlev) {
+  const size_t pl = (size_t)c->ld * c->ny;
+  const double cst = 1.0 / (c->nx * c->ny);
+  for (int l = 0; l < nlev; l++) {
+    double *O = out + pl * l; const double *F = f + pl * l;
+    for (size_t i = 0; i < pl; i++) O[i] = cst * F[i];
+    rfft2_fwd(&c->sm, O);
+    kill_oddballs(c, O);
+    mul_ik(c, k, O, O);
+    rfft2_inv(&c->sm, O);
+  }
+}
+void or_ddx(or_ctx *c, double *dfdx, const double *f, int nlev) { ddxy_one(c, dfdx, f, c->kx, nlev); }
+void or_ddy(or_ctx *c, double *dfdy, const double *f, int nlev) { ddxy_one(c, dfdy, f, c->ky, nlev); }
+
+void or_ddxy(or_ctx *c, double *dfdx, double *dfdy, const double *f, int nlev) {
+  const size_t pl = (size_t)c->ld * c->ny;
+  const double cst = 1.0 / (c->nx * c->ny);
+  for (int l = 0; l < nlev; l++) {
+    double *DX = dfdx + pl * l, *DY = dfdy + pl * l; const double *F = f + pl * l;
+    for (size_t i = 0; i < pl; i++) DX[i] = cst * F[i];
+    rfft2_fwd(&c->sm, DX);
+    kill_oddballs(c, DX);
+    mul_ik(c, c->ky, DX, DY);
+    mul_ik(c, c->kx, DX, DX);
+    rfft2_inv(&c->sm, DX);
+    rfft2_inv(&c->sm, DY);
+  }
+}
+
+/* test_filtermodule.f90:32-63 (ifilter=1 sharp cutoff) */
+void or_test_filter_init(or_ctx *c, double alpha, double *G) {
+  const int lh = c->lh, ny = c->ny;
+  const double delta = alpha * sqrt(c->dx * c->dy);
+  const double q = PI / delta, kc2 = q * q;
+  for (int i = 0; i < lh * ny; i++) {
+    G[i] = 1.0 / (c->nx * c->ny);
+    if (c->k2[i] >= kc2) G[i] = 0.0;
+  }
+  for (int y = 0; y < ny; y++) G[lh - 1 + lh * y] = 0.0;
+  for (int x = 0; x < lh; x++) G[x + lh * (ny / 2)] = 0.0;
+}
+
+/* test_filtermodule.f90:13-30 */
+void or_test_filter(or_ctx *c, double *plane, const double *G) {
+  rfft2_fwd(&c->sm, plane);
+  const size_t n = (size_t)c->lh * c->ny;
+  for (size_t i = 0; i < n; i++) { CRE(plane, i) *= G[i]; CIM(plane, i) *= G[i]; }
+  rfft2_inv(&c->sm, plane);
+}
+
+/* padd.f90:1-26 (complex planes) */
+void or_padd(or_ctx *c, double *big, const double *u) {
+  const int lh = c->lh, lhb = c->lh_big, nx = c->nx, ny = c->ny, ny2 = c->ny2;
+  memset(big, 0, sizeof(double) * 2 * (size_t)lhb * ny2);
+  for (int y = 0; y < ny / 2; y++)
+    for (int x = 0; x < nx / 2; x++) {
+      CRE(big, x + lhb * y) = CRE(u, x + lh * y); CIM(big, x + lhb * y) = CIM(u, x + lh * y);
+    }
+  for (int y = 0; y < ny / 2 - 1; y++)
+    for (int x = 0; x < nx / 2; x++) {
+      int yb = y + ny2 - ny / 2 + 1, ys = y + ny / 2 + 1;
+      CRE(big, x + lhb * yb) = CRE(u, x + lh * ys); CIM(big, x + lhb * yb) = CIM(u, x + lh * ys);
+    }
+}
+
+/* unpadd.f90:1-22 */
+void or_unpadd(or_ctx *c, double *cc, const double *big) {
+  const int lh = c->lh, lhb = c->lh_big, nx = c->nx, ny = c->ny, ny2 = c->ny2;
+  for (int y = 0; y < ny / 2; y++)
+    for (int x = 0; x < nx / 2; x++) {
+      CRE(cc, x + lh * y) = CRE(big, x + lhb * y); CIM(cc, x + lh * y) = CIM(big, x + lhb * y);
+    }
+  for (int y = 0; y < ny; y++) { CRE(cc, nx / 2 + lh * y) = 0.0; CIM(cc, nx / 2 + lh * y) = 0.0; }
+  for (int x = 0; x < lh; x++) { CRE(cc, x + lh * (ny / 2)) = 0.0; CIM(cc, x + lh * (ny / 2)) = 0.0; }
+  for (int y = 0; y < ny / 2 - 1; y++)
+    for (int x = 0; x < nx / 2; x++) {
+      int yb = y + ny2 - ny / 2 + 1, ys = y + ny / 2 + 1;
+      CRE(cc, x + lh * ys) = CRE(big, x + lhb * yb); CIM(cc, x + lh * ys) = CIM(big, x + lhb * yb);
+    }
+}
+
+/* dealias1.f90:1-31 */
+void or_dealias1(or_ctx *c, const double *u, double *u_big, int nlev) {
+  const size_t pl = (size_t)c->ld * c->ny, plb = (size_t)c->ld_big * c->ny2;
+  const double cst = 1.0 / (c->nx * c->ny);
+  double *tmp = (double *)malloc(sizeof(double) * pl);
+  for (int k = 0; k < nlev; k++) {
+    for (size_t i = 0; i < pl; i++) tmp[i] = cst * u[pl * k + i];
+    rfft2_fwd(&c->sm, tmp);
+    or_padd(c, u_big + plb * k, tmp);
+    rfft2_inv(&c->bg, u_big + plb * k);
+  }
+  free(tmp);
+}
+
+/* dealias2.f90:1-33 */
+void or_dealias2(or_ctx *c, double *u, double *u_big, int nlev) {
+  const size_t pl = (size_t)c->ld * c->ny, plb = (size_t)c->ld_big * c->ny2;
+  const double cst = 1.0 / (c->nx2 * c->ny2);
+  for (size_t i = 0; i < plb * nlev; i++) u_big[i] = cst * u_big[i];
+  for (int k = 0; k < nlev; k++) {
+    rfft2_fwd(&c->bg, u_big + plb * k);
+    or_unpadd(c, u + pl * k, u_big + plb * k);
+    rfft2_inv(&c->sm, u + pl * k);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* ddz_uv.f90:5-57, ddz_w.f90:6-51 (non-MPI)                            */
+/* ------------------------------------------------------------------ */
+void or_ddz_uv(or_ctx *c, double *dfdz, const double *f) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double cst = 1.0 / c->dz;
+  for (int k = 1; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = x + (size_t)ld * y;
+        dfdz[pl * k + i] = cst * (f[pl * k + i] - f[pl * (k - 1) + i]);
+      }
+  for (size_t i = 0; i < pl; i++) dfdz[pl * (nz - 1) + i] = 0.0;
+}
+
+void or_ddz_w(or_ctx *c, double *dfdz, const double *f) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double cst = 1.0 / c->dz;
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = x + (size_t)ld * y;
+        dfdz[pl * k + i] = cst * (f[pl * (k + 1) + i] - f[pl * k + i]);
+      }
+  for (size_t i = 0; i < pl; i++) dfdz[pl * (nz - 1) + i] = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* convec.f90:1-248 (lbc_mom='wall', non-MPI)                           */
+/* ------------------------------------------------------------------ */
+void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
+               const double *du1d2, const double *du1d3, const double *du2d1,
+               const double *du2d3, const double *du3d1, const double *du3d2,
+               double *cx, double *cy, double *cz) {
+  const int nz = c->nz;
+  const size_t pl = (size_t)c->ld * c->ny, plb = (size_t)c->ld_big * c->ny2;
+  double *b_u1 = (double *)calloc(plb * nz, sizeof(double));
+  double *b_u2 = (double *)calloc(plb * nz, sizeof(double));
+  double *b_u3 = (double *)calloc(plb * nz, sizeof(double));
+  double *b_v1 = (double *)calloc(plb * nz, sizeof(double));
+  double *b_v2 = (double *)calloc(plb * nz, sizeof(double));
+  double *b_v3 = (double *)calloc(plb * nz, sizeof(double));
+  double *cc = (double *)calloc(plb, sizeof(double));
+  double cst = 1.0 / (c->nx * c->ny);
+  /* convec.f90:45-65 */
+  for (int k = 0; k < nz; k++) {
+    double *X = cx + pl * k, *Y = cy + pl * k, *Z = cz + pl * k;
+    for (size_t i = 0; i < pl; i++) { X[i] = cst * u1[pl * k + i]; Y[i] = cst * u2[pl * k + i]; Z[i] = cst * u3[pl * k + i]; }
+    rfft2_fwd(&c->sm, X); rfft2_fwd(&c->sm, Y); rfft2_fwd(&c->sm, Z);
+    or_padd(c, b_u1 + plb * k, X); or_padd(c, b_u2 + plb * k, Y); or_padd(c, b_u3 + plb * k, Z);
+    rfft2_inv(&c->bg, b_u1 + plb * k); rfft2_inv(&c->bg, b_u2 + plb * k); rfft2_inv(&c->bg, b_u3 + plb * k);
+  }
+  /* convec.f90:67-114 vorticity */
+  for (int k = 0; k < nz; k++) {
+    double *X = cx + pl * k, *Y = cy + pl * k, *Z = cz + pl * k;
+    for (size_t i = 0; i < pl; i++) {
+      size_t a = pl * k + i;
+      if (k == 0) {
+        X[i] = cst * (0.5 * (du3d2[a] + du3d2[pl + i]) - du2d3[a]);
+        Y[i] = cst * (du1d3[a] - 0.5 * (du3d1[a] + du3d1[pl + i]));
+      } else {
+        X[i] = cst * (du3d2[a] - du2d3[a]);
+        Y[i] = cst * (du1d3[a] - du3d1[a]);
+      }
+      Z[i] = cst * (du2d1[a] - du1d2[a]);
+    }
+    rfft2_fwd(&c->sm, X); rfft2_fwd(&c->sm, Y); rfft2_fwd(&c->sm, Z);
+    or_padd(c, b_v1 + plb * k, X); or_padd(c, b_v2 + plb * k, Y); or_padd(c, b_v3 + plb * k, Z);
+    rfft2_inv(&c->bg, b_v1 + plb * k); rfft2_inv(&c->bg, b_v2 + plb * k); rfft2_inv(&c->bg, b_v3 + plb * k);
+  }
+  cst = 1.0 / (c->nx2 * c->ny2);
+  /* CX: convec.f90:121-152 */
+  for (int k = 0; k < nz - 1; k++) {
+    for (size_t i = 0; i < plb; i++) {
+      if (k == 0)
+        cc[i] = cst * (b_u2[i] * (-b_v3[i]) + 0.5 * b_u3[plb + i] * (b_v2[plb + i]));
+      else
+        cc[i] = cst * (b_u2[plb * k + i] * (-b_v3[plb * k + i]) +
+                       0.5 * (b_u3[plb * (k + 1) + i] * (b_v2[plb * (k + 1) + i]) +
+                              b_u3[plb * k + i] * (b_v2[plb * k + i])));
+    }
+    rfft2_fwd(&c->bg, cc);
+    or_unpadd(c, cx + pl * k, cc);
+    rfft2_inv(&c->sm, cx + pl * k);
+  }
+  /* CY: convec.f90:157-188 */
+  for (int k = 0; k < nz - 1; k++) {
+    for (size_t i = 0; i < plb; i++) {
+      if (k == 0)
+        cc[i] = cst * (b_u1[i] * (b_v3[i]) + 0.5 * b_u3[plb + i] * (-b_v1[plb + i]));
+      else
+        cc[i] = cst * (b_u1[plb * k + i] * (b_v3[plb * k + i]) +
+                       0.5 * (b_u3[plb * (k + 1) + i] * (-b_v1[plb * (k + 1) + i]) +
+                              b_u3[plb * k + i] * (-b_v1[plb * k + i])));
+    }
+    rfft2_fwd(&c->bg, cc);
+    or_unpadd(c, cy + pl * k, cc);
+    rfft2_inv(&c->sm, cy + pl * k);
+  }
+  /* CZ: convec.f90:192-233 */
+  for (int k = 0; k < nz - 1; k++) {
+    for (size_t i = 0; i < plb; i++) {
+      if (k == 0)
+        cc[i] = 0.0;
+      else
+        cc[i] = cst * 0.5 * ((b_u1[plb * k + i] + b_u1[plb * (k - 1) + i]) * (-b_v2[plb * k + i]) +
+                             (b_u2[plb * k + i] + b_u2[plb * (k - 1) + i]) * (b_v1[plb * k + i]));
+    }
+    rfft2_fwd(&c->bg, cc);
+    or_unpadd(c, cz + pl * k, cc);
+    rfft2_inv(&c->sm, cz + pl * k);
+  }
+  for (size_t i = 0; i < pl; i++) { cx[pl * (nz - 1) + i] = BOGUS; cy[pl * (nz - 1) + i] = BOGUS; cz[pl * (nz - 1) + i] = BOGUS; }
+  free(b_u1); free(b_u2); free(b_u3); free(b_v1); free(b_v2); free(b_v3); free(cc);
+}
+
+/* ------------------------------------------------------------------ */
+/* wallstress.f90:5-69, 'wall', WALL_HOMOG_FLAG=.false.                 */
+/* ------------------------------------------------------------------ */
+void or_wallstress(or_ctx *c, const double *u, const double *v, double *dudz,
+                   double *dvdz, double *txz, double *tyz, const double *psi_m,
+                   const double *psi_m0, const double *phi_m, double *ustar_avg) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld;
+  const size_t pl = (size_t)ld * ny;
+  double *u1 = (double *)malloc(sizeof(double) * pl), *v1 = (double *)malloc(sizeof(double) * pl);
+  memcpy(u1, u, sizeof(double) * pl); memcpy(v1, v, sizeof(double) * pl);
+  or_test_filter(c, u1, c->Gtt);
+  or_test_filter(c, v1, c->Gtt);
+  const double dz = c->dz;
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      size_t i = x + (size_t)ld * y, j = x + (size_t)nx * y;
+      double denom = log(0.5 * dz / c->P.z0) - psi_m[j] + psi_m0[j];
+      double u_avg = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
+      double us = u_avg * VONK / denom;
+      ustar_avg[j] = us;
+      double cst = -(us * us) / u_avg;
+      txz[i] = cst * u1[i];
+      tyz[i] = cst * v1[i];
+      double du = us / (0.5 * dz * VONK) * u1[i] / u_avg * phi_m[j];
+      double dv = us / (0.5 * dz * VONK) * v1[i] / u_avg * phi_m[j];
+      dudz[i] = (u[i] == 0.0) ? 0.0 : du;
+      dvdz[i] = (v[i] == 0.0) ? 0.0 : dv;
+    }
+  free(u1); free(v1);
+}
+
+/* ------------------------------------------------------------------ */
+/* sgs_stag.f90:371-498 calc_Sij ('wall', non-MPI)                      */
+/* ------------------------------------------------------------------ */
+void or_calc_sij(or_ctx *c, const double *const g[9], double *const S[6]) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  for (int s = 0; s < 6; s++) memset(S[s], 0, sizeof(double) * pl * nz);
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      size_t i = x + (size_t)ld * y;
+      double ux = g[0][i], uy = g[1][i], uz = g[2][i];
+      double vx = g[3][i], vy = g[4][i], vz = g[5][i];
+      double wx = 0.5 * (g[6][i] + g[6][pl + i]);
+      double wy = 0.5 * (g[7][i] + g[7][pl + i]);
+      double wz = g[8][i];
+      S[0][i] = ux; S[1][i] = 0.5 * (uy + vx); S[2][i] = 0.5 * (uz + wx);
+      S[3][i] = vy; S[4][i] = 0.5 * (vz + wy); S[5][i] = wz;
+    }
+  for (int k = 1; k < nz; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y, m = i - pl;
+        double ux = 0.5 * (g[0][i] + g[0][m]);
+        double uy = 0.5 * (g[1][i] + g[1][m]);
+        double uz = g[2][i];
+        double vx = 0.5 * (g[3][i] + g[3][m]);
+        double vy = 0.5 * (g[4][i] + g[4][m]);
+        double vz = g[5][i];
+        double wx = g[6][i];
+        double wy = g[7][i];
+        double wz = 0.5 * (g[8][i] + g[8][m]);
+        S[0][i] = ux; S[1][i] = 0.5 * (uy + vx); S[2][i] = 0.5 * (uz + wx);
+        S[3][i] = vy; S[4][i] = 0.5 * (vz + wy); S[5][i] = wz;
+      }
+}
+
+/* ------------------------------------------------------------------ */
+/* interpolag_Sdep.f90:2-380 (non-MPI, read_inflow_file=.false.)        */
+/* ------------------------------------------------------------------ */
+void or_interpolag_sdep(or_ctx *c, or_sgs_state *s) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  const double cs = (double)c->P.cs_count, dt = c->P.dt;
+  double *FF[4];
+  double *F[4] = { s->F_LM, s->F_MM, s->F_QN, s->F_NN };
+  for (int q = 0; q < 4; q++) { FF[q] = (double *)malloc(sizeof(double) * n3); memcpy(FF[q], F[q], sizeof(double) * n3); }
+  double *ul = s->u_lag, *vl = s->v_lag, *wl = s->w_lag;
+  for (size_t i = 0; i < n3; i++) { ul[i] = ul[i] / cs; vl[i] = vl[i] / cs; }
+  for (int k = nz - 1; k >= 1; k--)
+    for (size_t i = 0; i < pl; i++) {
+      ul[pl * k + i] = 0.5 * (ul[pl * k + i] + ul[pl * (k - 1) + i]);
+      vl[pl * k + i] = 0.5 * (vl[pl * k + i] + vl[pl * (k - 1) + i]);
+    }
+  for (size_t i = 0; i < n3; i++) wl[i] = wl[i] / cs;
+  for (size_t i = 0; i < pl; i++) wl[i] = 0.25 * wl[pl + i];
+  for (size_t i = 0; i < n3; i++) {
+    ul[i] = -ul[i] * dt * cs / c->dx;
+    vl[i] = -vl[i] * dt * cs / c->dy;
+    wl[i] = -wl[i] * dt * cs / c->dz;
+  }
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      size_t i = x + (size_t)ld * y;
+      wl[pl + i] = wl[pl + i] + fmin(wl[pl + i], 0.0);
+      wl[i] = wl[i] + fmax(wl[i], 0.0);
+    }
+  for (size_t i = 0; i < pl; i++) {
+    wl[pl + i] = fmin(1.0, wl[pl + i]);
+    wl[i] = fmin(1.0, wl[pl + i]);
+    wl[pl + i] = fmax(-1.0, wl[pl + i]);
+    wl[i] = fmax(-1.0, wl[pl + i]);
+  }
+  for (int k = 0; k < nz; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        int ax = signbit(ul[i]) ? -1 : 1, ay = signbit(vl[i]) ? -1 : 1, az = signbit(wl[i]) ? -1 : 1;
+        int xa = (x + ax + nx) % nx, ya = (y + ay + ny) % ny;
+        int za = k + az; if (za < 0) za = 0; if (za > nz - 1) za = nz - 1;
+        double cx = fabs(ul[i]), cy = fabs(vl[i]), cz = fabs(wl[i]);
+        double fx = 1.0 - cx, fy = 1.0 - cy, fz = 1.0 - cz;
+        size_t i000 = pl * k + x + (size_t)ld * y, i00z = pl * za + x + (size_t)ld * y;
+        size_t i0y0 = pl * k + x + (size_t)ld * ya, i0yz = pl * za + x + (size_t)ld * ya;
+        size_t ix00 = pl * k + xa + (size_t)ld * y, ix0z = pl * za + xa + (size_t)ld * y;
+        size_t ixy0 = pl * k + xa + (size_t)ld * ya, ixyz = pl * za + xa + (size_t)ld * ya;
+        for (int q = 0; q < 4; q++) {
+          const double *G = FF[q];
+          F[q][i] = fx * fy * (G[i000] * fz + G[i00z] * cz)
+                  + fx * cy * (G[i0y0] * fz + G[i0yz] * cz)
+                  + cx * fy * (G[ix00] * fz + G[ix0z] * cz)
+                  + cx * cy * (G[ixy0] * fz + G[ixyz] * cz);
+        }
+      }
+  memset(ul, 0, sizeof(double) * n3); memset(vl, 0, sizeof(double) * n3); memset(wl, 0, sizeof(double) * n3);
+  for (int q = 0; q < 4; q++) free(FF[q]);
+}
+
+/* ------------------------------------------------------------------ */
+/* lagrange_Sdep.f90:4-478 (non-MPI, no buildings, no inflow)           */
+/* ------------------------------------------------------------------ */
+static inline double fq(double x) { return (double)(float)x; }   /* Fortran real(x) */
+static const float F1EM24 = 1e-24f;                               /* 1E-24 literal */
+
+void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
+                      const double *v, const double *w, or_sgs_state *s) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double delta = c->delta, tf1 = 2.0, tf2 = 4.0, tf1_2 = tf1 * tf1, tf2_2 = tf2 * tf2;
+  const double opftdelta = 1.5 * delta, powcoeff = -1.0 / 8.0;
+  const double e24 = (double)F1EM24;
+  or_interpolag_sdep(c, s);
+  const double lagran_dt = c->P.dt * (double)c->P.cs_count;
+  const double cst = 2.0 * (delta * delta);
+  const double expo = log(tf1) / (log(tf2) - log(tf1));
+  enum { UB, VB, WB, UH, VH, WH, L11, L12, L13, L22, L23, L33, Q11, Q12, Q13, Q22, Q23, Q33,
+         SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
+         SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
+         SMAG, NT };
+  double *T[NT];
+  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+  const int jt = c->jt;
+  for (int k = 0; k < nz; k++) {
+    const size_t o = pl * k;
+    for (size_t i = 0; i < pl; i++) {
+      double ub, vb, wb;
+      if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; }
+      else { ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i]; }
+      T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb;
+      T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb;
+      T[L11][i] = ub * ub; T[L12][i] = ub * vb; T[L13][i] = ub * wb;
+      T[L23][i] = vb * wb; T[L22][i] = vb * vb; T[L33][i] = wb * wb;
+      T[Q11][i] = ub * ub; T[Q12][i] = ub * vb; T[Q13][i] = ub * wb;
+      T[Q22][i] = vb * vb; T[Q23][i] = vb * wb; T[Q33][i] = wb * wb;
+    }
+    or_test_filter(c, T[UB], c->Gt); or_test_filter(c, T[VB], c->Gt); or_test_filter(c, T[WB], c->Gt);
+    const int Lp[6][3] = { { L11, UB, UB }, { L12, UB, VB }, { L13, UB, WB }, { L22, VB, VB }, { L23, VB, WB }, { L33, WB, WB } };
+    for (int q = 0; q < 6; q++) {
+      or_test_filter(c, T[Lp[q][0]], c->Gt);
+      for (size_t i = 0; i < pl; i++) T[Lp[q][0]][i] = T[Lp[q][0]][i] - T[Lp[q][1]][i] * T[Lp[q][2]][i];
+    }
+    or_test_filter(c, T[UH], c->Gtt); or_test_filter(c, T[VH], c->Gtt); or_test_filter(c, T[WH], c->Gtt);
+    const int Qp[6][3] = { { Q11, UH, UH }, { Q12, UH, VH }, { Q13, UH, WH }, { Q22, VH, VH }, { Q23, VH, WH }, { Q33, WH, WH } };
+    for (int q = 0; q < 6; q++) {
+      or_test_filter(c, T[Qp[q][0]], c->Gtt);
+      for (size_t i = 0; i < pl; i++) T[Qp[q][0]][i] = T[Qp[q][0]][i] - T[Qp[q][1]][i] * T[Qp[q][2]][i];
+    }
+    memcpy(s->L11t + o, T[L11], sizeof(double) * pl);
+    memcpy(s->L22t + o, T[L22], sizeof(double) * pl);
+    memcpy(s->L33t + o, T[L33], sizeof(double) * pl);
+    memcpy(s->Q11t + o, T[Q11], sizeof(double) * pl);
+    memcpy(s->Q22t + o, T[Q22], sizeof(double) * pl);
+    memcpy(s->Q33t + o, T[Q33], sizeof(double) * pl);
+    for (size_t i = 0; i < pl; i++) {
+      double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+      double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+      double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+      T[SMAG][i] = Sm;
+      T[SB11][i] = s11; T[SB12][i] = s12; T[SB13][i] = s13; T[SB22][i] = s22; T[SB23][i] = s23; T[SB33][i] = s33;
+      T[SH11][i] = s11; T[SH12][i] = s12; T[SH13][i] = s13; T[SH22][i] = s22; T[SH23][i] = s23; T[SH33][i] = s33;
+      T[SSB11][i] = Sm * s11; T[SSB12][i] = Sm * s12; T[SSB13][i] = Sm * s13;
+      T[SSB22][i] = Sm * s22; T[SSB23][i] = Sm * s23; T[SSB33][i] = Sm * s33;
+      T[SSH11][i] = T[SSB11][i]; T[SSH12][i] = T[SSB12][i]; T[SSH13][i] = T[SSB13][i];
+      T[SSH22][i] = T[SSB22][i]; T[SSH23][i] = T[SSB23][i]; T[SSH33][i] = T[SSB33][i];
+    }
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSB11 + q], c->Gt);
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSH11 + q], c->Gtt);
+    double *FLM = s->F_LM + o, *FMM = s->F_MM + o, *FQN = s->F_QN + o, *FNN = s->F_NN + o;
+    int init_lm = c->P.inilag && !c->flm_init && (jt == c->P.cs_count || jt == 1);
+    int init_qn = c->P.inilag && !c->fqn_init && (jt == c->P.cs_count || jt == 1);
+    double sum2d = 0.0, sum4d = 0.0, sumcs = 0.0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < ld; x++) {
+        size_t i = x + (size_t)ld * y;
+        double Sb = sqrt(2.0 * (T[SB11][i] * T[SB11][i] + T[SB22][i] * T[SB22][i] + T[SB33][i] * T[SB33][i] +
+                                2.0 * (T[SB12][i] * T[SB12][i] + T[SB13][i] * T[SB13][i] + T[SB23][i] * T[SB23][i])));
+        double Sh = sqrt(2.0 * (T[SH11][i] * T[SH11][i] + T[SH22][i] * T[SH22][i] + T[SH33][i] * T[SH33][i] +
+                                2.0 * (T[SH12][i] * T[SH12][i] + T[SH13][i] * T[SH13][i] + T[SH23][i] * T[SH23][i])));
+        double M[6], N[6], Lv[6], Qv[6];
+        for (int q = 0; q < 6; q++) {
+          M[q] = cst * (T[SSB11 + q][i] - tf1_2 * Sb * T[SB11 + q][i]);
+          N[q] = cst * (T[SSH11 + q][i] - tf2_2 * Sh * T[SH11 + q][i]);
+        }
+        Lv[0] = T[L11][i]; Lv[1] = T[L12][i]; Lv[2] = T[L13][i]; Lv[3] = T[L22][i]; Lv[4] = T[L23][i]; Lv[5] = T[L33][i];
+        Qv[0] = T[Q11][i]; Qv[1] = T[Q12][i]; Qv[2] = T[Q13][i]; Qv[3] = T[Q22][i]; Qv[4] = T[Q23][i]; Qv[5] = T[Q33][i];
+        /* index map q: 0=11 1=12 2=13 3=22 4=23 5=33 */
+        double LM = Lv[0] * M[0] + Lv[3] * M[3] + Lv[5] * M[5] + 2.0 * (Lv[1] * M[1] + Lv[2] * M[2] + Lv[4] * M[4]);
+        double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+        double QN = Qv[0] * N[0] + Qv[3] * N[3] + Qv[5] * N[5] + 2.0 * (Qv[1] * N[1] + Qv[2] * N[2] + Qv[4] * N[4]);
+        double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+        if (init_lm) {
+          FMM[i] = MM; FLM[i] = 0.03 * MM;
+          if (x >= ld - 2) { FMM[i] = 1.0; FLM[i] = 1.0; }
+        }
+        double Tn = fq(fmax((double)(float)(FLM[i] * FMM[i]), e24));
+        s->TnLMMM[o + i] = Tn;
+        Tn = opftdelta * pow(Tn, powcoeff);
+        Tn = fmax(e24, fq(Tn));
+        double dumfac = lagran_dt / Tn;
+        double epsi = dumfac / (1.0 + dumfac);
+        FLM[i] = (epsi * LM + (1.0 - epsi) * FLM[i]);
+        FMM[i] = (epsi * MM + (1.0 - epsi) * FMM[i]);
+        FLM[i] = fmax(e24, fq(FLM[i]));
+        double cs2d = FLM[i] / FMM[i];
+        if (x >= ld - 2) cs2d = e24;
+        cs2d = fmax(e24, fq(cs2d));
+        if (init_qn) {
+          FNN[i] = NN; FQN[i] = 0.03 * NN;
+          if (x >= ld - 2) { FNN[i] = 1.0; FQN[i] = 1.0; }
+        }
+        Tn = fmax(fq(FQN[i] * FNN[i]), e24);
+        s->TnQNNN[o + i] = Tn;
+        Tn = opftdelta * pow(Tn, powcoeff);
+        Tn = fmax(e24, fq(Tn));
+        dumfac = lagran_dt / Tn;
+        epsi = dumfac / (1.0 + dumfac);
+        FQN[i] = (epsi * QN + (1.0 - epsi) * FQN[i]);
+        FNN[i] = (epsi * NN + (1.0 - epsi) * FNN[i]);
+        FQN[i] = fmax(e24, fq(FQN[i]));
+        double cs4d = FQN[i] / FNN[i];
+        if (x >= ld - 2) cs4d = e24;
+        cs4d = fmax(e24, fq(cs4d));
+        double beta = pow(cs4d / cs2d, expo);
+        if (k == nz - 1) beta = 1.0;
+        s->Beta[o + i] = beta;
+        double bclip = fmax(fq(beta), fq(1.0 / (tf1 * tf2)));
+        double cso = cs2d / bclip;
+        if (x >= ld - 2) cso = e24;
+        s->Cs_opt2[o + i] = fmax(e24, fq(cso));
+        if (x < nx) { sum2d += cs2d; sum4d += cs4d; sumcs += s->Cs_opt2[o + i]; }
+      }
+    if (init_lm && k == nz - 1) c->flm_init = 1;
+    if (init_qn && k == nz - 1) c->fqn_init = 1;
+    s->Beta_avg[k] = sum2d / sumcs;
+    s->Betaclip_avg[k] = sum4d / sum2d;
+  }
+  for (int t = 0; t < NT; t++) free(T[t]);
+}
+
+/* ------------------------------------------------------------------ */
+/* sgs_stag.f90:3-366 (model=5, sgs=.true., molec=.false., 'wall')      */
+/* ------------------------------------------------------------------ */
+void or_sgs_stag(or_ctx *c, const double *const g[9], const double *u,
+                 const double *v, const double *w, double *const tau[6],
+                 or_sgs_state *s) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  double *S[6];
+  for (int q = 0; q < 6; q++) S[q] = (double *)calloc(n3, sizeof(double));
+  or_calc_sij(c, g, S);
+  double *txx = tau[0], *txy = tau[1], *txz = tau[2], *tyy = tau[3], *tyz = tau[4], *tzz = tau[5];
+  double *txzp = (double *)malloc(sizeof(double) * pl), *tyzp = (double *)malloc(sizeof(double) * pl);
+  memcpy(txzp, txz, sizeof(double) * pl); memcpy(tyzp, tyz, sizeof(double) * pl);
+  for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
+  const int jt = c->jt;
+  const double l = c->delta;
+  if (jt == 1 && c->P.inilag) {
+    for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = 0.03;
+  } else if ((jt >= 1 || c->P.initu) && (jt % c->P.cs_count == 0)) {
+    or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
+  }
+  if (c->jt_total < 500) {
+    const double cs0 = (double)(0.16f * 0.16f);
+    for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = cs0;
+  }
+  for (int k = 0; k < nz; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        double Sm = sqrt(2.0 * (S[0][i] * S[0][i] + S[3][i] * S[3][i] + S[5][i] * S[5][i] +
+                                2.0 * (S[1][i] * S[1][i] + S[2][i] * S[2][i] + S[4][i] * S[4][i])));
+        s->Nu_t[i] = Sm * s->Cs_opt2[i] * (l * l);
+      }
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      size_t i = x + (size_t)ld * y;
+      double cst = s->Nu_t[i];
+      txx[i] = -2.0 * (cst + 0.0) * S[0][i];
+      txy[i] = -2.0 * (cst + 0.0) * S[1][i];
+      tyy[i] = -2.0 * (cst + 0.0) * S[3][i];
+      tzz[i] = -2.0 * (cst + 0.0) * S[5][i];
+    }
+  for (int k = 1; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y, p = i + pl;
+        double cst = 0.5 * (s->Nu_t[i] + s->Nu_t[p]);
+        txx[i] = -2.0 * (cst + 0.0) * 0.5 * (S[0][i] + S[0][p]);
+        txy[i] = -2.0 * (cst + 0.0) * 0.5 * (S[1][i] + S[1][p]);
+        tyy[i] = -2.0 * (cst + 0.0) * 0.5 * (S[3][i] + S[3][p]);
+        tzz[i] = -2.0 * (cst + 0.0) * 0.5 * (S[5][i] + S[5][p]);
+      }
+  for (int k = 1; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        double cst = s->Nu_t[i];
+        txz[i] = -2.0 * (cst + 0.0) * S[2][i];
+        tyz[i] = -2.0 * (cst + 0.0) * S[4][i];
+      }
+  memcpy(txz, txzp, sizeof(double) * pl); memcpy(tyz, tyzp, sizeof(double) * pl);
+  const size_t top = pl * (nz - 1);
+  for (size_t i = 0; i < pl; i++) {
+    txx[top + i] = BOGUS; txy[top + i] = BOGUS; tyy[top + i] = BOGUS; tzz[top + i] = BOGUS;
+    txz[top + i] = 0.0; tyz[top + i] = 0.0;
+  }
+  free(txzp); free(tyzp);
+  for (int q = 0; q < 6; q++) free(S[q]);
+}
+
+/* ------------------------------------------------------------------ */
+/* divstress_uv.f90:1-63, divstress_w.f90:4-86 (non-MPI)               */
+/* ------------------------------------------------------------------ */
+void or_divstress_uv(or_ctx *c, double *divt, const double *tx, const double *ty, const double *tz) {
+  const int nz = c->nz, ld = c->ld, ny = c->ny;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  double *a = (double *)calloc(n3, sizeof(double)), *b = (double *)calloc(n3, sizeof(double));
+  double *d = (double *)calloc(n3, sizeof(double));
+  or_ddx(c, a, tx, nz);
+  or_ddy(c, b, ty, nz);
+  or_ddz_w(c, d, tz);
+  for (size_t i = 0; i < pl * (nz - 1); i++) divt[i] = a[i] + b[i] + d[i];
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++) { divt[pl * k + ld - 2 + (size_t)ld * y] = 0.0; divt[pl * k + ld - 1 + (size_t)ld * y] = 0.0; }
+  for (size_t i = 0; i < pl; i++) divt[pl * (nz - 1) + i] = BOGUS;
+  free(a); free(b); free(d);
+}
+
+void or_divstress_w(or_ctx *c, double *divt, const double *tx, const double *ty, const double *tz) {
+  const int nz = c->nz, ld = c->ld, ny = c->ny, nx = c->nx;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  double *a = (double *)calloc(n3, sizeof(double)), *b = (double *)calloc(n3, sizeof(double));
+  double *d = (double *)calloc(n3, sizeof(double));
+  or_ddx(c, a, tx, nz);
+  or_ddy(c, b, ty, nz);
+  or_ddz_uv(c, d, tz);
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) { size_t i = x + (size_t)ld * y; divt[i] = a[i] + b[i]; }
+  for (int k = 1; k < nz; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) { size_t i = pl * k + x + (size_t)ld * y; divt[i] = a[i] + b[i] + d[i]; }
+  for (int k = 0; k < nz; k++)
+    for (int y = 0; y < ny; y++) { divt[pl * k + ld - 2 + (size_t)ld * y] = 0.0; divt[pl * k + ld - 1 + (size_t)ld * y] = 0.0; }
+  free(a); free(b); free(d);
+}
+
+/* ------------------------------------------------------------------ */
+/* tridag_array.f90:1-109 (non-MPI)                                    */
+/* ------------------------------------------------------------------ */
+int or_tridag_array(or_ctx *c, const double *a, const double *b, const double *cc, const double *r, double *u) {
+  const int lh = c->lh, ny = c->ny, n = c->nz + 1;
+  const size_t pl = (size_t)lh * ny;
+  double *bet = (double *)malloc(sizeof(double) * pl);
+  double *gam = (double *)calloc(pl * n, sizeof(double));
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < lh - 1; x++)
+      if (b[x + (size_t)lh * y] == 0.0) { free(bet); free(gam); return 1; }
+  for (size_t i = 0; i < pl; i++) {
+    bet[i] = b[i];
+    CRE(u, i) = CRE(r, i) / bet[i];
+    CIM(u, i) = CIM(r, i) / bet[i];
+  }
+  for (int j = 1; j < n; j++)
+    for (int y = 0; y < ny; y++) {
+      if (y == ny / 2) continue;
+      for (int x = 0; x < lh - 1; x++) {
+        if (x == 0 && y == 0) continue;
+        size_t m = x + (size_t)lh * y, i = pl * j + m, ip = i - pl;
+        gam[i] = cc[ip] / bet[m];
+        bet[m] = b[i] - a[i] * gam[i];
+        if (bet[m] == 0.0) { free(bet); free(gam); return 2; }
+        CRE(u, i) = (CRE(r, i) - a[i] * CRE(u, ip)) / bet[m];
+        CIM(u, i) = (CIM(r, i) - a[i] * CIM(u, ip)) / bet[m];
+      }
+    }
+  for (int j = n - 2; j >= 0; j--)
+    for (int y = 0; y < ny; y++) {
+      if (y == ny / 2) continue;
+      for (int x = 0; x < lh - 1; x++) {
+        if (x == 0 && y == 0) continue;
+        size_t m = x + (size_t)lh * y, i = pl * j + m, in = i + pl;
+        CRE(u, i) = CRE(u, i) - gam[in] * CRE(u, in);
+        CIM(u, i) = CIM(u, i) - gam[in] * CIM(u, in);
+      }
+    }
+  free(bet); free(gam);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* press_stag_array.f90:1-294 (non-MPI, lbc=1)                          */
+/* ------------------------------------------------------------------ */
+void or_press_stag_array(or_ctx *c, const double *u, const double *v, const double *w,
+                         const double *divtz, double *p, double *dpdx, double *dpdy) {
+  const int nz = c->nz, ld = c->ld, ny = c->ny, lh = c->lh;
+  const size_t pl = (size_t)ld * ny, cpl = (size_t)lh * ny;
+  const double dz = c->dz, dt = c->P.dt;
+  const double cst = 1.0 / (c->nx * c->ny);
+  double *Hx = (double *)calloc(pl * nz, sizeof(double));
+  double *Hy = (double *)calloc(pl * nz, sizeof(double));
+  double *Hz = (double *)calloc(pl * nz, sizeof(double));
+  double *topw = (double *)malloc(sizeof(double) * pl), *botw = (double *)malloc(sizeof(double) * pl);
+  const int n = nz + 1;
+  double *a = (double *)calloc(cpl * n, sizeof(double)), *b = (double *)calloc(cpl * n, sizeof(double));
+  double *cc = (double *)calloc(cpl * n, sizeof(double)), *R = (double *)calloc(2 * cpl * n, sizeof(double));
+  /* p_hat(:,:,0)=0 */
+  for (size_t i = 0; i < pl; i++) p[i] = 0.0;
+  /* jz = 1..nz-1 -> array level k = jz-1 */
+  for (int k = 0; k < nz - 1; k++) {
+    for (size_t i = 0; i < pl; i++) {
+      Hx[pl * k + i] = cst / TADV1 * (u[pl * k + i] / dt);
+      Hy[pl * k + i] = cst / TADV1 * (v[pl * k + i] / dt);
+      Hz[pl * k + i] = cst / TADV1 * (w[pl * k + i] / dt);
+    }
+    rfft2_fwd(&c->sm, Hx + pl * k); rfft2_fwd(&c->sm, Hy + pl * k); rfft2_fwd(&c->sm, Hz + pl * k);
+  }
+  /* H_x(:,:,nz) = H_y(:,:,nz) = BOGUS (complex), H_z(:,:,nz) = 0 */
+  for (size_t i = 0; i < cpl; i++) {
+    CRE(Hx + pl * (nz - 1), i) = BOGUS; CIM(Hx + pl * (nz - 1), i) = 0.0;
+    CRE(Hy + pl * (nz - 1), i) = BOGUS; CIM(Hy + pl * (nz - 1), i) = 0.0;
+    CRE(Hz + pl * (nz - 1), i) = 0.0; CIM(Hz + pl * (nz - 1), i) = 0.0;
+  }
+  for (size_t i = 0; i < pl; i++) { botw[i] = cst * divtz[i]; topw[i] = cst * divtz[pl * (nz - 1) + i]; }
+  rfft2_fwd(&c->sm, botw); rfft2_fwd(&c->sm, topw);
+  for (int k = 0; k < nz - 1; k++) { kill_oddballs(c, Hx + pl * k); kill_oddballs(c, Hy + pl * k); kill_oddballs(c, Hz + pl * k); }
+  kill_oddballs(c, topw); kill_oddballs(c, botw);
+  /* rows: RHS_col index j=1..nz+1 -> array j-1 */
+  for (size_t m = 0; m < cpl; m++) {
+    a[m] = BOGUS; b[m] = -1.0; cc[m] = 1.0;
+    CRE(R, m) = -dz * CRE(botw, m); CIM(R, m) = -dz * CIM(botw, m);
+    size_t t = cpl * nz + m;
+    a[t] = -1.0; b[t] = 1.0; cc[t] = BOGUS;
+    CRE(R, t) = -CRE(topw, m) * dz; CIM(R, t) = -CIM(topw, m) * dz;
+  }
+  for (int jz = 2; jz <= nz; jz++)
+    for (int y = 0; y < ny; y++) {
+      if (y == ny / 2) continue;
+      for (int x = 0; x < lh - 1; x++) {
+        if (x == 0 && y == 0) continue;
+        size_t m = x + (size_t)lh * y, t = cpl * (jz - 1) + m;
+        double kx = c->kx[m], ky = c->ky[m];
+        a[t] = 1.0 / (dz * dz);
+        b[t] = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+        cc[t] = 1.0 / (dz * dz);
+        const double *hxm = Hx + pl * (jz - 2), *hym = Hy + pl * (jz - 2);
+        const double *hzm = Hz + pl * (jz - 2), *hz = Hz + pl * (jz - 1);
+        /* eye*(kx*Hx + ky*Hy) + (Hz(jz) - Hz(jz-1))/dz */
+        double sr = kx * CRE(hxm, m) + ky * CRE(hym, m), si = kx * CIM(hxm, m) + ky * CIM(hym, m);
+        double er = 0.0 * sr - 1.0 * si, ei = 0.0 * si + 1.0 * sr;
+        CRE(R, t) = er + (CRE(hz, m) - CRE(hzm, m)) / dz;
+        CIM(R, t) = ei + (CIM(hz, m) - CIM(hzm, m)) / dz;
+      }
+    }
+  /* p_hat levels 0..nz = u(1..n): p array stores complex planes interleaved */
+  double *U = (double *)calloc(2 * cpl * n, sizeof(double));
+  /* tridag writes only solved modes for j>=2; other modes keep p_hat's prior
+     content, which is zeroed below (Nyquist) or set by the zero-mode chain */
+  int err = or_tridag_array(c, a, b, cc, R, U);
+  if (err) fprintf(stderr, "or_press_stag_array: tridag failed (%d)\n", err);
+  for (int j = 0; j < n; j++) {
+    double *P = p + pl * j;
+    for (size_t m = 0; m < cpl; m++) { CRE(P, m) = CRE(U + 2 * cpl * j, m); CIM(P, m) = CIM(U + 2 * cpl * j, m); }
+  }
+  /* zero mode: press_stag_array.f90:238-247 */
+  CRE(p, 0) = 0.0; CIM(p, 0) = 0.0;
+  CRE(p + pl, 0) = CRE(p, 0) - dz * CRE(botw, 0);
+  CIM(p + pl, 0) = CIM(p, 0) - dz * CIM(botw, 0);
+  for (int jz = 2; jz <= nz; jz++) {
+    const double *hz = Hz + pl * (jz - 1);
+    CRE(p + pl * jz, 0) = CRE(p + pl * (jz - 1), 0) + CRE(hz, 0) * dz;
+    CIM(p + pl * jz, 0) = CIM(p + pl * (jz - 1), 0) + CIM(hz, 0) * dz;
+  }
+  for (int j = 0; j < n; j++) kill_oddballs(c, p + pl * j);
+  rfft2_inv(&c->sm, p);
+  for (int k = 1; k <= nz - 1; k++) {
+    double *P = p + pl * k;
+    mul_ik(c, c->kx, P, dpdx + pl * (k - 1));
+    mul_ik(c, c->ky, P, dpdy + pl * (k - 1));
+    rfft2_inv(&c->sm, dpdx + pl * (k - 1));
+    rfft2_inv(&c->sm, dpdy + pl * (k - 1));
+    rfft2_inv(&c->sm, P);
+  }
+  for (size_t m = 0; m < cpl; m++) {
+    CRE(dpdx + pl * (nz - 1), m) = BOGUS; CIM(dpdx + pl * (nz - 1), m) = 0.0;
+    CRE(dpdy + pl * (nz - 1), m) = BOGUS; CIM(dpdy + pl * (nz - 1), m) = 0.0;
+    CRE(p + pl * nz, m) = BOGUS; CIM(p + pl * nz, m) = 0.0;
+  }
+  free(Hx); free(Hy); free(Hz); free(topw); free(botw); free(a); free(b); free(cc); free(R); free(U);
+}
+
+/* ------------------------------------------------------------------ */
+/* forcing.f90:128-219 project (no buildings/inflow, 'wall')           */
+/* ------------------------------------------------------------------ */
+void or_project(or_ctx *c, double *u, double *v, double *w, const double *dpdx,
+                const double *dpdy, const double *dpdz) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double dt = c->P.dt;
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        double R = -TADV1 * dpdx[i];
+        u[i] = (u[i] + dt * (R + 0.0));
+        R = -TADV1 * dpdy[i];
+        v[i] = (v[i] + dt * (R + 0.0));
+      }
+  for (int k = 1; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        double R = -TADV1 * dpdz[i];
+        w[i] = (w[i] + dt * (R + 0.0));
+      }
+  for (size_t i = 0; i < pl; i++) {
+    u[pl * (nz - 1) + i] = u[pl * (nz - 2) + i];
+    v[pl * (nz - 1) + i] = v[pl * (nz - 2) + i];
+    w[pl * (nz - 1) + i] = 0.0;
+    w[i] = 0.0;
+  }
+}
+
+/* rmsdiv.f90:1-113 (norm_magdu=.false.) */
+double or_rmsdiv(or_ctx *c, const double *du, const double *dv, const double *dw) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  double rms = 0.0;
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        rms = rms + fabs(du[i] + dv[i] + dw[i]);
+      }
+  return rms / ((double)nx * ny * (nz - 1));
+}
+
+/* scalars_module2.f90:22-70 (non-MPI).  out = {CFLu, CFLv, CFLw, visc_stab};
+   the non-MPI reference leaves v_res_max/w_res_max undefined, here they are
+   the per-component maxima of |v| and |w|. */
+void or_timestep_conditions(or_ctx *c, const double *u, const double *v, const double *w,
+                            const double *Nu_t, double out[4]) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  double umax = -1.0, vmax = 0.0, wmax = 0.0, numax = -INFINITY;
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        double m = u[i] * u[i] + v[i] * v[i] + w[i] * w[i];
+        if (m > umax) umax = m;
+        if (fabs(v[i]) > vmax) vmax = fabs(v[i]);
+        if (fabs(w[i]) > wmax) wmax = fabs(w[i]);
+        if (Nu_t[i] > numax) numax = Nu_t[i];
+      }
+  double ures = sqrt(umax), delta = fmin(c->dx, c->dy), dt = c->P.dt;
+  out[0] = ures * dt / c->dx;
+  out[1] = vmax * dt / c->dy;
+  out[2] = wmax * dt / c->dz;
+  out[3] = dt * numax / (delta * delta);
+}
+
+/* ------------------------------------------------------------------ */
+/* state + whole step (main.f90:202-415, S_FLAG=.false., model=5)      */
+/* ------------------------------------------------------------------ */
+static double *al3(const or_ctx *c, int nlev) { return (double *)calloc((size_t)c->ld * c->ny * nlev, sizeof(double)); }
+
+or_state *or_state_create(or_ctx *c) {
+  or_state *s = (or_state *)calloc(1, sizeof(or_state));
+  const int nz = c->nz;
+  s->u = al3(c, nz); s->v = al3(c, nz); s->w = al3(c, nz);
+  for (int q = 0; q < 9; q++) s->grad[q] = al3(c, nz);
+  s->RHSx = al3(c, nz); s->RHSy = al3(c, nz); s->RHSz = al3(c, nz);
+  s->RHSx_f = al3(c, nz); s->RHSy_f = al3(c, nz); s->RHSz_f = al3(c, nz);
+  for (int q = 0; q < 6; q++) s->tau[q] = al3(c, nz);
+  s->divtx = al3(c, nz); s->divty = al3(c, nz); s->divtz = al3(c, nz);
+  s->p = al3(c, nz + 1);
+  s->dpdx = al3(c, nz); s->dpdy = al3(c, nz); s->dpdz = al3(c, nz);
+  size_t n2 = (size_t)c->nx * c->ny;
+  s->psi_m = (double *)calloc(n2, sizeof(double)); s->psi_m0 = (double *)calloc(n2, sizeof(double));
+  s->phi_m = (double *)calloc(n2, sizeof(double)); s->ustar_avg = (double *)calloc(n2, sizeof(double));
+  for (size_t i = 0; i < n2; i++) s->phi_m[i] = 1.0;
+  or_sgs_state *g = &s->sgs;
+  g->F_LM = al3(c, nz); g->F_MM = al3(c, nz); g->F_QN = al3(c, nz); g->F_NN = al3(c, nz);
+  g->Beta = al3(c, nz); g->Cs_opt2 = al3(c, nz); g->Nu_t = al3(c, nz);
+  g->u_lag = al3(c, nz); g->v_lag = al3(c, nz); g->w_lag = al3(c, nz);
+  g->TnLMMM = al3(c, nz); g->TnQNNN = al3(c, nz);
+  g->L11t = al3(c, nz); g->L22t = al3(c, nz); g->L33t = al3(c, nz);
+  g->Q11t = al3(c, nz); g->Q22t = al3(c, nz); g->Q33t = al3(c, nz);
+  g->Beta_avg = (double *)calloc(nz, sizeof(double)); g->Betaclip_avg = (double *)calloc(nz, sizeof(double));
+  return s;
+}
+
+void or_state_destroy(or_state *s) {
+  if (!s) return;
+  free(s->u); free(s->v); free(s->w);
+  for (int q = 0; q < 9; q++) free(s->grad[q]);
+  free(s->RHSx); free(s->RHSy); free(s->RHSz); free(s->RHSx_f); free(s->RHSy_f); free(s->RHSz_f);
+  for (int q = 0; q < 6; q++) free(s->tau[q]);
+  free(s->divtx); free(s->divty); free(s->divtz); free(s->p); free(s->dpdx); free(s->dpdy); free(s->dpdz);
+  free(s->psi_m); free(s->psi_m0); free(s->phi_m); free(s->ustar_avg);
+  or_sgs_state *g = &s->sgs;
+  free(g->F_LM); free(g->F_MM); free(g->F_QN); free(g->F_NN); free(g->Beta); free(g->Cs_opt2); free(g->Nu_t);
+  free(g->u_lag); free(g->v_lag); free(g->w_lag); free(g->TnLMMM); free(g->TnQNNN);
+  free(g->L11t); free(g->L22t); free(g->L33t); free(g->Q11t); free(g->Q22t); free(g->Q33t);
+  free(g->Beta_avg); free(g->Betaclip_avg);
+  free(s);
+}
+
+double or_uniform(uint64_t seed, uint64_t index) {
+  uint64_t z = seed * 0x9E3779B97F4A7C15ULL + index + 0x632BE59BD9B4E019ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void or_state_init_ic(or_ctx *c, or_state *s, uint64_t seed, double amp, double ustar, double kappa_ic) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  double *f3[3] = { s->u, s->v, s->w };
+  for (int k = 0; k < nz - 1; k++) {
+    double z = (k + 0.5) * c->dz;
+    double ulog = (ustar / kappa_ic) * log(z / c->P.z0);
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = pl * k + x + (size_t)ld * y;
+        s->u[i] = ulog; s->v[i] = 0.0; s->w[i] = 0.0;
+        if (z < 0.5)
+          for (int q = 0; q < 3; q++) {
+            uint64_t idx = (((uint64_t)q * nz + k) * ny + y) * nx + x;
+            f3[q][i] += amp * ustar * (2.0 * or_uniform(seed, idx) - 1.0);
+          }
+      }
+  }
+  for (size_t i = 0; i < pl; i++) {
+    s->w[i] = 0.0;
+    s->u[pl * (nz - 1) + i] = s->u[pl * (nz - 2) + i];
+    s->v[pl * (nz - 1) + i] = s->v[pl * (nz - 2) + i];
+    s->w[pl * (nz - 1) + i] = 0.0;
+  }
+  s->jt = 0; s->jt_total = 0;
+  c->flm_init = 0; c->fqn_init = 0;
+}
+
+void or_step(or_ctx *c, or_state *s) {
+  const int nz = c->nz, nx = c->nx, ny = c->ny, ld = c->ld;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz, nv = pl * (nz - 1);
+  s->jt += 1; s->jt_total += 1;
+  c->jt = s->jt; c->jt_total = s->jt_total;
+  memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
+  memcpy(s->RHSy_f, s->RHSy, n3 * sizeof(double));
+  memcpy(s->RHSz_f, s->RHSz, n3 * sizeof(double));
+  /* obukhov, neutral: psi=0, phi=1 (scalars_module.f90:642-649) */
+  for (size_t i = 0; i < (size_t)nx * ny; i++) { s->psi_m[i] = 0.0; s->psi_m0[i] = 0.0; s->phi_m[i] = 1.0; }
+  double **g = s->grad;
+  or_filt_da(c, s->u, g[0], g[1], nz);
+  or_filt_da(c, s->v, g[3], g[4], nz);
+  or_filt_da(c, s->w, g[6], g[7], nz);
+  or_ddz_uv(c, g[2], s->u);
+  or_ddz_uv(c, g[5], s->v);
+  or_ddz_w(c, g[8], s->w);
+  or_wallstress(c, s->u, s->v, g[2], g[5], s->tau[2], s->tau[4], s->psi_m, s->psi_m0, s->phi_m, s->ustar_avg);
+  or_sgs_stag(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
+  or_divstress_uv(c, s->divtx, s->tau[0], s->tau[1], s->tau[2]);
+  or_divstress_uv(c, s->divty, s->tau[1], s->tau[3], s->tau[4]);
+  or_divstress_w(c, s->divtz, s->tau[2], s->tau[4], s->tau[5]);
+  or_convec(c, s->u, s->v, s->w, g[1], g[2], g[3], g[5], g[6], g[7], s->RHSx, s->RHSy, s->RHSz);
+  for (size_t i = 0; i < nv; i++) {
+    s->RHSx[i] = -s->RHSx[i] - s->divtx[i];
+    s->RHSy[i] = -s->RHSy[i] - s->divty[i];
+    s->RHSz[i] = -s->RHSz[i] - s->divtz[i];
+  }
+  const double force = c->P.force;
+  if (s->jt == 1 && !c->P.initu) {
+    memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
+    memcpy(s->RHSy_f, s->RHSy, n3 * sizeof(double));
+    memcpy(s->RHSz_f, s->RHSz, n3 * sizeof(double));
+  }
+  const double dt = c->P.dt;
+  for (size_t i = 0; i < nv; i++) {
+    s->u[i] = s->u[i] + dt * (TADV1 * s->RHSx[i] + TADV2 * s->RHSx_f[i] + force);
+    s->v[i] = s->v[i] + dt * (TADV1 * s->RHSy[i] + TADV2 * s->RHSy_f[i]);
+    s->w[i] = s->w[i] + dt * (TADV1 * s->RHSz[i] + TADV2 * s->RHSz_f[i]);
+  }
+  for (size_t i = 0; i < pl; i++) { s->u[nv + i] = BOGUS; s->v[nv + i] = BOGUS; s->w[nv + i] = BOGUS; }
+  or_press_stag_array(c, s->u, s->v, s->w, s->divtz, s->p, s->dpdx, s->dpdy);
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        size_t i = x + (size_t)ld * y;
+        s->dpdz[pl * k + i] = (s->p[pl * (k + 1) + i] - s->p[pl * k + i]) / c->dz;
+      }
+  for (size_t i = 0; i < pl; i++) s->dpdz[nv + i] = BOGUS;
+  for (size_t i = 0; i < nv; i++) {
+    s->RHSx[i] = s->RHSx[i] - s->dpdx[i];
+    s->RHSy[i] = s->RHSy[i] - s->dpdy[i];
+    s->RHSz[i] = s->RHSz[i] - s->dpdz[i];
+  }
+  or_project(c, s->u, s->v, s->w, s->dpdx, s->dpdy, s->dpdz);
+}
+
+void or_set_time(or_ctx *c, int jt, int jt_total) { c->jt = jt; c->jt_total = jt_total; }
+void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init) { c->flm_init = flm_init; c->fqn_init = fqn_init; }
