@@ -15,8 +15,8 @@ def pytest_configure(config):
 
 def relerr(a, b):
     """max|a-b| / max|b| (interior values)."""
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     den = np.max(np.abs(b))
     if den == 0.0:
         return float(np.max(np.abs(a)))

@@ -1,0 +1,626 @@
+"""Pins for the CPU oracle (oracle/les_oracle.c) -- what fixes each function
+independently of the way the oracle computes it:
+  * transforms against numpy.fft (a library routine);
+  * derivatives, filters, zero padding and dealiased products against closed
+    forms of band-limited Fourier modes;
+  * staggering rules against fields that are linear in z, for which the
+    interpolations of sgs_stag/convec are exact;
+  * the wall law against its uniform-flow closed form;
+  * the pressure projection against its defining invariant (discrete
+    divergence of the projected velocity = 0);
+  * the tridiagonal solver against numpy.linalg.solve;
+  * the Lagrangian model against invariants (Galilean, x<->y symmetry,
+    translation) and single-mode closed forms of L_ij;
+  * the whole step against translation invariance and divergence-freeness.
+The reference (Fortran + FFTW 2.1.5) cannot be built here, so these are the
+checks that make a dropped term, a wrong sign or index or a transposed
+operand in the oracle fail.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def grid_xy(o):
+    x = np.arange(o.nx) * o.dx
+    y = np.arange(o.ny) * o.dy
+    return np.meshgrid(x, y)  # arrays (ny, nx)
+
+
+def put(o, arr2d_per_level):
+    f = o.field(len(arr2d_per_level))
+    for k, a in enumerate(arr2d_per_level):
+        f[k, :, :o.nx] = a
+    return f
+
+
+# ---------------------------------------------------------------- FFT layer
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 72, 96, 128, 144, 192, 256, 384])
+def test_cfft1_matches_numpy(O, n):
+    rng = np.random.default_rng(n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    a = np.ascontiguousarray(z.copy())
+    O.lib().or_cfft1(n, a.ctypes.data_as(O.D), -1)
+    assert relerr(a, np.fft.fft(z)) < 1e-14
+    b = np.ascontiguousarray(z.copy())
+    O.lib().or_cfft1(n, b.ctypes.data_as(O.D), +1)
+    assert relerr(b, np.fft.ifft(z) * n) < 1e-14
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (48, 32), (16, 24)])
+def test_rfft2_matches_numpy(O, nx, ny):
+    o = O.Oracle(nx, ny, 5)
+    rng = np.random.default_rng(1)
+    for big in (False, True):
+        mx, my = (o.nx2, o.ny2) if big else (nx, ny)
+        ld = o.ld_big if big else o.ld
+        a = np.zeros((my, ld))
+        a[:, :mx] = rng.standard_normal((my, mx))
+        b = a.copy()
+        o.rfft2_fwd(b, big=big)
+        spec = b.view(np.complex128).reshape(my, ld // 2)
+        assert relerr(spec, np.fft.rfft2(a[:, :mx])) < 1e-14
+        o.rfft2_inv(b, big=big)
+        assert relerr(b[:, :mx] / (mx * my), a[:, :mx]) < 1e-14
+        assert np.all(b[:, mx:] == 0.0)   # pad columns defined = 0
+
+
+# ------------------------------------------------ horizontal spectral ops
+def test_filt_da_modes_and_nyquist(O):
+    o = O.Oracle(48, 32, 4, Lx=4 * math.pi, Ly=2 * math.pi)
+    X, Y = grid_xy(o)
+    a1, b1 = 2 * math.pi / o.params.Lx, 2 * math.pi / o.params.Ly
+    f0 = np.sin(3 * a1 * X + 2 * b1 * Y) + 0.5 * np.cos(5 * a1 * X - 7 * b1 * Y)
+    nyq = np.cos(math.pi * np.arange(o.nx))[None, :] * np.ones((o.ny, 1))  # kx = nx/2 mode
+    nyq_y = np.cos(math.pi * np.arange(o.ny))[:, None] * np.cos(a1 * X)     # ky = ny/2 mode
+    f = put(o, [f0 + nyq + nyq_y] * 4)
+    dx, dy = o.field(), o.field()
+    o.filt_da(f, dx, dy)
+    ex_dx = 3 * a1 * np.cos(3 * a1 * X + 2 * b1 * Y) - 0.5 * 5 * a1 * np.sin(5 * a1 * X - 7 * b1 * Y)
+    ex_dy = 2 * b1 * np.cos(3 * a1 * X + 2 * b1 * Y) + 0.5 * 7 * b1 * np.sin(5 * a1 * X - 7 * b1 * Y)
+    for k in range(4):
+        assert np.max(np.abs(f[k, :, :o.nx] - f0)) < 1e-13
+        assert np.max(np.abs(dx[k, :, :o.nx] - ex_dx)) < 1e-12
+        assert np.max(np.abs(dy[k, :, :o.nx] - ex_dy)) < 1e-12
+
+
+def test_ddx_ddy_ddxy_and_f_untouched(O):
+    o = O.Oracle(32, 48, 3, Lx=2 * math.pi, Ly=3 * math.pi)
+    X, Y = grid_xy(o)
+    a1, b1 = 2 * math.pi / o.params.Lx, 2 * math.pi / o.params.Ly
+    g = np.cos(4 * a1 * X) * np.sin(9 * b1 * Y)
+    f = put(o, [g, 2 * g, -g])
+    f0 = f.copy()
+    dx, dy = o.field(3), o.field(3)
+    o.ddx(dx, f)
+    o.ddy(dy, f)
+    assert np.array_equal(f, f0)
+    ex = -4 * a1 * np.sin(4 * a1 * X) * np.sin(9 * b1 * Y)
+    ey = 9 * b1 * np.cos(4 * a1 * X) * np.cos(9 * b1 * Y)
+    for k, s in enumerate((1, 2, -1)):
+        assert np.max(np.abs(dx[k, :, :o.nx] - s * ex)) < 1e-12
+        assert np.max(np.abs(dy[k, :, :o.nx] - s * ey)) < 1e-12
+    dx2, dy2 = o.field(3), o.field(3)
+    o.ddxy(dx2, dy2, f)
+    assert relerr(dx2[..., :o.nx], dx[..., :o.nx]) < 1e-15
+    assert relerr(dy2[..., :o.nx], dy[..., :o.nx]) < 1e-15
+
+
+def test_test_filter_sharp_cutoff(O):
+    o = O.Oracle(32, 32, 3)
+    X, Y = grid_xy(o)
+    delta = math.sqrt(o.dx * o.dy)
+    modes = [(1, 0), (3, 4), (5, 5), (8, 0), (6, 6), (0, 7), (2, 9), (12, 1)]
+    for alpha in (2.0, 4.0):
+        G = o.test_filter_init(alpha)
+        assert np.array_equal(G, o.tables()["G_test" if alpha == 2.0 else "G_test_test"])
+        kc2 = (math.pi / (alpha * delta)) ** 2
+        f = sum(np.cos(mx * X + my * Y + 0.3) for mx, my in modes)
+        keep = sum(np.cos(mx * X + my * Y + 0.3) for mx, my in modes if mx * mx + my * my < kc2)
+        pl = o.field(1)[0]
+        pl[:, :o.nx] = f
+        o.test_filter(pl, G)
+        assert np.max(np.abs(pl[:, :o.nx] - keep)) < 1e-12
+    # a mode exactly on the cutoff circle is removed (k2 >= kc2)
+    o2 = O.Oracle(32, 32, 3)
+    G = o2.test_filter_init(2.0)           # kc = pi/(2*dx) = 8 on this grid
+    pl = o2.field(1)[0]
+    pl[:, :32] = np.cos(8 * X)
+    o2.test_filter(pl, G)
+    assert np.max(np.abs(pl[:, :32])) < 1e-14
+
+
+def test_padd_unpadd_roundtrip(O):
+    o = O.Oracle(16, 24, 3)
+    rng = np.random.default_rng(3)
+    s = rng.standard_normal((o.ny, 2 * o.lh))
+    big = np.zeros((o.ny2, o.ld_big))
+    o.padd(big, s)
+    back = np.zeros_like(s)
+    o.unpadd(back, big)
+    expect = s.copy().view(np.complex128).reshape(o.ny, o.lh)
+    expect[:, -1] = 0
+    expect[o.ny // 2, :] = 0
+    assert np.array_equal(back.view(np.complex128).reshape(o.ny, o.lh), expect)
+
+
+def test_dealias1_evaluates_on_fine_grid(O):
+    o = O.Oracle(16, 24, 2, Lx=2 * math.pi, Ly=4 * math.pi)
+    X, Y = grid_xy(o)
+    b1 = 2 * math.pi / o.params.Ly
+    g = lambda x, y: np.sin(3 * x + 5 * b1 * y) + np.cos(7 * x)
+    u = put(o, [g(X, Y)] * 2)
+    ub = o.field(2, big=True)
+    o.dealias1(u, ub)
+    xb = np.arange(o.nx2) * (o.params.Lx / o.nx2)
+    yb = np.arange(o.ny2) * (o.params.Ly / o.ny2)
+    XB, YB = np.meshgrid(xb, yb)
+    assert np.max(np.abs(ub[0, :, :o.nx2] - g(XB, YB))) < 1e-13
+
+
+def test_dealias2_truncates_exact_product(O):
+    o = O.Oracle(16, 16, 2)
+    xb = np.arange(o.nx2) * (2 * math.pi / o.nx2)
+    XB, YB = np.meshgrid(xb, xb)
+    # sin(5x)*sin(6x) = [cos(x) - cos(11x)]/2: the k=11 part lies beyond the
+    # small grid and must vanish, k=1 must survive with weight 1/2
+    prod = np.sin(5 * XB) * np.sin(6 * XB) * np.cos(2 * YB)
+    ub = o.field(2, big=True)
+    ub[:, :, :o.nx2] = prod
+    u = o.field(2)
+    o.dealias2(u, ub)
+    X, Y = grid_xy(o)
+    assert np.max(np.abs(u[0, :, :o.nx] - 0.5 * np.cos(X) * np.cos(2 * Y))) < 1e-13
+
+
+# ------------------------------------------------------------ vertical FD
+def test_ddz_linear_profiles(O):
+    o = O.Oracle(8, 8, 7)
+    f = o.field()
+    for k in range(o.nz):
+        f[k] = 3.0 * k * o.dz
+    d = np.full_like(f, 7.0)
+    o.ddz_uv(d, f)
+    assert np.allclose(d[1:-1, :, :o.nx], 3.0) and np.all(d[0] == 7.0) and np.all(d[-1] == 0.0)
+    d2 = np.full_like(f, 7.0)
+    o.ddz_w(d2, f)
+    assert np.allclose(d2[:-1, :, :o.nx], 3.0) and np.all(d2[-1] == 0.0)
+
+
+# ------------------------------------------------------------------ convec
+def test_convec_closed_form(O):
+    """Single-mode fields: products are alias-free with 3/2 padding, and the
+    w-node products are averaged to the uvp level (convec.f90:121-139)."""
+    nz = 6
+    o = O.Oracle(16, 16, nz)
+    X, Y = grid_xy(o)
+    A = np.arange(1, nz + 1) * 0.3
+    B = np.arange(1, nz + 1) ** 2 * 0.1
+    C = np.linspace(0.0, 1.0, nz) ** 2
+    Dd = np.cos(np.arange(nz))
+    z2d = lambda a: [a] * nz
+    u = put(o, [A[k] * np.sin(X) for k in range(nz)])          # u1
+    v = put(o, [A[k] * np.sin(X) for k in range(nz)])          # u2
+    w = put(o, [C[k] * np.cos(X) for k in range(nz)])          # u3
+    zero = put(o, z2d(0 * X))
+    dvdx = put(o, [B[k] * np.cos(Y) for k in range(nz)])       # omega3 = dvdx - dudy
+    dudz = put(o, [Dd[k] * np.sin(Y) for k in range(nz)])      # omega2 = dudz - dwdx
+    cx, cy, cz = o.field(), o.field(), o.field()
+    o.convec(u, v, w, zero, dudz, dvdx, zero, zero, zero, cx, cy, cz)
+    for k in range(nz - 1):
+        om3 = B[k] * np.cos(Y)
+        if k == 0:
+            w2 = 0.5 * C[1] * Dd[1] * np.cos(X) * np.sin(Y)
+        else:
+            w2 = 0.5 * (C[k + 1] * Dd[k + 1] + C[k] * Dd[k]) * np.cos(X) * np.sin(Y)
+        ex = A[k] * np.sin(X) * (-om3) + w2
+        assert np.max(np.abs(cx[k, :, :o.nx] - ex)) < 1e-13, k
+        # cy = u1*omega3 + <u3*(-omega1)>, omega1 = 0 here
+        assert np.max(np.abs(cy[k, :, :o.nx] - A[k] * np.sin(X) * om3)) < 1e-13, k
+        # cz = 1/2[(u1(k)+u1(k-1))(-omega2(k)) + (u2(k)+u2(k-1)) omega1(k)]
+        if k == 0:
+            assert np.max(np.abs(cz[0, :, :o.nx])) < 1e-15
+        else:
+            ez = 0.5 * (A[k] + A[k - 1]) * np.sin(X) * (-Dd[k] * np.sin(Y))
+            assert np.max(np.abs(cz[k, :, :o.nx] - ez)) < 1e-13, k
+    assert np.all(cx[-1] == -1234567890.0)
+
+
+def test_convec_dealiasing_removes_high_products(O):
+    o = O.Oracle(16, 16, 3)
+    X, Y = grid_xy(o)
+    nz = 3
+    u = put(o, [np.sin(6 * X)] * nz)
+    zero = put(o, [0 * X] * nz)
+    dvdx = put(o, [np.sin(6 * X)] * nz)        # omega3 = sin(6x): u2*omega3 = (1-cos 12x)/2
+    cx, cy, cz = o.field(), o.field(), o.field()
+    o.convec(u, u, zero, zero, zero, dvdx, zero, zero, zero, cx, cy, cz)
+    # cy = u1*omega3 -> truncation keeps only the mean 1/2 (12 > 8 is dropped, not aliased)
+    assert np.max(np.abs(cy[1, :, :o.nx] - 0.5)) < 1e-13
+
+
+# --------------------------------------------------------------- wall model
+def test_wallstress_uniform_flow(O):
+    o = O.Oracle(16, 16, 9, z0=1e-4)
+    U = 7.0
+    u = np.full((o.nz, o.ny, o.ld), U)
+    v = np.zeros_like(u)
+    dudz, dvdz, txz, tyz = o.field(), o.field(), o.field(), o.field()
+    us = o.wallstress(u, v, dudz, dvdz, txz, tyz)
+    ustar = U * 0.41 / math.log(0.5 * o.dz / 1e-4)
+    assert np.allclose(us, ustar, rtol=1e-14)
+    assert np.allclose(txz[0, :, :o.nx], -ustar ** 2, rtol=1e-13)
+    assert np.allclose(dudz[0, :, :o.nx], ustar / (0.5 * o.dz * 0.41), rtol=1e-13)
+    assert np.all(tyz[0, :, :o.nx] == 0) and np.all(dvdz[0, :, :o.nx] == 0)
+
+
+# ------------------------------------------------------- strain and stress
+def test_calc_sij_staggering_linear_z(O):
+    """Gradients equal to c*z at their own nodes (uvp: (k-1/2)dz, w: (k-1)dz)
+    give S_ij = c*z at the w-node (uvp node at k=1): pins every average."""
+    o = O.Oracle(8, 8, 7)
+    nz = o.nz
+    zu = (np.arange(nz) + 0.5) * o.dz
+    zw = np.arange(nz) * o.dz
+    cs = [1.0, 2.0, 3.0, 5.0, 7.0, 11.0, 13.0, 17.0, 19.0]
+    # node of each gradient: dudx,dudy uvp; dudz w (uvp at k=1: set by wallstress);
+    # dvdx,dvdy uvp; dvdz w; dwdx,dwdy w; dwdz uvp
+    node = ["u", "u", "w", "u", "u", "w", "w", "w", "u"]
+    grads = []
+    for c, nd in zip(cs, node):
+        f = o.field()
+        for k in range(nz):
+            zz = zu[k] if nd == "u" else (zu[0] if k == 0 and nd == "w" and c in (3.0, 11.0) else zw[k])
+            f[k] = c * zz
+        grads.append(f)
+    S = o.calc_sij(grads)
+    zout = np.where(np.arange(nz) == 0, zu[0], zw)
+    # S11 = ux, S12 = (uy+vx)/2, S13 = (uz+wx)/2, S22 = vy, S23 = (vz+wy)/2, S33 = wz
+    ex = [cs[0], (cs[1] + cs[3]) / 2, (cs[2] + cs[6]) / 2, cs[4], (cs[5] + cs[7]) / 2, cs[8]]
+    for s, e in zip(S, ex):
+        for k in range(1, nz):
+            assert np.allclose(s[k, :, :o.nx], e * zout[k], rtol=1e-14, atol=1e-15), k
+    # wall level: uvp quantities, w-gradients averaged from w-nodes 1,2 -> dz/2
+    for s, e in zip(S, ex):
+        assert np.allclose(s[0, :, :o.nx], e * zu[0], rtol=1e-14)
+
+
+def test_sgs_stag_uniform_strain_constant_cs(O):
+    o = O.Oracle(8, 8, 6)
+    st, arr = o.sgs_state()
+    grads = [np.full((o.nz, o.ny, o.ld), g) for g in (0.1, 0.2, 0.3, -0.1, 0.4, 0.5, 0.05, -0.2, -0.5)]
+    u = v = w = o.field()
+    taus = [o.field() for _ in range(6)]
+    taus[2][0] = 3.0   # wall values kept
+    taus[4][0] = 4.0
+    o.set_time(1, 1, True, True)
+    o.sgs_stag(grads, u, v, w, taus, st)
+    cs0 = 0.025599999353289604
+    assert np.all(arr["Cs_opt2"] == cs0)
+    ux, uy, uz, vx, vy, vz, wx, wy, wz = (0.1, 0.2, 0.3, -0.1, 0.4, 0.5, 0.05, -0.2, -0.5)
+    S = dict(S11=ux, S12=(uy + vx) / 2, S13=(uz + wx) / 2, S22=vy, S23=(vz + wy) / 2, S33=wz)
+    smag = math.sqrt(2 * (S["S11"] ** 2 + S["S22"] ** 2 + S["S33"] ** 2 +
+                          2 * (S["S12"] ** 2 + S["S13"] ** 2 + S["S23"] ** 2)))
+    delta = math.sqrt(o.dx * o.dy)
+    nu = smag * cs0 * delta ** 2
+    assert np.allclose(arr["Nu_t"][:, :, :o.nx], nu, rtol=1e-13)
+    for t, key in zip(taus, ("S11", "S12", "S13", "S22", "S23", "S33")):
+        lv = slice(0, o.nz - 1) if key not in ("S13", "S23") else slice(1, o.nz - 1)
+        assert np.allclose(t[lv, :, :o.nx], -2 * nu * S[key], rtol=1e-13), key
+    assert np.all(taus[2][0] == 3.0) and np.all(taus[4][0] == 4.0)
+    assert np.all(taus[2][-1] == 0.0) and np.all(taus[0][-1] == -1234567890.0)
+    assert np.allclose(arr["u_lag"], 0.0)
+
+
+# ---------------------------------------------------------- Lagrangian SGS
+def _sgs(o, seed, lag_scale=0.0):
+    st, arr = o.sgs_state()
+    rng = np.random.default_rng(seed)
+    for n in ("F_LM", "F_MM", "F_QN", "F_NN"):
+        arr[n][...] = 0.5 + rng.random(arr[n].shape)
+    return st, arr
+
+
+def test_interpolag_identity_constant_and_shift(O):
+    o = O.Oracle(16, 16, 6, dt=1e-3)
+    st, arr = _sgs(o, 1)
+    F0 = {n: arr[n].copy() for n in ("F_LM", "F_MM", "F_QN", "F_NN")}
+    o.interpolag_sdep(st)                     # zero displacement
+    for n in F0:
+        assert np.array_equal(arr[n][..., :o.nx], F0[n][..., :o.nx])
+    # constant field stays constant for any displacement (weights sum to 1)
+    for n in F0:
+        arr[n][...] = 2.5
+    rng = np.random.default_rng(2)
+    arr["u_lag"][...] = 3.0 * rng.standard_normal(arr["u_lag"].shape)
+    arr["v_lag"][...] = 3.0 * rng.standard_normal(arr["v_lag"].shape)
+    arr["w_lag"][...] = 3.0 * rng.standard_normal(arr["w_lag"].shape)
+    o.interpolag_sdep(st)
+    for n in F0:
+        assert np.allclose(arr[n][..., :o.nx], 2.5, rtol=1e-15)
+    assert np.all(arr["u_lag"] == 0) and np.all(arr["w_lag"] == 0)
+    # exact one-cell back-trajectory in x: xp = -u_lag*dt/dx = +1 -> F(x+1)
+    F = np.random.default_rng(3).random(arr["F_LM"].shape)
+    arr["F_LM"][...] = F
+    arr["u_lag"][...] = -o.dx / o.params.dt   # sum over cs_count steps of u = -dx/(dt*cs)
+    o.interpolag_sdep(st)
+    assert np.allclose(arr["F_LM"][:, :, :o.nx], np.roll(F[:, :, :o.nx], -1, axis=2), rtol=1e-13)
+
+
+def test_interpolag_wall_cell_rule(O):
+    """The level-2 displacement is doubled when it is negative (the first
+    cell is dz/2 high), both wall levels then use the clamped level-2 value
+    (interpolag_Sdep.f90:274-295)."""
+    for zp, zpp in ((0.3, 0.3), (-0.3, -0.6), (-0.7, -1.0)):
+        o = O.Oracle(8, 8, 6, dt=1e-3)
+        st, arr = _sgs(o, 4)
+        F = arr["F_MM"].copy()
+        arr["w_lag"][1] = -zp * o.dz / o.params.dt   # sum of w over cs_count steps
+        o.interpolag_sdep(st)
+        a = abs(zpp)
+        if zpp > 0:
+            ex1 = (1 - a) * F[1] + a * F[2]
+            ex0 = (1 - a) * F[0] + a * F[1]
+        else:
+            ex1 = (1 - a) * F[1] + a * F[0]
+            ex0 = (1 - a) * F[0] + a * F[0]      # below level 1 the ghost repeats level 1
+        assert np.allclose(arr["F_MM"][1, :, :o.nx], ex1[:, :o.nx], rtol=1e-13), zp
+        assert np.allclose(arr["F_MM"][0, :, :o.nx], ex0[:, :o.nx], rtol=1e-13), zp
+        assert np.array_equal(arr["F_MM"][3, :, :o.nx], F[3, :, :o.nx])
+
+
+def _lag_inputs(o, seed, vel_shift=(0.0, 0.0)):
+    rng = np.random.default_rng(seed)
+    X, Y = grid_xy(o)
+    lev = lambda fn: put(o, [fn(k) for k in range(o.nz)])
+    u = lev(lambda k: 1.0 + 0.3 * np.sin(X + 0.1 * k) * np.cos(2 * Y) + 0.05 * rng.standard_normal(X.shape)) + vel_shift[0]
+    v = lev(lambda k: 0.2 * np.cos(3 * X - Y + k) + 0.05 * rng.standard_normal(X.shape)) + vel_shift[1]
+    w = lev(lambda k: 0.1 * np.sin(2 * X + 2 * Y) * (k > 0) + 0.02 * rng.standard_normal(X.shape))
+    S = [lev(lambda k, j=j: 0.2 * np.cos((j + 1) * X + k) + 0.1 * rng.standard_normal(X.shape)) for j in range(6)]
+    for a in [u, v, w] + S:
+        a[..., o.nx:] = 0.0
+    return u, v, w, S
+
+
+def test_lagrange_galilean_invariance(O):
+    o = O.Oracle(16, 16, 5, dt=1e-3)
+    outs = []
+    for shift in ((0.0, 0.0), (3.0, -2.0)):
+        st, arr = _sgs(o, 9)
+        u, v, w, S = _lag_inputs(o, 5)
+        u[..., :o.nx] += shift[0]
+        v[..., :o.nx] += shift[1]
+        o.set_time(5, 600, flm_init=True, fqn_init=True)
+        o.lagrange_sdep(S, u, v, w, st)
+        outs.append({k: arr[k][..., :o.nx].copy() for k in ("L11t", "Q22t", "F_LM", "F_QN", "Cs_opt2", "Beta")})
+    for k in outs[0]:
+        if k in ("L11t", "Q22t"):
+            assert relerr(outs[1][k], outs[0][k]) < 1e-9, k
+        else:   # FP32 quantisation points: allow isolated float-ulp flips
+            bad = np.abs(outs[1][k] - outs[0][k]) > 1e-9 * np.abs(outs[0][k]) + 1e-7 * np.abs(outs[0][k])
+            assert bad.mean() < 0.02, k
+
+
+def test_lagrange_xy_symmetry(O):
+    """Transposing x<->y (u<->v, S11<->S22, S13<->S23) transposes every output."""
+    o = O.Oracle(16, 16, 5, dt=1e-3)
+    u, v, w, S = _lag_inputs(o, 6)
+    T = lambda a: np.ascontiguousarray(np.pad(np.transpose(a[..., :o.nx], (0, 2, 1)), ((0, 0), (0, 0), (0, 2))))
+    res = []
+    for flip in (False, True):
+        st, arr = _sgs(o, 11)
+        if flip:
+            for n in ("F_LM", "F_MM", "F_QN", "F_NN"):
+                arr[n][...] = T(arr[n])
+            uu, vv, ww = T(v), T(u), T(w)
+            SS = [T(S[3]), T(S[1]), T(S[4]), T(S[0]), T(S[2]), T(S[5])]
+        else:
+            uu, vv, ww, SS = u, v, w, S
+        o.set_time(5, 600, flm_init=True, fqn_init=True)
+        o.lagrange_sdep(SS, uu, vv, ww, st)
+        res.append({k: arr[k].copy() for k in ("L11t", "L22t", "F_LM", "F_MM", "Cs_opt2")})
+    assert relerr(T(res[1]["L22t"])[..., :o.nx], res[0]["L11t"][..., :o.nx]) < 1e-12
+    for k in ("F_MM", "Cs_opt2"):
+        a, b = T(res[1][k])[..., :o.nx], res[0][k][..., :o.nx]
+        assert (np.abs(a - b) > 1e-7 * np.abs(b)).mean() < 0.02, k
+
+
+def test_lagrange_single_mode_leonard_stress(O):
+    """u = A sin(m x) with m inside the 2-Delta cutoff but 2m outside:
+    filt(u^2) = A^2/2, so L11 = A^2 cos(2 m x)/2 exactly (lagrange_Sdep.f90:145-173)."""
+    o = O.Oracle(32, 32, 4, dt=1e-3)
+    X, Y = grid_xy(o)
+    A, m = 0.7, 5          # 2-Delta cutoff kc = 8 on this grid: 5 < 8 < 10
+    u = put(o, [A * np.sin(m * X)] * o.nz)
+    v, w = o.field(), o.field()
+    S = [o.field() for _ in range(6)]
+    st, arr = _sgs(o, 12)
+    o.set_time(5, 600, flm_init=True, fqn_init=True)
+    o.lagrange_sdep(S, u, v, w, st)
+    ex = 0.5 * A * A * np.cos(2 * m * X)
+    for k in range(1, o.nz):
+        assert np.max(np.abs(arr["L11t"][k, :, :o.nx] - ex)) < 1e-13
+    # 4-Delta cutoff kc = 4 < 5: the mode is filtered away, Q11 = filt(u^2) = A^2/2
+    for k in range(1, o.nz):
+        assert np.max(np.abs(arr["Q11t"][k, :, :o.nx] - 0.5 * A * A)) < 1e-13
+
+
+def test_lagrange_relaxation_without_resolved_stress(O):
+    """S = 0 and u = const: L = M = Q = N = 0, so each F relaxes by (1 - eps)
+    with eps = (dt cs/T)/(1 + dt cs/T), T = 1.5 Delta (F_LM F_MM)^(-1/8)
+    (Meneveau-Lund-Cabot time scale), with the float rounding points."""
+    o = O.Oracle(8, 8, 4, dt=1e-3)
+    st, arr = _sgs(o, 13)
+    FLM, FMM = arr["F_LM"].copy(), arr["F_MM"].copy()
+    u = np.full((o.nz, o.ny, o.ld), 2.0)
+    z = o.field()
+    S = [o.field() for _ in range(6)]
+    o.set_time(5, 600, flm_init=True, fqn_init=True)
+    o.lagrange_sdep(S, u, z, z, st)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    delta = math.sqrt(o.dx * o.dy)
+    Tn = np.maximum(f32(FLM * FMM), float(np.float32(1e-24)))
+    T = np.maximum(float(np.float32(1e-24)), f32(1.5 * delta * Tn ** (-1.0 / 8)))
+    eps = (5e-3 / T) / (1 + 5e-3 / T)
+    exLM = np.maximum(float(np.float32(1e-24)), f32((1 - eps) * FLM))
+    exMM = (1 - eps) * FMM
+    sl = (slice(None), slice(None), slice(0, o.nx))
+    assert np.allclose(arr["F_LM"][sl], exLM[sl], rtol=1e-15)
+    assert np.allclose(arr["F_MM"][sl], exMM[sl], rtol=1e-15)
+    assert np.allclose(arr["TnLMMM"][sl], Tn[sl], rtol=0)
+    cs2 = np.maximum(float(np.float32(1e-24)), f32(exLM / exMM))
+    assert np.all(arr["Beta"][-1] == 1.0)
+    assert np.all(arr["Cs_opt2"][:, :, o.nx:] == float(np.float32(1e-24)))
+    beta = arr["Beta"][:-1][..., :o.nx]
+    expect = cs2[:-1][..., :o.nx] / np.maximum(f32(beta), 0.125)
+    assert np.allclose(arr["Cs_opt2"][:-1][..., :o.nx], np.maximum(1e-24, f32(expect)), rtol=1e-7)
+
+
+def test_lagrange_inilag_initialisation(O):
+    """At jt == cs_count with inilag: F_MM = MM, F_LM = 0.03 MM before the
+    first relaxation, pads 1 (lagrange_Sdep.f90:287-300)."""
+    o = O.Oracle(8, 8, 4, dt=1e-3)
+    st, arr = _sgs(o, 14)
+    u = np.full((o.nz, o.ny, o.ld), 2.0)
+    z = o.field()
+    S = [o.field() for _ in range(6)]
+    o.set_time(5, 5, flm_init=False, fqn_init=False)
+    o.lagrange_sdep(S, u, z, z, st)
+    # MM = 0 -> F_MM = 0, F_LM = max(1e-24, 0) ; pads 1
+    assert np.all(arr["F_MM"][..., :o.nx] == 0.0)
+    assert np.all(arr["F_LM"][..., :o.nx] == float(np.float32(1e-24)))
+    assert np.all(arr["F_MM"][..., o.nx:] != 0.0)
+
+
+# ---------------------------------------------------------------- pressure
+def test_tridag_matches_dense_solve(O):
+    o = O.Oracle(8, 8, 6)
+    n, lh, ny = o.nz + 1, o.lh, o.ny
+    rng = np.random.default_rng(7)
+    a = rng.random((n, ny, lh)) + 0.2
+    c = rng.random((n, ny, lh)) + 0.2
+    b = -(a + c) - 0.5 - rng.random((n, ny, lh))
+    r = rng.standard_normal((n, ny, 2 * lh))
+    u = np.zeros_like(r)
+    assert o.tridag_array(a, b, c, r, u) == 0
+    rc = r.view(np.complex128)
+    uc = u.view(np.complex128)
+    for y in range(ny):
+        for x in range(lh):
+            if y == ny // 2 or x == lh - 1 or (x == 0 and y == 0):
+                assert np.allclose(uc[0, y, x], rc[0, y, x] / b[0, y, x])
+                continue
+            M = np.diag(b[:, y, x]) + np.diag(a[1:, y, x], -1) + np.diag(c[:-1, y, x], 1)
+            sol = np.linalg.solve(M, rc[:, y, x])
+            assert np.allclose(uc[:, y, x], sol, rtol=1e-12, atol=1e-12)
+    b0 = b.copy()
+    b0[0, 1, 1] = 0.0
+    assert o.tridag_array(a, b0, c, r, u) != 0     # zero pivot reported
+
+
+def test_pressure_projection_makes_velocity_divergence_free(O):
+    """press_stag_array + dpdz + project: the spectral-x/y + ddz_w divergence
+    of the projected velocity vanishes on interior levels (the defining
+    property of the fractional step, press_stag_array.f90:192-210)."""
+    o = O.Oracle(16, 24, 9, Lx=2 * math.pi, Ly=3 * math.pi, dt=1e-3)
+    rng = np.random.default_rng(8)
+    u, v, w = (o.field() for _ in range(3))
+    for a in (u, v, w):
+        a[:, :, :o.nx] = rng.standard_normal((o.nz, o.ny, o.nx))
+        f, d1, d2 = a.copy(), o.field(), o.field()
+        o.filt_da(f, d1, d2)       # remove Nyquist content, like the step does
+        a[...] = f
+    w[0] = 0.0
+    divtz = o.field()
+    p, px, py = o.field(o.nz + 1), o.field(), o.field()
+    o.press_stag_array(u, v, w, divtz, p, px, py)
+    pz = o.field()
+    pz[:-1, :, :o.nx] = (p[1:o.nz, :, :o.nx] - p[0:o.nz - 1, :, :o.nx]) / o.dz
+    o.project(u, v, w, px, py, pz)
+    ux, vy, wz = o.field(), o.field(), o.field()
+    o.ddx(ux, u)
+    o.ddy(vy, v)
+    o.ddz_w(wz, w)
+    div = (ux + vy + wz)[1:o.nz - 2, :, :o.nx]
+    scale = np.max(np.abs(ux[1:o.nz - 2, :, :o.nx]))
+    assert np.max(np.abs(div)) < 1e-11 * scale
+    # the projection did change u (the test is not vacuous)
+    assert np.max(np.abs(px[:-1, :, :o.nx])) > 1e-3
+
+
+def test_project_boundary_rules(O):
+    o = O.Oracle(8, 8, 5, dt=0.1)
+    u, v, w = (np.full((o.nz, o.ny, o.ld), 1.0) for _ in range(3))
+    px, py, pz = (np.full((o.nz, o.ny, o.ld), 2.0) for _ in range(3))
+    o.project(u, v, w, px, py, pz)
+    assert np.allclose(u[:-1, :, :o.nx], 1.0 - 0.1 * 1.5 * 2.0)
+    assert np.array_equal(u[-1], u[-2]) and np.all(w[-1] == 0) and np.all(w[0] == 0)
+    assert np.allclose(w[1:-1, :, :o.nx], 1.0 - 0.3)
+
+
+def test_rmsdiv_and_timestep_closed_form(O):
+    o = O.Oracle(8, 8, 5, dt=0.01)
+    a = np.full((o.nz, o.ny, o.ld), 1.0)
+    b = np.full_like(a, -3.0)
+    c = np.full_like(a, 0.5)
+    assert math.isclose(o.rmsdiv(a, b, c), 1.5, rel_tol=1e-15)
+    out = o.timestep_conditions(a, b, c, np.full_like(a, 2.0))
+    assert math.isclose(out[0], math.sqrt(1 + 9 + 0.25) * 0.01 / o.dx, rel_tol=1e-14)
+    assert math.isclose(out[3], 0.01 * 2.0 / min(o.dx, o.dy) ** 2, rel_tol=1e-14)
+
+
+# --------------------------------------------------------------- whole step
+def test_step_translation_invariance(O):
+    """Rolling the initial condition by whole cells in x and y rolls the state
+    after 6 steps (incl. one dynamic update) -- pins every x/y index."""
+    nx, ny, nz = 16, 16, 9
+    sims = []
+    for shift in ((0, 0), (3, 5)):
+        o = O.Oracle(nx, ny, nz, dt=2e-4)
+        s = O.OracleSim(o, seed=7)
+        if shift != (0, 0):
+            for name in ("u", "v", "w"):
+                a = s[name]
+                a[..., :nx] = np.roll(a[..., :nx], shift, axis=(1, 2))
+        s.step(6)
+        sims.append((o, s))
+    for name in ("u", "v", "w", "p", "F_LM", "F_MM", "Cs_opt2"):
+        a = sims[0][1][name][:-1, :, :nx]          # top level holds BOGUS patterns
+        b = np.roll(sims[1][1][name][:-1, :, :nx], (-3, -5), axis=(1, 2))
+        if name in ("F_LM", "Cs_opt2"):     # FP32 rounding points: isolated float-ulp flips
+            bad = np.abs(a - b) > 1e-10 * np.max(np.abs(a)) + 2e-7 * np.abs(a)
+            assert bad.mean() < 1e-2, name
+        else:
+            assert relerr(b, a) < 1e-10, name
+
+
+def _nyquist_free(o, a):
+    f, d1, d2 = a.copy(), o.field(a.shape[0]), o.field(a.shape[0])
+    o.filt_da(f, d1, d2)
+    return f
+
+
+def test_step_velocity_divergence_free_and_finite(O):
+    """After a step the projected velocity has zero discrete divergence on
+    interior levels once the kx=nx/2 / ky=ny/2 modes are removed (the
+    pressure solve leaves those modes alone; divstress_w's physical-space
+    d(tzz)/dz feeds them into w*)."""
+    o = O.Oracle(16, 16, 9, dt=2e-4)
+    s = O.OracleSim(o, seed=3)
+    s.step(10)
+    u, v, w = s["u"], s["v"], s["w"]
+    assert np.all(np.isfinite(u[:, :, :16]))
+    ux, vy, wz = o.field(), o.field(), o.field()
+    o.ddx(ux, u)
+    o.ddy(vy, v)
+    o.ddz_w(wz, w)
+    div = _nyquist_free(o, ux + vy + wz)[1:o.nz - 1, :, :o.nx]
+    assert np.max(np.abs(div)) < 1e-10 * np.max(np.abs(ux[1:o.nz - 1, :, :o.nx]))
