@@ -49,6 +49,7 @@ int les_create(const les_grid *g, les_ctx **out);
 void les_destroy(les_ctx *c);
 const char *les_last_error(les_ctx *c);
 int les_set_stream(les_ctx *c, void *hip_stream);
+int les_get_grid(les_ctx *c, les_grid *out);
 int les_sync(les_ctx *c);
 int les_set_time(les_ctx *c, int jt, int jt_total);        /* param.f90:147-148 */
 int les_get_time(les_ctx *c, int *jt, int *jt_total);
@@ -151,6 +152,18 @@ int les_state_init_ic(les_ctx *c, uint64_t seed, double amp, double ustar, doubl
 int les_step(les_ctx *c, int nsteps);
 /* rmsdiv of the state's current derivatives and timestep_conditions */
 int les_state_diag(les_ctx *c, double *rms, double out4[4]);
+
+/* Device memory and host mirrors (no reference counterpart): the Fortran
+   drop-in shims keep one device mirror per host array, keyed by its address,
+   and push inputs / pull outputs around each routine-level call. */
+int les_dev_alloc(les_ctx *c, uint64_t bytes, void **ptr);
+int les_dev_free(les_ctx *c, void *ptr);
+int les_memcpy_h2d(les_ctx *c, void *dst, const void *src, uint64_t bytes);
+int les_memcpy_d2h(les_ctx *c, void *dst, const void *src, uint64_t bytes);
+/* device mirror of `host` (allocated on first use, grown if bytes grows) */
+void *les_mirror(les_ctx *c, const void *host, uint64_t bytes);
+int les_mirror_push(les_ctx *c, const void *host, uint64_t bytes);
+int les_mirror_pull(les_ctx *c, void *host, uint64_t bytes);
 
 /* Per-launch HIP-event timing on the context stream (no reference
    counterpart; measurement support).  les_profile_report writes a JSON object

@@ -283,6 +283,8 @@ void les_destroy(les_ctx* c) {
   free_tables(c->tf);
   for (void* p : c->arena.ptr)
     if (p) (void)hipFree(p);
+  for (auto& m : c->mirrors)
+    if (m.second.first) (void)hipFree(m.second.first);
   if (c->st) {
     for (void* p : c->st->owned) (void)hipFree(p);
     delete c->st;
@@ -296,6 +298,16 @@ const char* les_last_error(les_ctx* c) { return c ? c->err.c_str() : "null conte
 int les_set_stream(les_ctx* c, void* s) {
   GUARD(c);
   c->stream = (hipStream_t)s;
+  DONE(c);
+}
+int les_get_grid(les_ctx* c, les_grid* o) {
+  GUARD(c);
+  if (!o) return -1;
+  const Grid& g = c->g;
+  o->nx = g.nx; o->ny = g.ny; o->nz = g.nz;
+  o->Lx = g.Lx; o->Ly = g.Ly; o->Lz = g.Lz;
+  o->dt = g.dt; o->z0 = g.z0; o->force = g.force;
+  o->inilag = g.inilag; o->initu = g.initu; o->cs_count = g.cs_count;
   DONE(c);
 }
 int les_sync(les_ctx* c) {
@@ -678,6 +690,65 @@ int les_step(les_ctx* c, int nsteps) {
       step_once<float>(c);
     else
       step_once<double>(c);
+  }
+  DONE(c);
+}
+
+int les_dev_alloc(les_ctx* c, uint64_t bytes, void** ptr) {
+  GUARD(c);
+  *ptr = nullptr;
+  check(c, hipMalloc(ptr, bytes ? bytes : 16), "les_dev_alloc");
+  if (*ptr) check(c, hipMemset(*ptr, 0, bytes ? bytes : 16), "les_dev_alloc zero");
+  DONE(c);
+}
+int les_dev_free(les_ctx* c, void* ptr) {
+  GUARD(c);
+  check(c, hipStreamSynchronize(c->stream), "les_dev_free sync");
+  if (ptr) check(c, hipFree(ptr), "les_dev_free");
+  DONE(c);
+}
+int les_memcpy_h2d(les_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  GUARD(c);
+  check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "h2d");
+  check(c, hipStreamSynchronize(c->stream), "h2d sync");
+  DONE(c);
+}
+int les_memcpy_d2h(les_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  GUARD(c);
+  check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "d2h");
+  check(c, hipStreamSynchronize(c->stream), "d2h sync");
+  DONE(c);
+}
+void* les_mirror(les_ctx* c, const void* host, uint64_t bytes) {
+  if (!c || !host) return nullptr;
+  for (auto& m : c->mirrors)
+    if (m.first == host) {
+      if (m.second.second >= bytes) return m.second.first;
+      check(c, hipStreamSynchronize(c->stream), "mirror sync");
+      (void)hipFree(m.second.first);
+      m.second.first = nullptr;
+      check(c, hipMalloc(&m.second.first, bytes), "mirror grow");
+      m.second.second = m.second.first ? bytes : 0;
+      return m.second.first;
+    }
+  void* d = nullptr;
+  check(c, hipMalloc(&d, bytes ? bytes : 16), "mirror alloc");
+  if (!d) return nullptr;
+  c->mirrors.push_back({host, {d, (size_t)bytes}});
+  return d;
+}
+int les_mirror_push(les_ctx* c, const void* host, uint64_t bytes) {
+  GUARD(c);
+  void* d = les_mirror(c, host, bytes);
+  if (d) check(c, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c->stream), "mirror push");
+  DONE(c);
+}
+int les_mirror_pull(les_ctx* c, void* host, uint64_t bytes) {
+  GUARD(c);
+  void* d = les_mirror(c, host, bytes);
+  if (d) {
+    check(c, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream), "mirror pull");
+    check(c, hipStreamSynchronize(c->stream), "mirror pull sync");
   }
   DONE(c);
 }

@@ -69,6 +69,8 @@ struct les_ctx {
   bool prof = false;
   std::vector<les::ProfRec> prof_recs;
   std::string op_tag;
+  // host-address -> device mirror registry (les_mirror)
+  std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };
 
 namespace les {

@@ -1016,11 +1016,12 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
                                T* Q33t, double* red, int nx, int ld, int ny, int nz, double cst, double opftdelta,
                                double lagran_dt, double expo, int init_lm, int init_qn) {
   const long pl = (long)ld * ny, n3 = pl * nz;
-  const long n = (long)nx * ny * nz;
+  const int n2 = nx * ny;
   const double e24 = (double)1e-24f;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(q / ((long)nx * ny));
-    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+  const int k = blockIdx.y;  // one level per grid row: plane sums reduce per block
+  double acc2 = 0.0, acc4 = 0.0, accc = 0.0;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n2; r += gridDim.x * blockDim.x) {
+    const int y = r / nx, x = r - y * nx;
     const long i = (long)k * pl + (long)y * ld + x;
     auto f2 = [&](int f) { return (double)F2[f * n3 + i]; };
     auto f4 = [&](int f) { return (double)F4[f * n3 + i]; };
@@ -1072,17 +1073,28 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
     fnn = (epsi * NN + (1.0 - epsi) * fnn);
     fqn = fmax(e24, fq(fqn));
     const double cs4d = fmax(e24, fq(fqn / fnn));
-    double beta = pow(cs4d / cs2d, expo);
+    double beta = expo == 1.0 ? cs4d / cs2d : pow(cs4d / cs2d, expo);  // exponent ln2/(ln4-ln2) is 1
     if (k == nz - 1) beta = 1.0;
     const double bclip = fmax(fq(beta), fq(0.125));
     const double cso = fmax(e24, fq(cs2d / bclip));
     FLM[i] = (T)flm; FMM[i] = (T)fmm; FQN[i] = (T)fqn; FNN[i] = (T)fnn;
     Beta[i] = (T)beta;
     Cs[i] = (T)cso;
-    atomicAdd(red + 3 * k + 0, cs2d);
-    atomicAdd(red + 3 * k + 1, cs4d);
-    atomicAdd(red + 3 * k + 2, cso);
+    acc2 += cs2d;
+    acc4 += cs4d;
+    accc += cso;
   }
+  __shared__ double sh[3][256];
+  sh[0][threadIdx.x] = acc2;
+  sh[1][threadIdx.x] = acc4;
+  sh[2][threadIdx.x] = accc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) atomicAdd(red + 3 * k + threadIdx.x, sh[threadIdx.x][0]);
 }
 
 // pads of the lagrange outputs: lagrange_Sdep.f90:295-296,344-345,353-354,383-384,404,419-420
@@ -1144,7 +1156,8 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   const double delta = g.delta;
   {
     ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
-    hipLaunchKernelGGL((k_lag_epilogue<T>), grid1((size_t)g.nx * g.ny * nz), dim3(256), 0, c->stream, F, F + 21 * n3,
+    const unsigned bpl = (unsigned)((g.nx * g.ny + 255) / 256 < 64 ? (g.nx * g.ny + 255) / 256 : 64);
+    hipLaunchKernelGGL((k_lag_epilogue<T>), dim3(bpl, nz), dim3(256), 0, c->stream, F, F + 21 * n3,
                        S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
                        (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
                        (T*)s.Q11t, (T*)s.Q22t, (T*)s.Q33t, red, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
