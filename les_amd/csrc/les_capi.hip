@@ -200,9 +200,7 @@ static void step_once(les_ctx* c) {
               F<T>(c, "dvdz"), F<T>(c, "dwdx"), F<T>(c, "dwdy"), F<T>(c, "dwdz")};
   T* tau[6] = {F<T>(c, "txx"), F<T>(c, "txy"), F<T>(c, "txz"), F<T>(c, "tyy"), F<T>(c, "tyz"), F<T>(c, "tzz")};
   // obukhov (neutral): psi = 0, phi = 1 are the state's constant contents
-  op_filt_da<T>(c, u, gr[0], gr[1], g.nz);
-  op_filt_da<T>(c, v, gr[3], gr[4], g.nz);
-  op_filt_da<T>(c, w, gr[6], gr[7], g.nz);
+  op_filt_da3<T>(c, u, v, w, gr, g.nz);
   op_ddz_uv<T>(c, gr[2], u);
   op_ddz_uv<T>(c, gr[5], v);
   op_ddz_w<T>(c, gr[8], w);
@@ -216,9 +214,9 @@ static void step_once(les_ctx* c) {
   op_divstress<T>(c, dy, tau[1], tau[3], tau[4], 0);
   op_divstress<T>(c, dz, tau[2], tau[4], tau[5], 1);
   T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
-  op_convec<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz);
-  op_ab2<T>(c, u, v, w, Rx, Ry, Rz, F<T>(c, "RHSx_f"), F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz,
-            (c->jt == 1 && !g.initu) ? 1 : 0);
+  // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
+  op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
+                   F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, (c->jt == 1 && !g.initu) ? 1 : 0);
   T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
   op_press_stag<T>(c, u, v, w, dz, p, px, py);
   op_pressure_update<T>(c, u, v, w, Rx, Ry, Rz, p, px, py, pz);

@@ -131,3 +131,85 @@ __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* src, int 
 }
 
 }  // namespace les
+
+namespace les {
+
+// ---- compile-time-length variant: every index is a constant expression ----
+constexpr int fft_nstages(int n) {
+  int s = 0;
+  while (n % 4 == 0) { n /= 4; ++s; }
+  while (n % 2 == 0) { n /= 2; ++s; }
+  while (n % 3 == 0) { n /= 3; ++s; }
+  return n == 1 ? s : -1;
+}
+constexpr int fft_radix(int n, int s) {
+  int i = 0;
+  while (n % 4 == 0) { if (i == s) return 4; n /= 4; ++i; }
+  while (n % 2 == 0) { if (i == s) return 2; n /= 2; ++i; }
+  while (n % 3 == 0) { if (i == s) return 3; n /= 3; ++i; }
+  return 1;
+}
+
+// One Stockham stage of NTR transforms of length N in LDS (element e of
+// transform t at buf[t*TS + e*ES]) executed by BLOCK threads.
+template <typename T, int N, int R, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+__device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sign) {
+  constexpr int NB = N / R;
+  constexpr int ITEMS = NTR * NB;
+  constexpr int TWS = N / (NS * R);
+  constexpr int MAXB = (ITEMS + BLOCK - 1) / BLOCK;
+  cx<T> v[MAXB][R];
+  int dd[MAXB], tt[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int w = threadIdx.x + i * BLOCK;
+    tt[i] = -1;
+    dd[i] = 0;
+    if (ITEMS % BLOCK == 0 || w < ITEMS) {
+      const int t = LANE_T ? (w % NTR) : (w / NB);
+      const int b = LANE_T ? (w / NTR) : (w % NB);
+      const int k = b % NS;
+      cx<T>* base = buf + t * TS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> a = base[(b + r * NB) * ES];
+        if (r > 0 && NS > 1 && k > 0) {
+          cx<T> wv = tw[r * k * TWS];
+          if (sign > 0) wv.y = -wv.y;
+          a = cmul(a, wv);
+        }
+        v[i][r] = a;
+      }
+      butterfly<T, R>(v[i], sign);
+      tt[i] = t;
+      dd[i] = (b / NS) * NS * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    if (tt[i] >= 0) {
+      cx<T>* base = buf + tt[i] * TS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[(dd[i] + r * NS) * ES] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+__device__ __forceinline__ void fft_stages_c(cx<T>* buf, const cx<T>* tw, int sign) {
+  if constexpr (S < fft_nstages(N)) {
+    constexpr int R = fft_radix(N, S);
+    fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
+    fft_stages_c<T, N, S + 1, NS * R, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
+  }
+}
+
+template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+__device__ __forceinline__ void fft_lds_c(cx<T>* buf, const cx<T>* tw, int sign) {
+  static_assert(fft_nstages(N) > 0, "FFT length must be 2^a 3^b");
+  fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
+}
+
+}  // namespace les

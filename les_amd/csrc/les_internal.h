@@ -87,6 +87,7 @@ template <typename T> const Tables<T>& tables(const les_ctx* c);
 // ---- launchers (les_kernels.hip); all enqueue on c->stream ----
 // horizontal spectral operators, fields (ld,ny,nlev)
 template <typename T> void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev);
+template <typename T> void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev);
 template <typename T> void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev);  // null outputs skipped
 template <typename T> void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep);
 template <typename T> void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev);
@@ -99,6 +100,10 @@ template <typename T> void op_divstress(les_ctx* c, T* divt, const T* tx, const 
 template <typename T>
 void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,
                const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy, T* cx, T* cy, T* cz);
+template <typename T>
+void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
+                   const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
+                   const T* divtx, const T* divty, const T* divtz, int euler_first);
 template <typename T>
 void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz, T* tyz,
                    const double* psi_m, const double* psi_m0, const double* phi_m, double* ustar_avg);

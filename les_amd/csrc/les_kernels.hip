@@ -55,53 +55,76 @@ static const FftPlan& plan_for(const les_ctx* c, int n) {
 // ------------------------------------------------------------------------
 // row transforms
 // ------------------------------------------------------------------------
-template <typename T, class Loader>
-__global__ void __launch_bounds__(1024) k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int npairs,
-                                                  FftPlan p, const cx<T>* __restrict__ twg, int kxo) {
+// Rows are grouped by plane (field x level): blockIdx.y = plane,
+// blockIdx.x = group of P row pairs inside it, so loaders and storers get a
+// block-uniform plane index (no per-element integer division).  N > 0 is the
+// compile-time transform length; N == 0 runs the same code with run-time n.
+template <int N>
+struct RowGeom {
+  static constexpr int TPF = N > 0 ? N / 4 : 64;
+  static constexpr int P = (256 / TPF) > 0 ? 256 / TPF : 1;
+  static constexpr int BLOCK = P * TPF;
+};
+
+template <typename T, int N, class Loader>
+__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+    k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
+              int kxo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  const int n = p.n;
+  const int n = N > 0 ? N : p.n;
+  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
   cx<T>* buf = tw + n;
-  const int P = (blockDim.x * 4) / n;
-  load_twiddles(tw, twg, n);
+  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  const int plane = blockIdx.y;
+  const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
-    const int t = e / n, x = e - t * n, pg = pair0 + t;
+  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+    const int t = e / n, x = e - t * n, pp = pair0 + t;
     cx<T> z = {T(0), T(0)};
-    if (pg < npairs) {
-      z.x = L(2 * pg, x);
-      z.y = L(2 * pg + 1, x);
+    if (pp < npp) {
+      z.x = L(plane, 2 * pp, x);
+      z.y = L(plane, 2 * pp + 1, x);
     }
     buf[e] = z;
   }
   __syncthreads();
-  fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
-  for (int e = threadIdx.x; e < P * kxo; e += blockDim.x) {
-    const int t = e / kxo, k = e - t * kxo, pg = pair0 + t;
-    if (pg >= npairs) continue;
+  if constexpr (N > 0)
+    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, -1);
+  else
+    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
+  for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
+    const int t = e / kxo, k = e - t * kxo, pp = pair0 + t;
+    if (pp >= npp) continue;
     const cx<T> zk = buf[t * n + k], zn = buf[t * n + (k ? n - k : 0)];
     const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
     const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
-    out[(long)(2 * pg) * orow + k] = A;
-    out[(long)(2 * pg + 1) * orow + k] = B;
+    const long row = (long)plane * ny + 2 * pp;
+    out[row * orow + k] = A;
+    out[(row + 1) * orow + k] = B;
   }
 }
 
-template <typename T, class Storer>
-__global__ void __launch_bounds__(1024) k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S,
-                                                  int npairs, FftPlan p, const cx<T>* __restrict__ twg, int ldo) {
+template <typename T, int N, class Storer>
+__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+    k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
+              const cx<T>* __restrict__ twg, int ldo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  const int n = p.n;
+  const int n = N > 0 ? N : p.n;
+  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
   cx<T>* buf = tw + n;
-  const int P = (blockDim.x * 4) / n;
-  load_twiddles(tw, twg, n);
+  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  const int plane = blockIdx.y;
+  const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
-    const int t = e / n, j = e - t * n, pg = pair0 + t;
+  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+    const int t = e / n, j = e - t * n, pp = pair0 + t;
     cx<T> z = {T(0), T(0)};
-    if (pg < npairs) {
-      const cx<T>* a = in + (long)(2 * pg) * irow;
+    if (pp < npp) {
+      const cx<T>* a = in + ((long)plane * ny + 2 * pp) * irow;
       const cx<T>* b = a + irow;
       if (j == 0) {
         z = {a[0].x, b[0].x};  // Im of the DC entries is ignored (halfcomplex c2r)
@@ -116,76 +139,110 @@ __global__ void __launch_bounds__(1024) k_row_inv(const cx<T>* __restrict__ in, 
     buf[e] = z;
   }
   __syncthreads();
-  fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
-  for (int e = threadIdx.x; e < P * ldo; e += blockDim.x) {
-    const int t = e / ldo, x = e - t * ldo, pg = pair0 + t;
-    if (pg >= npairs) continue;
+  if constexpr (N > 0)
+    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, +1);
+  else
+    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
+  for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
+    const int t = e / ldo, x = e - t * ldo, pp = pair0 + t;
+    if (pp >= npp) continue;
     T v0 = T(0), v1 = T(0);
     if (x < n) {
       v0 = buf[t * n + x].x;
       v1 = buf[t * n + x].y;
     }
-    S(2 * pg, x, v0);
-    S(2 * pg + 1, x, v1);
+    S(plane, 2 * pp, x, v0);
+    S(plane, 2 * pp + 1, x, v1);
   }
 }
 
-template <typename T, class L>
-static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo) {
+// compile-time lengths with specialised kernels; anything else runs the
+// run-time-length kernels
+#define LES_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384)
+
+template <typename T, int N, class L>
+static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo) {
   const FftPlan& p = plan_for(c, n);
   const int tpf = n / 4;
   int P = 256 / tpf;
   if (P < 1) P = 1;
   const int threads = P * tpf;
-  const long npairs = nrows / 2;
-  const long blocks = (npairs + P - 1) / P;
+  const long blocks = (ny / 2 + P - 1) / P;
   const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
-  ProfScope ps(c, "row_fwd", (double)nrows * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
-  hipLaunchKernelGGL((k_row_fwd<T, L>), dim3((unsigned)blocks), dim3(threads), sm, c->stream, ld, out, orow,
-                     (int)npairs, p, tw_for<T>(c, n), kxo);
+  ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
+  hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
+                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo);
+  LCHK(c, hipGetLastError());
+}
+
+template <typename T, class L>
+static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo, int ny) {
+  const long nplanes = nrows / ny;
+  switch (n) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_row_fwd_n<T, NN>(c, ld, out, orow, nplanes, ny, n, kxo);
+    LES_FFT_SIZES(LES_CASE)
+#undef LES_CASE
+    default:
+      return launch_row_fwd_n<T, 0>(c, ld, out, orow, nplanes, ny, n, kxo);
+  }
+}
+
+template <typename T, int N, class S>
+static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
+                             int n, int ldo) {
+  const FftPlan& p = plan_for(c, n);
+  const int tpf = n / 4;
+  int P = 256 / tpf;
+  if (P < 1) P = 1;
+  const int threads = P * tpf;
+  const long blocks = (ny / 2 + P - 1) / P;
+  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  ProfScope ps(c, "row_inv", (double)nplanes * ny * (kxi * sizeof(cx<T>) + ldo * sizeof(T) * st.writes()));
+  hipLaunchKernelGGL((k_row_inv<T, N, S>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
+                     in, irow, kxi, st, ny, p, tw_for<T>(c, n), ldo);
   LCHK(c, hipGetLastError());
 }
 
 template <typename T, class S>
 static void launch_row_inv(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nrows, int n,
-                           int ldo) {
-  const FftPlan& p = plan_for(c, n);
-  const int tpf = n / 4;
-  int P = 256 / tpf;
-  if (P < 1) P = 1;
-  const int threads = P * tpf;
-  const long npairs = nrows / 2;
-  const long blocks = (npairs + P - 1) / P;
-  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
-  ProfScope ps(c, "row_inv", (double)nrows * (kxi * sizeof(cx<T>) + ldo * sizeof(T) * st.writes()));
-  hipLaunchKernelGGL((k_row_inv<T, S>), dim3((unsigned)blocks), dim3(threads), sm, c->stream, in, irow, kxi, st,
-                     (int)npairs, p, tw_for<T>(c, n), ldo);
-  LCHK(c, hipGetLastError());
+                           int ldo, int ny) {
+  const long nplanes = nrows / ny;
+  switch (n) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_row_inv_n<T, NN>(c, in, irow, kxi, st, nplanes, ny, n, ldo);
+    LES_FFT_SIZES(LES_CASE)
+#undef LES_CASE
+    default:
+      return launch_row_inv_n<T, 0>(c, in, irow, kxi, st, nplanes, ny, n, ldo);
+  }
 }
 
 // loaders / storers ------------------------------------------------------
 // row index = (field * nlev + level) * ny + y; fields are (ld, ny, nlev).
+// plane = field * planes_per_field + level; fields are (ld, ny, levels)
 template <typename T>
 struct LoadFields {
   const T* p[4];
-  int rows_per_field;
-  int ld;
+  int ppf, ny, ld;
   T scale;
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int row, int x) const {
-    const int f = row / rows_per_field, r = row - f * rows_per_field;
-    return scale * p[f][(long)r * ld + x];
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    return scale * p[f][((long)r * ny + y) * ld + x];
   }
 };
 template <typename T>
 struct StoreFields {
-  T* p[4];
-  int rows_per_field;
+  T* p[12];
+  int ppf, ny;
   int ld, nx;  // pads (x >= nx) are written 0
   static constexpr double writes() { return 1.0; }
-  __device__ void operator()(int row, int x, T v) const {
-    const int f = row / rows_per_field, r = row - f * rows_per_field;
-    p[f][(long)r * ld + x] = x < nx ? v : T(0);
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    p[f][((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
   }
 };
 
@@ -217,44 +274,68 @@ __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
   return yo + nyi - nyo;
 }
 
-template <typename T, int TX>
-__global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2,
-                                             cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a, FftPlan pin,
-                                             FftPlan pout, const cx<T>* __restrict__ twin,
-                                             const cx<T>* __restrict__ twout, const T* __restrict__ kxt,
-                                             const T* __restrict__ kyt, const T* __restrict__ G0,
-                                             const T* __restrict__ G1, const T* __restrict__ G2) {
+template <int TX, int NYI, int NYO>
+struct ColGeom {
+  static constexpr int NMAX = NYI > NYO ? NYI : NYO;
+  static constexpr int B0 = (TX * NMAX / 8 + 63) / 64 * 64;
+  static constexpr int BLOCK = B0 < 64 ? 64 : B0;
+};
+
+// NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
+template <typename T, int TX, int NYI, int NYO>
+__global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
+    k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
+          FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
+          const T* __restrict__ kxt, const T* __restrict__ kyt, const T* __restrict__ G0, const T* __restrict__ G1,
+          const T* __restrict__ G2) {
+  constexpr bool CT = NYI > 0;
+  constexpr int BLK = CT ? ColGeom<TX, NYI, NYO>::BLOCK : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nyi = CT ? NYI : a.ny_in;
+  const int nyo = CT ? NYO : a.ny_out;
+  const int bd = CT ? BLK : (int)blockDim.x;
   cx<T>* twi = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* two = twi + a.ny_in;
-  cx<T>* A = two + a.ny_out;
-  cx<T>* B = A + a.ny_in * TX;
-  load_twiddles(twi, twin, a.ny_in);
-  load_twiddles(two, twout, a.ny_out);
+  cx<T>* two = twi + nyi;
+  cx<T>* A = two + nyo;
+  cx<T>* B = A + nyi * TX;
+  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = twin[i];
+  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = twout[i];
+  auto fwd = [&](cx<T>* buf) {
+    if constexpr (CT)
+      fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, twi, -1);
+    else
+      fft_lds<T, true, 4>(buf, pin, TX, 1, TX, twi, -1);
+  };
+  auto inv = [&](cx<T>* buf) {
+    if constexpr (CT)
+      fft_lds_c<T, (CT ? NYO : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, two, +1);
+    else
+      fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
+  };
   const int kx0 = blockIdx.x * TX;
   const long pl = blockIdx.y;
   {
     const cx<T>* src = in + pl * a.in_plane;
-    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       A[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
-  if (a.do_fwd) fft_lds<T, true, 4>(A, pin, TX, 1, TX, twi, -1);
+  if (a.do_fwd) fwd(A);
   if (in2) {
     // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
     const cx<T>* src = in2 + pl * a.in2_plane;
-    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       B[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    fft_lds<T, true, 4>(B, pin, TX, 1, TX, twi, -1);
-    for (int e = threadIdx.x; e < a.ny_in * TX; e += blockDim.x) {
+    fwd(B);
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       cx<T> r = {T(0), T(0)};
-      if (y != a.ny_in / 2 && kk < a.nki) {
+      if (y != nyi / 2 && kk < a.nki) {
         const T kx = kxt[kk], ky = kyt[y];
         const cx<T> X = A[e], Y = B[e];
         r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
@@ -262,8 +343,8 @@ __global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, cons
       A[e] = r;
     }
     __syncthreads();
-    if (a.do_inv) fft_lds<T, true, 4>(A, pout, TX, 1, TX, two, +1);
-    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+    if (a.do_inv) inv(A);
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = A[e];
     }
@@ -273,9 +354,9 @@ __global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, cons
     if (kx0 >= a.nko[j]) continue;  // uniform over the block
     const int kind = a.kind[j];
     const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
-    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
       const int yo = e / TX, c = e - yo * TX, kk = kx0 + c;
-      const int yi = map_row(yo, a.ny_in, a.ny_out);
+      const int yi = map_row(yo, nyi, nyo);
       cx<T> v = {T(0), T(0)};
       if (yi >= 0 && kk < a.nki) {
         v = A[yi * TX + c];
@@ -293,9 +374,9 @@ __global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, cons
       B[e] = v;
     }
     __syncthreads();
-    if (a.do_inv) fft_lds<T, true, 4>(B, pout, TX, 1, TX, two, +1);
+    if (a.do_inv) inv(B);
     cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
-    for (int e = threadIdx.x; e < a.ny_out * TX; e += blockDim.x) {
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = B[e];
     }
@@ -303,21 +384,22 @@ __global__ void __launch_bounds__(1024) k_col(const cx<T>* __restrict__ in, cons
   }
 }
 
-template <typename T>
-static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
-                       const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
-                       const T* G2 = nullptr) {
+template <typename T, int NYI, int NYO>
+static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                         const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
   constexpr int TX = sizeof(T) == 8 ? 8 : 16;
   const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
-  // fft_lds<.., MAXB=4> needs blockDim >= TX*n/8 for its radix-2 stages
-  int threads = (TX * nmax / 8 + 63) / 64 * 64;
+  // the FFT stages need blockDim >= TX*n/8 (at most 4 butterflies per thread)
+  int threads = NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
   if (threads < 64) threads = 64;
   if (threads > 1024) {
     c->err = "k_col: column tile too large";
     return;
   }
   const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
-  if (sm > 65536) LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sm));
   const Tables<T>& t = tables<T>(c);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   double bytes = 0;
@@ -327,10 +409,26 @@ static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0,
     for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
   }
   ProfScope ps(c, "col", bytes);
-  hipLaunchKernelGGL((k_col<T, TX>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2, o0, o1, o2,
-                     a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in), tw_for<T>(c, a.ny_out),
-                     t.kx, t.ky, G0, G1, G2);
+  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
+                     o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
+                     tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
   LCHK(c, hipGetLastError());
+}
+
+#define LES_COL_PAIRS(X)                                                                                       \
+  X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
+  X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
+
+template <typename T>
+static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                       const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
+                       const T* G2 = nullptr) {
+#define LES_CASE(I, O)                                                                        \
+  if (a.ny_in == I && a.ny_out == O)                                                          \
+    return launch_col_n<T, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
+  LES_COL_PAIRS(LES_CASE)
+#undef LES_CASE
+  launch_col_n<T, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
 }
 
 static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {
@@ -402,29 +500,49 @@ static dim3 grid1(size_t n, int bs = 256) {
 // ------------------------------------------------------------------------
 // filt_da.f90 / ddx.f90 / ddy.f90 / ddxy.f90
 // ------------------------------------------------------------------------
+// nf fields (up to 3) -> for each field the outputs kinds[0..nout): one
+// row_fwd, one column pass and one row_inv over all fields.
 template <typename T>
-static void spectral_deriv(les_ctx* c, const T* f, int nlev, T* const outs[3], const int kinds[3], int nout) {
+static void spectral_deriv(les_ctx* c, const T* const* fs, int nf, int nlev, T* const* outs, const int kinds[3],
+                           int nout) {
   const Grid& g = c->g;
   const long rows = (long)nlev * g.ny;
-  const size_t xs = (size_t)rows * g.nxh * sizeof(cx<T>);
-  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, xs);
-  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs * nout);
-  LoadFields<T> L{{f, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, rows, g.nx, g.nxh);
+  const size_t fo = (size_t)nf * rows * g.nxh;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, fo * sizeof(cx<T>));
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, fo * nout * sizeof(cx<T>));
+  LoadFields<T> L{{fs[0], nf > 1 ? fs[1] : nullptr, nf > 2 ? fs[2] : nullptr, nullptr}, nlev, g.ny, g.ld,
+                  T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows * nf, g.nx, g.nxh, g.ny);
   ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
   for (int j = 0; j < nout; ++j) col_out(a, j, kinds[j], g.nxh, g.nxh, (long)g.ny * g.nxh);
-  const size_t fo = (size_t)rows * g.nxh;
-  launch_col<T>(c, X0, nullptr, X1, X1 + fo, X1 + 2 * fo, a, nlev, g.nxh);
-  StoreFields<T> S{{outs[0], outs[1], outs[2], nullptr}, (int)rows, g.ld, g.nx};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nout, g.nx, g.ld);
+  launch_col<T>(c, X0, nullptr, X1, X1 + fo, X1 + 2 * fo, a, (long)nlev * nf, g.nxh);
+  // output rows are ordered (kind j, field f): outs[j*nf + f]
+  StoreFields<T> S{};
+  for (int q = 0; q < nf * nout; ++q) S.p[q] = outs[q];
+  S.ppf = nlev;
+  S.ny = g.ny;
+  S.ld = g.ld;
+  S.nx = g.nx;
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nf * nout, g.nx, g.ld, g.ny);
 }
 
 template <typename T>
 void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev) {
   OpTag _tag(c, "filt_da");
+  const T* fs[1] = {f};
   T* outs[3] = {f, dfdx, dfdy};
   const int kinds[3] = {K_ID, K_IKX, K_IKY};
-  spectral_deriv<T>(c, f, nlev, outs, kinds, 3);
+  spectral_deriv<T>(c, fs, 1, nlev, outs, kinds, 3);
+}
+
+// filt_da on u, v, w in one pipeline (main.f90:231-233)
+template <typename T>
+void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev) {
+  OpTag _tag(c, "filt_da");
+  const T* fs[3] = {u, v, w};
+  T* outs[9] = {u, v, w, gr[0], gr[3], gr[6], gr[1], gr[4], gr[7]};
+  const int kinds[3] = {K_ID, K_IKX, K_IKY};
+  spectral_deriv<T>(c, fs, 3, nlev, outs, kinds, 3);
 }
 
 template <typename T>
@@ -435,7 +553,8 @@ void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev) {
   int n = 0;
   if (dfdx) { outs[n] = dfdx; kinds[n++] = K_IKX; }
   if (dfdy) { outs[n] = dfdy; kinds[n++] = K_IKY; }
-  if (n) spectral_deriv<T>(c, f, nlev, outs, kinds, n);
+  const T* fs[1] = {f};
+  if (n) spectral_deriv<T>(c, fs, 1, nlev, outs, kinds, n);
 }
 
 // test_filtermodule.f90:13-30 applied to `nplanes` (ld,ny) planes in place
@@ -448,14 +567,14 @@ void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep) {
   const size_t xs = (size_t)rows * keep * sizeof(cx<T>);
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, xs);
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
-  LoadFields<T> L{{planes, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1)};
-  launch_row_fwd<T>(c, L, X0, keep, rows, g.nx, keep);
+  LoadFields<T> L{{planes, nullptr, nullptr, nullptr}, nplanes, g.ny, g.ld, T(1)};
+  launch_row_fwd<T>(c, L, X0, keep, rows, g.nx, keep, g.ny);
   ColArgs a = col_args(g.ny, g.ny, keep, keep, (long)g.ny * keep);
   a.lhG = g.lh;
   col_out(a, 0, K_G, keep, keep, (long)g.ny * keep);
   launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nplanes, keep, G);
-  StoreFields<T> S{{planes, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
-  launch_row_inv<T>(c, X1, keep, keep, S, rows, g.nx, g.ld);
+  StoreFields<T> S{{planes, nullptr, nullptr, nullptr}, nplanes, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, keep, keep, S, rows, g.nx, g.ld, g.ny);
 }
 
 // dealias1.f90: u (ld,ny,nlev) -> u_big (ld_big,ny2,nlev); u untouched
@@ -466,14 +585,14 @@ void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev) {
   const long rows = (long)nlev * g.ny, rows2 = (long)nlev * g.ny2;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)rows2 * g.nxh * sizeof(cx<T>));
-  LoadFields<T> L{{u, nullptr, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, rows, g.nx, g.nxh);
+  LoadFields<T> L{{u, nullptr, nullptr, nullptr}, nlev, g.ny, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows, g.nx, g.nxh, g.ny);
   ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
   launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nlev, g.nxh);
   const int ldb = 2 * (g.nx2 / 2 + 1);
-  StoreFields<T> S{{u_big, nullptr, nullptr, nullptr}, (int)rows2, ldb, g.nx2};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows2, g.nx2, ldb);
+  StoreFields<T> S{{u_big, nullptr, nullptr, nullptr}, nlev, g.ny2, ldb, g.nx2};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows2, g.nx2, ldb, g.ny2);
 }
 
 // dealias2.f90: u_big (ld_big,ny2,nlev) -> u (ld,ny,nlev).  The reference
@@ -486,13 +605,13 @@ void op_dealias2(les_ctx* c, T* u, T* u_big, int nlev) {
   const int ldb = 2 * (g.nx2 / 2 + 1);
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows2 * g.nxh * sizeof(cx<T>));
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * g.nxh * sizeof(cx<T>));
-  LoadFields<T> L{{u_big, nullptr, nullptr, nullptr}, (int)rows2, ldb, T(1.0 / ((double)g.nx2 * g.ny2))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, rows2, g.nx2, g.nxh);
+  LoadFields<T> L{{u_big, nullptr, nullptr, nullptr}, nlev, g.ny2, ldb, T(1.0 / ((double)g.nx2 * g.ny2))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows2, g.nx2, g.nxh, g.ny2);
   ColArgs a = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
   launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, a, nlev, g.nxh);
-  StoreFields<T> S{{u, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld);
+  StoreFields<T> S{{u, nullptr, nullptr, nullptr}, nlev, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld, g.ny);
 }
 
 // padd.f90 / unpadd.f90 on complex (lh,ny) <-> (lh_big,ny2) planes
@@ -589,9 +708,9 @@ struct StoreDiv {
   int ny, ld, nx, nz, wnode;
   T rdz;
   static constexpr double writes() { return 2.0; }  // divt + tz read
-  __device__ void operator()(int row, int x, T v) const {
-    const int k = row / ny;
-    const long i = (long)row * ld + x;
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int k = plane;
+    const long i = ((long)plane * ny + y) * ld + x;
     const long pl = (long)ld * ny;
     if (!wnode && k == nz - 1) { divt[i] = T(BOGUS); return; }
     if (x >= nx) { divt[i] = T(0); return; }
@@ -616,14 +735,14 @@ void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, in
   const size_t xs = (size_t)rows * g.nxh * sizeof(cx<T>);
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, 2 * xs);
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
-  LoadFields<T> L{{tx, ty, nullptr, nullptr}, (int)rows, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, 2 * rows, g.nx, g.nxh);
+  LoadFields<T> L{{tx, ty, nullptr, nullptr}, nlev, g.ny, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, X0, g.nxh, 2 * rows, g.nx, g.nxh, g.ny);
   ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
   a.in2_plane = a.in_plane;
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
   launch_col<T>(c, X0, X0 + (size_t)rows * g.nxh, X1, nullptr, nullptr, a, nlev, g.nxh);
   StoreDiv<T> S{divt, tz, g.ny, g.ld, g.nx, g.nz, wnode, T(1.0 / g.dz)};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld);
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld, g.ny);
 }
 
 // ------------------------------------------------------------------------
@@ -638,10 +757,9 @@ struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
   int ny, nz, ld;
   T cst;
   static constexpr double reads() { return 1.5; }  // 9 fields feed 6 rows
-  __device__ T operator()(int row, int x) const {
-    const int rpf = nz * ny;
-    const int f = row / rpf, r = row - f * rpf, k = r / ny;
-    const long i = (long)r * ld + x, pl = (long)ld * ny;
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nz, k = plane - f * nz;
+    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
     switch (f) {
       case 0: return cst * u1[i];
       case 1: return cst * u2[i];
@@ -658,56 +776,203 @@ struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
 };
 
 template <typename T>
-struct LoadProducts {  // fields cx, cy, cz on the padded grid; levels 0..nz-2
-  const T* U;          // 6 big fields, (nx2, ny2, nz), row stride nx2
-  int ny2, nz, nx2;
-  T cst;
-  static constexpr double reads() { return 2.0; }  // 6 big fields feed 3 rows
-  __device__ T operator()(int row, int x) const {
-    const int nl = nz - 1;
-    const int rpf = nl * ny2;
-    const int f = row / rpf, r = row - f * rpf, k = r / ny2, y = r - k * ny2;
-    const long fs = (long)nx2 * ny2 * nz, pl = (long)nx2 * ny2;
-    const long i = (long)k * pl + (long)y * nx2 + x;
-    const T* u1 = U;
-    const T* u2 = U + fs;
-    const T* u3 = U + 2 * fs;
-    const T* v1 = U + 3 * fs;
-    const T* v2 = U + 4 * fs;
-    const T* v3 = U + 5 * fs;
-    if (f == 0) {
-      if (k == 0) return cst * (u2[i] * (-v3[i]) + T(0.5) * u3[i + pl] * (v2[i + pl]));
-      return cst * (u2[i] * (-v3[i]) + T(0.5) * (u3[i + pl] * (v2[i + pl]) + u3[i] * (v2[i])));
-    } else if (f == 1) {
-      if (k == 0) return cst * (u1[i] * (v3[i]) + T(0.5) * u3[i + pl] * (-v1[i + pl]));
-      return cst * (u1[i] * (v3[i]) + T(0.5) * (u3[i + pl] * (-v1[i + pl]) + u3[i] * (-v1[i])));
-    }
-    if (k == 0) return T(0);
-    return cst * T(0.5) * ((u1[i] + u1[i - pl]) * (-v2[i]) + (u2[i] + u2[i - pl]) * (v1[i]));
-  }
-};
-
-template <typename T>
-struct StoreBig {
-  T* p;
-  int nx2;
-  static constexpr double writes() { return 1.0; }
-  __device__ void operator()(int row, int x, T v) const {
-    if (x < nx2) p[(long)row * nx2 + x] = v;
-  }
-};
-
-template <typename T>
 struct StoreConvec {  // 3 outputs, levels 0..nz-2 ; level nz-1 BOGUS handled apart
   T *ox, *oy, *oz;
-  int rows_per_field, ld, nx;
+  int ppf, ny, ld, nx;
   static constexpr double writes() { return 1.0; }
-  __device__ void operator()(int row, int x, T v) const {
-    const int f = row / rows_per_field, r = row - f * rows_per_field;
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
     T* p = f == 0 ? ox : (f == 1 ? oy : oz);
-    p[(long)r * ld + x] = x < nx ? v : T(0);
+    p[((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
   }
 };
+
+// Padded-grid part of convec.f90:117-233 in one pass.  One workgroup owns
+// one padded row yb and a chunk of output levels; it marches up in z keeping
+// the physical padded rows of u1,u2,u3,omega1,omega2,omega3 for levels
+// k-1, k, k+1 in an LDS ring, forms the staggered products of level k and
+// transforms them back to x-spectral rows (kx < nx/2).  The padded physical
+// fields never touch HBM.
+template <typename T, int N>
+__global__ void __launch_bounds__((N > 0 ? 3 * N / 4 : 1024))
+    k_convec_big(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int nxh, int chunk, FftPlan p,
+                 const cx<T>* __restrict__ twg, T cst) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = N > 0 ? N : p.n;
+  auto fft3 = [&](cx<T>* buf) {
+    if constexpr (N > 0)
+      fft_lds_c<T, (N > 0 ? N : 4), 1, (N > 0 ? N : 4), 3, (N > 0 ? 3 * N / 4 : 64), false>(buf, reinterpret_cast<cx<T>*>(smem_raw), +1);
+    else
+      fft_lds<T, false, 2>(buf, p, 1, n, 3, reinterpret_cast<cx<T>*>(smem_raw), +1);
+  };
+  auto fft2 = [&](cx<T>* buf) {
+    if constexpr (N > 0)
+      fft_lds_c<T, (N > 0 ? N : 4), 1, (N > 0 ? N : 4), 2, (N > 0 ? 3 * N / 4 : 64), false>(buf, reinterpret_cast<cx<T>*>(smem_raw), -1);
+    else
+      fft_lds<T, false, 2>(buf, p, 1, n, 2, reinterpret_cast<cx<T>*>(smem_raw), -1);
+  };
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* fb = tw + n;
+  T* win = reinterpret_cast<T*>(fb + 3 * n);
+  load_twiddles(tw, twg, n);
+  const int yb = blockIdx.x;
+  const int k0 = blockIdx.y * chunk;
+  const int k1 = min(k0 + chunk, nz - 1);
+  auto W = [&](int kk, int f) -> T* { return win + ((kk % 3) * 6 + f) * n; };
+  auto load_level = [&](int kk) {
+    for (int e = threadIdx.x; e < 3 * n; e += blockDim.x) {
+      const int t = e / n, j = e - t * n;
+      const cx<T>* a = XB + (((long)(2 * t) * nz + kk) * ny2 + yb) * nxh;
+      const cx<T>* b = XB + (((long)(2 * t + 1) * nz + kk) * ny2 + yb) * nxh;
+      cx<T> z = {T(0), T(0)};
+      if (j == 0) {
+        z = {a[0].x, b[0].x};
+      } else if (j < nxh) {
+        const cx<T> A = a[j], B = b[j];
+        z = {A.x - B.y, A.y + B.x};
+      } else if (n - j < nxh) {
+        const cx<T> A = a[n - j], B = b[n - j];
+        z = {A.x + B.y, B.x - A.y};
+      }
+      fb[e] = z;
+    }
+    __syncthreads();
+    fft3(fb);
+    for (int e = threadIdx.x; e < 3 * n; e += blockDim.x) {
+      const int t = e / n, x = e - t * n;
+      W(kk, 2 * t)[x] = fb[e].x;
+      W(kk, 2 * t + 1)[x] = fb[e].y;
+    }
+    __syncthreads();
+  };
+  if (k0 >= 1) load_level(k0 - 1);
+  load_level(k0);
+  for (int k = k0; k < k1; ++k) {
+    load_level(k + 1);
+    const T *u1 = W(k, 0), *u2 = W(k, 1), *u3 = W(k, 2), *v1 = W(k, 3), *v2 = W(k, 4), *v3 = W(k, 5);
+    const T *u3p = W(k + 1, 2), *v1p = W(k + 1, 3), *v2p = W(k + 1, 4);
+    const T *u1m = W(k + 2, 0), *u2m = W(k + 2, 1);  // slot (k+2)%3 == (k-1)%3
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
+      T a, b, c;
+      if (k == 0) {
+        a = cst * (u2[x] * (-v3[x]) + T(0.5) * u3p[x] * (v2p[x]));
+        b = cst * (u1[x] * (v3[x]) + T(0.5) * u3p[x] * (-v1p[x]));
+        c = T(0);
+      } else {
+        a = cst * (u2[x] * (-v3[x]) + T(0.5) * (u3p[x] * (v2p[x]) + u3[x] * (v2[x])));
+        b = cst * (u1[x] * (v3[x]) + T(0.5) * (u3p[x] * (-v1p[x]) + u3[x] * (-v1[x])));
+        c = cst * T(0.5) * ((u1[x] + u1m[x]) * (-v2[x]) + (u2[x] + u2m[x]) * (v1[x]));
+      }
+      fb[x] = {a, b};
+      fb[n + x] = {c, T(0)};
+    }
+    __syncthreads();
+    fft2(fb);
+    const int nl = nz - 1;
+    for (int e = threadIdx.x; e < 2 * nxh; e += blockDim.x) {
+      const int t = e / nxh, kk = e - t * nxh;
+      const cx<T> zk = fb[t * n + kk], zn = fb[t * n + (kk ? n - kk : 0)];
+      const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+      if (t == 0) {
+        const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+        XP[(((long)0 * nl + k) * ny2 + yb) * nxh + kk] = A;
+        XP[(((long)1 * nl + k) * ny2 + yb) * nxh + kk] = B;
+      } else {
+        XP[(((long)2 * nl + k) * ny2 + yb) * nxh + kk] = A;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// main.f90:309-378 fused into the c2r epilogue of convec: RHS = -conv - divt,
+// u* = u + dt*(1.5 RHS - 0.5 RHS_f + force) (force on u only).  Arithmetic
+// and evaluation order as k_ab2.
+template <typename T>
+struct StoreConvecAB2 {
+  T* R[3];
+  T* Rf[3];
+  const T* dv[3];
+  T* U[3];
+  int ppf, ny, ld, nx;
+  T dt, force;
+  int euler;
+  static constexpr double writes() { return 5.0; }
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    const long i = ((long)r * ny + y) * ld + x;
+    const T conv = x < nx ? v : T(0);
+    const T rhs = -conv - dv[f][i];
+    R[f][i] = rhs;
+    T fo;
+    if (euler) {
+      fo = rhs;
+      Rf[f][i] = rhs;
+    } else {
+      fo = Rf[f][i];
+    }
+    if (f == 0)
+      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo + force);
+    else
+      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo);
+  }
+};
+
+template <typename T, int N>
+static void launch_convec_big(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const int n = g.nx2, chunk = 16;
+  const FftPlan& p = plan_for(c, n);
+  const int threads = 3 * n / 4;
+  const size_t sm = (size_t)4 * n * sizeof(cx<T>) + (size_t)18 * n * sizeof(T);
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_convec_big<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const unsigned gy = (unsigned)((nz - 1 + chunk - 1) / chunk);
+  ProfScope ps(c, "convec_big", (double)(6.0 * nz * g.ny2 + 3.0 * (nz - 1) * g.ny2) * g.nxh * sizeof(cx<T>));
+  hipLaunchKernelGGL((k_convec_big<T, N>), dim3(g.ny2, gy), dim3(threads), sm, c->stream, XB, XP, g.ny2, nz, g.nxh,
+                     chunk, p, tw_for<T>(c, n), T(1.0 / ((double)g.nx2 * g.ny2)));
+  LCHK(c, hipGetLastError());
+}
+
+// convec.f90:43-233 up to the small-grid x-spectral products (3 fields,
+// levels 1..nz-1); returns the scratch holding them.
+template <typename T>
+static cx<T>* convec_spectral(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,
+                              const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy) {
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const long rows6 = 6L * nz * g.ny;
+  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
+  LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+  launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh, g.ny);
+  // pad in y: 6*nz planes
+  const long rowsB = 6L * nz * g.ny2;
+  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
+  // padded-grid products without materialising the padded fields
+  const long rowsP = 3L * (nz - 1) * g.ny2;
+  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)rowsP * g.nxh * sizeof(cx<T>));
+  switch (g.nx2) {
+#define LES_CASE(NN) \
+  case NN:           \
+    launch_convec_big<T, NN>(c, XB, XP);  \
+    break;
+    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
+#undef LES_CASE
+    default:
+      launch_convec_big<T, 0>(c, XB, XP);
+  }
+  // truncate in y
+  const long rows3 = 3L * (nz - 1) * g.ny;
+  cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)rows3 * g.nxh * sizeof(cx<T>));
+  ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, 3L * (nz - 1), g.nxh);
+  return X3;
+}
 
 template <typename T>
 void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz, const T* dvdx,
@@ -715,37 +980,29 @@ void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, co
   OpTag _tag(c, "convec");
   const Grid& g = c->g;
   const int nz = g.nz;
-  const long rows6 = 6L * nz * g.ny;
-  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
-  LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh);
-  // pad in y: 6*nz planes
-  const long rowsB = 6L * nz * g.ny2;
-  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
-  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
-  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
-  // big physical fields
-  T* U = (T*)scratch(c, S_BIG0, (size_t)rowsB * g.nx2 * sizeof(T));
-  StoreBig<T> SB{U, g.nx2};
-  launch_row_inv<T>(c, XB, g.nxh, g.nxh, SB, rowsB, g.nx2, g.nx2);
-  // products -> x-spectral (kx < nx/2) on the padded y grid
-  const long rowsP = 3L * (nz - 1) * g.ny2;
-  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)rowsP * g.nxh * sizeof(cx<T>));
-  LoadProducts<T> LP{U, g.ny2, nz, g.nx2, T(1.0 / ((double)g.nx2 * g.ny2))};
-  launch_row_fwd<T>(c, LP, XP, g.nxh, rowsP, g.nx2, g.nxh);
-  // truncate in y
-  const long rows3 = 3L * (nz - 1) * g.ny;
-  cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)rows3 * g.nxh * sizeof(cx<T>));
-  ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
-  col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, 3L * (nz - 1), g.nxh);
-  StoreConvec<T> SC{ox, oy, oz, (int)((nz - 1) * g.ny), g.ld, g.nx};
-  launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, rows3, g.nx, g.ld);
+  cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy);
+  StoreConvec<T> SC{ox, oy, oz, nz - 1, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * (nz - 1) * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
   op_fill<T>(c, ox + top, pl, T(BOGUS));
   op_fill<T>(c, oy + top, pl, T(BOGUS));
   op_fill<T>(c, oz + top, pl, T(BOGUS));
+}
+
+template <typename T>
+void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
+                   const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
+                   const T* divtx, const T* divty, const T* divtz, int euler_first) {
+  OpTag _tag(c, "convec");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy);
+  StoreConvecAB2<T> SC{{RHSx, RHSy, RHSz}, {RHSx_f, RHSy_f, RHSz_f}, {divtx, divty, divtz}, {u, v, w},
+                       nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first};
+  launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * (nz - 1) * g.ny, g.nx, g.ld, g.ny);
+  const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
+  T* tops[9] = {RHSx, RHSy, RHSz, u, v, w, RHSx_f, RHSy_f, RHSz_f};
+  for (int q = 0; q < (euler_first ? 9 : 6); ++q) op_fill<T>(c, tops[q] + top, pl, T(BOGUS));
 }
 
 // ------------------------------------------------------------------------
@@ -979,10 +1236,9 @@ struct LoadLag {  // field order: ub vb wb | uu uv uw vv vw ww | S11 S12 S13 S22
   const T *u, *v, *w, *S11, *S12, *S13, *S22, *S23, *S33;
   int ny, nz, ld;
   static constexpr double reads() { return 9.0 / 21.0; }  // u,v,w,S_ij feed 21 rows
-  __device__ T operator()(int row, int x) const {
-    const int rpf = nz * ny;
-    const int f = row / rpf, r = row - f * rpf, k = r / ny;
-    const long i = (long)r * ld + x, pl = (long)ld * ny;
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nz, k = plane - f * nz;
+    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
     if (f < 9) {
       T ub, vb, wb;
       if (k == 0) { ub = u[i]; vb = v[i]; wb = T(0.25) * w[i + pl]; }
@@ -1132,7 +1388,7 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   const long rows = 21L * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
   LoadLag<T> L{u, v, w, S[0], S[1], S[2], S[3], S[4], S[5], g.ny, nz, g.ld};
-  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf);
+  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
   cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
   cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
   ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);
@@ -1145,10 +1401,10 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   T* F = (T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
   // row_inv of 21 fields: StoreFields handles up to 4 fields, so use one
   // "field" spanning all 21 (rows are contiguous field-major)
-  StoreFields<T> S2{{F, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
-  launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld);
-  StoreFields<T> S4{{F + 21 * n3, nullptr, nullptr, nullptr}, (int)rows, g.ld, g.nx};
-  launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld);
+  StoreFields<T> S2{{F, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
+  StoreFields<T> S4{{F + 21 * n3, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
   double* red = (double*)scratch(c, S_RED, 3 * nz * sizeof(double));
   op_fill<double>(c, red, 3 * (size_t)nz, 0.0);
   const int init_lm = g.inilag && !c->flm_init && (c->jt == g.cs_count || c->jt == 1);
@@ -1220,10 +1476,9 @@ struct LoadPress {  // fields Hx, Hy, Hz on levels 0..nz-2, scaled (const/tadv1)
   int ny, nl, ld;
   T cst, dt;
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int row, int x) const {
-    const int rpf = nl * ny;
-    const int f = row / rpf, r = row - f * rpf;
-    const long i = (long)r * ld + x;
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nl, r = plane - f * nl;
+    const long i = ((long)r * ny + y) * ld + x;
     const T* p = f == 0 ? u : (f == 1 ? v : w);
     return cst * (p[i] / dt);
   }
@@ -1234,9 +1489,8 @@ struct LoadPlanes2 {  // two planes of one field: level 0 and level kt
   int ny, ld, kt;
   T cst;
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int row, int x) const {
-    const int j = row / ny, y = row - j * ny;
-    const int k = j == 0 ? 0 : kt;
+  __device__ T operator()(int plane, int y, int x) const {
+    const int k = plane == 0 ? 0 : kt;
     return cst * f[((long)k * ny + y) * ld + x];
   }
 };
@@ -1331,14 +1585,15 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
 template <typename T>
 struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: dpdy
   T *p, *dpdx, *dpdy;
-  int rows_per_field, ld, nx, ny;
+  int ppf, ld, nx, ny;
   static constexpr double writes() { return 1.0; }
-  __device__ void operator()(int row, int x, T v) const {
-    const int f = row / rows_per_field, r = row - f * rows_per_field;
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
     const T val = x < nx ? v : T(0);
-    if (f == 0) p[(long)(r + ny) * ld + x] = val;  // p(:,:,1:nz-1) of p(ld,ny,0:nz)
-    else if (f == 1) dpdx[(long)r * ld + x] = val;
-    else dpdy[(long)r * ld + x] = val;
+    const long row = (long)r * ny + y;
+    if (f == 0) p[(row + ny) * ld + x] = val;  // p(:,:,1:nz-1) of p(ld,ny,0:nz)
+    else if (f == 1) dpdx[row * ld + x] = val;
+    else dpdy[row * ld + x] = val;
   }
 };
 
@@ -1352,10 +1607,10 @@ void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divt
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)(rows + 2 * g.ny) * nxh * sizeof(cx<T>));
   const double cstd = 1.0 / ((double)g.nx * g.ny);
   LoadPress<T> L{u, v, w, g.ny, nl, g.ld, (T)(cstd / TADV1), (T)g.dt};
-  launch_row_fwd<T>(c, L, X0, nxh, rows, g.nx, nxh);
+  launch_row_fwd<T>(c, L, X0, nxh, rows, g.nx, nxh, g.ny);
   LoadPlanes2<T> LB{divtz, g.ny, g.ld, nz - 1, (T)cstd};
   cx<T>* XB = X0 + (size_t)rows * nxh;
-  launch_row_fwd<T>(c, LB, XB, nxh, 2L * g.ny, g.nx, nxh);
+  launch_row_fwd<T>(c, LB, XB, nxh, 2L * g.ny, g.nx, nxh, g.ny);
   // forward y transforms only -> spectral H (3*nl planes) and bottom/top (2)
   cx<T>* H = (cx<T>*)scratch(c, S_PSPEC, (size_t)(3 * nl + 2) * ps * sizeof(cx<T>));
   ColArgs a = col_args(g.ny, g.ny, nxh, nxh, ps);
@@ -1390,10 +1645,10 @@ void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divt
   col_out(b0, 0, K_ID, nxh, nxh, ps);
   cx<T>* X1b = X1 + 3 * fo;
   launch_col<T>(c, PH, nullptr, X1b, nullptr, nullptr, b0, 1, nxh);
-  StorePress<T> SP{p, dpdx, dpdy, (int)(nl * g.ny), g.ld, g.nx, g.ny};
-  launch_row_inv<T>(c, X1, nxh, nxh, SP, rowsP, g.nx, g.ld);
-  StoreFields<T> S0{{p, nullptr, nullptr, nullptr}, g.ny, g.ld, g.nx};
-  launch_row_inv<T>(c, X1b, nxh, nxh, S0, g.ny, g.nx, g.ld);
+  StorePress<T> SP{p, dpdx, dpdy, nl, g.ld, g.nx, g.ny};
+  launch_row_inv<T>(c, X1, nxh, nxh, SP, rowsP, g.nx, g.ld, g.ny);
+  StoreFields<T> S0{{p, nullptr, nullptr, nullptr}, 1, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X1b, nxh, nxh, S0, g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny;
   hipLaunchKernelGGL((k_fill_bogus_cplx<T>), grid1(pl), dim3(256), 0, c->stream, p + pl * nz, pl);
   hipLaunchKernelGGL((k_fill_bogus_cplx<T>), grid1(pl), dim3(256), 0, c->stream, dpdx + pl * (nz - 1), pl);
@@ -1651,6 +1906,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
 // ------------------------------------------------------------------------
 #define LES_INST(T)                                                                                              \
   template void op_filt_da<T>(les_ctx*, T*, T*, T*, int);                                                        \
+  template void op_filt_da3<T>(les_ctx*, T*, T*, T*, T* const[9], int);                                          \
   template void op_ddxy<T>(les_ctx*, T*, T*, const T*, int);                                                     \
   template void op_test_filter<T>(les_ctx*, T*, int, const T*, int);                                             \
   template void op_dealias1<T>(les_ctx*, const T*, T*, int);                                                     \
@@ -1662,6 +1918,8 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_divstress<T>(les_ctx*, T*, const T*, const T*, const T*, int);                                \
   template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
                              const T*, const T*, T*, T*, T*);                                                    \
+  template void op_convec_ab2<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,         \
+                                 const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int);           \
   template void op_wallstress<T>(les_ctx*, const T*, const T*, T*, T*, T*, T*, const double*, const double*,     \
                                  const double*, double*);                                                        \
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
