@@ -23,12 +23,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps):
+ERR_FIELDS = ("u", "v", "w", "p")
+
+
+def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False):
     import torch
     from les_amd.api import LesSim
-    sim = LesSim(nx, ny, nz, precision=prec, dt=1e-4)
+    sim = LesSim(nx, ny, nz, precision=prec, dt=1e-4, graph=graph)
     sim.init_ic(seed=42)
-    sim.step(warmup)
+    sim.set_graph(False)
+    sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
+    sim.set_graph(graph)
+    sim.graph_prepare()         # capture the replay window now, outside the timed region
     sim.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -40,6 +46,7 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps):
     prof = None
     if profile_steps:
         # separate pass with per-launch HIP events (same stream as the kernels)
+        sim.set_graph(False)
         sim._chk(sim.L.les_profile(sim.ctx, 1))
         sim.step(profile_steps)
         buf = ctypes.create_string_buffer(1 << 16)
@@ -50,13 +57,31 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps):
             prof[k][1] /= profile_steps   # ms per step
             prof[k][0] /= profile_steps   # launches per step
             prof[k][2] /= profile_steps   # bytes per step
+    if extra_steps:
+        sim.step(extra_steps)     # untimed: bring the step count level with the profiled run
+    fields = {n: sim.get(n) for n in ERR_FIELDS} if keep else None
     rms = ctypes.c_double()
     out4 = (ctypes.c_double * 4)()
     sim._chk(sim.L.les_state_diag(sim.ctx, ctypes.byref(rms), out4))
     finite = math.isfinite(rms.value) and all(math.isfinite(x) for x in out4)
     del sim
     torch.cuda.empty_cache()
-    return ms, prof, finite, rms.value, list(out4)
+    return ms, prof, finite, rms.value, list(out4), fields
+
+
+def fp32_error(f32, f64, nx, nz):
+    """Per-field relative L2 error of the FP32-bulk state against the FP64
+    state after the same steps, and the worst plane-mean profile error."""
+    out = {}
+    for n in ERR_FIELDS:
+        a, b = f32[n], f64[n]
+        lev = slice(1, nz) if n == "p" else (slice(1, nz - 1) if n == "w" else slice(0, nz - 1))
+        a, b = a[lev, :, :nx], b[lev, :, :nx]
+        out[n] = float(((a - b) ** 2).sum() ** 0.5 / max((b ** 2).sum() ** 0.5, 1e-300))
+        if n in ("u", "v"):
+            pa, pb = a.mean(axis=(1, 2)), b.mean(axis=(1, 2))
+            out[n + "_mean_profile"] = float(abs(pa - pb).max() / max(abs(pb).max(), 1e-300))
+    return out
 
 
 def cpu_baseline(nx, ny, nz, nz_sample):
@@ -82,7 +107,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of HIP-graph replay")
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=129)
@@ -98,13 +124,19 @@ def main():
     nx, ny, nz, K, W = args.nx, args.ny, args.nz, args.steps, args.warmup
     pts = nx * ny * (nz - 1)
 
-    ms32, prof32, ok32, rms32, cfl32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps)
+    graph = not args.no_graph
+    ms32, prof32, ok32, rms32, cfl32, f32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
+                                                    keep=not args.no_fp64)
     print(f"fp32-bulk: {ms32:.3f} ms/step finite={ok32} rmsdiv={rms32:.3e} cfl={cfl32[0]:.3e}",
           file=sys.stderr, flush=True)
-    ms64 = None
+    ms64, err = None, None
     if not args.no_fp64:
-        ms64, prof64, ok64, rms64, cfl64 = run_gpu(nx, ny, nz, "fp64", K, W, 0)
+        ms64, prof64, ok64, rms64, cfl64, f64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
+                                                        extra_steps=args.profile_steps, keep=True)
         print(f"fp64: {ms64:.3f} ms/step finite={ok64} rmsdiv={rms64:.3e}", file=sys.stderr, flush=True)
+        err = fp32_error(f32, f64, nx, nz)
+        err["steps"] = W + K + args.profile_steps
+        del f32, f64
 
     value = pts / (ms32 * 1e-3)
     roof = None
@@ -136,10 +168,12 @@ def main():
         "dtype": "f32 (bulk; f64 tridiagonal, wall stress, reductions)", "data": "synthetic",
         "config": {"workload": f"neutral ABL {nx}x{ny}x{nz - 1} (nz={nz}), Lagrangian scale-dependent SGS "
                                "(model 5), log-law wall z0=1e-4, forcing 1, dt=1e-4, seeded IC",
-                   "grid": [nx, ny, nz], "parallelism": "1 GPU, whole domain"},
+                   "grid": [nx, ny, nz], "parallelism": "1 GPU, whole domain",
+                   "launch": "HIP graph replay of 10-step windows" if graph else "eager"},
         "fp64_value": (pts / (ms64 * 1e-3)) if ms64 else None,
         "fp64_ms_per_step": ms64,
         "fp32_over_fp64": (ms64 / ms32) if ms64 else None,
+        "fp32_rel_error": err,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

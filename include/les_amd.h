@@ -150,6 +150,14 @@ void *les_state_field(les_ctx *c, const char *name, int *nlev, int *elsize);
    SGS state. */
 int les_state_init_ic(les_ctx *c, uint64_t seed, double amp, double ustar, double kappa_ic);
 int les_step(les_ctx *c, int nsteps);
+/* Replay the step through HIP graphs (no reference counterpart): windows of
+   lcm(2, cs_count) steps whose host-side branches repeat (no Euler start, no
+   lagrange_Sdep initialisation, not crossing jt_total = 500) are captured
+   once and relaunched; other steps run eagerly.  Results are identical. */
+int les_set_graph(les_ctx *c, int enable);
+/* capture the graph for the window starting at the current step without
+   running it (keeps graph capture out of a timed region) */
+int les_graph_prepare(les_ctx *c);
 /* rmsdiv of the state's current derivatives and timestep_conditions */
 int les_state_diag(les_ctx *c, double *rms, double out4[4]);
 

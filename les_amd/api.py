@@ -211,10 +211,20 @@ class Les:
 class LesSim(Les):
     """Whole-step driver on context-owned device state (main.f90:202-415)."""
 
-    def __init__(self, nx, ny, nz, precision="fp64", **kw):
-        super().__init__(nx, ny, nz, **kw)
+    def __init__(self, nx, ny, nz, precision="fp64", graph=False, **kw):
+        # a stream of its own (default priority): graph capture needs a
+        # non-default stream, and the state is only touched through the library
+        self.stream = torch.cuda.Stream()
+        super().__init__(nx, ny, nz, stream=self.stream, **kw)
         self.precision = precision
         self._chk(self.L.les_state_alloc(self.ctx, 1 if precision == "fp32" else 0))
+        self._chk(self.L.les_set_graph(self.ctx, 1 if graph else 0))
+
+    def set_graph(self, enable):
+        self._chk(self.L.les_set_graph(self.ctx, 1 if enable else 0))
+
+    def graph_prepare(self):
+        self._chk(self.L.les_graph_prepare(self.ctx))
 
     def init_ic(self, seed=42, amp=0.1, ustar=1.0, kappa_ic=0.4):
         self._chk(self.L.les_state_init_ic(self.ctx, seed, amp, ustar, kappa_ic))

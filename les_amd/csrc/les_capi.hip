@@ -288,6 +288,7 @@ void les_destroy(les_ctx* c) {
     delete c->st;
   }
   delete[] c->G2_host;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   delete c;
 }
 
@@ -674,6 +675,92 @@ int les_state_init_ic(les_ctx* c, uint64_t seed, double amp, double ustar, doubl
   c->jt_total = 0;
   c->flm_init = 0;
   c->fqn_init = 0;
+  c->eager_steps = 0;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  DONE(c);
+}
+
+static void step_eager(les_ctx* c) {
+  if (c->st->prec)
+    step_once<float>(c);
+  else
+    step_once<double>(c);
+  c->eager_steps += 1;
+}
+
+// The host-side branches of one step depend on jt mod cs_count (dynamic
+// update), the RHS buffer parity, jt == 1 (Euler start), the lagrange_Sdep
+// initialisation (jt == cs_count) and jt_total < 500 (constant Cs,
+// sgs_stag.f90:166-173).  A window of W = lcm(2, cs_count) steps therefore
+// repeats exactly while none of the one-off conditions occurs inside it; such
+// a window is captured once into a HIP graph and replayed.
+static int graph_window(const les_ctx* c) {
+  const int cs = c->g.cs_count;
+  return cs % 2 == 0 ? cs : 2 * cs;
+}
+static bool window_is_regular(const les_ctx* c, int W, int* use_const) {
+  const Grid& g = c->g;
+  if (c->jt < 1 && !g.initu) return false;                      // Euler first step inside
+  const bool lag_init = g.inilag && (!c->flm_init || !c->fqn_init);
+  if (lag_init && c->jt < g.cs_count) return false;             // initialisation inside
+  const int first = c->jt_total + 1, last = c->jt_total + W;
+  if (first < 500 && last >= 500) return false;                // constant-Cs switch inside
+  // every routine must have run eagerly once, so scratch is at full size
+  if (c->eager_steps < W) return false;
+  *use_const = last < 500 ? 1 : 0;
+  return true;
+}
+static bool graph_matches(const les_ctx* c, int W, int uc) {
+  return c->gexec && c->g_start_mod == c->jt % W && c->g_use_const == uc && c->g_swap == c->st->rhs_swap;
+}
+// capture W steps from the current position into c->gexec without running
+// them; host-side counters are restored afterwards
+static void graph_capture(les_ctx* c, int W, int uc) {
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  const int jt0 = c->jt, jtt0 = c->jt_total, swap0 = c->st->rhs_swap, fl0 = c->flm_init, fq0 = c->fqn_init;
+  const long es0 = c->eager_steps;
+  check(c, hipStreamSynchronize(c->stream), "pre-capture sync");
+  hipGraph_t graph = nullptr;
+  check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  for (int i = 0; i < W; ++i) step_eager(c);
+  check(c, hipStreamEndCapture(c->stream, &graph), "hipStreamEndCapture");
+  c->jt = jt0;
+  c->jt_total = jtt0;
+  c->st->rhs_swap = swap0;
+  c->flm_init = fl0;
+  c->fqn_init = fq0;
+  c->eager_steps = es0;
+  if (graph && c->err.empty()) {
+    check(c, hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    c->g_start_mod = jt0 % W;
+    c->g_use_const = uc;
+    c->g_swap = swap0;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+}
+
+int les_set_graph(les_ctx* c, int enable) {
+  GUARD(c);
+  c->use_graph = enable != 0;
+  if (!c->use_graph && c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  DONE(c);
+}
+
+int les_graph_prepare(les_ctx* c) {
+  GUARD(c);
+  if (!c->st) return -1;
+  const int W = graph_window(c);
+  int uc = 0;
+  if (c->use_graph && window_is_regular(c, W, &uc) && !graph_matches(c, W, uc)) graph_capture(c, W, uc);
   DONE(c);
 }
 
@@ -683,11 +770,22 @@ int les_step(les_ctx* c, int nsteps) {
     c->err = "les_step: no state (call les_state_alloc)";
     return 1;
   }
-  for (int i = 0; i < nsteps && c->err.empty(); ++i) {
-    if (c->st->prec)
-      step_once<float>(c);
-    else
-      step_once<double>(c);
+  const int W = graph_window(c);
+  int done = 0;
+  while (done < nsteps && c->err.empty()) {
+    int uc = 0;
+    if (c->use_graph && !c->prof && nsteps - done >= W && window_is_regular(c, W, &uc)) {
+      if (!graph_matches(c, W, uc)) graph_capture(c, W, uc);
+      if (c->gexec && c->err.empty()) {
+        check(c, hipGraphLaunch(c->gexec, c->stream), "hipGraphLaunch");
+        c->jt += W;
+        c->jt_total += W;  // W is even: the RHS buffer parity is unchanged
+        done += W;
+        continue;
+      }
+    }
+    step_eager(c);
+    ++done;
   }
   DONE(c);
 }

@@ -69,6 +69,11 @@ struct les_ctx {
   bool prof = false;
   std::vector<les::ProfRec> prof_recs;
   std::string op_tag;
+  // captured multi-step window of the whole step (les_set_graph)
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  int g_start_mod = -1, g_use_const = -1, g_swap = -1;
+  long eager_steps = 0;  // steps run eagerly since the state was initialised
   // host-address -> device mirror registry (les_mirror)
   std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };

@@ -1040,8 +1040,8 @@ void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz,
   const Grid& g = c->g;
   const size_t pl = (size_t)g.ld * g.ny;
   T* planes = (T*)scratch(c, S_PLANE, 2 * pl * sizeof(T));
-  LCHK(c, hipMemcpyAsync(planes, u, pl * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-  LCHK(c, hipMemcpyAsync(planes + pl, v, pl * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+  op_copy<T>(c, planes, u, pl);  // kernels, not memcpy nodes: the step is graph-captured
+  op_copy<T>(c, planes + pl, v, pl);
   op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);
   {
     ProfScope _ps(c, "wall", 8.0 * g.nx * g.ny * sizeof(T));
