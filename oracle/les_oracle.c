@@ -142,6 +142,9 @@ struct or_ctx {
   plan2 sm, bg;
   int flm_init, fqn_init;   /* SAVE flags, lagrange_Sdep.f90:67-68 */
   int jt, jt_total;         /* mirrors param jt/jt_total for routine calls */
+  or_scalar_params S;       /* scalar switches and constants (zero: no scalars) */
+  double *wt_s, *wq_s;      /* (nx,ny) kinematic surface fluxes */
+  int glm_init, gqn_init;   /* SAVE flags, lagrange_scalar_Sdep.f90:65-66 */
 };
 
 static void plan2_init(plan2 *p, int nx, int ny) {
@@ -195,6 +198,7 @@ or_ctx *or_create(const or_params *p) {
 void or_destroy(or_ctx *c) {
   if (!c) return;
   free(c->kx); free(c->ky); free(c->k2); free(c->Gt); free(c->Gtt);
+  free(c->wt_s); free(c->wq_s);
   fplan_free(&c->sm.px); fplan_free(&c->sm.py); fplan_free(&c->bg.px); fplan_free(&c->bg.py);
   free(c);
 }
@@ -603,14 +607,14 @@ void or_calc_sij(or_ctx *c, const double *const g[9], double *const S[6]) {
 /* ------------------------------------------------------------------ */
 /* interpolag_Sdep.f90:2-380 (non-MPI, read_inflow_file=.false.)        */
 /* ------------------------------------------------------------------ */
-void or_interpolag_sdep(or_ctx *c, or_sgs_state *s) {
+/* the same interpolation serves interpolag_scalar_Sdep.f90:30-316 on
+   G_LM..G_NN with u_lag_s..w_lag_s */
+static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *vl, double *wl) {
   const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
   const size_t pl = (size_t)ld * ny, n3 = pl * nz;
   const double cs = (double)c->P.cs_count, dt = c->P.dt;
   double *FF[4];
-  double *F[4] = { s->F_LM, s->F_MM, s->F_QN, s->F_NN };
   for (int q = 0; q < 4; q++) { FF[q] = (double *)malloc(sizeof(double) * n3); memcpy(FF[q], F[q], sizeof(double) * n3); }
-  double *ul = s->u_lag, *vl = s->v_lag, *wl = s->w_lag;
   for (size_t i = 0; i < n3; i++) { ul[i] = ul[i] / cs; vl[i] = vl[i] / cs; }
   for (int k = nz - 1; k >= 1; k--)
     for (size_t i = 0; i < pl; i++) {
@@ -659,6 +663,16 @@ void or_interpolag_sdep(or_ctx *c, or_sgs_state *s) {
       }
   memset(ul, 0, sizeof(double) * n3); memset(vl, 0, sizeof(double) * n3); memset(wl, 0, sizeof(double) * n3);
   for (int q = 0; q < 4; q++) free(FF[q]);
+}
+
+void or_interpolag_sdep(or_ctx *c, or_sgs_state *s) {
+  double *F[4] = { s->F_LM, s->F_MM, s->F_QN, s->F_NN };
+  interpolag_core(c, F, s->u_lag, s->v_lag, s->w_lag);
+}
+
+void or_interpolag_scalar_sdep(or_ctx *c, or_scalar_sgs *g) {
+  double *F[4] = { g->G_LM, g->G_MM, g->G_QN, g->G_NN };
+  interpolag_core(c, F, g->u_lag_s, g->v_lag_s, g->w_lag_s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -807,9 +821,28 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
 /* ------------------------------------------------------------------ */
 /* sgs_stag.f90:3-366 (model=5, sgs=.true., molec=.false., 'wall')      */
 /* ------------------------------------------------------------------ */
+static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
+                          const double *v, const double *w, double *theta,
+                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss);
+
 void or_sgs_stag(or_ctx *c, const double *const g[9], const double *u,
                  const double *v, const double *w, double *const tau[6],
                  or_sgs_state *s) {
+  sgs_stag_impl(c, g, u, v, w, NULL, tau, s, NULL);
+}
+
+/* model=7: also u_lag_s accumulation (sgs_stag.f90:64-66), the scalar
+   Lagrangian model after lagrange_Sdep (:150-153) and Pr_t=Pr while
+   jt_total<500 (:171) */
+void or_sgs_stag_scalar(or_ctx *c, const double *const g[9], const double *u,
+                        const double *v, const double *w, double *theta,
+                        double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss) {
+  sgs_stag_impl(c, g, u, v, w, theta, tau, s, ss);
+}
+
+static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
+                          const double *v, const double *w, double *theta,
+                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss) {
   const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
   const size_t pl = (size_t)ld * ny, n3 = pl * nz;
   double *S[6];
@@ -819,16 +852,20 @@ void or_sgs_stag(or_ctx *c, const double *const g[9], const double *u,
   double *txzp = (double *)malloc(sizeof(double) * pl), *tyzp = (double *)malloc(sizeof(double) * pl);
   memcpy(txzp, txz, sizeof(double) * pl); memcpy(tyzp, tyz, sizeof(double) * pl);
   for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
+  if (ss)
+    for (size_t i = 0; i < n3; i++) { ss->u_lag_s[i] += u[i]; ss->v_lag_s[i] += v[i]; ss->w_lag_s[i] += w[i]; }
   const int jt = c->jt;
   const double l = c->delta;
   if (jt == 1 && c->P.inilag) {
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = 0.03;
   } else if ((jt >= 1 || c->P.initu) && (jt % c->P.cs_count == 0)) {
     or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
+    if (ss) or_lagrange_scalar_sdep(c, (const double *const *)S, u, v, w, theta, s, ss);
   }
   if (c->jt_total < 500) {
     const double cs0 = (double)(0.16f * 0.16f);
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = cs0;
+    if (ss) for (size_t i = 0; i < n3; i++) ss->Pr_t[i] = c->S.Pr;
   }
   for (int k = 0; k < nz; k++)
     for (int y = 0; y < ny; y++)
@@ -1154,6 +1191,20 @@ or_state *or_state_create(or_ctx *c) {
   g->L11t = al3(c, nz); g->L22t = al3(c, nz); g->L33t = al3(c, nz);
   g->Q11t = al3(c, nz); g->Q22t = al3(c, nz); g->Q33t = al3(c, nz);
   g->Beta_avg = (double *)calloc(nz, sizeof(double)); g->Betaclip_avg = (double *)calloc(nz, sizeof(double));
+  s->theta = al3(c, nz); s->qmix = al3(c, nz);
+  s->dTdx = al3(c, nz); s->dTdy = al3(c, nz); s->dTdz = al3(c, nz);
+  s->dqdx = al3(c, nz); s->dqdy = al3(c, nz); s->dqdz = al3(c, nz);
+  s->RHS_T = al3(c, nz); s->RHS_Tf = al3(c, nz); s->RHS_q = al3(c, nz); s->RHS_qf = al3(c, nz);
+  s->beta_scal = al3(c, nz);
+  s->sgs_t3 = al3(c, 1); s->sgs_q3 = al3(c, 1);
+  s->psi_h = (double *)calloc(n2, sizeof(double)); s->psi_h0 = (double *)calloc(n2, sizeof(double));
+  s->phi_h = (double *)calloc(n2, sizeof(double));
+  for (size_t i = 0; i < n2; i++) s->phi_h[i] = 1.0;
+  or_scalar_sgs *h = &s->ssgs;
+  h->G_LM = al3(c, nz); h->G_MM = al3(c, nz); h->G_QN = al3(c, nz); h->G_NN = al3(c, nz);
+  h->u_lag_s = al3(c, nz); h->v_lag_s = al3(c, nz); h->w_lag_s = al3(c, nz);
+  h->Pr_t = al3(c, nz); h->Cs2_Pr = al3(c, nz);
+  h->phi_h = s->phi_h; h->psi_m = s->psi_m;
   return s;
 }
 
@@ -1170,6 +1221,13 @@ void or_state_destroy(or_state *s) {
   free(g->u_lag); free(g->v_lag); free(g->w_lag); free(g->TnLMMM); free(g->TnQNNN);
   free(g->L11t); free(g->L22t); free(g->L33t); free(g->Q11t); free(g->Q22t); free(g->Q33t);
   free(g->Beta_avg); free(g->Betaclip_avg);
+  free(s->theta); free(s->qmix); free(s->dTdx); free(s->dTdy); free(s->dTdz);
+  free(s->dqdx); free(s->dqdy); free(s->dqdz);
+  free(s->RHS_T); free(s->RHS_Tf); free(s->RHS_q); free(s->RHS_qf); free(s->beta_scal);
+  free(s->sgs_t3); free(s->sgs_q3); free(s->psi_h); free(s->psi_h0); free(s->phi_h);
+  or_scalar_sgs *h = &s->ssgs;
+  free(h->G_LM); free(h->G_MM); free(h->G_QN); free(h->G_NN);
+  free(h->u_lag_s); free(h->v_lag_s); free(h->w_lag_s); free(h->Pr_t); free(h->Cs2_Pr);
   free(s);
 }
 
@@ -1207,7 +1265,10 @@ void or_state_init_ic(or_ctx *c, or_state *s, uint64_t seed, double amp, double 
   }
   s->jt = 0; s->jt_total = 0;
   c->flm_init = 0; c->fqn_init = 0;
+  c->glm_init = 0; c->gqn_init = 0;
 }
+
+static void theta_q_all_in_one(or_ctx *c, or_state *s, int which);
 
 void or_step(or_ctx *c, or_state *s) {
   const int nz = c->nz, nx = c->nx, ny = c->ny, ld = c->ld;
@@ -1217,8 +1278,15 @@ void or_step(or_ctx *c, or_state *s) {
   memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
   memcpy(s->RHSy_f, s->RHSy, n3 * sizeof(double));
   memcpy(s->RHSz_f, s->RHSz, n3 * sizeof(double));
-  /* obukhov, neutral: psi=0, phi=1 (scalars_module.f90:642-649) */
-  for (size_t i = 0; i < (size_t)nx * ny; i++) { s->psi_m[i] = 0.0; s->psi_m0[i] = 0.0; s->phi_m[i] = 1.0; }
+  const int scal = c->S.enable;
+  if (scal) {
+    or_obukhov(c, s->u, s->v, s->theta, s->qmix, s->sgs_t3, s->sgs_q3, s->psi_m, s->psi_m0,
+               s->psi_h, s->psi_h0, s->phi_m, s->phi_h, s->ustar_avg);
+  } else {
+    /* obukhov with theta = 0: L is NaN, the neutral branch gives psi=0, phi=1
+       (scalars_module.f90:642-649) */
+    for (size_t i = 0; i < (size_t)nx * ny; i++) { s->psi_m[i] = 0.0; s->psi_m0[i] = 0.0; s->phi_m[i] = 1.0; }
+  }
   double **g = s->grad;
   or_filt_da(c, s->u, g[0], g[1], nz);
   or_filt_da(c, s->v, g[3], g[4], nz);
@@ -1227,7 +1295,16 @@ void or_step(or_ctx *c, or_state *s) {
   or_ddz_uv(c, g[5], s->v);
   or_ddz_w(c, g[8], s->w);
   or_wallstress(c, s->u, s->v, g[2], g[5], s->tau[2], s->tau[4], s->psi_m, s->psi_m0, s->phi_m, s->ustar_avg);
-  or_sgs_stag(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
+  if (scal && c->S.model == 7)
+    or_sgs_stag_scalar(c, (const double *const *)g, s->u, s->v, s->w, s->theta, s->tau, &s->sgs, &s->ssgs);
+  else
+    or_sgs_stag(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
+  if (scal) {                          /* main.f90:273-276 */
+    theta_q_all_in_one(c, s, 0);
+    theta_q_all_in_one(c, s, 1);
+  } else {
+    memset(s->beta_scal, 0, n3 * sizeof(double));
+  }
   or_divstress_uv(c, s->divtx, s->tau[0], s->tau[1], s->tau[2]);
   or_divstress_uv(c, s->divty, s->tau[1], s->tau[3], s->tau[4]);
   or_divstress_w(c, s->divtz, s->tau[2], s->tau[4], s->tau[5]);
@@ -1237,6 +1314,8 @@ void or_step(or_ctx *c, or_state *s) {
     s->RHSy[i] = -s->RHSy[i] - s->divty[i];
     s->RHSz[i] = -s->RHSz[i] - s->divtz[i];
   }
+  if (scal && !c->S.passive)           /* main.f90:312-317 */
+    for (size_t i = 0; i < nv; i++) s->RHSz[i] = s->RHSz[i] + s->beta_scal[i];
   const double force = c->P.force;
   if (s->jt == 1 && !c->P.initu) {
     memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
@@ -1268,3 +1347,464 @@ void or_step(or_ctx *c, or_state *s) {
 
 void or_set_time(or_ctx *c, int jt, int jt_total) { c->jt = jt; c->jt_total = jt_total; }
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init) { c->flm_init = flm_init; c->fqn_init = fqn_init; }
+
+/* ================================================================== */
+/* Scalars: potential temperature (theta) and humidity (q), lbc=1     */
+/* ================================================================== */
+void or_set_scalars(or_ctx *c, const or_scalar_params *p, const double *wt_s, const double *wq_s) {
+  const size_t n2 = (size_t)c->nx * c->ny;
+  c->S = *p;
+  if (!c->wt_s) c->wt_s = (double *)calloc(n2, sizeof(double));
+  if (!c->wq_s) c->wq_s = (double *)calloc(n2, sizeof(double));
+  for (size_t i = 0; i < n2; i++) { c->wt_s[i] = wt_s ? wt_s[i] : 0.0; c->wq_s[i] = wq_s ? wq_s[i] : 0.0; }
+}
+
+void or_set_scalar_lag_init(or_ctx *c, int glm_init, int gqn_init) { c->glm_init = glm_init; c->gqn_init = gqn_init; }
+
+/* psi_m of the unstable Businger-Dyer form, scalars_module.f90:621-622 */
+static double mo_psi_m_unstable(double x) {
+  return 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + PI / 2.0;
+}
+
+/* obukhov, scalars_module.f90:508-690 (patch_flag=1) */
+void or_obukhov(or_ctx *c, const double *u, const double *v, const double *theta,
+                const double *qmix, double *sgs_t3, double *sgs_q3, double *psi_m,
+                double *psi_m0, double *psi_h, double *psi_h0, double *phi_m,
+                double *phi_h, double *ustar_avg) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld;
+  const size_t pl = (size_t)ld * ny, n2 = (size_t)nx * ny;
+  const or_scalar_params *S = &c->S;
+  const double dz = c->dz, z0 = c->P.z0, z0s = c->P.z0 / 7.39;
+  if (c->jt == 1) {                                         /* :521-538, SCAL_init = 1 */
+    for (size_t i = 0; i < n2; i++) { psi_m[i] = 0.0; psi_m0[i] = 0.0; psi_h[i] = 0.0; psi_h0[i] = 0.0; }
+    if (!S->initsc)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          sgs_t3[x + (size_t)ld * y] = c->wt_s[x + (size_t)nx * y] / (S->u_star * S->T_scale);
+          sgs_q3[x + (size_t)ld * y] = c->wq_s[x + (size_t)nx * y] / (S->u_star * S->q_scale);
+        }
+  }
+  const double g_ = (S->g * S->z_i) / (S->u_star * S->u_star);                 /* :541 */
+  double tsum = 0.0;
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) tsum += theta[x + (size_t)ld * y];
+  const double theta_ = tsum / (double)(float)(nx * ny);                        /* :556 */
+  double *u1 = (double *)malloc(sizeof(double) * pl), *v1 = (double *)malloc(sizeof(double) * pl);
+  memcpy(u1, u, sizeof(double) * pl); memcpy(v1, v, sizeof(double) * pl);
+  or_test_filter(c, u1, c->Gtt);                                                /* :560-563 */
+  or_test_filter(c, v1, c->Gtt);
+  for (int y = 0; y < ny; y++)                                                  /* :565-573 */
+    for (int x = 0; x < nx; x++) {
+      const size_t i = x + (size_t)ld * y, j = x + (size_t)nx * y;
+      const double ua = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
+      ustar_avg[j] = ua * VONK / (log(0.5 * dz / z0) - psi_m[j] + psi_m0[j]);
+    }
+  for (int y = 0; y < ny; y++)                                                  /* :577-657 */
+    for (int x = 0; x < nx; x++) {
+      const size_t i = x + (size_t)ld * y, j = x + (size_t)nx * y;
+      const double wt = sgs_t3[i];
+      const double wtv = sgs_t3[i] + 0.61 * theta[i] * (0.001 * S->q_scale * sgs_q3[i]);
+      const double us = ustar_avg[j];
+      const double L = -(us * us * us) / (VONK * (g_ / theta_) * wtv);
+      double zeta = 0.5 * dz / L, zeta0 = z0 / L, zeta0s = z0s / L;
+      if (zeta < -30.0) zeta = -30.0;
+      if (zeta > 2.0) zeta = 2.0;
+      if (zeta0 < -30.0) zeta0 = -30.0;
+      if (zeta0 > 2.0) zeta0 = 2.0;
+      if (zeta0s < -30.0) zeta0s = -30.0;
+      if (zeta0s > 2.0) zeta0s = 2.0;
+      if (L < 0.0 && wt != 0.0) {
+        const double xx = pow(1.0 - 16.0 * zeta, 0.25);
+        psi_m[j] = mo_psi_m_unstable(xx);
+        psi_h[j] = 2.0 * log((1.0 + xx * xx) / 2.0);
+        phi_m[j] = 1.0 / xx;
+        phi_h[j] = 1.0 / (xx * xx);
+        const double x0 = pow(1.0 - 16.0 * zeta0, 0.25), x0s = pow(1.0 - 16.0 * zeta0s, 0.25);
+        psi_m0[j] = mo_psi_m_unstable(x0);
+        psi_h0[j] = 2.0 * log((1.0 + x0s * x0s) / 2.0);
+      } else if (L > 0.0 && wt != 0.0) {
+        phi_m[j] = 1.0 + 5.0 * zeta;
+        phi_h[j] = 1.0 + 5.0 * zeta;
+        psi_m[j] = -1.0 * 5.0 * zeta;
+        psi_h[j] = -1.0 * 5.0 * zeta;
+        psi_m0[j] = -1.0 * 5.0 * zeta0;
+        psi_h0[j] = -1.0 * 5.0 * zeta0s;
+      } else {
+        psi_m[j] = 0.0; psi_m0[j] = 0.0; psi_h[j] = 0.0; psi_h0[j] = 0.0;
+        phi_m[j] = 1.0; phi_h[j] = 1.0;
+      }
+    }
+  free(u1); free(v1);
+}
+
+/* calcbeta, scalars_module.f90:175-228 (non-MPI: k = 2..nz-1) */
+void or_calcbeta(or_ctx *c, const double *theta, const double *qmix, double *beta) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const or_scalar_params *S = &c->S;
+  const double g_hat = S->g * (S->z_i / (S->u_star * S->u_star));
+  memset(beta, 0, sizeof(double) * pl * nz);
+  double *bar = (double *)calloc(nz, sizeof(double));
+  for (int k = 0; k < nz; k++) {
+    double a = 0.0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = pl * k + x + (size_t)ld * y;
+        a = a + theta[i] * (1.0 + 0.61 * 0.001 * qmix[i] * S->q_scale);
+      }
+    bar[k] = a / (double)(nx * ny);
+  }
+  for (int k = 1; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = pl * k + x + (size_t)ld * y;
+        const double above = (theta[i] * (1.0 + 0.61 * 0.001 * qmix[i] * S->q_scale) - bar[k]) / bar[k];
+        const double below = (theta[i - pl] * (1.0 + 0.61 * 0.001 * qmix[i - pl] * S->q_scale) - bar[k - 1]) / bar[k - 1];
+        beta[i] = g_hat * (above + below) / 2.0;
+      }
+  free(bar);
+}
+
+/* surface flux of scalar `which` (lbc=1): scalars_module.f90:259-266,
+   scalars_module_q.f90:139-145.  The reference applies it only while
+   scalar(1,1,1) < 2 and leaves it undefined otherwise; here it always applies. */
+static double surf_flux_of(const or_ctx *c, int which, size_t j) {
+  const or_scalar_params *S = &c->S;
+  return which == 0 ? c->wt_s[j] / (S->T_scale * S->u_star) : c->wq_s[j] / S->q_scale / S->u_star;
+}
+
+/* scalar_RHS_calc, scalars_module.f90:232-454 / scalars_module_q.f90:114-330.
+   Level nz of RHS is not defined by the reference (temp(nz) is never set);
+   here it is 0. */
+void or_scalar_rhs_calc(or_ctx *c, int which, const double *dsdx, const double *dsdy,
+                        double *dsdz, const double *u, const double *v, const double *w,
+                        const double *Nu_t, const double *Pr_, const double *ustar_avg,
+                        const double *phi_h, double *RHS, double *sgs_vert1) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz, nx2 = c->nx2, ny2 = c->ny2;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz, plb = (size_t)c->ld_big * ny2, nb = plb * nz;
+  const double dz = c->dz;
+  for (int y = 0; y < ny; y++)                                                   /* :278-283 */
+    for (int x = 0; x < nx; x++) {
+      const size_t i = x + (size_t)ld * y, j = x + (size_t)nx * y;
+      dsdz[i] = -phi_h[j] * surf_flux_of(c, which, j) / (ustar_avg[j] * VONK * dz / 2.0);
+    }
+  double *um = (double *)calloc(nb, sizeof(double)), *vm = (double *)calloc(nb, sizeof(double));
+  double *wm = (double *)calloc(nb, sizeof(double)), *sxm = (double *)calloc(nb, sizeof(double));
+  double *sym = (double *)calloc(nb, sizeof(double)), *szm = (double *)calloc(nb, sizeof(double));
+  double *Rm = (double *)calloc(nb, sizeof(double));
+  or_dealias1(c, u, um, nz); or_dealias1(c, v, vm, nz); or_dealias1(c, w, wm, nz);   /* :286-291 */
+  or_dealias1(c, dsdx, sxm, nz); or_dealias1(c, dsdy, sym, nz); or_dealias1(c, dsdz, szm, nz);
+  for (int k = 0; k < nz; k++)                                                   /* :322-349 */
+    for (int y = 0; y < ny2; y++)
+      for (int x = 0; x < nx2; x++) {
+        const size_t i = plb * k + x + (size_t)c->ld_big * y;
+        if (k == 0)
+          Rm[i] = um[i] * sxm[i] + vm[i] * sym[i] + (0.5 * wm[i + plb]) * szm[i + plb];
+        else if (k < nz - 1)
+          Rm[i] = um[i] * sxm[i] + vm[i] * sym[i] + (wm[i] * szm[i] + wm[i + plb] * szm[i + plb]) / 2.0;
+        else
+          Rm[i] = um[i] * sxm[i] + vm[i] * sym[i];
+      }
+  or_dealias2(c, RHS, Rm, nz);                                                   /* :352 */
+  double *temp = (double *)calloc(n3, sizeof(double)), *dtemp = (double *)calloc(n3, sizeof(double));
+  for (int pass = 0; pass < 2; pass++) {                                         /* :366-432 */
+    const double *ds = pass == 0 ? dsdx : dsdy;
+    memset(temp, 0, sizeof(double) * n3);
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = x + (size_t)ld * y;
+        temp[i] = (1.0 / Pr_[i]) * Nu_t[i] * ds[i];
+        for (int k = 1; k < nz - 1; k++) {
+          const size_t q = pl * k + i;
+          temp[q] = (1.0 / (0.5 * (Pr_[q] + Pr_[q + pl]))) * 0.5 * (Nu_t[q] + Nu_t[q + pl]) * ds[q];
+        }
+      }
+    if (pass == 0) or_ddx(c, dtemp, temp, nz); else or_ddy(c, dtemp, temp, nz);
+    for (int k = 0; k < (pass == 0 ? nz - 1 : nz); k++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const size_t q = pl * k + x + (size_t)ld * y;
+          RHS[q] = pass == 0 ? (-1.0 * RHS[q] + dtemp[q]) : RHS[q] + dtemp[q];
+        }
+  }
+  memset(temp, 0, sizeof(double) * n3);                                          /* :436-452 */
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      const size_t i = x + (size_t)ld * y, j = x + (size_t)nx * y;
+      const double sf = surf_flux_of(c, which, j);
+      temp[i] = -1.0 * sf;
+      sgs_vert1[i] = sf;
+      for (int k = 1; k < nz; k++) {
+        const size_t q = pl * k + i;
+        temp[q] = (1.0 / Pr_[q]) * Nu_t[q] * dsdz[q];
+      }
+    }
+  or_ddz_w(c, dtemp, temp);
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t q = pl * k + x + (size_t)ld * y;
+        RHS[q] = RHS[q] + dtemp[q];
+      }
+  memset(RHS + pl * (nz - 1), 0, sizeof(double) * pl);
+  free(temp); free(dtemp);
+  free(um); free(vm); free(wm); free(sxm); free(sym); free(szm); free(Rm);
+}
+
+/* value added at the lid per level (step_scalar :491-502): the lid
+   gradient applies when the domain is taller than z_i */
+static double lid_gradient(const or_ctx *c, int which) {
+  const or_scalar_params *S = &c->S;
+  return which == 0 ? S->dTdz_top / S->T_scale * S->z_i : S->dqdz_top / S->q_scale * S->z_i;
+}
+
+/* step_scalar, scalars_module.f90:457-504 / step_scalar_q */
+void or_step_scalar(or_ctx *c, int which, double *sc, const double *R, const double *Rf) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double dt = c->P.dt;
+  for (int k = 0; k < nz - 1; k++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = pl * k + x + (size_t)ld * y;
+        sc[i] = sc[i] + dt * (1.5 * R[i] - 0.5 * Rf[i]);
+      }
+  const size_t top = pl * (nz - 1);
+  if (c->P.Lz > 1.0) {
+    const double gz = lid_gradient(c, which) * c->dz;
+    for (size_t i = 0; i < pl; i++) sc[top + i] = sc[top - pl + i] + gz;
+  } else {
+    for (size_t i = 0; i < pl; i++) sc[top + i] = sc[top - pl + i];
+  }
+}
+
+/* theta_all_in_one (scalars_module.f90:57-171, which=0) and q_all_in_one
+   (scalars_module_q.f90:31-111, which=1); lbc=1, ubc=0, jt >= SCAL_init */
+static void theta_q_all_in_one(or_ctx *c, or_state *s, int which) {
+  const int nz = c->nz;
+  const size_t pl = (size_t)c->ld * c->ny, n3 = pl * nz;
+  double *sc = which ? s->qmix : s->theta;
+  double *dx = which ? s->dqdx : s->dTdx, *dy = which ? s->dqdy : s->dTdy, *dzf = which ? s->dqdz : s->dTdz;
+  double *R = which ? s->RHS_q : s->RHS_T, *Rf = which ? s->RHS_qf : s->RHS_Tf;
+  double *Pr_ = (double *)malloc(sizeof(double) * n3);
+  for (size_t i = 0; i < n3; i++) Pr_[i] = c->S.model == 7 ? s->ssgs.Pr_t[i] : c->S.Pr;   /* :73-81 */
+  or_filt_da(c, sc, dx, dy, nz);                                                  /* :87-88 */
+  or_ddz_uv(c, dzf, sc);
+  const double top = lid_gradient(c, which);
+  for (size_t i = 0; i < pl; i++) dzf[pl * (nz - 1) + i] = top;                   /* :90-92 */
+  memcpy(Rf, R, sizeof(double) * n3);                                             /* :121 */
+  or_scalar_rhs_calc(c, which, dx, dy, dzf, s->u, s->v, s->w, s->sgs.Nu_t, Pr_, s->ustar_avg, s->phi_h,
+                     R, which ? s->sgs_q3 : s->sgs_t3);
+  if (which == 0) or_calcbeta(c, s->theta, s->qmix, s->beta_scal);               /* :154 */
+  if (c->jt == 1 && !c->S.initsc) memcpy(Rf, R, sizeof(double) * n3);           /* :157-159 */
+  or_step_scalar(c, which, sc, R, Rf);
+  free(Pr_);
+}
+
+/* lagrange_scalar_Sdep.f90:1-467 (non-MPI, lbc=1, no inflow) */
+void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *u,
+                             const double *v, const double *w, double *theta,
+                             or_sgs_state *m, or_scalar_sgs *g) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  const or_scalar_params *P = &c->S;
+  const double delta = c->delta, tf1 = 2.0, tf2 = 4.0;
+  const double opftdelta = 1.5 * delta, powcoeff = -1.0 / 8.0;
+  const double e24 = (double)F1EM24;
+  const double expo = log(tf1) / (log(tf2) - log(tf1));
+  double *theta_t = (double *)malloc(sizeof(double) * n3);
+  memcpy(theta_t, theta, sizeof(double) * n3);                                    /* :84-86 */
+  or_interpolag_scalar_sdep(c, g);                                                /* :88 */
+  const double lagran_dt = c->P.dt * (double)c->P.cs_count;
+  const double cst = delta * delta;
+  double *dTdx = (double *)calloc(n3, sizeof(double)), *dTdy = (double *)calloc(n3, sizeof(double));
+  double *dTdz = (double *)calloc(n3, sizeof(double));
+  or_filt_da(c, theta, dTdx, dTdy, nz);                                           /* :95-99 */
+  or_ddz_uv(c, dTdz, theta);
+  const double top = lid_gradient(c, 0);
+  for (size_t i = 0; i < pl; i++) dTdz[pl * (nz - 1) + i] = top;
+  {                                                                               /* :108-136 */
+    double slog = 0.0;
+    for (int i = 0; i < nx * ny; i++) slog += log(c->P.z0);
+    const double zo_avg = exp(slog / (double)(float)(nx * ny));
+    double su = 0.0, spsi = 0.0, swt = 0.0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) su += sqrt(u[x + (size_t)ld * y] * u[x + (size_t)ld * y] + v[x + (size_t)ld * y] * v[x + (size_t)ld * y]);
+    for (int j = 0; j < nx * ny; j++) spsi += g->psi_m[j];
+    for (int j = 0; j < nx * ny; j++) swt += c->wt_s[j];
+    const double ustar_ = su / (double)(float)(nx * ny) * VONK /
+                          (log(0.5 * c->dz / zo_avg) - spsi / (double)(float)(nx * ny));
+    const double sflux = (swt / (double)(nx * ny)) / P->T_scale / P->u_star;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++)
+        dTdz[x + (size_t)ld * y] = -g->phi_h[x + (size_t)nx * y] * sflux / (ustar_ * VONK * c->dz / 2.0);
+  }
+  enum { UB, VB, WB, TB, UH, VH, WH, TH, L1, L2, L3, Q1, Q2, Q3,
+         SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
+         SXB, SYB, SZB, SXH, SYH, SZH, DXB, DYB, DZB, DXH, DYH, DZH, SMAG, NT };
+  double *T[NT];
+  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+  const int jt = c->jt;
+  for (int k = 0; k < nz; k++) {                                                  /* :138-398 */
+    const size_t o = pl * k;
+    for (size_t i = 0; i < pl; i++) {
+      double ub, vb, wb, tb;
+      if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; tb = theta_t[i]; }
+      else {
+        ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]);
+        wb = w[o + i]; tb = 0.5 * (theta_t[o + i] + theta_t[o - pl + i]);
+      }
+      T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb; T[TB][i] = tb;
+      T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb; T[TH][i] = tb;
+      T[L1][i] = ub * tb; T[L2][i] = vb * tb; T[L3][i] = wb * tb;
+      T[Q1][i] = ub * tb; T[Q2][i] = vb * tb; T[Q3][i] = wb * tb;
+    }
+    for (int q = 0; q < 4; q++) or_test_filter(c, T[UB + q], c->Gt);
+    for (int q = 0; q < 3; q++) {
+      or_test_filter(c, T[L1 + q], c->Gt);
+      for (size_t i = 0; i < pl; i++) T[L1 + q][i] = T[L1 + q][i] - T[UB + q][i] * T[TB][i];
+    }
+    for (int q = 0; q < 4; q++) or_test_filter(c, T[UH + q], c->Gtt);
+    for (int q = 0; q < 3; q++) {
+      or_test_filter(c, T[Q1 + q], c->Gtt);
+      for (size_t i = 0; i < pl; i++) T[Q1 + q][i] = T[Q1 + q][i] - T[UH + q][i] * T[TH][i];
+    }
+    for (size_t i = 0; i < pl; i++) {
+      double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+      double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+      T[SMAG][i] = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+      for (int q = 0; q < 6; q++) { T[SB11 + q][i] = S[q][o + i]; T[SH11 + q][i] = S[q][o + i]; }
+    }
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
+    for (size_t i = 0; i < pl; i++) {
+      const double Sm = T[SMAG][i];
+      double sx, sy, sz, dxv, dyv, dzv;
+      if (k == 0) {
+        sx = Sm * dTdx[i]; sy = Sm * dTdy[i]; sz = Sm * (dTdz[i] + dTdz[i + pl]) / 2.0;
+        dxv = dTdx[i]; dyv = dTdy[i]; dzv = (dTdz[i] + dTdz[i + pl]) / 2.0;
+      } else {
+        sx = Sm * (dTdx[o + i] + dTdx[o - pl + i]) / 2.0;
+        sy = Sm * (dTdy[o + i] + dTdy[o - pl + i]) / 2.0;
+        sz = Sm * dTdz[o + i];
+        dxv = (dTdx[o + i] + dTdx[o - pl + i]) / 2.0;
+        dyv = (dTdy[o + i] + dTdy[o - pl + i]) / 2.0;
+        dzv = dTdz[o + i];
+      }
+      T[SXB][i] = sx; T[SYB][i] = sy; T[SZB][i] = sz; T[SXH][i] = sx; T[SYH][i] = sy; T[SZH][i] = sz;
+      T[DXB][i] = dxv; T[DYB][i] = dyv; T[DZB][i] = dzv; T[DXH][i] = dxv; T[DYH][i] = dyv; T[DZH][i] = dzv;
+    }
+    for (int q = 0; q < 3; q++) { or_test_filter(c, T[SXB + q], c->Gt); or_test_filter(c, T[SXH + q], c->Gtt); }
+    for (int q = 0; q < 3; q++) { or_test_filter(c, T[DXB + q], c->Gt); or_test_filter(c, T[DXH + q], c->Gtt); }
+    double *GLM = g->G_LM + o, *GMM = g->G_MM + o, *GQN = g->G_QN + o, *GNN = g->G_NN + o;
+    const int init_lm = c->P.inilag && !c->glm_init && (jt == c->P.cs_count || jt == 1);
+    const int init_qn = c->P.inilag && !c->gqn_init && (jt == c->P.cs_count || jt == 1);
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < ld; x++) {
+        const size_t i = x + (size_t)ld * y;
+        const double Sb = sqrt(2.0 * (T[SB11][i] * T[SB11][i] + T[SB22][i] * T[SB22][i] + T[SB33][i] * T[SB33][i] +
+                                      2.0 * (T[SB12][i] * T[SB12][i] + T[SB13][i] * T[SB13][i] + T[SB23][i] * T[SB23][i])));
+        const double Sh = sqrt(2.0 * (T[SH11][i] * T[SH11][i] + T[SH22][i] * T[SH22][i] + T[SH33][i] * T[SH33][i] +
+                                      2.0 * (T[SH12][i] * T[SH12][i] + T[SH13][i] * T[SH13][i] + T[SH23][i] * T[SH23][i])));
+        double M[3], N[3];
+        for (int q = 0; q < 3; q++) {                                             /* :268-274 */
+          M[q] = cst * (T[SXB + q][i] - 4.0 * Sb * T[DXB + q][i]);
+          N[q] = cst * (T[SXH + q][i] - 16.0 * Sh * T[DXH + q][i]);
+        }
+        const double LM = T[L1][i] * M[0] + T[L2][i] * M[1] + T[L3][i] * M[2];  /* :284-287 */
+        const double MM = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
+        const double QN = T[Q1][i] * N[0] + T[Q2][i] * N[1] + T[Q3][i] * N[2];
+        const double NN = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+        if (init_lm) {                                                            /* :289-299 */
+          GMM[i] = MM; GLM[i] = 0.03 * MM;
+          if (x >= ld - 2) { GMM[i] = 1.0; GLM[i] = 1.0; }
+        }
+        double Tn = m->TnLMMM[o + i];                                              /* :318-331 */
+        Tn = opftdelta * pow(Tn, powcoeff);
+        Tn = fmax(e24, fq(Tn));
+        double dumfac = lagran_dt / Tn;
+        double epsi = dumfac / (1.0 + dumfac);
+        GLM[i] = (epsi * LM + (1.0 - epsi) * GLM[i]);
+        GMM[i] = (epsi * MM + (1.0 - epsi) * GMM[i]);
+        GLM[i] = fmax(e24, fq(GLM[i]));
+        double c2 = GLM[i] / GMM[i];
+        if (x >= ld - 2) c2 = e24;
+        c2 = fmax(e24, fq(c2));
+        if (init_qn) {                                                            /* :338-348 */
+          GNN[i] = NN; GQN[i] = 0.03 * NN;
+          if (x >= ld - 2) { GNN[i] = 1.0; GQN[i] = 1.0; }
+        }
+        Tn = m->TnQNNN[o + i];                                                     /* :350-363 */
+        Tn = opftdelta * pow(Tn, powcoeff);
+        Tn = fmax(e24, fq(Tn));
+        dumfac = lagran_dt / Tn;
+        epsi = dumfac / (1.0 + dumfac);
+        GQN[i] = (epsi * QN + (1.0 - epsi) * GQN[i]);
+        GNN[i] = (epsi * NN + (1.0 - epsi) * GNN[i]);
+        GQN[i] = fmax(e24, fq(GQN[i]));
+        double c4 = GQN[i] / GNN[i];
+        if (x >= ld - 2) c4 = e24;
+        c4 = fmax(e24, fq(c4));
+        double beta = pow(c4 / c2, expo);                                          /* :370 */
+        if (k == nz - 1) beta = 1.0;                                              /* :379-382 */
+        m->Beta[o + i] = beta;
+        const double bclip = fmax(fq(beta), fq(1.0 / (tf1 * tf2)));               /* :384-389 */
+        double cpr = c2 / bclip;
+        if (x >= ld - 2) cpr = e24;
+        cpr = fmax(e24, fq(cpr));
+        g->Cs2_Pr[o + i] = cpr;
+        g->Pr_t[o + i] = m->Cs_opt2[o + i] / cpr;                                  /* :395 */
+      }
+    if (init_lm && k == nz - 1) c->glm_init = 1;
+    if (init_qn && k == nz - 1) c->gqn_init = 1;
+  }
+  for (int t = 0; t < NT; t++) free(T[t]);
+  free(theta_t); free(dTdx); free(dTdy); free(dTdz);
+}
+
+/* seeded convective initial condition for theta and q (project IC, not a
+   reference routine): theta = T_init below z_inv, + inv_strength (K/m)
+   above, uniform perturbation of amplitude amp_K below z_inv; q uniform */
+void or_state_init_scalar_ic(or_ctx *c, or_state *s, uint64_t seed, double T_init,
+                             double z_inv, double inv_strength, double amp_K, double q_init) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz, n2 = (size_t)nx * ny;
+  const or_scalar_params *S = &c->S;
+  memset(s->theta, 0, n3 * sizeof(double)); memset(s->qmix, 0, n3 * sizeof(double));
+  for (int k = 0; k < nz - 1; k++) {
+    const double z = (k + 0.5) * c->dz;
+    const double tk = T_init + (z > z_inv ? inv_strength * (z - z_inv) * S->z_i : 0.0);
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = pl * k + x + (size_t)ld * y;
+        double t = tk;
+        if (z < z_inv) {
+          const uint64_t idx = (((uint64_t)3 * nz + k) * ny + y) * nx + x;
+          t += amp_K * (2.0 * or_uniform(seed, idx) - 1.0);
+        }
+        s->theta[i] = t / S->T_scale;
+        s->qmix[i] = q_init / S->q_scale;
+      }
+  }
+  const size_t top = pl * (nz - 1);
+  for (size_t i = 0; i < pl; i++) {
+    const int x = (int)(i % ld);
+    if (x >= nx) continue;
+    s->theta[top + i] = s->theta[top - pl + i] + (c->P.Lz > 1.0 ? lid_gradient(c, 0) * c->dz : 0.0);
+    s->qmix[top + i] = s->qmix[top - pl + i] + (c->P.Lz > 1.0 ? lid_gradient(c, 1) * c->dz : 0.0);
+  }
+  memset(s->RHS_T, 0, n3 * sizeof(double)); memset(s->RHS_Tf, 0, n3 * sizeof(double));
+  memset(s->RHS_q, 0, n3 * sizeof(double)); memset(s->RHS_qf, 0, n3 * sizeof(double));
+  memset(s->beta_scal, 0, n3 * sizeof(double));
+  memset(s->dTdx, 0, n3 * sizeof(double)); memset(s->dTdy, 0, n3 * sizeof(double)); memset(s->dTdz, 0, n3 * sizeof(double));
+  memset(s->dqdx, 0, n3 * sizeof(double)); memset(s->dqdy, 0, n3 * sizeof(double)); memset(s->dqdz, 0, n3 * sizeof(double));
+  memset(s->sgs_t3, 0, pl * sizeof(double)); memset(s->sgs_q3, 0, pl * sizeof(double));
+  for (size_t j = 0; j < n2; j++) { s->psi_h[j] = 0.0; s->psi_h0[j] = 0.0; s->phi_h[j] = 1.0; }
+  or_scalar_sgs *h = &s->ssgs;
+  memset(h->G_LM, 0, n3 * sizeof(double)); memset(h->G_MM, 0, n3 * sizeof(double));
+  memset(h->G_QN, 0, n3 * sizeof(double)); memset(h->G_NN, 0, n3 * sizeof(double));
+  memset(h->u_lag_s, 0, n3 * sizeof(double)); memset(h->v_lag_s, 0, n3 * sizeof(double));
+  memset(h->w_lag_s, 0, n3 * sizeof(double)); memset(h->Cs2_Pr, 0, n3 * sizeof(double));
+  for (size_t i = 0; i < n3; i++) h->Pr_t[i] = S->Pr;
+  c->glm_init = 0; c->gqn_init = 0;
+}

@@ -96,6 +96,66 @@ void or_sgs_stag(or_ctx *c, const double *const grad[9], const double *u,
                  const double *v, const double *w, double *const tau[6],
                  or_sgs_state *s);
 
+/* ---- scalars (S_FLAG=.true.): scalars_module.f90, scalars_module_q.f90,
+   lagrange_scalar_Sdep.f90, interpolag_scalar_Sdep.f90, bottombc.f90 ---- */
+/* param.f90:53,72-90,139-146,174 and the lbc=1 / patch_flag=1 surface of
+   bottombc.f90:26-63 */
+typedef struct or_scalar_params {
+  int    enable;     /* S_FLAG: theta_all_in_one + q_all_in_one every step */
+  int    model;      /* 7: Pr_ = Pr_t from lagrange_scalar_Sdep; else Pr_ = Pr */
+  int    initsc;     /* scalar history given: no Euler scalar step at jt==1 */
+  int    passive;    /* passive_scalar: no buoyancy in RHSz */
+  double Pr;         /* 0.55 */
+  double z_i, u_star, T_scale, q_scale, g;
+  double dTdz_top;   /* K/m at the lid */
+  double dqdz_top;   /* (g/kg)/m at the lid */
+} or_scalar_params;
+
+/* wt_s (K m/s) and wq_s ((g/kg) m/s): (nx,ny) kinematic surface fluxes,
+   i.e. after the unit conversion of bottombc.f90:57-58.  z_os = z0/7.39. */
+void or_set_scalars(or_ctx *c, const or_scalar_params *p, const double *wt_s, const double *wq_s);
+/* lagrange_scalar_Sdep.f90:65-66 SAVE flags */
+void or_set_scalar_lag_init(or_ctx *c, int glm_init, int gqn_init);
+
+/* Scalar Lagrangian SGS state, sgsmodule.f90 (G_*, u_lag_s, Pr_t) plus the
+   surface functions the scalar model reads (bottombc phi_h, psi_m). */
+typedef struct or_scalar_sgs {
+  double *G_LM, *G_MM, *G_QN, *G_NN;       /* (ld,ny,nz) */
+  double *u_lag_s, *v_lag_s, *w_lag_s;     /* (ld,ny,nz) */
+  double *Pr_t, *Cs2_Pr;                   /* (ld,ny,nz) */
+  double *phi_h, *psi_m;                   /* (nx,ny) */
+} or_scalar_sgs;
+
+/* obukhov, scalars_module.f90:508-690.  theta/qmix: (ld,ny,nz) (level 1 read);
+   sgs_t3/sgs_q3: the level-1 plane (ld,ny); the rest (nx,ny). */
+void or_obukhov(or_ctx *c, const double *u, const double *v, const double *theta,
+                const double *qmix, double *sgs_t3, double *sgs_q3, double *psi_m,
+                double *psi_m0, double *psi_h, double *psi_h0, double *phi_m,
+                double *phi_h, double *ustar_avg);
+/* calcbeta, scalars_module.f90:175-228 -> beta_scal (ld,ny,nz) */
+void or_calcbeta(or_ctx *c, const double *theta, const double *qmix, double *beta_scal);
+/* scalar_RHS_calc, scalars_module.f90:232-454 (which=0, theta) and
+   scalars_module_q.f90:114-330 (which=1, q); lbc=1.  dsdz level 1 is
+   overwritten with the surface gradient; sgs_vert1 = level-1 plane of
+   sgs_t3/sgs_q3 (ld,ny). */
+void or_scalar_rhs_calc(or_ctx *c, int which, const double *dsdx, const double *dsdy,
+                        double *dsdz, const double *u, const double *v, const double *w,
+                        const double *Nu_t, const double *Pr_, const double *ustar_avg,
+                        const double *phi_h, double *RHS, double *sgs_vert1);
+/* step_scalar, scalars_module.f90:457-504 / step_scalar_q */
+void or_step_scalar(or_ctx *c, int which, double *scalar, const double *RHS, const double *RHS_f);
+/* interpolag_scalar_Sdep.f90:1-316 */
+void or_interpolag_scalar_sdep(or_ctx *c, or_scalar_sgs *g);
+/* lagrange_scalar_Sdep.f90:1-467; theta is Nyquist-filtered in place
+   (filt_da at :90); reads TnLMMM, TnQNNN, Cs_opt2 of m, overwrites m->Beta */
+void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *u,
+                             const double *v, const double *w, double *theta,
+                             or_sgs_state *m, or_scalar_sgs *g);
+/* sgs_stag with model=7 (sgs_stag.f90:59-69,150-153,166-173) */
+void or_sgs_stag_scalar(or_ctx *c, const double *const grad[9], const double *u,
+                        const double *v, const double *w, double *theta,
+                        double *const tau[6], or_sgs_state *s, or_scalar_sgs *g);
+
 /* ---- divstress_uv.f90, divstress_w.f90 ---- */
 void or_divstress_uv(or_ctx *c, double *divt, const double *tx, const double *ty,
                      const double *tz);
@@ -131,6 +191,13 @@ typedef struct or_state {
   double *psi_m, *psi_m0, *phi_m, *ustar_avg;  /* (nx,ny) */
   or_sgs_state sgs;
   int jt, jt_total;
+  /* scalars (allocated always, used when or_set_scalars enabled them) */
+  double *theta, *qmix;                    /* (ld,ny,nz) */
+  double *dTdx, *dTdy, *dTdz, *dqdx, *dqdy, *dqdz;
+  double *RHS_T, *RHS_Tf, *RHS_q, *RHS_qf, *beta_scal;
+  double *sgs_t3, *sgs_q3;                 /* level-1 plane (ld,ny) */
+  double *psi_h, *psi_h0, *phi_h;          /* (nx,ny) */
+  or_scalar_sgs ssgs;
 } or_state;
 
 or_state *or_state_create(or_ctx *c);
@@ -139,6 +206,10 @@ void      or_state_destroy(or_state *s);
 void or_state_init_ic(or_ctx *c, or_state *s, uint64_t seed, double amp,
                       double ustar, double kappa_ic);
 void or_step(or_ctx *c, or_state *s);
+/* theta (K), inversion (K/m above z_inv, z in units of z_i), uniform
+   perturbation amplitude amp_K where z < z_inv, humidity q_init (g/kg) */
+void or_state_init_scalar_ic(or_ctx *c, or_state *s, uint64_t seed, double T_init,
+                             double z_inv, double inv_strength, double amp_K, double q_init);
 
 /* jt / jt_total seen by routine-level calls (param.f90:147-148) and the
    lagrange_Sdep SAVE init flags (lagrange_Sdep.f90:67-68) */

@@ -37,13 +37,39 @@ class SgsState(C.Structure):
 SgsState._fields_names = staticmethod(lambda: [f[0] for f in SgsState._fields_])
 
 
+class ScalarParams(C.Structure):
+    _fields_ = [("enable", C.c_int), ("model", C.c_int), ("initsc", C.c_int), ("passive", C.c_int),
+                ("Pr", C.c_double), ("z_i", C.c_double), ("u_star", C.c_double), ("T_scale", C.c_double),
+                ("q_scale", C.c_double), ("g", C.c_double), ("dTdz_top", C.c_double), ("dqdz_top", C.c_double)]
+
+
+SCALAR_SGS_NAMES = ("G_LM", "G_MM", "G_QN", "G_NN", "u_lag_s", "v_lag_s", "w_lag_s", "Pr_t", "Cs2_Pr",
+                    "phi_h", "psi_m")
+
+
+class ScalarSgs(C.Structure):
+    _fields_ = [(n, D) for n in SCALAR_SGS_NAMES]
+
+
 class State(C.Structure):
     _fields_ = ([("u", D), ("v", D), ("w", D), ("grad", D * 9),
                  ("RHSx", D), ("RHSy", D), ("RHSz", D), ("RHSx_f", D), ("RHSy_f", D), ("RHSz_f", D),
                  ("tau", D * 6), ("divtx", D), ("divty", D), ("divtz", D),
                  ("p", D), ("dpdx", D), ("dpdy", D), ("dpdz", D),
                  ("psi_m", D), ("psi_m0", D), ("phi_m", D), ("ustar_avg", D),
-                 ("sgs", SgsState), ("jt", C.c_int), ("jt_total", C.c_int)])
+                 ("sgs", SgsState), ("jt", C.c_int), ("jt_total", C.c_int),
+                 ("theta", D), ("qmix", D), ("dTdx", D), ("dTdy", D), ("dTdz", D),
+                 ("dqdx", D), ("dqdy", D), ("dqdz", D),
+                 ("RHS_T", D), ("RHS_Tf", D), ("RHS_q", D), ("RHS_qf", D), ("beta_scal", D),
+                 ("sgs_t3", D), ("sgs_q3", D), ("psi_h", D), ("psi_h0", D), ("phi_h", D),
+                 ("ssgs", ScalarSgs)])
+
+
+def scalar_params(enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_star=1.0, T_scale=400.0,
+                  q_scale=100.0, g=9.81, dTdz_top=0.01, dqdz_top=-0.004):
+    """param.f90:53,72-90,139-146,174 defaults; dTdz_top = the inversion of the
+    convective configuration (0.01 K/m)."""
+    return ScalarParams(enable, model, initsc, passive, Pr, z_i, u_star, T_scale, q_scale, g, dTdz_top, dqdz_top)
 
 
 def build():
@@ -65,6 +91,9 @@ def lib():
         L.or_state_create.argtypes = [vp]
         L.or_state_destroy.argtypes = [C.POINTER(State)]
         L.or_state_init_ic.argtypes = [vp, C.POINTER(State), C.c_uint64, C.c_double, C.c_double, C.c_double]
+        L.or_state_init_scalar_ic.argtypes = [vp, C.POINTER(State), C.c_uint64, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_double]
+        L.or_set_scalars.argtypes = [vp, C.POINTER(ScalarParams), D, D]
         L.or_step.argtypes = [vp, C.POINTER(State)]
         L.or_uniform.restype = C.c_double
         L.or_uniform.argtypes = [C.c_uint64, C.c_uint64]
@@ -126,6 +155,53 @@ class Oracle:
         arrs["Betaclip_avg"] = np.zeros(self.nz)
         st = SgsState(*[_p(arrs[n]) for n in SgsState._fields_names()])
         return st, arrs
+
+    # ---- scalars ----
+    def set_scalars(self, params, wt_s=None, wq_s=None):
+        """params: ScalarParams; wt_s, wq_s: (ny, nx) kinematic surface fluxes."""
+        self.sp = params
+        self._wt = np.ascontiguousarray(np.zeros((self.ny, self.nx)) if wt_s is None else wt_s, dtype=np.float64)
+        self._wq = np.ascontiguousarray(np.zeros((self.ny, self.nx)) if wq_s is None else wq_s, dtype=np.float64)
+        self.L.or_set_scalars(C.c_void_p(self.ctx), C.byref(params), _p(self._wt), _p(self._wq))
+
+    def set_scalar_lag_init(self, glm_init, gqn_init):
+        self.L.or_set_scalar_lag_init(C.c_void_p(self.ctx), int(glm_init), int(gqn_init))
+
+    def scalar_sgs_state(self, phi_h=None, psi_m=None):
+        arrs = {n: self.field() for n in SCALAR_SGS_NAMES[:9]}
+        arrs["phi_h"] = np.ones((self.ny, self.nx)) if phi_h is None else phi_h
+        arrs["psi_m"] = np.zeros((self.ny, self.nx)) if psi_m is None else psi_m
+        st = ScalarSgs(*[_p(arrs[n]) for n in SCALAR_SGS_NAMES])
+        return st, arrs
+
+    def obukhov(self, u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, ustar_avg):
+        self.L.or_obukhov(C.c_void_p(self.ctx), *[_p(a) for a in (u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0,
+                                                                   psi_h, psi_h0, phi_m, phi_h, ustar_avg)])
+
+    def calcbeta(self, theta, qmix, beta):
+        self.L.or_calcbeta(C.c_void_p(self.ctx), _p(theta), _p(qmix), _p(beta))
+
+    def scalar_rhs_calc(self, which, dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg, phi_h, rhs, sgs_vert1):
+        self.L.or_scalar_rhs_calc(C.c_void_p(self.ctx), int(which),
+                                  *[_p(a) for a in (dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg, phi_h, rhs,
+                                                    sgs_vert1)])
+
+    def step_scalar(self, which, scalar, rhs, rhs_f):
+        self.L.or_step_scalar(C.c_void_p(self.ctx), int(which), _p(scalar), _p(rhs), _p(rhs_f))
+
+    def interpolag_scalar_sdep(self, sst):
+        self.L.or_interpolag_scalar_sdep(C.c_void_p(self.ctx), C.byref(sst))
+
+    def lagrange_scalar_sdep(self, S, u, v, w, theta, st, sst):
+        s6 = (D * 6)(*[_p(a) for a in S])
+        self.L.or_lagrange_scalar_sdep(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(theta), C.byref(st),
+                                       C.byref(sst))
+
+    def sgs_stag_scalar(self, grads, u, v, w, theta, taus, st, sst):
+        g = (D * 9)(*[_p(a) for a in grads])
+        t = (D * 6)(*[_p(a) for a in taus])
+        self.L.or_sgs_stag_scalar(C.c_void_p(self.ctx), g, _p(u), _p(v), _p(w), _p(theta), t, C.byref(st),
+                                  C.byref(sst))
 
     def interpolag_sdep(self, st):
         self.L.or_interpolag_sdep(C.c_void_p(self.ctx), C.byref(st))
@@ -238,11 +314,22 @@ class OracleSim:
     GRADS = ("dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz")
     TAUS = ("txx", "txy", "txz", "tyy", "tyz", "tzz")
 
-    def __init__(self, orc: Oracle, seed=42, amp=0.1, ustar=1.0, kappa_ic=0.4):
+    SCAL3 = ("theta", "qmix", "dTdx", "dTdy", "dTdz", "dqdx", "dqdy", "dqdz", "RHS_T", "RHS_Tf", "RHS_q",
+             "RHS_qf", "beta_scal")
+    SSGS3 = ("G_LM", "G_MM", "G_QN", "G_NN", "u_lag_s", "v_lag_s", "w_lag_s", "Pr_t", "Cs2_Pr")
+
+    def __init__(self, orc: Oracle, seed=42, amp=0.1, ustar=1.0, kappa_ic=0.4, scalar_ic=None):
+        """scalar_ic: dict(T_init, z_inv, inv_strength, amp_K, q_init) when the
+        oracle's scalars are enabled (Oracle.set_scalars)."""
         self.o = orc
         self.L = orc.L
         self.s = self.L.or_state_create(C.c_void_p(orc.ctx))
         self.L.or_state_init_ic(C.c_void_p(orc.ctx), self.s, seed, amp, ustar, kappa_ic)
+        if scalar_ic is not None:
+            k = dict(T_init=283.0, z_inv=0.8, inv_strength=0.01, amp_K=0.1, q_init=1.0)
+            k.update(scalar_ic)
+            self.L.or_state_init_scalar_ic(C.c_void_p(orc.ctx), self.s, seed, k["T_init"], k["z_inv"],
+                                           k["inv_strength"], k["amp_K"], k["q_init"])
 
     def __del__(self):
         try:
@@ -269,8 +356,14 @@ class OracleSim:
             return self._view(getattr(st.sgs, name))
         if name in ("Beta_avg", "Betaclip_avg"):
             return np.ctypeslib.as_array(getattr(st.sgs, name), shape=(self.o.nz,))
-        if name in ("psi_m", "psi_m0", "phi_m", "ustar_avg"):
+        if name in ("psi_m", "psi_m0", "phi_m", "ustar_avg", "psi_h", "psi_h0", "phi_h"):
             return np.ctypeslib.as_array(getattr(st, name), shape=(self.o.ny, self.o.nx))
+        if name in self.SCAL3:
+            return self._view(getattr(st, name))
+        if name in ("sgs_t3", "sgs_q3"):
+            return self._view(getattr(st, name), 1)[0]
+        if name in self.SSGS3:
+            return self._view(getattr(st.ssgs, name))
         raise KeyError(name)
 
     @property
