@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
 """Throughput of the LES time step on one MI355X.
 
-Workload: neutral ABL at 256 x 256 x 128 (nz = 129 w-levels), Lagrangian
-scale-dependent SGS (model 5, dynamic update every 5 steps), log-law wall,
-mean pressure-gradient forcing 1, seeded log-law + 10% perturbation initial
-condition, dt = 1e-4.  A "step" is one pass of main.f90:202-415.
+Default workload (--workload convective): convective ABL at 256 x 256 x 128
+(nz = 129 w-levels) with potential temperature and humidity (S_FLAG), the
+Lagrangian scale-dependent SGS model for momentum and heat (model 7, dynamic
+update every 5 steps), Monin-Obukhov surface layer over a 4 x 4 checkerboard
+of 0.1 / 0.02 K m/s heat flux, z0 = 1e-4, capping inversion of 0.01 K/m above
+z = 0.8 z_i, seeded 0.1 K theta and 10% velocity perturbations, mean
+pressure-gradient forcing 1, dt = 1e-4.  --workload neutral drops the
+scalars (model 5).  A "step" is one pass of main.f90:202-415.
 Headline `value` = FP32-bulk grid-point-steps/s (FP64 kept for the pressure
-tridiagonal, wall stress and reductions); the plain FP64 rate is reported
-beside it.  Inputs are resident in HBM before the timed region.
+tridiagonal, wall stress, surface layer and reductions); the plain FP64 rate
+is reported beside it.  Inputs are resident in HBM before the timed region.
 """
 import argparse
 import ctypes
@@ -24,13 +28,20 @@ HBM_PEAK_GBS = 8000.0
 
 
 ERR_FIELDS = ("u", "v", "w", "p")
+ERR_FIELDS_CONV = ERR_FIELDS + ("theta",)
+SCALARS = dict(enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_star=1.0, T_scale=400.0,
+               q_scale=100.0, g=9.81, dTdz_top=0.01, dqdz_top=0.0)
+SCALAR_IC = dict(T_init=283.0, z_inv=0.8, inv_strength=0.01, amp_K=0.1, q_init=1.0)
 
 
-def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False):
+def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False, conv=True):
     import torch
-    from les_amd.api import LesSim
-    sim = LesSim(nx, ny, nz, precision=prec, dt=1e-4, graph=graph)
+    from les_amd.api import LesSim, checkerboard_flux
+    scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
+    sim = LesSim(nx, ny, nz, precision=prec, dt=1e-4, graph=graph, scalars=scal)
     sim.init_ic(seed=42)
+    if conv:
+        sim.init_scalar_ic(seed=42, **SCALAR_IC)
     sim.set_graph(False)
     sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
     sim.set_graph(graph)
@@ -59,7 +70,7 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
             prof[k][2] /= profile_steps   # bytes per step
     if extra_steps:
         sim.step(extra_steps)     # untimed: bring the step count level with the profiled run
-    fields = {n: sim.get(n) for n in ERR_FIELDS} if keep else None
+    fields = {n: sim.get(n) for n in (ERR_FIELDS_CONV if conv else ERR_FIELDS)} if keep else None
     rms = ctypes.c_double()
     out4 = (ctypes.c_double * 4)()
     sim._chk(sim.L.les_state_diag(sim.ctx, ctypes.byref(rms), out4))
@@ -73,24 +84,29 @@ def fp32_error(f32, f64, nx, nz):
     """Per-field relative L2 error of the FP32-bulk state against the FP64
     state after the same steps, and the worst plane-mean profile error."""
     out = {}
-    for n in ERR_FIELDS:
+    for n in f64:
         a, b = f32[n], f64[n]
         lev = slice(1, nz) if n == "p" else (slice(1, nz - 1) if n == "w" else slice(0, nz - 1))
         a, b = a[lev, :, :nx], b[lev, :, :nx]
         out[n] = float(((a - b) ** 2).sum() ** 0.5 / max((b ** 2).sum() ** 0.5, 1e-300))
-        if n in ("u", "v"):
+        if n in ("u", "v", "theta"):
             pa, pb = a.mean(axis=(1, 2)), b.mean(axis=(1, 2))
             out[n + "_mean_profile"] = float(abs(pa - pb).max() / max(abs(pb).max(), 1e-300))
     return out
 
 
-def cpu_baseline(nx, ny, nz, nz_sample):
+def cpu_baseline(nx, ny, nz, nz_sample, conv=True):
     """Oracle (single-threaded C restatement) on the full x-y grid and
     nz_sample w-levels: one plain step and one step with the dynamic update;
     per-step time scaled linearly to the full level count."""
-    from oracle.oracle import Oracle, OracleSim
+    from oracle.oracle import Oracle, OracleSim, ScalarParams
+    from les_amd.api import checkerboard_flux
     o = Oracle(nx, ny, nz_sample, dt=1e-4)
-    so = OracleSim(o, seed=42)
+    if conv:
+        o.set_scalars(ScalarParams(*SCALARS.values()), checkerboard_flux(nx, ny), None)
+        so = OracleSim(o, seed=42, scalar_ic=SCALAR_IC)
+    else:
+        so = OracleSim(o, seed=42)
     t0 = time.perf_counter()
     so.step(1)
     t1 = time.perf_counter()
@@ -116,6 +132,7 @@ def main():
     ap.add_argument("--cpu-nz", type=int, default=17, help="w-levels in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", choices=("convective", "neutral"), default="convective")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -125,14 +142,15 @@ def main():
     pts = nx * ny * (nz - 1)
 
     graph = not args.no_graph
+    conv = args.workload == "convective"
     ms32, prof32, ok32, rms32, cfl32, f32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
-                                                    keep=not args.no_fp64)
+                                                    keep=not args.no_fp64, conv=conv)
     print(f"fp32-bulk: {ms32:.3f} ms/step finite={ok32} rmsdiv={rms32:.3e} cfl={cfl32[0]:.3e}",
           file=sys.stderr, flush=True)
     ms64, err = None, None
     if not args.no_fp64:
         ms64, prof64, ok64, rms64, cfl64, f64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
-                                                        extra_steps=args.profile_steps, keep=True)
+                                                        extra_steps=args.profile_steps, keep=True, conv=conv)
         print(f"fp64: {ms64:.3f} ms/step finite={ok64} rmsdiv={rms64:.3e}", file=sys.stderr, flush=True)
         err = fp32_error(f32, f64, nx, nz)
         err["steps"] = W + K + args.profile_steps
@@ -156,7 +174,7 @@ def main():
                 "profiled_ms_per_step": tot}
     cpu = None
     if not args.no_cpu:
-        rate, tp, tl = cpu_baseline(nx, ny, nz, args.cpu_nz)
+        rate, tp, tl = cpu_baseline(nx, ny, nz, args.cpu_nz, conv=conv)
         cpu = {"value": rate, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
                "sample": f"oracle (C FP64 restatement, 1 thread) on {nx}x{ny}x{args.cpu_nz - 1}: one plain step "
                          f"({tp:.2f} s) and one dynamic-update step ({tl:.2f} s), weighted 4:1 and scaled to "
@@ -165,9 +183,13 @@ def main():
         "metric": "grid-point-steps/s at 256^2x128, FP32-bulk vs FP64 GPU; HBM roofline fraction",
         "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (bulk; f64 tridiagonal, wall stress, reductions)", "data": "synthetic",
-        "config": {"workload": f"neutral ABL {nx}x{ny}x{nz - 1} (nz={nz}), Lagrangian scale-dependent SGS "
-                               "(model 5), log-law wall z0=1e-4, forcing 1, dt=1e-4, seeded IC",
+        "dtype": "f32 (bulk; f64 tridiagonal, wall stress, surface layer, reductions)", "data": "synthetic",
+        "config": {"workload": (f"convective ABL {nx}x{ny}x{nz - 1} (nz={nz}), theta + q, Lagrangian "
+                                "scale-dependent SGS for momentum and heat (model 7), 4x4 checkerboard surface "
+                                "heat flux 0.1/0.02 K m/s, z0=1e-4, 0.01 K/m inversion above 0.8 z_i, forcing 1, "
+                                "dt=1e-4, seeded IC") if conv else
+                               (f"neutral ABL {nx}x{ny}x{nz - 1} (nz={nz}), Lagrangian scale-dependent SGS "
+                                "(model 5), log-law wall z0=1e-4, forcing 1, dt=1e-4, seeded IC"),
                    "grid": [nx, ny, nz], "parallelism": "1 GPU, whole domain",
                    "launch": "HIP graph replay of 10-step windows" if graph else "eager"},
         "fp64_value": (pts / (ms64 * 1e-3)) if ms64 else None,

@@ -101,6 +101,61 @@ int les_lagrange_sdep(les_ctx *c, double *const S[6], const double *u, const dou
                       const les_sgs_fields *s);
 int les_interpolag_sdep(les_ctx *c, const les_sgs_fields *s);
 
+/* ---- scalars: potential temperature theta and humidity q (S_FLAG) ----
+   Replaces the scalar parameters of param.f90:53,72-90,139-146,174 and the
+   surface set-up of bottombc.f90:26-63 (lbc=1, patch_flag=1; z_os = z0/7.39).
+   Must precede les_state_alloc for the state to carry the scalars. */
+typedef struct les_scalar_params {
+  int enable;      /* S_FLAG: theta_all_in_one + q_all_in_one every step */
+  int model;       /* 7: Pr_ = Pr_t of lagrange_scalar_Sdep; otherwise Pr */
+  int initsc;      /* scalar history given (no Euler scalar step at jt==1) */
+  int passive;     /* passive_scalar: no buoyancy in RHSz */
+  double Pr, z_i, u_star, T_scale, q_scale, g;
+  double dTdz_top; /* K/m at the lid */
+  double dqdz_top; /* (g/kg)/m at the lid */
+} les_scalar_params;
+/* wt_s (K m/s), wq_s ((g/kg) m/s): host (nx,ny) kinematic surface fluxes after
+   the conversion of bottombc.f90:57-58; NULL means 0 */
+int les_set_scalars(les_ctx *c, const les_scalar_params *p, const double *wt_s, const double *wq_s);
+int les_set_scalar_lag_init(les_ctx *c, int glm_init, int gqn_init); /* lagrange_scalar_Sdep.f90:65-66 */
+/* FP32-bulk state only (no reference counterpart): theta and qmix are stored
+   as deviations from theta_ref = T_init/T_scale and q_ref = q_init/q_scale of
+   les_state_init_scalar_ic; both are 0 in FP64 mode */
+int les_scalar_ref(les_ctx *c, double *theta_ref, double *q_ref);
+
+/* sgsmodule.f90 G_LM..G_NN, u_lag_s..w_lag_s, Pr_t (ld,ny,nz) and the
+   Cs2_Pr output of lagrange_scalar_Sdep; phi_h, psi_m (nx,ny) from bottombc */
+typedef struct les_scalar_sgs_fields {
+  void *G_LM, *G_MM, *G_QN, *G_NN, *u_lag_s, *v_lag_s, *w_lag_s, *Pr_t, *Cs2_Pr;
+  const double *phi_h, *psi_m;
+} les_scalar_sgs_fields;
+
+/* obukhov, scalars_module.f90:508-690: theta/qmix (ld,ny,nz) (level 1 read);
+   sgs_t3/sgs_q3 are the level-1 planes (ld,ny); psi/phi/ustar (nx,ny) */
+int les_obukhov(les_ctx *c, const double *u, const double *v, const double *theta, const double *qmix,
+                double *sgs_t3, double *sgs_q3, double *psi_m, double *psi_m0, double *psi_h, double *psi_h0,
+                double *phi_m, double *phi_h, double *ustar_avg);
+/* calcbeta, scalars_module.f90:175-228 */
+int les_calcbeta(les_ctx *c, const double *theta, const double *qmix, double *beta_scal);
+/* scalar_RHS_calc, scalars_module.f90:232-454 (which=0) / scalars_module_q.f90:114-330
+   (which=1); dsdz level 1 is overwritten with the surface gradient, sgs_vert1 is
+   the level-1 plane of sgs_t3/sgs_q3; RHS level nz is set to 0 */
+int les_scalar_rhs_calc(les_ctx *c, int which, const double *dsdx, const double *dsdy, double *dsdz,
+                        const double *u, const double *v, const double *w, const double *Nu_t, const double *Pr_,
+                        const double *ustar_avg, const double *phi_h, double *RHS, double *sgs_vert1);
+/* step_scalar, scalars_module.f90:457-504 / step_scalar_q, scalars_module_q.f90:357-397 */
+int les_step_scalar(les_ctx *c, int which, double *scalar, const double *RHS, const double *RHS_f);
+/* interpolag_scalar_Sdep.f90:1-316 */
+int les_interpolag_scalar_sdep(les_ctx *c, const les_scalar_sgs_fields *s);
+/* lagrange_scalar_Sdep.f90:1-467 (theta is Nyquist-filtered in place, :95);
+   reads TnLMMM, TnQNNN, Cs_opt2 of m and overwrites m->Beta */
+int les_lagrange_scalar_sdep(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                             double *theta, const les_sgs_fields *m, const les_scalar_sgs_fields *s);
+/* sgs_stag.f90:3-500 with model=7 (:59-69,150-153,166-173) */
+int les_sgs_stag_scalar(les_ctx *c, double *const grad[9], const double *u, const double *v, const double *w,
+                        double *theta, double *const tau[6], const les_sgs_fields *m,
+                        const les_scalar_sgs_fields *s);
+
 /* ---- divstress_uv.f90:1-63, divstress_w.f90:4-86 ---- */
 int les_divstress_uv(les_ctx *c, double *divt, const double *tx, const double *ty, const double *tz);
 int les_divstress_w(les_ctx *c, double *divt, const double *tx, const double *ty, const double *tz);
@@ -134,9 +189,11 @@ int les_divstress_w_f32(les_ctx *c, float *divt, const float *tx, const float *t
 int les_press_stag_array_f32(les_ctx *c, const float *u, const float *v, const float *w,
                              const float *divtz, float *p, float *dpdx, float *dpdy);
 
-/* ---- whole time step (main.f90:202-415, neutral model=5) on
-   context-owned device state.  precision: 0 = FP64, 1 = FP32-bulk (FP64 kept
-   for the tridiagonal solve, wall stress and reductions) ---- */
+/* ---- whole time step (main.f90:202-415) on context-owned device state:
+   neutral model=5, or with les_set_scalars(enable) the convective step with
+   obukhov, theta/q transport, buoyancy and (model=7) the scalar Lagrangian
+   model.  precision: 0 = FP64, 1 = FP32-bulk (FP64 kept for the tridiagonal
+   solve, wall stress, surface layer and reductions) ---- */
 int les_state_alloc(les_ctx *c, int precision);
 /* device pointer of a state field by reference name (u, v, w, dudx ..., RHSx,
    RHSx_f, txx ..., divtx ..., p, dpdx ..., F_LM ..., Cs_opt2, Nu_t, Beta,
@@ -149,6 +206,11 @@ void *les_state_field(les_ctx *c, const char *name, int *nlev, int *elsize);
    u(nz)=u(nz-1), v(nz)=v(nz-1), RHS=RHS_f=0.  Resets jt, jt_total and the
    SGS state. */
 int les_state_init_ic(les_ctx *c, uint64_t seed, double amp, double ustar, double kappa_ic);
+/* seeded scalar initial condition (after les_state_init_ic): theta = T_init
+   (K) below z_inv (units of z_i), + inv_strength (K/m) above, plus amp_K*(2U-1)
+   below z_inv (U of field index 3); q = q_init (g/kg); Pr_t = Pr, phi_h = 1 */
+int les_state_init_scalar_ic(les_ctx *c, uint64_t seed, double T_init, double z_inv, double inv_strength,
+                             double amp_K, double q_init);
 int les_step(les_ctx *c, int nsteps);
 /* Replay the step through HIP graphs (no reference counterpart): windows of
    lcm(2, cs_count) steps whose host-side branches repeat (no Euler start, no

@@ -30,6 +30,20 @@ class SgsFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in SGS_NAMES] + [("Beta_avg", C.c_void_p), ("Betaclip_avg", C.c_void_p)]
 
 
+class ScalarParams(C.Structure):
+    """les_scalar_params (param.f90:53,72-90,139-146,174; bottombc.f90:26-63)."""
+    _fields_ = [("enable", C.c_int), ("model", C.c_int), ("initsc", C.c_int), ("passive", C.c_int),
+                ("Pr", C.c_double), ("z_i", C.c_double), ("u_star", C.c_double), ("T_scale", C.c_double),
+                ("q_scale", C.c_double), ("g", C.c_double), ("dTdz_top", C.c_double), ("dqdz_top", C.c_double)]
+
+
+SCALAR_SGS_NAMES = ("G_LM", "G_MM", "G_QN", "G_NN", "u_lag_s", "v_lag_s", "w_lag_s", "Pr_t", "Cs2_Pr")
+
+
+class ScalarSgsFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in SCALAR_SGS_NAMES] + [("phi_h", C.c_void_p), ("psi_m", C.c_void_p)]
+
+
 def declared_symbols():
     """Names of the functions include/les_amd.h declares."""
     txt = open(HEADER).read()
@@ -57,6 +71,9 @@ def load():
     L.les_state_field.restype = vp
     L.les_state_init_ic.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, C.c_double]
     L.les_test_filter_init.argtypes = [vp, C.c_double, vp]
+    L.les_set_scalars.argtypes = [vp, C.POINTER(ScalarParams), vp, vp]
+    L.les_state_init_scalar_ic.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_double]
     for name in declared_symbols():
         if name not in ("les_destroy", "les_last_error", "les_state_field"):
             getattr(L, name).restype = C.c_int

@@ -14,7 +14,7 @@ import math
 import torch
 
 from . import _lib
-from ._lib import Grid, SgsFields, SGS_NAMES
+from ._lib import Grid, SgsFields, SGS_NAMES, ScalarParams, ScalarSgsFields, SCALAR_SGS_NAMES
 
 
 class LesError(RuntimeError):
@@ -181,6 +181,67 @@ class Les:
         st = self._sgs_struct(sgs)
         self._chk(self.L.les_interpolag_sdep(self.ctx, C.byref(st)))
 
+    # ---- scalars (scalars_module.f90, scalars_module_q.f90, lagrange_scalar_Sdep.f90) ----
+    def set_scalars(self, enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_star=1.0,
+                    T_scale=400.0, q_scale=100.0, g=9.81, dTdz_top=0.01, dqdz_top=-0.004, wt_s=None, wq_s=None):
+        """Scalar switches/constants and the (ny, nx) kinematic surface fluxes
+        wt_s (K m/s), wq_s ((g/kg) m/s) as host arrays; call before the state."""
+        import numpy as np
+        self.scalar_params = ScalarParams(enable, model, initsc, passive, Pr, z_i, u_star, T_scale, q_scale, g,
+                                          dTdz_top, dqdz_top)
+        self._wt = None if wt_s is None else np.ascontiguousarray(wt_s, dtype=np.float64)
+        self._wq = None if wq_s is None else np.ascontiguousarray(wq_s, dtype=np.float64)
+        pw = None if self._wt is None else self._wt.ctypes.data_as(C.c_void_p)
+        pq = None if self._wq is None else self._wq.ctypes.data_as(C.c_void_p)
+        self._chk(self.L.les_set_scalars(self.ctx, C.byref(self.scalar_params), pw, pq))
+
+    def set_scalar_lag_init(self, glm_init, gqn_init):
+        self._chk(self.L.les_set_scalar_lag_init(self.ctx, int(glm_init), int(gqn_init)))
+
+    def obukhov(self, u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, ustar_avg):
+        self._chk(self.L.les_obukhov(self.ctx, *[_ptr(t) for t in (u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0,
+                                                                   psi_h, psi_h0, phi_m, phi_h, ustar_avg)]))
+
+    def calcbeta(self, theta, qmix, beta):
+        self._chk(self.L.les_calcbeta(self.ctx, _ptr(theta), _ptr(qmix), _ptr(beta)))
+
+    def scalar_rhs_calc(self, which, dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg, phi_h, rhs, sgs_vert1):
+        self._chk(self.L.les_scalar_rhs_calc(self.ctx, int(which),
+                                             *[_ptr(t) for t in (dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg,
+                                                                 phi_h, rhs, sgs_vert1)]))
+
+    def step_scalar(self, which, scalar, rhs, rhs_f):
+        self._chk(self.L.les_step_scalar(self.ctx, int(which), _ptr(scalar), _ptr(rhs), _ptr(rhs_f)))
+
+    def scalar_sgs_fields(self, phi_h=None, psi_m=None, dtype=torch.float64):
+        d = {n: self.field(dtype=dtype) for n in SCALAR_SGS_NAMES}
+        n2 = (self.ny, self.nx)
+        d["phi_h"] = torch.ones(n2, dtype=torch.float64, device="cuda") if phi_h is None else phi_h
+        d["psi_m"] = torch.zeros(n2, dtype=torch.float64, device="cuda") if psi_m is None else psi_m
+        return d
+
+    @staticmethod
+    def _ssgs_struct(d):
+        return ScalarSgsFields(*[C.c_void_p(d[n].data_ptr()) for n in SCALAR_SGS_NAMES],
+                               C.c_void_p(d["phi_h"].data_ptr()), C.c_void_p(d["psi_m"].data_ptr()))
+
+    def interpolag_scalar_sdep(self, ssgs):
+        st = self._ssgs_struct(ssgs)
+        self._chk(self.L.les_interpolag_scalar_sdep(self.ctx, C.byref(st)))
+
+    def lagrange_scalar_sdep(self, S, u, v, w, theta, sgs, ssgs):
+        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        m, st = self._sgs_struct(sgs), self._ssgs_struct(ssgs)
+        self._chk(self.L.les_lagrange_scalar_sdep(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(theta), C.byref(m),
+                                                  C.byref(st)))
+
+    def sgs_stag_scalar(self, grads, u, v, w, theta, taus, sgs, ssgs):
+        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
+        t = (C.c_void_p * 6)(*[x.data_ptr() for x in taus])
+        m, st = self._sgs_struct(sgs), self._ssgs_struct(ssgs)
+        self._chk(self.L.les_sgs_stag_scalar(self.ctx, g, _ptr(u), _ptr(v), _ptr(w), _ptr(theta), t, C.byref(m),
+                                             C.byref(st)))
+
     def divstress_uv(self, divt, tx, ty, tz):
         self._chk(getattr(self.L, "les_divstress_uv" + self._sfx(divt))(self.ctx, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)))
 
@@ -211,11 +272,15 @@ class Les:
 class LesSim(Les):
     """Whole-step driver on context-owned device state (main.f90:202-415)."""
 
-    def __init__(self, nx, ny, nz, precision="fp64", graph=False, **kw):
+    def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, **kw):
+        """scalars: None (neutral) or a dict of set_scalars arguments (the
+        convective step with theta, q, buoyancy and the scalar SGS model)."""
         # a stream of its own (default priority): graph capture needs a
         # non-default stream, and the state is only touched through the library
         self.stream = torch.cuda.Stream()
         super().__init__(nx, ny, nz, stream=self.stream, **kw)
+        if scalars is not None:
+            self.set_scalars(**scalars)
         self.precision = precision
         self._chk(self.L.les_state_alloc(self.ctx, 1 if precision == "fp32" else 0))
         self._chk(self.L.les_set_graph(self.ctx, 1 if graph else 0))
@@ -228,6 +293,10 @@ class LesSim(Les):
 
     def init_ic(self, seed=42, amp=0.1, ustar=1.0, kappa_ic=0.4):
         self._chk(self.L.les_state_init_ic(self.ctx, seed, amp, ustar, kappa_ic))
+
+    def init_scalar_ic(self, seed=42, T_init=283.0, z_inv=0.8, inv_strength=0.01, amp_K=0.1, q_init=1.0):
+        """Convective scalar IC (after init_ic; needs set_scalars before the state)."""
+        self._chk(self.L.les_state_init_scalar_ic(self.ctx, seed, T_init, z_inv, inv_strength, amp_K, q_init))
 
     def step(self, n=1):
         self._chk(self.L.les_step(self.ctx, n))
@@ -242,8 +311,10 @@ class LesSim(Les):
     def view(self, name):
         """Zero-copy torch view of a state field (reference layout)."""
         p, nlev, es = self.ptr(name)
-        if name in ("psi_m", "psi_m0", "phi_m", "ustar_avg"):
+        if name in ("psi_m", "psi_m0", "phi_m", "ustar_avg", "psi_h", "psi_h0", "phi_h"):
             shape, typestr = (self.ny, self.nx), "<f8"
+        elif name in ("sgs_t3", "sgs_q3"):
+            shape, typestr = (self.ny, self.ld), "<f8"
         elif name in ("Beta_avg", "Betaclip_avg"):
             shape, typestr = (self.nz,), "<f8"
         else:
@@ -255,12 +326,28 @@ class LesSim(Les):
         return torch.as_tensor(_Dev(), device="cuda")
 
     def get(self, name):
-        """Copy a state field to a host numpy array (float64)."""
+        """Copy a state field to a host numpy array (float64).  theta and qmix
+        are returned in the reference convention (the FP32-bulk state stores
+        deviations from a reference value)."""
         self.sync()
-        return self.view(name).double().cpu().numpy()
+        a = self.view(name).double().cpu().numpy()
+        if name in ("theta", "qmix") and self.precision == "fp32":
+            tr, qr = C.c_double(), C.c_double()
+            self._chk(self.L.les_scalar_ref(self.ctx, C.byref(tr), C.byref(qr)))
+            a[:, :, :self.nx] += tr.value if name == "theta" else qr.value
+        return a
 
     @property
     def jt(self):
         a, b = C.c_int(), C.c_int()
         self._chk(self.L.les_get_time(self.ctx, C.byref(a), C.byref(b)))
         return a.value
+
+
+def checkerboard_flux(nx, ny, hi=0.1, lo=0.02, patches=4):
+    """(ny, nx) surface heat flux of patches x patches alternating tiles
+    (kinematic, K m/s): the heterogeneous surface of the convective case."""
+    import numpy as np
+    px = (np.arange(nx) * patches) // nx
+    py = (np.arange(ny) * patches) // ny
+    return np.where((px[None, :] + py[:, None]) % 2 == 0, hi, lo).astype(np.float64)

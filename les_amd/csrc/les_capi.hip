@@ -147,6 +147,7 @@ struct State {
   size_t es = 8;
   std::map<std::string, FieldInfo> f;
   int rhs_swap = 0;  // RHS <-> RHS_f buffer parity
+  int scal = 0;      // scalar fields allocated and stepped
   std::vector<void*> owned;
 };
 
@@ -156,13 +157,24 @@ static const char* F3[] = {"u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy"
                            "Beta", "Cs_opt2", "Nu_t", "u_lag", "v_lag", "w_lag", "TnLMMM", "TnQNNN", "L11t", "L22t",
                            "L33t", "Q11t", "Q22t", "Q33t"};
 
+// scalar fields allocated when the scalars are enabled (sim_param theta/qmix,
+// scalars_module(_q) gradients/RHS, calcbeta's beta_scal, sgsmodule G_*,
+// u_lag_s, Pr_t, and lagrange_scalar_Sdep's Cs2_Pr)
+static const char* F3S[] = {"theta", "qmix", "dTdx", "dTdy", "dTdz", "dqdx", "dqdy", "dqdz", "RHS_T", "RHS_Tf",
+                            "RHS_q", "RHS_qf", "beta_scal", "G_LM", "G_MM", "G_QN", "G_NN", "u_lag_s", "v_lag_s",
+                            "w_lag_s", "Pr_t", "Cs2_Pr"};
+
+// RHS <-> RHS_f pairs that exchange buffers each step instead of copying
+static const char* RHS_PAIRS[][2] = {{"RHSx", "RHSx_f"}, {"RHSy", "RHSy_f"}, {"RHSz", "RHSz_f"},
+                                     {"RHS_T", "RHS_Tf"}, {"RHS_q", "RHS_qf"}};
+
 static void* state_ptr(les_ctx* c, const char* name) {
   State* s = c->st;
   std::string n(name);
   if (s->rhs_swap) {
-    if (n.size() >= 4 && n.compare(0, 3, "RHS") == 0) {
-      if (n.size() == 4) n += "_f";
-      else if (n.size() == 6) n = n.substr(0, 4);
+    for (auto& pr : RHS_PAIRS) {
+      if (n == pr[0]) { n = pr[1]; break; }
+      if (n == pr[1]) { n = pr[0]; break; }
     }
   }
   auto it = s->f.find(n);
@@ -189,17 +201,57 @@ static SgsPtrs sgs_ptrs(les_ctx* c) {
 }
 
 template <typename T>
+static ScalarSgsPtrs scalar_sgs_ptrs(les_ctx* c) {
+  ScalarSgsPtrs p;
+  p.G_LM = F<T>(c, "G_LM"); p.G_MM = F<T>(c, "G_MM"); p.G_QN = F<T>(c, "G_QN"); p.G_NN = F<T>(c, "G_NN");
+  p.u_lag_s = F<T>(c, "u_lag_s"); p.v_lag_s = F<T>(c, "v_lag_s"); p.w_lag_s = F<T>(c, "w_lag_s");
+  p.Pr_t = F<T>(c, "Pr_t"); p.Cs2_Pr = F<T>(c, "Cs2_Pr");
+  p.phi_h = (const double*)state_ptr(c, "phi_h");
+  p.psi_m = (const double*)state_ptr(c, "psi_m");
+  return p;
+}
+
+// theta_all_in_one (scalars_module.f90:57-171) and q_all_in_one
+// (scalars_module_q.f90:31-111) of one step, both scalars batched.  calcbeta
+// reads theta after its filt_da and q before its own, as in the reference.
+template <typename T>
+static void scalars_step(les_ctx* c, const T* u, const T* v, const T* w) {
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  T *th = F<T>(c, "theta"), *q = F<T>(c, "qmix");
+  ScalarJob jobs[2];
+  jobs[0] = ScalarJob{th, F<T>(c, "dTdx"), F<T>(c, "dTdy"), F<T>(c, "dTdz"), F<T>(c, "RHS_T"), F<T>(c, "RHS_Tf"),
+                      (double*)state_ptr(c, "sgs_t3"), 0};
+  jobs[1] = ScalarJob{q, F<T>(c, "dqdx"), F<T>(c, "dqdy"), F<T>(c, "dqdz"), F<T>(c, "RHS_q"), F<T>(c, "RHS_qf"),
+                      (double*)state_ptr(c, "sgs_q3"), 1};
+  op_filt_da<T>(c, th, (T*)jobs[0].dsdx, (T*)jobs[0].dsdy, g.nz);
+  op_calcbeta<T>(c, th, q, F<T>(c, "beta_scal"), P.theta_ref, P.q_ref);
+  op_filt_da<T>(c, q, (T*)jobs[1].dsdx, (T*)jobs[1].dsdy, g.nz);
+  op_scalar_dz<T>(c, jobs, 2, (const double*)state_ptr(c, "phi_h"), (const double*)state_ptr(c, "ustar_avg"), 0);
+  op_scalar_rhs<T>(c, jobs, 2, u, v, w, F<T>(c, "Nu_t"), P.model == 7 ? F<T>(c, "Pr_t") : (const T*)nullptr, 1,
+                   (c->jt == 1 && !P.initsc) ? 1 : 0);
+}
+
+template <typename T>
 static void step_once(les_ctx* c) {
   const Grid& g = c->g;
   State* s = c->st;
+  const int scal = s->scal;
   c->jt += 1;
   c->jt_total += 1;
-  s->rhs_swap ^= 1;  // RHS_f = RHS (main.f90:212-214) as a buffer swap
+  s->rhs_swap ^= 1;  // RHS_f = RHS (main.f90:212-214, scalars_module.f90:121) as a buffer swap
   T *u = F<T>(c, "u"), *v = F<T>(c, "v"), *w = F<T>(c, "w");
   T* gr[9] = {F<T>(c, "dudx"), F<T>(c, "dudy"), F<T>(c, "dudz"), F<T>(c, "dvdx"), F<T>(c, "dvdy"),
               F<T>(c, "dvdz"), F<T>(c, "dwdx"), F<T>(c, "dwdy"), F<T>(c, "dwdz")};
   T* tau[6] = {F<T>(c, "txx"), F<T>(c, "txy"), F<T>(c, "txz"), F<T>(c, "tyy"), F<T>(c, "tyz"), F<T>(c, "tzz")};
-  // obukhov (neutral): psi = 0, phi = 1 are the state's constant contents
+  if (scal) {  // main.f90:217
+    op_obukhov<T>(c, u, v, F<T>(c, "theta"), (double*)state_ptr(c, "sgs_t3"), (double*)state_ptr(c, "sgs_q3"),
+                  (double*)state_ptr(c, "psi_m"), (double*)state_ptr(c, "psi_m0"), (double*)state_ptr(c, "psi_h"),
+                  (double*)state_ptr(c, "psi_h0"), (double*)state_ptr(c, "phi_m"), (double*)state_ptr(c, "phi_h"),
+                  (double*)state_ptr(c, "ustar_avg"), c->scal.theta_ref);
+  }
+  // without scalars obukhov's neutral branch leaves psi = 0, phi = 1: the
+  // state's constant contents
   op_filt_da3<T>(c, u, v, w, gr, g.nz);
   op_ddz_uv<T>(c, gr[2], u);
   op_ddz_uv<T>(c, gr[5], v);
@@ -208,7 +260,13 @@ static void step_once(les_ctx* c) {
                    (const double*)state_ptr(c, "psi_m0"), (const double*)state_ptr(c, "phi_m"),
                    (double*)state_ptr(c, "ustar_avg"));
   const T* grc[9] = {gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]};
-  op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c));
+  if (scal && c->scal.model == 7) {
+    const ScalarSgsPtrs ss = scalar_sgs_ptrs<T>(c);
+    op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c), F<T>(c, "theta"), &ss);
+  } else {
+    op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c));
+  }
+  if (scal) scalars_step<T>(c, u, v, w);  // main.f90:273-276
   T *dx = F<T>(c, "divtx"), *dy = F<T>(c, "divty"), *dz = F<T>(c, "divtz");
   op_divstress<T>(c, dx, tau[0], tau[1], tau[2], 0);
   op_divstress<T>(c, dy, tau[1], tau[3], tau[4], 0);
@@ -216,7 +274,8 @@ static void step_once(les_ctx* c) {
   T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
   // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
   op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
-                   F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, (c->jt == 1 && !g.initu) ? 1 : 0);
+                   F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, (c->jt == 1 && !g.initu) ? 1 : 0,
+                   (scal && !c->scal.passive) ? (const T*)F<T>(c, "beta_scal") : (const T*)nullptr);
   T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
   op_press_stag<T>(c, u, v, w, dz, p, px, py);
   op_pressure_update<T>(c, u, v, w, Rx, Ry, Rz, p, px, py, pz);
@@ -289,6 +348,8 @@ void les_destroy(les_ctx* c) {
   }
   delete[] c->G2_host;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->wt_s) (void)hipFree(c->wt_s);
+  if (c->wq_s) (void)hipFree(c->wq_s);
   delete c;
 }
 
@@ -598,6 +659,15 @@ int les_state_alloc(les_ctx* c, int precision) {
   add2("ustar_avg", n2, 0.0);
   add2("Beta_avg", g.nz, 0.0);
   add2("Betaclip_avg", g.nz, 0.0);
+  if (c->scal.enable) {
+    s->scal = 1;
+    for (const char* n : F3S) add(n, g.nz, s->es);
+    add2("psi_h", n2, 0.0);
+    add2("psi_h0", n2, 0.0);
+    add2("phi_h", n2, 1.0);
+    add2("sgs_t3", pl, 0.0);  // level-1 planes (ld,ny) of sgs_t3 / sgs_q3, FP64
+    add2("sgs_q3", pl, 0.0);
+  }
   DONE(c);
 }
 
@@ -651,9 +721,11 @@ int les_state_init_ic(les_ctx* c, uint64_t seed, double amp, double ustar, doubl
     f[2][pl * (g.nz - 1) + i] = 0.0;
   }
   for (auto& kv : s->f) {
-    if (kv.first == "phi_m" || kv.first == "psi_m" || kv.first == "psi_m0") continue;
-    const size_t bytes = (kv.first == "ustar_avg") ? (size_t)g.nx * g.ny * 8
-                         : (kv.first == "Beta_avg" || kv.first == "Betaclip_avg") ? (size_t)g.nz * 8
+    const std::string& n = kv.first;
+    if (n == "phi_m" || n == "psi_m" || n == "psi_m0" || n == "phi_h" || n == "psi_h" || n == "psi_h0") continue;
+    const size_t bytes = (n == "ustar_avg") ? (size_t)g.nx * g.ny * 8
+                         : (n == "Beta_avg" || n == "Betaclip_avg") ? (size_t)g.nz * 8
+                         : (n == "sgs_t3" || n == "sgs_q3") ? pl * 8
                          : pl * kv.second.nlev * s->es;
     check(c, hipMemsetAsync(kv.second.p, 0, bytes, c->stream), "state reset");
   }
@@ -675,6 +747,8 @@ int les_state_init_ic(les_ctx* c, uint64_t seed, double amp, double ustar, doubl
   c->jt_total = 0;
   c->flm_init = 0;
   c->fqn_init = 0;
+  c->glm_init = 0;
+  c->gqn_init = 0;
   c->eager_steps = 0;
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);
@@ -704,7 +778,8 @@ static int graph_window(const les_ctx* c) {
 static bool window_is_regular(const les_ctx* c, int W, int* use_const) {
   const Grid& g = c->g;
   if (c->jt < 1 && !g.initu) return false;                      // Euler first step inside
-  const bool lag_init = g.inilag && (!c->flm_init || !c->fqn_init);
+  const bool lag_init = g.inilag && (!c->flm_init || !c->fqn_init ||
+                                     (c->st->scal && c->scal.model == 7 && (!c->glm_init || !c->gqn_init)));
   if (lag_init && c->jt < g.cs_count) return false;             // initialisation inside
   const int first = c->jt_total + 1, last = c->jt_total + W;
   if (first < 500 && last >= 500) return false;                // constant-Cs switch inside
@@ -724,6 +799,7 @@ static void graph_capture(les_ctx* c, int W, int uc) {
     c->gexec = nullptr;
   }
   const int jt0 = c->jt, jtt0 = c->jt_total, swap0 = c->st->rhs_swap, fl0 = c->flm_init, fq0 = c->fqn_init;
+  const int gl0 = c->glm_init, gq0 = c->gqn_init;
   const long es0 = c->eager_steps;
   check(c, hipStreamSynchronize(c->stream), "pre-capture sync");
   hipGraph_t graph = nullptr;
@@ -734,6 +810,8 @@ static void graph_capture(les_ctx* c, int W, int uc) {
   c->jt_total = jtt0;
   c->st->rhs_swap = swap0;
   c->flm_init = fl0;
+  c->glm_init = gl0;
+  c->gqn_init = gq0;
   c->fqn_init = fq0;
   c->eager_steps = es0;
   if (graph && c->err.empty()) {
@@ -889,6 +967,185 @@ int les_profile_report(les_ctx* c, char* buf, int buflen) {
     buf[buflen - 1] = 0;
   }
   if ((int)out.size() >= buflen) c->err = "les_profile_report: buffer too small";
+  DONE(c);
+}
+
+// ---- scalars ----
+int les_set_scalars(les_ctx* c, const les_scalar_params* p, const double* wt_s, const double* wq_s) {
+  GUARD(c);
+  if (!p) return -1;
+  ScalarParams& P = c->scal;
+  P.enable = p->enable; P.model = p->model; P.initsc = p->initsc; P.passive = p->passive;
+  P.Pr = p->Pr; P.z_i = p->z_i; P.u_star = p->u_star; P.T_scale = p->T_scale; P.q_scale = p->q_scale;
+  P.g = p->g; P.dTdz_top = p->dTdz_top; P.dqdz_top = p->dqdz_top;
+  const Grid& g = c->g;
+  const size_t n2 = (size_t)g.nx * g.ny;
+  std::vector<double> wt(n2, 0.0), wq(n2, 0.0);
+  if (wt_s) std::memcpy(wt.data(), wt_s, n2 * sizeof(double));
+  if (wq_s) std::memcpy(wq.data(), wq_s, n2 * sizeof(double));
+  for (double** d : {&c->wt_s, &c->wq_s})
+    if (!*d) check(c, hipMalloc(d, n2 * sizeof(double)), "surface flux alloc");
+  check(c, hipMemcpy(c->wt_s, wt.data(), n2 * sizeof(double), hipMemcpyHostToDevice), "wt_s upload");
+  check(c, hipMemcpy(c->wq_s, wq.data(), n2 * sizeof(double), hipMemcpyHostToDevice), "wq_s upload");
+  // plane constants of lagrange_scalar_Sdep.f90:110,114 in the reference's summation order
+  double slog = 0.0, swt = 0.0;
+  for (size_t j = 0; j < n2; ++j) slog += log(g.z0);
+  for (size_t j = 0; j < n2; ++j) swt += wt[j];
+  c->zo_avg = exp(slog / (double)(float)n2);
+  c->wt_mean = swt / (double)n2;
+  DONE(c);
+}
+
+int les_scalar_ref(les_ctx* c, double* theta_ref, double* q_ref) {
+  GUARD(c);
+  if (theta_ref) *theta_ref = c->scal.theta_ref;
+  if (q_ref) *q_ref = c->scal.q_ref;
+  DONE(c);
+}
+
+int les_set_scalar_lag_init(les_ctx* c, int glm_init, int gqn_init) {
+  GUARD(c);
+  c->glm_init = glm_init;
+  c->gqn_init = gqn_init;
+  DONE(c);
+}
+
+static les::ScalarSgsPtrs to_sptrs(const les_scalar_sgs_fields* s) {
+  les::ScalarSgsPtrs p;
+  p.G_LM = s->G_LM; p.G_MM = s->G_MM; p.G_QN = s->G_QN; p.G_NN = s->G_NN;
+  p.u_lag_s = s->u_lag_s; p.v_lag_s = s->v_lag_s; p.w_lag_s = s->w_lag_s;
+  p.Pr_t = s->Pr_t; p.Cs2_Pr = s->Cs2_Pr; p.phi_h = s->phi_h; p.psi_m = s->psi_m;
+  return p;
+}
+
+int les_obukhov(les_ctx* c, const double* u, const double* v, const double* theta, const double* qmix,
+                double* sgs_t3, double* sgs_q3, double* psi_m, double* psi_m0, double* psi_h, double* psi_h0,
+                double* phi_m, double* phi_h, double* ustar_avg) {
+  GUARD(c);
+  (void)qmix;  // read by the reference only for its printed diagnostics
+  if (!c->wt_s) { c->err = "les_obukhov: call les_set_scalars first"; return 1; }
+  op_obukhov<double>(c, u, v, theta, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, ustar_avg);
+  DONE(c);
+}
+
+int les_calcbeta(les_ctx* c, const double* theta, const double* qmix, double* beta_scal) {
+  GUARD(c);
+  op_calcbeta<double>(c, theta, qmix, beta_scal);
+  DONE(c);
+}
+
+int les_scalar_rhs_calc(les_ctx* c, int which, const double* dsdx, const double* dsdy, double* dsdz,
+                        const double* u, const double* v, const double* w, const double* Nu_t, const double* Pr_,
+                        const double* ustar_avg, const double* phi_h, double* RHS, double* sgs_vert1) {
+  GUARD(c);
+  if (!c->wt_s) { c->err = "les_scalar_rhs_calc: call les_set_scalars first"; return 1; }
+  if (which != 0 && which != 1) { c->err = "les_scalar_rhs_calc: which must be 0 (theta) or 1 (q)"; return 1; }
+  ScalarJob j{nullptr, (void*)dsdx, (void*)dsdy, dsdz, RHS, nullptr, sgs_vert1, which};
+  op_scalar_dz<double>(c, &j, 1, phi_h, ustar_avg, 1);
+  op_scalar_rhs<double>(c, &j, 1, u, v, w, Nu_t, Pr_, 0, 0);
+  DONE(c);
+}
+
+int les_step_scalar(les_ctx* c, int which, double* scalar, const double* RHS, const double* RHS_f) {
+  GUARD(c);
+  if (which != 0 && which != 1) { c->err = "les_step_scalar: which must be 0 (theta) or 1 (q)"; return 1; }
+  op_step_scalar<double>(c, which, scalar, RHS, RHS_f);
+  DONE(c);
+}
+
+int les_interpolag_scalar_sdep(les_ctx* c, const les_scalar_sgs_fields* s) {
+  GUARD(c);
+  if (!s) return -1;
+  double* G[4] = {(double*)s->G_LM, (double*)s->G_MM, (double*)s->G_QN, (double*)s->G_NN};
+  les::OpTag _t(c, "interpolag_scalar");
+  op_interpolag4<double>(c, G, (double*)s->u_lag_s, (double*)s->v_lag_s, (double*)s->w_lag_s);
+  DONE(c);
+}
+
+int les_lagrange_scalar_sdep(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                             double* theta, const les_sgs_fields* m, const les_scalar_sgs_fields* s) {
+  GUARD(c);
+  if (!m || !s) return -1;
+  if (!c->wt_s) { c->err = "les_lagrange_scalar_sdep: call les_set_scalars first"; return 1; }
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_lagrange_scalar<double>(c, Sc, u, v, w, theta, to_ptrs(m), to_sptrs(s), false);
+  DONE(c);
+}
+
+int les_sgs_stag_scalar(les_ctx* c, double* const grad[9], const double* u, const double* v, const double* w,
+                        double* theta, double* const tau[6], const les_sgs_fields* m,
+                        const les_scalar_sgs_fields* s) {
+  GUARD(c);
+  if (!m || !s) return -1;
+  if (!c->wt_s) { c->err = "les_sgs_stag_scalar: call les_set_scalars first"; return 1; }
+  const double* g[9];
+  for (int i = 0; i < 9; ++i) g[i] = grad[i];
+  const les::ScalarSgsPtrs ss = to_sptrs(s);
+  op_sgs_stag<double>(c, g, u, v, w, tau, to_ptrs(m), theta, &ss);
+  DONE(c);
+}
+
+int les_state_init_scalar_ic(les_ctx* c, uint64_t seed, double T_init, double z_inv, double inv_strength,
+                             double amp_K, double q_init) {
+  GUARD(c);
+  if (!c->st || !c->st->scal) { c->err = "les_state_init_scalar_ic: state has no scalars"; return 1; }
+  State* s = c->st;
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  const size_t pl = (size_t)g.ld * g.ny, n3 = pl * g.nz, n2 = (size_t)g.nx * g.ny;
+  // FP32-bulk stores deviations from the initial reference values
+  c->scal.theta_ref = s->prec ? T_init / P.T_scale : 0.0;
+  c->scal.q_ref = s->prec ? q_init / P.q_scale : 0.0;
+  std::vector<double> th(n3, 0.0), q(n3, 0.0), pr(n3, P.Pr);
+  for (int k = 0; k < g.nz - 1; ++k) {
+    const double z = (k + 0.5) * g.dz;
+    const double tk = T_init + (z > z_inv ? inv_strength * (z - z_inv) * P.z_i : 0.0);
+    for (int y = 0; y < g.ny; ++y)
+      for (int x = 0; x < g.nx; ++x) {
+        const size_t i = pl * k + x + (size_t)g.ld * y;
+        double t = tk;
+        if (z < z_inv) {
+          const uint64_t idx = (((uint64_t)3 * g.nz + k) * g.ny + y) * g.nx + x;
+          const double U = (double)(splitmix(seed, idx) >> 11) * (1.0 / 9007199254740992.0);
+          t += amp_K * (2.0 * U - 1.0);
+        }
+        th[i] = t / P.T_scale;
+        q[i] = q_init / P.q_scale;
+      }
+  }
+  for (size_t i = 0; i < n3; ++i)
+    if ((int)(i % g.ld) < g.nx && i < pl * (g.nz - 1)) {
+      th[i] -= c->scal.theta_ref;
+      q[i] -= c->scal.q_ref;
+    }
+  const size_t top = pl * (g.nz - 1);
+  const double gt = g.Lz > 1.0 ? P.dTdz_top / P.T_scale * P.z_i * g.dz : 0.0;
+  const double gq = g.Lz > 1.0 ? P.dqdz_top / P.q_scale * P.z_i * g.dz : 0.0;
+  for (size_t i = 0; i < pl; ++i) {
+    if ((int)(i % g.ld) >= g.nx) continue;
+    th[top + i] = th[top - pl + i] + gt;
+    q[top + i] = q[top - pl + i] + gq;
+  }
+  auto up = [&](const char* name, const std::vector<double>& h) {
+    void* d = state_ptr(c, name);
+    if (s->prec) {
+      std::vector<float> f(n3);
+      for (size_t i = 0; i < n3; ++i) f[i] = (float)h[i];
+      check(c, hipMemcpy(d, f.data(), n3 * 4, hipMemcpyHostToDevice), "scalar ic upload");
+    } else {
+      check(c, hipMemcpy(d, h.data(), n3 * 8, hipMemcpyHostToDevice), "scalar ic upload");
+    }
+  };
+  check(c, hipStreamSynchronize(c->stream), "scalar ic sync");
+  up("theta", th);
+  up("qmix", q);
+  up("Pr_t", pr);
+  std::vector<double> one(n2, 1.0);
+  check(c, hipMemcpy(state_ptr(c, "phi_h"), one.data(), n2 * 8, hipMemcpyHostToDevice), "phi_h");
+  check(c, hipMemset(state_ptr(c, "psi_h"), 0, n2 * 8), "psi_h");
+  check(c, hipMemset(state_ptr(c, "psi_h0"), 0, n2 * 8), "psi_h0");
+  c->glm_init = 0;
+  c->gqn_init = 0;
   DONE(c);
 }
 

@@ -41,6 +41,22 @@ struct Arena {
   std::vector<size_t> cap;
 };
 
+// Scalar transport switches and constants (param.f90:53,72-90,139-146,174),
+// the lbc=1 surface of bottombc.f90:26-63.
+struct ScalarParams {
+  int enable = 0;   // S_FLAG: theta and q transported
+  int model = 5;    // 7: Pr_ = Pr_t from lagrange_scalar_Sdep; else constant Pr
+  int initsc = 1;   // scalar history given: no Euler scalar step at jt == 1
+  int passive = 0;  // passive_scalar: no buoyancy in RHSz
+  double Pr = 0.55, z_i = 1000.0, u_star = 1.0, T_scale = 400.0, q_scale = 100.0, g = 9.81;
+  double dTdz_top = 0.0, dqdz_top = 0.0;  // K/m, (g/kg)/m at the lid
+  // FP32-bulk state stores theta - theta_ref and q - q_ref: a float around
+  // 0.7 keeps only ~4 digits of a 1e-4 perturbation.  Every equation but
+  // calcbeta and obukhov sees gradients only; those two add the reference
+  // back in FP64.  0 in FP64 mode and for the routine-level entry points.
+  double theta_ref = 0.0, q_ref = 0.0;
+};
+
 struct State;  // whole-step device state (les_capi.hip)
 
 struct ProfRec {
@@ -74,6 +90,13 @@ struct les_ctx {
   hipGraphExec_t gexec = nullptr;
   int g_start_mod = -1, g_use_const = -1, g_swap = -1;
   long eager_steps = 0;  // steps run eagerly since the state was initialised
+  // scalars (les_set_scalars)
+  les::ScalarParams scal;
+  double* wt_s = nullptr;          // (nx,ny) kinematic surface fluxes, device
+  double* wq_s = nullptr;
+  double zo_avg = 0.0;             // exp(mean log zo), lagrange_scalar_Sdep.f90:110
+  double wt_mean = 0.0;            // sum(wt_s)/(nx*ny), lagrange_scalar_Sdep.f90:114
+  int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
   // host-address -> device mirror registry (les_mirror)
   std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };
@@ -83,11 +106,14 @@ namespace les {
 // scratch slots
 enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
-  S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE, S_NSLOTS
+  S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
+  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 
 template <typename T> const Tables<T>& tables(const les_ctx* c);
+template <> const Tables<double>& tables<double>(const les_ctx* c);
+template <> const Tables<float>& tables<float>(const les_ctx* c);
 
 // ---- launchers (les_kernels.hip); all enqueue on c->stream ----
 // horizontal spectral operators, fields (ld,ny,nlev)
@@ -108,7 +134,7 @@ void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, co
 template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
-                   const T* divtx, const T* divty, const T* divtz, int euler_first);
+                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta = nullptr);
 template <typename T>
 void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz, T* tyz,
                    const double* psi_m, const double* psi_m0, const double* phi_m, double* ustar_avg);
@@ -118,13 +144,52 @@ struct SgsPtrs {  // device pointers, (ld,ny,nz) unless noted
   void *TnLMMM, *TnQNNN, *L11t, *L22t, *L33t, *Q11t, *Q22t, *Q33t;
   void *Beta_avg, *Betaclip_avg;  // (nz), double in every precision
 };
+// scalar Lagrangian SGS state (sgsmodule.f90 G_*, u_lag_s, Pr_t) plus the
+// surface functions the scalar model reads (bottombc phi_h, psi_m, (nx,ny) double)
+struct ScalarSgsPtrs {
+  void *G_LM, *G_MM, *G_QN, *G_NN, *u_lag_s, *v_lag_s, *w_lag_s, *Pr_t, *Cs2_Pr;
+  const double *phi_h, *psi_m;
+};
 template <typename T> void op_calc_sij(les_ctx* c, const T* const g[9], T* const S[6]);
 template <typename T> void op_interpolag(les_ctx* c, const SgsPtrs& s);
+template <typename T> void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl);
+template <typename T> T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w);
 template <typename T>
 void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s);
+// theta/ss may be null (model 5); with ss the model-7 scalar model runs after lagrange_Sdep
 template <typename T>
 void op_sgs_stag(les_ctx* c, const T* const g[9], const T* u, const T* v, const T* w, T* const tau[6],
-                 const SgsPtrs& s);
+                 const SgsPtrs& s, T* theta = nullptr, const ScalarSgsPtrs* ss = nullptr);
+
+// ---- scalars (les_scalars.hip) ----
+// obukhov, scalars_module.f90:508-690.  sgs_t3/sgs_q3: level-1 planes (ld,ny)
+// double; psi/phi/ustar (nx,ny) double.
+// th_ref/q_ref: references added to the stored scalars (see ScalarParams)
+template <typename T>
+void op_obukhov(les_ctx* c, const T* u, const T* v, const T* theta, double* sgs_t3, double* sgs_q3,
+                double* psi_m, double* psi_m0, double* psi_h, double* psi_h0, double* phi_m, double* phi_h,
+                double* ustar_avg, double th_ref = 0.0);
+// calcbeta, scalars_module.f90:175-228
+template <typename T>
+void op_calcbeta(les_ctx* c, const T* theta, const T* qmix, T* beta, double th_ref = 0.0, double q_ref = 0.0);
+// one or two scalars through scalar_RHS_calc (+ step_scalar when do_step)
+struct ScalarJob {
+  void *s, *dsdx, *dsdy, *dsdz, *RHS, *RHS_f;  // (ld,ny,nz)
+  double* sgs_vert1;                           // level-1 plane (ld,ny)
+  int which;                                   // 0 theta, 1 q
+};
+template <typename T>
+void op_scalar_dz(les_ctx* c, const ScalarJob* jobs, int ns, const double* phi_h, const double* ustar_avg,
+                  int surface_only);
+template <typename T>
+void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const T* v, const T* w, const T* Nu_t,
+                   const T* Pr_t, int do_step, int euler);
+template <typename T> void op_step_scalar(les_ctx* c, int which, T* s, const T* RHS, const T* RHS_f);
+// lagrange_scalar_Sdep.f90:1-467 (theta Nyquist-filtered in place); with
+// reuse_mom the momentum model's filtered fields of this step are reused
+template <typename T>
+void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, T* theta,
+                        const SgsPtrs& m, const ScalarSgsPtrs& ss, bool reuse_mom);
 
 template <typename T>
 void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy);

@@ -13,489 +13,21 @@
 // Loaders / storers are functors so that pointwise work before and after a
 // transform (vorticity, staggered products, d/dz terms) is fused into the
 // transform kernels instead of making extra passes over HBM.
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-
-#include "les_internal.h"
+#include "les_xform.h"
 
 namespace les {
-
-constexpr double BOGUS = -1234567890.0;  // param.f90:50
-constexpr double VONK = 0.41;            // param.f90:98
-constexpr double TADV1 = 1.5, TADV2 = -0.5;
 
 void check(les_ctx* c, hipError_t e, const char* what) {
   if (e != hipSuccess && c->err.empty()) {
     c->err = std::string(what) + ": " + hipGetErrorString(e);
   }
 }
-#define LCHK(c, call) check((c), (call), #call)
 
 template <>
 const Tables<double>& tables<double>(const les_ctx* c) { return c->td; }
 template <>
 const Tables<float>& tables<float>(const les_ctx* c) { return c->tf; }
 
-template <typename T>
-static const cx<T>* tw_for(const les_ctx* c, int n) {
-  const Tables<T>& t = tables<T>(c);
-  if (n == c->g.nx) return t.tw_nx;
-  if (n == c->g.ny) return t.tw_ny;
-  if (n == c->g.nx2) return t.tw_nx2;
-  return t.tw_ny2;
-}
-static const FftPlan& plan_for(const les_ctx* c, int n) {
-  if (n == c->g.nx) return c->p_nx;
-  if (n == c->g.ny) return c->p_ny;
-  if (n == c->g.nx2) return c->p_nx2;
-  return c->p_ny2;
-}
-
-// ------------------------------------------------------------------------
-// row transforms
-// ------------------------------------------------------------------------
-// Rows are grouped by plane (field x level): blockIdx.y = plane,
-// blockIdx.x = group of P row pairs inside it, so loaders and storers get a
-// block-uniform plane index (no per-element integer division).  N > 0 is the
-// compile-time transform length; N == 0 runs the same code with run-time n.
-template <int N>
-struct RowGeom {
-  static constexpr int TPF = N > 0 ? N / 4 : 64;
-  static constexpr int P = (256 / TPF) > 0 ? 256 / TPF : 1;
-  static constexpr int BLOCK = P * TPF;
-};
-
-template <typename T, int N, class Loader>
-__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
-    k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
-              int kxo) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  const int n = N > 0 ? N : p.n;
-  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
-  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
-  cx<T>* buf = tw + n;
-  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
-  const int plane = blockIdx.y;
-  const int npp = ny / 2;
-  const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
-    const int t = e / n, x = e - t * n, pp = pair0 + t;
-    cx<T> z = {T(0), T(0)};
-    if (pp < npp) {
-      z.x = L(plane, 2 * pp, x);
-      z.y = L(plane, 2 * pp + 1, x);
-    }
-    buf[e] = z;
-  }
-  __syncthreads();
-  if constexpr (N > 0)
-    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, -1);
-  else
-    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
-  for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
-    const int t = e / kxo, k = e - t * kxo, pp = pair0 + t;
-    if (pp >= npp) continue;
-    const cx<T> zk = buf[t * n + k], zn = buf[t * n + (k ? n - k : 0)];
-    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
-    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
-    const long row = (long)plane * ny + 2 * pp;
-    out[row * orow + k] = A;
-    out[(row + 1) * orow + k] = B;
-  }
-}
-
-template <typename T, int N, class Storer>
-__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
-    k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
-              const cx<T>* __restrict__ twg, int ldo) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  const int n = N > 0 ? N : p.n;
-  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
-  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
-  cx<T>* buf = tw + n;
-  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
-  const int plane = blockIdx.y;
-  const int npp = ny / 2;
-  const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
-    const int t = e / n, j = e - t * n, pp = pair0 + t;
-    cx<T> z = {T(0), T(0)};
-    if (pp < npp) {
-      const cx<T>* a = in + ((long)plane * ny + 2 * pp) * irow;
-      const cx<T>* b = a + irow;
-      if (j == 0) {
-        z = {a[0].x, b[0].x};  // Im of the DC entries is ignored (halfcomplex c2r)
-      } else if (j < kxi) {
-        const cx<T> A = a[j], B = b[j];
-        z = {A.x - B.y, A.y + B.x};
-      } else if (n - j < kxi) {
-        const cx<T> A = a[n - j], B = b[n - j];
-        z = {A.x + B.y, B.x - A.y};
-      }
-    }
-    buf[e] = z;
-  }
-  __syncthreads();
-  if constexpr (N > 0)
-    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, +1);
-  else
-    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
-  for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
-    const int t = e / ldo, x = e - t * ldo, pp = pair0 + t;
-    if (pp >= npp) continue;
-    T v0 = T(0), v1 = T(0);
-    if (x < n) {
-      v0 = buf[t * n + x].x;
-      v1 = buf[t * n + x].y;
-    }
-    S(plane, 2 * pp, x, v0);
-    S(plane, 2 * pp + 1, x, v1);
-  }
-}
-
-// compile-time lengths with specialised kernels; anything else runs the
-// run-time-length kernels
-#define LES_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384)
-
-template <typename T, int N, class L>
-static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo) {
-  const FftPlan& p = plan_for(c, n);
-  const int tpf = n / 4;
-  int P = 256 / tpf;
-  if (P < 1) P = 1;
-  const int threads = P * tpf;
-  const long blocks = (ny / 2 + P - 1) / P;
-  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
-  ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
-  hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
-                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo);
-  LCHK(c, hipGetLastError());
-}
-
-template <typename T, class L>
-static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo, int ny) {
-  const long nplanes = nrows / ny;
-  switch (n) {
-#define LES_CASE(NN) \
-  case NN:           \
-    return launch_row_fwd_n<T, NN>(c, ld, out, orow, nplanes, ny, n, kxo);
-    LES_FFT_SIZES(LES_CASE)
-#undef LES_CASE
-    default:
-      return launch_row_fwd_n<T, 0>(c, ld, out, orow, nplanes, ny, n, kxo);
-  }
-}
-
-template <typename T, int N, class S>
-static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
-                             int n, int ldo) {
-  const FftPlan& p = plan_for(c, n);
-  const int tpf = n / 4;
-  int P = 256 / tpf;
-  if (P < 1) P = 1;
-  const int threads = P * tpf;
-  const long blocks = (ny / 2 + P - 1) / P;
-  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
-  ProfScope ps(c, "row_inv", (double)nplanes * ny * (kxi * sizeof(cx<T>) + ldo * sizeof(T) * st.writes()));
-  hipLaunchKernelGGL((k_row_inv<T, N, S>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
-                     in, irow, kxi, st, ny, p, tw_for<T>(c, n), ldo);
-  LCHK(c, hipGetLastError());
-}
-
-template <typename T, class S>
-static void launch_row_inv(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nrows, int n,
-                           int ldo, int ny) {
-  const long nplanes = nrows / ny;
-  switch (n) {
-#define LES_CASE(NN) \
-  case NN:           \
-    return launch_row_inv_n<T, NN>(c, in, irow, kxi, st, nplanes, ny, n, ldo);
-    LES_FFT_SIZES(LES_CASE)
-#undef LES_CASE
-    default:
-      return launch_row_inv_n<T, 0>(c, in, irow, kxi, st, nplanes, ny, n, ldo);
-  }
-}
-
-// loaders / storers ------------------------------------------------------
-// row index = (field * nlev + level) * ny + y; fields are (ld, ny, nlev).
-// plane = field * planes_per_field + level; fields are (ld, ny, levels)
-template <typename T>
-struct LoadFields {
-  const T* p[4];
-  int ppf, ny, ld;
-  T scale;
-  static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int plane, int y, int x) const {
-    const int f = plane / ppf, r = plane - f * ppf;
-    return scale * p[f][((long)r * ny + y) * ld + x];
-  }
-};
-template <typename T>
-struct StoreFields {
-  T* p[12];
-  int ppf, ny;
-  int ld, nx;  // pads (x >= nx) are written 0
-  static constexpr double writes() { return 1.0; }
-  __device__ void operator()(int plane, int y, int x, T v) const {
-    const int f = plane / ppf, r = plane - f * ppf;
-    p[f][((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
-  }
-};
-
-// ------------------------------------------------------------------------
-// column (y) kernel with fused spectral operation
-// ------------------------------------------------------------------------
-enum Kind : int { K_ID = 0, K_IKX = 1, K_IKY = 2, K_G = 3 };
-
-struct ColArgs {
-  int ny_in, ny_out, do_fwd, do_inv;
-  int nki;               // input kx columns present
-  int nout;              // 1..3 outputs
-  int kind[3];
-  int nko[3];            // output kx columns written
-  long in_row, in_plane, in2_plane;
-  long out_row[3], out_plane[3];
-  int lhG;               // row length of the G tables (lh)
-};
-
-__device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
-  if (nyi == nyo) return (yo == nyo / 2) ? -1 : yo;
-  if (nyo > nyi) {  // zero-pad (padd.f90)
-    if (yo < nyi / 2) return yo;
-    if (yo >= nyo - nyi / 2 + 1) return yo - nyo + nyi;
-    return -1;
-  }
-  if (yo < nyo / 2) return yo;  // truncate (unpadd.f90)
-  if (yo == nyo / 2) return -1;
-  return yo + nyi - nyo;
-}
-
-template <int TX, int NYI, int NYO>
-struct ColGeom {
-  static constexpr int NMAX = NYI > NYO ? NYI : NYO;
-  static constexpr int B0 = (TX * NMAX / 8 + 63) / 64 * 64;
-  static constexpr int BLOCK = B0 < 64 ? 64 : B0;
-};
-
-// NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
-template <typename T, int TX, int NYI, int NYO>
-__global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
-    k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
-          FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
-          const T* __restrict__ kxt, const T* __restrict__ kyt, const T* __restrict__ G0, const T* __restrict__ G1,
-          const T* __restrict__ G2) {
-  constexpr bool CT = NYI > 0;
-  constexpr int BLK = CT ? ColGeom<TX, NYI, NYO>::BLOCK : 0;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nyi = CT ? NYI : a.ny_in;
-  const int nyo = CT ? NYO : a.ny_out;
-  const int bd = CT ? BLK : (int)blockDim.x;
-  cx<T>* twi = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* two = twi + nyi;
-  cx<T>* A = two + nyo;
-  cx<T>* B = A + nyi * TX;
-  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = twin[i];
-  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = twout[i];
-  auto fwd = [&](cx<T>* buf) {
-    if constexpr (CT)
-      fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, twi, -1);
-    else
-      fft_lds<T, true, 4>(buf, pin, TX, 1, TX, twi, -1);
-  };
-  auto inv = [&](cx<T>* buf) {
-    if constexpr (CT)
-      fft_lds_c<T, (CT ? NYO : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, two, +1);
-    else
-      fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
-  };
-  const int kx0 = blockIdx.x * TX;
-  const long pl = blockIdx.y;
-  {
-    const cx<T>* src = in + pl * a.in_plane;
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      A[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
-    }
-  }
-  __syncthreads();
-  if (a.do_fwd) fwd(A);
-  if (in2) {
-    // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
-    const cx<T>* src = in2 + pl * a.in2_plane;
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      B[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
-    }
-    __syncthreads();
-    fwd(B);
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      cx<T> r = {T(0), T(0)};
-      if (y != nyi / 2 && kk < a.nki) {
-        const T kx = kxt[kk], ky = kyt[y];
-        const cx<T> X = A[e], Y = B[e];
-        r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
-      }
-      A[e] = r;
-    }
-    __syncthreads();
-    if (a.do_inv) inv(A);
-    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = A[e];
-    }
-    return;
-  }
-  for (int j = 0; j < a.nout; ++j) {
-    if (kx0 >= a.nko[j]) continue;  // uniform over the block
-    const int kind = a.kind[j];
-    const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
-    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
-      const int yo = e / TX, c = e - yo * TX, kk = kx0 + c;
-      const int yi = map_row(yo, nyi, nyo);
-      cx<T> v = {T(0), T(0)};
-      if (yi >= 0 && kk < a.nki) {
-        v = A[yi * TX + c];
-        if (kind == K_IKX) {
-          const T k = kxt[kk];
-          v = {-k * v.y, k * v.x};
-        } else if (kind == K_IKY) {
-          const T k = kyt[yo];
-          v = {-k * v.y, k * v.x};
-        } else if (kind == K_G) {
-          const T g = G[(long)yo * a.lhG + kk];
-          v = {g * v.x, g * v.y};
-        }
-      }
-      B[e] = v;
-    }
-    __syncthreads();
-    if (a.do_inv) inv(B);
-    cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
-    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = B[e];
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T, int NYI, int NYO>
-static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
-                         const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
-  constexpr int TX = sizeof(T) == 8 ? 8 : 16;
-  const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
-  // the FFT stages need blockDim >= TX*n/8 (at most 4 butterflies per thread)
-  int threads = NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
-  if (threads < 64) threads = 64;
-  if (threads > 1024) {
-    c->err = "k_col: column tile too large";
-    return;
-  }
-  const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
-  if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)sm));
-  const Tables<T>& t = tables<T>(c);
-  const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
-  double bytes = 0;
-  {
-    const double es = 2.0 * sizeof(T);
-    bytes += (double)nplanes * a.ny_in * (ncols < a.nki ? ncols : a.nki) * es * (in2 ? 2 : 1);
-    for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
-  }
-  ProfScope ps(c, "col", bytes);
-  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
-                     o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
-                     tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
-  LCHK(c, hipGetLastError());
-}
-
-#define LES_COL_PAIRS(X)                                                                                       \
-  X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
-  X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
-
-template <typename T>
-static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
-                       const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
-                       const T* G2 = nullptr) {
-#define LES_CASE(I, O)                                                                        \
-  if (a.ny_in == I && a.ny_out == O)                                                          \
-    return launch_col_n<T, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
-  LES_COL_PAIRS(LES_CASE)
-#undef LES_CASE
-  launch_col_n<T, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
-}
-
-static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {
-  ColArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.ny_in = ny_in;
-  a.ny_out = ny_out;
-  a.do_fwd = 1;
-  a.do_inv = 1;
-  a.nki = nki;
-  a.in_row = in_row;
-  a.in_plane = in_plane;
-  return a;
-}
-static void col_out(ColArgs& a, int j, int kind, int nko, long out_row, long out_plane) {
-  a.kind[j] = kind;
-  a.nko[j] = nko;
-  a.out_row[j] = out_row;
-  a.out_plane[j] = out_plane;
-  if (a.nout < j + 1) a.nout = j + 1;
-}
-
-// ------------------------------------------------------------------------
-// simple elementwise kernels
-// ------------------------------------------------------------------------
-template <typename T>
-__global__ void k_fill(T* a, size_t n, T v) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
-}
-template <typename T>
-void op_fill(les_ctx* c, T* a, size_t n, T val) {
-  if (!n) return;
-  size_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  {
-    ProfScope _ps(c, "fill", (double)n * sizeof(T));
-    hipLaunchKernelGGL((k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, a, n, val);
-  }
-  LCHK(c, hipGetLastError());
-}
-template <typename T>
-__global__ void k_copy(T* d, const T* s, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
-}
-template <typename T>
-void op_copy(les_ctx* c, T* dst, const T* src, size_t n) {
-  if (!n) return;
-  size_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  {
-    ProfScope _ps(c, "copy", 2.0 * n * sizeof(T));
-    hipLaunchKernelGGL((k_copy<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, dst, src, n);
-  }
-  LCHK(c, hipGetLastError());
-}
-// complex-(BOGUS,0) pattern written by press_stag_array.f90:288-290
-template <typename T>
-__global__ void k_fill_bogus_cplx(T* a, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    a[i] = (i & 1) ? T(0) : T(BOGUS);
-}
-
-static dim3 grid1(size_t n, int bs = 256) {
-  size_t b = (n + bs - 1) / bs;
-  if (b > 1048576) b = 1048576;
-  return dim3((unsigned)b);
-}
 
 // ------------------------------------------------------------------------
 // filt_da.f90 / ddx.f90 / ddy.f90 / ddxy.f90
@@ -897,12 +429,14 @@ struct StoreConvecAB2 {
   int ppf, ny, ld, nx;
   T dt, force;
   int euler;
+  const T* beta;  // buoyancy added to RHSz (main.f90:312-317), or null
   static constexpr double writes() { return 5.0; }
   __device__ void operator()(int plane, int y, int x, T v) const {
     const int f = plane / ppf, r = plane - f * ppf;
     const long i = ((long)r * ny + y) * ld + x;
     const T conv = x < nx ? v : T(0);
-    const T rhs = -conv - dv[f][i];
+    T rhs = -conv - dv[f][i];
+    if (f == 2 && beta) rhs = rhs + beta[i];
     R[f][i] = rhs;
     T fo;
     if (euler) {
@@ -992,13 +526,13 @@ void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, co
 template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
-                   const T* divtx, const T* divty, const T* divtz, int euler_first) {
+                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta) {
   OpTag _tag(c, "convec");
   const Grid& g = c->g;
   const int nz = g.nz;
   cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy);
   StoreConvecAB2<T> SC{{RHSx, RHSy, RHSz}, {RHSx_f, RHSy_f, RHSz_f}, {divtx, divty, divtz}, {u, v, w},
-                       nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first};
+                       nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
   launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * (nz - 1) * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
   T* tops[9] = {RHSx, RHSy, RHSz, u, v, w, RHSx_f, RHSy_f, RHSz_f};
@@ -1060,7 +594,7 @@ __global__ void k_calc_sij(const T* __restrict__ ux_, const T* __restrict__ uy_,
                            const T* __restrict__ wx_, const T* __restrict__ wy_, const T* __restrict__ wz_,
                            T* S11, T* S12, T* S13, T* S22, T* S23, T* S33, int nx, int ld, int ny, int nz,
                            const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w, T* u_lag,
-                           T* v_lag, T* w_lag) {
+                           T* v_lag, T* w_lag, T* u_lag_s, T* v_lag_s, T* w_lag_s) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const int k = (int)(i / pl);
@@ -1069,6 +603,11 @@ __global__ void k_calc_sij(const T* __restrict__ ux_, const T* __restrict__ uy_,
       u_lag[i] = u_lag[i] + u[i];
       v_lag[i] = v_lag[i] + v[i];
       w_lag[i] = w_lag[i] + w[i];
+    }
+    if (u_lag_s) {  // sgs_stag.f90:66-68
+      u_lag_s[i] = u_lag_s[i] + u[i];
+      v_lag_s[i] = v_lag_s[i] + v[i];
+      w_lag_s[i] = w_lag_s[i] + w[i];
     }
     T s11 = T(0), s12 = T(0), s13 = T(0), s22 = T(0), s23 = T(0), s33 = T(0);
     if (x < nx) {
@@ -1104,25 +643,22 @@ void op_calc_sij(les_ctx* c, const T* const gr[9], T* const S[6]) {
   hipLaunchKernelGGL((k_calc_sij<T>), grid1((size_t)g.ld * g.ny * g.nz), dim3(256), 0, c->stream, gr[0], gr[1],
                      gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx,
                      g.ld, g.ny, g.nz, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr,
-                     (T*)nullptr, (T*)nullptr);
+                     (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr, (T*)nullptr);
   LCHK(c, hipGetLastError());
 }
 
-template <typename T>
-__device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
-  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
-}
 
 template <typename T>
 __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, const T* __restrict__ S13,
                           const T* __restrict__ S22, const T* __restrict__ S23, const T* __restrict__ S33,
                           const T* __restrict__ Cs, T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy,
-                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2) {
+                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2, T* Pr_t, T Pr) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const int k = (int)(i / pl);
     const int x = (int)((i - k * pl) % ld);
     if (use_const) Cs_out[i] = cs_const;  // sgs_stag.f90:166-173, whole arrays
+    if (use_const && Pr_t) Pr_t[i] = Pr;
     if (k == nz - 1) {                    // sgs_stag.f90:356-364
       txx[i] = T(BOGUS); txy[i] = T(BOGUS); tyy[i] = T(BOGUS); tzz[i] = T(BOGUS);
       txz[i] = T(0); tyz[i] = T(0);
@@ -1205,27 +741,31 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
   }
 }
 
+// interpolag_Sdep.f90:2-380 on (F_LM, F_MM, F_QN, F_NN) with u_lag; the same
+// scheme serves interpolag_scalar_Sdep.f90:1-316 on G_* with u_lag_s
 template <typename T>
-void op_interpolag(les_ctx* c, const SgsPtrs& s) {
-  OpTag _tag(c, "interpolag");
+void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl) {
   const Grid& g = c->g;
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
   T* FF = (T*)scratch(c, S_FF, 4 * n3 * sizeof(T));
-  op_copy<T>(c, FF, (const T*)s.F_LM, n3);
-  op_copy<T>(c, FF + n3, (const T*)s.F_MM, n3);
-  op_copy<T>(c, FF + 2 * n3, (const T*)s.F_QN, n3);
-  op_copy<T>(c, FF + 3 * n3, (const T*)s.F_NN, n3);
+  for (int q = 0; q < 4; ++q) op_copy<T>(c, FF + q * n3, F[q], n3);
   {
     ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, FF, (T*)s.F_LM, (T*)s.F_MM,
-                       (T*)s.F_QN, (T*)s.F_NN, g.nx, g.ld, g.ny, g.nz, T(g.cs_count), T(g.dt), T(g.dx), T(g.dy),
-                       T(g.dz));
+                       (const T*)ul, (const T*)vl, (const T*)wl, FF, F[0], F[1], F[2], F[3], g.nx, g.ld, g.ny, g.nz,
+                       T(g.cs_count), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
   }
   LCHK(c, hipGetLastError());
-  op_fill<T>(c, (T*)s.u_lag, n3, T(0));
-  op_fill<T>(c, (T*)s.v_lag, n3, T(0));
-  op_fill<T>(c, (T*)s.w_lag, n3, T(0));
+  op_fill<T>(c, ul, n3, T(0));
+  op_fill<T>(c, vl, n3, T(0));
+  op_fill<T>(c, wl, n3, T(0));
+}
+
+template <typename T>
+void op_interpolag(les_ctx* c, const SgsPtrs& s) {
+  OpTag _tag(c, "interpolag");
+  T* F[4] = {(T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN};
+  op_interpolag4<T>(c, F, (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag);
 }
 
 // ------------------------------------------------------------------------
@@ -1262,7 +802,6 @@ struct LoadLag {  // field order: ub vb wb | uu uv uw vv vw ww | S11 S12 S13 S22
   }
 };
 
-__device__ __forceinline__ double fq(double x) { return (double)(float)x; }  // Fortran real(x)
 
 template <typename T>
 __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F4, const T* __restrict__ S11,
@@ -1272,12 +811,13 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
                                T* Q33t, double* red, int nx, int ld, int ny, int nz, double cst, double opftdelta,
                                double lagran_dt, double expo, int init_lm, int init_qn) {
   const long pl = (long)ld * ny, n3 = pl * nz;
-  const int n2 = nx * ny;
+  const int n2 = ld * ny;  // pad columns too: the reference loops over ld there
   const double e24 = (double)1e-24f;
   const int k = blockIdx.y;  // one level per grid row: plane sums reduce per block
   double acc2 = 0.0, acc4 = 0.0, accc = 0.0;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n2; r += gridDim.x * blockDim.x) {
-    const int y = r / nx, x = r - y * nx;
+    const int y = r / ld, x = r - y * ld;
+    const bool pad = x >= nx;  // the filtered fields are 0 there; lagrange_Sdep.f90:295-296,344-420
     const long i = (long)k * pl + (long)y * ld + x;
     auto f2 = [&](int f) { return (double)F2[f * n3 + i]; };
     auto f4 = [&](int f) { return (double)F4[f * n3 + i]; };
@@ -1307,7 +847,7 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
     const double QN = Qv[0] * N[0] + Qv[3] * N[3] + Qv[5] * N[5] + 2.0 * (Qv[1] * N[1] + Qv[2] * N[2] + Qv[4] * N[4]);
     const double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
     double flm = FLM[i], fmm = FMM[i], fqn = FQN[i], fnn = FNN[i];
-    if (init_lm) { fmm = MM; flm = 0.03 * MM; }
+    if (init_lm) { fmm = pad ? 1.0 : MM; flm = pad ? 1.0 : 0.03 * MM; }
     double Tn = fmax(fq(flm * fmm), e24);
     TnLM[i] = (T)Tn;
     Tn = opftdelta * pow(Tn, -0.125);
@@ -1317,8 +857,8 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
     flm = (epsi * LM + (1.0 - epsi) * flm);
     fmm = (epsi * MM + (1.0 - epsi) * fmm);
     flm = fmax(e24, fq(flm));
-    const double cs2d = fmax(e24, fq(flm / fmm));
-    if (init_qn) { fnn = NN; fqn = 0.03 * NN; }
+    const double cs2d = fmax(e24, fq(pad ? e24 : flm / fmm));
+    if (init_qn) { fnn = pad ? 1.0 : NN; fqn = pad ? 1.0 : 0.03 * NN; }
     Tn = fmax(fq(fqn * fnn), e24);
     TnQN[i] = (T)Tn;
     Tn = opftdelta * pow(Tn, -0.125);
@@ -1328,17 +868,19 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
     fqn = (epsi * QN + (1.0 - epsi) * fqn);
     fnn = (epsi * NN + (1.0 - epsi) * fnn);
     fqn = fmax(e24, fq(fqn));
-    const double cs4d = fmax(e24, fq(fqn / fnn));
+    const double cs4d = fmax(e24, fq(pad ? e24 : fqn / fnn));
     double beta = expo == 1.0 ? cs4d / cs2d : pow(cs4d / cs2d, expo);  // exponent ln2/(ln4-ln2) is 1
     if (k == nz - 1) beta = 1.0;
     const double bclip = fmax(fq(beta), fq(0.125));
-    const double cso = fmax(e24, fq(cs2d / bclip));
+    const double cso = fmax(e24, fq(pad ? e24 : cs2d / bclip));
     FLM[i] = (T)flm; FMM[i] = (T)fmm; FQN[i] = (T)fqn; FNN[i] = (T)fnn;
     Beta[i] = (T)beta;
     Cs[i] = (T)cso;
-    acc2 += cs2d;
-    acc4 += cs4d;
-    accc += cso;
+    if (!pad) {
+      acc2 += cs2d;
+      acc4 += cs4d;
+      accc += cso;
+    }
   }
   __shared__ double sh[3][256];
   sh[0][threadIdx.x] = acc2;
@@ -1353,24 +895,6 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
   if (threadIdx.x < 3) atomicAdd(red + 3 * k + threadIdx.x, sh[threadIdx.x][0]);
 }
 
-// pads of the lagrange outputs: lagrange_Sdep.f90:295-296,344-345,353-354,383-384,404,419-420
-template <typename T>
-__global__ void k_lag_pads(T* FLM, T* FMM, T* FQN, T* FNN, T* Beta, T* Cs, int nx, int ld, int ny, int nz,
-                           int init_lm, int init_qn) {
-  const long pl = (long)ld * ny;
-  const int np = (ld - nx) * ny * nz;
-  const double e24 = (double)1e-24f;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < np; q += gridDim.x * blockDim.x) {
-    const int k = q / ((ld - nx) * ny);
-    const int r = q - k * (ld - nx) * ny, y = r / (ld - nx), x = nx + (r - y * (ld - nx));
-    const long i = (long)k * pl + (long)y * ld + x;
-    if (init_lm) { FLM[i] = T(1); FMM[i] = T(1); }
-    if (init_qn) { FQN[i] = T(1); FNN[i] = T(1); }
-    Beta[i] = T(1);
-    Cs[i] = (T)e24;
-  }
-}
-
 __global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_avg, int nz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nz) {
@@ -1379,11 +903,12 @@ __global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_
   }
 }
 
+// The 21 lagrange_Sdep fields (lagrange_Sdep.f90:131-266) filtered at 2 Delta
+// and 4 Delta into the S_LAGF scratch: F2 = fields 0..20, F4 = 21..41, each
+// (ld,ny,nz).  Field order: ub vb wb | uu uv uw vv vw ww | S11..S33 | |S|S_ij.
 template <typename T>
-void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s) {
-  OpTag _tag(c, "lagrange");
+T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w) {
   const Grid& g = c->g;
-  op_interpolag<T>(c, s);
   const int nz = g.nz, k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4;
   const long rows = 21L * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
@@ -1399,12 +924,22 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, 21L * nz, kf, t.G2, t.G4);
   const size_t n3 = (size_t)g.ld * g.ny * nz;
   T* F = (T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
-  // row_inv of 21 fields: StoreFields handles up to 4 fields, so use one
-  // "field" spanning all 21 (rows are contiguous field-major)
+  // row_inv of 21 fields: one "field" spanning all 21 (rows are field-major)
   StoreFields<T> S2{{F, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
   StoreFields<T> S4{{F + 21 * n3, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
+  return F;
+}
+
+template <typename T>
+void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s) {
+  OpTag _tag(c, "lagrange");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  op_interpolag<T>(c, s);
+  T* F = lag_filter_mom<T>(c, S, u, v, w);
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
   double* red = (double*)scratch(c, S_RED, 3 * nz * sizeof(double));
   op_fill<double>(c, red, 3 * (size_t)nz, 0.0);
   const int init_lm = g.inilag && !c->flm_init && (c->jt == g.cs_count || c->jt == 1);
@@ -1412,17 +947,13 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   const double delta = g.delta;
   {
     ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
-    const unsigned bpl = (unsigned)((g.nx * g.ny + 255) / 256 < 64 ? (g.nx * g.ny + 255) / 256 : 64);
+    const unsigned bpl = (unsigned)((g.ld * g.ny + 255) / 256 < 64 ? (g.ld * g.ny + 255) / 256 : 64);
     hipLaunchKernelGGL((k_lag_epilogue<T>), dim3(bpl, nz), dim3(256), 0, c->stream, F, F + 21 * n3,
                        S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
                        (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
                        (T*)s.Q11t, (T*)s.Q22t, (T*)s.Q33t, red, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
                        1.5 * delta, g.dt * (double)g.cs_count, log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
   }
-  LCHK(c, hipGetLastError());
-  hipLaunchKernelGGL((k_lag_pads<T>), grid1((size_t)(g.ld - g.nx) * g.ny * nz), dim3(256), 0, c->stream,
-                     (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN, (T*)s.Beta, (T*)s.Cs_opt2, g.nx, g.ld, g.ny,
-                     nz, init_lm, init_qn);
   LCHK(c, hipGetLastError());
   if (s.Beta_avg && s.Betaclip_avg) {
     hipLaunchKernelGGL(k_lag_avg, dim3((nz + 63) / 64), dim3(64), 0, c->stream, red, (double*)s.Beta_avg,
@@ -1435,17 +966,18 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
 
 template <typename T>
 void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const T* w, T* const tau[6],
-                 const SgsPtrs& s) {
+                 const SgsPtrs& s, T* theta, const ScalarSgsPtrs* ss) {
   OpTag _tag(c, "sgs_stag");
   const Grid& g = c->g;
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
   T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
   T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
   {
-    ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + 3) * n3 * sizeof(T));
+    ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + (ss ? 6 : 3)) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_calc_sij<T>), grid1(n3), dim3(256), 0, c->stream, gr[0], gr[1], gr[2], gr[3], gr[4], gr[5],
                        gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx, g.ld, g.ny, g.nz, u, v, w,
-                       (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag);
+                       (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
+                       ss ? (T*)ss->v_lag_s : (T*)nullptr, ss ? (T*)ss->w_lag_s : (T*)nullptr);
   }
   LCHK(c, hipGetLastError());
   const int jt = c->jt;
@@ -1454,6 +986,7 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   } else if ((jt >= 1 || g.initu) && (jt % g.cs_count == 0)) {
     const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
     op_lagrange<T>(c, Sc, u, v, w, s);
+    if (ss) op_lagrange_scalar<T>(c, Sc, u, v, w, theta, s, *ss, true);  // sgs_stag.f90:150-153
   }
   const int use_const = c->jt_total < 500;
   const T cs0 = (T)(double)(0.16f * 0.16f);
@@ -1462,7 +995,8 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
     ProfScope _ps2(c, "sgs_tau", (6.0 + 1 + 1 + 6) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_sgs_tau<T>), grid1(n3), dim3(256), 0, c->stream, S[0], S[1], S[2], S[3], S[4], S[5],
                        (const T*)s.Cs_opt2, cs0, use_const, (T*)s.Cs_opt2, (T*)s.Nu_t, tau[0], tau[1], tau[2], tau[3],
-                       tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2);
+                       tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2, ss ? (T*)ss->Pr_t : (T*)nullptr,
+                       (T)c->scal.Pr);
   }
   LCHK(c, hipGetLastError());
 }
@@ -1919,14 +1453,16 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
                              const T*, const T*, T*, T*, T*);                                                    \
   template void op_convec_ab2<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,         \
-                                 const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int);           \
+                                 const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int, const T*); \
   template void op_wallstress<T>(les_ctx*, const T*, const T*, T*, T*, T*, T*, const double*, const double*,     \
                                  const double*, double*);                                                        \
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
   template void op_interpolag<T>(les_ctx*, const SgsPtrs&);                                                      \
+  template void op_interpolag4<T>(les_ctx*, T* const[4], T*, T*, T*);                                            \
+  template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*);                      \
   template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \
   template void op_sgs_stag<T>(les_ctx*, const T* const[9], const T*, const T*, const T*, T* const[6],           \
-                               const SgsPtrs&);                                                                  \
+                               const SgsPtrs&, T*, const ScalarSgsPtrs*);                                        \
   template void op_press_stag<T>(les_ctx*, const T*, const T*, const T*, const T*, T*, T*, T*);                  \
   template void op_project<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*);                               \
   template double op_rmsdiv<T>(les_ctx*, const T*, const T*, const T*);                                          \

@@ -1,0 +1,822 @@
+// les_scalars.hip -- scalar transport of the convective configuration on
+// gfx950: potential temperature theta and humidity q (S_FLAG), the
+// Monin-Obukhov surface layer, buoyancy and the scalar Lagrangian
+// scale-dependent SGS model (model 7).
+//
+// Both scalars go through one batched pipeline per step:
+//   scalar_RHS_calc's dealiased advection u.grad(s) uses one forward row pass
+//   over u,v,w and the three gradients of each scalar, the 3/2 zero pad in y,
+//   and k_scalar_big, which keeps the padded physical rows of two z-levels in
+//   LDS, forms the staggered products and transforms them back (the padded
+//   fields never touch HBM).  The SGS flux divergence d/dx, d/dy is a single
+//   spectral divergence pass whose c2r also receives the advection spectrum
+//   (weight -1); its epilogue adds the d/dz flux term and takes the AB2 step
+//   of step_scalar, including the lid rule.
+#include "les_xform.h"
+
+namespace les {
+
+// ------------------------------------------------------------------------
+// deterministic plane sums (one block per level, double accumulation)
+// ------------------------------------------------------------------------
+template <class Fn>
+__global__ void k_level_sum(Fn fn, int nx, int ny, double* out) {
+  const int k = blockIdx.x;
+  double a = 0.0;
+  for (int j = threadIdx.x; j < nx * ny; j += blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    a += fn(k, y, x);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = sh[0];
+}
+
+template <typename T>
+struct SumPlane {  // f(x, y, level 0) + ref
+  const T* f;
+  int ld;
+  double ref;
+  __device__ double operator()(int, int y, int x) const { return (double)f[(long)y * ld + x] + ref; }
+};
+template <typename T>
+struct SumThetaV {  // theta (1 + 0.61e-3 q q_scale) per level, scalars_module.f90:199-206
+  const T *th, *q;
+  int ld, ny;
+  double qs, tref, qref;
+  __device__ double operator()(int k, int y, int x) const {
+    const long i = ((long)k * ny + y) * ld + x;
+    return ((double)th[i] + tref) * (1.0 + 0.61 * 0.001 * ((double)q[i] + qref) * qs);
+  }
+};
+
+// lagrange_scalar_Sdep.f90:111-113: sum |u(1)| over the plane and sum psi_m
+template <typename T>
+__global__ void k_lag_scalar_sums(const T* u, const T* v, const double* psi_m, int nx, int ny, int ld,
+                                  double* out) {
+  double a = 0.0, b = 0.0;
+  for (int j = threadIdx.x; j < nx * ny; j += blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    const double uu = (double)u[(long)y * ld + x], vv = (double)v[(long)y * ld + x];
+    a += sqrt(uu * uu + vv * vv);
+    b += psi_m[j];
+  }
+  __shared__ double sa[256], sb[256];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sa[0];
+    out[1] = sb[0];
+  }
+}
+
+// slots of the S_RED2 scratch (doubles)
+constexpr int R2_TSUM = 0, R2_LAG = 2, R2_BETA = 8;
+
+// ------------------------------------------------------------------------
+// obukhov, scalars_module.f90:508-690 (patch_flag = 1), FP64 per point
+// ------------------------------------------------------------------------
+static __device__ double mo_psi_m_unstable(double x) {  // :621-622
+  return 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + 3.1415926535897932384626433 / 2.0;
+}
+
+template <typename T>
+__global__ void k_obukhov(const T* __restrict__ u1f, const T* __restrict__ v1f, const T* __restrict__ th,
+                          double* t3, double* q3, const double* __restrict__ wt, const double* __restrict__ wq,
+                          double* psi_m, double* psi_m0, double* psi_h, double* psi_h0, double* phi_m, double* phi_h,
+                          double* ustar, const double* __restrict__ tsum, int nx, int ny, int ld, double dz, double z0,
+                          double g_, double u_star, double T_scale, double q_scale, int first, int initsc,
+                          double th_ref) {
+  const int n = nx * ny;
+  const double theta_ = tsum[0] / (double)(float)(nx * ny);  // :556
+  const double z0s = z0 / 7.39;                               // bottombc.f90:55 (z_os = zo/7.39)
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    const long i = x + (long)ld * y;
+    if (first) {  // :521-538, jt == SCAL_init
+      psi_m[j] = 0.0; psi_m0[j] = 0.0; psi_h[j] = 0.0; psi_h0[j] = 0.0;
+      if (!initsc) {
+        t3[i] = wt[j] / (u_star * T_scale);
+        q3[i] = wq[j] / (u_star * q_scale);
+      }
+    }
+    const double u1 = (double)u1f[i], v1 = (double)v1f[i];
+    const double ua = sqrt(u1 * u1 + v1 * v1);
+    const double us = ua * VONK / (log(0.5 * dz / z0) - psi_m[j] + psi_m0[j]);  // :565-573
+    ustar[j] = us;
+    const double wtv = t3[i] + 0.61 * ((double)th[i] + th_ref) * (0.001 * q_scale * q3[i]);  // :546-547
+    const double L = -(us * us * us) / (VONK * (g_ / theta_) * wtv);              // :579
+    double zeta = 0.5 * dz / L, zeta0 = z0 / L, zeta0s = z0s / L;
+    if (zeta < -30.0) zeta = -30.0;
+    if (zeta > 2.0) zeta = 2.0;
+    if (zeta0 < -30.0) zeta0 = -30.0;
+    if (zeta0 > 2.0) zeta0 = 2.0;
+    if (zeta0s < -30.0) zeta0s = -30.0;
+    if (zeta0s > 2.0) zeta0s = 2.0;
+    const double wt1 = t3[i];
+    if (L < 0.0 && wt1 != 0.0) {  // :613-634
+      const double xx = pow(1.0 - 16.0 * zeta, 0.25);
+      psi_m[j] = mo_psi_m_unstable(xx);
+      psi_h[j] = 2.0 * log((1.0 + xx * xx) / 2.0);
+      phi_m[j] = 1.0 / xx;
+      phi_h[j] = 1.0 / (xx * xx);
+      const double x0 = pow(1.0 - 16.0 * zeta0, 0.25), x0s = pow(1.0 - 16.0 * zeta0s, 0.25);
+      psi_m0[j] = mo_psi_m_unstable(x0);
+      psi_h0[j] = 2.0 * log((1.0 + x0s * x0s) / 2.0);
+    } else if (L > 0.0 && wt1 != 0.0) {  // :636-646
+      phi_m[j] = 1.0 + 5.0 * zeta;
+      phi_h[j] = 1.0 + 5.0 * zeta;
+      psi_m[j] = -1.0 * 5.0 * zeta;
+      psi_h[j] = -1.0 * 5.0 * zeta;
+      psi_m0[j] = -1.0 * 5.0 * zeta0;
+      psi_h0[j] = -1.0 * 5.0 * zeta0s;
+    } else {  // :648-656
+      psi_m[j] = 0.0; psi_m0[j] = 0.0; psi_h[j] = 0.0; psi_h0[j] = 0.0;
+      phi_m[j] = 1.0; phi_h[j] = 1.0;
+    }
+  }
+}
+
+template <typename T>
+void op_obukhov(les_ctx* c, const T* u, const T* v, const T* theta, double* sgs_t3, double* sgs_q3, double* psi_m,
+                double* psi_m0, double* psi_h, double* psi_h0, double* phi_m, double* phi_h, double* ustar_avg,
+                double th_ref) {
+  OpTag _tag(c, "obukhov");
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  const size_t pl = (size_t)g.ld * g.ny;
+  T* planes = (T*)scratch(c, S_PLANE, 2 * pl * sizeof(T));
+  op_copy<T>(c, planes, u, pl);
+  op_copy<T>(c, planes + pl, v, pl);
+  op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);  // :560-563
+  double* red = (double*)scratch(c, S_RED2, (R2_BETA + g.nz) * sizeof(double));
+  {
+    ProfScope _ps(c, "plane_sum", (double)g.nx * g.ny * sizeof(T));
+    hipLaunchKernelGGL((k_level_sum<SumPlane<T>>), dim3(1), dim3(256), 0, c->stream, SumPlane<T>{theta, g.ld, th_ref}, g.nx,
+                       g.ny, red + R2_TSUM);
+  }
+  LCHK(c, hipGetLastError());
+  {
+    ProfScope _ps(c, "obukhov", 12.0 * g.nx * g.ny * 8);
+    hipLaunchKernelGGL((k_obukhov<T>), grid1((size_t)g.nx * g.ny), dim3(256), 0, c->stream, planes, planes + pl, theta,
+                       sgs_t3, sgs_q3, c->wt_s, c->wq_s, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, ustar_avg,
+                       red + R2_TSUM, g.nx, g.ny, g.ld, g.dz, g.z0, (P.g * P.z_i) / (P.u_star * P.u_star), P.u_star,
+                       P.T_scale, P.q_scale, c->jt == 1 ? 1 : 0, P.initsc, th_ref);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// calcbeta, scalars_module.f90:175-228
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_beta(const T* __restrict__ th, const T* __restrict__ q, const double* __restrict__ sums, T* beta,
+                       int nx, int ld, int ny, int nz, double g_hat, double qs, double tref, double qref) {
+  const long pl = (long)ld * ny, n = pl * nz;
+  const double nxy = (double)(nx * ny);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    double b = 0.0;
+    if (k >= 1 && k < nz - 1 && x < nx) {
+      const double bk = sums[k] / nxy, bm = sums[k - 1] / nxy;
+      const double tv = ((double)th[i] + tref) * (1.0 + 0.61 * 0.001 * ((double)q[i] + qref) * qs);
+      const double tvm = ((double)th[i - pl] + tref) * (1.0 + 0.61 * 0.001 * ((double)q[i - pl] + qref) * qs);
+      const double above = (tv - bk) / bk;
+      const double below = (tvm - bm) / bm;
+      b = g_hat * (above + below) / 2.0;
+    }
+    beta[i] = (T)b;
+  }
+}
+
+template <typename T>
+void op_calcbeta(les_ctx* c, const T* theta, const T* qmix, T* beta, double th_ref, double q_ref) {
+  OpTag _tag(c, "calcbeta");
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  double* red = (double*)scratch(c, S_RED2, (R2_BETA + g.nz) * sizeof(double));
+  {
+    ProfScope _ps(c, "level_sum", 2.0 * g.nx * g.ny * g.nz * sizeof(T));
+    hipLaunchKernelGGL((k_level_sum<SumThetaV<T>>), dim3(g.nz), dim3(256), 0, c->stream,
+                       SumThetaV<T>{theta, qmix, g.ld, g.ny, P.q_scale, th_ref, q_ref}, g.nx, g.ny, red + R2_BETA);
+  }
+  LCHK(c, hipGetLastError());
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  {
+    ProfScope _ps(c, "beta", 3.0 * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_beta<T>), grid1(n3), dim3(256), 0, c->stream, theta, qmix, red + R2_BETA, beta, g.nx, g.ld,
+                       g.ny, g.nz, P.g * (P.z_i / (P.u_star * P.u_star)), P.q_scale, th_ref, q_ref);
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// vertical scalar gradient: ddz_uv (levels 2..nz-1), the lid value
+// (scalars_module.f90:90-92) and the surface gradient of scalar_RHS_calc
+// (:278-283) or of lagrange_scalar_Sdep (:133-136)
+// ------------------------------------------------------------------------
+struct SurfaceFlux {  // lbc = 1 prescribed flux of one scalar
+  const double* flux;
+  int which;  // 0: wt_s/(T_scale u_star), 1: wq_s/q_scale/u_star
+  double T_scale, q_scale, u_star;
+  __device__ double operator()(int j) const {
+    return which == 0 ? flux[j] / (T_scale * u_star) : flux[j] / q_scale / u_star;
+  }
+};
+
+template <typename T>
+struct DzArgs {
+  const T* s[2];
+  T* dz[2];
+  SurfaceFlux sf[2];
+  T top[2];
+  int ns, surface_only, lag;
+  const double *phi_h, *ustar;
+  const double* lagsums;  // lag: {sum |u(1)|, sum psi_m}
+  double zlog, sflux;     // lag: ln(dz/2/zo_avg), mean surface flux
+};
+
+template <typename T>
+__global__ void k_scalar_dz(DzArgs<T> a, int nx, int ld, int ny, int nz, double dz, T cst) {
+  const long pl = (long)ld * ny, n3 = pl * nz, n = n3 * a.ns;
+  const double fn = (double)(float)(nx * ny);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(t / n3);
+    const long i = t - q * n3;
+    const int k = (int)(i / pl);
+    const int r = (int)(i - k * pl), y = r / ld, x = r - y * ld;
+    if (a.surface_only && k != 0) continue;
+    T* d = a.dz[q];
+    if (k == nz - 1) {
+      d[i] = a.top[q];
+      continue;
+    }
+    if (x >= nx) continue;
+    if (k == 0) {
+      const int j = x + nx * y;
+      double v;
+      if (a.lag) {
+        const double ustar_ = a.lagsums[0] / fn * VONK / (a.zlog - a.lagsums[1] / fn);
+        v = -a.phi_h[j] * a.sflux / (ustar_ * VONK * dz / 2.0);
+      } else {
+        v = -a.phi_h[j] * a.sf[q](j) / (a.ustar[j] * VONK * dz / 2.0);
+      }
+      d[i] = (T)v;
+    } else {
+      const T* s = a.s[q];
+      d[i] = cst * (s[i] - s[i - pl]);
+    }
+  }
+}
+
+static SurfaceFlux surface_flux(const les_ctx* c, int which) {
+  const ScalarParams& P = c->scal;
+  return SurfaceFlux{which == 0 ? c->wt_s : c->wq_s, which, P.T_scale, P.q_scale, P.u_star};
+}
+static double lid_gradient(const les_ctx* c, int which) {  // scalars_module.f90:90-92, _q
+  const ScalarParams& P = c->scal;
+  return which == 0 ? P.dTdz_top / P.T_scale * P.z_i : P.dqdz_top / P.q_scale * P.z_i;
+}
+
+template <typename T>
+void op_scalar_dz(les_ctx* c, const ScalarJob* jobs, int ns, const double* phi_h, const double* ustar_avg,
+                  int surface_only) {
+  const Grid& g = c->g;
+  DzArgs<T> a{};
+  a.ns = ns;
+  a.surface_only = surface_only;
+  for (int q = 0; q < ns; ++q) {
+    a.s[q] = (const T*)jobs[q].s;
+    a.dz[q] = (T*)jobs[q].dsdz;
+    a.sf[q] = surface_flux(c, jobs[q].which);
+    a.top[q] = (T)lid_gradient(c, jobs[q].which);
+  }
+  a.phi_h = phi_h;
+  a.ustar = ustar_avg;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  {
+    ProfScope _ps(c, "scalar_dz", (surface_only ? 0.02 : 2.0) * ns * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_scalar_dz<T>), grid1(ns * n3), dim3(256), 0, c->stream, a, g.nx, g.ld, g.ny, g.nz, g.dz,
+                       T(1.0 / g.dz));
+  }
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// scalar_RHS_calc advection: padded-grid products in LDS
+// ------------------------------------------------------------------------
+// One workgroup owns one padded row yb and a chunk of levels.  XB holds the
+// x-spectral rows of NF = 3 + 3 NS fields (u, v, w, then ds/dx, ds/dy, ds/dz
+// per scalar) on the padded y grid; two consecutive levels are kept as
+// physical padded rows in LDS.  The products of scalars_module.f90:322-338
+// are transformed back and stored as x-spectral rows (kx < nx/2) in XP.
+template <typename T, int N, int NS>
+__global__ void __launch_bounds__((N > 0 ? 3 * N / 4 : 1024))
+    k_scalar_big(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int nxh, int chunk, FftPlan p,
+                 const cx<T>* __restrict__ twg, T cst) {
+  constexpr int NF = 3 + 3 * NS, NP = (NF + 1) / 2;
+  constexpr int NB = N > 0 ? N : 4;
+  constexpr int BLK = N > 0 ? 3 * N / 4 : 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = N > 0 ? N : p.n;
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* fb = tw + n;
+  T* win = reinterpret_cast<T*>(fb + NP * n);
+  load_twiddles(tw, twg, n);
+  const int yb = blockIdx.x;
+  const int k0 = blockIdx.y * chunk;
+  const int k1 = min(k0 + chunk, nz - 1);
+  auto W = [&](int kk, int f) -> T* { return win + ((kk & 1) * NF + f) * n; };
+  auto load_level = [&](int kk) {
+    for (int e = threadIdx.x; e < NP * n; e += blockDim.x) {
+      const int t = e / n, j = e - t * n;
+      const int fa = 2 * t, fb2 = 2 * t + 1;
+      const cx<T>* a = XB + (((long)fa * nz + kk) * ny2 + yb) * nxh;
+      const cx<T>* b = XB + (((long)fb2 * nz + kk) * ny2 + yb) * nxh;
+      const bool hb = fb2 < NF;
+      cx<T> z = {T(0), T(0)};
+      if (j == 0) {
+        z = {a[0].x, hb ? b[0].x : T(0)};
+      } else if (j < nxh) {
+        const cx<T> A = a[j], B = hb ? b[j] : cx<T>{T(0), T(0)};
+        z = {A.x - B.y, A.y + B.x};
+      } else if (n - j < nxh) {
+        const cx<T> A = a[n - j], B = hb ? b[n - j] : cx<T>{T(0), T(0)};
+        z = {A.x + B.y, B.x - A.y};
+      }
+      fb[e] = z;
+    }
+    __syncthreads();
+    if constexpr (N > 0)
+      fft_lds_c<T, NB, 1, NB, NP, BLK, false>(fb, tw, +1);
+    else
+      fft_lds<T, false, 4>(fb, p, 1, n, NP, tw, +1);
+    for (int e = threadIdx.x; e < NP * n; e += blockDim.x) {
+      const int t = e / n, x = e - t * n;
+      W(kk, 2 * t)[x] = fb[e].x;
+      if (2 * t + 1 < NF) W(kk, 2 * t + 1)[x] = fb[e].y;
+    }
+    __syncthreads();
+  };
+  load_level(k0);
+  for (int k = k0; k < k1; ++k) {
+    load_level(k + 1);
+    const T *u = W(k, 0), *v = W(k, 1), *w = W(k, 2), *wp = W(k + 1, 2);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
+      T r[2] = {T(0), T(0)};
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const T *sx = W(k, 3 + 3 * s), *sy = W(k, 4 + 3 * s), *sz = W(k, 5 + 3 * s), *szp = W(k + 1, 5 + 3 * s);
+        if (k == 0)
+          r[s] = cst * (u[x] * sx[x] + v[x] * sy[x] + (T(0.5) * wp[x]) * szp[x]);
+        else
+          r[s] = cst * (u[x] * sx[x] + v[x] * sy[x] + (w[x] * sz[x] + wp[x] * szp[x]) / T(2));
+      }
+      fb[x] = {r[0], r[1]};
+    }
+    __syncthreads();
+    if constexpr (N > 0)
+      fft_lds_c<T, NB, 1, NB, 1, BLK, false>(fb, tw, -1);
+    else
+      fft_lds<T, false, 4>(fb, p, 1, n, 1, tw, -1);
+    const int nl = nz - 1;
+    for (int kk = threadIdx.x; kk < nxh; kk += blockDim.x) {
+      const cx<T> zk = fb[kk], zn = fb[kk ? n - kk : 0];
+      const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+      XP[(((long)0 * nl + k) * ny2 + yb) * nxh + kk] = A;
+      if (NS > 1) {
+        const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+        XP[(((long)1 * nl + k) * ny2 + yb) * nxh + kk] = B;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int NS>
+static void launch_scalar_big_n(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
+  const Grid& g = c->g;
+  constexpr int NF = 3 + 3 * NS, NP = (NF + 1) / 2;
+  const int n = g.nx2, chunk = 16;
+  const FftPlan& p = plan_for(c, n);
+  int threads = 3 * n / 4;
+  if (N == 0) {
+    threads = (threads + 63) / 64 * 64;
+    if (threads > 1024) threads = 1024;
+  }
+  const size_t sm = (size_t)(1 + NP) * n * sizeof(cx<T>) + (size_t)2 * NF * n * sizeof(T);
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_scalar_big<T, N, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sm));
+  const unsigned gy = (unsigned)((g.nz - 1 + chunk - 1) / chunk);
+  ProfScope ps(c, "scalar_big",
+               (double)(NF * (double)g.nz * g.ny2 + NS * (double)(g.nz - 1) * g.ny2) * g.nxh * sizeof(cx<T>));
+  hipLaunchKernelGGL((k_scalar_big<T, N, NS>), dim3(g.ny2, gy), dim3(threads), sm, c->stream, XB, XP, g.ny2, g.nz,
+                     g.nxh, chunk, p, tw_for<T>(c, n), T(1.0 / ((double)g.nx2 * g.ny2)));
+  LCHK(c, hipGetLastError());
+}
+
+template <typename T, int NS>
+static void launch_scalar_big(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
+  switch (c->g.nx2) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_scalar_big_n<T, NN, NS>(c, XB, XP);
+    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
+#undef LES_CASE
+    default:
+      return launch_scalar_big_n<T, 0, NS>(c, XB, XP);
+  }
+}
+
+// SGS fluxes of scalars_module.f90:366-432 (times the ddx/ddy normalisation):
+// field f = dir * NS + s, dir 0 -> ds/dx, 1 -> ds/dy; levels 1..nz-1
+template <typename T>
+struct LoadScalarTemp {
+  const T* ds[2][2];  // [dir][scalar]
+  const T *Nu, *Pr;
+  T Prc;  // constant Pr_ when Pr == nullptr
+  int ns, ppf, ny, ld;
+  T cst;
+  static constexpr double reads() { return 4.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / ppf, k = plane - f * ppf;
+    const int dir = f / ns, s = f - dir * ns;
+    const long pl = (long)ld * ny, i = ((long)k * ny + y) * ld + x;
+    const T p0 = Pr ? Pr[i] : Prc;
+    const T d = ds[dir][s][i];
+    T t;
+    if (k == 0) {
+      t = (T(1) / p0) * Nu[i] * d;
+    } else {
+      const T p1 = Pr ? Pr[i + pl] : Prc;
+      t = (T(1) / (T(0.5) * (p0 + p1))) * T(0.5) * (Nu[i] + Nu[i + pl]) * d;
+    }
+    return cst * t;
+  }
+};
+
+// c2r epilogue: RHS = -advection + d/dx + d/dy (spectral) + d/dz of the
+// vertical SGS flux (scalars_module.f90:436-452), then step_scalar (:457-504)
+template <typename T>
+struct StoreScalarRHS {
+  T* s[2];
+  T* R[2];
+  T* Rf[2];
+  const T* dsdz[2];
+  double* sv1[2];
+  SurfaceFlux sf[2];
+  T gz[2];  // lid increment (dTdz_top/T_scale z_i dz when the domain is taller than z_i)
+  const T *Nu, *Pr;
+  T Prc;
+  int ppf, ny, ld, nx, nz;
+  T rdz, dt;
+  int do_step, euler;
+  static constexpr double writes() { return 7.0; }
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int q = plane / ppf, k = plane - q * ppf;
+    const long pl = (long)ld * ny, i = ((long)k * ny + y) * ld + x;
+    T* Rq = R[q];
+    if (k == nz - 2) Rq[i + pl] = T(0);  // level nz: not defined by the reference, 0 here
+    if (x >= nx) {
+      Rq[i] = T(0);
+      if (do_step && k == nz - 2) s[q][i + pl] = s[q][i] + gz[q];
+      return;
+    }
+    const T p0 = Pr ? Pr[i] : Prc, p1 = Pr ? Pr[i + pl] : Prc;
+    T t0;
+    if (k == 0) {
+      const double f = sf[q](x + nx * y);
+      sv1[q][x + (long)ld * y] = f;
+      t0 = (T)(-1.0 * f);
+    } else {
+      t0 = (T(1) / p0) * Nu[i] * dsdz[q][i];
+    }
+    const T t1 = (T(1) / p1) * Nu[i + pl] * dsdz[q][i + pl];
+    const T rhs = v + rdz * (t1 - t0);
+    Rq[i] = rhs;
+    if (!do_step) return;
+    T fo;
+    if (euler) {
+      fo = rhs;
+      Rf[q][i] = rhs;
+    } else {
+      fo = Rf[q][i];
+    }
+    const T sn = s[q][i] + dt * (T(1.5) * rhs - T(0.5) * fo);
+    s[q][i] = sn;
+    if (k == nz - 2) s[q][i + pl] = sn + gz[q];
+  }
+};
+
+template <typename T>
+void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const T* v, const T* w, const T* Nu_t,
+                   const T* Pr_t, int do_step, int euler) {
+  OpTag _tag(c, "scalar_rhs");
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1, nf = 3 + 3 * ns;
+  // 1) dealias1 of u, v, w and the gradients: x-spectral rows, padded in y
+  const long rows = (long)nf * nz * g.ny;
+  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
+  LoadFieldsN<T, 9> L{};
+  L.p[0] = u; L.p[1] = v; L.p[2] = w;
+  for (int q = 0; q < ns; ++q) {
+    L.p[3 + 3 * q] = (const T*)jobs[q].dsdx;
+    L.p[4 + 3 * q] = (const T*)jobs[q].dsdy;
+    L.p[5 + 3 * q] = (const T*)jobs[q].dsdz;
+  }
+  L.ppf = nz; L.ny = g.ny; L.ld = g.ld; L.scale = T(1.0 / ((double)g.nx * g.ny));
+  launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
+  const long rowsB = (long)nf * nz * g.ny2;
+  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, (long)nf * nz, g.nxh);
+  // 2) products on the padded grid (dealias2 scaling included)
+  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)ns * nl * g.ny2 * g.nxh * sizeof(cx<T>));
+  if (ns == 2)
+    launch_scalar_big<T, 2>(c, XB, XP);
+  else
+    launch_scalar_big<T, 1>(c, XB, XP);
+  // 3) truncate in y (unpadd)
+  cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
+  ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, (long)ns * nl, g.nxh);
+  // 4) horizontal SGS fluxes -> spectral divergence
+  const long rowsT = 2L * ns * nl * g.ny;
+  cx<T>* XT = (cx<T>*)scratch(c, S_XS0, (size_t)rowsT * g.nxh * sizeof(cx<T>));
+  LoadScalarTemp<T> LT{};
+  for (int q = 0; q < ns; ++q) {
+    LT.ds[0][q] = (const T*)jobs[q].dsdx;
+    LT.ds[1][q] = (const T*)jobs[q].dsdy;
+  }
+  LT.Nu = Nu_t; LT.Pr = Pr_t; LT.Prc = (T)c->scal.Pr;
+  LT.ns = ns; LT.ppf = nl; LT.ny = g.ny; LT.ld = g.ld; LT.cst = T(1.0 / ((double)g.nx * g.ny));
+  launch_row_fwd<T>(c, LT, XT, g.nxh, rowsT, g.nx, g.nxh, g.ny);
+  cx<T>* XD = (cx<T>*)scratch(c, S_XS1, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
+  ColArgs d = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  d.in2_plane = d.in_plane;
+  col_out(d, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XT, XT + (size_t)ns * nl * g.ny * g.nxh, XD, nullptr, nullptr, d, (long)ns * nl, g.nxh);
+  // 5) c2r of (divergence - advection) with the d/dz term and the AB2 step
+  StoreScalarRHS<T> S{};
+  for (int q = 0; q < ns; ++q) {
+    S.s[q] = (T*)jobs[q].s;
+    S.R[q] = (T*)jobs[q].RHS;
+    S.Rf[q] = (T*)jobs[q].RHS_f;
+    S.dsdz[q] = (const T*)jobs[q].dsdz;
+    S.sv1[q] = jobs[q].sgs_vert1;
+    S.sf[q] = surface_flux(c, jobs[q].which);
+    S.gz[q] = (T)(g.Lz > 1.0 ? lid_gradient(c, jobs[q].which) * g.dz : 0.0);
+  }
+  S.Nu = Nu_t; S.Pr = Pr_t; S.Prc = (T)c->scal.Pr;
+  S.ppf = nl; S.ny = g.ny; S.ld = g.ld; S.nx = g.nx; S.nz = nz;
+  S.rdz = T(1.0 / g.dz); S.dt = T(g.dt);
+  S.do_step = do_step; S.euler = euler;
+  launch_row_inv<T>(c, XD, g.nxh, g.nxh, S, (long)ns * nl * g.ny, g.nx, g.ld, g.ny, X3, T(-1));
+}
+
+// step_scalar, scalars_module.f90:457-504 (routine-level entry point)
+template <typename T>
+__global__ void k_step_scalar(T* s, const T* R, const T* Rf, int nx, int ld, int ny, int nz, T dt, T gz) {
+  const long pl = (long)ld * ny, n = pl * (nz - 1);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    T sn = s[i];
+    if (x < nx) {
+      sn = s[i] + dt * (T(1.5) * R[i] - T(0.5) * Rf[i]);
+      s[i] = sn;
+    }
+    if (k == nz - 2) s[i + pl] = sn + gz;
+  }
+}
+
+template <typename T>
+void op_step_scalar(les_ctx* c, int which, T* s, const T* RHS, const T* RHS_f) {
+  OpTag _tag(c, "step_scalar");
+  const Grid& g = c->g;
+  const T gz = (T)(g.Lz > 1.0 ? lid_gradient(c, which) * g.dz : 0.0);
+  hipLaunchKernelGGL((k_step_scalar<T>), grid1((size_t)g.ld * g.ny * (g.nz - 1)), dim3(256), 0, c->stream, s, RHS,
+                     RHS_f, g.nx, g.ld, g.ny, g.nz, T(g.dt), gz);
+  LCHK(c, hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// lagrange_scalar_Sdep.f90: 10 new fields, each filtered at 2 Delta and
+// 4 Delta; u, v, w bars and S_ij come from the momentum model's filtered set
+// ------------------------------------------------------------------------
+template <typename T>
+struct LoadLagScalar {  // theta_b | u_b th_b, v_b th_b, w_b th_b | |S| dT (x,y,z) | dT bars (x,y,z)
+  const T *u, *v, *w, *th, *S11, *S12, *S13, *S22, *S23, *S33, *dx, *dy, *dz;
+  int ny, nz, ld;
+  static constexpr double reads() { return 13.0 / 10.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nz, k = plane - f * nz;
+    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
+    if (f < 4) {  // :139-150
+      T tb;
+      if (k == 0) tb = th[i];
+      else tb = T(0.5) * (th[i] + th[i - pl]);
+      if (f == 0) return tb;
+      T ub;
+      if (f == 1) ub = k == 0 ? u[i] : T(0.5) * (u[i] + u[i - pl]);
+      else if (f == 2) ub = k == 0 ? v[i] : T(0.5) * (v[i] + v[i - pl]);
+      else ub = k == 0 ? T(0.25) * w[i + pl] : w[i];
+      return ub * tb;
+    }
+    if (f < 7) {  // :240-248
+      const T S = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);
+      if (k == 0) {
+        if (f == 4) return S * dx[i];
+        if (f == 5) return S * dy[i];
+        return S * (dz[i] + dz[i + pl]) / T(2);
+      }
+      if (f == 4) return S * (dx[i] + dx[i - pl]) / T(2);
+      if (f == 5) return S * (dy[i] + dy[i - pl]) / T(2);
+      return S * dz[i];
+    }
+    if (k == 0) {  // :262-270
+      if (f == 7) return dx[i];
+      if (f == 8) return dy[i];
+      return (dz[i] + dz[i + pl]) / T(2);
+    }
+    if (f == 7) return (dx[i] + dx[i - pl]) / T(2);
+    if (f == 8) return (dy[i] + dy[i - pl]) / T(2);
+    return dz[i];
+  }
+};
+
+template <typename T>
+__global__ void k_lag_scalar_epilogue(const T* __restrict__ F, const T* __restrict__ Fs, const T* __restrict__ TnLM,
+                                      const T* __restrict__ TnQN, const T* __restrict__ Cs, T* GLM, T* GMM, T* GQN,
+                                      T* GNN, T* Beta, T* Cs2_Pr, T* Pr_t, int nx, int ld, int ny, int nz, double cst,
+                                      double opftdelta, double lagran_dt, double expo, int init_lm, int init_qn) {
+  const long pl = (long)ld * ny, n3 = pl * nz;
+  const double e24 = (double)1e-24f;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / pl);
+    const int x = (int)((i - k * pl) % ld);
+    const bool pad = x >= nx;
+    auto f2m = [&](int f) { return (double)F[f * n3 + i]; };
+    auto f4m = [&](int f) { return (double)F[(21 + f) * n3 + i]; };
+    auto f2s = [&](int f) { return (double)Fs[f * n3 + i]; };
+    auto f4s = [&](int f) { return (double)Fs[(10 + f) * n3 + i]; };
+    const double ub = f2m(0), vb = f2m(1), wb = f2m(2), uh = f4m(0), vh = f4m(1), wh = f4m(2);
+    const double tb = f2s(0), th = f4s(0);
+    const double L1 = f2s(1) - ub * tb, L2 = f2s(2) - vb * tb, L3 = f2s(3) - wb * tb;  // :164-169
+    const double Q1 = f4s(1) - uh * th, Q2 = f4s(2) - vh * th, Q3 = f4s(3) - wh * th;  // :176-181
+    double sb[6], sh[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { sb[m] = f2m(9 + m); sh[m] = f4m(9 + m); }
+    const double Sb = sqrt(2.0 * (sb[0] * sb[0] + sb[3] * sb[3] + sb[5] * sb[5] +
+                                  2.0 * (sb[1] * sb[1] + sb[2] * sb[2] + sb[4] * sb[4])));
+    const double Sh = sqrt(2.0 * (sh[0] * sh[0] + sh[3] * sh[3] + sh[5] * sh[5] +
+                                  2.0 * (sh[1] * sh[1] + sh[2] * sh[2] + sh[4] * sh[4])));
+    double M[3], N[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {  // :268-274
+      M[q] = cst * (f2s(4 + q) - 4.0 * Sb * f2s(7 + q));
+      N[q] = cst * (f4s(4 + q) - 16.0 * Sh * f4s(7 + q));
+    }
+    const double LM = L1 * M[0] + L2 * M[1] + L3 * M[2];
+    const double MM = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
+    const double QN = Q1 * N[0] + Q2 * N[1] + Q3 * N[2];
+    const double NN = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+    double glm = GLM[i], gmm = GMM[i], gqn = GQN[i], gnn = GNN[i];
+    if (init_lm) { gmm = pad ? 1.0 : MM; glm = pad ? 1.0 : 0.03 * MM; }  // :289-299
+    double Tn = (double)TnLM[i];                                          // :318-331
+    Tn = opftdelta * pow(Tn, -0.125);
+    Tn = fmax(e24, fq(Tn));
+    double dumfac = lagran_dt / Tn;
+    double epsi = dumfac / (1.0 + dumfac);
+    glm = (epsi * LM + (1.0 - epsi) * glm);
+    gmm = (epsi * MM + (1.0 - epsi) * gmm);
+    glm = fmax(e24, fq(glm));
+    const double c2 = fmax(e24, fq(pad ? e24 : glm / gmm));
+    if (init_qn) { gnn = pad ? 1.0 : NN; gqn = pad ? 1.0 : 0.03 * NN; }  // :338-348
+    Tn = (double)TnQN[i];                                                 // :350-363
+    Tn = opftdelta * pow(Tn, -0.125);
+    Tn = fmax(e24, fq(Tn));
+    dumfac = lagran_dt / Tn;
+    epsi = dumfac / (1.0 + dumfac);
+    gqn = (epsi * QN + (1.0 - epsi) * gqn);
+    gnn = (epsi * NN + (1.0 - epsi) * gnn);
+    gqn = fmax(e24, fq(gqn));
+    const double c4 = fmax(e24, fq(pad ? e24 : gqn / gnn));
+    double beta = expo == 1.0 ? c4 / c2 : pow(c4 / c2, expo);  // :370; the exponent is ln2/(ln4-ln2) = 1
+    if (k == nz - 1) beta = 1.0;
+    const double bclip = fmax(fq(beta), fq(0.125));
+    const double cpr = fmax(e24, fq(pad ? e24 : c2 / bclip));  // :384-392
+    GLM[i] = (T)glm; GMM[i] = (T)gmm; GQN[i] = (T)gqn; GNN[i] = (T)gnn;
+    Beta[i] = (T)beta;
+    Cs2_Pr[i] = (T)cpr;
+    Pr_t[i] = (T)((double)Cs[i] / cpr);  // :395
+  }
+}
+
+template <typename T>
+void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, T* theta,
+                        const SgsPtrs& m, const ScalarSgsPtrs& ss, bool reuse_mom) {
+  OpTag _tag(c, "lagrange_scalar");
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  const int nz = g.nz;
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  {  // interpolag_scalar_Sdep (:88)
+    OpTag _t2(c, "interpolag_scalar");
+    T* G[4] = {(T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN, (T*)ss.G_NN};
+    op_interpolag4<T>(c, G, (T*)ss.u_lag_s, (T*)ss.v_lag_s, (T*)ss.w_lag_s);
+  }
+  // theta_temp = theta (:84-86), then filt_da(theta) in place and ddz_uv (:95-99)
+  T* tht = (T*)scratch(c, S_SC0, n3 * sizeof(T));
+  T* dTx = (T*)scratch(c, S_SC1, n3 * sizeof(T));
+  T* dTy = (T*)scratch(c, S_SC2, n3 * sizeof(T));
+  T* dTz = (T*)scratch(c, S_SC3, n3 * sizeof(T));
+  op_copy<T>(c, tht, theta, n3);
+  op_filt_da<T>(c, theta, dTx, dTy, nz);
+  double* red = (double*)scratch(c, S_RED2, (R2_BETA + nz) * sizeof(double));
+  hipLaunchKernelGGL((k_lag_scalar_sums<T>), dim3(1), dim3(256), 0, c->stream, u, v, ss.psi_m, g.nx, g.ny, g.ld,
+                     red + R2_LAG);
+  LCHK(c, hipGetLastError());
+  {
+    DzArgs<T> a{};
+    a.ns = 1;
+    a.s[0] = theta;
+    a.dz[0] = dTz;
+    a.top[0] = (T)lid_gradient(c, 0);
+    a.lag = 1;
+    a.phi_h = ss.phi_h;
+    a.lagsums = red + R2_LAG;
+    a.zlog = log(0.5 * g.dz / c->zo_avg);
+    a.sflux = c->wt_mean / P.T_scale / P.u_star;  // :114
+    ProfScope _ps(c, "scalar_dz", 2.0 * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_scalar_dz<T>), grid1(n3), dim3(256), 0, c->stream, a, g.nx, g.ld, g.ny, nz, g.dz,
+                       T(1.0 / g.dz));
+  }
+  LCHK(c, hipGetLastError());
+  const T* F = reuse_mom ? (const T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T)) : lag_filter_mom<T>(c, S, u, v, w);
+  // the ten scalar fields through both test filters
+  const int k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4;
+  const long rows = 10L * nz * g.ny;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
+  LoadLagScalar<T> L{u, v, w, tht, S[0], S[1], S[2], S[3], S[4], S[5], dTx, dTy, dTz, g.ny, nz, g.ld};
+  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
+  cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
+  cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);
+  a.lhG = g.lh;
+  col_out(a, 0, K_G, k2, k2, (long)g.ny * k2);
+  col_out(a, 1, K_G, k4, k4, (long)g.ny * k4);
+  const Tables<T>& t = tables<T>(c);
+  launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, 10L * nz, kf, t.G2, t.G4);
+  T* Fs = (T*)scratch(c, S_LAGF2, 20 * n3 * sizeof(T));
+  StoreFields<T> S2{{Fs, nullptr, nullptr, nullptr}, 10 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
+  StoreFields<T> S4{{Fs + 10 * n3, nullptr, nullptr, nullptr}, 10 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
+  const int init_lm = g.inilag && !c->glm_init && (c->jt == g.cs_count || c->jt == 1);
+  const int init_qn = g.inilag && !c->gqn_init && (c->jt == g.cs_count || c->jt == 1);
+  const double delta = g.delta;
+  {
+    ProfScope _ps(c, "lag_scalar_epilogue", (6.0 + 20 + 2 + 1 + 4 + 4 + 3) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_lag_scalar_epilogue<T>), grid1(n3), dim3(256), 0, c->stream, F, Fs, (const T*)m.TnLMMM,
+                       (const T*)m.TnQNNN, (const T*)m.Cs_opt2, (T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN, (T*)ss.G_NN,
+                       (T*)m.Beta, (T*)ss.Cs2_Pr, (T*)ss.Pr_t, g.nx, g.ld, g.ny, nz, delta * delta, 1.5 * delta,
+                       g.dt * (double)g.cs_count, log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
+  }
+  LCHK(c, hipGetLastError());
+  if (init_lm) c->glm_init = 1;
+  if (init_qn) c->gqn_init = 1;
+}
+
+#define LES_SCAL_INST(T)                                                                                         \
+  template void op_obukhov<T>(les_ctx*, const T*, const T*, const T*, double*, double*, double*, double*,        \
+                              double*, double*, double*, double*, double*, double);                              \
+  template void op_calcbeta<T>(les_ctx*, const T*, const T*, T*, double, double);                                \
+  template void op_scalar_dz<T>(les_ctx*, const ScalarJob*, int, const double*, const double*, int);             \
+  template void op_scalar_rhs<T>(les_ctx*, const ScalarJob*, int, const T*, const T*, const T*, const T*,        \
+                                 const T*, int, int);                                                            \
+  template void op_step_scalar<T>(les_ctx*, int, T*, const T*, const T*);                                        \
+  template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
+                                      const SgsPtrs&, const ScalarSgsPtrs&, bool);
+
+LES_SCAL_INST(double)
+LES_SCAL_INST(float)
+
+}  // namespace les

@@ -1,0 +1,521 @@
+// les_xform.h -- transform kernels shared by the hot-path translation units:
+// batched row (x) r2c / c2r transforms with fused loader / storer functors,
+// the LDS column (y) kernel with its fused spectral operations, and small
+// elementwise helpers.  Everything here is a template or has internal
+// linkage, so each .hip file that includes it gets its own instantiations.
+//
+//   k_row_fwd : real rows (x) -> first `kxo` Fourier coefficients, two real
+//               rows packed into one complex FFT (x-spectral rows)
+//   k_col     : LDS tile of TX adjacent kx columns x all ky of one plane;
+//               forward y-FFT, a fused spectral operation (i kx, i ky, test
+//               filter G, 3/2 zero-pad / truncate, divergence), inverse y-FFT
+//   k_row_inv : x-spectral rows -> real rows (c2r), with a fused epilogue
+#pragma once
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "les_internal.h"
+
+namespace les {
+
+constexpr double BOGUS = -1234567890.0;  // param.f90:50
+constexpr double VONK = 0.41;            // param.f90:98
+constexpr double TADV1 = 1.5, TADV2 = -0.5;
+
+__device__ __forceinline__ double fq(double x) { return (double)(float)x; }  // Fortran real(x)
+
+// |S| = sqrt(2 S_ij S_ij) in the reference's evaluation order
+template <typename T>
+__device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
+  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
+#define LCHK(c, call) check((c), (call), #call)
+
+
+template <typename T>
+static const cx<T>* tw_for(const les_ctx* c, int n) {
+  const Tables<T>& t = tables<T>(c);
+  if (n == c->g.nx) return t.tw_nx;
+  if (n == c->g.ny) return t.tw_ny;
+  if (n == c->g.nx2) return t.tw_nx2;
+  return t.tw_ny2;
+}
+static const FftPlan& plan_for(const les_ctx* c, int n) {
+  if (n == c->g.nx) return c->p_nx;
+  if (n == c->g.ny) return c->p_ny;
+  if (n == c->g.nx2) return c->p_nx2;
+  return c->p_ny2;
+}
+
+// ------------------------------------------------------------------------
+// row transforms
+// ------------------------------------------------------------------------
+// Rows are grouped by plane (field x level): blockIdx.y = plane,
+// blockIdx.x = group of P row pairs inside it, so loaders and storers get a
+// block-uniform plane index (no per-element integer division).  N > 0 is the
+// compile-time transform length; N == 0 runs the same code with run-time n.
+template <int N>
+struct RowGeom {
+  static constexpr int TPF = N > 0 ? N / 4 : 64;
+  static constexpr int P = (256 / TPF) > 0 ? 256 / TPF : 1;
+  static constexpr int BLOCK = P * TPF;
+};
+
+template <typename T, int N, class Loader>
+__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+    k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
+              int kxo) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n = N > 0 ? N : p.n;
+  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  cx<T>* buf = tw + n;
+  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  const int plane = blockIdx.y;
+  const int npp = ny / 2;
+  const int pair0 = blockIdx.x * P;
+  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+    const int t = e / n, x = e - t * n, pp = pair0 + t;
+    cx<T> z = {T(0), T(0)};
+    if (pp < npp) {
+      z.x = L(plane, 2 * pp, x);
+      z.y = L(plane, 2 * pp + 1, x);
+    }
+    buf[e] = z;
+  }
+  __syncthreads();
+  if constexpr (N > 0)
+    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, -1);
+  else
+    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
+  for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
+    const int t = e / kxo, k = e - t * kxo, pp = pair0 + t;
+    if (pp >= npp) continue;
+    const cx<T> zk = buf[t * n + k], zn = buf[t * n + (k ? n - k : 0)];
+    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+    const long row = (long)plane * ny + 2 * pp;
+    out[row * orow + k] = A;
+    out[(row + 1) * orow + k] = B;
+  }
+}
+
+template <typename T, int N, class Storer>
+__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+    k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
+              const cx<T>* __restrict__ twg, int ldo, const cx<T>* __restrict__ in2, T c2) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int n = N > 0 ? N : p.n;
+  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  cx<T>* buf = tw + n;
+  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  const int plane = blockIdx.y;
+  const int npp = ny / 2;
+  const int pair0 = blockIdx.x * P;
+  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+    const int t = e / n, j = e - t * n, pp = pair0 + t;
+    cx<T> z = {T(0), T(0)};
+    if (pp < npp) {
+      const long r0 = ((long)plane * ny + 2 * pp) * irow;
+      const cx<T>* a = in + r0;
+      const cx<T>* b = a + irow;
+      // optional second spectral input added with weight c2 (same layout)
+      auto get = [&](const cx<T>* q, int jj) {
+        cx<T> v = q[jj];
+        if (in2) {
+          const cx<T> w = in2[(q - in) + jj];
+          v = {v.x + c2 * w.x, v.y + c2 * w.y};
+        }
+        return v;
+      };
+      if (j == 0) {
+        const cx<T> A = get(a, 0), B = get(b, 0);
+        z = {A.x, B.x};  // Im of the DC entries is ignored (halfcomplex c2r)
+      } else if (j < kxi) {
+        const cx<T> A = get(a, j), B = get(b, j);
+        z = {A.x - B.y, A.y + B.x};
+      } else if (n - j < kxi) {
+        const cx<T> A = get(a, n - j), B = get(b, n - j);
+        z = {A.x + B.y, B.x - A.y};
+      }
+    }
+    buf[e] = z;
+  }
+  __syncthreads();
+  if constexpr (N > 0)
+    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, +1);
+  else
+    fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
+  for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
+    const int t = e / ldo, x = e - t * ldo, pp = pair0 + t;
+    if (pp >= npp) continue;
+    T v0 = T(0), v1 = T(0);
+    if (x < n) {
+      v0 = buf[t * n + x].x;
+      v1 = buf[t * n + x].y;
+    }
+    S(plane, 2 * pp, x, v0);
+    S(plane, 2 * pp + 1, x, v1);
+  }
+}
+
+// compile-time lengths with specialised kernels; anything else runs the
+// run-time-length kernels
+#define LES_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384)
+
+template <typename T, int N, class L>
+static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo) {
+  const FftPlan& p = plan_for(c, n);
+  const int tpf = n / 4;
+  int P = 256 / tpf;
+  if (P < 1) P = 1;
+  const int threads = P * tpf;
+  const long blocks = (ny / 2 + P - 1) / P;
+  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
+  hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
+                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo);
+  LCHK(c, hipGetLastError());
+}
+
+template <typename T, class L>
+static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo, int ny) {
+  const long nplanes = nrows / ny;
+  switch (n) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_row_fwd_n<T, NN>(c, ld, out, orow, nplanes, ny, n, kxo);
+    LES_FFT_SIZES(LES_CASE)
+#undef LES_CASE
+    default:
+      return launch_row_fwd_n<T, 0>(c, ld, out, orow, nplanes, ny, n, kxo);
+  }
+}
+
+template <typename T, int N, class S>
+static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
+                             int n, int ldo, const cx<T>* in2, T c2) {
+  const FftPlan& p = plan_for(c, n);
+  const int tpf = n / 4;
+  int P = 256 / tpf;
+  if (P < 1) P = 1;
+  const int threads = P * tpf;
+  const long blocks = (ny / 2 + P - 1) / P;
+  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  ProfScope ps(c, "row_inv",
+               (double)nplanes * ny * (kxi * sizeof(cx<T>) * (in2 ? 2 : 1) + ldo * sizeof(T) * st.writes()));
+  hipLaunchKernelGGL((k_row_inv<T, N, S>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
+                     in, irow, kxi, st, ny, p, tw_for<T>(c, n), ldo, in2, c2);
+  LCHK(c, hipGetLastError());
+}
+
+// in2 (optional): a second x-spectral input of the same layout, added with
+// weight c2 before the inverse transform
+template <typename T, class S>
+static void launch_row_inv(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nrows, int n,
+                           int ldo, int ny, const cx<T>* in2 = nullptr, T c2 = T(1)) {
+  const long nplanes = nrows / ny;
+  switch (n) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_row_inv_n<T, NN>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2);
+    LES_FFT_SIZES(LES_CASE)
+#undef LES_CASE
+    default:
+      return launch_row_inv_n<T, 0>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2);
+  }
+}
+
+// loaders / storers ------------------------------------------------------
+// row index = (field * nlev + level) * ny + y; fields are (ld, ny, nlev).
+// plane = field * planes_per_field + level; fields are (ld, ny, levels)
+template <typename T>
+struct LoadFields {
+  const T* p[4];
+  int ppf, ny, ld;
+  T scale;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    return scale * p[f][((long)r * ny + y) * ld + x];
+  }
+};
+template <typename T, int NF>
+struct LoadFieldsN {  // up to NF fields, plane = field * ppf + level
+  const T* p[NF];
+  int ppf, ny, ld;
+  T scale;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    return scale * p[f][((long)r * ny + y) * ld + x];
+  }
+};
+template <typename T>
+struct StoreFields {
+  T* p[12];
+  int ppf, ny;
+  int ld, nx;  // pads (x >= nx) are written 0
+  static constexpr double writes() { return 1.0; }
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    p[f][((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
+  }
+};
+
+// ------------------------------------------------------------------------
+// column (y) kernel with fused spectral operation
+// ------------------------------------------------------------------------
+enum Kind : int { K_ID = 0, K_IKX = 1, K_IKY = 2, K_G = 3 };
+
+struct ColArgs {
+  int ny_in, ny_out, do_fwd, do_inv;
+  int nki;               // input kx columns present
+  int nout;              // 1..3 outputs
+  int kind[3];
+  int nko[3];            // output kx columns written
+  long in_row, in_plane, in2_plane;
+  long out_row[3], out_plane[3];
+  int lhG;               // row length of the G tables (lh)
+};
+
+__device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
+  if (nyi == nyo) return (yo == nyo / 2) ? -1 : yo;
+  if (nyo > nyi) {  // zero-pad (padd.f90)
+    if (yo < nyi / 2) return yo;
+    if (yo >= nyo - nyi / 2 + 1) return yo - nyo + nyi;
+    return -1;
+  }
+  if (yo < nyo / 2) return yo;  // truncate (unpadd.f90)
+  if (yo == nyo / 2) return -1;
+  return yo + nyi - nyo;
+}
+
+template <int TX, int NYI, int NYO>
+struct ColGeom {
+  static constexpr int NMAX = NYI > NYO ? NYI : NYO;
+  static constexpr int B0 = (TX * NMAX / 8 + 63) / 64 * 64;
+  static constexpr int BLOCK = B0 < 64 ? 64 : B0;
+};
+
+// NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
+template <typename T, int TX, int NYI, int NYO>
+__global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
+    k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
+          FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
+          const T* __restrict__ kxt, const T* __restrict__ kyt, const T* __restrict__ G0, const T* __restrict__ G1,
+          const T* __restrict__ G2) {
+  constexpr bool CT = NYI > 0;
+  constexpr int BLK = CT ? ColGeom<TX, NYI, NYO>::BLOCK : 0;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nyi = CT ? NYI : a.ny_in;
+  const int nyo = CT ? NYO : a.ny_out;
+  const int bd = CT ? BLK : (int)blockDim.x;
+  cx<T>* twi = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* two = twi + nyi;
+  cx<T>* A = two + nyo;
+  cx<T>* B = A + nyi * TX;
+  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = twin[i];
+  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = twout[i];
+  auto fwd = [&](cx<T>* buf) {
+    if constexpr (CT)
+      fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, twi, -1);
+    else
+      fft_lds<T, true, 4>(buf, pin, TX, 1, TX, twi, -1);
+  };
+  auto inv = [&](cx<T>* buf) {
+    if constexpr (CT)
+      fft_lds_c<T, (CT ? NYO : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, two, +1);
+    else
+      fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
+  };
+  const int kx0 = blockIdx.x * TX;
+  const long pl = blockIdx.y;
+  {
+    const cx<T>* src = in + pl * a.in_plane;
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      A[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+    }
+  }
+  __syncthreads();
+  if (a.do_fwd) fwd(A);
+  if (in2) {
+    // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
+    const cx<T>* src = in2 + pl * a.in2_plane;
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      B[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    fwd(B);
+    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      cx<T> r = {T(0), T(0)};
+      if (y != nyi / 2 && kk < a.nki) {
+        const T kx = kxt[kk], ky = kyt[y];
+        const cx<T> X = A[e], Y = B[e];
+        r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
+      }
+      A[e] = r;
+    }
+    __syncthreads();
+    if (a.do_inv) inv(A);
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = A[e];
+    }
+    return;
+  }
+  for (int j = 0; j < a.nout; ++j) {
+    if (kx0 >= a.nko[j]) continue;  // uniform over the block
+    const int kind = a.kind[j];
+    const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
+      const int yo = e / TX, c = e - yo * TX, kk = kx0 + c;
+      const int yi = map_row(yo, nyi, nyo);
+      cx<T> v = {T(0), T(0)};
+      if (yi >= 0 && kk < a.nki) {
+        v = A[yi * TX + c];
+        if (kind == K_IKX) {
+          const T k = kxt[kk];
+          v = {-k * v.y, k * v.x};
+        } else if (kind == K_IKY) {
+          const T k = kyt[yo];
+          v = {-k * v.y, k * v.x};
+        } else if (kind == K_G) {
+          const T g = G[(long)yo * a.lhG + kk];
+          v = {g * v.x, g * v.y};
+        }
+      }
+      B[e] = v;
+    }
+    __syncthreads();
+    if (a.do_inv) inv(B);
+    cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
+    for (int e = threadIdx.x; e < nyo * TX; e += bd) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = B[e];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NYI, int NYO>
+static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                         const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
+  constexpr int TX = sizeof(T) == 8 ? 8 : 16;
+  const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
+  // the FFT stages need blockDim >= TX*n/8 (at most 4 butterflies per thread)
+  int threads = NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
+  if (threads < 64) threads = 64;
+  if (threads > 1024) {
+    c->err = "k_col: column tile too large";
+    return;
+  }
+  const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sm));
+  const Tables<T>& t = tables<T>(c);
+  const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
+  double bytes = 0;
+  {
+    const double es = 2.0 * sizeof(T);
+    bytes += (double)nplanes * a.ny_in * (ncols < a.nki ? ncols : a.nki) * es * (in2 ? 2 : 1);
+    for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
+  }
+  ProfScope ps(c, "col", bytes);
+  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
+                     o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
+                     tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
+  LCHK(c, hipGetLastError());
+}
+
+#define LES_COL_PAIRS(X)                                                                                       \
+  X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
+  X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
+
+template <typename T>
+static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                       const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
+                       const T* G2 = nullptr) {
+#define LES_CASE(I, O)                                                                        \
+  if (a.ny_in == I && a.ny_out == O)                                                          \
+    return launch_col_n<T, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
+  LES_COL_PAIRS(LES_CASE)
+#undef LES_CASE
+  launch_col_n<T, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
+}
+
+static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {
+  ColArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.ny_in = ny_in;
+  a.ny_out = ny_out;
+  a.do_fwd = 1;
+  a.do_inv = 1;
+  a.nki = nki;
+  a.in_row = in_row;
+  a.in_plane = in_plane;
+  return a;
+}
+static void col_out(ColArgs& a, int j, int kind, int nko, long out_row, long out_plane) {
+  a.kind[j] = kind;
+  a.nko[j] = nko;
+  a.out_row[j] = out_row;
+  a.out_plane[j] = out_plane;
+  if (a.nout < j + 1) a.nout = j + 1;
+}
+
+// ------------------------------------------------------------------------
+// simple elementwise kernels
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* a, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+template <typename T>
+void op_fill(les_ctx* c, T* a, size_t n, T val) {
+  if (!n) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  {
+    ProfScope _ps(c, "fill", (double)n * sizeof(T));
+    hipLaunchKernelGGL((k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, a, n, val);
+  }
+  LCHK(c, hipGetLastError());
+}
+template <typename T>
+__global__ void k_copy(T* d, const T* s, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+template <typename T>
+void op_copy(les_ctx* c, T* dst, const T* src, size_t n) {
+  if (!n) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  {
+    ProfScope _ps(c, "copy", 2.0 * n * sizeof(T));
+    hipLaunchKernelGGL((k_copy<T>), dim3((unsigned)blocks), dim3(256), 0, c->stream, dst, src, n);
+  }
+  LCHK(c, hipGetLastError());
+}
+// complex-(BOGUS,0) pattern written by press_stag_array.f90:288-290
+template <typename T>
+__global__ void k_fill_bogus_cplx(T* a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = (i & 1) ? T(0) : T(BOGUS);
+}
+
+static dim3 grid1(size_t n, int bs = 256) {
+  size_t b = (n + bs - 1) / bs;
+  if (b > 1048576) b = 1048576;
+  return dim3((unsigned)b);
+}
+}  // namespace les
