@@ -48,17 +48,32 @@ static FftPlan make_plan(int n) {
   return p;
 }
 
+// exp(-2 pi i k/n) for k < n (run-time-length kernels), then the
+// stage-major table of the compile-time-length kernels (les_fft.h
+// fft_tw_off): stage s of radix R after NS = R_0..R_{s-1} holds
+// W^(r k n/(NS R)), r = 1..R-1, k < NS.  Computed in long double.
 template <typename T>
 static cx<T>* upload_twiddles(les_ctx* c, int n) {
-  std::vector<cx<T>> h(n);
-  for (int k = 0; k < n; ++k) {
+  std::vector<cx<T>> h(2 * n);
+  auto w = [&](long k) {
     long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
-    h[k].x = (T)cosl(a);
-    h[k].y = (T)sinl(a);
+    return cx<T>{(T)cosl(a), (T)sinl(a)};
+  };
+  for (int k = 0; k < n; ++k) h[k] = w(k);
+  const FftPlan p = make_plan(n);
+  if (p.n > 0) {
+    int off = n, ns = 1;
+    for (int s = 0; s < p.ns; ++s) {
+      const int R = p.rad[s], tws = n / (ns * R);
+      for (int r = 1; r < R; ++r)
+        for (int k = 0; k < ns; ++k) h[off + (r - 1) * ns + k] = w((long)r * k * tws);
+      off += (R - 1) * ns;
+      ns *= R;
+    }
   }
   cx<T>* d = nullptr;
-  check(c, hipMalloc(&d, n * sizeof(cx<T>)), "twiddle alloc");
-  check(c, hipMemcpy(d, h.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice), "twiddle upload");
+  check(c, hipMalloc(&d, 2 * n * sizeof(cx<T>)), "twiddle alloc");
+  check(c, hipMemcpy(d, h.data(), 2 * n * sizeof(cx<T>), hipMemcpyHostToDevice), "twiddle upload");
   return d;
 }
 template <typename T>
@@ -211,25 +226,27 @@ static ScalarSgsPtrs scalar_sgs_ptrs(les_ctx* c) {
   return p;
 }
 
-// theta_all_in_one (scalars_module.f90:57-171) and q_all_in_one
-// (scalars_module_q.f90:31-111) of one step, both scalars batched.  calcbeta
-// reads theta after its filt_da and q before its own, as in the reference.
 template <typename T>
-static void scalars_step(les_ctx* c, const T* u, const T* v, const T* w) {
+static void scalar_jobs(les_ctx* c, ScalarJob jobs[2]) {
+  jobs[0] = ScalarJob{F<T>(c, "theta"), F<T>(c, "dTdx"), F<T>(c, "dTdy"), F<T>(c, "dTdz"), F<T>(c, "RHS_T"),
+                      F<T>(c, "RHS_Tf"), (double*)state_ptr(c, "sgs_t3"), 0};
+  jobs[1] = ScalarJob{F<T>(c, "qmix"), F<T>(c, "dqdx"), F<T>(c, "dqdy"), F<T>(c, "dqdz"), F<T>(c, "RHS_q"),
+                      F<T>(c, "RHS_qf"), (double*)state_ptr(c, "sgs_q3"), 1};
+}
+
+// theta_all_in_one (scalars_module.f90:57-171) and q_all_in_one
+// (scalars_module_q.f90:31-111) up to scalar_RHS_calc: filt_da, ddz_uv and
+// the surface gradient of both scalars and calcbeta, which reads theta after
+// its filt_da and q before its own, as in the reference.  The advection, SGS
+// flux divergence and step_scalar follow inside op_convec_scalar_ab2.
+template <typename T>
+static void scalars_prep(les_ctx* c, const ScalarJob jobs[2]) {
   const Grid& g = c->g;
   const ScalarParams& P = c->scal;
-  T *th = F<T>(c, "theta"), *q = F<T>(c, "qmix");
-  ScalarJob jobs[2];
-  jobs[0] = ScalarJob{th, F<T>(c, "dTdx"), F<T>(c, "dTdy"), F<T>(c, "dTdz"), F<T>(c, "RHS_T"), F<T>(c, "RHS_Tf"),
-                      (double*)state_ptr(c, "sgs_t3"), 0};
-  jobs[1] = ScalarJob{q, F<T>(c, "dqdx"), F<T>(c, "dqdy"), F<T>(c, "dqdz"), F<T>(c, "RHS_q"), F<T>(c, "RHS_qf"),
-                      (double*)state_ptr(c, "sgs_q3"), 1};
-  op_filt_da<T>(c, th, (T*)jobs[0].dsdx, (T*)jobs[0].dsdy, g.nz);
-  op_calcbeta<T>(c, th, q, F<T>(c, "beta_scal"), P.theta_ref, P.q_ref);
-  op_filt_da<T>(c, q, (T*)jobs[1].dsdx, (T*)jobs[1].dsdy, g.nz);
+  op_filt_da<T>(c, (T*)jobs[0].s, (T*)jobs[0].dsdx, (T*)jobs[0].dsdy, g.nz);
+  op_calcbeta<T>(c, (T*)jobs[0].s, (T*)jobs[1].s, F<T>(c, "beta_scal"), P.theta_ref, P.q_ref);
+  op_filt_da<T>(c, (T*)jobs[1].s, (T*)jobs[1].dsdx, (T*)jobs[1].dsdy, g.nz);
   op_scalar_dz<T>(c, jobs, 2, (const double*)state_ptr(c, "phi_h"), (const double*)state_ptr(c, "ustar_avg"), 0);
-  op_scalar_rhs<T>(c, jobs, 2, u, v, w, F<T>(c, "Nu_t"), P.model == 7 ? F<T>(c, "Pr_t") : (const T*)nullptr, 1,
-                   (c->jt == 1 && !P.initsc) ? 1 : 0);
 }
 
 template <typename T>
@@ -266,16 +283,34 @@ static void step_once(les_ctx* c) {
   } else {
     op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c));
   }
-  if (scal) scalars_step<T>(c, u, v, w);  // main.f90:273-276
+  ScalarJob jobs[2];
+  if (scal) {  // main.f90:273-276
+    scalar_jobs<T>(c, jobs);
+    scalars_prep<T>(c, jobs);
+  }
   T *dx = F<T>(c, "divtx"), *dy = F<T>(c, "divty"), *dz = F<T>(c, "divtz");
   op_divstress<T>(c, dx, tau[0], tau[1], tau[2], 0);
   op_divstress<T>(c, dy, tau[1], tau[3], tau[4], 0);
   op_divstress<T>(c, dz, tau[2], tau[4], tau[5], 1);
   T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
-  // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
-  op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
-                   F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, (c->jt == 1 && !g.initu) ? 1 : 0,
-                   (scal && !c->scal.passive) ? (const T*)F<T>(c, "beta_scal") : (const T*)nullptr);
+  const int euler = (c->jt == 1 && !g.initu) ? 1 : 0;
+  if (scal) {
+    // convec + RHS assembly + AB2 (main.f90:304-390) and the scalars'
+    // scalar_RHS_calc + step_scalar in one padded-grid pass
+    const ScalarParams& P = c->scal;
+    T* R3[3] = {Rx, Ry, Rz};
+    T* Rf3[3] = {F<T>(c, "RHSx_f"), F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f")};
+    const T* dv3[3] = {dx, dy, dz};
+    const T* grc2[9] = {gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]};
+    op_convec_scalar_ab2<T>(c, u, v, w, grc2, R3, Rf3, dv3, euler,
+                            P.passive ? (const T*)nullptr : (const T*)F<T>(c, "beta_scal"), jobs, F<T>(c, "Nu_t"),
+                            P.model == 7 ? (const T*)F<T>(c, "Pr_t") : (const T*)nullptr,
+                            (c->jt == 1 && !P.initsc) ? 1 : 0);
+  } else {
+    // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
+    op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
+                     F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, euler);
+  }
   T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
   op_press_stag<T>(c, u, v, w, dz, p, px, py);
   op_pressure_update<T>(c, u, v, w, Rx, Ry, Rz, p, px, py, pz);

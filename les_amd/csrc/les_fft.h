@@ -129,6 +129,15 @@ template <typename T>
 __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* src, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
+// Device twiddle arrays hold the base table exp(-2 pi i k/n), k < n (run-time
+// lengths), followed by the stage-major table (compile-time lengths).
+template <typename T>
+__device__ __forceinline__ void load_twiddles_for(cx<T>* dst, const cx<T>* src, int n, bool stage_major) {
+  if (stage_major)
+    load_twiddles(dst, src + n, n - 1);
+  else
+    load_twiddles(dst, src, n);
+}
 
 }  // namespace les
 
@@ -150,14 +159,38 @@ constexpr int fft_radix(int n, int s) {
   return 1;
 }
 
+// Stage-major twiddle table of the compile-time transforms: stage s with
+// radix R and NS = product of the earlier radices stores W^(r k N/(NS R)) at
+// [fft_tw_off(N, s) + (r-1) NS + k], r = 1..R-1, k = 0..NS-1 (N-1 entries in
+// all).  Lanes that work on consecutive k read consecutive words, where a
+// single base table exp(-2 pi i j/N) indexed by r k N/(NS R) makes them
+// stride through the LDS banks.
+constexpr int fft_tw_off(int n, int s) {
+  int off = 0, ns = 1;
+  for (int i = 0; i < s; ++i) {
+    const int r = fft_radix(n, i);
+    off += (r - 1) * ns;
+    ns *= r;
+  }
+  return off;
+}
+
+// Padded LDS index of element e inside one contiguous (ES = 1) transform:
+// one spare slot every 16 complex words, so the stride-R writes of the early
+// Stockham stages spread over the LDS banks instead of piling onto a few.
+// A padded transform occupies lds_row(N) words.
+__host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int lds_row(int n) { return n + (n >> 4) + 1; }
+
 // One Stockham stage of NTR transforms of length N in LDS (element e of
-// transform t at buf[t*TS + e*ES]) executed by BLOCK threads.
-template <typename T, int N, int R, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+// transform t at buf[t*TS + e*ES], or buf[t*TS + lds_pad(e)] with PAD)
+// executed by BLOCK threads.
+template <typename T, int N, int R, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int OFF>
 __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sign) {
   constexpr int NB = N / R;
   constexpr int ITEMS = NTR * NB;
-  constexpr int TWS = N / (NS * R);
   constexpr int MAXB = (ITEMS + BLOCK - 1) / BLOCK;
+  auto at = [](int e) { return PAD ? lds_pad(e) : e * ES; };
   cx<T> v[MAXB][R];
   int dd[MAXB], tt[MAXB];
 #pragma unroll
@@ -172,9 +205,9 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
       cx<T>* base = buf + t * TS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        cx<T> a = base[(b + r * NB) * ES];
+        cx<T> a = base[at(b + r * NB)];
         if (r > 0 && NS > 1 && k > 0) {
-          cx<T> wv = tw[r * k * TWS];
+          cx<T> wv = tw[OFF + (r - 1) * NS + k];
           if (sign > 0) wv.y = -wv.y;
           a = cmul(a, wv);
         }
@@ -191,25 +224,28 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
     if (tt[i] >= 0) {
       cx<T>* base = buf + tt[i] * TS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[(dd[i] + r * NS) * ES] = v[i][r];
+      for (int r = 0; r < R; ++r) base[at(dd[i] + r * NS)] = v[i][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD>
 __device__ __forceinline__ void fft_stages_c(cx<T>* buf, const cx<T>* tw, int sign) {
   if constexpr (S < fft_nstages(N)) {
     constexpr int R = fft_radix(N, S);
-    fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
-    fft_stages_c<T, N, S + 1, NS * R, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
+    fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T, PAD, fft_tw_off(N, S)>(buf, tw, sign);
+    fft_stages_c<T, N, S + 1, NS * R, ES, TS, NTR, BLOCK, LANE_T, PAD>(buf, tw, sign);
   }
 }
 
-template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T>
+// tw: the stage-major table of N (fft_tw_off).
+// PAD (with ES = 1, TS = lds_row(N)): padded contiguous transforms
+template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD = false>
 __device__ __forceinline__ void fft_lds_c(cx<T>* buf, const cx<T>* tw, int sign) {
   static_assert(fft_nstages(N) > 0, "FFT length must be 2^a 3^b");
-  fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T>(buf, tw, sign);
+  static_assert(!PAD || (ES == 1 && TS >= lds_row(N)), "padded transforms are contiguous");
+  fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T, PAD>(buf, tw, sign);
 }
 
 }  // namespace les

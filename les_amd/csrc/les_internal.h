@@ -185,6 +185,12 @@ template <typename T>
 void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const T* v, const T* w, const T* Nu_t,
                    const T* Pr_t, int do_step, int euler);
 template <typename T> void op_step_scalar(les_ctx* c, int which, T* s, const T* RHS, const T* RHS_f);
+// convec + RHS assembly + AB2 of the momentum and both scalars' advection,
+// SGS flux divergence and step, sharing one padded-grid pass
+template <typename T>
+void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
+                          const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
+                          const T* Nu_t, const T* Pr_t, int scal_euler);
 // lagrange_scalar_Sdep.f90:1-467 (theta Nyquist-filtered in place); with
 // reuse_mom the momentum model's filtered fields of this step are reused
 template <typename T>

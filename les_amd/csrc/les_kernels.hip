@@ -284,30 +284,6 @@ void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, in
 // big-grid forward-transform loader.
 // ------------------------------------------------------------------------
 template <typename T>
-struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
-  const T *u1, *u2, *u3, *du1d2, *du1d3, *du2d1, *du2d3, *du3d1, *du3d2;
-  int ny, nz, ld;
-  T cst;
-  static constexpr double reads() { return 1.5; }  // 9 fields feed 6 rows
-  __device__ T operator()(int plane, int y, int x) const {
-    const int f = plane / nz, k = plane - f * nz;
-    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
-    switch (f) {
-      case 0: return cst * u1[i];
-      case 1: return cst * u2[i];
-      case 2: return cst * u3[i];
-      case 3:
-        if (k == 0) return cst * (T(0.5) * (du3d2[i] + du3d2[i + pl]) - du2d3[i]);
-        return cst * (du3d2[i] - du2d3[i]);
-      case 4:
-        if (k == 0) return cst * (du1d3[i] - T(0.5) * (du3d1[i] + du3d1[i + pl]));
-        return cst * (du1d3[i] - du3d1[i]);
-      default: return cst * (du2d1[i] - du1d2[i]);
-    }
-  }
-};
-
-template <typename T>
 struct StoreConvec {  // 3 outputs, levels 0..nz-2 ; level nz-1 BOGUS handled apart
   T *ox, *oy, *oz;
   int ppf, ny, ld, nx;
@@ -318,156 +294,6 @@ struct StoreConvec {  // 3 outputs, levels 0..nz-2 ; level nz-1 BOGUS handled ap
     p[((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
   }
 };
-
-// Padded-grid part of convec.f90:117-233 in one pass.  One workgroup owns
-// one padded row yb and a chunk of output levels; it marches up in z keeping
-// the physical padded rows of u1,u2,u3,omega1,omega2,omega3 for levels
-// k-1, k, k+1 in an LDS ring, forms the staggered products of level k and
-// transforms them back to x-spectral rows (kx < nx/2).  The padded physical
-// fields never touch HBM.
-template <typename T, int N>
-__global__ void __launch_bounds__((N > 0 ? 3 * N / 4 : 1024))
-    k_convec_big(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int nxh, int chunk, FftPlan p,
-                 const cx<T>* __restrict__ twg, T cst) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = N > 0 ? N : p.n;
-  auto fft3 = [&](cx<T>* buf) {
-    if constexpr (N > 0)
-      fft_lds_c<T, (N > 0 ? N : 4), 1, (N > 0 ? N : 4), 3, (N > 0 ? 3 * N / 4 : 64), false>(buf, reinterpret_cast<cx<T>*>(smem_raw), +1);
-    else
-      fft_lds<T, false, 2>(buf, p, 1, n, 3, reinterpret_cast<cx<T>*>(smem_raw), +1);
-  };
-  auto fft2 = [&](cx<T>* buf) {
-    if constexpr (N > 0)
-      fft_lds_c<T, (N > 0 ? N : 4), 1, (N > 0 ? N : 4), 2, (N > 0 ? 3 * N / 4 : 64), false>(buf, reinterpret_cast<cx<T>*>(smem_raw), -1);
-    else
-      fft_lds<T, false, 2>(buf, p, 1, n, 2, reinterpret_cast<cx<T>*>(smem_raw), -1);
-  };
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* fb = tw + n;
-  T* win = reinterpret_cast<T*>(fb + 3 * n);
-  load_twiddles(tw, twg, n);
-  const int yb = blockIdx.x;
-  const int k0 = blockIdx.y * chunk;
-  const int k1 = min(k0 + chunk, nz - 1);
-  auto W = [&](int kk, int f) -> T* { return win + ((kk % 3) * 6 + f) * n; };
-  auto load_level = [&](int kk) {
-    for (int e = threadIdx.x; e < 3 * n; e += blockDim.x) {
-      const int t = e / n, j = e - t * n;
-      const cx<T>* a = XB + (((long)(2 * t) * nz + kk) * ny2 + yb) * nxh;
-      const cx<T>* b = XB + (((long)(2 * t + 1) * nz + kk) * ny2 + yb) * nxh;
-      cx<T> z = {T(0), T(0)};
-      if (j == 0) {
-        z = {a[0].x, b[0].x};
-      } else if (j < nxh) {
-        const cx<T> A = a[j], B = b[j];
-        z = {A.x - B.y, A.y + B.x};
-      } else if (n - j < nxh) {
-        const cx<T> A = a[n - j], B = b[n - j];
-        z = {A.x + B.y, B.x - A.y};
-      }
-      fb[e] = z;
-    }
-    __syncthreads();
-    fft3(fb);
-    for (int e = threadIdx.x; e < 3 * n; e += blockDim.x) {
-      const int t = e / n, x = e - t * n;
-      W(kk, 2 * t)[x] = fb[e].x;
-      W(kk, 2 * t + 1)[x] = fb[e].y;
-    }
-    __syncthreads();
-  };
-  if (k0 >= 1) load_level(k0 - 1);
-  load_level(k0);
-  for (int k = k0; k < k1; ++k) {
-    load_level(k + 1);
-    const T *u1 = W(k, 0), *u2 = W(k, 1), *u3 = W(k, 2), *v1 = W(k, 3), *v2 = W(k, 4), *v3 = W(k, 5);
-    const T *u3p = W(k + 1, 2), *v1p = W(k + 1, 3), *v2p = W(k + 1, 4);
-    const T *u1m = W(k + 2, 0), *u2m = W(k + 2, 1);  // slot (k+2)%3 == (k-1)%3
-    for (int x = threadIdx.x; x < n; x += blockDim.x) {
-      T a, b, c;
-      if (k == 0) {
-        a = cst * (u2[x] * (-v3[x]) + T(0.5) * u3p[x] * (v2p[x]));
-        b = cst * (u1[x] * (v3[x]) + T(0.5) * u3p[x] * (-v1p[x]));
-        c = T(0);
-      } else {
-        a = cst * (u2[x] * (-v3[x]) + T(0.5) * (u3p[x] * (v2p[x]) + u3[x] * (v2[x])));
-        b = cst * (u1[x] * (v3[x]) + T(0.5) * (u3p[x] * (-v1p[x]) + u3[x] * (-v1[x])));
-        c = cst * T(0.5) * ((u1[x] + u1m[x]) * (-v2[x]) + (u2[x] + u2m[x]) * (v1[x]));
-      }
-      fb[x] = {a, b};
-      fb[n + x] = {c, T(0)};
-    }
-    __syncthreads();
-    fft2(fb);
-    const int nl = nz - 1;
-    for (int e = threadIdx.x; e < 2 * nxh; e += blockDim.x) {
-      const int t = e / nxh, kk = e - t * nxh;
-      const cx<T> zk = fb[t * n + kk], zn = fb[t * n + (kk ? n - kk : 0)];
-      const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
-      if (t == 0) {
-        const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
-        XP[(((long)0 * nl + k) * ny2 + yb) * nxh + kk] = A;
-        XP[(((long)1 * nl + k) * ny2 + yb) * nxh + kk] = B;
-      } else {
-        XP[(((long)2 * nl + k) * ny2 + yb) * nxh + kk] = A;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// main.f90:309-378 fused into the c2r epilogue of convec: RHS = -conv - divt,
-// u* = u + dt*(1.5 RHS - 0.5 RHS_f + force) (force on u only).  Arithmetic
-// and evaluation order as k_ab2.
-template <typename T>
-struct StoreConvecAB2 {
-  T* R[3];
-  T* Rf[3];
-  const T* dv[3];
-  T* U[3];
-  int ppf, ny, ld, nx;
-  T dt, force;
-  int euler;
-  const T* beta;  // buoyancy added to RHSz (main.f90:312-317), or null
-  static constexpr double writes() { return 5.0; }
-  __device__ void operator()(int plane, int y, int x, T v) const {
-    const int f = plane / ppf, r = plane - f * ppf;
-    const long i = ((long)r * ny + y) * ld + x;
-    const T conv = x < nx ? v : T(0);
-    T rhs = -conv - dv[f][i];
-    if (f == 2 && beta) rhs = rhs + beta[i];
-    R[f][i] = rhs;
-    T fo;
-    if (euler) {
-      fo = rhs;
-      Rf[f][i] = rhs;
-    } else {
-      fo = Rf[f][i];
-    }
-    if (f == 0)
-      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo + force);
-    else
-      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo);
-  }
-};
-
-template <typename T, int N>
-static void launch_convec_big(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
-  const Grid& g = c->g;
-  const int nz = g.nz;
-  const int n = g.nx2, chunk = 16;
-  const FftPlan& p = plan_for(c, n);
-  const int threads = 3 * n / 4;
-  const size_t sm = (size_t)4 * n * sizeof(cx<T>) + (size_t)18 * n * sizeof(T);
-  if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_convec_big<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  const unsigned gy = (unsigned)((nz - 1 + chunk - 1) / chunk);
-  ProfScope ps(c, "convec_big", (double)(6.0 * nz * g.ny2 + 3.0 * (nz - 1) * g.ny2) * g.nxh * sizeof(cx<T>));
-  hipLaunchKernelGGL((k_convec_big<T, N>), dim3(g.ny2, gy), dim3(threads), sm, c->stream, XB, XP, g.ny2, nz, g.nxh,
-                     chunk, p, tw_for<T>(c, n), T(1.0 / ((double)g.nx2 * g.ny2)));
-  LCHK(c, hipGetLastError());
-}
 
 // convec.f90:43-233 up to the small-grid x-spectral products (3 fields,
 // levels 1..nz-1); returns the scratch holding them.
@@ -489,16 +315,8 @@ static cx<T>* convec_spectral(les_ctx* c, const T* u, const T* v, const T* w, co
   // padded-grid products without materialising the padded fields
   const long rowsP = 3L * (nz - 1) * g.ny2;
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)rowsP * g.nxh * sizeof(cx<T>));
-  switch (g.nx2) {
-#define LES_CASE(NN) \
-  case NN:           \
-    launch_convec_big<T, NN>(c, XB, XP);  \
-    break;
-    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
-#undef LES_CASE
-    default:
-      launch_convec_big<T, 0>(c, XB, XP);
-  }
+  launch_pad_products<T, PolConvec<T>, 8, 3>(c, XB, XP, PolConvec<T>{T(1.0 / ((double)g.nx2 * g.ny2))},
+                                             "convec_big");
   // truncate in y
   const long rows3 = 3L * (nz - 1) * g.ny;
   cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)rows3 * g.nxh * sizeof(cx<T>));

@@ -3,27 +3,30 @@
 // Monin-Obukhov surface layer, buoyancy and the scalar Lagrangian
 // scale-dependent SGS model (model 7).
 //
-// Both scalars go through one batched pipeline per step:
-//   scalar_RHS_calc's dealiased advection u.grad(s) uses one forward row pass
-//   over u,v,w and the three gradients of each scalar, the 3/2 zero pad in y,
-//   and k_scalar_big, which keeps the padded physical rows of two z-levels in
-//   LDS, forms the staggered products and transforms them back (the padded
-//   fields never touch HBM).  The SGS flux divergence d/dx, d/dy is a single
-//   spectral divergence pass whose c2r also receives the advection spectrum
-//   (weight -1); its epilogue adds the d/dz flux term and takes the AB2 step
-//   of step_scalar, including the lid rule.
+// In the whole step both scalars share the momentum's padded-grid pass:
+//   one forward row transform of u, v, w, the vorticity and the three
+//   gradients of each scalar, the 3/2 zero pad in y, and k_pad_products
+//   (PolConvecScalar), which forms convec's products and scalar_RHS_calc's
+//   staggered advection products in LDS (the padded fields never touch HBM).
+//   The SGS flux divergence d/dx, d/dy of the scalars is a single spectral
+//   divergence pass whose c2r also receives the advection spectrum (weight
+//   -1); its epilogue adds the d/dz flux term and takes the AB2 step of
+//   step_scalar, including the lid rule.
 #include "les_xform.h"
 
 namespace les {
 
 // ------------------------------------------------------------------------
-// deterministic plane sums (one block per level, double accumulation)
+// deterministic plane sums in double: blockIdx.y = level, NPART blocks per
+// level write partial sums, a second kernel adds them in a fixed order
 // ------------------------------------------------------------------------
+constexpr int NPART = 32;
+
 template <class Fn>
-__global__ void k_level_sum(Fn fn, int nx, int ny, double* out) {
-  const int k = blockIdx.x;
+__global__ void k_level_partial(Fn fn, int nx, int ny, double* part) {
+  const int k = blockIdx.y, n = nx * ny;
   double a = 0.0;
-  for (int j = threadIdx.x; j < nx * ny; j += blockDim.x) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const int y = j / nx, x = j - y * nx;
     a += fn(k, y, x);
   }
@@ -34,7 +37,26 @@ __global__ void k_level_sum(Fn fn, int nx, int ny, double* out) {
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[k] = sh[0];
+  if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = sh[0];
+}
+__global__ void k_level_final(const double* part, int nparts, int nlev, double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nlev) return;
+  double a = 0.0;
+  for (int p = 0; p < nparts; ++p) a += part[k * nparts + p];
+  out[k] = a;
+}
+template <class Fn>
+static void level_sums(les_ctx* c, const Fn& fn, int nlev, double* out, double bytes) {
+  const Grid& g = c->g;
+  double* part = (double*)scratch(c, S_RED, (size_t)NPART * nlev * sizeof(double));
+  {
+    ProfScope _ps(c, "level_sum", bytes);
+    hipLaunchKernelGGL((k_level_partial<Fn>), dim3(NPART, nlev), dim3(256), 0, c->stream, fn, g.nx, g.ny, part);
+  }
+  LCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_level_final, dim3((nlev + 63) / 64), dim3(64), 0, c->stream, part, NPART, nlev, out);
+  LCHK(c, hipGetLastError());
 }
 
 template <typename T>
@@ -55,33 +77,16 @@ struct SumThetaV {  // theta (1 + 0.61e-3 q q_scale) per level, scalars_module.f
   }
 };
 
-// lagrange_scalar_Sdep.f90:111-113: sum |u(1)| over the plane and sum psi_m
+// lagrange_scalar_Sdep.f90:111: |u(1)| summed over the plane
 template <typename T>
-__global__ void k_lag_scalar_sums(const T* u, const T* v, const double* psi_m, int nx, int ny, int ld,
-                                  double* out) {
-  double a = 0.0, b = 0.0;
-  for (int j = threadIdx.x; j < nx * ny; j += blockDim.x) {
-    const int y = j / nx, x = j - y * nx;
+struct SumSpeed {
+  const T *u, *v;
+  int ld;
+  __device__ double operator()(int, int y, int x) const {
     const double uu = (double)u[(long)y * ld + x], vv = (double)v[(long)y * ld + x];
-    a += sqrt(uu * uu + vv * vv);
-    b += psi_m[j];
+    return sqrt(uu * uu + vv * vv);
   }
-  __shared__ double sa[256], sb[256];
-  sa[threadIdx.x] = a;
-  sb[threadIdx.x] = b;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      sa[threadIdx.x] += sa[threadIdx.x + s];
-      sb[threadIdx.x] += sb[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    out[0] = sa[0];
-    out[1] = sb[0];
-  }
-}
+};
 
 // slots of the S_RED2 scratch (doubles)
 constexpr int R2_TSUM = 0, R2_LAG = 2, R2_BETA = 8;
@@ -163,12 +168,7 @@ void op_obukhov(les_ctx* c, const T* u, const T* v, const T* theta, double* sgs_
   op_copy<T>(c, planes + pl, v, pl);
   op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);  // :560-563
   double* red = (double*)scratch(c, S_RED2, (R2_BETA + g.nz) * sizeof(double));
-  {
-    ProfScope _ps(c, "plane_sum", (double)g.nx * g.ny * sizeof(T));
-    hipLaunchKernelGGL((k_level_sum<SumPlane<T>>), dim3(1), dim3(256), 0, c->stream, SumPlane<T>{theta, g.ld, th_ref}, g.nx,
-                       g.ny, red + R2_TSUM);
-  }
-  LCHK(c, hipGetLastError());
+  level_sums(c, SumPlane<T>{theta, g.ld, th_ref}, 1, red + R2_TSUM, (double)g.nx * g.ny * sizeof(T));
   {
     ProfScope _ps(c, "obukhov", 12.0 * g.nx * g.ny * 8);
     hipLaunchKernelGGL((k_obukhov<T>), grid1((size_t)g.nx * g.ny), dim3(256), 0, c->stream, planes, planes + pl, theta,
@@ -209,12 +209,8 @@ void op_calcbeta(les_ctx* c, const T* theta, const T* qmix, T* beta, double th_r
   const Grid& g = c->g;
   const ScalarParams& P = c->scal;
   double* red = (double*)scratch(c, S_RED2, (R2_BETA + g.nz) * sizeof(double));
-  {
-    ProfScope _ps(c, "level_sum", 2.0 * g.nx * g.ny * g.nz * sizeof(T));
-    hipLaunchKernelGGL((k_level_sum<SumThetaV<T>>), dim3(g.nz), dim3(256), 0, c->stream,
-                       SumThetaV<T>{theta, qmix, g.ld, g.ny, P.q_scale, th_ref, q_ref}, g.nx, g.ny, red + R2_BETA);
-  }
-  LCHK(c, hipGetLastError());
+  level_sums(c, SumThetaV<T>{theta, qmix, g.ld, g.ny, P.q_scale, th_ref, q_ref}, g.nz, red + R2_BETA,
+             2.0 * g.nx * g.ny * g.nz * sizeof(T));
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
   {
     ProfScope _ps(c, "beta", 3.0 * n3 * sizeof(T));
@@ -316,133 +312,6 @@ void op_scalar_dz(les_ctx* c, const ScalarJob* jobs, int ns, const double* phi_h
   LCHK(c, hipGetLastError());
 }
 
-// ------------------------------------------------------------------------
-// scalar_RHS_calc advection: padded-grid products in LDS
-// ------------------------------------------------------------------------
-// One workgroup owns one padded row yb and a chunk of levels.  XB holds the
-// x-spectral rows of NF = 3 + 3 NS fields (u, v, w, then ds/dx, ds/dy, ds/dz
-// per scalar) on the padded y grid; two consecutive levels are kept as
-// physical padded rows in LDS.  The products of scalars_module.f90:322-338
-// are transformed back and stored as x-spectral rows (kx < nx/2) in XP.
-template <typename T, int N, int NS>
-__global__ void __launch_bounds__((N > 0 ? 3 * N / 4 : 1024))
-    k_scalar_big(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int nxh, int chunk, FftPlan p,
-                 const cx<T>* __restrict__ twg, T cst) {
-  constexpr int NF = 3 + 3 * NS, NP = (NF + 1) / 2;
-  constexpr int NB = N > 0 ? N : 4;
-  constexpr int BLK = N > 0 ? 3 * N / 4 : 64;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = N > 0 ? N : p.n;
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* fb = tw + n;
-  T* win = reinterpret_cast<T*>(fb + NP * n);
-  load_twiddles(tw, twg, n);
-  const int yb = blockIdx.x;
-  const int k0 = blockIdx.y * chunk;
-  const int k1 = min(k0 + chunk, nz - 1);
-  auto W = [&](int kk, int f) -> T* { return win + ((kk & 1) * NF + f) * n; };
-  auto load_level = [&](int kk) {
-    for (int e = threadIdx.x; e < NP * n; e += blockDim.x) {
-      const int t = e / n, j = e - t * n;
-      const int fa = 2 * t, fb2 = 2 * t + 1;
-      const cx<T>* a = XB + (((long)fa * nz + kk) * ny2 + yb) * nxh;
-      const cx<T>* b = XB + (((long)fb2 * nz + kk) * ny2 + yb) * nxh;
-      const bool hb = fb2 < NF;
-      cx<T> z = {T(0), T(0)};
-      if (j == 0) {
-        z = {a[0].x, hb ? b[0].x : T(0)};
-      } else if (j < nxh) {
-        const cx<T> A = a[j], B = hb ? b[j] : cx<T>{T(0), T(0)};
-        z = {A.x - B.y, A.y + B.x};
-      } else if (n - j < nxh) {
-        const cx<T> A = a[n - j], B = hb ? b[n - j] : cx<T>{T(0), T(0)};
-        z = {A.x + B.y, B.x - A.y};
-      }
-      fb[e] = z;
-    }
-    __syncthreads();
-    if constexpr (N > 0)
-      fft_lds_c<T, NB, 1, NB, NP, BLK, false>(fb, tw, +1);
-    else
-      fft_lds<T, false, 4>(fb, p, 1, n, NP, tw, +1);
-    for (int e = threadIdx.x; e < NP * n; e += blockDim.x) {
-      const int t = e / n, x = e - t * n;
-      W(kk, 2 * t)[x] = fb[e].x;
-      if (2 * t + 1 < NF) W(kk, 2 * t + 1)[x] = fb[e].y;
-    }
-    __syncthreads();
-  };
-  load_level(k0);
-  for (int k = k0; k < k1; ++k) {
-    load_level(k + 1);
-    const T *u = W(k, 0), *v = W(k, 1), *w = W(k, 2), *wp = W(k + 1, 2);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) {
-      T r[2] = {T(0), T(0)};
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const T *sx = W(k, 3 + 3 * s), *sy = W(k, 4 + 3 * s), *sz = W(k, 5 + 3 * s), *szp = W(k + 1, 5 + 3 * s);
-        if (k == 0)
-          r[s] = cst * (u[x] * sx[x] + v[x] * sy[x] + (T(0.5) * wp[x]) * szp[x]);
-        else
-          r[s] = cst * (u[x] * sx[x] + v[x] * sy[x] + (w[x] * sz[x] + wp[x] * szp[x]) / T(2));
-      }
-      fb[x] = {r[0], r[1]};
-    }
-    __syncthreads();
-    if constexpr (N > 0)
-      fft_lds_c<T, NB, 1, NB, 1, BLK, false>(fb, tw, -1);
-    else
-      fft_lds<T, false, 4>(fb, p, 1, n, 1, tw, -1);
-    const int nl = nz - 1;
-    for (int kk = threadIdx.x; kk < nxh; kk += blockDim.x) {
-      const cx<T> zk = fb[kk], zn = fb[kk ? n - kk : 0];
-      const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
-      XP[(((long)0 * nl + k) * ny2 + yb) * nxh + kk] = A;
-      if (NS > 1) {
-        const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
-        XP[(((long)1 * nl + k) * ny2 + yb) * nxh + kk] = B;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T, int N, int NS>
-static void launch_scalar_big_n(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
-  const Grid& g = c->g;
-  constexpr int NF = 3 + 3 * NS, NP = (NF + 1) / 2;
-  const int n = g.nx2, chunk = 16;
-  const FftPlan& p = plan_for(c, n);
-  int threads = 3 * n / 4;
-  if (N == 0) {
-    threads = (threads + 63) / 64 * 64;
-    if (threads > 1024) threads = 1024;
-  }
-  const size_t sm = (size_t)(1 + NP) * n * sizeof(cx<T>) + (size_t)2 * NF * n * sizeof(T);
-  if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_scalar_big<T, N, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)sm));
-  const unsigned gy = (unsigned)((g.nz - 1 + chunk - 1) / chunk);
-  ProfScope ps(c, "scalar_big",
-               (double)(NF * (double)g.nz * g.ny2 + NS * (double)(g.nz - 1) * g.ny2) * g.nxh * sizeof(cx<T>));
-  hipLaunchKernelGGL((k_scalar_big<T, N, NS>), dim3(g.ny2, gy), dim3(threads), sm, c->stream, XB, XP, g.ny2, g.nz,
-                     g.nxh, chunk, p, tw_for<T>(c, n), T(1.0 / ((double)g.nx2 * g.ny2)));
-  LCHK(c, hipGetLastError());
-}
-
-template <typename T, int NS>
-static void launch_scalar_big(les_ctx* c, const cx<T>* XB, cx<T>* XP) {
-  switch (c->g.nx2) {
-#define LES_CASE(NN) \
-  case NN:           \
-    return launch_scalar_big_n<T, NN, NS>(c, XB, XP);
-    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
-#undef LES_CASE
-    default:
-      return launch_scalar_big_n<T, 0, NS>(c, XB, XP);
-  }
-}
-
 // SGS fluxes of scalars_module.f90:366-432 (times the ddx/ddy normalisation):
 // field f = dir * NS + s, dir 0 -> ds/dx, 1 -> ds/dy; levels 1..nz-1
 template <typename T>
@@ -523,13 +392,53 @@ struct StoreScalarRHS {
   }
 };
 
+// horizontal SGS flux divergence of the scalars (spectral), plus -advection
+// from X3 (x-spectral, ns*(nz-1) planes), then the d/dz flux term and the AB2
+// step in the c2r epilogue
+template <typename T>
+static void scalar_sgs_finish(les_ctx* c, const ScalarJob* jobs, int ns, const T* Nu_t, const T* Pr_t,
+                              const cx<T>* X3, int do_step, int euler) {
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1;
+  const long rowsT = 2L * ns * nl * g.ny;
+  cx<T>* XT = (cx<T>*)scratch(c, S_XS0, (size_t)rowsT * g.nxh * sizeof(cx<T>));
+  LoadScalarTemp<T> LT{};
+  for (int q = 0; q < ns; ++q) {
+    LT.ds[0][q] = (const T*)jobs[q].dsdx;
+    LT.ds[1][q] = (const T*)jobs[q].dsdy;
+  }
+  LT.Nu = Nu_t; LT.Pr = Pr_t; LT.Prc = (T)c->scal.Pr;
+  LT.ns = ns; LT.ppf = nl; LT.ny = g.ny; LT.ld = g.ld; LT.cst = T(1.0 / ((double)g.nx * g.ny));
+  launch_row_fwd<T>(c, LT, XT, g.nxh, rowsT, g.nx, g.nxh, g.ny);
+  cx<T>* XD = (cx<T>*)scratch(c, S_XS1, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
+  ColArgs d = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  d.in2_plane = d.in_plane;
+  col_out(d, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XT, XT + (size_t)ns * nl * g.ny * g.nxh, XD, nullptr, nullptr, d, (long)ns * nl, g.nxh);
+  StoreScalarRHS<T> S{};
+  for (int q = 0; q < ns; ++q) {
+    S.s[q] = (T*)jobs[q].s;
+    S.R[q] = (T*)jobs[q].RHS;
+    S.Rf[q] = (T*)jobs[q].RHS_f;
+    S.dsdz[q] = (const T*)jobs[q].dsdz;
+    S.sv1[q] = jobs[q].sgs_vert1;
+    S.sf[q] = surface_flux(c, jobs[q].which);
+    S.gz[q] = (T)(g.Lz > 1.0 ? lid_gradient(c, jobs[q].which) * g.dz : 0.0);
+  }
+  S.Nu = Nu_t; S.Pr = Pr_t; S.Prc = (T)c->scal.Pr;
+  S.ppf = nl; S.ny = g.ny; S.ld = g.ld; S.nx = g.nx; S.nz = nz;
+  S.rdz = T(1.0 / g.dz); S.dt = T(g.dt);
+  S.do_step = do_step; S.euler = euler;
+  launch_row_inv<T>(c, XD, g.nxh, g.nxh, S, (long)ns * nl * g.ny, g.nx, g.ld, g.ny, X3, T(-1));
+}
+
 template <typename T>
 void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const T* v, const T* w, const T* Nu_t,
                    const T* Pr_t, int do_step, int euler) {
   OpTag _tag(c, "scalar_rhs");
   const Grid& g = c->g;
   const int nz = g.nz, nl = nz - 1, nf = 3 + 3 * ns;
-  // 1) dealias1 of u, v, w and the gradients: x-spectral rows, padded in y
+  // dealias1 of u, v, w and the gradients: x-spectral rows, padded in y
   const long rows = (long)nf * nz * g.ny;
   cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
   LoadFieldsN<T, 9> L{};
@@ -546,49 +455,75 @@ void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const 
   ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
   launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, (long)nf * nz, g.nxh);
-  // 2) products on the padded grid (dealias2 scaling included)
+  // products on the padded grid (dealias2 scaling included)
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)ns * nl * g.ny2 * g.nxh * sizeof(cx<T>));
+  const T cst = T(1.0 / ((double)g.nx2 * g.ny2));
   if (ns == 2)
-    launch_scalar_big<T, 2>(c, XB, XP);
+    launch_pad_products<T, PolScalar<T, 2>, 7, 3>(c, XB, XP, PolScalar<T, 2>{cst}, "scalar_big");
   else
-    launch_scalar_big<T, 1>(c, XB, XP);
-  // 3) truncate in y (unpadd)
+    launch_pad_products<T, PolScalar<T, 1>, 8, 4>(c, XB, XP, PolScalar<T, 1>{cst}, "scalar_big");
+  // truncate in y (unpadd)
   cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
   ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
   col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
   launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, (long)ns * nl, g.nxh);
-  // 4) horizontal SGS fluxes -> spectral divergence
-  const long rowsT = 2L * ns * nl * g.ny;
-  cx<T>* XT = (cx<T>*)scratch(c, S_XS0, (size_t)rowsT * g.nxh * sizeof(cx<T>));
-  LoadScalarTemp<T> LT{};
-  for (int q = 0; q < ns; ++q) {
-    LT.ds[0][q] = (const T*)jobs[q].dsdx;
-    LT.ds[1][q] = (const T*)jobs[q].dsdy;
+  scalar_sgs_finish<T>(c, jobs, ns, Nu_t, Pr_t, X3, do_step, euler);
+}
+
+// convec + RHS assembly + AB2 of the momentum (main.f90:304-390) and both
+// scalars' scalar_RHS_calc + step_scalar in one padded-grid pass: u, v, w and
+// the vorticity are transformed once for both
+template <typename T>
+struct LoadConvecScalar {
+  LoadConvec<T> lc;
+  const T* gs[6];  // dtheta/dx,dy,dz, dq/dx,dy,dz
+  static constexpr double reads() { return 15.0 / 12.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / lc.nz;
+    if (f < 6) return lc(plane, y, x);
+    const int k = plane - f * lc.nz;
+    return lc.cst * gs[f - 6][((long)k * lc.ny + y) * lc.ld + x];
   }
-  LT.Nu = Nu_t; LT.Pr = Pr_t; LT.Prc = (T)c->scal.Pr;
-  LT.ns = ns; LT.ppf = nl; LT.ny = g.ny; LT.ld = g.ld; LT.cst = T(1.0 / ((double)g.nx * g.ny));
-  launch_row_fwd<T>(c, LT, XT, g.nxh, rowsT, g.nx, g.nxh, g.ny);
-  cx<T>* XD = (cx<T>*)scratch(c, S_XS1, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
-  ColArgs d = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  d.in2_plane = d.in_plane;
-  col_out(d, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  launch_col<T>(c, XT, XT + (size_t)ns * nl * g.ny * g.nxh, XD, nullptr, nullptr, d, (long)ns * nl, g.nxh);
-  // 5) c2r of (divergence - advection) with the d/dz term and the AB2 step
-  StoreScalarRHS<T> S{};
-  for (int q = 0; q < ns; ++q) {
-    S.s[q] = (T*)jobs[q].s;
-    S.R[q] = (T*)jobs[q].RHS;
-    S.Rf[q] = (T*)jobs[q].RHS_f;
-    S.dsdz[q] = (const T*)jobs[q].dsdz;
-    S.sv1[q] = jobs[q].sgs_vert1;
-    S.sf[q] = surface_flux(c, jobs[q].which);
-    S.gz[q] = (T)(g.Lz > 1.0 ? lid_gradient(c, jobs[q].which) * g.dz : 0.0);
+};
+
+template <typename T>
+void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
+                          const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
+                          const T* Nu_t, const T* Pr_t, int scal_euler) {
+  OpTag _tag(c, "convec_scalar");
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1;
+  const long rows = 12L * nz * g.ny;
+  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
+  LoadConvecScalar<T> L{};
+  L.lc = LoadConvec<T>{u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], g.ny, nz, g.ld,
+                       T(1.0 / ((double)g.nx * g.ny))};
+  for (int q = 0; q < 2; ++q) {
+    L.gs[3 * q] = (const T*)jobs[q].dsdx;
+    L.gs[3 * q + 1] = (const T*)jobs[q].dsdy;
+    L.gs[3 * q + 2] = (const T*)jobs[q].dsdz;
   }
-  S.Nu = Nu_t; S.Pr = Pr_t; S.Prc = (T)c->scal.Pr;
-  S.ppf = nl; S.ny = g.ny; S.ld = g.ld; S.nx = g.nx; S.nz = nz;
-  S.rdz = T(1.0 / g.dz); S.dt = T(g.dt);
-  S.do_step = do_step; S.euler = euler;
-  launch_row_inv<T>(c, XD, g.nxh, g.nxh, S, (long)ns * nl * g.ny, g.nx, g.ld, g.ny, X3, T(-1));
+  launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
+  const long rowsB = 12L * nz * g.ny2;
+  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 12L * nz, g.nxh);
+  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)5 * nl * g.ny2 * g.nxh * sizeof(cx<T>));
+  launch_pad_products<T, PolConvecScalar<T>, 4, 1>(c, XB, XP, PolConvecScalar<T>{T(1.0 / ((double)g.nx2 * g.ny2))},
+                                                   "convec_scalar_big");
+  cx<T>* X3 = (cx<T>*)scratch(c, S_XS3, (size_t)5 * nl * g.ny * g.nxh * sizeof(cx<T>));
+  ColArgs b = col_args(g.ny2, g.ny, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, 5L * nl, g.nxh);
+  // scalars first: their epilogue reads no velocity, the momentum one updates it
+  scalar_sgs_finish<T>(c, jobs, 2, Nu_t, Pr_t, X3 + (size_t)3 * nl * g.ny * g.nxh, 1, scal_euler);
+  StoreConvecAB2<T> SC{{R[0], R[1], R[2]}, {Rf[0], Rf[1], Rf[2]}, {divt[0], divt[1], divt[2]}, {u, v, w},
+                       nl, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
+  launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * nl * g.ny, g.nx, g.ld, g.ny);
+  const size_t pl = (size_t)g.ld * g.ny, top = pl * nl;
+  T* tops[9] = {R[0], R[1], R[2], u, v, w, Rf[0], Rf[1], Rf[2]};
+  for (int q = 0; q < (euler_first ? 9 : 6); ++q) op_fill<T>(c, tops[q] + top, pl, T(BOGUS));
 }
 
 // step_scalar, scalars_module.f90:457-504 (routine-level entry point)
@@ -751,9 +686,8 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   op_copy<T>(c, tht, theta, n3);
   op_filt_da<T>(c, theta, dTx, dTy, nz);
   double* red = (double*)scratch(c, S_RED2, (R2_BETA + nz) * sizeof(double));
-  hipLaunchKernelGGL((k_lag_scalar_sums<T>), dim3(1), dim3(256), 0, c->stream, u, v, ss.psi_m, g.nx, g.ny, g.ld,
-                     red + R2_LAG);
-  LCHK(c, hipGetLastError());
+  level_sums(c, SumSpeed<T>{u, v, g.ld}, 1, red + R2_LAG, 2.0 * g.nx * g.ny * sizeof(T));
+  level_sums(c, SumPlane<double>{ss.psi_m, g.nx, 0.0}, 1, red + R2_LAG + 1, (double)g.nx * g.ny * 8);
   {
     DzArgs<T> a{};
     a.ns = 1;
@@ -813,6 +747,9 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   template void op_scalar_rhs<T>(les_ctx*, const ScalarJob*, int, const T*, const T*, const T*, const T*,        \
                                  const T*, int, int);                                                            \
   template void op_step_scalar<T>(les_ctx*, int, T*, const T*, const T*);                                        \
+  template void op_convec_scalar_ab2<T>(les_ctx*, T*, T*, T*, const T* const[9], T* const[3], T* const[3],        \
+                                        const T* const[3], int, const T*, const ScalarJob*, const T*, const T*,   \
+                                        int);                                                                     \
   template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
                                       const SgsPtrs&, const ScalarSgsPtrs&, bool);
 

@@ -72,8 +72,13 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int n = N > 0 ? N : p.n;
   const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
   const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  constexpr int RS = N > 0 ? lds_row(N) : 0;  // padded row stride (compile-time lengths)
+  auto ix = [&](int t, int e) { return N > 0 ? t * RS + lds_pad(e) : t * n + e; };
   cx<T>* buf = tw + n;
-  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  if (N > 0)
+    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + i];
+  else
+    for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
@@ -84,17 +89,17 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
       z.x = L(plane, 2 * pp, x);
       z.y = L(plane, 2 * pp + 1, x);
     }
-    buf[e] = z;
+    buf[ix(t, x)] = z;
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, -1);
+    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, -1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
   for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
     const int t = e / kxo, k = e - t * kxo, pp = pair0 + t;
     if (pp >= npp) continue;
-    const cx<T> zk = buf[t * n + k], zn = buf[t * n + (k ? n - k : 0)];
+    const cx<T> zk = buf[ix(t, k)], zn = buf[ix(t, k ? n - k : 0)];
     const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
     const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
     const long row = (long)plane * ny + 2 * pp;
@@ -112,8 +117,13 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int n = N > 0 ? N : p.n;
   const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
   const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  constexpr int RS = N > 0 ? lds_row(N) : 0;  // padded row stride (compile-time lengths)
+  auto ix = [&](int t, int e) { return N > 0 ? t * RS + lds_pad(e) : t * n + e; };
   cx<T>* buf = tw + n;
-  for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
+  if (N > 0)
+    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + i];
+  else
+    for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
@@ -144,11 +154,11 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
         z = {A.x + B.y, B.x - A.y};
       }
     }
-    buf[e] = z;
+    buf[ix(t, j)] = z;
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, N, 1, N, RowGeom<N>::P, RowGeom<N>::BLOCK, false>(buf, tw, +1);
+    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, +1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
   for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
@@ -156,8 +166,8 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     if (pp >= npp) continue;
     T v0 = T(0), v1 = T(0);
     if (x < n) {
-      v0 = buf[t * n + x].x;
-      v1 = buf[t * n + x].y;
+      v0 = buf[ix(t, x)].x;
+      v1 = buf[ix(t, x)].y;
     }
     S(plane, 2 * pp, x, v0);
     S(plane, 2 * pp + 1, x, v1);
@@ -176,7 +186,7 @@ static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, lon
   if (P < 1) P = 1;
   const int threads = P * tpf;
   const long blocks = (ny / 2 + P - 1) / P;
-  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
   hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
                      ld, out, orow, ny, p, tw_for<T>(c, n), kxo);
@@ -206,7 +216,7 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
   if (P < 1) P = 1;
   const int threads = P * tpf;
   const long blocks = (ny / 2 + P - 1) / P;
-  const size_t sm = (size_t)(n + P * n) * sizeof(cx<T>);
+  const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_inv",
                (double)nplanes * ny * (kxi * sizeof(cx<T>) * (in2 ? 2 : 1) + ldo * sizeof(T) * st.writes()));
   hipLaunchKernelGGL((k_row_inv<T, N, S>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
@@ -320,8 +330,8 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
   cx<T>* two = twi + nyi;
   cx<T>* A = two + nyo;
   cx<T>* B = A + nyi * TX;
-  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = twin[i];
-  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = twout[i];
+  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = CT ? (i < nyi - 1 ? twin[nyi + i] : cx<T>{}) : twin[i];
+  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = CT ? (i < nyo - 1 ? twout[nyo + i] : cx<T>{}) : twout[i];
   auto fwd = [&](cx<T>* buf) {
     if constexpr (CT)
       fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, twi, -1);
@@ -518,4 +528,273 @@ static dim3 grid1(size_t n, int bs = 256) {
   if (b > 1048576) b = 1048576;
   return dim3((unsigned)b);
 }
+
+// ------------------------------------------------------------------------
+// convec.f90 loader / AB2 epilogue (shared by the neutral and the convective
+// steps)
+// ------------------------------------------------------------------------
+template <typename T>
+struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
+  const T *u1, *u2, *u3, *du1d2, *du1d3, *du2d1, *du2d3, *du3d1, *du3d2;
+  int ny, nz, ld;
+  T cst;
+  static constexpr double reads() { return 1.5; }  // 9 fields feed 6 rows
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nz, k = plane - f * nz;
+    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
+    switch (f) {
+      case 0: return cst * u1[i];
+      case 1: return cst * u2[i];
+      case 2: return cst * u3[i];
+      case 3:
+        if (k == 0) return cst * (T(0.5) * (du3d2[i] + du3d2[i + pl]) - du2d3[i]);
+        return cst * (du3d2[i] - du2d3[i]);
+      case 4:
+        if (k == 0) return cst * (du1d3[i] - T(0.5) * (du3d1[i] + du3d1[i + pl]));
+        return cst * (du1d3[i] - du3d1[i]);
+      default: return cst * (du2d1[i] - du1d2[i]);
+    }
+  }
+};
+
+// main.f90:309-378 fused into the c2r epilogue of convec: RHS = -conv - divt,
+// u* = u + dt*(1.5 RHS - 0.5 RHS_f + force) (force on u only).  Arithmetic
+// and evaluation order as k_ab2.
+template <typename T>
+struct StoreConvecAB2 {
+  T* R[3];
+  T* Rf[3];
+  const T* dv[3];
+  T* U[3];
+  int ppf, ny, ld, nx;
+  T dt, force;
+  int euler;
+  const T* beta;  // buoyancy added to RHSz (main.f90:312-317), or null
+  static constexpr double writes() { return 5.0; }
+  __device__ void operator()(int plane, int y, int x, T v) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    const long i = ((long)r * ny + y) * ld + x;
+    const T conv = x < nx ? v : T(0);
+    T rhs = -conv - dv[f][i];
+    if (f == 2 && beta) rhs = rhs + beta[i];
+    R[f][i] = rhs;
+    T fo;
+    if (euler) {
+      fo = rhs;
+      Rf[f][i] = rhs;
+    } else {
+      fo = Rf[f][i];
+    }
+    if (f == 0)
+      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo + force);
+    else
+      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo);
+  }
+};
+
+
+// ------------------------------------------------------------------------
+// 3/2-dealiased pointwise products on the padded grid, level-batched.
+//
+// XB holds x-spectral rows (kx < nx/2) of NF real fields on the padded y grid,
+// field-major: row ((f * nz + k) * ny2 + y).  One workgroup owns one padded
+// row yb and CH consecutive output levels.  It loads the CH + HB + HA levels
+// it needs (HB below, HA above), pairs the fields two per complex transform,
+// runs ONE batched inverse x-FFT of length nx2 over all of them in LDS,
+// forms the policy's products for the CH levels, runs ONE batched forward
+// FFT and stores the kx < nx/2 coefficients of the NOUT outputs to XP (row
+// ((o * (nz-1) + k) * ny2 + y)).  Batching the levels keeps the number of
+// workgroup barriers per level small; the padded physical fields stay in LDS.
+// ------------------------------------------------------------------------
+template <typename T, int N, class Pol, int CH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    k_pad_products(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int nxh, FftPlan p,
+                   const cx<T>* __restrict__ twg, Pol pol) {
+  constexpr int NF = Pol::NF, NP = (NF + 1) / 2, NO = Pol::NOUT, NOP = (NO + 1) / 2;
+  constexpr int HB = Pol::HB, NL = CH + Pol::HB + Pol::HA;
+  constexpr int NB = N > 0 ? N : 4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = N > 0 ? N : p.n;
+  constexpr int RS = N > 0 ? lds_row(N) : 0;  // padded transform stride (compile-time lengths)
+  const int rs = N > 0 ? RS : n;
+  auto ix = [&](int e) { return N > 0 ? lds_pad(e) : e; };
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* win = tw + n;              // [NL][NP][rs]: field pairs, physical after the inverse FFT
+  cx<T>* out = win + NL * NP * rs;  // [CH][NOP][rs]: product pairs
+  load_twiddles_for(tw, twg, n, N > 0);
+  const int yb = blockIdx.x;
+  const int nl = nz - 1;
+  const int k0 = blockIdx.y * CH;
+  for (int e = threadIdx.x; e < NL * NP * n; e += BLOCK) {
+    const int lv = e / (NP * n), r = e - lv * NP * n, t = r / n, j = r - t * n;
+    const int kk = k0 - HB + lv;
+    cx<T> z = {T(0), T(0)};
+    if (kk >= 0 && kk < nz) {
+      const bool hb = 2 * t + 1 < NF;
+      const cx<T>* a = XB + (((long)(2 * t) * nz + kk) * ny2 + yb) * nxh;
+      const cx<T>* b = XB + (((long)(2 * t + 1) * nz + kk) * ny2 + yb) * nxh;
+      const cx<T> zero = {T(0), T(0)};
+      if (j == 0) {
+        z = {a[0].x, hb ? b[0].x : T(0)};
+      } else if (j < nxh) {
+        const cx<T> A = a[j], B = hb ? b[j] : zero;
+        z = {A.x - B.y, A.y + B.x};
+      } else if (n - j < nxh) {
+        const cx<T> A = a[n - j], B = hb ? b[n - j] : zero;
+        z = {A.x + B.y, B.x - A.y};
+      }
+    }
+    win[(lv * NP + t) * rs + ix(j)] = z;
+  }
+  __syncthreads();
+  if constexpr (N > 0)
+    fft_lds_c<T, NB, 1, RS, NL * NP, BLOCK, false, true>(win, tw, +1);
+  else
+    fft_lds<T, false, 8>(win, p, 1, n, NL * NP, tw, +1);
+  for (int e = threadIdx.x; e < CH * n; e += BLOCK) {
+    const int lv = e / n, x = e - lv * n, k = k0 + lv;
+    T o[2 * NOP];
+#pragma unroll
+    for (int q = 0; q < 2 * NOP; ++q) o[q] = T(0);
+    if (k < nl) {
+      auto F = [&](int dk, int f) -> T {
+        const cx<T> z = win[((lv + HB + dk) * NP + (f >> 1)) * rs + ix(x)];
+        return (f & 1) ? z.y : z.x;
+      };
+      pol.product(k, F, o);
+    }
+#pragma unroll
+    for (int q = 0; q < NOP; ++q) out[(lv * NOP + q) * rs + ix(x)] = {o[2 * q], o[2 * q + 1]};
+  }
+  __syncthreads();
+  if constexpr (N > 0)
+    fft_lds_c<T, NB, 1, RS, CH * NOP, BLOCK, false, true>(out, tw, -1);
+  else
+    fft_lds<T, false, 8>(out, p, 1, n, CH * NOP, tw, -1);
+  for (int e = threadIdx.x; e < CH * NO * nxh; e += BLOCK) {
+    const int lv = e / (NO * nxh), r = e - lv * NO * nxh, o = r / nxh, kk = r - o * nxh;
+    const int k = k0 + lv;
+    if (k >= nl) continue;
+    const cx<T>* z = out + (lv * NOP + (o >> 1)) * rs;
+    const cx<T> zk = z[ix(kk)], zn = z[ix(kk ? n - kk : 0)];
+    const cx<T> v = (o & 1) ? cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)}
+                            : cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+    XP[(((long)Pol::slot(o) * nl + k) * ny2 + yb) * nxh + kk] = v;
+  }
+}
+
+// LDS bytes of k_pad_products (padded rows for compile-time lengths)
+template <typename T, class Pol, int CH>
+constexpr size_t pad_products_lds(int n, bool padded) {
+  return sizeof(cx<T>) * ((size_t)n + (size_t)(padded ? lds_row(n) : n) *
+                                          ((size_t)(CH + Pol::HB + Pol::HA) * ((Pol::NF + 1) / 2) +
+                                           (size_t)CH * ((Pol::NOUT + 1) / 2)));
+}
+
+template <typename T, int N, class Pol, int CH>
+static void launch_pad_products_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Pol& pol, const char* tag) {
+  constexpr int BLOCK = (N >= 192 || N == 0) ? 1024 : 256;
+  static_assert(N == 0 || pad_products_lds<T, Pol, CH>(N, true) <= 163840, "padded-product tile exceeds LDS");
+  const Grid& g = c->g;
+  const int n = g.nx2, nz = g.nz;
+  const size_t sm = pad_products_lds<T, Pol, CH>(n, N > 0);
+  if (sm > 163840) {
+    c->err = "padded-product tile exceeds LDS for this grid";
+    return;
+  }
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_pad_products<T, N, Pol, CH, BLOCK>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const unsigned gy = (unsigned)((nz - 1 + CH - 1) / CH);
+  ProfScope ps(c, tag,
+               ((double)Pol::NF * nz + (double)Pol::NOUT * (nz - 1)) * g.ny2 * g.nxh * sizeof(cx<T>));
+  hipLaunchKernelGGL((k_pad_products<T, N, Pol, CH, BLOCK>), dim3(g.ny2, gy), dim3(BLOCK), sm, c->stream, XB, XP,
+                     g.ny2, nz, g.nxh, plan_for(c, n), tw_for<T>(c, n), pol);
+  LCHK(c, hipGetLastError());
+}
+
+// CH32 / CH64: levels per workgroup in FP32 / FP64 (sized to the 160 KB LDS)
+template <typename T, class Pol, int CH32, int CH64>
+static void launch_pad_products(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Pol& pol, const char* tag) {
+  constexpr int CH = sizeof(T) == 4 ? CH32 : CH64;
+  switch (c->g.nx2) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return launch_pad_products_n<T, NN, Pol, CH>(c, XB, XP, pol, tag);
+    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
+#undef LES_CASE
+    default:
+      return launch_pad_products_n<T, 0, Pol, CH>(c, XB, XP, pol, tag);
+  }
+}
+
+// convec.f90:117-233: -(u x omega) at uvp (cx, cy) and w nodes (cz) from
+// u1,u2,u3, omega1..3 (fields 0..5); cst = 1/(nx2 ny2)
+template <typename T>
+struct PolConvec {
+  static constexpr int NF = 6, NOUT = 3, HB = 1, HA = 1;
+  __device__ static int slot(int o) { return o; }  // output o -> plane block of XP
+  T cst;
+  template <class Fn>
+  __device__ void product(int k, Fn F, T* o) const {
+    const T u1 = F(0, 0), u2 = F(0, 1), u3 = F(0, 2), v1 = F(0, 3), v2 = F(0, 4), v3 = F(0, 5);
+    const T u3p = F(1, 2), v1p = F(1, 3), v2p = F(1, 4);
+    if (k == 0) {
+      o[0] = cst * (u2 * (-v3) + T(0.5) * u3p * (v2p));
+      o[1] = cst * (u1 * (v3) + T(0.5) * u3p * (-v1p));
+      o[2] = T(0);
+    } else {
+      const T u1m = F(-1, 0), u2m = F(-1, 1);
+      o[0] = cst * (u2 * (-v3) + T(0.5) * (u3p * (v2p) + u3 * (v2)));
+      o[1] = cst * (u1 * (v3) + T(0.5) * (u3p * (-v1p) + u3 * (-v1)));
+      o[2] = cst * T(0.5) * ((u1 + u1m) * (-v2) + (u2 + u2m) * (v1));
+    }
+  }
+};
+
+// scalar_RHS_calc advection (scalars_module.f90:322-338) of NS scalars: fields
+// u, v, w (F0 + 0..2) and ds/dx, ds/dy, ds/dz per scalar (F0 + 3 + 3s ..)
+template <typename T>
+__device__ __forceinline__ T scalar_adv(int k, T u, T v, T w, T wp, T sx, T sy, T sz, T szp, T cst) {
+  if (k == 0) return cst * (u * sx + v * sy + (T(0.5) * wp) * szp);
+  return cst * (u * sx + v * sy + (w * sz + wp * szp) / T(2));
+}
+template <typename T, int NS>
+struct PolScalar {
+  static constexpr int NF = 3 + 3 * NS, NOUT = NS, HB = 0, HA = 1;
+  __device__ static int slot(int o) { return o; }
+  T cst;
+  template <class Fn>
+  __device__ void product(int k, Fn F, T* o) const {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      o[s] = scalar_adv<T>(k, F(0, 0), F(0, 1), F(0, 2), F(1, 2), F(0, 3 + 3 * s), F(0, 4 + 3 * s), F(0, 5 + 3 * s),
+                           F(1, 5 + 3 * s), cst);
+  }
+};
+// convec and both scalars in one pass: fields u1,u2,u3, omega1..3, then
+// dtheta/dx,dy,dz and dq/dx,dy,dz.  Two real outputs share one complex
+// transform, so the small scalar advection terms are paired with each other
+// (o = cx, cy | adv theta, adv q | cz) and not with a momentum term, whose
+// rounding would swamp them in FP32; XP keeps the order cx, cy, cz, theta, q.
+template <typename T>
+struct PolConvecScalar {
+  static constexpr int NF = 12, NOUT = 5, HB = 1, HA = 1;
+  __device__ static int slot(int o) { return o < 2 ? o : (o == 4 ? 2 : o + 1); }
+  T cst;
+  template <class Fn>
+  __device__ void product(int k, Fn F, T* o) const {
+    T m[3];
+    PolConvec<T>{cst}.product(k, F, m);
+    o[0] = m[0];
+    o[1] = m[1];
+    o[4] = m[2];
+    const T u = F(0, 0), v = F(0, 1), w = F(0, 2), wp = F(1, 2);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      o[2 + s] = scalar_adv<T>(k, u, v, w, wp, F(0, 6 + 3 * s), F(0, 7 + 3 * s), F(0, 8 + 3 * s), F(1, 8 + 3 * s),
+                               cst);
+  }
+};
+
 }  // namespace les
