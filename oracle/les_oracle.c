@@ -823,12 +823,19 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
 /* ------------------------------------------------------------------ */
 static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
                           const double *v, const double *w, double *theta,
-                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss);
+                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss, int model);
 
 void or_sgs_stag(or_ctx *c, const double *const g[9], const double *u,
                  const double *v, const double *w, double *const tau[6],
                  or_sgs_state *s) {
-  sgs_stag_impl(c, g, u, v, w, NULL, tau, s, NULL);
+  sgs_stag_impl(c, g, u, v, w, NULL, tau, s, NULL, 5);
+}
+
+/* model=3: the plane-averaged scale-dependent model (sgs_stag.f90:126-137) */
+void or_sgs_stag_model3(or_ctx *c, const double *const g[9], const double *u,
+                        const double *v, const double *w, double *const tau[6],
+                        or_sgs_state *s) {
+  sgs_stag_impl(c, g, u, v, w, NULL, tau, s, NULL, 3);
 }
 
 /* model=7: also u_lag_s accumulation (sgs_stag.f90:64-66), the scalar
@@ -837,12 +844,12 @@ void or_sgs_stag(or_ctx *c, const double *const g[9], const double *u,
 void or_sgs_stag_scalar(or_ctx *c, const double *const g[9], const double *u,
                         const double *v, const double *w, double *theta,
                         double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss) {
-  sgs_stag_impl(c, g, u, v, w, theta, tau, s, ss);
+  sgs_stag_impl(c, g, u, v, w, theta, tau, s, ss, 7);
 }
 
 static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
                           const double *v, const double *w, double *theta,
-                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss) {
+                          double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss, int model) {
   const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
   const size_t pl = (size_t)ld * ny, n3 = pl * nz;
   double *S[6];
@@ -851,7 +858,8 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
   double *txx = tau[0], *txy = tau[1], *txz = tau[2], *tyy = tau[3], *tyz = tau[4], *tzz = tau[5];
   double *txzp = (double *)malloc(sizeof(double) * pl), *tyzp = (double *)malloc(sizeof(double) * pl);
   memcpy(txzp, txz, sizeof(double) * pl); memcpy(tyzp, tyz, sizeof(double) * pl);
-  for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
+  if (model != 3)                                               /* sgs_stag.f90:59-69 */
+    for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
   if (ss)
     for (size_t i = 0; i < n3; i++) { ss->u_lag_s[i] += u[i]; ss->v_lag_s[i] += v[i]; ss->w_lag_s[i] += w[i]; }
   const int jt = c->jt;
@@ -859,10 +867,21 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
   if (jt == 1 && c->P.inilag) {
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = 0.03;
   } else if ((jt >= 1 || c->P.initu) && (jt % c->P.cs_count == 0)) {
-    or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
-    if (ss) or_lagrange_scalar_sdep(c, (const double *const *)S, u, v, w, theta, s, ss);
+    if (model == 3) {
+      /* sgs_stag.f90:132 passes (ziko,u,v,w,S11,...,S33) to the 7-argument
+         routine: its S11..S33 are u, v, w, S11, S12, S13 */
+      const double *Sd[6] = { u, v, w, S[0], S[1], S[2] };
+      double *ziko = (double *)calloc(nz, sizeof(double));
+      or_scaledep_dynamic(c, Sd, u, v, w, ziko, NULL);
+      for (int k = 0; k < nz; k++)
+        for (size_t i = 0; i < pl; i++) s->Cs_opt2[pl * k + i] = ziko[k];
+      free(ziko);
+    } else {
+      or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
+      if (ss) or_lagrange_scalar_sdep(c, (const double *const *)S, u, v, w, theta, s, ss);
+    }
   }
-  if (c->jt_total < 500) {
+  if (model != 3 && c->jt_total < 500) {                        /* :165-174 (models 5, 6, 7) */
     const double cs0 = (double)(0.16f * 0.16f);
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = cs0;
     if (ss) for (size_t i = 0; i < n3; i++) ss->Pr_t[i] = c->S.Pr;
@@ -1807,4 +1826,143 @@ void or_state_init_scalar_ic(or_ctx *c, or_state *s, uint64_t seed, double T_ini
   memset(h->w_lag_s, 0, n3 * sizeof(double)); memset(h->Cs2_Pr, 0, n3 * sizeof(double));
   for (size_t i = 0; i < n3; i++) h->Pr_t[i] = S->Pr;
   c->glm_init = 0; c->gqn_init = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* rtnewt, sgsmodule.f90:37-59: Newton from x=7.5, |dx| < 1e-3, at most 100
+   iterations, else beta = 1                                            */
+/* ------------------------------------------------------------------ */
+double or_rtnewt(const double A[6]) {
+  double x = 0.5 * (0.0 + 15.0);
+  for (int j = 1; j <= 100; j++) {
+    double f = A[0] + x * (A[1] + x * (A[2] + x * (A[3] + x * (A[4] + x * A[5]))));
+    double df = A[1] + x * (2.0 * A[2] + x * (3.0 * A[3] + x * (4.0 * A[4] + x * (5.0 * A[5]))));
+    double dx = f / df;
+    x = x - dx;
+    if (fabs(dx) < 0.001) return x;
+  }
+  return 1.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* scaledep_dynamic.f90:4-255 (non-MPI).  S[0..5] are the routine's dummy
+   arguments S11, S12, S13, S22, S23, S33 in that order: the caller passes
+   what sgs_stag.f90:132 binds to them (u, v, w, S11, S12, S13).  u, v, w are
+   sim_param's.  Plane sums run over the full (ld, ny) arrays.  Outputs
+   Cs_opt2(nz) and, if beta != NULL, the per-level beta of rtnewt.     */
+/* ------------------------------------------------------------------ */
+void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                         const double *v, const double *w, double *Cs_opt2,
+                         double *beta_out) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double delta = c->delta;
+  enum { UB, VB, WB, UH, VH, WH, L11, L12, L13, L22, L23, L33, Q11, Q12, Q13, Q22, Q23, Q33,
+         SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
+         SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
+         SMAG, SBAR, SHAT, NT };
+  double *T[NT];
+  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+  const double nxy = (double)(nx * ny);
+  for (int k = 0; k < nz; k++) {
+    const size_t o = pl * k;
+    for (size_t i = 0; i < pl; i++) {                                      /* :44-73 */
+      double ub, vb, wb;
+      if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; }
+      else { ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i]; }
+      T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb;
+      T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb;
+      T[L11][i] = ub * ub; T[L12][i] = ub * vb; T[L13][i] = ub * wb;
+      T[L22][i] = vb * vb; T[L23][i] = vb * wb; T[L33][i] = wb * wb;
+    }
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[UB + q], c->Gt);        /* :78-80 */
+    const int Lp[6][3] = { { L11, UB, UB }, { L12, UB, VB }, { L13, UB, WB }, { L22, VB, VB }, { L23, VB, WB }, { L33, WB, WB } };
+    for (int q = 0; q < 6; q++) {                                           /* :81-92 */
+      or_test_filter(c, T[Lp[q][0]], c->Gt);
+      for (size_t i = 0; i < pl; i++) T[Lp[q][0]][i] = T[Lp[q][0]][i] - T[Lp[q][1]][i] * T[Lp[q][2]][i];
+    }
+    for (size_t i = 0; i < pl; i++) {                                       /* :94-99 products of the 2-Delta bars */
+      T[Q11][i] = T[UB][i] * T[UB][i]; T[Q12][i] = T[UB][i] * T[VB][i]; T[Q13][i] = T[UB][i] * T[WB][i];
+      T[Q22][i] = T[VB][i] * T[VB][i]; T[Q23][i] = T[VB][i] * T[WB][i]; T[Q33][i] = T[WB][i] * T[WB][i];
+    }
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[UH + q], c->Gtt);       /* :101-103 */
+    const int Qp[6][3] = { { Q11, UH, UH }, { Q12, UH, VH }, { Q13, UH, WH }, { Q22, VH, VH }, { Q23, VH, WH }, { Q33, WH, WH } };
+    for (int q = 0; q < 6; q++) {                                           /* :104-115 */
+      or_test_filter(c, T[Qp[q][0]], c->Gtt);
+      for (size_t i = 0; i < pl; i++) T[Qp[q][0]][i] = T[Qp[q][0]][i] - T[Qp[q][1]][i] * T[Qp[q][2]][i];
+    }
+    for (size_t i = 0; i < pl; i++) {                                       /* :117-139 */
+      double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+      double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+      double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+      T[SMAG][i] = Sm;
+      for (int q = 0; q < 6; q++) {
+        T[SB11 + q][i] = S[q][o + i]; T[SH11 + q][i] = S[q][o + i];
+        T[SSB11 + q][i] = Sm * S[q][o + i]; T[SSH11 + q][i] = Sm * S[q][o + i];
+      }
+    }
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);    /* :140-151 */
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSB11 + q], c->Gt);   /* :170-181 */
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSH11 + q], c->Gtt);
+    double sa1 = 0, sb1 = 0, sc1 = 0, sd1 = 0, se1 = 0, sa2 = 0, sb2 = 0, sc2 = 0, sd2 = 0, se2 = 0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < ld; x++) {
+        const size_t i = x + (size_t)ld * y;
+        const double sb11 = T[SB11][i], sb12 = T[SB12][i], sb13 = T[SB13][i], sb22 = T[SB22][i], sb23 = T[SB23][i], sb33 = T[SB33][i];
+        const double sh11 = T[SH11][i], sh12 = T[SH12][i], sh13 = T[SH13][i], sh22 = T[SH22][i], sh23 = T[SH23][i], sh33 = T[SH33][i];
+        const double Sb = sqrt(2.0 * (sb11 * sb11 + sb22 * sb22 + sb33 * sb33 + 2.0 * (sb12 * sb12 + sb13 * sb13 + sb23 * sb23)));
+        const double Sh = sqrt(2.0 * (sh11 * sh11 + sh22 * sh22 + sh33 * sh33 + 2.0 * (sh12 * sh12 + sh13 * sh13 + sh23 * sh23)));
+        T[SBAR][i] = Sb; T[SHAT][i] = Sh;
+        const double l11 = T[L11][i], l12 = T[L12][i], l13 = T[L13][i], l22 = T[L22][i], l23 = T[L23][i], l33 = T[L33][i];
+        const double q11 = T[Q11][i], q12 = T[Q12][i], q13 = T[Q13][i], q22 = T[Q22][i], q23 = T[Q23][i], q33 = T[Q33][i];
+        const double ssb11 = T[SSB11][i], ssb12 = T[SSB12][i], ssb13 = T[SSB13][i], ssb22 = T[SSB22][i], ssb23 = T[SSB23][i], ssb33 = T[SSB33][i];
+        const double ssh11 = T[SSH11][i], ssh12 = T[SSH12][i], ssh13 = T[SSH13][i], ssh22 = T[SSH22][i], ssh23 = T[SSH23][i], ssh33 = T[SSH33][i];
+        sa1 += Sb * (sb11 * l11 + sb22 * l22 + sb33 * l33 + 2.0 * (sb12 * l12 + sb13 * l13 + sb23 * l23));
+        sb1 += ssb11 * l11 + ssb22 * l22 + ssb33 * l33 + 2.0 * (ssb12 * l12 + ssb13 * l13 + ssb23 * l23);
+        sc1 += ssb11 * ssb11 + ssb22 * ssb22 + ssb33 * ssb33 + 2.0 * (ssb12 * ssb12 + ssb13 * ssb13 + ssb23 * ssb23);
+        sd1 += 0.5 * ((Sb * Sb) * (Sb * Sb));       /* S_bar**4 as an integer power */
+        se1 += Sb * (sb11 * ssb11 + sb22 * ssb22 + sb33 * ssb33 + 2.0 * (sb12 * ssb12 + sb13 * ssb13 + sb23 * ssb23));
+        sa2 += Sh * (sh11 * q11 + sh22 * q22 + sh33 * q33 + 2.0 * (sh12 * q12 + sh13 * q13 + sh23 * q23));
+        sb2 += ssh11 * q11 + ssh22 * q22 + ssh33 * q33 + 2.0 * (ssh12 * q12 + ssh13 * q13 + ssh23 * q23);
+        sc2 += ssh11 * ssh11 + ssh22 * ssh22 + ssh33 * ssh33 + 2.0 * (ssh12 * ssh12 + ssh13 * ssh13 + ssh23 * ssh23);
+        sd2 += 0.5 * ((Sh * Sh) * (Sh * Sh));
+        se2 += Sh * (sh11 * ssh11 + sh22 * ssh22 + sh33 * ssh33 + 2.0 * (sh12 * ssh12 + sh13 * ssh13 + sh23 * ssh23));
+      }
+    const double d2 = delta * delta, cc = (2.0 * d2) * (2.0 * d2);          /* :187-215 */
+    const double a1 = -(2.0 * d2 * 4.0 * sa1 / nxy);
+    const double b1 = -(2.0 * d2 * sb1 / nxy);
+    const double c1 = cc * sc1 / nxy;
+    const double d1 = cc * 16.0 * sd1 / nxy;
+    const double e1 = 2.0 * cc * 4.0 * se1 / nxy;
+    const double a2 = -(2.0 * d2 * 16.0 * sa2 / nxy);
+    const double b2 = -(2.0 * d2 * sb2 / nxy);
+    const double c2 = cc * sc2 / nxy;
+    const double d2c = cc * 256.0 * sd2 / nxy;
+    const double e2 = 2.0 * cc * 16.0 * se2 / nxy;
+    double A[6];                                                            /* :217-222 */
+    A[0] = b2 * c1 - b1 * c2;
+    A[1] = a1 * c2 - b2 * e1;
+    A[2] = b2 * d1 + b1 * e2 - a2 * c1;
+    A[3] = a2 * e1 - a1 * e2;
+    A[4] = -a2 * d1 - b1 * d2c;
+    A[5] = a1 * d2c;
+    const double beta = or_rtnewt(A);
+    if (beta_out) beta_out[k] = beta;
+    const double cst = 2.0 * d2;                                            /* :225-244 */
+    double num = 0.0, den = 0.0;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < ld; x++) {
+        const size_t i = x + (size_t)ld * y;
+        const double Sb = T[SBAR][i];
+        double M[6];
+        for (int q = 0; q < 6; q++) M[q] = cst * (T[SSB11 + q][i] - 4.0 * beta * Sb * T[SB11 + q][i]);
+        /* q map: 0=11 1=12 2=13 3=22 4=23 5=33 */
+        num += T[L11][i] * M[0] + T[L22][i] * M[3] + T[L33][i] * M[5] +
+               2.0 * (T[L12][i] * M[1] + T[L13][i] * M[2] + T[L23][i] * M[4]);
+        den += M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+      }
+    Cs_opt2[k] = fmax(0.0, num / den);
+  }
+  for (int t = 0; t < NT; t++) free(T[t]);
 }

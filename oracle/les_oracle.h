@@ -156,6 +156,18 @@ void or_sgs_stag_scalar(or_ctx *c, const double *const grad[9], const double *u,
                         const double *v, const double *w, double *theta,
                         double *const tau[6], or_sgs_state *s, or_scalar_sgs *g);
 
+/* ---- scaledep_dynamic.f90:4-255 + rtnewt (sgsmodule.f90:37-59), model=3.
+   S[0..5] = the dummy arguments S11..S33; sgs_stag.f90:132 binds them to
+   (u, v, w, S11, S12, S13) (argument mismatch kept).  Cs_opt2, beta: (nz). */
+double or_rtnewt(const double A[6]);
+void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                         const double *v, const double *w, double *Cs_opt2,
+                         double *beta);
+
+void or_sgs_stag_model3(or_ctx *c, const double *const grad[9], const double *u,
+                        const double *v, const double *w, double *const tau[6],
+                        or_sgs_state *s);
+
 /* ---- divstress_uv.f90, divstress_w.f90 ---- */
 void or_divstress_uv(or_ctx *c, double *divt, const double *tx, const double *ty,
                      const double *tz);

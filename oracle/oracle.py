@@ -203,6 +203,23 @@ class Oracle:
         self.L.or_sgs_stag_scalar(C.c_void_p(self.ctx), g, _p(u), _p(v), _p(w), _p(theta), t, C.byref(st),
                                   C.byref(sst))
 
+    def rtnewt(self, A):
+        a = np.ascontiguousarray(A, dtype=np.float64)
+        self.L.or_rtnewt.restype = C.c_double
+        return self.L.or_rtnewt(_p(a))
+
+    def scaledep_dynamic(self, S, u, v, w):
+        """S: the six dummy-argument fields S11..S33; returns (Cs_opt2, beta) per level."""
+        cs, beta = np.zeros(self.nz), np.zeros(self.nz)
+        s6 = (D * 6)(*[_p(a) for a in S])
+        self.L.or_scaledep_dynamic(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(cs), _p(beta))
+        return cs, beta
+
+    def sgs_stag_model3(self, grads, u, v, w, taus, st):
+        g = (D * 9)(*[_p(a) for a in grads])
+        t = (D * 6)(*[_p(a) for a in taus])
+        self.L.or_sgs_stag_model3(C.c_void_p(self.ctx), g, _p(u), _p(v), _p(w), t, C.byref(st))
+
     def interpolag_sdep(self, st):
         self.L.or_interpolag_sdep(C.c_void_p(self.ctx), C.byref(st))
 

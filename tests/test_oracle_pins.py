@@ -624,3 +624,152 @@ def test_step_velocity_divergence_free_and_finite(O):
     o.ddz_w(wz, w)
     div = _nyquist_free(o, ux + vy + wz)[1:o.nz - 1, :, :o.nx]
     assert np.max(np.abs(div)) < 1e-10 * np.max(np.abs(ux[1:o.nz - 1, :, :o.nx]))
+
+
+# ------------------------------------------------ plane-averaged model (model 3)
+def test_rtnewt_matches_numpy_roots(O):
+    """rtnewt (sgsmodule.f90:37-59): Newton from 7.5 converges to a real root of
+    the quintic (numpy.roots); without a real root it gives up at 1."""
+    o = O.Oracle(8, 8, 3)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        roots = np.sort(rng.uniform(0.2, 14.0, 5))
+        A = np.poly(roots)[::-1] * rng.uniform(0.5, 2.0)   # A[0] + A[1] x + ... + A[5] x^5
+        r = o.rtnewt(A)
+        assert np.min(np.abs(roots - r)) < 2e-3, (roots, r)
+    assert o.rtnewt([1.0, 0.0, 1.0, 0.0, 0.0, 0.0]) == 1.0    # x^2 + 1: no real root
+
+
+def _plane_model_inputs(o, seed):
+    """Band-limited u, v, w per level and the strain rates of those fields
+    (exact derivatives, no z dependence), so that L and M are correlated."""
+    rng = np.random.default_rng(seed)
+    X, Y = grid_xy(o)
+    f = lambda: o.field()
+    u, v, w = f(), f(), f()
+    d = {n: f() for n in ("ux", "uy", "vx", "vy", "wx", "wy")}
+    modes = ((1, 0, 1.0), (0, 2, 0.5), (3, 2, 0.3))
+    for k in range(o.nz):
+        for a, dx, dy in ((u, "ux", "uy"), (v, "vx", "vy"), (w, "wx", "wy")):
+            for (mx, my, amp) in modes:
+                ph = rng.uniform(0, 2 * np.pi)
+                arg = mx * X + my * Y + ph
+                a[k, :, :o.nx] += amp * np.sin(arg)
+                d[dx][k, :, :o.nx] += amp * mx * np.cos(arg)
+                d[dy][k, :, :o.nx] += amp * my * np.cos(arg)
+    S = [d["ux"], 0.5 * (d["uy"] + d["vx"]), 0.5 * d["wx"], d["vy"], 0.5 * d["wy"], f()]
+    return u, v, w, [np.ascontiguousarray(a) for a in S]
+
+
+def test_scaledep_dynamic_galilean_and_xy_symmetry(O):
+    """Cs^2 and beta of scaledep_dynamic.f90 are invariant under u -> u + c
+    (L_ij is, M_ij does not see u) and under the x <-> y exchange
+    (u<->v, S11<->S22, S13<->S23)."""
+    o = O.Oracle(16, 16, 5, dt=1e-3)
+    u, v, w, S = _plane_model_inputs(o, 4)
+    cs0, b0 = o.scaledep_dynamic(S, u, v, w)
+    # Random-phase fields carry no net forward scatter, so some levels clip to
+    # Cs^2 = 0; the others and every beta are the non-trivial part of the pin.
+    assert np.all(np.isfinite(cs0)) and np.sum(cs0 > 1e-5) >= 2
+    assert np.all(np.isfinite(b0)) and np.ptp(b0) > 0.1
+    u2, v2 = u.copy(), v.copy()
+    u2[..., :o.nx] += 2.0
+    v2[..., :o.nx] -= 1.5
+    cs1, b1 = o.scaledep_dynamic(S, u2, v2, w)
+    # beta is Newton's last iterate (stops at |dx| < 1e-3): at a level whose
+    # quintic has a near-double root, a 1e-13 change in A moves it ~1e-6.
+    assert relerr(cs1, cs0) < 1e-9 and relerr(b1, b0) < 1e-5
+    T = lambda a: np.ascontiguousarray(np.pad(np.transpose(a[..., :o.nx], (0, 2, 1)), ((0, 0), (0, 0), (0, 2))))
+    cs2, b2 = o.scaledep_dynamic([T(S[3]), T(S[1]), T(S[4]), T(S[0]), T(S[2]), T(S[5])], T(v), T(u), T(w))
+    assert relerr(cs2, cs0) < 1e-9 and relerr(b2, b0) < 1e-5
+
+
+def test_scaledep_dynamic_pure_strain_has_no_resolved_stress(O):
+    """u = v = w = 0: L = Q = 0, so every coefficient of the quintic but the
+    strain-only ones vanish, beta stays finite and Cs^2 = max(0, 0/M.M) = 0."""
+    o = O.Oracle(16, 16, 4, dt=1e-3)
+    _, _, _, S = _plane_model_inputs(o, 5)
+    z = o.field()
+    cs, beta = o.scaledep_dynamic(S, z, z, z)
+    assert np.all(cs == 0.0)
+    assert np.all(np.isfinite(beta))
+
+
+def test_sgs_stag_model3_binds_velocity_to_strain_arguments(O):
+    """sgs_stag.f90:132 calls the 7-argument routine with 10 arguments, so its
+    S11..S33 are (u, v, w, S11, S12, S13); Cs_opt2 is that per-level value on
+    every plane point, Nu_t = |S| Cs^2 Delta^2, no jt_total < 500 override."""
+    o = O.Oracle(16, 16, 5, dt=1e-3)
+    u, v, w, _ = _plane_model_inputs(o, 6)
+    grads = [0.3 * a for a in _plane_model_inputs(o, 7)[3]] + [0.2 * a for a in _plane_model_inputs(o, 8)[3][:3]]
+    st, arr = o.sgs_state()
+    taus = [o.field() for _ in range(6)]
+    o.set_time(5, 3)
+    o.sgs_stag_model3(grads, u, v, w, taus, st)
+    S = o.calc_sij(grads)
+    cs, _ = o.scaledep_dynamic([u, v, w, S[0], S[1], S[2]], u, v, w)
+    for k in range(o.nz):
+        assert np.all(arr["Cs_opt2"][k] == cs[k])
+    assert np.all(arr["u_lag"] == 0)   # model 3 keeps no Lagrangian averages
+
+
+def test_scaledep_dynamic_constant_norm_strain_closed_form(O):
+    """S11 = a cos(x+phi), S12 = a sin(x+phi), S22 = -S11 passes both test
+    filters unchanged and has the constant norm |S| = 2a, so every filtered
+    |S|S_ij is |S| S_ij.  With X1 = sum S:L, X2 = sum S:Q over the plane,
+    p_n = -2 Delta^2 |S| X_n / (nx ny) and q = (2 Delta^2)^2 |S|^4,
+    scaledep_dynamic.f90:187-222 collapse to the quintic
+      q[(p2-p1)/2 + (2p1-4p2)b + 16p1 b^2 + 64(p2-p1)b^3 - 128(p1+p2)b^4 + 512p1 b^5]
+    (so beta is rtnewt's iterate on that polynomial) and :225-244 to Cs^2 = max(0, X1 / (Delta^2 |S|^3 (1-4b) nx ny))."""
+    o = O.Oracle(16, 16, 4, dt=1e-3)
+    X, Y = grid_xy(o)
+    G = o.tables()
+    rng = np.random.default_rng(11)
+    u, v, w = o.field(), o.field(), o.field()
+    for k in range(o.nz):
+        for a in (u, v, w):
+            # pairs (5,0)-(4,0) and (3,0)-(2,0) give L and Q an x-mode-1 part
+            for (mx, my) in ((1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (0, 1), (2, 1)):
+                a[k, :, :o.nx] += rng.uniform(0.2, 1.0) * np.sin(mx * X + my * Y + rng.uniform(0, 2 * np.pi))
+    S = [o.field() for _ in range(6)]
+    amp = [0.7, 1.3, 0.4, 2.1]
+    for k in range(o.nz):
+        ph = rng.uniform(0, 2 * np.pi)
+        S[0][k, :, :o.nx] = amp[k] * np.cos(X + ph)
+        S[1][k, :, :o.nx] = amp[k] * np.sin(X + ph)
+        S[3][k, :, :o.nx] = -amp[k] * np.cos(X + ph)
+    cs, beta = o.scaledep_dynamic(S, u, v, w)
+    delta = math.sqrt(o.dx * o.dy)
+    nxy = o.nx * o.ny
+
+    def filt(p, g):
+        q = np.ascontiguousarray(p.copy())
+        o.test_filter(q, g)
+        return q
+
+    for k in range(o.nz):
+        if k == 0:
+            ub, vb, wb = u[0], v[0], 0.25 * w[1]
+        else:
+            ub, vb, wb = 0.5 * (u[k] + u[k - 1]), 0.5 * (v[k] + v[k - 1]), w[k]
+        vel = (ub, vb, wb)
+        pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+        fb = [filt(a, G["G_test"]) for a in vel]
+        fh = [filt(a, G["G_test_test"]) for a in vel]
+        L = [filt(vel[i] * vel[j], G["G_test"]) - fb[i] * fb[j] for i, j in pairs]
+        Q = [filt(fb[i] * fb[j], G["G_test_test"]) - fh[i] * fh[j] for i, j in pairs]
+        wgt = (1, 2, 2, 1, 2, 1)
+        s = 2.0 * amp[k]
+        X1 = sum(wg * np.sum(S[q][k] * L[q]) for q, wg in enumerate(wgt))
+        X2 = sum(wg * np.sum(S[q][k] * Q[q]) for q, wg in enumerate(wgt))
+        p1 = -2 * delta ** 2 * s * X1 / nxy
+        p2 = -2 * delta ** 2 * s * X2 / nxy
+        coef = [(p2 - p1) / 2, 2 * p1 - 4 * p2, 16 * p1, 64 * (p2 - p1), -128 * (p1 + p2), 512 * p1]
+        # rtnewt is pinned against numpy's roots above; Newton is invariant
+        # under scaling of the coefficients, so the same iterate must come out
+        # of the closed-form quintic (beta = 1/4 is always one of its roots).
+        assert abs(sum(cf * 0.25 ** n for n, cf in enumerate(coef))) < 1e-9 * max(abs(cf) for cf in coef)
+        assert abs(o.rtnewt(coef) - beta[k]) < 1e-6, (k, beta[k], o.rtnewt(coef))
+        want = max(0.0, X1 / (delta ** 2 * s ** 3 * (1 - 4 * beta[k]) * nxy))
+        assert abs(cs[k] - want) <= 1e-10 * max(abs(want), 1e-300), (k, cs[k], want)
+    assert np.sum(cs > 0) >= 1
