@@ -101,6 +101,20 @@ int les_lagrange_sdep(les_ctx *c, double *const S[6], const double *u, const dou
                       const les_sgs_fields *s);
 int les_interpolag_sdep(les_ctx *c, const les_sgs_fields *s);
 
+/* ---- SGS model selection, param.f90 'model' (sgs_stag.f90:128-160):
+   5 = Lagrangian scale-dependent (default), 3 = scale-dependent dynamic with
+   plane-averaged coefficients.  Model 3 keeps no Lagrangian averages, fills
+   Cs_opt2 level by level and has no jt_total < 500 constant-Cs period; it
+   has no scalar SGS model, so it cannot be combined with les_set_scalars. ---- */
+int les_set_sgs_model(les_ctx *c, int model);
+/* ---- scaledep_dynamic.f90:4-255 with rtnewt (sgsmodule.f90:37-59).
+   S: the routine's dummy arguments S11, S12, S13, S22, S23, S33 (ld,ny,nz);
+   sgs_stag.f90:132 binds them to u, v, w, S11, S12, S13.  u, v, w are
+   sim_param's.  Cs_opt2: (nz) device doubles; beta: (nz) device doubles or
+   NULL (the routine's per-level root, which it writes to output/beta). ---- */
+int les_scaledep_dynamic(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                         double *Cs_opt2, double *beta);
+
 /* ---- scalars: potential temperature theta and humidity q (S_FLAG) ----
    Replaces the scalar parameters of param.f90:53,72-90,139-146,174 and the
    surface set-up of bottombc.f90:26-63 (lbc=1, patch_flag=1; z_os = z0/7.39).

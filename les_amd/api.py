@@ -181,6 +181,19 @@ class Les:
         st = self._sgs_struct(sgs)
         self._chk(self.L.les_interpolag_sdep(self.ctx, C.byref(st)))
 
+    def set_sgs_model(self, model):
+        """param.f90 'model': 5 (Lagrangian scale-dependent) or 3 (plane-averaged)."""
+        self._chk(self.L.les_set_sgs_model(self.ctx, int(model)))
+
+    def scaledep_dynamic(self, S, u, v, w):
+        """scaledep_dynamic.f90 on the dummies S (S11..S33 in that order):
+        returns the per-level Cs_opt2 and beta as (nz,) float64 GPU tensors."""
+        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        cs = torch.zeros(self.nz, dtype=torch.float64, device=u.device)
+        beta = torch.zeros(self.nz, dtype=torch.float64, device=u.device)
+        self._chk(self.L.les_scaledep_dynamic(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(cs), _ptr(beta)))
+        return cs, beta
+
     # ---- scalars (scalars_module.f90, scalars_module_q.f90, lagrange_scalar_Sdep.f90) ----
     def set_scalars(self, enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_star=1.0,
                     T_scale=400.0, q_scale=100.0, g=9.81, dTdz_top=0.01, dqdz_top=-0.004, wt_s=None, wq_s=None):
@@ -272,13 +285,16 @@ class Les:
 class LesSim(Les):
     """Whole-step driver on context-owned device state (main.f90:202-415)."""
 
-    def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, **kw):
+    def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, sgs_model=5, **kw):
         """scalars: None (neutral) or a dict of set_scalars arguments (the
-        convective step with theta, q, buoyancy and the scalar SGS model)."""
+        convective step with theta, q, buoyancy and the scalar SGS model).
+        sgs_model: 5 (Lagrangian scale-dependent) or 3 (plane-averaged)."""
         # a stream of its own (default priority): graph capture needs a
         # non-default stream, and the state is only touched through the library
         self.stream = torch.cuda.Stream()
         super().__init__(nx, ny, nz, stream=self.stream, **kw)
+        if sgs_model != 5:
+            self.set_sgs_model(sgs_model)
         if scalars is not None:
             self.set_scalars(**scalars)
         self.precision = precision

@@ -594,6 +594,27 @@ int les_lagrange_sdep(les_ctx* c, double* const S[6], const double* u, const dou
   op_lagrange<double>(c, Sc, u, v, w, to_ptrs(s));
   DONE(c);
 }
+int les_set_sgs_model(les_ctx* c, int model) {
+  GUARD(c);
+  if (model != 3 && model != 5) {
+    c->err = "les_set_sgs_model: model must be 3 (plane-averaged scale-dependent) or 5 (Lagrangian)";
+    return 1;
+  }
+  if (model == 3 && c->scal.enable) {
+    c->err = "les_set_sgs_model: model 3 has no scalar SGS model (sgs_stag.f90:150-153 runs under model 5 only)";
+    return 1;
+  }
+  c->sgs_model = model;
+  DONE(c);
+}
+int les_scaledep_dynamic(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                         double* Cs_opt2, double* beta) {
+  GUARD(c);
+  if (!S || !Cs_opt2) return -1;
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_scaledep_dynamic<double>(c, Sc, u, v, w, Cs_opt2, beta, (double*)nullptr);
+  DONE(c);
+}
 int les_interpolag_sdep(les_ctx* c, const les_sgs_fields* s) {
   GUARD(c);
   if (!s) return -1;
@@ -1009,6 +1030,10 @@ int les_profile_report(les_ctx* c, char* buf, int buflen) {
 int les_set_scalars(les_ctx* c, const les_scalar_params* p, const double* wt_s, const double* wq_s) {
   GUARD(c);
   if (!p) return -1;
+  if (p->enable && c->sgs_model == 3) {
+    c->err = "les_set_scalars: SGS model 3 has no scalar SGS model; select model 5 first";
+    return 1;
+  }
   ScalarParams& P = c->scal;
   P.enable = p->enable; P.model = p->model; P.initsc = p->initsc; P.passive = p->passive;
   P.Pr = p->Pr; P.z_i = p->z_i; P.u_star = p->u_star; P.T_scale = p->T_scale; P.q_scale = p->q_scale;
@@ -1113,6 +1138,7 @@ int les_sgs_stag_scalar(les_ctx* c, double* const grad[9], const double* u, cons
   GUARD(c);
   if (!m || !s) return -1;
   if (!c->wt_s) { c->err = "les_sgs_stag_scalar: call les_set_scalars first"; return 1; }
+  if (c->sgs_model == 3) { c->err = "les_sgs_stag_scalar: the scalar SGS model needs SGS model 5"; return 1; }
   const double* g[9];
   for (int i = 0; i < 9; ++i) g[i] = grad[i];
   const les::ScalarSgsPtrs ss = to_sptrs(s);

@@ -97,6 +97,7 @@ struct les_ctx {
   double zo_avg = 0.0;             // exp(mean log zo), lagrange_scalar_Sdep.f90:110
   double wt_mean = 0.0;            // sum(wt_s)/(nx*ny), lagrange_scalar_Sdep.f90:114
   int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
+  int sgs_model = 5;               // param.f90 'model': 5 Lagrangian scale-dependent, 3 plane-averaged
   // host-address -> device mirror registry (les_mirror)
   std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };
@@ -107,7 +108,7 @@ namespace les {
 enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
-  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_NSLOTS
+  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 
@@ -160,6 +161,14 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
 template <typename T>
 void op_sgs_stag(les_ctx* c, const T* const g[9], const T* u, const T* v, const T* w, T* const tau[6],
                  const SgsPtrs& s, T* theta = nullptr, const ScalarSgsPtrs* ss = nullptr);
+
+// SGS model 3 (les_dynamic.hip): scaledep_dynamic.f90:4-255 with rtnewt.
+// S: the routine's dummies S11..S33; per-level Cs_opt2 and beta go to
+// cs_lev/beta_lev (device doubles, nz; null: internal scratch); with
+// Cs_planes every point of level k receives Cs_opt2(k) (sgs_stag.f90:133-135)
+template <typename T>
+void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                         double* beta_lev, T* Cs_planes);
 
 // ---- scalars (les_scalars.hip) ----
 // obukhov, scalars_module.f90:508-690.  sgs_t3/sgs_q3: level-1 planes (ld,ny)

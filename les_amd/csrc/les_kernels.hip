@@ -788,13 +788,14 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   OpTag _tag(c, "sgs_stag");
   const Grid& g = c->g;
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  const bool m3 = c->sgs_model == 3;  // no Lagrangian averages, plane-averaged Cs (sgs_stag.f90:128-140)
   T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
   T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
   {
     ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + (ss ? 6 : 3)) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_calc_sij<T>), grid1(n3), dim3(256), 0, c->stream, gr[0], gr[1], gr[2], gr[3], gr[4], gr[5],
                        gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx, g.ld, g.ny, g.nz, u, v, w,
-                       (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
+                       m3 ? (T*)nullptr : (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
                        ss ? (T*)ss->v_lag_s : (T*)nullptr, ss ? (T*)ss->w_lag_s : (T*)nullptr);
   }
   LCHK(c, hipGetLastError());
@@ -802,11 +803,18 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   if (jt == 1 && g.inilag) {
     op_fill<T>(c, (T*)s.Cs_opt2, n3, T(0.03));
   } else if ((jt >= 1 || g.initu) && (jt % g.cs_count == 0)) {
-    const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
-    op_lagrange<T>(c, Sc, u, v, w, s);
-    if (ss) op_lagrange_scalar<T>(c, Sc, u, v, w, theta, s, *ss, true);  // sgs_stag.f90:150-153
+    if (m3) {
+      // sgs_stag.f90:132 passes ten arguments to the seven-argument routine,
+      // so its S11..S33 are bound to u, v, w, S11, S12, S13
+      const T* Sd[6] = {u, v, w, S[0], S[1], S[2]};
+      op_scaledep_dynamic<T>(c, Sd, u, v, w, nullptr, nullptr, (T*)s.Cs_opt2);
+    } else {
+      const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+      op_lagrange<T>(c, Sc, u, v, w, s);
+      if (ss) op_lagrange_scalar<T>(c, Sc, u, v, w, theta, s, *ss, true);  // sgs_stag.f90:150-153
+    }
   }
-  const int use_const = c->jt_total < 500;
+  const int use_const = !m3 && c->jt_total < 500;  // :165-174 applies to models 5-7
   const T cs0 = (T)(double)(0.16f * 0.16f);
   const T l2 = (T)(g.delta * g.delta);
   {

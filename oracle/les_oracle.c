@@ -142,6 +142,7 @@ struct or_ctx {
   plan2 sm, bg;
   int flm_init, fqn_init;   /* SAVE flags, lagrange_Sdep.f90:67-68 */
   int jt, jt_total;         /* mirrors param jt/jt_total for routine calls */
+  int sgs_model;            /* param.f90 'model' of the whole step: 5 (default) or 3 */
   or_scalar_params S;       /* scalar switches and constants (zero: no scalars) */
   double *wt_s, *wq_s;      /* (nx,ny) kinematic surface fluxes */
   int glm_init, gqn_init;   /* SAVE flags, lagrange_scalar_Sdep.f90:65-66 */
@@ -1316,6 +1317,8 @@ void or_step(or_ctx *c, or_state *s) {
   or_wallstress(c, s->u, s->v, g[2], g[5], s->tau[2], s->tau[4], s->psi_m, s->psi_m0, s->phi_m, s->ustar_avg);
   if (scal && c->S.model == 7)
     or_sgs_stag_scalar(c, (const double *const *)g, s->u, s->v, s->w, s->theta, s->tau, &s->sgs, &s->ssgs);
+  else if (c->sgs_model == 3)
+    or_sgs_stag_model3(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
   else
     or_sgs_stag(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
   if (scal) {                          /* main.f90:273-276 */
@@ -1365,6 +1368,7 @@ void or_step(or_ctx *c, or_state *s) {
 }
 
 void or_set_time(or_ctx *c, int jt, int jt_total) { c->jt = jt; c->jt_total = jt_total; }
+void or_set_sgs_model(or_ctx *c, int model) { c->sgs_model = model; }
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init) { c->flm_init = flm_init; c->fqn_init = fqn_init; }
 
 /* ================================================================== */

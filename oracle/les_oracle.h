@@ -226,6 +226,8 @@ void or_state_init_scalar_ic(or_ctx *c, or_state *s, uint64_t seed, double T_ini
 /* jt / jt_total seen by routine-level calls (param.f90:147-148) and the
    lagrange_Sdep SAVE init flags (lagrange_Sdep.f90:67-68) */
 void or_set_time(or_ctx *c, int jt, int jt_total);
+/* SGS model of or_step: 5 (Lagrangian scale-dependent, default) or 3 */
+void or_set_sgs_model(or_ctx *c, int model);
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init);
 
 /* counter-based uniform in [0,1): splitmix64 of (seed, index) */

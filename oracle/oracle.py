@@ -142,6 +142,9 @@ class Oracle:
         self.L.or_get_tables(C.c_void_p(self.ctx), *[_p(o) for o in out])
         return dict(zip(("kx", "ky", "k2", "G_test", "G_test_test"), out))
 
+    def set_sgs_model(self, model):
+        self.L.or_set_sgs_model(C.c_void_p(self.ctx), int(model))
+
     def set_time(self, jt, jt_total, flm_init=None, fqn_init=None):
         """jt/jt_total as the routine-level calls see them (param.f90:147-148)."""
         self.L.or_set_time(C.c_void_p(self.ctx), jt, jt_total)
