@@ -19,6 +19,12 @@ module les_amd
     integer(c_int) :: inilag, initu, cs_count
   end type les_grid
 
+  ! les_scalar_params: param.f90:53,72-90,139-146,174 and bottombc.f90:26-63
+  type, bind(C) :: les_scalar_params
+    integer(c_int) :: enable, model, initsc, passive
+    real(c_double) :: Pr, z_i, u_star, T_scale, q_scale, g, dTdz_top, dqdz_top
+  end type les_scalar_params
+
   interface
     integer(c_int) function les_create(g, ctx) bind(C, name='les_create')
       import :: c_int, c_ptr, les_grid
@@ -160,6 +166,43 @@ module les_amd
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function les_set_graph(ctx, enable) bind(C, name='les_set_graph')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+    ! --- SGS model selection (param.f90 'model': 5 or 3) and scaledep_dynamic.f90 ---
+    integer(c_int) function les_set_sgs_model(ctx, model) bind(C, name='les_set_sgs_model')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: model
+    end function
+    integer(c_int) function les_scaledep_dynamic(ctx, S, u, v, w, Cs_opt2, beta) &
+         bind(C, name='les_scaledep_dynamic')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(in) :: S(6)
+      type(c_ptr), value :: u, v, w, Cs_opt2, beta
+    end function
+    ! --- scalars: theta and q (S_FLAG), convective surface fluxes ---
+    integer(c_int) function les_set_scalars(ctx, p, wt_s, wq_s) bind(C, name='les_set_scalars')
+      import :: c_int, c_ptr, les_scalar_params
+      type(c_ptr), value :: ctx
+      type(les_scalar_params), intent(in) :: p
+      type(c_ptr), value :: wt_s, wq_s
+    end function
+    integer(c_int) function les_state_init_scalar_ic(ctx, seed, T_init, z_inv, inv_strength, amp_K, q_init) &
+         bind(C, name='les_state_init_scalar_ic')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: T_init, z_inv, inv_strength, amp_K, q_init
+    end function
+    integer(c_int) function les_scalar_ref(ctx, theta_ref, q_ref) bind(C, name='les_scalar_ref')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: theta_ref, q_ref
     end function
   end interface
 
