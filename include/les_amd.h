@@ -256,6 +256,45 @@ int les_mirror_pull(les_ctx *c, void *host, uint64_t bytes);
 int les_profile(les_ctx *c, int enable);
 int les_profile_report(les_ctx *c, char *buf, int buflen);
 
+/* ---- learned SGS closure (no counterpart in the reference code).  An MLP
+   8 -> 64 -> 64 -> 64 -> 2 (SiLU) maps local invariants of the strain rate to
+   the (Cs_opt2, Beta) that lagrange_Sdep.f90:390-421 produces, with BF16
+   MFMA / FP32-accumulate kernels for inference and training.  Inputs per
+   point (levels 1..nz-1 of lagrange_Sdep's w-nodes): asinh of |S|, |S_bar|,
+   |S_hat|, S:S_bar, S_bar:(|S|S)_bar, S_hat:(|S|S)_hat and the rotation
+   magnitude |Omega|, then z/Delta.  Arrays are device pointers; X (P,8) raw
+   features, Y/T (P,2) raw (Cs_opt2, Beta); the model normalises internally
+   with les_mlp_set_norm / les_mlp_fit_norm.  theta: flat parameters W1 (64x8)
+   b1 W2 (64x64) b2 W3 b3 W4 (2x64) b4, row-major [out][in], host floats. ---- */
+int les_mlp_nparam(void);
+int les_mlp_init(les_ctx *c, uint64_t seed);
+int les_mlp_set_params(les_ctx *c, const float *theta);
+int les_mlp_get_params(les_ctx *c, float *theta);
+int les_mlp_set_norm(les_ctx *c, const float *mu, const float *sd, const float *ymu, const float *ysd);
+int les_mlp_get_norm(les_ctx *c, float *mu, float *sd, float *ymu, float *ysd);
+int les_mlp_fit_norm(les_ctx *c, const float *X, const float *Y, int64_t P);
+int les_mlp_forward(les_ctx *c, const float *X, int64_t P, float *Y);
+/* gradient (device, les_mlp_nparam() floats) of L = 1/(2P) sum |y_n - t_n|^2
+   in normalised units over all P samples, and L (device float) */
+int les_mlp_grad(les_ctx *c, const float *X, const float *T, int64_t P, float *grad, float *loss);
+/* `steps` Adam steps on batches of `batch` samples drawn from the P;
+   loss_hist: device floats (steps) or NULL */
+int les_mlp_train(les_ctx *c, const float *X, const float *T, int64_t P, int batch, int steps, float lr,
+                  uint64_t seed, float *loss_hist);
+/* features and labels of the state's last step, which must have run the
+   Lagrangian update (jt a multiple of the update period); X: (nx*ny*(nz-1), 8),
+   Y: (nx*ny*(nz-1), 2) or NULL; point order x fastest, then y, then level */
+int les_mlp_harvest(les_ctx *c, float *X, float *Y);
+/* mode 0: the dynamic procedure every cs_count steps (the reference).  mode 1:
+   lagrange_Sdep every `period` steps (a multiple of cs_count, which is then its
+   averaging interval), the network at the other multiples of cs_count.  Each
+   dynamic update also judges the network: relative L2 error of its Cs_opt2
+   against the dynamic one; above `threshold` the network's output is not
+   used (the dynamic coefficients are kept) until a later update passes. */
+int les_set_closure(les_ctx *c, int mode, int period, double threshold);
+/* out[4]: sum (net - dyn)^2, sum dyn^2, relative error, 1 if the network is in use */
+int les_closure_judge(les_ctx *c, double *out);
+
 #ifdef __cplusplus
 }
 #endif

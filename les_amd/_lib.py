@@ -74,6 +74,13 @@ def load():
     L.les_set_scalars.argtypes = [vp, C.POINTER(ScalarParams), vp, vp]
     L.les_state_init_scalar_ic.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double]
+    i64, u64, fp = C.c_int64, C.c_uint64, C.c_void_p
+    L.les_mlp_init.argtypes = [vp, u64]
+    L.les_mlp_fit_norm.argtypes = [vp, fp, fp, i64]
+    L.les_mlp_forward.argtypes = [vp, fp, i64, fp]
+    L.les_mlp_grad.argtypes = [vp, fp, fp, i64, fp, fp]
+    L.les_mlp_train.argtypes = [vp, fp, fp, i64, C.c_int, C.c_int, C.c_float, u64, fp]
+    L.les_set_closure.argtypes = [vp, C.c_int, C.c_int, C.c_double]
     for name in declared_symbols():
         if name not in ("les_destroy", "les_last_error", "les_state_field"):
             getattr(L, name).restype = C.c_int

@@ -194,6 +194,69 @@ class Les:
         self._chk(self.L.les_scaledep_dynamic(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(cs), _ptr(beta)))
         return cs, beta
 
+    # ---- learned closure (no counterpart in the reference code) ----
+    MLP_NIN, MLP_NOUT = 8, 2
+
+    def mlp_init(self, seed=0):
+        self._chk(self.L.les_mlp_init(self.ctx, int(seed)))
+        self.mlp_np = self.L.les_mlp_nparam()
+
+    def mlp_params(self):
+        import numpy as np
+        th = np.zeros(self.L.les_mlp_nparam(), dtype=np.float32)
+        self._chk(self.L.les_mlp_get_params(self.ctx, th.ctypes.data_as(C.c_void_p)))
+        return th
+
+    def mlp_set_params(self, theta):
+        import numpy as np
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.size == self.L.les_mlp_nparam()
+        self._chk(self.L.les_mlp_set_params(self.ctx, th.ctypes.data_as(C.c_void_p)))
+
+    def mlp_set_norm(self, mu, sd, ymu, ysd):
+        import numpy as np
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (mu, sd, ymu, ysd)]
+        self._chk(self.L.les_mlp_set_norm(self.ctx, *[x.ctypes.data_as(C.c_void_p) for x in a]))
+
+    def mlp_norm(self):
+        import numpy as np
+        a = [np.zeros(n, dtype=np.float32) for n in (8, 8, 2, 2)]
+        self._chk(self.L.les_mlp_get_norm(self.ctx, *[x.ctypes.data_as(C.c_void_p) for x in a]))
+        return a
+
+    def mlp_fit_norm(self, X, Y):
+        self._chk(self.L.les_mlp_fit_norm(self.ctx, _ptr(X), _ptr(Y), X.shape[0]))
+
+    def mlp_forward(self, X):
+        """(P, 8) float32 raw features on the GPU -> (P, 2) (Cs_opt2, Beta)."""
+        Y = torch.empty((X.shape[0], 2), dtype=torch.float32, device=X.device)
+        self._chk(self.L.les_mlp_forward(self.ctx, _ptr(X), X.shape[0], _ptr(Y)))
+        self.sync()
+        return Y
+
+    def mlp_grad(self, X, T):
+        g = torch.empty(self.L.les_mlp_nparam(), dtype=torch.float32, device=X.device)
+        loss = torch.empty(1, dtype=torch.float32, device=X.device)
+        self._chk(self.L.les_mlp_grad(self.ctx, _ptr(X), _ptr(T), X.shape[0], _ptr(g), _ptr(loss)))
+        self.sync()
+        return float(loss.item()), g
+
+    def mlp_train(self, X, T, batch=4096, steps=1000, lr=1e-3, seed=0):
+        hist = torch.empty(max(steps, 1), dtype=torch.float32, device=X.device)
+        self._chk(self.L.les_mlp_train(self.ctx, _ptr(X), _ptr(T), X.shape[0], int(batch), int(steps),
+                                       float(lr), int(seed), _ptr(hist)))
+        self.sync()
+        return hist[:steps]
+
+    def set_closure(self, mode, period=10, threshold=0.2):
+        self._chk(self.L.les_set_closure(self.ctx, int(mode), int(period), float(threshold)))
+
+    def closure_judge(self):
+        import numpy as np
+        out = np.zeros(4)
+        self._chk(self.L.les_closure_judge(self.ctx, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     # ---- scalars (scalars_module.f90, scalars_module_q.f90, lagrange_scalar_Sdep.f90) ----
     def set_scalars(self, enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_star=1.0,
                     T_scale=400.0, q_scale=100.0, g=9.81, dTdz_top=0.01, dqdz_top=-0.004, wt_s=None, wq_s=None):
@@ -303,6 +366,16 @@ class LesSim(Les):
 
     def set_graph(self, enable):
         self._chk(self.L.les_set_graph(self.ctx, 1 if enable else 0))
+
+    def mlp_harvest(self, labels=True):
+        """Features (P, 8) and labels (P, 2) of the last step, which must have
+        run the Lagrangian update; P = nx*ny*(nz-1)."""
+        P = self.nx * self.ny * (self.nz - 1)
+        X = torch.empty((P, 8), dtype=torch.float32, device="cuda")
+        Y = torch.empty((P, 2), dtype=torch.float32, device="cuda") if labels else None
+        self._chk(self.L.les_mlp_harvest(self.ctx, _ptr(X), _ptr(Y)))
+        self.sync()
+        return X, Y
 
     def graph_prepare(self):
         self._chk(self.L.les_graph_prepare(self.ctx))

@@ -383,6 +383,7 @@ void les_destroy(les_ctx* c) {
   }
   delete[] c->G2_host;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  les::mlp_free(c);
   if (c->wt_s) (void)hipFree(c->wt_s);
   if (c->wq_s) (void)hipFree(c->wq_s);
   delete c;
@@ -828,7 +829,8 @@ static void step_eager(les_ctx* c) {
 // repeats exactly while none of the one-off conditions occurs inside it; such
 // a window is captured once into a HIP graph and replayed.
 static int graph_window(const les_ctx* c) {
-  const int cs = c->g.cs_count;
+  int cs = c->g.cs_count;
+  if (c->closure_mode && c->closure_period > cs) cs = c->closure_period;  // a multiple of cs_count
   return cs % 2 == 0 ? cs : 2 * cs;
 }
 static bool window_is_regular(const les_ctx* c, int W, int* use_const) {
@@ -836,7 +838,7 @@ static bool window_is_regular(const les_ctx* c, int W, int* use_const) {
   if (c->jt < 1 && !g.initu) return false;                      // Euler first step inside
   const bool lag_init = g.inilag && (!c->flm_init || !c->fqn_init ||
                                      (c->st->scal && c->scal.model == 7 && (!c->glm_init || !c->gqn_init)));
-  if (lag_init && c->jt < g.cs_count) return false;             // initialisation inside
+  if (lag_init && c->jt < lag_count(c)) return false;           // initialisation inside
   const int first = c->jt_total + 1, last = c->jt_total + W;
   if (first < 500 && last >= 500) return false;                // constant-Cs switch inside
   // every routine must have run eagerly once, so scratch is at full size
@@ -877,6 +879,134 @@ static void graph_capture(les_ctx* c, int W, int uc) {
     c->g_swap = swap0;
   }
   if (graph) (void)hipGraphDestroy(graph);
+}
+
+// ---- learned closure ----
+int les_mlp_nparam(void) { return les::mlp_nparam(); }
+#define NEED_MLP(c)                                            \
+  if (!(c)->mlp) {                                             \
+    (c)->err = "learned closure: call les_mlp_init first";     \
+    return 1;                                                  \
+  }
+int les_mlp_init(les_ctx* c, uint64_t seed) {
+  GUARD(c);
+  les::mlp_init(c, seed);
+  DONE(c);
+}
+int les_mlp_set_params(les_ctx* c, const float* theta) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!theta) return -1;
+  les::mlp_set_params(c, theta);
+  DONE(c);
+}
+int les_mlp_get_params(les_ctx* c, float* theta) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!theta) return -1;
+  les::mlp_get_params(c, theta);
+  DONE(c);
+}
+int les_mlp_set_norm(les_ctx* c, const float* mu, const float* sd, const float* ymu, const float* ysd) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!mu || !sd || !ymu || !ysd) return -1;
+  for (int i = 0; i < 8; ++i)
+    if (!(sd[i] > 0.0f)) { c->err = "les_mlp_set_norm: sd must be positive"; return 1; }
+  for (int i = 0; i < 2; ++i)
+    if (!(ysd[i] > 0.0f)) { c->err = "les_mlp_set_norm: ysd must be positive"; return 1; }
+  les::mlp_set_norm(c, mu, sd, ymu, ysd);
+  DONE(c);
+}
+int les_mlp_get_norm(les_ctx* c, float* mu, float* sd, float* ymu, float* ysd) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!mu || !sd || !ymu || !ysd) return -1;
+  les::mlp_get_norm(c, mu, sd, ymu, ysd);
+  DONE(c);
+}
+int les_mlp_fit_norm(les_ctx* c, const float* X, const float* Y, int64_t P) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!X || !Y || P < 2) return -1;
+  les::mlp_fit_norm(c, X, Y, (long)P);
+  DONE(c);
+}
+int les_mlp_forward(les_ctx* c, const float* X, int64_t P, float* Y) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!X || !Y || P < 0) return -1;
+  if (P > 0) les::op_mlp_forward(c, X, (long)P, Y);
+  DONE(c);
+}
+int les_mlp_grad(les_ctx* c, const float* X, const float* T, int64_t P, float* grad, float* loss) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!X || !T || P < 1) return -1;
+  les::op_mlp_grad(c, X, T, (long)P, grad, loss);
+  DONE(c);
+}
+int les_mlp_train(les_ctx* c, const float* X, const float* T, int64_t P, int batch, int steps, float lr,
+                  uint64_t seed, float* loss_hist) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!X || !T || P < 1 || batch < 1 || steps < 0 || !(lr > 0.0f)) return -1;
+  les::op_mlp_train(c, X, T, (long)P, batch, steps, lr, seed, loss_hist);
+  DONE(c);
+}
+extern "C++" {
+template <typename T>
+static void harvest_t(les_ctx* c, float* X, float* Y) {
+  const Grid& g = c->g;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  const T* Fs = (const T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
+  const T* Sb = (const T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
+  const T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
+  const T* gr[9] = {F<T>(c, "dudx"), F<T>(c, "dudy"), F<T>(c, "dudz"), F<T>(c, "dvdx"), F<T>(c, "dvdy"),
+                    F<T>(c, "dvdz"), F<T>(c, "dwdx"), F<T>(c, "dwdy"), F<T>(c, "dwdz")};
+  op_mlp_harvest<T>(c, Fs, S, gr, F<T>(c, "Cs_opt2"), F<T>(c, "Beta"), X, Y);
+}
+}  // extern "C++"
+int les_mlp_harvest(les_ctx* c, float* X, float* Y) {
+  GUARD(c);
+  if (!X) return -1;
+  if (!c->st) { c->err = "les_mlp_harvest: no state"; return 1; }
+  if (c->sgs_model != 5 || c->last_lag_jt != c->jt || c->jt < 1) {
+    c->err = "les_mlp_harvest: the last step was not a Lagrangian dynamic update (lagrange_Sdep)";
+    return 1;
+  }
+  if (c->st->prec) harvest_t<float>(c, X, Y);
+  else harvest_t<double>(c, X, Y);
+  DONE(c);
+}
+int les_set_closure(les_ctx* c, int mode, int period, double threshold) {
+  GUARD(c);
+  if (mode != 0 && mode != 1) { c->err = "les_set_closure: mode must be 0 (dynamic) or 1 (network)"; return 1; }
+  if (mode == 1) {
+    NEED_MLP(c);
+    if (c->sgs_model != 5) { c->err = "les_set_closure: the network replaces the model-5 update"; return 1; }
+    if (period < c->g.cs_count || period % c->g.cs_count != 0) {
+      c->err = "les_set_closure: period must be a positive multiple of cs_count";
+      return 1;
+    }
+    if (!(threshold > 0.0)) { c->err = "les_set_closure: threshold must be positive"; return 1; }
+  }
+  c->closure_mode = mode;
+  c->closure_period = mode ? period : c->g.cs_count;
+  c->closure_thresh = threshold;
+  c->lag_cs = mode ? period : 0;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  DONE(c);
+}
+int les_closure_judge(les_ctx* c, double* out) {
+  GUARD(c);
+  NEED_MLP(c);
+  if (!out) return -1;
+  les::mlp_judge_read(c, out);
+  DONE(c);
 }
 
 int les_set_graph(les_ctx* c, int enable) {

@@ -98,6 +98,12 @@ struct les_ctx {
   double wt_mean = 0.0;            // sum(wt_s)/(nx*ny), lagrange_scalar_Sdep.f90:114
   int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
   int sgs_model = 5;               // param.f90 'model': 5 Lagrangian scale-dependent, 3 plane-averaged
+  int lag_cs = 0;                  // steps between Lagrangian updates when it differs from cs_count (closure)
+  int last_lag_jt = -1;            // jt of the last lagrange_Sdep update
+  // learned closure (les_mlp.hip): the network between dynamic updates
+  void* mlp = nullptr;             // les::MlpModel
+  int closure_mode = 0, closure_period = 10;
+  double closure_thresh = 0.2;
   // host-address -> device mirror registry (les_mirror)
   std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };
@@ -108,7 +114,7 @@ namespace les {
 enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
-  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_NSLOTS
+  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 
@@ -154,7 +160,12 @@ struct ScalarSgsPtrs {
 template <typename T> void op_calc_sij(les_ctx* c, const T* const g[9], T* const S[6]);
 template <typename T> void op_interpolag(les_ctx* c, const SgsPtrs& s);
 template <typename T> void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl);
-template <typename T> T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w);
+// fields f0..20 of the lagrange_Sdep filter set (f0 = 9: strain rates only)
+template <typename T>
+T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, int f0 = 0);
+// steps between Lagrangian updates: the averaging count of u_lag and the
+// Lagrangian time step (interpolag_Sdep.f90:228-262, lagrange_Sdep.f90:321-331)
+inline int lag_count(const les_ctx* c) { return c->lag_cs > 0 ? c->lag_cs : c->g.cs_count; }
 template <typename T>
 void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s);
 // theta/ss may be null (model 5); with ss the model-7 scalar model runs after lagrange_Sdep
@@ -169,6 +180,27 @@ void op_sgs_stag(les_ctx* c, const T* const g[9], const T* u, const T* v, const 
 template <typename T>
 void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
                          double* beta_lev, T* Cs_planes);
+
+// ---- learned closure (les_mlp.hip) ----
+int mlp_nparam();
+void mlp_init(les_ctx* c, uint64_t seed);
+void mlp_free(les_ctx* c);
+void mlp_set_params(les_ctx* c, const float* host);
+void mlp_get_params(les_ctx* c, float* host);
+void mlp_set_norm(les_ctx* c, const float* mu, const float* sd, const float* ymu, const float* ysd);
+void mlp_get_norm(les_ctx* c, float* mu, float* sd, float* ymu, float* ysd);
+void mlp_fit_norm(les_ctx* c, const float* X, const float* Y, long P);
+void mlp_judge_read(les_ctx* c, double out[4]);
+void op_mlp_forward(les_ctx* c, const float* X, long P, float* Y);
+void op_mlp_grad(les_ctx* c, const float* X, const float* T, long P, float* grad, float* loss);
+void op_mlp_train(les_ctx* c, const float* X, const float* T, long P, int batch, int steps, float lr,
+                  uint64_t seed, float* loss_hist);
+template <typename T>
+void op_mlp_harvest(les_ctx* c, const T* F, const T* const S[6], const T* const gr[9], const T* Cs, const T* Beta,
+                    float* X, float* Y);
+template <typename T>
+void op_mlp_closure(les_ctx* c, const T* F, const T* const S[6], const T* const gr[9], T* Cs, T* Beta, int judge,
+                    double thresh);
 
 // ---- scalars (les_scalars.hip) ----
 // obukhov, scalars_module.f90:508-690.  sgs_t3/sgs_q3: level-1 planes (ld,ny)

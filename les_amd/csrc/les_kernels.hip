@@ -571,7 +571,7 @@ void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl) {
     ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
                        (const T*)ul, (const T*)vl, (const T*)wl, FF, F[0], F[1], F[2], F[3], g.nx, g.ld, g.ny, g.nz,
-                       T(g.cs_count), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+                       T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, ul, n3, T(0));
@@ -593,9 +593,10 @@ template <typename T>
 struct LoadLag {  // field order: ub vb wb | uu uv uw vv vw ww | S11 S12 S13 S22 S23 S33 | |S|S_ij
   const T *u, *v, *w, *S11, *S12, *S13, *S22, *S23, *S33;
   int ny, nz, ld;
+  int f0;  // first field transformed
   static constexpr double reads() { return 9.0 / 21.0; }  // u,v,w,S_ij feed 21 rows
   __device__ T operator()(int plane, int y, int x) const {
-    const int f = plane / nz, k = plane - f * nz;
+    const int fr = plane / nz, k = plane - fr * nz, f = fr + f0;
     const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
     if (f < 9) {
       T ub, vb, wb;
@@ -725,12 +726,12 @@ __global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_
 // and 4 Delta into the S_LAGF scratch: F2 = fields 0..20, F4 = 21..41, each
 // (ld,ny,nz).  Field order: ub vb wb | uu uv uw vv vw ww | S11..S33 | |S|S_ij.
 template <typename T>
-T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w) {
+T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, int f0) {
   const Grid& g = c->g;
-  const int nz = g.nz, k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4;
-  const long rows = 21L * nz * g.ny;
+  const int nz = g.nz, k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4, nf = 21 - f0;
+  const long rows = (long)nf * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
-  LoadLag<T> L{u, v, w, S[0], S[1], S[2], S[3], S[4], S[5], g.ny, nz, g.ld};
+  LoadLag<T> L{u, v, w, S[0], S[1], S[2], S[3], S[4], S[5], g.ny, nz, g.ld, f0};
   launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
   cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
   cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
@@ -739,13 +740,13 @@ T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const
   col_out(a, 0, K_G, k2, k2, (long)g.ny * k2);
   col_out(a, 1, K_G, k4, k4, (long)g.ny * k4);
   const Tables<T>& t = tables<T>(c);
-  launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, 21L * nz, kf, t.G2, t.G4);
+  launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, (long)nf * nz, kf, t.G2, t.G4);
   const size_t n3 = (size_t)g.ld * g.ny * nz;
   T* F = (T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
-  // row_inv of 21 fields: one "field" spanning all 21 (rows are field-major)
-  StoreFields<T> S2{{F, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
+  // row_inv of fields f0..20: one "field" spanning them (rows are field-major)
+  StoreFields<T> S2{{F + f0 * n3, nullptr, nullptr, nullptr}, nf * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
-  StoreFields<T> S4{{F + 21 * n3, nullptr, nullptr, nullptr}, 21 * nz, g.ny, g.ld, g.nx};
+  StoreFields<T> S4{{F + (21 + f0) * n3, nullptr, nullptr, nullptr}, nf * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
   return F;
 }
@@ -755,13 +756,14 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   OpTag _tag(c, "lagrange");
   const Grid& g = c->g;
   const int nz = g.nz;
+  c->last_lag_jt = c->jt;
   op_interpolag<T>(c, s);
   T* F = lag_filter_mom<T>(c, S, u, v, w);
   const size_t n3 = (size_t)g.ld * g.ny * nz;
   double* red = (double*)scratch(c, S_RED, 3 * nz * sizeof(double));
   op_fill<double>(c, red, 3 * (size_t)nz, 0.0);
-  const int init_lm = g.inilag && !c->flm_init && (c->jt == g.cs_count || c->jt == 1);
-  const int init_qn = g.inilag && !c->fqn_init && (c->jt == g.cs_count || c->jt == 1);
+  const int init_lm = g.inilag && !c->flm_init && (c->jt == lag_count(c) || c->jt == 1);
+  const int init_qn = g.inilag && !c->fqn_init && (c->jt == lag_count(c) || c->jt == 1);
   const double delta = g.delta;
   {
     ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
@@ -770,7 +772,7 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
                        S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
                        (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
                        (T*)s.Q11t, (T*)s.Q22t, (T*)s.Q33t, red, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
-                       1.5 * delta, g.dt * (double)g.cs_count, log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
+                       1.5 * delta, g.dt * (double)lag_count(c), log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
   }
   LCHK(c, hipGetLastError());
   if (s.Beta_avg && s.Betaclip_avg) {
@@ -808,10 +810,20 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
       // so its S11..S33 are bound to u, v, w, S11, S12, S13
       const T* Sd[6] = {u, v, w, S[0], S[1], S[2]};
       op_scaledep_dynamic<T>(c, Sd, u, v, w, nullptr, nullptr, (T*)s.Cs_opt2);
+    } else if (c->closure_mode && c->mlp && jt % c->closure_period != 0) {
+      // learned closure between dynamic updates: only the filtered strain
+      // rates are needed (fields 9..20 of the lagrange_Sdep filter set)
+      const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+      const T* F = lag_filter_mom<T>(c, Sc, u, v, w, 9);
+      op_mlp_closure<T>(c, F, Sc, gr, (T*)s.Cs_opt2, (T*)s.Beta, 0, c->closure_thresh);
     } else {
       const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
       op_lagrange<T>(c, Sc, u, v, w, s);
       if (ss) op_lagrange_scalar<T>(c, Sc, u, v, w, theta, s, *ss, true);  // sgs_stag.f90:150-153
+      if (c->closure_mode && c->mlp) {  // the dynamic step judges the network
+        const T* F = (const T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T));
+        op_mlp_closure<T>(c, F, Sc, gr, (T*)s.Cs_opt2, (T*)s.Beta, 1, c->closure_thresh);
+      }
     }
   }
   const int use_const = !m3 && c->jt_total < 500;  // :165-174 applies to models 5-7
@@ -1285,7 +1297,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
   template void op_interpolag<T>(les_ctx*, const SgsPtrs&);                                                      \
   template void op_interpolag4<T>(les_ctx*, T* const[4], T*, T*, T*);                                            \
-  template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*);                      \
+  template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, int);                 \
   template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \
   template void op_sgs_stag<T>(les_ctx*, const T* const[9], const T*, const T*, const T*, T* const[6],           \
                                const SgsPtrs&, T*, const ScalarSgsPtrs*);                                        \

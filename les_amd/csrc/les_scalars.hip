@@ -724,15 +724,15 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
   StoreFields<T> S4{{Fs + 10 * n3, nullptr, nullptr, nullptr}, 10 * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
-  const int init_lm = g.inilag && !c->glm_init && (c->jt == g.cs_count || c->jt == 1);
-  const int init_qn = g.inilag && !c->gqn_init && (c->jt == g.cs_count || c->jt == 1);
+  const int init_lm = g.inilag && !c->glm_init && (c->jt == lag_count(c) || c->jt == 1);
+  const int init_qn = g.inilag && !c->gqn_init && (c->jt == lag_count(c) || c->jt == 1);
   const double delta = g.delta;
   {
     ProfScope _ps(c, "lag_scalar_epilogue", (6.0 + 20 + 2 + 1 + 4 + 4 + 3) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_lag_scalar_epilogue<T>), grid1(n3), dim3(256), 0, c->stream, F, Fs, (const T*)m.TnLMMM,
                        (const T*)m.TnQNNN, (const T*)m.Cs_opt2, (T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN, (T*)ss.G_NN,
                        (T*)m.Beta, (T*)ss.Cs2_Pr, (T*)ss.Pr_t, g.nx, g.ld, g.ny, nz, delta * delta, 1.5 * delta,
-                       g.dt * (double)g.cs_count, log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
+                       g.dt * (double)lag_count(c), log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
   }
   LCHK(c, hipGetLastError());
   if (init_lm) c->glm_init = 1;

@@ -236,4 +236,15 @@ double or_uniform(uint64_t seed, uint64_t index);
 #ifdef __cplusplus
 }
 #endif
+
+/* ---- learned SGS closure reference (les_mlp_oracle.c): 8 -> 64 -> 64 -> 64
+   -> 2 MLP with SiLU; theta flat W1 b1 W2 b2 W3 b3 W4 b4 ([out][in]); x
+   (P,8), y/t (P,2); bf16 != 0 rounds operands where the GPU's MFMA does ---- */
+#define OR_MLP_NIN 8
+#define OR_MLP_NH 64
+#define OR_MLP_NOUT 2
+int    or_mlp_nparam(void);
+void   or_mlp_forward(const double *theta, const double *x, long P, double *y, int bf16);
+double or_mlp_loss_grad(const double *theta, const double *x, const double *t, long P, double *grad, int bf16);
+
 #endif

@@ -395,5 +395,46 @@ class OracleSim:
             self.L.or_step(C.c_void_p(self.o.ctx), self.s)
 
 
+MLP_NIN, MLP_NH, MLP_NOUT = 8, 64, 2
+
+
+def mlp_nparam():
+    return lib().or_mlp_nparam()
+
+
+def mlp_forward(theta, x, bf16=True):
+    """Learned-closure MLP on (P, 8) inputs -> (P, 2); theta flat float64."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    P = x.shape[0]
+    y = np.zeros((P, MLP_NOUT))
+    lib().or_mlp_forward(_p(theta), _p(x), C.c_long(P), _p(y), int(bf16))
+    return y
+
+
+def mlp_loss_grad(theta, x, t, bf16=True):
+    """Loss 1/(2P) sum (y - t)^2 and its gradient with respect to theta."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.zeros_like(theta)
+    L = lib()
+    L.or_mlp_loss_grad.restype = C.c_double
+    loss = L.or_mlp_loss_grad(_p(theta), _p(x), _p(t), C.c_long(x.shape[0]), _p(g), int(bf16))
+    return loss, g
+
+
+def mlp_slices():
+    """Offsets of W1 b1 W2 b2 W3 b3 W4 b4 in the flat parameter vector."""
+    shapes = [(MLP_NH, MLP_NIN), (MLP_NH,), (MLP_NH, MLP_NH), (MLP_NH,), (MLP_NH, MLP_NH), (MLP_NH,),
+              (MLP_NOUT, MLP_NH), (MLP_NOUT,)]
+    out, o = [], 0
+    for sh in shapes:
+        n = int(np.prod(sh))
+        out.append((o, sh))
+        o += n
+    return out
+
+
 def uniform(seed, index):
     return lib().or_uniform(seed, index)
