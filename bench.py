@@ -12,6 +12,9 @@ scalars (model 5).  A "step" is one pass of main.f90:202-415.
 Headline `value` = FP32-bulk grid-point-steps/s (FP64 kept for the pressure
 tridiagonal, wall stress, surface layer and reductions); the plain FP64 rate
 is reported beside it.  Inputs are resident in HBM before the timed region.
+`learned_closure`: the same FP32-bulk step with the MLP closure (lagrange_Sdep
+every 10 steps, the BF16-MFMA network at the other updates), after training
+the network on the GPU on 2e6 samples harvested from this workload.
 """
 import argparse
 import ctypes
@@ -80,6 +83,80 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     return ms, prof, finite, rms.value, list(out4), fields
 
 
+def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train_steps=2000, period=10,
+                threshold=0.2, profile_steps=10):
+    """Learned-closure run: the FP32-bulk step with lagrange_Sdep every `period`
+    steps and the network at the other cs_count multiples.  The network is
+    first trained on the GPU on `samples` (features, Cs_opt2/Beta) pairs
+    harvested from the dynamic updates of the same workload."""
+    import torch
+    from les_amd.api import LesSim, checkerboard_flux
+    scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
+    sim = LesSim(nx, ny, nz, precision="fp32", dt=1e-4, graph=False, scalars=scal)
+    sim.init_ic(seed=42)
+    if conv:
+        sim.init_scalar_ic(seed=42, **SCALAR_IC)
+    sim.mlp_init(1)
+    nharv = 4
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    Xs, Ys = [], []
+    sim.step(10)
+    for _ in range(nharv):          # dynamic updates at jt = 15, 20, 25, 30
+        sim.step(5)
+        X, Y = sim.mlp_harvest()
+        idx = torch.randint(0, X.shape[0], (samples // nharv,), device="cuda", generator=gen)
+        Xs.append(X[idx].contiguous())
+        Ys.append(Y[idx].contiguous())
+        del X, Y
+    X, Y = torch.cat(Xs), torch.cat(Ys)
+    del Xs, Ys
+    sim.mlp_fit_norm(X, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = sim.mlp_train(X, Y, batch=8192, steps=train_steps, lr=2e-3, seed=0)
+    torch.cuda.synchronize()
+    train_ms = (time.perf_counter() - t0) * 1e3
+    loss0, loss1 = float(hist[:20].mean()), float(hist[-100:].mean())
+    del X, Y
+    sim.set_closure(1, period=period, threshold=threshold)
+    sim.step(max(warmup, 2 * period))  # eager: the closure's windows run once before capture
+    judge = sim.closure_judge()
+    sim.set_graph(graph)
+    sim.graph_prepare()
+    sim.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    judge2 = sim.closure_judge()
+    prof = None
+    if profile_steps:
+        sim.set_graph(False)
+        sim._chk(sim.L.les_profile(sim.ctx, 1))
+        sim.step(profile_steps)
+        buf = ctypes.create_string_buffer(1 << 16)
+        sim._chk(sim.L.les_profile_report(sim.ctx, buf, len(buf)))
+        sim._chk(sim.L.les_profile(sim.ctx, 0))
+        prof = json.loads(buf.value.decode())
+    del sim
+    torch.cuda.empty_cache()
+    out = {"ms_per_step": ms, "period": period, "threshold": threshold, "train_samples": samples,
+           "train_steps": train_steps, "train_batch": 8192, "train_ms": train_ms, "loss_first": loss0,
+           "loss_last": loss1, "judge_rel_err": float(judge2[2]), "judge_rel_err_first": float(judge[2]),
+           "network_in_use": bool(judge2[3] == 1.0)}
+    if prof:
+        P = nx * ny * (nz - 1)
+        flop = 2.0 * (8 * 64 + 64 * 64 * 2 + 64 * 2)  # per point, forward
+        for tag in ("mlp_apply/mlp_infer", "mlp_judge/mlp_infer"):
+            if tag in prof:
+                n, t_ms, b = prof[tag]
+                out[tag] = {"launches": n, "avg_ms": t_ms / n, "GB/s": b / n / (t_ms / n * 1e-3) / 1e9,
+                            "TFLOP/s": P * flop / (t_ms / n * 1e-3) / 1e12}
+    return out
+
+
 def fp32_error(f32, f64, nx, nz):
     """Per-field relative L2 error of the FP32-bulk state against the FP64
     state after the same steps, and the worst plane-mean profile error."""
@@ -133,6 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", choices=("convective", "neutral"), default="convective")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -155,6 +233,15 @@ def main():
         err = fp32_error(f32, f64, nx, nz)
         err["steps"] = W + K + args.profile_steps
         del f32, f64
+
+    closure = None
+    if not args.no_closure:
+        closure = run_closure(nx, ny, nz, K, W, graph, conv)
+        closure["value"] = pts / (closure["ms_per_step"] * 1e-3)
+        closure["speedup_vs_dynamic"] = ms32 / closure["ms_per_step"]
+        print(f"learned closure: {closure['ms_per_step']:.3f} ms/step judge err {closure['judge_rel_err']:.4f} "
+              f"in use {closure['network_in_use']} (train {closure['train_ms']:.0f} ms, loss "
+              f"{closure['loss_first']:.3g} -> {closure['loss_last']:.3g})", file=sys.stderr, flush=True)
 
     value = pts / (ms32 * 1e-3)
     roof = None
@@ -198,6 +285,7 @@ def main():
         "fp32_rel_error": err,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_closure": closure,
     }
     print(json.dumps(line), flush=True)
 
