@@ -256,6 +256,21 @@ int les_mirror_pull(les_ctx *c, void *host, uint64_t bytes);
 int les_profile(les_ctx *c, int enable);
 int les_profile_report(les_ctx *c, char *buf, int buflen);
 
+/* ---- ensembles (no counterpart in the reference code): independent
+   contexts, each with its own grid, surface parameters, precision and state
+   (les_state_alloc + initial condition done), advanced together.  A window of
+   W steps (W = lcm of the members' windows) of every member is one HIP graph
+   whose branches run on the members' streams; windows in which a member has
+   a one-off condition run eagerly.  Each member's results are bit-identical
+   to stepping it alone.  Members must outlive the ensemble. ---- */
+typedef struct les_ensemble les_ensemble;
+int les_ensemble_create(les_ctx *const *members, int n, les_ensemble **out);
+void les_ensemble_destroy(les_ensemble *e);
+const char *les_ensemble_last_error(les_ensemble *e);
+int les_ensemble_window(les_ensemble *e);
+int les_ensemble_step(les_ensemble *e, int nsteps);
+int les_ensemble_sync(les_ensemble *e);
+
 /* ---- learned SGS closure (no counterpart in the reference code).  An MLP
    8 -> 64 -> 64 -> 64 -> 2 (SiLU) maps local invariants of the strain rate to
    the (Cs_opt2, Beta) that lagrange_Sdep.f90:390-421 produces, with BF16

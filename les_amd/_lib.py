@@ -81,8 +81,16 @@ def load():
     L.les_mlp_grad.argtypes = [vp, fp, fp, i64, fp, fp]
     L.les_mlp_train.argtypes = [vp, fp, fp, i64, C.c_int, C.c_int, C.c_float, u64, fp]
     L.les_set_closure.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+    L.les_ensemble_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.les_ensemble_destroy.argtypes = [vp]
+    L.les_ensemble_destroy.restype = None
+    L.les_ensemble_last_error.argtypes = [vp]
+    L.les_ensemble_last_error.restype = C.c_char_p
+    for nm in ("les_ensemble_window", "les_ensemble_step", "les_ensemble_sync"):
+        getattr(L, nm).argtypes = [vp] + ([C.c_int] if nm == "les_ensemble_step" else [])
     for name in declared_symbols():
-        if name not in ("les_destroy", "les_last_error", "les_state_field"):
+        if name not in ("les_destroy", "les_last_error", "les_state_field", "les_ensemble_destroy",
+                        "les_ensemble_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -881,6 +881,231 @@ static void graph_capture(les_ctx* c, int W, int uc) {
   if (graph) (void)hipGraphDestroy(graph);
 }
 
+// host-side counters after a replayed window of W steps (the captured steps
+// advanced and then restored them): W is even, so the RHS buffer parity is
+// unchanged; the last Lagrangian update is the last multiple of its period
+static void advance_after_replay(les_ctx* c, int W) {
+  const int jt0 = c->jt;
+  c->jt += W;
+  c->jt_total += W;
+  const int lc = lag_count(c), last = (c->jt / lc) * lc;
+  if (c->sgs_model == 5 && last > jt0 && last > 1) c->last_lag_jt = last;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// ensembles: independent contexts (grids, surface parameters, precisions)
+// advanced together.  One HIP graph holds a window of W steps of every
+// member: the capture stream forks to each member's stream, the member's
+// steps are recorded there, and the branches join back, so a window of the
+// whole ensemble is one hipGraphLaunch and independent branches may run
+// concurrently.  Windows in which any member has a one-off condition run
+// eagerly, member by member, each on its own stream.
+// ------------------------------------------------------------------------
+struct les_ensemble {
+  std::vector<les_ctx*> m;
+  hipStream_t stream = nullptr;
+  std::vector<hipStream_t> own;  // streams created for members that had none
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int> key_mod, key_uc, key_swap;
+  int W = 2;
+  std::string err;
+};
+
+static int gcd_i(int a, int b) { return b ? gcd_i(b, a % b) : a; }
+
+static bool ens_check(les_ensemble* e) {
+  for (les_ctx* c : e->m)
+    if (!c->err.empty()) {
+      e->err = "member: " + c->err;
+      return false;
+    }
+  return e->err.empty();
+}
+static bool ens_regular(les_ensemble* e, std::vector<int>& uc) {
+  uc.assign(e->m.size(), 0);
+  for (size_t i = 0; i < e->m.size(); ++i)
+    if (!window_is_regular(e->m[i], e->W, &uc[i])) return false;
+  return true;
+}
+static bool ens_matches(const les_ensemble* e, const std::vector<int>& uc) {
+  if (!e->gexec) return false;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    const les_ctx* c = e->m[i];
+    if (e->key_mod[i] != c->jt % e->W || e->key_uc[i] != uc[i] || e->key_swap[i] != c->st->rhs_swap) return false;
+  }
+  return true;
+}
+// ensemble stream waits for every member's queued work, or every member
+// waits for the ensemble stream
+static void ens_join_members(les_ensemble* e) {
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    (void)hipEventRecord(e->join[i], e->m[i]->stream);
+    (void)hipStreamWaitEvent(e->stream, e->join[i], 0);
+  }
+}
+static void ens_fork_members(les_ensemble* e) {
+  (void)hipEventRecord(e->fork, e->stream);
+  for (les_ctx* c : e->m) (void)hipStreamWaitEvent(c->stream, e->fork, 0);
+}
+static void ens_capture(les_ensemble* e, const std::vector<int>& uc) {
+  if (e->gexec) {
+    (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+  }
+  struct Saved { int jt, jtt, swap, fl, fq, gl, gq, lag; long es; };
+  std::vector<Saved> sv;
+  for (les_ctx* c : e->m) {
+    check(c, hipStreamSynchronize(c->stream), "ensemble pre-capture sync");
+    sv.push_back({c->jt, c->jt_total, c->st->rhs_swap, c->flm_init, c->fqn_init, c->glm_init, c->gqn_init,
+                  c->last_lag_jt, c->eager_steps});
+  }
+  (void)hipStreamSynchronize(e->stream);
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    e->err = "ensemble: hipStreamBeginCapture failed";
+    return;
+  }
+  ens_fork_members(e);
+  for (size_t i = 0; i < e->m.size(); ++i)
+    for (int s = 0; s < e->W; ++s) step_eager(e->m[i]);
+  ens_join_members(e);
+  if (hipStreamEndCapture(e->stream, &graph) != hipSuccess) e->err = "ensemble: hipStreamEndCapture failed";
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    les_ctx* c = e->m[i];
+    const Saved& v = sv[i];
+    c->jt = v.jt;
+    c->jt_total = v.jtt;
+    c->st->rhs_swap = v.swap;
+    c->flm_init = v.fl;
+    c->fqn_init = v.fq;
+    c->glm_init = v.gl;
+    c->gqn_init = v.gq;
+    c->last_lag_jt = v.lag;
+    c->eager_steps = v.es;
+  }
+  if (graph && ens_check(e)) {
+    if (hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      e->err = "ensemble: hipGraphInstantiate failed";
+      e->gexec = nullptr;
+    }
+    e->key_mod.clear(); e->key_uc.clear(); e->key_swap.clear();
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      e->key_mod.push_back(e->m[i]->jt % e->W);
+      e->key_uc.push_back(uc[i]);
+      e->key_swap.push_back(e->m[i]->st->rhs_swap);
+    }
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+}
+
+extern "C" {
+
+int les_ensemble_create(les_ctx* const* members, int n, les_ensemble** out) {
+  if (!members || n < 1 || !out) return -1;
+  *out = nullptr;
+  les_ensemble* e = new les_ensemble();
+  for (int i = 0; i < n; ++i) {
+    les_ctx* c = members[i];
+    if (!c || !c->st) {
+      delete e;
+      return -1;
+    }
+    for (les_ctx* d : e->m)
+      if (d == c) {
+        delete e;
+        return -1;
+      }
+    if (c->device != members[0]->device) {
+      delete e;
+      return -1;
+    }
+    e->m.push_back(c);
+  }
+  int W = 2;
+  for (les_ctx* c : e->m) {
+    const int w = graph_window(c);
+    W = W / gcd_i(W, w) * w;
+  }
+  e->W = W;
+  (void)hipSetDevice(members[0]->device);
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return 1;
+  }
+  for (les_ctx* c : e->m)
+    if (!c->stream) {  // the legacy stream cannot join a capture
+      hipStream_t s = nullptr;
+      (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+      c->stream = s;
+      e->own.push_back(s);
+    }
+  (void)hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  e->join.resize(e->m.size());
+  for (auto& j : e->join) (void)hipEventCreateWithFlags(&j, hipEventDisableTiming);
+  *out = e;
+  return 0;
+}
+
+void les_ensemble_destroy(les_ensemble* e) {
+  if (!e) return;
+  (void)hipStreamSynchronize(e->stream);
+  for (les_ctx* c : e->m) (void)hipStreamSynchronize(c->stream);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  (void)hipEventDestroy(e->fork);
+  for (auto j : e->join) (void)hipEventDestroy(j);
+  // members keep working after the ensemble is gone: give back the streams
+  // created here only when no member still uses them
+  for (hipStream_t s : e->own) {
+    bool used = false;
+    for (les_ctx* c : e->m) used = used || c->stream == s;
+    if (!used) (void)hipStreamDestroy(s);
+  }
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* les_ensemble_last_error(les_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int les_ensemble_window(les_ensemble* e) { return e ? e->W : -1; }
+
+int les_ensemble_step(les_ensemble* e, int nsteps) {
+  if (!e) return -1;
+  e->err.clear();
+  for (les_ctx* c : e->m) c->err.clear();
+  int done = 0;
+  std::vector<int> uc;
+  while (done < nsteps && ens_check(e)) {
+    if (nsteps - done >= e->W && ens_regular(e, uc)) {
+      if (!ens_matches(e, uc)) ens_capture(e, uc);
+      if (e->gexec && ens_check(e)) {
+        ens_join_members(e);  // eager work queued on the members comes first
+        if (hipGraphLaunch(e->gexec, e->stream) != hipSuccess) {
+          e->err = "ensemble: hipGraphLaunch failed";
+          break;
+        }
+        ens_fork_members(e);  // and later member work waits for the window
+        for (les_ctx* c : e->m) advance_after_replay(c, e->W);
+        done += e->W;
+        continue;
+      }
+    }
+    for (les_ctx* c : e->m) step_eager(c);
+    ++done;
+  }
+  return ens_check(e) ? 0 : 1;
+}
+
+int les_ensemble_sync(les_ensemble* e) {
+  if (!e) return -1;
+  bool ok = hipStreamSynchronize(e->stream) == hipSuccess;
+  for (les_ctx* c : e->m) ok = (hipStreamSynchronize(c->stream) == hipSuccess) && ok;
+  return ok ? 0 : 1;
+}
+
 // ---- learned closure ----
 int les_mlp_nparam(void) { return les::mlp_nparam(); }
 #define NEED_MLP(c)                                            \
@@ -1042,8 +1267,7 @@ int les_step(les_ctx* c, int nsteps) {
       if (!graph_matches(c, W, uc)) graph_capture(c, W, uc);
       if (c->gexec && c->err.empty()) {
         check(c, hipGraphLaunch(c->gexec, c->stream), "hipGraphLaunch");
-        c->jt += W;
-        c->jt_total += W;  // W is even: the RHS buffer parity is unchanged
+        advance_after_replay(c, W);
         done += W;
         continue;
       }

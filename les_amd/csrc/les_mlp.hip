@@ -57,9 +57,13 @@ struct MlpModel {
 
 __device__ __forceinline__ int kidx(int s, int g, int e) { return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3); }
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + expf(-z)); }
+// SiLU with the hardware exponential (v_exp_f32 through __expf) and
+// reciprocal (v_rcp_f32): a few ulp of float, far below the bf16 rounding
+// the result goes through; for z < -88 the reciprocal of inf gives -0
+__device__ __forceinline__ float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
 __device__ __forceinline__ float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = sigmoid_f(z);
   return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ short bf_bits(float x) { return __builtin_bit_cast(short, (__bf16)x); }
@@ -172,28 +176,49 @@ __device__ void load_block_consts(const uint4* __restrict__ ops_g, const float* 
 }
 
 // ---- inference: Feat(p, f[8]) gives the raw features of point p; Out(p, y0, y1)
+// A wave takes 64 consecutive points: every lane computes the features of one
+// point (coalesced loads), the normalised features go through LDS into four
+// 16-point MFMA tiles, and the outputs come back through LDS so that every
+// lane stores one point.  Trip counts are block-uniform (barriers).
 template <class Feat, class Out>
 __global__ void __launch_bounds__(256) k_mlp_infer(Feat feat, Out out, long P, const uint4* __restrict__ ops_g,
                                                    const float* __restrict__ th, MlpNorm nm) {
   __shared__ uint4 ops[NOPS_FWD * 64];
   __shared__ float sb[NBIAS];
+  __shared__ float xs[4][64][MI + 1];
+  __shared__ float ys[4][64][MO];
   load_block_consts(ops_g, th, ops, sb, NOPS_FWD);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
-  const long ntiles = (P + 15) / 16, stride = (long)gridDim.x * 4;
-  for (long tile = (long)blockIdx.x * 4 + wave; tile < ntiles; tile += stride) {  // wave-uniform
-    const long p = tile * 16 + c;
-    bf16x8 xin;
+  const long ngroups = (P + 63) / 64, nw = (long)gridDim.x * 4, gw = (long)blockIdx.x * 4 + wave;
+  const long iters = (ngroups + nw - 1) / nw;
+  for (long it = 0; it < iters; ++it) {
+    const long grp = it * nw + gw, p = grp * 64 + lane;
+    const bool valid = grp < ngroups && p < P;
+    float f[MI];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) xin[e] = (__bf16)0.0f;
-    if (g == 0 && p < P) {
-      float f[MI];
+    for (int e = 0; e < MI; ++e) f[e] = 0.0f;
+    if (valid) {
       feat(p, f);
 #pragma unroll
-      for (int e = 0; e < MI; ++e) xin[e] = (__bf16)((f[e] - nm.mu[e]) * nm.isd[e]);
+      for (int e = 0; e < MI; ++e) f[e] = (f[e] - nm.mu[e]) * nm.isd[e];
     }
-    MlpTile T;
-    mlp_forward_tile(ops, sb, xin, T, lane);
-    if (g == 0 && p < P) out(p, T.y[0] * nm.ysd[0] + nm.ymu[0], T.y[1] * nm.ysd[1] + nm.ymu[1]);
+#pragma unroll
+    for (int e = 0; e < MI; ++e) xs[wave][lane][e] = f[e];
+    __syncthreads();
+#pragma unroll 1
+    for (int tt = 0; tt < 4; ++tt) {
+      bf16x8 xin;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xin[e] = (__bf16)(g == 0 ? xs[wave][16 * tt + c][e] : 0.0f);
+      MlpTile T;
+      mlp_forward_tile(ops, sb, xin, T, lane);
+      if (g == 0) {
+        ys[wave][16 * tt + c][0] = T.y[0];
+        ys[wave][16 * tt + c][1] = T.y[1];
+      }
+    }
+    __syncthreads();
+    if (valid) out(p, ys[wave][lane][0] * nm.ysd[0] + nm.ymu[0], ys[wave][lane][1] * nm.ysd[1] + nm.ymu[1]);
   }
 }
 
@@ -493,13 +518,15 @@ struct FeatLag {
       uz = gr[2][i]; vz = gr[5][i]; wx = gr[6][i]; wy = gr[7][i];
     }
     const double o12 = 0.5 * (uy - vx), o13 = 0.5 * (uz - wx), o23 = 0.5 * (vz - wy);
-    f[0] = (float)asinh(sqrt(2.0 * dot(s, s)));
-    f[1] = (float)asinh(sqrt(2.0 * dot(sb, sb)));
-    f[2] = (float)asinh(sqrt(2.0 * dot(sh, sh)));
-    f[3] = (float)asinh(dot(s, sb));
-    f[4] = (float)asinh(dot(sb, ssb));
-    f[5] = (float)asinh(dot(sh, ssh));
-    f[6] = (float)asinh(2.0 * sqrt(o12 * o12 + o13 * o13 + o23 * o23));
+    // contractions in double, the compressing transforms in float (the
+    // network sees the inputs rounded to bf16)
+    f[0] = asinhf(sqrtf((float)(2.0 * dot(s, s))));
+    f[1] = asinhf(sqrtf((float)(2.0 * dot(sb, sb))));
+    f[2] = asinhf(sqrtf((float)(2.0 * dot(sh, sh))));
+    f[3] = asinhf((float)dot(s, sb));
+    f[4] = asinhf((float)dot(sb, ssb));
+    f[5] = asinhf((float)dot(sh, ssh));
+    f[6] = asinhf(2.0f * sqrtf((float)(o12 * o12 + o13 * o13 + o23 * o23)));
     f[7] = (k == 0 ? 0.5f * dz : (float)k * dz) * idelta;
   }
 };
@@ -702,8 +729,8 @@ void mlp_fit_norm(les_ctx* c, const float* X, const float* Y, long P) {
 void op_mlp_forward(les_ctx* c, const float* X, long P, float* Y) {
   OpTag _tag(c, "mlp");
   MlpModel* M = model(c);
-  const long tiles = (P + 15) / 16;
-  const unsigned blocks = (unsigned)((tiles + 3) / 4 < 2048 ? (tiles + 3) / 4 : 2048);
+  const long groups = (P + 63) / 64;
+  const unsigned blocks = (unsigned)((groups + 3) / 4 < 2048 ? (groups + 3) / 4 : 2048);
   ProfScope _ps(c, "mlp_infer", (double)P * (MI + MO) * 4);
   hipLaunchKernelGGL((k_mlp_infer<FeatArray, OutArray>), dim3(blocks > 0 ? blocks : 1), dim3(256), 0, c->stream,
                      FeatArray{X}, OutArray{Y}, P, (const uint4*)M->ops, (const float*)M->theta, M->norm);
@@ -787,8 +814,8 @@ void op_mlp_closure(les_ctx* c, const T* F, const T* const S[6], const T* const 
   const Grid& g = c->g;
   const long P = (long)g.nx * g.ny * (g.nz - 1);
   const FeatLag<T> fl = feat_lag<T>(c, F, S, gr);
-  const long tiles = (P + 15) / 16;
-  const unsigned blocks = (unsigned)((tiles + 3) / 4 < 4096 ? (tiles + 3) / 4 : 4096);
+  const long groups = (P + 63) / 64;
+  const unsigned blocks = (unsigned)((groups + 3) / 4 < 2048 ? (groups + 3) / 4 : 2048);
   if (judge) {
     float* pred = (float*)scratch(c, S_MLPP, (size_t)P * sizeof(float));
     {
