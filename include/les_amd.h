@@ -256,6 +256,17 @@ int les_mirror_pull(les_ctx *c, void *host, uint64_t bytes);
 int les_profile(les_ctx *c, int enable);
 int les_profile_report(les_ctx *c, char *buf, int buflen);
 
+/* ---- autotuner (no counterpart in the reference code): times the column-FFT
+   tile width (what & 1; kx columns per block, FP32 8/16/32, FP64 4/8/16) and
+   the tridiagonal schedule (what & 2; where the Thomas sweep keeps gamma and
+   U -- HBM, gamma in LDS, both in LDS -- and modes per block) on this
+   context's state, and keeps the fastest.  Results do not depend on the
+   choice.  report: JSON with the measured ms per candidate (may be NULL).
+   les_get_tune / les_set_tune: {col_tx FP32, col_tx FP64, tri_mode, tri_block}. ---- */
+int les_autotune(les_ctx *c, int what, char *report, int report_len);
+int les_get_tune(les_ctx *c, int *out4);
+int les_set_tune(les_ctx *c, const int *in4);
+
 /* ---- ensembles (no counterpart in the reference code): independent
    contexts, each with its own grid, surface parameters, precision and state
    (les_state_alloc + initial condition done), advanced together.  A window of

@@ -194,6 +194,27 @@ class Les:
         self._chk(self.L.les_scaledep_dynamic(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(cs), _ptr(beta)))
         return cs, beta
 
+    # ---- autotuner (launch shapes only; results do not depend on them) ----
+    def autotune(self, what=3):
+        """Time column-FFT tile widths (what & 1) and tridiagonal schedules
+        (what & 2) on this context's state and keep the fastest; returns the
+        measured ms per candidate and the choice."""
+        import json
+        buf = C.create_string_buffer(4096)
+        self._chk(self.L.les_autotune(self.ctx, int(what), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def get_tune(self):
+        import numpy as np
+        a = np.zeros(4, dtype=np.int32)
+        self._chk(self.L.les_get_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
+        return dict(col_tx_fp32=int(a[0]), col_tx_fp64=int(a[1]), tri_mode=int(a[2]), tri_block=int(a[3]))
+
+    def set_tune(self, col_tx_fp32=16, col_tx_fp64=8, tri_mode=0, tri_block=128):
+        import numpy as np
+        a = np.array([col_tx_fp32, col_tx_fp64, tri_mode, tri_block], dtype=np.int32)
+        self._chk(self.L.les_set_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
+
     # ---- learned closure (no counterpart in the reference code) ----
     MLP_NIN, MLP_NOUT = 8, 2
 

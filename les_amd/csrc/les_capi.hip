@@ -36,6 +36,15 @@ void* scratch(les_ctx* c, int slot, size_t bytes) {
   return a.ptr[slot];
 }
 
+void scratch_release(les_ctx* c, int slot) {
+  Arena& a = c->arena;
+  if ((int)a.ptr.size() <= slot || !a.ptr[slot]) return;
+  check(c, hipStreamSynchronize(c->stream), "scratch release sync");
+  check(c, hipFree(a.ptr[slot]), "scratch release");
+  a.ptr[slot] = nullptr;
+  a.cap[slot] = 0;
+}
+
 static FftPlan make_plan(int n) {
   FftPlan p;
   std::memset(&p, 0, sizeof(p));
@@ -1104,6 +1113,153 @@ int les_ensemble_sync(les_ensemble* e) {
   bool ok = hipStreamSynchronize(e->stream) == hipSuccess;
   for (les_ctx* c : e->m) ok = (hipStreamSynchronize(c->stream) == hipSuccess) && ok;
   return ok ? 0 : 1;
+}
+
+// ---- autotuner: launch shapes timed on this context's own state ----
+extern "C++" {
+template <class Fn>
+static double tune_time(les_ctx* c, Fn&& fn, int reps = 3) {
+  fn();  // warm-up: scratch sizes, kernel attributes
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(a, c->stream);
+  for (int r = 0; r < reps; ++r) fn();
+  (void)hipEventRecord(b, c->stream);
+  (void)hipEventSynchronize(b);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (double)ms / reps;
+}
+
+template <typename T>
+static void tune_run(les_ctx* c, int what, std::string& rep) {
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1;
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  const size_t ldb = 2 * (size_t)(g.nx2 / 2 + 1), nbig = ldb * g.ny2 * nz;
+  const T* u = F<T>(c, "u");
+  char buf[160];
+  rep += "{";
+  if (what & 1) {
+    // column-FFT tile: x/y derivatives (ny -> ny), zero padding (ny -> 3ny/2)
+    // and truncation (3ny/2 -> ny), the three kinds of column pass of a step
+    T* A = (T*)scratch(c, S_TUNE0, n3 * sizeof(T));
+    T* B = (T*)scratch(c, S_TUNE1, n3 * sizeof(T));
+    T* Bg = (T*)scratch(c, S_TUNE2, nbig * sizeof(T));
+    const int p = sizeof(T) == 8 ? 1 : 0;
+    const int c32[3] = {8, 16, 32}, c64[3] = {4, 8, 16};
+    const int* cand = p ? c64 : c32;
+    int best = c->tune.col_tx[p];
+    double tbest = 1e30;
+    rep += "\"col_tx\": {";
+    for (int i = 0; i < 3 && c->err.empty(); ++i) {
+      c->tune.col_tx[p] = cand[i];
+      const double ms = tune_time(c, [&] {
+        op_ddxy<T>(c, A, B, u, nz);
+        op_dealias1<T>(c, u, Bg, nz);
+        op_dealias2<T>(c, A, Bg, nz);
+      });
+      std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", i ? ", " : "", cand[i], ms);
+      rep += buf;
+      if (ms < tbest) { tbest = ms; best = cand[i]; }
+    }
+    c->tune.col_tx[p] = best;
+    rep += "}";
+    scratch_release(c, S_TUNE0);
+    scratch_release(c, S_TUNE1);
+    scratch_release(c, S_TUNE2);
+  }
+  if ((what & 2) && c->err.empty()) {
+    // tridiagonal schedule, on the spectral right-hand side of this state
+    T* P = (T*)scratch(c, S_TUNE0, (size_t)g.ld * g.ny * (nz + 1) * sizeof(T));
+    T* dx = (T*)scratch(c, S_TUNE1, n3 * sizeof(T));
+    T* dy = (T*)scratch(c, S_TUNE2, n3 * sizeof(T));
+    op_press_stag<T>(c, u, F<T>(c, "v"), F<T>(c, "w"), F<T>(c, "divtz"), P, dx, dy);
+    const long ps = (long)g.ny * g.nxh;
+    cx<T>* H = (cx<T>*)scratch(c, S_PSPEC, (size_t)(3 * nl + 2) * ps * sizeof(cx<T>));
+    cx<T>* PH = (cx<T>*)scratch(c, S_PHAT, (size_t)nz * ps * sizeof(cx<T>));
+    char* tri = (char*)scratch(c, S_TRI, (size_t)(nz + 1) * ps * (sizeof(double) + sizeof(cx<double>)) + 256);
+    int* err = (int*)scratch(c, S_TMP2, 256);
+    const int cand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {2, 32}, {2, 64}};
+    const Tune keep = c->tune;
+    Tune best = keep;
+    double tbest = 1e30;
+    rep += (what & 1) ? ", " : "";
+    rep += "\"tridiag\": {";
+    bool first = true;
+    for (auto& cd : cand) {
+      if (tridiag_lds(cd[0], cd[1], nz) > 160 * 1024 || !c->err.empty()) continue;
+      c->tune.tri_mode = cd[0];
+      c->tune.tri_block = cd[1];
+      const double ms = tune_time(c, [&] {
+        launch_tridiag<T>(c, H, H + (size_t)3 * nl * ps, PH, (double*)tri,
+                          (cx<double>*)(tri + (size_t)(nz + 1) * ps * sizeof(double)), err);
+      }, 5);
+      std::snprintf(buf, sizeof buf, "%s\"%d/%d\": %.4f", first ? "" : ", ", cd[0], cd[1], ms);
+      rep += buf;
+      first = false;
+      if (ms < tbest) {
+        tbest = ms;
+        best = c->tune;
+      }
+    }
+    c->tune.tri_mode = best.tri_mode;
+    c->tune.tri_block = best.tri_block;
+    rep += "}";
+    scratch_release(c, S_TUNE0);
+    scratch_release(c, S_TUNE1);
+    scratch_release(c, S_TUNE2);
+  }
+  std::snprintf(buf, sizeof buf, ", \"chosen\": {\"col_tx\": [%d, %d], \"tri_mode\": %d, \"tri_block\": %d}}",
+                c->tune.col_tx[0], c->tune.col_tx[1], c->tune.tri_mode, c->tune.tri_block);
+  rep += buf;
+}
+}  // extern "C++"
+
+int les_autotune(les_ctx* c, int what, char* report, int report_len) {
+  GUARD(c);
+  if (!c->st) { c->err = "les_autotune: needs a state (les_state_alloc + initial condition)"; return 1; }
+  if (c->gexec) {  // captured windows hold the old launch shapes
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  std::string rep;
+  if (c->st->prec) tune_run<float>(c, what, rep);
+  else tune_run<double>(c, what, rep);
+  if (report && report_len > 0) {
+    std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  DONE(c);
+}
+int les_get_tune(les_ctx* c, int* out4) {
+  GUARD(c);
+  if (!out4) return -1;
+  out4[0] = c->tune.col_tx[0];
+  out4[1] = c->tune.col_tx[1];
+  out4[2] = c->tune.tri_mode;
+  out4[3] = c->tune.tri_block;
+  DONE(c);
+}
+int les_set_tune(les_ctx* c, const int* in4) {
+  GUARD(c);
+  if (!in4) return -1;
+  const bool ok32 = in4[0] == 8 || in4[0] == 16 || in4[0] == 32, ok64 = in4[1] == 4 || in4[1] == 8 || in4[1] == 16;
+  const bool okt = in4[2] >= 0 && in4[2] <= 2 && in4[3] >= 32 && in4[3] <= 1024 && in4[3] % 32 == 0 &&
+                   tridiag_lds(in4[2], in4[3], c->g.nz) <= 160 * 1024;
+  if (!ok32 || !ok64 || !okt) { c->err = "les_set_tune: launch shape out of range"; return 1; }
+  c->tune.col_tx[0] = in4[0];
+  c->tune.col_tx[1] = in4[1];
+  c->tune.tri_mode = in4[2];
+  c->tune.tri_block = in4[3];
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  DONE(c);
 }
 
 // ---- learned closure ----

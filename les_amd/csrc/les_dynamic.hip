@@ -12,6 +12,12 @@
 // thread per level, so the whole model stays inside a captured step.
 #include "les_xform.h"
 
+// Evaluated as written (no multiply-add contraction), like the CPU
+// restatement: rtnewt's Newton path from x = 7.5 can be long, and its
+// rounding should differ from the reference's only through the order of
+// the plane sums.
+#pragma clang fp contract(off)
+
 namespace les {
 
 constexpr int SD_NPART = 32;

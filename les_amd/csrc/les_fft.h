@@ -12,6 +12,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// The transform arithmetic is evaluated as written, without contracting a
+// multiply and an add into an FMA: the compiler's choice would otherwise
+// depend on the surrounding kernel (tile width, unrolling), and a transform
+// must give the same bits whichever launch shape runs it.
+#pragma clang fp contract(off)
+
 namespace les {
 
 template <typename T>
@@ -249,3 +255,5 @@ __device__ __forceinline__ void fft_lds_c(cx<T>* buf, const cx<T>* tw, int sign)
 }
 
 }  // namespace les
+
+#pragma clang fp contract(fast)

@@ -41,6 +41,13 @@ struct Arena {
   std::vector<size_t> cap;
 };
 
+// Launch shapes chosen by les_autotune (results do not depend on them).
+struct Tune {
+  int col_tx[2] = {16, 8};  // column-FFT tile width (kx columns per block): FP32, FP64
+  int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS
+  int tri_block = 128;      // horizontal modes per block of the tridiagonal solve
+};
+
 // Scalar transport switches and constants (param.f90:53,72-90,139-146,174),
 // the lbc=1 surface of bottombc.f90:26-63.
 struct ScalarParams {
@@ -98,6 +105,7 @@ struct les_ctx {
   double wt_mean = 0.0;            // sum(wt_s)/(nx*ny), lagrange_scalar_Sdep.f90:114
   int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
   int sgs_model = 5;               // param.f90 'model': 5 Lagrangian scale-dependent, 3 plane-averaged
+  les::Tune tune;
   int lag_cs = 0;                  // steps between Lagrangian updates when it differs from cs_count (closure)
   int last_lag_jt = -1;            // jt of the last lagrange_Sdep update
   // learned closure (les_mlp.hip): the network between dynamic updates
@@ -114,9 +122,10 @@ namespace les {
 enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
-  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_NSLOTS
+  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_TUNE0, S_TUNE1, S_TUNE2, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
+void scratch_release(les_ctx* c, int slot);  // give a slot's memory back (stream drained first)
 
 template <typename T> const Tables<T>& tables(const les_ctx* c);
 template <> const Tables<double>& tables<double>(const les_ctx* c);
@@ -241,6 +250,10 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
 template <typename T>
 void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy);
 int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u);
+// the pressure solve's tridiagonal sweep under the schedule in c->tune
+size_t tridiag_lds(int mode, int block, int nz);
+template <typename T>
+void launch_tridiag(les_ctx* c, const cx<T>* H, const cx<T>* BT, cx<T>* PH, double* gam, cx<double>* Ub, int* err);
 template <typename T>
 void op_project(les_ctx* c, T* u, T* v, T* w, const T* dpdx, const T* dpdy, const T* dpdz);
 template <typename T> double op_rmsdiv(les_ctx* c, const T* dudx, const T* dvdy, const T* dwdz);

@@ -871,14 +871,23 @@ struct LoadPlanes2 {  // two planes of one field: level 0 and level kt
 // FP64 along z with the boundary rows of press_stag_array.f90:142-165 and the
 // zero-mode integration of :238-247.  H: 3 fields x (nz-1) planes of
 // (ny, nxh) complex; BT: bottom/top planes; P: (nz) planes of p_hat.
-template <typename T>
+// MODE (the schedule les_autotune picks): where the sweep keeps gamma and
+// the eliminated right-hand side U -- 0 both in HBM scratch, 1 gamma in LDS,
+// 2 both in LDS (one column of (nz+1) values per thread).
+template <typename T, int MODE>
 __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__ BT, cx<T>* __restrict__ P,
-                          double* __restrict__ gam, cx<double>* __restrict__ U, const T* __restrict__ kxt,
+                          double* __restrict__ gam_g, cx<double>* __restrict__ U_g, const T* __restrict__ kxt,
                           const T* __restrict__ kyt, const double* __restrict__ kxd, const double* __restrict__ kyd,
                           int nxh, int ny, int nz, double dz, int* err) {
+  extern __shared__ __align__(16) unsigned char tri_smem[];
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const int nm = nxh * ny;
   if (m >= nm) return;
+  const int tid = threadIdx.x, bd = blockDim.x;
+  double* gl = reinterpret_cast<double*>(tri_smem);
+  cx<double>* ul = reinterpret_cast<cx<double>*>(tri_smem + (size_t)(nz + 1) * bd * sizeof(double));
+  auto gam = [&](int j) -> double& { return MODE >= 1 ? gl[j * bd + tid] : gam_g[(long)j * nm + m]; };
+  auto U = [&](int j) -> cx<double>& { return MODE >= 2 ? ul[j * bd + tid] : U_g[(long)j * nm + m]; };
   const int y = m / nxh, x = m - y * nxh;
   const long ps = (long)nm;               // plane stride (complex)
   const long fs = ps * (nz - 1);          // field stride in H
@@ -909,7 +918,7 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
   // row 0: b=-1, c=1, r=-dz*bottom
   double bet = -1.0;
   cx<double> u0 = {(-dz * bot.x) / bet, (-dz * bot.y) / bet};
-  U[m] = u0;
+  U(0) = u0;
   double cprev = 1.0;
   cx<double> uprev = u0;
   cx<double> hxm = {(double)Hx[m].x, (double)Hx[m].y};
@@ -938,16 +947,16 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
     const double gj = cprev / bet;
     bet = bj - aj * gj;
     if (bet == 0.0) atomicExch(err, 2);
-    gam[(long)j * nm + m] = gj;
+    gam(j) = gj;
     uprev = {(rj.x - aj * uprev.x) / bet, (rj.y - aj * uprev.y) / bet};
-    U[(long)j * nm + m] = uprev;
+    U(j) = uprev;
     cprev = j < nz ? c_in : 0.0;
   }
   // back substitution; p_hat levels 0..nz-1 are kept (level nz is BOGUS later)
   cx<double> un = uprev;
   for (int j = nz - 1; j >= 0; --j) {
-    cx<double> uj = U[(long)j * nm + m];
-    const double gn = gam[(long)(j + 1) * nm + m];
+    cx<double> uj = U(j);
+    const double gn = gam(j + 1);
     uj = {uj.x - gn * un.x, uj.y - gn * un.y};
     P[(long)j * ps + m] = {(T)uj.x, (T)uj.y};
     un = uj;
@@ -968,6 +977,40 @@ struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: 
     else dpdy[row * ld + x] = val;
   }
 };
+
+// LDS bytes of a tridiagonal schedule (0: the mode does not use LDS)
+size_t tridiag_lds(int mode, int block, int nz) {
+  return (size_t)(nz + 1) * block * (mode >= 1 ? sizeof(double) : 0) +
+         (size_t)(nz + 1) * block * (mode >= 2 ? sizeof(cx<double>) : 0);
+}
+
+template <typename T>
+void launch_tridiag(les_ctx* c, const cx<T>* H, const cx<T>* BT, cx<T>* PH, double* gam, cx<double>* Ub, int* err) {
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1, nxh = g.nxh, mode = c->tune.tri_mode, bs = c->tune.tri_block;
+  const long ps = (long)g.ny * nxh;
+  const Tables<T>& t = tables<T>(c);
+  const size_t sm = tridiag_lds(mode, bs, nz);
+  const dim3 grid((unsigned)((ps + bs - 1) / bs)), block((unsigned)bs);
+  ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) +
+                                  (nz + 1) * ps * 2.0 * ((mode >= 1 ? 0 : sizeof(double)) +
+                                                         (mode >= 2 ? 0 : sizeof(cx<double>))));
+  if (mode == 0) {
+    hipLaunchKernelGGL((k_tridiag<T, 0>), grid, block, 0, c->stream, H, BT, PH, gam, Ub, t.kx, t.ky, c->td.kx,
+                       c->td.ky, nxh, g.ny, nz, g.dz, err);
+  } else if (mode == 1) {
+    if (sm > 65536)
+      LCHK(c, hipFuncSetAttribute((const void*)k_tridiag<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL((k_tridiag<T, 1>), grid, block, sm, c->stream, H, BT, PH, gam, Ub, t.kx, t.ky, c->td.kx,
+                       c->td.ky, nxh, g.ny, nz, g.dz, err);
+  } else {
+    if (sm > 65536)
+      LCHK(c, hipFuncSetAttribute((const void*)k_tridiag<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL((k_tridiag<T, 2>), grid, block, sm, c->stream, H, BT, PH, gam, Ub, t.kx, t.ky, c->td.kx,
+                       c->td.ky, nxh, g.ny, nz, g.dz, err);
+  }
+  LCHK(c, hipGetLastError());
+}
 
 template <typename T>
 void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy) {
@@ -996,12 +1039,8 @@ void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divt
   cx<double>* Ub = (cx<double>*)(tri + (size_t)(nz + 1) * ps * sizeof(double));
   int* err = (int*)scratch(c, S_TMP2, 256);
   const Tables<T>& t = tables<T>(c);
-  {
-    ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) + (nz + 1) * ps * 2.0 * (sizeof(double) + sizeof(cx<double>)));
-    hipLaunchKernelGGL((k_tridiag<T>), dim3((unsigned)((ps + 127) / 128)), dim3(128), 0, c->stream, H,
-                       H + (size_t)3 * nl * ps, PH, gam, Ub, t.kx, t.ky, c->td.kx, c->td.ky, nxh, g.ny, nz, g.dz, err);
-  }
-  LCHK(c, hipGetLastError());
+  (void)t;
+  launch_tridiag<T>(c, H, H + (size_t)3 * nl * ps, PH, gam, Ub, err);
   // inverse y transforms: levels 1..nz-1 -> p, i kx p, i ky p ; level 0 -> p
   const long rowsP = 3L * nl * g.ny;
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)(rowsP + g.ny) * nxh * sizeof(cx<T>));
@@ -1298,6 +1337,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_interpolag<T>(les_ctx*, const SgsPtrs&);                                                      \
   template void op_interpolag4<T>(les_ctx*, T* const[4], T*, T*, T*);                                            \
   template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, int);                 \
+  template void launch_tridiag<T>(les_ctx*, const cx<T>*, const cx<T>*, cx<T>*, double*, cx<double>*, int*);   \
   template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \
   template void op_sgs_stag<T>(les_ctx*, const T* const[9], const T*, const T*, const T*, T* const[6],           \
                                const SgsPtrs&, T*, const ScalarSgsPtrs*);                                        \

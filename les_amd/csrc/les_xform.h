@@ -416,10 +416,20 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
   }
 }
 
-template <typename T, int NYI, int NYO>
+// column tile width: TX kx columns per block (a 64-lane wavefront reads
+// TX*sizeof(cx<T>) contiguous bytes per row); the default is 128 B rows,
+// les_autotune may pick another (les_coltx.hip holds those variants)
+template <typename T>
+constexpr int col_tx_default() { return sizeof(T) == 8 ? 8 : 16; }
+template <typename T, int TX, int NYI, int NYO>
+constexpr bool col_tile_fits() {
+  return ColGeom<TX, NYI, NYO>::BLOCK <= 1024 &&
+         (size_t)(NYI + NYO + TX * NYI + TX * NYO) * sizeof(cx<T>) <= 160 * 1024;
+}
+
+template <typename T, int TX, int NYI, int NYO>
 static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
                          const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
-  constexpr int TX = sizeof(T) == 8 ? 8 : 16;
   const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
   // the FFT stages need blockDim >= TX*n/8 (at most 4 butterflies per thread)
   int threads = NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
@@ -451,16 +461,25 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
   X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
   X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
 
+// the tuned tile widths other than the default (les_coltx.hip); false when
+// that width has no variant for these lengths
+template <typename T>
+bool launch_col_alt(les_ctx* c, int tx, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
+                    const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2);
+
 template <typename T>
 static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
                        const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
                        const T* G2 = nullptr) {
+  constexpr int TXD = col_tx_default<T>();
+  const int tx = c->tune.col_tx[sizeof(T) == 8 ? 1 : 0];
+  if (tx != TXD && launch_col_alt<T>(c, tx, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2)) return;
 #define LES_CASE(I, O)                                                                        \
   if (a.ny_in == I && a.ny_out == O)                                                          \
-    return launch_col_n<T, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
+    return launch_col_n<T, TXD, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
   LES_COL_PAIRS(LES_CASE)
 #undef LES_CASE
-  launch_col_n<T, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
+  launch_col_n<T, TXD, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
 }
 
 static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {

@@ -1,0 +1,65 @@
+"""The autotuner's launch shapes: every column-FFT tile width and tridiagonal
+schedule gives bit-identical states (the per-element arithmetic does not
+depend on them), and les_autotune picks one of the measured candidates."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "w", "p", "RHSx", "Cs_opt2", "Nu_t", "txz")
+
+
+def run(prec, tune=None, steps=12, conv=False, autotune=False):
+    from les_amd.api import LesSim, checkerboard_flux
+    kw = dict(scalars=dict(wt_s=checkerboard_flux(32, 32), dTdz_top=0.01)) if conv else {}
+    s = LesSim(32, 32, 17, precision=prec, dt=2e-4, **kw)
+    s.init_ic(seed=3)
+    if conv:
+        s.init_scalar_ic(seed=3)
+    rep = None
+    if tune:
+        s.set_tune(**tune)
+    if autotune:
+        rep = s.autotune(3)
+    s.step(steps)
+    return s, rep
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_launch_shapes_do_not_change_results(torch_mod, prec):
+    print("tune shapes", prec, flush=True)
+    base, _ = run(prec, conv=True)
+    ref = {n: base.get(n) for n in FIELDS + ("theta",)}
+    txs = (8, 32) if prec == "fp32" else (4, 16)
+    shapes = [dict(col_tx_fp32=tx if prec == "fp32" else 16, col_tx_fp64=tx if prec == "fp64" else 8)
+              for tx in txs]
+    shapes += [dict(tri_mode=m, tri_block=b) for (m, b) in ((0, 64), (0, 256), (1, 128), (2, 32))]
+    for sh in shapes:
+        s, _ = run(prec, tune=sh, conv=True)
+        assert s.get_tune()["tri_mode"] == sh.get("tri_mode", 0)
+        for n, a in ref.items():
+            assert np.array_equal(s.get(n), a), (sh, n)
+
+
+def test_autotune_chooses_a_measured_candidate(torch_mod):
+    print("autotune", flush=True)
+    base, _ = run("fp32", steps=15)
+    s, rep = run("fp32", steps=15, autotune=True)
+    print(rep)
+    ch = rep["chosen"]
+    t = s.get_tune()
+    assert str(t["col_tx_fp32"]) in rep["col_tx"]
+    assert f"{t['tri_mode']}/{t['tri_block']}" in rep["tridiag"]
+    assert ch["tri_mode"] == t["tri_mode"] and ch["tri_block"] == t["tri_block"]
+    assert all(v > 0 for v in rep["col_tx"].values()) and all(v > 0 for v in rep["tridiag"].values())
+    for n in FIELDS:
+        assert np.array_equal(s.get(n), base.get(n)), n
+
+
+def test_set_tune_rejects_bad_shapes(torch_mod):
+    from les_amd.api import Les, LesError
+    g = Les(16, 16, 9)
+    with pytest.raises(LesError):
+        g.set_tune(col_tx_fp32=12)
+    with pytest.raises(LesError):
+        g.set_tune(tri_mode=2, tri_block=1024)   # (nz+1) x 1024 x 24 B does not fit in LDS
