@@ -59,21 +59,23 @@ static FftPlan make_plan(int n) {
 
 // exp(-2 pi i k/n) for k < n (run-time-length kernels), then the
 // stage-major table of the compile-time-length kernels (les_fft.h
-// fft_tw_off): stage s of radix R after NS = R_0..R_{s-1} holds
+// fft_radix / fft_tw_off): stage s of radix R after NS = R_0..R_{s-1} holds
 // W^(r k n/(NS R)), r = 1..R-1, k < NS.  Computed in long double.
 template <typename T>
 static cx<T>* upload_twiddles(les_ctx* c, int n) {
-  std::vector<cx<T>> h(2 * n);
+  const size_t len = (size_t)(1 + FFT_NPLANS) * n;
+  std::vector<cx<T>> h(len);
   auto w = [&](long k) {
     long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
     return cx<T>{(T)cosl(a), (T)sinl(a)};
   };
   for (int k = 0; k < n; ++k) h[k] = w(k);
-  const FftPlan p = make_plan(n);
-  if (p.n > 0) {
-    int off = n, ns = 1;
-    for (int s = 0; s < p.ns; ++s) {
-      const int R = p.rad[s], tws = n / (ns * R);
+  for (int plan = 0; plan < FFT_NPLANS; ++plan) {  // the compile-time plans (les_fft.h)
+    const int nst = fft_nstages(n, plan);
+    if (nst <= 0) continue;
+    int off = n + plan * n, ns = 1;
+    for (int s = 0; s < nst; ++s) {
+      const int R = fft_radix(n, s, plan), tws = n / (ns * R);
       for (int r = 1; r < R; ++r)
         for (int k = 0; k < ns; ++k) h[off + (r - 1) * ns + k] = w((long)r * k * tws);
       off += (R - 1) * ns;
@@ -81,8 +83,8 @@ static cx<T>* upload_twiddles(les_ctx* c, int n) {
     }
   }
   cx<T>* d = nullptr;
-  check(c, hipMalloc(&d, 2 * n * sizeof(cx<T>)), "twiddle alloc");
-  check(c, hipMemcpy(d, h.data(), 2 * n * sizeof(cx<T>), hipMemcpyHostToDevice), "twiddle upload");
+  check(c, hipMalloc(&d, len * sizeof(cx<T>)), "twiddle alloc");
+  check(c, hipMemcpy(d, h.data(), len * sizeof(cx<T>), hipMemcpyHostToDevice), "twiddle upload");
   return d;
 }
 template <typename T>

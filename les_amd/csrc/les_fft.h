@@ -26,7 +26,7 @@ struct cx {
 };
 
 template <typename T>
-__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+__host__ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
@@ -38,7 +38,7 @@ struct FftPlan {
 
 // R-point DFT in registers, sign -1 (forward) or +1 (backward).
 template <typename T, int R>
-__device__ __forceinline__ void butterfly(cx<T> (&v)[R], int sign) {
+__host__ __device__ __forceinline__ void butterfly(cx<T> (&v)[R], int sign) {
   if constexpr (R == 2) {
     cx<T> a = v[0], b = v[1];
     v[0] = {a.x + b.x, a.y + b.y};
@@ -136,11 +136,13 @@ __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* src, int 
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 // Device twiddle arrays hold the base table exp(-2 pi i k/n), k < n (run-time
-// lengths), followed by the stage-major table (compile-time lengths).
+// lengths), then the stage-major table of each compile-time plan, n entries
+// apart (plan p at n + p n).
 template <typename T>
-__device__ __forceinline__ void load_twiddles_for(cx<T>* dst, const cx<T>* src, int n, bool stage_major) {
+__device__ __forceinline__ void load_twiddles_for(cx<T>* dst, const cx<T>* src, int n, bool stage_major,
+                                                  int plan = 0) {
   if (stage_major)
-    load_twiddles(dst, src + n, n - 1);
+    load_twiddles(dst, src + n + plan * n, n - 1);
   else
     load_twiddles(dst, src, n);
 }
@@ -150,19 +152,158 @@ __device__ __forceinline__ void load_twiddles_for(cx<T>* dst, const cx<T>* src, 
 namespace les {
 
 // ---- compile-time-length variant: every index is a constant expression ----
-constexpr int fft_nstages(int n) {
+// Two plans.  Plan 0: radix-4, -2, -3 Stockham passes.  Plan 1: two passes
+// through LDS, N = R0 * R1 with R0 = 16 (8 where that fits better) and
+// R1 <= 24, each an in-register R-point DFT (reg_dft); the first pass writes
+// at stride 16, which the lds_pad spare slot every 16 words keeps free of
+// bank conflicts, the second writes where it read.  Plan 1 moves 2-2.5x less
+// data through LDS but gives one transform fewer threads: it pays where a
+// block holds many transforms (k_pad_products), plan 0 elsewhere.
+__host__ __device__ constexpr bool fft_smooth(int n) {
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  return n == 1;
+}
+__host__ __device__ constexpr int fft_first_radix(int n) {
+  if (n <= 24) return n;
+  if (n % 16 == 0 && n / 16 <= 24) return 16;
+  if (n % 8 == 0 && n / 8 <= 24) return 8;
+  return 0;  // no two-pass plan
+}
+// plan 0: radix 4, then 2, then 3 passes (many threads per transform, short
+// register DFTs); plan 1: the two-pass plan (for lengths without one, plan 0)
+__host__ __device__ constexpr int fft_nstages(int n, int plan = 0) {
+  if (!fft_smooth(n) || n < 2) return -1;
+  const int r0 = plan == 1 ? fft_first_radix(n) : 0;
+  if (r0 == n) return 1;
+  if (r0 > 0) return 2;
   int s = 0;
   while (n % 4 == 0) { n /= 4; ++s; }
   while (n % 2 == 0) { n /= 2; ++s; }
   while (n % 3 == 0) { n /= 3; ++s; }
-  return n == 1 ? s : -1;
+  return s;
 }
-constexpr int fft_radix(int n, int s) {
+__host__ __device__ constexpr int fft_radix(int n, int s, int plan = 0) {
+  const int r0 = plan == 1 ? fft_first_radix(n) : 0;
+  if (r0 > 0) return s == 0 ? r0 : (s == 1 && r0 < n ? n / r0 : 1);
   int i = 0;
   while (n % 4 == 0) { if (i == s) return 4; n /= 4; ++i; }
   while (n % 2 == 0) { if (i == s) return 2; n /= 2; ++i; }
   while (n % 3 == 0) { if (i == s) return 3; n /= 3; ++i; }
   return 1;
+}
+// most butterflies per transform over the passes (threads one transform uses)
+__host__ __device__ constexpr int fft_max_nb(int n, int plan = 0) {
+  int m = 0;
+  for (int s = 0; s < fft_nstages(n, plan); ++s)
+    m = n / fft_radix(n, s, plan) > m ? n / fft_radix(n, s, plan) : m;
+  return m;
+}
+constexpr int FFT_NPLANS = 2;
+
+// cos(2 pi j / 144) (all the in-register DFT sizes divide 144), from a
+// quarter-period table evaluated to 22 digits
+__host__ __device__ constexpr double dft_cos144_q(int j) {
+  switch (j) {
+    case 0: return 1.0000000000000000000000e+0;
+    case 1: return 9.9904822158185776240372e-1;
+    case 2: return 9.9619469809174553229501e-1;
+    case 3: return 9.9144486137381041114456e-1;
+    case 4: return 9.8480775301220805936674e-1;
+    case 5: return 9.7629600711993336597089e-1;
+    case 6: return 9.6592582628906828674974e-1;
+    case 7: return 9.5371695074822692114385e-1;
+    case 8: return 9.3969262078590838405411e-1;
+    case 9: return 9.2387953251128675612818e-1;
+    case 10: return 9.0630778703664996324255e-1;
+    case 11: return 8.8701083317822170105461e-1;
+    case 12: return 8.6602540378443864676372e-1;
+    case 13: return 8.4339144581288570127286e-1;
+    case 14: return 8.1915204428899178968449e-1;
+    case 15: return 7.9335334029123516457978e-1;
+    case 16: return 7.6604444311897803520239e-1;
+    case 17: return 7.3727733681012404138429e-1;
+    case 18: return 7.0710678118654752440084e-1;
+    case 19: return 6.7559020761566024434834e-1;
+    case 20: return 6.4278760968653932632264e-1;
+    case 21: return 6.0876142900872063941610e-1;
+    case 22: return 5.7357643635104609610803e-1;
+    case 23: return 5.3729960834682383184079e-1;
+    case 24: return 5.0000000000000000000000e-1;
+    case 25: return 4.6174861323503393056293e-1;
+    case 26: return 4.2261826174069943618698e-1;
+    case 27: return 3.8268343236508977172846e-1;
+    case 28: return 3.4202014332566873304410e-1;
+    case 29: return 3.0070579950427312162255e-1;
+    case 30: return 2.5881904510252076234890e-1;
+    case 31: return 2.1643961393810287975955e-1;
+    case 32: return 1.7364817766693034885172e-1;
+    case 33: return 1.3052619222005159154841e-1;
+    case 34: return 8.7155742747658173558064e-2;
+    case 35: return 4.3619387365335999781753e-2;
+    case 36: return 0.0;
+    default: return 0.0;
+  }
+}
+__host__ __device__ constexpr double dft_cos144(int j) {
+  j %= 144;
+  if (j < 0) j += 144;
+  if (j <= 36) return dft_cos144_q(j);
+  if (j <= 72) return -dft_cos144_q(72 - j);
+  if (j <= 108) return -dft_cos144_q(j - 72);
+  return dft_cos144_q(144 - j);
+}
+__host__ __device__ constexpr double dft_sin144(int j) { return dft_cos144(j - 36); }
+
+// a * W_R^m, W_R = exp(sign 2 pi i / R); quarter turns exactly
+template <typename T, int R>
+__host__ __device__ __forceinline__ cx<T> twiddle_const(cx<T> a, int m, int sign) {
+  m %= R;
+  if (m == 0) return a;
+  if (4 * m == R) return sign < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
+  if (2 * m == R) return cx<T>{-a.x, -a.y};
+  if (4 * m == 3 * R) return sign < 0 ? cx<T>{-a.y, a.x} : cx<T>{a.y, -a.x};
+  const int j = m * (144 / R);
+  const T c = (T)dft_cos144(j), sn = (T)(sign < 0 ? -dft_sin144(j) : dft_sin144(j));
+  return cx<T>{a.x * c - a.y * sn, a.x * sn + a.y * c};
+}
+
+// R = P Q split of the in-register DFT sizes
+template <int R>
+struct DftSplit {
+  static constexpr int P = R == 6 ? 2 : R == 8 ? 4 : R == 9 ? 3 : R == 12 ? 4 : R == 16 ? 4 : R == 18 ? 6 : R == 24 ? 8 : R;
+};
+
+// R-point DFT in registers: radix 2/3/4 butterflies, larger R by one
+// Cooley-Tukey split R = P Q (n = Q n1 + n2, k = k1 + P k2): P-point DFTs over
+// n1, the twiddles W_R^(n2 k1), Q-point DFTs over n2
+template <typename T, int R>
+__host__ __device__ __forceinline__ void reg_dft(cx<T> (&v)[R], int sign) {
+  if constexpr (R == 2 || R == 3 || R == 4) {
+    butterfly<T, R>(v, sign);
+  } else if constexpr (R > 4) {
+    constexpr int P = DftSplit<R>::P, Q = R / P;
+    static_assert(P > 1 && Q > 1 && P * Q == R, "no split for this DFT size");
+    cx<T> y[Q][P];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+      cx<T> t[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) t[n1] = v[Q * n1 + n2];
+      reg_dft<T, P>(t, sign);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) y[n2][k1] = twiddle_const<T, R>(t[k1], n2 * k1, sign);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      cx<T> t[Q];
+#pragma unroll
+      for (int n2 = 0; n2 < Q; ++n2) t[n2] = y[n2][k1];
+      reg_dft<T, Q>(t, sign);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = t[k2];
+    }
+  }
 }
 
 // Stage-major twiddle table of the compile-time transforms: stage s with
@@ -171,10 +312,10 @@ constexpr int fft_radix(int n, int s) {
 // all).  Lanes that work on consecutive k read consecutive words, where a
 // single base table exp(-2 pi i j/N) indexed by r k N/(NS R) makes them
 // stride through the LDS banks.
-constexpr int fft_tw_off(int n, int s) {
+constexpr int fft_tw_off(int n, int s, int plan = 0) {
   int off = 0, ns = 1;
   for (int i = 0; i < s; ++i) {
-    const int r = fft_radix(n, i);
+    const int r = fft_radix(n, i, plan);
     off += (r - 1) * ns;
     ns *= r;
   }
@@ -219,7 +360,7 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
         }
         v[i][r] = a;
       }
-      butterfly<T, R>(v[i], sign);
+      reg_dft<T, R>(v[i], sign);
       tt[i] = t;
       dd[i] = (b / NS) * NS * R + k;
     }
@@ -236,22 +377,22 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
   __syncthreads();
 }
 
-template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD>
+template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int PLAN>
 __device__ __forceinline__ void fft_stages_c(cx<T>* buf, const cx<T>* tw, int sign) {
-  if constexpr (S < fft_nstages(N)) {
-    constexpr int R = fft_radix(N, S);
-    fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T, PAD, fft_tw_off(N, S)>(buf, tw, sign);
-    fft_stages_c<T, N, S + 1, NS * R, ES, TS, NTR, BLOCK, LANE_T, PAD>(buf, tw, sign);
+  if constexpr (S < fft_nstages(N, PLAN)) {
+    constexpr int R = fft_radix(N, S, PLAN);
+    fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T, PAD, fft_tw_off(N, S, PLAN)>(buf, tw, sign);
+    fft_stages_c<T, N, S + 1, NS * R, ES, TS, NTR, BLOCK, LANE_T, PAD, PLAN>(buf, tw, sign);
   }
 }
 
-// tw: the stage-major table of N (fft_tw_off).
 // PAD (with ES = 1, TS = lds_row(N)): padded contiguous transforms
-template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD = false>
+// tw: the stage-major table of N for PLAN (load_twiddles_for)
+template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD = false, int PLAN = 0>
 __device__ __forceinline__ void fft_lds_c(cx<T>* buf, const cx<T>* tw, int sign) {
-  static_assert(fft_nstages(N) > 0, "FFT length must be 2^a 3^b");
+  static_assert(fft_nstages(N, PLAN) > 0, "FFT length must be 2^a 3^b");
   static_assert(!PAD || (ES == 1 && TS >= lds_row(N)), "padded transforms are contiguous");
-  fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T, PAD>(buf, tw, sign);
+  fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T, PAD, PLAN>(buf, tw, sign);
 }
 
 }  // namespace les

@@ -56,6 +56,8 @@ static const FftPlan& plan_for(const les_ctx* c, int n) {
 // blockIdx.x = group of P row pairs inside it, so loaders and storers get a
 // block-uniform plane index (no per-element integer division).  N > 0 is the
 // compile-time transform length; N == 0 runs the same code with run-time n.
+// TPF threads per transform (n/4: plan 0 passes, radix-2 passes take two
+// butterflies per thread), P transforms per block
 template <int N>
 struct RowGeom {
   static constexpr int TPF = N > 0 ? N / 4 : 64;
@@ -181,10 +183,15 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
 template <typename T, int N, class L>
 static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo) {
   const FftPlan& p = plan_for(c, n);
-  const int tpf = n / 4;
-  int P = 256 / tpf;
-  if (P < 1) P = 1;
-  const int threads = P * tpf;
+  int P = 1, threads = 64;
+  if constexpr (N > 0) {
+    P = RowGeom<N>::P;
+    threads = RowGeom<N>::BLOCK;
+  } else {  // run-time length: radix-4/2/3 passes, n/4 threads per transform
+    const int tpf = n / 4;
+    P = 256 / tpf > 0 ? 256 / tpf : 1;
+    threads = P * tpf;
+  }
   const long blocks = (ny / 2 + P - 1) / P;
   const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
@@ -211,10 +218,15 @@ template <typename T, int N, class S>
 static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
                              int n, int ldo, const cx<T>* in2, T c2) {
   const FftPlan& p = plan_for(c, n);
-  const int tpf = n / 4;
-  int P = 256 / tpf;
-  if (P < 1) P = 1;
-  const int threads = P * tpf;
+  int P = 1, threads = 64;
+  if constexpr (N > 0) {
+    P = RowGeom<N>::P;
+    threads = RowGeom<N>::BLOCK;
+  } else {  // run-time length: radix-4/2/3 passes, n/4 threads per transform
+    const int tpf = n / 4;
+    P = 256 / tpf > 0 ? 256 / tpf : 1;
+    threads = P * tpf;
+  }
   const long blocks = (ny / 2 + P - 1) / P;
   const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_inv",
@@ -640,7 +652,7 @@ __global__ void __launch_bounds__(BLOCK)
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* win = tw + n;              // [NL][NP][rs]: field pairs, physical after the inverse FFT
   cx<T>* out = win + NL * NP * rs;  // [CH][NOP][rs]: product pairs
-  load_twiddles_for(tw, twg, n, N > 0);
+  load_twiddles_for(tw, twg, n, N > 0, 1);
   const int yb = blockIdx.x;
   const int nl = nz - 1;
   const int k0 = blockIdx.y * CH;
@@ -667,7 +679,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, NB, 1, RS, NL * NP, BLOCK, false, true>(win, tw, +1);
+    fft_lds_c<T, NB, 1, RS, NL * NP, BLOCK, true, true, 1>(win, tw, +1);
   else
     fft_lds<T, false, 8>(win, p, 1, n, NL * NP, tw, +1);
   for (int e = threadIdx.x; e < CH * n; e += BLOCK) {
@@ -687,7 +699,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, NB, 1, RS, CH * NOP, BLOCK, false, true>(out, tw, -1);
+    fft_lds_c<T, NB, 1, RS, CH * NOP, BLOCK, true, true, 1>(out, tw, -1);
   else
     fft_lds<T, false, 8>(out, p, 1, n, CH * NOP, tw, -1);
   for (int e = threadIdx.x; e < CH * NO * nxh; e += BLOCK) {
