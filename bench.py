@@ -15,6 +15,8 @@ is reported beside it.  Inputs are resident in HBM before the timed region.
 `learned_closure`: the same FP32-bulk step with the MLP closure (lagrange_Sdep
 every 10 steps, the BF16-MFMA network at the other updates), after training
 the network on the GPU on 2e6 samples harvested from this workload.
+`ensemble`: 24 autotuned instances on three uneven grids advanced by one HIP
+graph per window (aggregate grid-point-steps/s, and per-member launches).
 """
 import argparse
 import ctypes
@@ -37,7 +39,8 @@ SCALARS = dict(enable=1, model=7, initsc=0, passive=0, Pr=0.55, z_i=1000.0, u_st
 SCALAR_IC = dict(T_init=283.0, z_inv=0.8, inv_strength=0.01, amp_K=0.1, q_init=1.0)
 
 
-def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False, conv=True):
+def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False, conv=True,
+            tune=True):
     import torch
     from les_amd.api import LesSim, checkerboard_flux
     scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
@@ -45,6 +48,7 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
+    tuned = sim.autotune(3)["chosen"] if tune else None  # launch shapes only, results unchanged
     sim.set_graph(False)
     sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
     sim.set_graph(graph)
@@ -80,7 +84,20 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     finite = math.isfinite(rms.value) and all(math.isfinite(x) for x in out4)
     del sim
     torch.cuda.empty_cache()
-    return ms, prof, finite, rms.value, list(out4), fields
+    return ms, prof, finite, rms.value, list(out4), fields, tuned
+
+
+def measured_traffic(tag, nx, ny, nz):
+    """HBM bytes per dispatch of the kernel behind a profiling tag, from the
+    committed rocprofv3 --pmc summary of this grid (None if there is none)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{nx}x{ny}x{nz - 1}_fp32.json")
+    if not os.path.exists(path):
+        return None, None
+    d = json.load(open(path))
+    k = d.get("tags", {}).get(tag)
+    if not k or k not in d.get("kernels", {}):
+        return None, None
+    return d["kernels"][k]["hbm_bytes_per_dispatch"], os.path.relpath(path, ROOT)
 
 
 def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train_steps=2000, period=10,
@@ -96,6 +113,7 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
+    sim.autotune(3)
     sim.mlp_init(1)
     nharv = 4
     gen = torch.Generator(device="cuda").manual_seed(0)
@@ -157,6 +175,70 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     return out
 
 
+def run_ensemble(steps, warmup, n_each=8, grids=((64, 64, 49), (96, 96, 65), (128, 128, 65))):
+    """24 independent instances, 8 each on three uneven grids, z0 log-uniform
+    in [1e-5, 1e-3], seeds 0..23, odd members stably stratified (surface
+    cooling); FP32-bulk; every member autotuned on its own grid; all advanced
+    by one HIP graph per window.  Beside it: the same members each replaying
+    its own graph on its own stream (one launch per member per window)."""
+    import numpy as np
+    import torch
+    from les_amd.api import LesEnsemble, LesSim
+    rng = np.random.default_rng(2024)
+    members, pts, tunes = [], 0, {}
+    i = 0
+    for (nx, ny, nz) in grids:
+        for _ in range(n_each):
+            z0 = float(10 ** rng.uniform(-5, -3))
+            stable = i % 2 == 1
+            kw = dict(scalars=dict(wt_s=np.full((ny, nx), -0.01), dTdz_top=0.003)) if stable else {}
+            m = LesSim(nx, ny, nz, precision="fp32", dt=1e-4, z0=z0, graph=True, **kw)
+            m.init_ic(seed=i)
+            if stable:
+                m.init_scalar_ic(seed=i, amp_K=0.05)
+            if (nx, stable) not in tunes:
+                tunes[(nx, stable)] = m.autotune(3)["chosen"]
+            else:
+                ch = tunes[(nx, stable)]
+                m.set_tune(col_tx_fp32=ch["col_tx"][0], col_tx_fp64=ch["col_tx"][1], tri_mode=ch["tri_mode"],
+                           tri_block=ch["tri_block"])
+            members.append(m)
+            pts += nx * ny * (nz - 1)
+            i += 1
+    ens = LesEnsemble(members)
+    W = ens.window
+    ens.step(max(warmup, W) + W)   # eager warm-up, then the first capture
+    ens.sync()
+    K = max(W, steps // W * W)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(K)
+    ens.sync()
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    for m in members:               # each member's own graph, captured outside the timing
+        m.graph_prepare()
+    for m in members:
+        m.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in members:
+        m.step(K)
+    for m in members:
+        m.sync()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    ok = all(np.isfinite(m.get("u")).all() for m in members)
+    ens.close()
+    del members
+    torch.cuda.empty_cache()
+    return {"instances": len(grids) * n_each, "grids": [list(g) for g in grids], "steps": K,
+            "value": pts * K / t_ens, "unit": "grid-point-steps/s (all instances)",
+            "ms_per_step": t_ens * 1e3 / K, "per_member_launch_value": pts * K / t_seq,
+            "per_member_launch_ms_per_step": t_seq * 1e3 / K, "finite": ok,
+            "tune": {f"{k[0]}{'_stable' if k[1] else ''}": v for k, v in tunes.items()}}
+
+
 def fp32_error(f32, f64, nx, nz):
     """Per-field relative L2 error of the FP32-bulk state against the FP64
     state after the same steps, and the worst plane-mean profile error."""
@@ -211,6 +293,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", choices=("convective", "neutral"), default="convective")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure run")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of les_autotune")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-instance ensemble run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -221,14 +305,15 @@ def main():
 
     graph = not args.no_graph
     conv = args.workload == "convective"
-    ms32, prof32, ok32, rms32, cfl32, f32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
-                                                    keep=not args.no_fp64, conv=conv)
+    ms32, prof32, ok32, rms32, cfl32, f32, tune32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
+                                                            keep=not args.no_fp64, conv=conv, tune=not args.no_tune)
     print(f"fp32-bulk: {ms32:.3f} ms/step finite={ok32} rmsdiv={rms32:.3e} cfl={cfl32[0]:.3e}",
           file=sys.stderr, flush=True)
     ms64, err = None, None
     if not args.no_fp64:
-        ms64, prof64, ok64, rms64, cfl64, f64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
-                                                        extra_steps=args.profile_steps, keep=True, conv=conv)
+        ms64, prof64, ok64, rms64, cfl64, f64, tune64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
+                                                                extra_steps=args.profile_steps, keep=True, conv=conv,
+                                                                tune=not args.no_tune)
         print(f"fp64: {ms64:.3f} ms/step finite={ok64} rmsdiv={rms64:.3e}", file=sys.stderr, flush=True)
         err = fp32_error(f32, f64, nx, nz)
         err["steps"] = W + K + args.profile_steps
@@ -243,6 +328,13 @@ def main():
               f"in use {closure['network_in_use']} (train {closure['train_ms']:.0f} ms, loss "
               f"{closure['loss_first']:.3g} -> {closure['loss_last']:.3g})", file=sys.stderr, flush=True)
 
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = run_ensemble(K, W)
+        print(f"ensemble: {ensemble['instances']} instances {ensemble['ms_per_step']:.3f} ms/step "
+              f"({ensemble['value']:.3e} pts/s; per-member launches {ensemble['per_member_launch_value']:.3e})",
+              file=sys.stderr, flush=True)
+
     value = pts / (ms32 * 1e-3)
     roof = None
     if prof32:
@@ -255,8 +347,9 @@ def main():
         per_launch_bytes = v[2] / v[0]
         avg_s = v[1] * 1e-3 / v[0]
         achieved = per_launch_bytes / avg_s / 1e9
+        traffic, tsrc = measured_traffic(dom, nx, ny, nz)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc, "kernel": dom,
                 "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3,
                 "profiled_ms_per_step": tot}
     cpu = None
@@ -278,7 +371,8 @@ def main():
                                (f"neutral ABL {nx}x{ny}x{nz - 1} (nz={nz}), Lagrangian scale-dependent SGS "
                                 "(model 5), log-law wall z0=1e-4, forcing 1, dt=1e-4, seeded IC"),
                    "grid": [nx, ny, nz], "parallelism": "1 GPU, whole domain",
-                   "launch": "HIP graph replay of 10-step windows" if graph else "eager"},
+                   "launch": "HIP graph replay of 10-step windows" if graph else "eager",
+                   "autotuned": tune32},
         "fp64_value": (pts / (ms64 * 1e-3)) if ms64 else None,
         "fp64_ms_per_step": ms64,
         "fp32_over_fp64": (ms64 / ms32) if ms64 else None,
@@ -286,6 +380,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned_closure": closure,
+        "ensemble": ensemble,
     }
     print(json.dumps(line), flush=True)
 
