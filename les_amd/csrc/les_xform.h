@@ -679,7 +679,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, NB, 1, RS, NL * NP, BLOCK, true, true, 1>(win, tw, +1);
+    fft_lds_c<T, NB, 1, RS, NL * NP, BLOCK, false, true, 1>(win, tw, +1);
   else
     fft_lds<T, false, 8>(win, p, 1, n, NL * NP, tw, +1);
   for (int e = threadIdx.x; e < CH * n; e += BLOCK) {
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, NB, 1, RS, CH * NOP, BLOCK, true, true, 1>(out, tw, -1);
+    fft_lds_c<T, NB, 1, RS, CH * NOP, BLOCK, false, true, 1>(out, tw, -1);
   else
     fft_lds<T, false, 8>(out, p, 1, n, CH * NOP, tw, -1);
   for (int e = threadIdx.x; e < CH * NO * nxh; e += BLOCK) {
