@@ -199,9 +199,7 @@ def run_ensemble(steps, warmup, n_each=8, grids=((64, 64, 49), (96, 96, 65), (12
             if (nx, stable) not in tunes:
                 tunes[(nx, stable)] = m.autotune(3)["chosen"]
             else:
-                ch = tunes[(nx, stable)]
-                m.set_tune(col_tx_fp32=ch["col_tx"][0], col_tx_fp64=ch["col_tx"][1], tri_mode=ch["tri_mode"],
-                           tri_block=ch["tri_block"])
+                m.set_tune_choice(tunes[(nx, stable)])
             members.append(m)
             pts += nx * ny * (nz - 1)
             i += 1

@@ -206,14 +206,21 @@ class Les:
 
     def get_tune(self):
         import numpy as np
-        a = np.zeros(4, dtype=np.int32)
+        a = np.zeros(6, dtype=np.int32)
         self._chk(self.L.les_get_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
-        return dict(col_tx_fp32=int(a[0]), col_tx_fp64=int(a[1]), tri_mode=int(a[2]), tri_block=int(a[3]))
+        return dict(col_tx_fp32=int(a[0]), col_tx_fp64=int(a[1]), tri_mode=int(a[2]), tri_block=int(a[3]),
+                    col_plan_fp32=int(a[4]), col_plan_fp64=int(a[5]))
 
-    def set_tune(self, col_tx_fp32=16, col_tx_fp64=8, tri_mode=0, tri_block=128):
+    def set_tune(self, col_tx_fp32=16, col_tx_fp64=8, tri_mode=0, tri_block=128, col_plan_fp32=0, col_plan_fp64=0):
         import numpy as np
-        a = np.array([col_tx_fp32, col_tx_fp64, tri_mode, tri_block], dtype=np.int32)
+        a = np.array([col_tx_fp32, col_tx_fp64, tri_mode, tri_block, col_plan_fp32, col_plan_fp64], dtype=np.int32)
         self._chk(self.L.les_set_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
+
+    def set_tune_choice(self, chosen):
+        """Apply the 'chosen' entry of an autotune report."""
+        self.set_tune(col_tx_fp32=chosen["col_tx"][0], col_tx_fp64=chosen["col_tx"][1],
+                      tri_mode=chosen["tri_mode"], tri_block=chosen["tri_block"],
+                      col_plan_fp32=chosen["col_plan"][0], col_plan_fp64=chosen["col_plan"][1])
 
     # ---- learned closure (no counterpart in the reference code) ----
     MLP_NIN, MLP_NOUT = 8, 2

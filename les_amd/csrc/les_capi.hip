@@ -1151,24 +1151,31 @@ static void tune_run(les_ctx* c, int what, std::string& rep) {
     T* A = (T*)scratch(c, S_TUNE0, n3 * sizeof(T));
     T* B = (T*)scratch(c, S_TUNE1, n3 * sizeof(T));
     T* Bg = (T*)scratch(c, S_TUNE2, nbig * sizeof(T));
+    // (tile width, plan) candidates: plan 0 at three widths, plan 2 at two
     const int p = sizeof(T) == 8 ? 1 : 0;
-    const int c32[3] = {8, 16, 32}, c64[3] = {4, 8, 16};
-    const int* cand = p ? c64 : c32;
-    int best = c->tune.col_tx[p];
+    const int c32[5][2] = {{8, 0}, {16, 0}, {32, 0}, {8, 2}, {16, 2}};
+    const int c64[5][2] = {{4, 0}, {8, 0}, {16, 0}, {4, 2}, {8, 2}};
+    int best_tx = c->tune.col_tx[p], best_plan = c->tune.col_plan[p];
     double tbest = 1e30;
-    rep += "\"col_tx\": {";
-    for (int i = 0; i < 3 && c->err.empty(); ++i) {
-      c->tune.col_tx[p] = cand[i];
+    rep += "\"col\": {";
+    for (int i = 0; i < 5 && c->err.empty(); ++i) {
+      c->tune.col_tx[p] = p ? c64[i][0] : c32[i][0];
+      c->tune.col_plan[p] = p ? c64[i][1] : c32[i][1];
       const double ms = tune_time(c, [&] {
         op_ddxy<T>(c, A, B, u, nz);
         op_dealias1<T>(c, u, Bg, nz);
         op_dealias2<T>(c, A, Bg, nz);
       });
-      std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", i ? ", " : "", cand[i], ms);
+      std::snprintf(buf, sizeof buf, "%s\"%d/%d\": %.4f", i ? ", " : "", c->tune.col_tx[p], c->tune.col_plan[p], ms);
       rep += buf;
-      if (ms < tbest) { tbest = ms; best = cand[i]; }
+      if (ms < tbest) {
+        tbest = ms;
+        best_tx = c->tune.col_tx[p];
+        best_plan = c->tune.col_plan[p];
+      }
     }
-    c->tune.col_tx[p] = best;
+    c->tune.col_tx[p] = best_tx;
+    c->tune.col_plan[p] = best_plan;
     rep += "}";
     scratch_release(c, S_TUNE0);
     scratch_release(c, S_TUNE1);
@@ -1215,8 +1222,10 @@ static void tune_run(les_ctx* c, int what, std::string& rep) {
     scratch_release(c, S_TUNE1);
     scratch_release(c, S_TUNE2);
   }
-  std::snprintf(buf, sizeof buf, ", \"chosen\": {\"col_tx\": [%d, %d], \"tri_mode\": %d, \"tri_block\": %d}}",
-                c->tune.col_tx[0], c->tune.col_tx[1], c->tune.tri_mode, c->tune.tri_block);
+  std::snprintf(buf, sizeof buf,
+                ", \"chosen\": {\"col_tx\": [%d, %d], \"col_plan\": [%d, %d], \"tri_mode\": %d, \"tri_block\": %d}}",
+                c->tune.col_tx[0], c->tune.col_tx[1], c->tune.col_plan[0], c->tune.col_plan[1], c->tune.tri_mode,
+                c->tune.tri_block);
   rep += buf;
 }
 }  // extern "C++"
@@ -1237,26 +1246,32 @@ int les_autotune(les_ctx* c, int what, char* report, int report_len) {
   }
   DONE(c);
 }
-int les_get_tune(les_ctx* c, int* out4) {
+int les_get_tune(les_ctx* c, int* out6) {
   GUARD(c);
-  if (!out4) return -1;
-  out4[0] = c->tune.col_tx[0];
-  out4[1] = c->tune.col_tx[1];
-  out4[2] = c->tune.tri_mode;
-  out4[3] = c->tune.tri_block;
+  if (!out6) return -1;
+  out6[0] = c->tune.col_tx[0];
+  out6[1] = c->tune.col_tx[1];
+  out6[2] = c->tune.tri_mode;
+  out6[3] = c->tune.tri_block;
+  out6[4] = c->tune.col_plan[0];
+  out6[5] = c->tune.col_plan[1];
   DONE(c);
 }
-int les_set_tune(les_ctx* c, const int* in4) {
+int les_set_tune(les_ctx* c, const int* in6) {
   GUARD(c);
-  if (!in4) return -1;
-  const bool ok32 = in4[0] == 8 || in4[0] == 16 || in4[0] == 32, ok64 = in4[1] == 4 || in4[1] == 8 || in4[1] == 16;
-  const bool okt = in4[2] >= 0 && in4[2] <= 2 && in4[3] >= 32 && in4[3] <= 1024 && in4[3] % 32 == 0 &&
-                   tridiag_lds(in4[2], in4[3], c->g.nz) <= 160 * 1024;
-  if (!ok32 || !ok64 || !okt) { c->err = "les_set_tune: launch shape out of range"; return 1; }
-  c->tune.col_tx[0] = in4[0];
-  c->tune.col_tx[1] = in4[1];
-  c->tune.tri_mode = in4[2];
-  c->tune.tri_block = in4[3];
+  if (!in6) return -1;
+  const bool ok32 = in6[0] == 8 || in6[0] == 16 || in6[0] == 32, ok64 = in6[1] == 4 || in6[1] == 8 || in6[1] == 16;
+  const bool okt = in6[2] >= 0 && in6[2] <= 2 && in6[3] >= 32 && in6[3] <= 1024 && in6[3] % 32 == 0 &&
+                   tridiag_lds(in6[2], in6[3], c->g.nz) <= 160 * 1024;
+  const bool okp = (in6[4] == 0 || (in6[4] == 2 && (in6[0] == 8 || in6[0] == 16))) &&
+                   (in6[5] == 0 || (in6[5] == 2 && (in6[1] == 4 || in6[1] == 8)));
+  if (!ok32 || !ok64 || !okt || !okp) { c->err = "les_set_tune: launch shape out of range"; return 1; }
+  c->tune.col_tx[0] = in6[0];
+  c->tune.col_tx[1] = in6[1];
+  c->tune.tri_mode = in6[2];
+  c->tune.tri_block = in6[3];
+  c->tune.col_plan[0] = in6[4];
+  c->tune.col_plan[1] = in6[5];
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;

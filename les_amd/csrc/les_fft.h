@@ -170,13 +170,33 @@ __host__ __device__ constexpr int fft_first_radix(int n) {
   if (n % 8 == 0 && n / 8 <= 24) return 8;
   return 0;  // no two-pass plan
 }
+// in-register DFT sizes (reg_dft)
+__host__ __device__ constexpr bool fft_reg_size(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 6 || r == 8 || r == 9 || r == 12 || r == 16 || r == 18 || r == 24;
+}
+// plan 2: radix-8 passes while more than 24 points remain, then one pass of
+// the rest (256 = 8 8 4, 384 = 8 8 6): the thread count of plan 0 with one
+// LDS pass in two or three fewer
+__host__ __device__ constexpr int fft_r8_stages(int n) {
+  int s = 0;
+  while (n > 24 && n % 8 == 0) { n /= 8; ++s; }
+  if (n == 1) return s;
+  return fft_reg_size(n) ? s + 1 : -1;
+}
+__host__ __device__ constexpr int fft_r8_radix(int n, int s) {
+  int i = 0;
+  while (n > 24 && n % 8 == 0) { if (i == s) return 8; n /= 8; ++i; }
+  return i == s ? n : 1;
+}
 // plan 0: radix 4, then 2, then 3 passes (many threads per transform, short
-// register DFTs); plan 1: the two-pass plan (for lengths without one, plan 0)
+// register DFTs); plan 1: the two-pass plan; plan 2: radix-8 passes.  A
+// length without a plan-1/2 decomposition uses plan 0.
 __host__ __device__ constexpr int fft_nstages(int n, int plan = 0) {
   if (!fft_smooth(n) || n < 2) return -1;
   const int r0 = plan == 1 ? fft_first_radix(n) : 0;
   if (r0 == n) return 1;
   if (r0 > 0) return 2;
+  if (plan == 2 && fft_r8_stages(n) > 0) return fft_r8_stages(n);
   int s = 0;
   while (n % 4 == 0) { n /= 4; ++s; }
   while (n % 2 == 0) { n /= 2; ++s; }
@@ -186,6 +206,7 @@ __host__ __device__ constexpr int fft_nstages(int n, int plan = 0) {
 __host__ __device__ constexpr int fft_radix(int n, int s, int plan = 0) {
   const int r0 = plan == 1 ? fft_first_radix(n) : 0;
   if (r0 > 0) return s == 0 ? r0 : (s == 1 && r0 < n ? n / r0 : 1);
+  if (plan == 2 && fft_r8_stages(n) > 0) return fft_r8_radix(n, s);
   int i = 0;
   while (n % 4 == 0) { if (i == s) return 4; n /= 4; ++i; }
   while (n % 2 == 0) { if (i == s) return 2; n /= 2; ++i; }
@@ -199,7 +220,7 @@ __host__ __device__ constexpr int fft_max_nb(int n, int plan = 0) {
     m = n / fft_radix(n, s, plan) > m ? n / fft_radix(n, s, plan) : m;
   return m;
 }
-constexpr int FFT_NPLANS = 2;
+constexpr int FFT_NPLANS = 3;
 
 // cos(2 pi j / 144) (all the in-register DFT sizes divide 144), from a
 // quarter-period table evaluated to 22 digits

@@ -41,9 +41,10 @@ struct Arena {
   std::vector<size_t> cap;
 };
 
-// Launch shapes chosen by les_autotune (results do not depend on them).
+// Launch shapes chosen by les_autotune (only the FFT plan changes rounding).
 struct Tune {
   int col_tx[2] = {16, 8};  // column-FFT tile width (kx columns per block): FP32, FP64
+  int col_plan[2] = {0, 0}; // column-FFT plan (les_fft.h): 0 radix-4 passes, 2 radix-8 passes
   int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS
   int tri_block = 128;      // horizontal modes per block of the tridiagonal solve
 };

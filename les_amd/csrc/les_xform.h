@@ -318,6 +318,8 @@ __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
   return yo + nyi - nyo;
 }
 
+// block: TX * n/8 threads (plans 0 and 2: at most two butterflies per thread
+// in any pass)
 template <int TX, int NYI, int NYO>
 struct ColGeom {
   static constexpr int NMAX = NYI > NYO ? NYI : NYO;
@@ -326,7 +328,7 @@ struct ColGeom {
 };
 
 // NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
-template <typename T, int TX, int NYI, int NYO>
+template <typename T, int TX, int NYI, int NYO, int PLAN = 0>
 __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
     k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
           FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
@@ -342,17 +344,19 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
   cx<T>* two = twi + nyi;
   cx<T>* A = two + nyo;
   cx<T>* B = A + nyi * TX;
-  for (int i = threadIdx.x; i < nyi; i += bd) twi[i] = CT ? (i < nyi - 1 ? twin[nyi + i] : cx<T>{}) : twin[i];
-  for (int i = threadIdx.x; i < nyo; i += bd) two[i] = CT ? (i < nyo - 1 ? twout[nyo + i] : cx<T>{}) : twout[i];
+  for (int i = threadIdx.x; i < nyi; i += bd)
+    twi[i] = CT ? (i < nyi - 1 ? twin[nyi + PLAN * nyi + i] : cx<T>{}) : twin[i];
+  for (int i = threadIdx.x; i < nyo; i += bd)
+    two[i] = CT ? (i < nyo - 1 ? twout[nyo + PLAN * nyo + i] : cx<T>{}) : twout[i];
   auto fwd = [&](cx<T>* buf) {
     if constexpr (CT)
-      fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, twi, -1);
+      fft_lds_c<T, (CT ? NYI : 4), TX, 1, TX, (CT ? BLK : 64), true, false, PLAN>(buf, twi, -1);
     else
       fft_lds<T, true, 4>(buf, pin, TX, 1, TX, twi, -1);
   };
   auto inv = [&](cx<T>* buf) {
     if constexpr (CT)
-      fft_lds_c<T, (CT ? NYO : 4), TX, 1, TX, (CT ? BLK : 64), true>(buf, two, +1);
+      fft_lds_c<T, (CT ? NYO : 4), TX, 1, TX, (CT ? BLK : 64), true, false, PLAN>(buf, two, +1);
     else
       fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
   };
@@ -439,7 +443,7 @@ constexpr bool col_tile_fits() {
          (size_t)(NYI + NYO + TX * NYI + TX * NYO) * sizeof(cx<T>) <= 160 * 1024;
 }
 
-template <typename T, int TX, int NYI, int NYO>
+template <typename T, int TX, int NYI, int NYO, int PLAN = 0>
 static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
                          const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
   const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
@@ -452,8 +456,8 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
   }
   const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
   if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)sm));
+    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO, PLAN>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   const Tables<T>& t = tables<T>(c);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   double bytes = 0;
@@ -463,7 +467,7 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
   }
   ProfScope ps(c, "col", bytes);
-  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
+  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
                      o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
                      tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
   LCHK(c, hipGetLastError());
@@ -473,19 +477,20 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
   X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
   X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
 
-// the tuned tile widths other than the default (les_coltx.hip); false when
-// that width has no variant for these lengths
+// the tuned tile widths / FFT plans other than the default (les_coltx.hip);
+// false when that variant does not exist for these lengths
 template <typename T>
-bool launch_col_alt(les_ctx* c, int tx, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
-                    const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2);
+bool launch_col_alt(les_ctx* c, int tx, int plan, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1,
+                    cx<T>* o2, const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2);
 
 template <typename T>
 static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
                        const ColArgs& a, long nplanes, int ncols, const T* G0 = nullptr, const T* G1 = nullptr,
                        const T* G2 = nullptr) {
   constexpr int TXD = col_tx_default<T>();
-  const int tx = c->tune.col_tx[sizeof(T) == 8 ? 1 : 0];
-  if (tx != TXD && launch_col_alt<T>(c, tx, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2)) return;
+  const int tx = c->tune.col_tx[sizeof(T) == 8 ? 1 : 0], plan = c->tune.col_plan[sizeof(T) == 8 ? 1 : 0];
+  if ((tx != TXD || plan != 0) && launch_col_alt<T>(c, tx, plan, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2))
+    return;
 #define LES_CASE(I, O)                                                                        \
   if (a.ny_in == I && a.ny_out == O)                                                          \
     return launch_col_n<T, TXD, I, O>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
