@@ -48,7 +48,7 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
-    tuned = sim.autotune(3)["chosen"] if tune else None  # launch shapes only, results unchanged
+    tuned = sim.autotune(7)["chosen"] if tune else None  # launch shapes only, results unchanged
     sim.set_graph(False)
     sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
     sim.set_graph(graph)
@@ -113,7 +113,7 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
-    sim.autotune(3)
+    sim.autotune(7)
     sim.mlp_init(1)
     nharv = 4
     gen = torch.Generator(device="cuda").manual_seed(0)
@@ -197,7 +197,7 @@ def run_ensemble(steps, warmup, n_each=8, grids=((64, 64, 49), (96, 96, 65), (12
             if stable:
                 m.init_scalar_ic(seed=i, amp_K=0.05)
             if (nx, stable) not in tunes:
-                tunes[(nx, stable)] = m.autotune(3)["chosen"]
+                tunes[(nx, stable)] = m.autotune(7)["chosen"]
             else:
                 m.set_tune_choice(tunes[(nx, stable)])
             members.append(m)

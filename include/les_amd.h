@@ -258,16 +258,19 @@ int les_profile_report(les_ctx *c, char *buf, int buflen);
 
 /* ---- autotuner (no counterpart in the reference code): times the column-FFT
    variant (what & 1; kx columns per block -- FP32 8/16/32, FP64 4/8/16 -- and
-   the FFT plan, radix-4 or radix-8 passes) and the tridiagonal schedule
+   the FFT plan, radix-4 or radix-8 passes), the tridiagonal schedule
    (what & 2; where the Thomas sweep keeps gamma and U -- HBM, gamma in LDS,
-   both in LDS -- and modes per block) on this context's state, and keeps the
-   fastest.  Widths and schedules give identical results; the plan changes
+   both in LDS -- and modes per block) and the padded-product pass of the
+   convective step (what & 4; groups of levels per workgroup, or streaming
+   runs of levels with the radix-8 plan) on this context's state, and keeps
+   the fastest.  Widths and schedules give identical results; the plans change
    only the rounding of the transforms.  report: JSON with the measured ms per
    candidate (may be NULL).  les_get_tune / les_set_tune: {col_tx FP32,
-   col_tx FP64, tri_mode, tri_block, col_plan FP32, col_plan FP64}. ---- */
+   col_tx FP64, tri_mode, tri_block, col_plan FP32, col_plan FP64,
+   pad_mode FP32, pad_mode FP64}. ---- */
 int les_autotune(les_ctx *c, int what, char *report, int report_len);
-int les_get_tune(les_ctx *c, int *out6);
-int les_set_tune(les_ctx *c, const int *in6);
+int les_get_tune(les_ctx *c, int *out8);
+int les_set_tune(les_ctx *c, const int *in8);
 
 /* ---- ensembles (no counterpart in the reference code): independent
    contexts, each with its own grid, surface parameters, precision and state

@@ -195,10 +195,11 @@ class Les:
         return cs, beta
 
     # ---- autotuner (launch shapes only; results do not depend on them) ----
-    def autotune(self, what=3):
-        """Time column-FFT tile widths (what & 1) and tridiagonal schedules
-        (what & 2) on this context's state and keep the fastest; returns the
-        measured ms per candidate and the choice."""
+    def autotune(self, what=7):
+        """Time column-FFT tile widths and plans (what & 1), tridiagonal
+        schedules (what & 2) and the padded-product variant (what & 4) on this
+        context's state and keep the fastest; returns the measured ms per
+        candidate and the choice."""
         import json
         buf = C.create_string_buffer(4096)
         self._chk(self.L.les_autotune(self.ctx, int(what), buf, len(buf)))
@@ -206,21 +207,25 @@ class Les:
 
     def get_tune(self):
         import numpy as np
-        a = np.zeros(6, dtype=np.int32)
+        a = np.zeros(8, dtype=np.int32)
         self._chk(self.L.les_get_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
         return dict(col_tx_fp32=int(a[0]), col_tx_fp64=int(a[1]), tri_mode=int(a[2]), tri_block=int(a[3]),
-                    col_plan_fp32=int(a[4]), col_plan_fp64=int(a[5]))
+                    col_plan_fp32=int(a[4]), col_plan_fp64=int(a[5]), pad_mode_fp32=int(a[6]),
+                    pad_mode_fp64=int(a[7]))
 
-    def set_tune(self, col_tx_fp32=16, col_tx_fp64=8, tri_mode=0, tri_block=128, col_plan_fp32=0, col_plan_fp64=0):
+    def set_tune(self, col_tx_fp32=16, col_tx_fp64=8, tri_mode=0, tri_block=128, col_plan_fp32=0, col_plan_fp64=0,
+                 pad_mode_fp32=0, pad_mode_fp64=0):
         import numpy as np
-        a = np.array([col_tx_fp32, col_tx_fp64, tri_mode, tri_block, col_plan_fp32, col_plan_fp64], dtype=np.int32)
+        a = np.array([col_tx_fp32, col_tx_fp64, tri_mode, tri_block, col_plan_fp32, col_plan_fp64, pad_mode_fp32,
+                      pad_mode_fp64], dtype=np.int32)
         self._chk(self.L.les_set_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
 
     def set_tune_choice(self, chosen):
         """Apply the 'chosen' entry of an autotune report."""
         self.set_tune(col_tx_fp32=chosen["col_tx"][0], col_tx_fp64=chosen["col_tx"][1],
                       tri_mode=chosen["tri_mode"], tri_block=chosen["tri_block"],
-                      col_plan_fp32=chosen["col_plan"][0], col_plan_fp64=chosen["col_plan"][1])
+                      col_plan_fp32=chosen["col_plan"][0], col_plan_fp64=chosen["col_plan"][1],
+                      pad_mode_fp32=chosen["pad_mode"][0], pad_mode_fp64=chosen["pad_mode"][1])
 
     # ---- learned closure (no counterpart in the reference code) ----
     MLP_NIN, MLP_NOUT = 8, 2

@@ -1222,10 +1222,34 @@ static void tune_run(les_ctx* c, int what, std::string& rep) {
     scratch_release(c, S_TUNE1);
     scratch_release(c, S_TUNE2);
   }
+  if ((what & 4) && c->err.empty()) {
+    // padded products: level groups per workgroup or streaming level runs, on
+    // the pass the step runs (momentum with both scalars, or momentum alone)
+    const int p = sizeof(T) == 8 ? 1 : 0;
+    const int scal = c->scal.enable ? 1 : 0;
+    const int keep = c->tune.pad_mode[p];
+    int best = keep;
+    double tbest = 1e30;
+    rep += (what & 3) ? ", " : "";
+    rep += "\"pad\": {";
+    for (int m = 0; m < 2 && c->err.empty(); ++m) {
+      c->tune.pad_mode[p] = m;
+      const double ms = tune_time(c, [&] { op_pad_products_probe<T>(c, scal); }, 5);
+      std::snprintf(buf, sizeof buf, "%s\"%d\": %.4f", m ? ", " : "", m, ms);
+      rep += buf;
+      if (ms < tbest) {
+        tbest = ms;
+        best = m;
+      }
+    }
+    c->tune.pad_mode[p] = best;
+    rep += "}";
+  }
   std::snprintf(buf, sizeof buf,
-                ", \"chosen\": {\"col_tx\": [%d, %d], \"col_plan\": [%d, %d], \"tri_mode\": %d, \"tri_block\": %d}}",
+                ", \"chosen\": {\"col_tx\": [%d, %d], \"col_plan\": [%d, %d], \"tri_mode\": %d, \"tri_block\": %d, "
+                "\"pad_mode\": [%d, %d]}}",
                 c->tune.col_tx[0], c->tune.col_tx[1], c->tune.col_plan[0], c->tune.col_plan[1], c->tune.tri_mode,
-                c->tune.tri_block);
+                c->tune.tri_block, c->tune.pad_mode[0], c->tune.pad_mode[1]);
   rep += buf;
 }
 }  // extern "C++"
@@ -1246,32 +1270,38 @@ int les_autotune(les_ctx* c, int what, char* report, int report_len) {
   }
   DONE(c);
 }
-int les_get_tune(les_ctx* c, int* out6) {
+int les_get_tune(les_ctx* c, int* out8) {
   GUARD(c);
-  if (!out6) return -1;
-  out6[0] = c->tune.col_tx[0];
-  out6[1] = c->tune.col_tx[1];
-  out6[2] = c->tune.tri_mode;
-  out6[3] = c->tune.tri_block;
-  out6[4] = c->tune.col_plan[0];
-  out6[5] = c->tune.col_plan[1];
+  if (!out8) return -1;
+  out8[0] = c->tune.col_tx[0];
+  out8[1] = c->tune.col_tx[1];
+  out8[2] = c->tune.tri_mode;
+  out8[3] = c->tune.tri_block;
+  out8[4] = c->tune.col_plan[0];
+  out8[5] = c->tune.col_plan[1];
+  out8[6] = c->tune.pad_mode[0];
+  out8[7] = c->tune.pad_mode[1];
   DONE(c);
 }
-int les_set_tune(les_ctx* c, const int* in6) {
+int les_set_tune(les_ctx* c, const int* in8) {
   GUARD(c);
-  if (!in6) return -1;
+  if (!in8) return -1;
+  const int* in6 = in8;
   const bool ok32 = in6[0] == 8 || in6[0] == 16 || in6[0] == 32, ok64 = in6[1] == 4 || in6[1] == 8 || in6[1] == 16;
   const bool okt = in6[2] >= 0 && in6[2] <= 2 && in6[3] >= 32 && in6[3] <= 1024 && in6[3] % 32 == 0 &&
                    tridiag_lds(in6[2], in6[3], c->g.nz) <= 160 * 1024;
   const bool okp = (in6[4] == 0 || (in6[4] == 2 && (in6[0] == 8 || in6[0] == 16))) &&
                    (in6[5] == 0 || (in6[5] == 2 && (in6[1] == 4 || in6[1] == 8)));
-  if (!ok32 || !ok64 || !okt || !okp) { c->err = "les_set_tune: launch shape out of range"; return 1; }
+  const bool okm = (in8[6] == 0 || in8[6] == 1) && (in8[7] == 0 || in8[7] == 1);
+  if (!ok32 || !ok64 || !okt || !okp || !okm) { c->err = "les_set_tune: launch shape out of range"; return 1; }
   c->tune.col_tx[0] = in6[0];
   c->tune.col_tx[1] = in6[1];
   c->tune.tri_mode = in6[2];
   c->tune.tri_block = in6[3];
   c->tune.col_plan[0] = in6[4];
   c->tune.col_plan[1] = in6[5];
+  c->tune.pad_mode[0] = in8[6];
+  c->tune.pad_mode[1] = in8[7];
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;

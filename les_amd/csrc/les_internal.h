@@ -47,6 +47,7 @@ struct Tune {
   int col_plan[2] = {0, 0}; // column-FFT plan (les_fft.h): 0 radix-4 passes, 2 radix-8 passes
   int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS
   int tri_block = 128;      // horizontal modes per block of the tridiagonal solve
+  int pad_mode[2] = {0, 0}; // padded products: 0 level groups per workgroup, 1 streaming level runs
 };
 
 // Scalar transport switches and constants (param.f90:53,72-90,139-146,174),
@@ -238,6 +239,8 @@ void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const 
 template <typename T> void op_step_scalar(les_ctx* c, int which, T* s, const T* RHS, const T* RHS_f);
 // convec + RHS assembly + AB2 of the momentum and both scalars' advection,
 // SGS flux divergence and step, sharing one padded-grid pass
+template <typename T>
+void op_pad_products_probe(les_ctx* c, int scal);
 template <typename T>
 void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
                           const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,

@@ -739,6 +739,23 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   if (init_qn) c->gqn_init = 1;
 }
 
+// the padded-product pass of a convective step alone, on the scratch buffers
+// the step uses (the autotuner times its variants): momentum and both scalars
+// (scal = 1) or momentum only
+template <typename T>
+void op_pad_products_probe(les_ctx* c, int scal) {
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1, nf = scal ? 12 : 6, no = scal ? 5 : 3;
+  cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)nf * nz * g.ny2 * g.nxh * sizeof(cx<T>));
+  cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)no * nl * g.ny2 * g.nxh * sizeof(cx<T>));
+  if (!c->err.empty()) return;
+  const T cst = T(1.0 / ((double)g.nx2 * g.ny2));
+  if (scal)
+    launch_pad_products<T, PolConvecScalar<T>, 4, 1>(c, XB, XP, PolConvecScalar<T>{cst}, "pad_probe");
+  else
+    launch_pad_products<T, PolConvec<T>, 8, 3>(c, XB, XP, PolConvec<T>{cst}, "pad_probe");
+}
+
 #define LES_SCAL_INST(T)                                                                                         \
   template void op_obukhov<T>(les_ctx*, const T*, const T*, const T*, double*, double*, double*, double*,        \
                               double*, double*, double*, double*, double*, double);                              \
@@ -751,7 +768,8 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
                                         const T* const[3], int, const T*, const ScalarJob*, const T*, const T*,   \
                                         int);                                                                     \
   template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
-                                      const SgsPtrs&, const ScalarSgsPtrs&, bool);
+                                      const SgsPtrs&, const ScalarSgsPtrs&, bool);                               \
+  template void op_pad_products_probe<T>(les_ctx*, int);
 
 LES_SCAL_INST(double)
 LES_SCAL_INST(float)

@@ -749,16 +749,172 @@ static void launch_pad_products_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const 
   LCHK(c, hipGetLastError());
 }
 
-// CH32 / CH64: levels per workgroup in FP32 / FP64 (sized to the 160 KB LDS)
+// ------------------------------------------------------------------------
+// Streaming variant of the padded products.  One workgroup owns padded row yb
+// and a run of output levels [k0, k1).  The physical fields of the levels the
+// products need (k - HB .. k + HA) stay in an LDS ring; each level step
+// inverse-transforms only the one new level, whose spectral rows were loaded
+// into registers during the previous step, so the HBM reads of level k + 1 + HA
+// are in flight while level k is transformed, multiplied and transformed back.
+// Each level is inverse-transformed once per run (the batched kernel redoes
+// the HB + HA halo levels of every CH-level group), and the transforms use the
+// radix-8 plan, whose passes give a level's NP rows more work-items than the
+// two-pass plan.  The physical values equal the batched kernel's up to the
+// rounding of the transform plan.
+// ------------------------------------------------------------------------
+template <typename T, int N, class Pol>
+struct PadStream {
+  static constexpr int NF = Pol::NF, NP = (NF + 1) / 2, NO = Pol::NOUT, NOP = (NO + 1) / 2;
+  static constexpr int HB = Pol::HB, HA = Pol::HA, NR = HB + HA + 1;  // ring slots
+  static constexpr int RS = lds_row(N), NXH = N / 3, H2 = NXH / 2;    // nx2 = 3 nxh
+  static constexpr int BLOCK = N >= 384 ? 512 : 256;
+  static constexpr int ITEMS = NP * H2;  // one item: two coefficients of a field pair
+  static constexpr int MAXI = (ITEMS + BLOCK - 1) / BLOCK;
+  static constexpr int PLAN = 2;
+  static constexpr size_t LDS = sizeof(cx<T>) * ((size_t)N + (size_t)RS * (NR * NP + NOP));
+};
+
+template <typename T, int N, class Pol>
+__global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
+    k_pad_stream(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int lc,
+                 const cx<T>* __restrict__ twg, Pol pol) {
+  using G = PadStream<T, N, Pol>;
+  constexpr int NF = G::NF, NP = G::NP, NO = G::NO, NOP = G::NOP, HB = G::HB, HA = G::HA, NR = G::NR;
+  constexpr int RS = G::RS, NXH = G::NXH, H2 = G::H2, BLOCK = G::BLOCK, MAXI = G::MAXI, PLAN = G::PLAN;
+  constexpr int ITEMS = G::ITEMS;
+  struct alignas(2 * sizeof(cx<T>)) cx2 {
+    cx<T> a, b;
+  };
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* ring = tw + N;              // [NR][NP][RS]: field pairs of levels k-HB .. k+HA, physical
+  cx<T>* out = ring + NR * NP * RS;  // [NOP][RS]: product pairs of level k
+  const int yb = blockIdx.x, nl = nz - 1;
+  const int k0 = blockIdx.y * lc, k1 = min(k0 + lc, nl);
+  auto slot = [&](int L) { return ring + ((L - k0 + HB) % NR) * (NP * RS); };
+  cx2 ra[MAXI], rb[MAXI];
+  auto fetch = [&](int L) {  // spectral rows of level L -> registers (zero outside 0..nz-1)
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      const int w = threadIdx.x + i * BLOCK;
+      ra[i] = cx2{{T(0), T(0)}, {T(0), T(0)}};
+      rb[i] = ra[i];
+      if ((ITEMS % BLOCK == 0 || w < ITEMS) && L >= 0 && L < nz) {
+        const int t = w / H2, jj = w - t * H2;
+        ra[i] = reinterpret_cast<const cx2*>(XB + (((long)(2 * t) * nz + L) * ny2 + yb) * NXH)[jj];
+        if (2 * t + 1 < NF)
+          rb[i] = reinterpret_cast<const cx2*>(XB + (((long)(2 * t + 1) * nz + L) * ny2 + yb) * NXH)[jj];
+      }
+    }
+  };
+  // field pair (a, b) as one complex row: z_j = a_j + i b_j and its conjugate
+  // partner z_{N-j} = conj(a_j) + i conj(b_j); kx = 0 keeps the real parts
+  auto put = [](cx<T>* row, int j, cx<T> A, cx<T> B) {
+    if (j == 0) {
+      row[lds_pad(0)] = {A.x, B.x};
+    } else {
+      row[lds_pad(j)] = {A.x - B.y, A.y + B.x};
+      row[lds_pad(N - j)] = {A.x + B.y, B.x - A.y};
+    }
+  };
+  auto stage = [&](int L) {  // registers -> ring slot of level L, zero band [NXH, N - NXH]
+    cx<T>* s = slot(L);
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      const int w = threadIdx.x + i * BLOCK;
+      if (ITEMS % BLOCK == 0 || w < ITEMS) {
+        const int t = w / H2, jj = w - t * H2;
+        put(s + t * RS, 2 * jj, ra[i].a, rb[i].a);
+        put(s + t * RS, 2 * jj + 1, ra[i].b, rb[i].b);
+      }
+    }
+    constexpr int NZB = N - 2 * NXH + 1;
+    for (int e = threadIdx.x; e < NP * NZB; e += BLOCK) {
+      const int t = e / NZB, j = NXH + (e - t * NZB);
+      s[t * RS + lds_pad(j)] = {T(0), T(0)};
+    }
+  };
+  load_twiddles_for(tw, twg, N, true, PLAN);
+  for (int L = k0 - HB; L < k0 + HA; ++L) {  // the levels below the first new one
+    fetch(L);
+    stage(L);
+    __syncthreads();
+    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(L), tw, +1);
+  }
+  fetch(k0 + HA);
+  for (int k = k0; k < k1; ++k) {
+    stage(k + HA);
+    if (k + 1 < k1) fetch(k + 1 + HA);  // in flight during this level's transforms
+    __syncthreads();
+    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(k + HA), tw, +1);
+    for (int x = threadIdx.x; x < N; x += BLOCK) {
+      T o[2 * NOP];
+#pragma unroll
+      for (int q = 0; q < 2 * NOP; ++q) o[q] = T(0);
+      auto F = [&](int dk, int f) -> T {
+        const cx<T> z = slot(k + dk)[(f >> 1) * RS + lds_pad(x)];
+        return (f & 1) ? z.y : z.x;
+      };
+      pol.product(k, F, o);
+#pragma unroll
+      for (int q = 0; q < NOP; ++q) out[q * RS + lds_pad(x)] = {o[2 * q], o[2 * q + 1]};
+    }
+    __syncthreads();
+    fft_lds_c<T, N, 1, RS, NOP, BLOCK, false, true, PLAN>(out, tw, -1);
+    for (int e = threadIdx.x; e < NO * NXH; e += BLOCK) {
+      const int o = e / NXH, kx = e - o * NXH;
+      const cx<T>* z = out + (o >> 1) * RS;
+      const cx<T> zk = z[lds_pad(kx)], zn = z[lds_pad(kx ? N - kx : 0)];
+      const cx<T> v = (o & 1) ? cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)}
+                              : cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+      XP[(((long)Pol::slot(o) * nl + k) * ny2 + yb) * NXH + kx] = v;
+    }
+  }
+}
+
+// level runs per padded row of the streaming kernel (workgroups = ny2 * runs)
+constexpr int PAD_STREAM_RUNS = 4;
+
+template <typename T, int N, class Pol>
+static void launch_pad_stream_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Pol& pol, const char* tag) {
+  using G = PadStream<T, N, Pol>;
+  static_assert(G::LDS <= 163840, "streaming padded-product ring exceeds LDS");
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1;
+  if (g.nx2 != N || 3 * g.nxh != N || g.nxh % 2) {
+    c->err = "streaming padded products: grid does not match the kernel";
+    return;
+  }
+  if (G::LDS > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_pad_stream<T, N, Pol>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)G::LDS));
+  const int lc = (nl + PAD_STREAM_RUNS - 1) / PAD_STREAM_RUNS;
+  const unsigned gy = (unsigned)((nl + lc - 1) / lc);
+  ProfScope ps(c, tag,
+               ((double)Pol::NF * nz + (double)Pol::NOUT * (nz - 1)) * g.ny2 * g.nxh * sizeof(cx<T>));
+  hipLaunchKernelGGL((k_pad_stream<T, N, Pol>), dim3(g.ny2, gy), dim3(G::BLOCK), G::LDS, c->stream, XB, XP, g.ny2,
+                     nz, lc, tw_for<T>(c, N), pol);
+  LCHK(c, hipGetLastError());
+}
+
+// CH32 / CH64: levels per workgroup of the batched kernel in FP32 / FP64
+// (sized to the 160 KB LDS); c->tune.pad_mode 1 selects the streaming kernel
+// where it is instantiated (nx2 >= 96)
 template <typename T, class Pol, int CH32, int CH64>
 static void launch_pad_products(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Pol& pol, const char* tag) {
   constexpr int CH = sizeof(T) == 4 ? CH32 : CH64;
+  const bool stream = c->tune.pad_mode[sizeof(T) == 8 ? 1 : 0] == 1;
   switch (c->g.nx2) {
 #define LES_CASE(NN) \
   case NN:           \
     return launch_pad_products_n<T, NN, Pol, CH>(c, XB, XP, pol, tag);
-    LES_CASE(24) LES_CASE(48) LES_CASE(96) LES_CASE(144) LES_CASE(192) LES_CASE(384)
+#define LES_CASE_S(NN)                                                           \
+  case NN:                                                                       \
+    if (stream) return launch_pad_stream_n<T, NN, Pol>(c, XB, XP, pol, tag);    \
+    return launch_pad_products_n<T, NN, Pol, CH>(c, XB, XP, pol, tag);
+    LES_CASE(24) LES_CASE(48) LES_CASE_S(96) LES_CASE_S(144) LES_CASE_S(192) LES_CASE_S(384)
 #undef LES_CASE
+#undef LES_CASE_S
     default:
       return launch_pad_products_n<T, 0, Pol, CH>(c, XB, XP, pol, tag);
   }

@@ -283,6 +283,24 @@ def test_convective_step_vs_oracle(torch_mod, prec):
         assert bad.mean() < 1e-3, name
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_convective_step_streaming_products_vs_oracle(torch_mod, prec):
+    """One convective step with the streaming padded-product kernel (64^2
+    grid, padded length 96) against the oracle, at the tolerances of the
+    default kernel."""
+    print("convective step, streaming products", prec, flush=True)
+    o, so, sg = _conv_pair(prec, nx=64, ny=64, nz=17)
+    sg.set_tune(**{"pad_mode_" + prec: 1})
+    so.step(1)
+    sg.step(1)
+    nx = o.nx
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    for name in ("u", "v", "w", "RHSz", "p", "theta", "qmix", "RHS_T"):
+        a, b = sg.get(name), so[name]
+        nl = b.shape[0] - 1
+        assert relerr(a[:nl, :, :nx], b[:nl, :, :nx]) < tol, name
+
+
 def test_convective_graph_replay_matches_eager(torch_mod):
     print("convective graph", flush=True)
     _, _, a = _conv_pair("fp32")

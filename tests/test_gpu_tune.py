@@ -1,7 +1,8 @@
 """The autotuner's launch shapes: every column-FFT tile width and tridiagonal
 schedule gives bit-identical states (the per-element arithmetic does not
-depend on them); the radix-8 column plan is another order of the same DFTs
-(FP64 states agree to rounding); les_autotune picks a measured candidate."""
+depend on them); the radix-8 column plan and the streaming padded products
+are other orders of the same DFTs (FP64 states agree to rounding);
+les_autotune picks a measured candidate."""
 import numpy as np
 import pytest
 
@@ -10,10 +11,10 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "RHSx", "Cs_opt2", "Nu_t", "txz")
 
 
-def run(prec, tune=None, steps=12, conv=False, autotune=False):
+def run(prec, tune=None, steps=12, conv=False, autotune=False, n=32):
     from les_amd.api import LesSim, checkerboard_flux
-    kw = dict(scalars=dict(wt_s=checkerboard_flux(32, 32), dTdz_top=0.01)) if conv else {}
-    s = LesSim(32, 32, 17, precision=prec, dt=2e-4, **kw)
+    kw = dict(scalars=dict(wt_s=checkerboard_flux(n, n), dTdz_top=0.01)) if conv else {}
+    s = LesSim(n, n, 17, precision=prec, dt=2e-4, **kw)
     s.init_ic(seed=3)
     if conv:
         s.init_scalar_ic(seed=3)
@@ -21,7 +22,7 @@ def run(prec, tune=None, steps=12, conv=False, autotune=False):
     if tune:
         s.set_tune(**tune)
     if autotune:
-        rep = s.autotune(3)
+        rep = s.autotune(7)
     s.step(steps)
     return s, rep
 
@@ -52,8 +53,9 @@ def test_autotune_chooses_a_measured_candidate(torch_mod):
     assert f"{t['col_tx_fp32']}/{t['col_plan_fp32']}" in rep["col"]
     assert f"{t['tri_mode']}/{t['tri_block']}" in rep["tridiag"]
     assert ch["tri_mode"] == t["tri_mode"] and ch["tri_block"] == t["tri_block"]
-    assert all(v > 0 for v in rep["col"].values()) and all(v > 0 for v in rep["tridiag"].values())
-    if t["col_plan_fp32"] == 0:
+    assert str(t["pad_mode_fp32"]) in rep["pad"] and ch["pad_mode"][0] == t["pad_mode_fp32"]
+    assert all(v > 0 for k in ("col", "tridiag", "pad") for v in rep[k].values())
+    if t["col_plan_fp32"] == 0 and t["pad_mode_fp32"] == 0:
         for n in FIELDS:
             assert np.array_equal(s.get(n), base.get(n)), n
 
@@ -76,6 +78,27 @@ def test_radix8_column_plan(torch_mod, prec, tx):
         assert err < tol, (n, err)
 
 
+@pytest.mark.parametrize("prec,conv", [("fp64", True), ("fp32", True), ("fp64", False), ("fp32", False)])
+def test_streaming_padded_products(torch_mod, prec, conv):
+    """Streaming level runs (radix-8 plan) against level groups (two-pass
+    plan) for the padded products, on a 64^2 grid (padded length 96, where the
+    streaming kernel exists): momentum with both scalars, and momentum alone.
+    Tolerances as for the radix-8 column plan."""
+    print("streaming padded products", prec, conv, flush=True)
+    key = "pad_mode_" + prec
+    base, _ = run(prec, conv=conv, n=64, steps=8)
+    s, _ = run(prec, tune={key: 1}, conv=conv, n=64, steps=8)
+    assert s.get_tune()[key] == 1
+    tol = 1e-12 if prec == "fp64" else 2e-4
+    names = ("u", "v", "w", "p") + (("theta",) if conv else ())
+    for n in names:
+        a, b = s.get(n)[..., :64], base.get(n)[..., :64]
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(n, err)
+        assert err < tol, (n, err)
+    assert any(not np.array_equal(s.get(n), base.get(n)) for n in names)  # the streaming kernel ran
+
+
 def test_set_tune_rejects_bad_shapes(torch_mod):
     from les_amd.api import Les, LesError
     g = Les(16, 16, 9)
@@ -83,3 +106,5 @@ def test_set_tune_rejects_bad_shapes(torch_mod):
         g.set_tune(col_tx_fp32=12)
     with pytest.raises(LesError):
         g.set_tune(tri_mode=2, tri_block=1024)   # (nz+1) x 1024 x 24 B does not fit in LDS
+    with pytest.raises(LesError):
+        g.set_tune(pad_mode_fp32=2)
