@@ -362,23 +362,39 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
   };
   const int kx0 = blockIdx.x * TX;
   const long pl = blockIdx.y;
-  {
-    const cx<T>* src = in + pl * a.in_plane;
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      A[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+  // input tile -> LDS; with compile-time lengths every load of the tile is
+  // issued before the first LDS store, so the HBM latency is paid once
+  auto load_tile = [&](cx<T>* dst, const cx<T>* src) {
+    if constexpr (CT) {
+      constexpr int NI = (CT ? NYI : 1) * TX, MAXI = (NI + (CT ? BLK : 64) - 1) / (CT ? BLK : 64);
+      cx<T> rg[MAXI];
+#pragma unroll
+      for (int i = 0; i < MAXI; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        rg[i] = cx<T>{T(0), T(0)};
+        if (NI % BLK == 0 || e < NI) {
+          const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+          if (kk < a.nki) rg[i] = src[(long)y * a.in_row + kk];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MAXI; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NI % BLK == 0 || e < NI) dst[e] = rg[i];
+      }
+    } else {
+      for (int e = threadIdx.x; e < nyi * TX; e += bd) {
+        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+        dst[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+      }
     }
-  }
+  };
+  load_tile(A, in + pl * a.in_plane);
+  if (in2) load_tile(B, in2 + pl * a.in2_plane);
   __syncthreads();
   if (a.do_fwd) fwd(A);
   if (in2) {
     // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
-    const cx<T>* src = in2 + pl * a.in2_plane;
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      B[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
-    }
-    __syncthreads();
     fwd(B);
     for (int e = threadIdx.x; e < nyi * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
