@@ -84,14 +84,36 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
-    const int t = e / n, x = e - t * n, pp = pair0 + t;
-    cx<T> z = {T(0), T(0)};
-    if (pp < npp) {
-      z.x = L(plane, 2 * pp, x);
-      z.y = L(plane, 2 * pp + 1, x);
+  if constexpr (N > 0) {
+    // every row load of the block is issued before the first LDS store
+    constexpr int NE = RowGeom<N>::P * (N > 0 ? N : 1), BL = RowGeom<N>::BLOCK, MAXE = (NE + BL - 1) / BL;
+    cx<T> z[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+      const int e = threadIdx.x + i * BL;
+      const int t = e / N, x = e - t * N, pp = pair0 + t;
+      z[i] = {T(0), T(0)};
+      if ((NE % BL == 0 || e < NE) && pp < npp) {
+        z[i].x = L(plane, 2 * pp, x);
+        z[i].y = L(plane, 2 * pp + 1, x);
+      }
     }
-    buf[ix(t, x)] = z;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+      const int e = threadIdx.x + i * BL;
+      const int t = e / N, x = e - t * N;
+      if (NE % BL == 0 || e < NE) buf[ix(t, x)] = z[i];
+    }
+  } else {
+    for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+      const int t = e / n, x = e - t * n, pp = pair0 + t;
+      cx<T> z = {T(0), T(0)};
+      if (pp < npp) {
+        z.x = L(plane, 2 * pp, x);
+        z.y = L(plane, 2 * pp + 1, x);
+      }
+      buf[ix(t, x)] = z;
+    }
   }
   __syncthreads();
   if constexpr (N > 0)
@@ -129,8 +151,9 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
-  for (int e = threadIdx.x; e < P * n; e += BLOCK) {
-    const int t = e / n, j = e - t * n, pp = pair0 + t;
+  // element j of row pair t: z_j = a_j + i b_j (conjugate partners above n/2)
+  auto element = [&](int t, int j) {
+    const int pp = pair0 + t;
     cx<T> z = {T(0), T(0)};
     if (pp < npp) {
       const long r0 = ((long)plane * ny + 2 * pp) * irow;
@@ -156,7 +179,29 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
         z = {A.x + B.y, B.x - A.y};
       }
     }
-    buf[ix(t, j)] = z;
+    return z;
+  };
+  if constexpr (N > 0) {
+    // every row load of the block is issued before the first LDS store
+    constexpr int NE = RowGeom<N>::P * (N > 0 ? N : 1), BL = RowGeom<N>::BLOCK, MAXE = (NE + BL - 1) / BL;
+    cx<T> z[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+      const int e = threadIdx.x + i * BL;
+      const int t = e / N, j = e - t * N;
+      z[i] = (NE % BL == 0 || e < NE) ? element(t, j) : cx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+      const int e = threadIdx.x + i * BL;
+      const int t = e / N, j = e - t * N;
+      if (NE % BL == 0 || e < NE) buf[ix(t, j)] = z[i];
+    }
+  } else {
+    for (int e = threadIdx.x; e < P * n; e += BLOCK) {
+      const int t = e / n, j = e - t * n;
+      buf[ix(t, j)] = element(t, j);
+    }
   }
   __syncthreads();
   if constexpr (N > 0)
