@@ -12,6 +12,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 // The transform arithmetic is evaluated as written, without contracting a
 // multiply and an add into an FMA: the compiler's choice would otherwise
 // depend on the surrounding kernel (tile width, unrolling), and a transform
@@ -25,9 +27,11 @@ struct cx {
   T x, y;
 };
 
+// complex product with two fused multiply-adds (explicit, so the rounding
+// does not depend on what the compiler would contract)
 template <typename T>
 __host__ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+  return {std::fma(a.x, b.x, -(a.y * b.y)), std::fma(a.x, b.y, a.y * b.x)};
 }
 
 struct FftPlan {
@@ -286,7 +290,7 @@ __host__ __device__ __forceinline__ cx<T> twiddle_const(cx<T> a, int m, int sign
   if (4 * m == 3 * R) return sign < 0 ? cx<T>{-a.y, a.x} : cx<T>{a.y, -a.x};
   const int j = m * (144 / R);
   const T c = (T)dft_cos144(j), sn = (T)(sign < 0 ? -dft_sin144(j) : dft_sin144(j));
-  return cx<T>{a.x * c - a.y * sn, a.x * sn + a.y * c};
+  return cx<T>{std::fma(a.x, c, -(a.y * sn)), std::fma(a.x, sn, a.y * c)};
 }
 
 // R = P Q split of the in-register DFT sizes

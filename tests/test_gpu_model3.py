@@ -58,16 +58,21 @@ def test_scaledep_dynamic(torch_mod, grid, binding):
     cs_g, b_g = g.scaledep_dynamic([G(a) for a in D], G(u), G(v), G(w))
     cs_g, b_g = H(cs_g), H(b_g)
     assert np.all(np.isfinite(b_g)) and np.all(cs_g >= 0.0)
-    # levels where Newton is ill-conditioned: a 1e-13 perturbation of the
-    # input moves the oracle's own beta by more than 1e-5 (or stops Newton
-    # at its 100-iteration limit, beta = 1); there any summation order may
-    # give another iterate, so only finiteness is required
-    rng = np.random.default_rng(17)
-    pert = lambda a: a * (1.0 + 1e-13 * rng.standard_normal(a.shape))
-    up, vp, wp = pert(u), pert(v), pert(w)
-    Dp = [up, vp, wp] + [pert(a) for a in D[3:]] if binding == "sgs_stag" else [pert(a) for a in D]
-    cs_p, b_p = o.scaledep_dynamic(Dp, up, vp, wp)
-    good = (np.abs(b_p - b_o) <= 1e-5 * np.maximum(1.0, np.abs(b_o))) & (b_o != 1.0) & (b_p != 1.0)
+    # levels where Newton is ill-conditioned: one of 16 random relative
+    # perturbations of the input (8 at 1e-14, 8 at 1e-13) moves the oracle's
+    # own beta by more than 1e-5 (or stops Newton at its 100-iteration limit,
+    # beta = 1); there any summation order may give another iterate, so only
+    # finiteness is required.  (On the strain case at 32x32x9 the lid level
+    # flips under some 1e-15 perturbations and not under others.)
+    good = b_o != 1.0
+    for seed in range(16):
+        rng = np.random.default_rng(seed)
+        mag = 1e-14 if seed < 8 else 1e-13
+        pert = lambda a: a * (1.0 + mag * rng.standard_normal(a.shape))
+        up, vp, wp = pert(u), pert(v), pert(w)
+        Dp = [up, vp, wp] + [pert(a) for a in D[3:]] if binding == "sgs_stag" else [pert(a) for a in D]
+        _, b_p = o.scaledep_dynamic(Dp, up, vp, wp)
+        good &= (np.abs(b_p - b_o) <= 1e-5 * np.maximum(1.0, np.abs(b_o))) & (b_p != 1.0)
     print("well-conditioned levels", np.flatnonzero(good), "of", o.nz)
     assert good.sum() >= o.nz - 2
     assert relerr(b_g[good], b_o[good]) < 1e-4, relerr(b_g[good], b_o[good])
