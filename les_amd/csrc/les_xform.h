@@ -527,7 +527,7 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     bytes += (double)nplanes * a.ny_in * (ncols < a.nki ? ncols : a.nki) * es * (in2 ? 2 : 1);
     for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
   }
-  ProfScope ps(c, "col", bytes);
+  ProfScope ps(c, a.ny_in < a.ny_out ? "col_pad" : (a.ny_in > a.ny_out ? "col_trunc" : "col"), bytes);
   hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
                      o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
                      tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
