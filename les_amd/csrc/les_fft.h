@@ -378,7 +378,8 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         cx<T> a = base[at(b + r * NB)];
-        if (r > 0 && NS > 1 && k > 0) {
+        // k = 0 multiplies by the table's exact (1, -0): no branch needed
+        if (r > 0 && NS > 1) {
           cx<T> wv = tw[OFF + (r - 1) * NS + k];
           if (sign > 0) wv.y = -wv.y;
           a = cmul(a, wv);
