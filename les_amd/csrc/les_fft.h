@@ -354,6 +354,20 @@ constexpr int fft_tw_off(int n, int s, int plan = 0) {
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
 __host__ __device__ constexpr int lds_row(int n) { return n + (n >> 4) + 1; }
 
+// Whether, in a stage of radix R after NS points, every butterfly's R padded
+// positions lie at constant distances r s + (r s)/16 from its first one
+// (reads: s = N/R from b; writes: s = NS from the output base), so the
+// addresses fold into the LDS instructions' immediate offsets.
+__host__ __device__ constexpr bool pad_offsets_const(int n, int R, int NS, bool rd) {
+  const int NB = n / R, s = rd ? NB : NS;
+  for (int b = 0; b < NB; ++b) {
+    const int e0 = rd ? b : (b / NS) * NS * R + b % NS;
+    for (int r = 0; r < R; ++r)
+      if (lds_pad(e0 + r * s) - lds_pad(e0) != r * s + ((r * s) >> 4)) return false;
+  }
+  return true;
+}
+
 // One Stockham stage of NTR transforms of length N in LDS (element e of
 // transform t at buf[t*TS + e*ES], or buf[t*TS + lds_pad(e)] with PAD)
 // executed by BLOCK threads.
@@ -362,7 +376,12 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
   constexpr int NB = N / R;
   constexpr int ITEMS = NTR * NB;
   constexpr int MAXB = (ITEMS + BLOCK - 1) / BLOCK;
+  constexpr bool RD_C = !PAD || pad_offsets_const(N, R, NS, true);
+  constexpr bool WR_C = !PAD || pad_offsets_const(N, R, NS, false);
   auto at = [](int e) { return PAD ? lds_pad(e) : e * ES; };
+  // position of element e0 + d: a compile-time distance from e0's where that holds
+  auto at_rd = [&](int e0, int d) { return RD_C ? at(e0) + (PAD ? d + (d >> 4) : d * ES) : at(e0 + d); };
+  auto at_wr = [&](int e0, int d) { return WR_C ? at(e0) + (PAD ? d + (d >> 4) : d * ES) : at(e0 + d); };
   cx<T> v[MAXB][R];
   int dd[MAXB], tt[MAXB];
 #pragma unroll
@@ -377,7 +396,7 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
       cx<T>* base = buf + t * TS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        cx<T> a = base[at(b + r * NB)];
+        cx<T> a = base[at_rd(b, r * NB)];
         // k = 0 multiplies by the table's exact (1, -0): no branch needed
         if (r > 0 && NS > 1) {
           cx<T> wv = tw[OFF + (r - 1) * NS + k];
@@ -397,7 +416,7 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
     if (tt[i] >= 0) {
       cx<T>* base = buf + tt[i] * TS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[at(dd[i] + r * NS)] = v[i][r];
+      for (int r = 0; r < R; ++r) base[at_wr(dd[i], r * NS)] = v[i][r];
     }
   }
   __syncthreads();
