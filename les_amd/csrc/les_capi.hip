@@ -356,8 +356,9 @@ int les_create(const les_grid* gi, les_ctx** out) {
   c->p_nx = make_plan(g.nx); c->p_ny = make_plan(g.ny);
   c->p_nx2 = make_plan(g.nx2); c->p_ny2 = make_plan(g.ny2);
   if (c->p_nx.n < 0 || c->p_ny.n < 0 || c->p_nx2.n < 0 || c->p_ny2.n < 0 || g.nx > 1024 || g.ny > 1024 ||
-      g.nz < 3 || g.nx % 8 || g.ny % 8) {
-    fprintf(stderr, "les_create: nx, ny must be multiples of 8 of the form 2^a 3^b (<= 1024), nz >= 3\n");
+      g.nz < 3 || g.nx % 8 || g.ny % 8 || (long)g.ld * g.ny * (g.nz + 1) >= (1L << 31)) {
+    fprintf(stderr, "les_create: nx, ny must be multiples of 8 of the form 2^a 3^b (<= 1024), nz >= 3, "
+                    "and ld*ny*(nz+1) < 2^31\n");
     delete c;
     return 2;
   }

@@ -196,7 +196,7 @@ __global__ void k_sd_cs(const double* __restrict__ part, int nparts, int nz, dou
 template <typename T>
 __global__ void k_sd_fill(const double* __restrict__ cs, T* Cs, long pl, long n3) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x)
-    Cs[i] = (T)cs[i / pl];
+    Cs[i] = (T)cs[(unsigned)i / (unsigned)pl];
 }
 
 template <typename T>

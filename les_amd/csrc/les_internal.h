@@ -12,6 +12,18 @@
 
 namespace les {
 
+// Level k and column x of the linear index i of an (ld, ny, nlev) array, in
+// 32-bit arithmetic (les_create keeps every array below 2^31 points; 64-bit
+// division is a long software sequence on the GPU).
+struct LevelCol {
+  int k, x;
+};
+__device__ __forceinline__ LevelCol level_col(long i, long pl, int ld) {
+  const unsigned ii = (unsigned)i, p = (unsigned)pl;
+  const unsigned k = ii / p;
+  return {(int)k, (int)((ii - k * p) % (unsigned)ld)};
+}
+
 // Grid and run constants (param.f90:56-65,98,100,144-151 of the reference),
 // plus the derived sizes used by the kernels.
 struct Grid {

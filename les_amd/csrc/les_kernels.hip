@@ -196,8 +196,8 @@ __global__ void k_ddz(T* __restrict__ dfdz, const T* __restrict__ f, int nx, int
                       int wnode) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     if (k == nz - 1) {
       dfdz[i] = T(0);
     } else if (x < nx) {
@@ -415,8 +415,8 @@ __global__ void k_calc_sij(const T* __restrict__ ux_, const T* __restrict__ uy_,
                            T* v_lag, T* w_lag, T* u_lag_s, T* v_lag_s, T* w_lag_s) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     if (u_lag) {  // sgs_stag.f90:62-64, whole arrays
       u_lag[i] = u_lag[i] + u[i];
       v_lag[i] = v_lag[i] + v[i];
@@ -473,8 +473,8 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
                           T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2, T* Pr_t, T Pr) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     if (use_const) Cs_out[i] = cs_const;  // sgs_stag.f90:166-173, whole arrays
     if (use_const && Pr_t) Pr_t[i] = Pr;
     if (k == nz - 1) {                    // sgs_stag.f90:356-364
@@ -1116,8 +1116,8 @@ __global__ void k_project(T* u, T* v, T* w, const T* dpdx, const T* dpdy, const 
   const long pl = (long)ld * ny;
   const long n = pl * (nz - 1);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     T uu = u[i], vv = v[i], ww = w[i];
     if (x < nx) {
       uu = (uu + dt * (-T(TADV1) * dpdx[i] + T(0)));
@@ -1187,8 +1187,8 @@ __global__ void k_pressure_update(T* u, T* v, T* w, T* Rx, T* Ry, T* Rz, const T
       dpdz[i] = T(BOGUS);
       continue;
     }
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     T dz = dpdz[i];
     if (x < nx) {
       dz = (p[i + pl] - p[i]) * rdz;  // p is (ld,ny,0:nz): p(k) at i+pl

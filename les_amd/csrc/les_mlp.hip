@@ -493,7 +493,7 @@ struct FeatLag {
   }
   __device__ void operator()(long p, float* f) const {
     const long i = index(p);
-    const int k = (int)(i / pl);
+    const int k = (int)((unsigned)i / (unsigned)pl);
     double s[6], sb[6], sh[6], ssb[6], ssh[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {

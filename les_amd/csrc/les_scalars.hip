@@ -188,8 +188,8 @@ __global__ void k_beta(const T* __restrict__ th, const T* __restrict__ q, const 
   const long pl = (long)ld * ny, n = pl * nz;
   const double nxy = (double)(nx * ny);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     double b = 0.0;
     if (k >= 1 && k < nz - 1 && x < nx) {
       const double bk = sums[k] / nxy, bm = sums[k - 1] / nxy;
@@ -251,9 +251,10 @@ __global__ void k_scalar_dz(DzArgs<T> a, int nx, int ld, int ny, int nz, double 
   const long pl = (long)ld * ny, n3 = pl * nz, n = n3 * a.ns;
   const double fn = (double)(float)(nx * ny);
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const int q = (int)(t / n3);
-    const long i = t - q * n3;
-    const int k = (int)(i / pl);
+    const unsigned tt = (unsigned)t, q3 = (unsigned)n3;  // n = ns * n3 < 2^32
+    const int q = (int)(tt / q3);
+    const long i = (long)(tt - (unsigned)q * q3);
+    const int k = (int)((unsigned)i / (unsigned)pl);
     const int r = (int)(i - k * pl), y = r / ld, x = r - y * ld;
     if (a.surface_only && k != 0) continue;
     T* d = a.dz[q];
@@ -531,8 +532,8 @@ template <typename T>
 __global__ void k_step_scalar(T* s, const T* R, const T* Rf, int nx, int ld, int ny, int nz, T dt, T gz) {
   const long pl = (long)ld * ny, n = pl * (nz - 1);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     T sn = s[i];
     if (x < nx) {
       sn = s[i] + dt * (T(1.5) * R[i] - T(0.5) * Rf[i]);
@@ -605,8 +606,8 @@ __global__ void k_lag_scalar_epilogue(const T* __restrict__ F, const T* __restri
   const long pl = (long)ld * ny, n3 = pl * nz;
   const double e24 = (double)1e-24f;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(i / pl);
-    const int x = (int)((i - k * pl) % ld);
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
     const bool pad = x >= nx;
     auto f2m = [&](int f) { return (double)F[f * n3 + i]; };
     auto f4m = [&](int f) { return (double)F[(21 + f) * n3 + i]; };
