@@ -281,9 +281,7 @@ static void step_once(les_ctx* c) {
   // without scalars obukhov's neutral branch leaves psi = 0, phi = 1: the
   // state's constant contents
   op_filt_da3<T>(c, u, v, w, gr, g.nz);
-  op_ddz_uv<T>(c, gr[2], u);
-  op_ddz_uv<T>(c, gr[5], v);
-  op_ddz_w<T>(c, gr[8], w);
+  op_ddz3<T>(c, gr[2], gr[5], gr[8], u, v, w);
   op_wallstress<T>(c, u, v, gr[2], gr[5], tau[2], tau[4], (const double*)state_ptr(c, "psi_m"),
                    (const double*)state_ptr(c, "psi_m0"), (const double*)state_ptr(c, "phi_m"),
                    (double*)state_ptr(c, "ustar_avg"));

@@ -157,6 +157,7 @@ template <typename T> void op_padd(les_ctx* c, T* big_c, const T* small_c, int n
 template <typename T> void op_unpadd(les_ctx* c, T* small_c, const T* big_c, int nplanes);
 template <typename T> void op_ddz_uv(les_ctx* c, T* dfdz, const T* f);
 template <typename T> void op_ddz_w(les_ctx* c, T* dfdz, const T* f);
+template <typename T> void op_ddz3(les_ctx* c, T* dudz, T* dvdz, T* dwdz, const T* u, const T* v, const T* w);
 template <typename T> void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, int wnode);
 template <typename T>
 void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,

@@ -208,6 +208,45 @@ __global__ void k_ddz(T* __restrict__ dfdz, const T* __restrict__ f, int nx, int
     }
   }
 }
+// the three vertical derivatives of a step in one pass: du/dz and dv/dz at
+// w nodes (ddz_uv), dw/dz at uvp nodes (ddz_w); TPT consecutive points per
+// thread so that each wave keeps several loads of every field in flight
+template <typename T, int TPT>
+__global__ void k_ddz3(T* __restrict__ dudz, T* __restrict__ dvdz, T* __restrict__ dwdz, const T* __restrict__ u,
+                       const T* __restrict__ v, const T* __restrict__ w, int nx, int ld, int ny, int nz, T cst) {
+  const unsigned pl = (unsigned)ld * ny, n = pl * nz;
+  const unsigned i0 = (blockIdx.x * blockDim.x) * TPT + threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < TPT; ++t) {
+    const unsigned i = i0 + t * blockDim.x;
+    if (i >= n) break;
+    const unsigned k = i / pl;
+    const int x = (int)((i - k * pl) % (unsigned)ld);
+    if (k == (unsigned)nz - 1) {
+      dudz[i] = T(0);
+      dvdz[i] = T(0);
+      dwdz[i] = T(0);
+    } else if (x < nx) {
+      dwdz[i] = cst * (w[i + pl] - w[i]);
+      if (k >= 1) {
+        dudz[i] = cst * (u[i] - u[i - pl]);
+        dvdz[i] = cst * (v[i] - v[i - pl]);
+      }
+    }
+  }
+}
+template <typename T>
+void op_ddz3(les_ctx* c, T* dudz, T* dvdz, T* dwdz, const T* u, const T* v, const T* w) {
+  const Grid& g = c->g;
+  constexpr int TPT = 4;
+  const size_t n = (size_t)g.ld * g.ny * g.nz;
+  {
+    ProfScope _ps(c, "ddz", 6.0 * n * sizeof(T));
+    hipLaunchKernelGGL((k_ddz3<T, TPT>), dim3((unsigned)((n + 256 * TPT - 1) / (256 * TPT))), dim3(256), 0,
+                       c->stream, dudz, dvdz, dwdz, u, v, w, g.nx, g.ld, g.ny, g.nz, T(1.0 / g.dz));
+  }
+  LCHK(c, hipGetLastError());
+}
 template <typename T>
 void op_ddz_uv(les_ctx* c, T* dfdz, const T* f) {
   const Grid& g = c->g;
@@ -353,8 +392,9 @@ void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, c
                        nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
   launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * (nz - 1) * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
-  T* tops[9] = {RHSx, RHSy, RHSz, u, v, w, RHSx_f, RHSy_f, RHSz_f};
-  for (int q = 0; q < (euler_first ? 9 : 6); ++q) op_fill<T>(c, tops[q] + top, pl, T(BOGUS));
+  T* tops[9] = {RHSx + top, RHSy + top, RHSz + top, u + top, v + top, w + top, RHSx_f + top, RHSy_f + top,
+                RHSz_f + top};
+  op_fill_n<T>(c, tops, euler_first ? 9 : 6, pl, T(BOGUS));
 }
 
 // ------------------------------------------------------------------------
@@ -1326,6 +1366,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_unpadd<T>(les_ctx*, T*, const T*, int);                                                       \
   template void op_ddz_uv<T>(les_ctx*, T*, const T*);                                                            \
   template void op_ddz_w<T>(les_ctx*, T*, const T*);                                                             \
+  template void op_ddz3<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*);                                  \
   template void op_divstress<T>(les_ctx*, T*, const T*, const T*, const T*, int);                                \
   template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
                              const T*, const T*, T*, T*, T*);                                                    \

@@ -523,8 +523,9 @@ void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T*
                        nl, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
   launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * nl * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * nl;
-  T* tops[9] = {R[0], R[1], R[2], u, v, w, Rf[0], Rf[1], Rf[2]};
-  for (int q = 0; q < (euler_first ? 9 : 6); ++q) op_fill<T>(c, tops[q] + top, pl, T(BOGUS));
+  T* tops[9] = {R[0] + top, R[1] + top, R[2] + top, u + top, v + top, w + top, Rf[0] + top, Rf[1] + top,
+                Rf[2] + top};
+  op_fill_n<T>(c, tops, euler_first ? 9 : 6, pl, T(BOGUS));
 }
 
 // step_scalar, scalars_module.f90:457-504 (routine-level entry point)

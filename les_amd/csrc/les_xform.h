@@ -587,6 +587,32 @@ template <typename T>
 __global__ void k_fill(T* a, size_t n, T v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
 }
+// the same value into the first n entries of up to 9 arrays, one launch
+template <typename T>
+struct FillPtrs {
+  T* p[9];
+};
+template <typename T>
+__global__ void k_fill_n(FillPtrs<T> a, int cnt, unsigned n, T v) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (int)blockIdx.y < cnt) a.p[blockIdx.y][i] = v;
+}
+template <typename T>
+void op_fill_n(les_ctx* c, T* const* ptrs, int cnt, size_t n, T val) {
+  if (!n || cnt <= 0) return;
+  if (cnt > 9 || n >= (1u << 31)) {
+    c->err = "op_fill_n: at most 9 arrays below 2^31 entries";
+    return;
+  }
+  FillPtrs<T> a{};
+  for (int q = 0; q < cnt; ++q) a.p[q] = ptrs[q];
+  {
+    ProfScope _ps(c, "fill", (double)n * cnt * sizeof(T));
+    hipLaunchKernelGGL((k_fill_n<T>), dim3((unsigned)((n + 255) / 256), (unsigned)cnt), dim3(256), 0, c->stream, a,
+                       cnt, (unsigned)n, val);
+  }
+  LCHK(c, hipGetLastError());
+}
 template <typename T>
 void op_fill(les_ctx* c, T* a, size_t n, T val) {
   if (!n) return;
