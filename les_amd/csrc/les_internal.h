@@ -151,6 +151,7 @@ template <typename T> void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nl
 template <typename T> void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev);
 template <typename T> void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev);  // null outputs skipped
 template <typename T> void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep);
+template <typename T> void op_test_filter_level0(les_ctx* c, const T* a, const T* b, T* dst, const T* G, int keep);
 template <typename T> void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev);
 template <typename T> void op_dealias2(les_ctx* c, T* u, T* u_big, int nlev);
 template <typename T> void op_padd(les_ctx* c, T* big_c, const T* small_c, int nplanes);

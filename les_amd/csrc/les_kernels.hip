@@ -109,6 +109,28 @@ void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep) {
   launch_row_inv<T>(c, X1, keep, keep, S, rows, g.nx, g.ld, g.ny);
 }
 
+// test_filtermodule.f90:13-30 on the first level of two fields, filtered
+// into two (ld,ny) planes of dst: the same transforms as op_test_filter on
+// copies of those levels, without the copies
+template <typename T>
+void op_test_filter_level0(les_ctx* c, const T* a, const T* b, T* dst, const T* G, int keep) {
+  OpTag _tag(c, "test_filter");
+  const Grid& g = c->g;
+  if (keep <= 0 || keep > g.nxh) keep = g.nxh;
+  const long rows = 2L * g.ny;
+  const size_t xs = (size_t)rows * keep * sizeof(cx<T>);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, xs);
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
+  LoadFields<T> L{{a, b, nullptr, nullptr}, 1, g.ny, g.ld, T(1)};
+  launch_row_fwd<T>(c, L, X0, keep, rows, g.nx, keep, g.ny);
+  ColArgs ca = col_args(g.ny, g.ny, keep, keep, (long)g.ny * keep);
+  ca.lhG = g.lh;
+  col_out(ca, 0, K_G, keep, keep, (long)g.ny * keep);
+  launch_col<T>(c, X0, nullptr, X1, nullptr, nullptr, ca, 2, keep, G);
+  StoreFields<T> S{{dst, nullptr, nullptr, nullptr}, 2, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X1, keep, keep, S, rows, g.nx, g.ld, g.ny);
+}
+
 // dealias1.f90: u (ld,ny,nlev) -> u_big (ld_big,ny2,nlev); u untouched
 template <typename T>
 void op_dealias1(les_ctx* c, const T* u, T* u_big, int nlev) {
@@ -432,9 +454,7 @@ void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz,
   const Grid& g = c->g;
   const size_t pl = (size_t)g.ld * g.ny;
   T* planes = (T*)scratch(c, S_PLANE, 2 * pl * sizeof(T));
-  op_copy<T>(c, planes, u, pl);  // kernels, not memcpy nodes: the step is graph-captured
-  op_copy<T>(c, planes + pl, v, pl);
-  op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);
+  op_test_filter_level0<T>(c, u, v, planes, tables<T>(c).G4, c->keep4);
   {
     ProfScope _ps(c, "wall", 8.0 * g.nx * g.ny * sizeof(T));
     hipLaunchKernelGGL((k_wall<T>), grid1((size_t)g.nx * g.ny), dim3(256), 0, c->stream, u, v, planes, planes + pl,
@@ -1360,6 +1380,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_filt_da3<T>(les_ctx*, T*, T*, T*, T* const[9], int);                                          \
   template void op_ddxy<T>(les_ctx*, T*, T*, const T*, int);                                                     \
   template void op_test_filter<T>(les_ctx*, T*, int, const T*, int);                                             \
+  template void op_test_filter_level0<T>(les_ctx*, const T*, const T*, T*, const T*, int);                       \
   template void op_dealias1<T>(les_ctx*, const T*, T*, int);                                                     \
   template void op_dealias2<T>(les_ctx*, T*, T*, int);                                                           \
   template void op_padd<T>(les_ctx*, T*, const T*, int);                                                         \

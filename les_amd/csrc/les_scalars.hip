@@ -164,9 +164,7 @@ void op_obukhov(les_ctx* c, const T* u, const T* v, const T* theta, double* sgs_
   const ScalarParams& P = c->scal;
   const size_t pl = (size_t)g.ld * g.ny;
   T* planes = (T*)scratch(c, S_PLANE, 2 * pl * sizeof(T));
-  op_copy<T>(c, planes, u, pl);
-  op_copy<T>(c, planes + pl, v, pl);
-  op_test_filter<T>(c, planes, 2, tables<T>(c).G4, c->keep4);  // :560-563
+  op_test_filter_level0<T>(c, u, v, planes, tables<T>(c).G4, c->keep4);  // :560-563
   double* red = (double*)scratch(c, S_RED2, (R2_BETA + g.nz) * sizeof(double));
   level_sums(c, SumPlane<T>{theta, g.ld, th_ref}, 1, red + R2_TSUM, (double)g.nx * g.ny * sizeof(T));
   {
@@ -246,36 +244,40 @@ struct DzArgs {
   double zlog, sflux;     // lag: ln(dz/2/zo_avg), mean surface flux
 };
 
-template <typename T>
-__global__ void k_scalar_dz(DzArgs<T> a, int nx, int ld, int ny, int nz, double dz, T cst) {
-  const long pl = (long)ld * ny, n3 = pl * nz, n = n3 * a.ns;
+// TPT consecutive points per thread, both scalars per point; surface_only
+// launches cover the first level only
+template <typename T, int TPT>
+__global__ void k_scalar_dz(DzArgs<T> a, int nx, int ld, int ny, int nz, double dz, T cst, unsigned n) {
+  const unsigned pl = (unsigned)ld * ny;
   const double fn = (double)(float)(nx * ny);
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const unsigned tt = (unsigned)t, q3 = (unsigned)n3;  // n = ns * n3 < 2^32
-    const int q = (int)(tt / q3);
-    const long i = (long)(tt - (unsigned)q * q3);
-    const int k = (int)((unsigned)i / (unsigned)pl);
+  const unsigned i0 = blockIdx.x * blockDim.x * TPT + threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < TPT; ++t) {
+    const unsigned i = i0 + t * blockDim.x;
+    if (i >= n) break;
+    const int k = (int)(i / pl);
     const int r = (int)(i - k * pl), y = r / ld, x = r - y * ld;
-    if (a.surface_only && k != 0) continue;
-    T* d = a.dz[q];
-    if (k == nz - 1) {
-      d[i] = a.top[q];
-      continue;
-    }
-    if (x >= nx) continue;
-    if (k == 0) {
-      const int j = x + nx * y;
-      double v;
-      if (a.lag) {
-        const double ustar_ = a.lagsums[0] / fn * VONK / (a.zlog - a.lagsums[1] / fn);
-        v = -a.phi_h[j] * a.sflux / (ustar_ * VONK * dz / 2.0);
-      } else {
-        v = -a.phi_h[j] * a.sf[q](j) / (a.ustar[j] * VONK * dz / 2.0);
+    for (int q = 0; q < a.ns; ++q) {
+      T* d = a.dz[q];
+      if (k == nz - 1) {
+        d[i] = a.top[q];
+        continue;
       }
-      d[i] = (T)v;
-    } else {
-      const T* s = a.s[q];
-      d[i] = cst * (s[i] - s[i - pl]);
+      if (x >= nx) continue;
+      if (k == 0) {
+        const int j = x + nx * y;
+        double v;
+        if (a.lag) {
+          const double ustar_ = a.lagsums[0] / fn * VONK / (a.zlog - a.lagsums[1] / fn);
+          v = -a.phi_h[j] * a.sflux / (ustar_ * VONK * dz / 2.0);
+        } else {
+          v = -a.phi_h[j] * a.sf[q](j) / (a.ustar[j] * VONK * dz / 2.0);
+        }
+        d[i] = (T)v;
+      } else {
+        const T* s = a.s[q];
+        d[i] = cst * (s[i] - s[i - pl]);
+      }
     }
   }
 }
@@ -304,11 +306,12 @@ void op_scalar_dz(les_ctx* c, const ScalarJob* jobs, int ns, const double* phi_h
   }
   a.phi_h = phi_h;
   a.ustar = ustar_avg;
-  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz, n = surface_only ? (size_t)g.ld * g.ny : n3;
+  constexpr int TPT = 4;
   {
     ProfScope _ps(c, "scalar_dz", (surface_only ? 0.02 : 2.0) * ns * n3 * sizeof(T));
-    hipLaunchKernelGGL((k_scalar_dz<T>), grid1(ns * n3), dim3(256), 0, c->stream, a, g.nx, g.ld, g.ny, g.nz, g.dz,
-                       T(1.0 / g.dz));
+    hipLaunchKernelGGL((k_scalar_dz<T, TPT>), dim3((unsigned)((n + 256 * TPT - 1) / (256 * TPT))), dim3(256), 0,
+                       c->stream, a, g.nx, g.ld, g.ny, g.nz, g.dz, T(1.0 / g.dz), (unsigned)n);
   }
   LCHK(c, hipGetLastError());
 }
@@ -702,8 +705,8 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
     a.zlog = log(0.5 * g.dz / c->zo_avg);
     a.sflux = c->wt_mean / P.T_scale / P.u_star;  // :114
     ProfScope _ps(c, "scalar_dz", 2.0 * n3 * sizeof(T));
-    hipLaunchKernelGGL((k_scalar_dz<T>), grid1(n3), dim3(256), 0, c->stream, a, g.nx, g.ld, g.ny, nz, g.dz,
-                       T(1.0 / g.dz));
+    hipLaunchKernelGGL((k_scalar_dz<T, 4>), dim3((unsigned)((n3 + 1023) / 1024)), dim3(256), 0, c->stream, a, g.nx,
+                       g.ld, g.ny, nz, g.dz, T(1.0 / g.dz), (unsigned)n3);
   }
   LCHK(c, hipGetLastError());
   const T* F = reuse_mom ? (const T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T)) : lag_filter_mom<T>(c, S, u, v, w);
