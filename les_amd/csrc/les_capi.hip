@@ -1287,7 +1287,7 @@ int les_set_tune(les_ctx* c, const int* in8) {
   if (!in8) return -1;
   const int* in6 = in8;
   const bool ok32 = in6[0] == 8 || in6[0] == 16 || in6[0] == 32, ok64 = in6[1] == 4 || in6[1] == 8 || in6[1] == 16;
-  const bool okt = in6[2] >= 0 && in6[2] <= 2 && in6[3] >= 32 && in6[3] <= 1024 && in6[3] % 32 == 0 &&
+  const bool okt = in6[2] >= 0 && in6[2] <= 2 && in6[3] >= 32 && in6[3] <= 512 && in6[3] % 32 == 0 &&
                    tridiag_lds(in6[2], in6[3], c->g.nz) <= 160 * 1024;
   const bool okp = (in6[4] == 0 || (in6[4] == 2 && (in6[0] == 8 || in6[0] == 16))) &&
                    (in6[5] == 0 || (in6[5] == 2 && (in6[1] == 4 || in6[1] == 8)));

@@ -927,98 +927,104 @@ struct LoadPlanes2 {  // two planes of one field: level 0 and level kt
   }
 };
 
-// One thread per horizontal mode (kx < nx/2, all ky); Thomas algorithm in
-// FP64 along z with the boundary rows of press_stag_array.f90:142-165 and the
+// Two threads per horizontal mode (kx < nx/2, all ky), one per component of
+// the complex right-hand side (the matrix is real); Thomas algorithm in FP64
+// along z with the boundary rows of press_stag_array.f90:142-165 and the
 // zero-mode integration of :238-247.  H: 3 fields x (nz-1) planes of
-// (ny, nxh) complex; BT: bottom/top planes; P: (nz) planes of p_hat.
+// (ny, nxh) complex; BT: bottom/top planes; P: (nz) planes of p_hat.  The
+// real part of i (kx Hx + ky Hy) takes the imaginary parts of Hx, Hy and vice
+// versa.  Both threads of a mode form the same elimination factors and each
+// writes and re-reads its own copy of gamma's slot (same address, same value).
 // MODE (the schedule les_autotune picks): where the sweep keeps gamma and
 // the eliminated right-hand side U -- 0 both in HBM scratch, 1 gamma in LDS,
-// 2 both in LDS (one column of (nz+1) values per thread).
+// 2 both in LDS (one column of (nz+1) values per mode / component).
 template <typename T, int MODE>
 __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__ BT, cx<T>* __restrict__ P,
                           double* __restrict__ gam_g, cx<double>* __restrict__ U_g, const T* __restrict__ kxt,
                           const T* __restrict__ kyt, const double* __restrict__ kxd, const double* __restrict__ kyd,
                           int nxh, int ny, int nz, double dz, int* err) {
   extern __shared__ __align__(16) unsigned char tri_smem[];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = t >> 1, comp = t & 1;
   const int nm = nxh * ny;
-  if (m >= nm) return;
-  const int tid = threadIdx.x, bd = blockDim.x;
+  if (m >= nm) return;  // blockDim is even: both threads of a mode leave together
+  const int tid = threadIdx.x, bd = blockDim.x, bm = bd >> 1, tm = tid >> 1;
   double* gl = reinterpret_cast<double*>(tri_smem);
-  cx<double>* ul = reinterpret_cast<cx<double>*>(tri_smem + (size_t)(nz + 1) * bd * sizeof(double));
-  auto gam = [&](int j) -> double& { return MODE >= 1 ? gl[j * bd + tid] : gam_g[(long)j * nm + m]; };
-  auto U = [&](int j) -> cx<double>& { return MODE >= 2 ? ul[j * bd + tid] : U_g[(long)j * nm + m]; };
+  double* ul = reinterpret_cast<double*>(tri_smem + (size_t)(nz + 1) * bm * sizeof(double));
+  double* Ug = reinterpret_cast<double*>(U_g);
+  auto gam = [&](int j) -> double& { return MODE >= 1 ? gl[j * bm + tm] : gam_g[(long)j * nm + m]; };
+  auto U = [&](int j) -> double& { return MODE >= 2 ? ul[j * bd + tid] : Ug[2 * ((long)j * nm + m) + comp]; };
   const int y = m / nxh, x = m - y * nxh;
   const long ps = (long)nm;               // plane stride (complex)
   const long fs = ps * (nz - 1);          // field stride in H
-  const cx<T>* Hx = H;
-  const cx<T>* Hy = H + fs;
-  const cx<T>* Hz = H + 2 * fs;
-  const cx<double> bot = {(double)BT[m].x, (double)BT[m].y};
-  const cx<double> top = {(double)BT[ps + m].x, (double)BT[ps + m].y};
+  const T* Hx = reinterpret_cast<const T*>(H);
+  const T* Hy = reinterpret_cast<const T*>(H + fs);
+  const T* Hz = reinterpret_cast<const T*>(H + 2 * fs);
+  const T* B2 = reinterpret_cast<const T*>(BT);
+  T* P2 = reinterpret_cast<T*>(P);
+  auto at = [&](long j) { return 2 * (j * ps + m); };  // element of mode m at plane j
+  const double bot = (double)B2[at(0) + comp], top = (double)B2[at(1) + comp];
   if (y == ny / 2) {
-    for (int j = 0; j < nz; ++j) P[j * ps + m] = {T(0), T(0)};
+    for (int j = 0; j < nz; ++j) P2[at(j) + comp] = T(0);
     return;
   }
   if (m == 0) {
-    cx<double> pv = {0.0, 0.0};
-    P[0] = {T(0), T(0)};
-    pv = {pv.x - dz * bot.x, pv.y - dz * bot.y};
-    P[ps] = {(T)pv.x, (T)pv.y};
+    double pv = 0.0;
+    P2[at(0) + comp] = T(0);
+    pv = pv - dz * bot;
+    P2[at(1) + comp] = (T)pv;
     for (int j = 2; j < nz; ++j) {
-      const cx<T> h = Hz[(long)(j - 1) * ps];
-      pv = {pv.x + (double)h.x * dz, pv.y + (double)h.y * dz};
-      P[j * ps + m] = {(T)pv.x, (T)pv.y};
+      pv = pv + (double)Hz[at(j - 1) + comp] * dz;
+      P2[at(j) + comp] = (T)pv;
     }
     return;
   }
   const double kx = kxd[x], ky = kyd[y];
   const double a_in = 1.0 / (dz * dz), c_in = 1.0 / (dz * dz);
   const double b_in = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+  const int oc = 1 - comp;                 // component of Hx, Hy this one takes
+  const double sg = comp == 0 ? -1.0 : 1.0;
   // row 0: b=-1, c=1, r=-dz*bottom
   double bet = -1.0;
-  cx<double> u0 = {(-dz * bot.x) / bet, (-dz * bot.y) / bet};
+  double u0 = (-dz * bot) / bet;
   U(0) = u0;
   double cprev = 1.0;
-  cx<double> uprev = u0;
-  cx<double> hxm = {(double)Hx[m].x, (double)Hx[m].y};
-  cx<double> hym = {(double)Hy[m].x, (double)Hy[m].y};
-  cx<double> hzm = {(double)Hz[m].x, (double)Hz[m].y};
+  double uprev = u0;
+  double hxm = (double)Hx[at(0) + oc], hym = (double)Hy[at(0) + oc], hzm = (double)Hz[at(0) + comp];
   for (int j = 1; j <= nz; ++j) {
-    double aj, bj;
-    cx<double> rj;
+    double aj, bj, rj;
     if (j < nz) {
       aj = a_in;
       bj = b_in;
-      cx<double> hz = {0.0, 0.0};
-      if (j < nz - 1) hz = {(double)Hz[(long)j * ps + m].x, (double)Hz[(long)j * ps + m].y};
-      const double sr = kx * hxm.x + ky * hym.x, si = kx * hxm.y + ky * hym.y;
-      rj = {-si + (hz.x - hzm.x) / dz, sr + (hz.y - hzm.y) / dz};
+      double hz = 0.0;
+      if (j < nz - 1) hz = (double)Hz[at(j) + comp];
+      const double sv = kx * hxm + ky * hym;
+      rj = sg * sv + (hz - hzm) / dz;
       if (j < nz - 1) {
-        hxm = {(double)Hx[(long)j * ps + m].x, (double)Hx[(long)j * ps + m].y};
-        hym = {(double)Hy[(long)j * ps + m].x, (double)Hy[(long)j * ps + m].y};
+        hxm = (double)Hx[at(j) + oc];
+        hym = (double)Hy[at(j) + oc];
         hzm = hz;
       }
     } else {
       aj = -1.0;
       bj = 1.0;
-      rj = {-top.x * dz, -top.y * dz};
+      rj = -top * dz;
     }
     const double gj = cprev / bet;
     bet = bj - aj * gj;
     if (bet == 0.0) atomicExch(err, 2);
     gam(j) = gj;
-    uprev = {(rj.x - aj * uprev.x) / bet, (rj.y - aj * uprev.y) / bet};
+    uprev = (rj - aj * uprev) / bet;
     U(j) = uprev;
     cprev = j < nz ? c_in : 0.0;
   }
   // back substitution; p_hat levels 0..nz-1 are kept (level nz is BOGUS later)
-  cx<double> un = uprev;
+  double un = uprev;
   for (int j = nz - 1; j >= 0; --j) {
-    cx<double> uj = U(j);
+    double uj = U(j);
     const double gn = gam(j + 1);
-    uj = {uj.x - gn * un.x, uj.y - gn * un.y};
-    P[(long)j * ps + m] = {(T)uj.x, (T)uj.y};
+    uj = uj - gn * un;
+    P2[at(j) + comp] = (T)uj;
     un = uj;
   }
 }
@@ -1038,7 +1044,8 @@ struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: 
   }
 };
 
-// LDS bytes of a tridiagonal schedule (0: the mode does not use LDS)
+// LDS bytes of a tridiagonal schedule with `block` modes per workgroup (0:
+// the mode does not use LDS): gamma per mode, U per mode and component
 size_t tridiag_lds(int mode, int block, int nz) {
   return (size_t)(nz + 1) * block * (mode >= 1 ? sizeof(double) : 0) +
          (size_t)(nz + 1) * block * (mode >= 2 ? sizeof(cx<double>) : 0);
@@ -1051,7 +1058,8 @@ void launch_tridiag(les_ctx* c, const cx<T>* H, const cx<T>* BT, cx<T>* PH, doub
   const long ps = (long)g.ny * nxh;
   const Tables<T>& t = tables<T>(c);
   const size_t sm = tridiag_lds(mode, bs, nz);
-  const dim3 grid((unsigned)((ps + bs - 1) / bs)), block((unsigned)bs);
+  // bs modes per block, two threads per mode
+  const dim3 grid((unsigned)((ps + bs - 1) / bs)), block((unsigned)(2 * bs));
   ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) +
                                   (nz + 1) * ps * 2.0 * ((mode >= 1 ? 0 : sizeof(double)) +
                                                          (mode >= 2 ? 0 : sizeof(cx<double>))));
