@@ -33,6 +33,11 @@ __device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
 
 #define LCHK(c, call) check((c), (call), #call)
 
+// e / d for 0 <= e, 0 < d and e + d < 2^22 through a float reciprocal:
+// (e + 1/2)/d sits at least 1/(2d) from an integer and the two roundings move
+// it by less than (e/d + 1) 2^-23 < 1/(2d), so the truncation is exact
+__device__ __forceinline__ int div_small(int e, float inv_d) { return (int)(((float)e + 0.5f) * inv_d); }
+
 
 template <typename T>
 static const cx<T>* tw_for(const les_ctx* c, int n) {
@@ -120,8 +125,9 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, -1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
+  const float inv_kxo = 1.0f / (float)kxo;
   for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
-    const int t = e / kxo, k = e - t * kxo, pp = pair0 + t;
+    const int t = div_small(e, inv_kxo), k = e - t * kxo, pp = pair0 + t;
     if (pp >= npp) continue;
     const cx<T> zk = buf[ix(t, k)], zn = buf[ix(t, k ? n - k : 0)];
     const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
@@ -208,8 +214,9 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, +1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
+  const float inv_ldo = 1.0f / (float)ldo;
   for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
-    const int t = e / ldo, x = e - t * ldo, pp = pair0 + t;
+    const int t = div_small(e, inv_ldo), x = e - t * ldo, pp = pair0 + t;
     if (pp >= npp) continue;
     T v0 = T(0), v1 = T(0);
     if (x < n) {
