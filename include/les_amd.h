@@ -44,7 +44,9 @@ typedef struct les_sgs_fields {
 } les_sgs_fields;
 
 /* ---- context (replaces init_fft fft.f90:35-51, init_wavenumber
-   fft.f90:54-81 and the two test_filter_init calls main.f90:166-170) ---- */
+   fft.f90:54-81 and the two test_filter_init calls main.f90:166-170).
+   Grids: nx, ny multiples of 8 of the form 2^a 3^b up to 1024, nz >= 3, and
+   (nx + 2) ny (nz + 1) < 2^31 points per array; returns 2 otherwise. ---- */
 int les_create(const les_grid *g, les_ctx **out);
 void les_destroy(les_ctx *c);
 const char *les_last_error(les_ctx *c);

@@ -36,3 +36,19 @@ def test_product_path_does_not_import_oracle():
         if f.endswith((".py", ".hip", ".h", ".cpp", ".f90")):
             txt = open(f, errors="ignore").read().lower()
             assert "oracle" not in txt, f"product file {f} mentions the oracle"
+
+
+def test_create_rejects_unsupported_grids():
+    """les_create validates the grid before it touches a device: lengths that
+    are not 2^a 3^b multiples of 8, nz < 3, and arrays of 2^31 points or more
+    return 2."""
+    import les_amd
+    from les_amd._lib import Grid
+    lib = ctypes.CDLL(les_amd.LIB_PATH)
+    lib.les_create.restype = ctypes.c_int
+    lib.les_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(ctypes.c_void_p)]
+    for nx, ny, nz in ((20, 32, 9), (32, 40, 9), (32, 32, 2), (1024, 1024, 2100), (2048, 16, 9)):
+        g = Grid(nx=nx, ny=ny, nz=nz, Lx=1.0, Ly=1.0, Lz=1.0, dt=1e-4, z0=1e-4, force=1.0)
+        out = ctypes.c_void_p()
+        assert lib.les_create(ctypes.byref(g), ctypes.byref(out)) == 2, (nx, ny, nz)
+        assert not out.value
