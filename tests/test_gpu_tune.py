@@ -78,25 +78,27 @@ def test_radix8_column_plan(torch_mod, prec, tx):
         assert err < tol, (n, err)
 
 
-@pytest.mark.parametrize("prec,conv", [("fp64", True), ("fp32", True), ("fp64", False), ("fp32", False)])
-def test_streaming_padded_products(torch_mod, prec, conv):
+@pytest.mark.parametrize("prec,conv,n,steps", [("fp64", True, 64, 8), ("fp32", True, 64, 8), ("fp64", False, 64, 8),
+                                                ("fp32", False, 64, 8), ("fp64", True, 256, 3),
+                                                ("fp32", False, 256, 3)])
+def test_streaming_padded_products(torch_mod, prec, conv, n, steps):
     """Streaming level runs (radix-8 plan) against level groups (two-pass
-    plan) for the padded products, on a 64^2 grid (padded length 96, where the
-    streaming kernel exists): momentum with both scalars, and momentum alone.
-    Tolerances as for the radix-8 column plan."""
-    print("streaming padded products", prec, conv, flush=True)
+    plan) for the padded products, on 64^2 and 256^2 grids (padded lengths 96
+    and 384; the 384 FP64 ring is the largest LDS tile): momentum with both
+    scalars, and momentum alone.  Tolerances as for the radix-8 column plan."""
+    print("streaming padded products", prec, conv, n, flush=True)
     key = "pad_mode_" + prec
-    base, _ = run(prec, conv=conv, n=64, steps=8)
-    s, _ = run(prec, tune={key: 1}, conv=conv, n=64, steps=8)
+    base, _ = run(prec, conv=conv, n=n, steps=steps)
+    s, _ = run(prec, tune={key: 1}, conv=conv, n=n, steps=steps)
     assert s.get_tune()[key] == 1
     tol = 1e-12 if prec == "fp64" else 2e-4
     names = ("u", "v", "w", "p") + (("theta",) if conv else ())
-    for n in names:
-        a, b = s.get(n)[..., :64], base.get(n)[..., :64]
+    for nm in names:
+        a, b = s.get(nm)[..., :n], base.get(nm)[..., :n]
         err = np.max(np.abs(a - b)) / np.max(np.abs(b))
-        print(n, err)
-        assert err < tol, (n, err)
-    assert any(not np.array_equal(s.get(n), base.get(n)) for n in names)  # the streaming kernel ran
+        print(nm, err)
+        assert err < tol, (nm, err)
+    assert any(not np.array_equal(s.get(nm), base.get(nm)) for nm in names)  # the streaming kernel ran
 
 
 def test_set_tune_rejects_bad_shapes(torch_mod):
