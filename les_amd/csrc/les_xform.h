@@ -380,7 +380,11 @@ struct ColGeom {
 };
 
 // NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
-template <typename T, int TX, int NYI, int NYO, int PLAN = 0>
+// IP: the single identity output of a zero-pad or truncation (padd.f90 /
+// unpadd.f90) resampled in place in the input tile: the rows that move never
+// overlap their destinations, so the tile needs max(NYI, NYO) rows and no
+// second buffer (more workgroups per CU, the same values).
+template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false>
 __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
     k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
           FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
@@ -441,6 +445,37 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
       }
     }
   };
+  if constexpr (IP && NYI > 0 && NYI != NYO) {
+    cx<T>* R = two + nyo;  // TX * max(NYI, NYO), in place of A and B
+    load_tile(R, in + pl * a.in_plane);
+    __syncthreads();
+    fwd(R);
+    if constexpr (NYO > NYI) {
+      // rows NYI/2+1 .. NYI-1 move up by NYO-NYI, then rows NYI/2 .. NYO-NYI/2 are zero
+      constexpr int NMV = (NYI / 2 - 1) * TX, NZR = (NYO - NYI + 1) * TX;
+      for (int e = threadIdx.x; e < NMV; e += BLK) {
+        const int yi = NYI / 2 + 1 + e / TX, c = e % TX;
+        R[(yi + NYO - NYI) * TX + c] = R[yi * TX + c];
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < NZR; e += BLK) R[(NYI / 2) * TX + e] = cx<T>{T(0), T(0)};
+    } else {
+      // rows NYI-NYO/2+1 .. NYI-1 move down by NYI-NYO, row NYO/2 is zero
+      constexpr int NMV = (NYO / 2 - 1) * TX;
+      for (int e = threadIdx.x; e < NMV; e += BLK) {
+        const int yo = NYO / 2 + 1 + e / TX, c = e % TX;
+        R[yo * TX + c] = R[(yo + NYI - NYO) * TX + c];
+      }
+      for (int e = threadIdx.x; e < TX; e += BLK) R[(NYO / 2) * TX + e] = cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    inv(R);
+    for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = R[e];
+    }
+    return;
+  }
   load_tile(A, in + pl * a.in_plane);
   if (in2) load_tile(B, in2 + pl * a.in2_plane);
   __syncthreads();
@@ -511,7 +546,7 @@ constexpr bool col_tile_fits() {
          (size_t)(NYI + NYO + TX * NYI + TX * NYO) * sizeof(cx<T>) <= 160 * 1024;
 }
 
-template <typename T, int TX, int NYI, int NYO, int PLAN = 0>
+template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false>
 static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0, cx<T>* o1, cx<T>* o2,
                          const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
   const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
@@ -522,9 +557,9 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     c->err = "k_col: column tile too large";
     return;
   }
-  const size_t sm = (size_t)(a.ny_in + a.ny_out + TX * a.ny_in + TX * a.ny_out) * sizeof(cx<T>);
+  const size_t sm = (size_t)(a.ny_in + a.ny_out + (IP ? TX * nmax : TX * a.ny_in + TX * a.ny_out)) * sizeof(cx<T>);
   if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO, PLAN>,
+    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO, PLAN, IP>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   const Tables<T>& t = tables<T>(c);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
@@ -535,7 +570,7 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
   }
   ProfScope ps(c, a.ny_in < a.ny_out ? "col_pad" : (a.ny_in > a.ny_out ? "col_trunc" : "col"), bytes);
-  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
+  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN, IP>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
                      o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
                      tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
   LCHK(c, hipGetLastError());
@@ -544,6 +579,31 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
 #define LES_COL_PAIRS(X)                                                                                       \
   X(16, 16) X(16, 24) X(24, 16) X(32, 32) X(32, 48) X(48, 32) X(64, 64) X(64, 96) X(96, 64) X(96, 96) X(96, 144) \
   X(144, 96) X(128, 128) X(128, 192) X(192, 128) X(256, 256) X(256, 384) X(384, 256)
+
+// zero-pad / truncation passes with a single identity output: the in-place
+// tile at 128-byte rows (16 FP32 / 8 FP64 columns), in the column plan the
+// autotuner chose; false for lengths without a compile-time variant
+#define LES_COL_RESAMPLE(X) \
+  X(16, 24) X(24, 16) X(32, 48) X(48, 32) X(64, 96) X(96, 64) X(96, 144) X(144, 96) X(128, 192) X(192, 128) \
+  X(256, 384) X(384, 256)
+template <typename T>
+static bool launch_col_ip(les_ctx* c, int plan, const cx<T>* in, cx<T>* o0, const ColArgs& a, long nplanes,
+                          int ncols) {
+  constexpr int TXI = sizeof(T) == 8 ? 8 : 16;
+#define LES_CASE(I, O)                                                                                             \
+  if (a.ny_in == I && a.ny_out == O) {                                                                             \
+    if (plan == 2)                                                                                                 \
+      launch_col_n<T, TXI, I, O, 2, true>(c, in, nullptr, o0, nullptr, nullptr, a, nplanes, ncols, nullptr,      \
+                                          nullptr, nullptr);                                                       \
+    else                                                                                                           \
+      launch_col_n<T, TXI, I, O, 0, true>(c, in, nullptr, o0, nullptr, nullptr, a, nplanes, ncols, nullptr,      \
+                                          nullptr, nullptr);                                                       \
+    return true;                                                                                                   \
+  }
+  LES_COL_RESAMPLE(LES_CASE)
+#undef LES_CASE
+  return false;
+}
 
 // the tuned tile widths / FFT plans other than the default (les_coltx.hip);
 // false when that variant does not exist for these lengths
@@ -557,6 +617,9 @@ static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0,
                        const T* G2 = nullptr) {
   constexpr int TXD = col_tx_default<T>();
   const int tx = c->tune.col_tx[sizeof(T) == 8 ? 1 : 0], plan = c->tune.col_plan[sizeof(T) == 8 ? 1 : 0];
+  if (a.nout == 1 && a.kind[0] == K_ID && !in2 && a.do_fwd && a.do_inv && a.ny_in != a.ny_out &&
+      launch_col_ip<T>(c, plan, in, o0, a, nplanes, ncols))
+    return;
   if ((tx != TXD || plan != 0) && launch_col_alt<T>(c, tx, plan, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2))
     return;
 #define LES_CASE(I, O)                                                                        \
