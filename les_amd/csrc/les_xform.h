@@ -482,16 +482,43 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
   if (a.do_fwd) fwd(A);
   if (in2) {
     // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
-    fwd(B);
-    for (int e = threadIdx.x; e < nyi * TX; e += bd) {
-      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      cx<T> r = {T(0), T(0)};
-      if (y != nyi / 2 && kk < a.nki) {
-        const T kx = kxt[kk], ky = kyt[y];
-        const cx<T> X = A[e], Y = B[e];
-        r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
+    if constexpr (CT) {
+      // wavenumbers first (a thread keeps one column), then the transform
+      constexpr int NE = (CT ? NYI : 1) * TX, ME = (NE + (CT ? BLK : 64) - 1) / (CT ? BLK : 64);
+      const int c0 = threadIdx.x % TX;
+      const T kx = kx0 + c0 < a.nki ? kxt[kx0 + c0] : T(0);
+      T ky[ME];
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        ky[i] = (NE % BLK == 0 || e < NE) ? kyt[e / TX] : T(0);
       }
-      A[e] = r;
+      fwd(B);
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NE % BLK == 0 || e < NE) {
+          const int y = e / TX, kk = kx0 + c0;
+          cx<T> r = {T(0), T(0)};
+          if (y != NYI / 2 && kk < a.nki) {
+            const cx<T> X = A[e], Y = B[e];
+            r = {-kx * X.y - ky[i] * Y.y, kx * X.x + ky[i] * Y.x};
+          }
+          A[e] = r;
+        }
+      }
+    } else {
+      fwd(B);
+      for (int e = threadIdx.x; e < nyi * TX; e += bd) {
+        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+        cx<T> r = {T(0), T(0)};
+        if (y != nyi / 2 && kk < a.nki) {
+          const T kx = kxt[kk], ky = kyt[y];
+          const cx<T> X = A[e], Y = B[e];
+          r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
+        }
+        A[e] = r;
+      }
     }
     __syncthreads();
     if (a.do_inv) inv(A);
