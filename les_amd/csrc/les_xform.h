@@ -476,6 +476,110 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
     }
     return;
   }
+  if constexpr (CT && NYI == NYO && !IP) {
+    // same-length passes: the spectrum stays in registers (each thread its
+    // own tile elements) and one LDS tile serves every transform
+    constexpr int NE = (CT ? NYI : 1) * TX, ME = (NE + (CT ? BLK : 64) - 1) / (CT ? BLK : 64);
+    cx<T>* R = two + nyo;
+    cx<T> ra[ME], rb[ME];
+    auto fetch = [&](cx<T> (&r)[ME], const cx<T>* src) {
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        r[i] = cx<T>{T(0), T(0)};
+        if (NE % BLK == 0 || e < NE) {
+          const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+          if (kk < a.nki) r[i] = src[(long)y * a.in_row + kk];
+        }
+      }
+    };
+    auto put = [&](const cx<T> (&r)[ME]) {
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NE % BLK == 0 || e < NE) R[e] = r[i];
+      }
+    };
+    auto get = [&](cx<T> (&r)[ME]) {
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NE % BLK == 0 || e < NE) r[i] = R[e];
+      }
+    };
+    fetch(ra, in + pl * a.in_plane);
+    if (in2) fetch(rb, in2 + pl * a.in2_plane);
+    if (a.do_fwd) {
+      put(ra);
+      __syncthreads();
+      fwd(R);
+      get(ra);
+    }
+    if (in2) {
+      // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
+      const int c0 = threadIdx.x % TX;
+      const T kx = kx0 + c0 < a.nki ? kxt[kx0 + c0] : T(0);
+      put(rb);
+      __syncthreads();
+      fwd(R);
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NE % BLK == 0 || e < NE) {
+          const int y = e / TX, kk = kx0 + c0;
+          cx<T> r = {T(0), T(0)};
+          if (y != NYI / 2 && kk < a.nki) {
+            const T ky = kyt[y];
+            const cx<T> X = ra[i], Y = R[e];
+            r = {-kx * X.y - ky * Y.y, kx * X.x + ky * Y.x};
+          }
+          R[e] = r;
+        }
+      }
+      __syncthreads();
+      if (a.do_inv) inv(R);
+      for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
+        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+        if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = R[e];
+      }
+      return;
+    }
+    for (int j = 0; j < a.nout; ++j) {
+      if (kx0 >= a.nko[j]) continue;  // uniform over the block
+      const int kind = a.kind[j];
+      const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
+#pragma unroll
+      for (int i = 0; i < ME; ++i) {
+        const int e = threadIdx.x + i * BLK;
+        if (NE % BLK == 0 || e < NE) {
+          const int yo = e / TX, c = e - yo * TX, kk = kx0 + c;
+          cx<T> v = {T(0), T(0)};
+          if (yo != NYO / 2 && kk < a.nki) {  // map_row for equal lengths: the Nyquist row is zero
+            v = ra[i];
+            if (kind == K_IKX) {
+              const T k = kxt[kk];
+              v = {-k * v.y, k * v.x};
+            } else if (kind == K_IKY) {
+              const T k = kyt[yo];
+              v = {-k * v.y, k * v.x};
+            } else if (kind == K_G) {
+              const T g = G[(long)yo * a.lhG + kk];
+              v = {g * v.x, g * v.y};
+            }
+          }
+          R[e] = v;
+        }
+      }
+      __syncthreads();
+      if (a.do_inv) inv(R);
+      cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
+      for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
+        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+        if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = R[e];
+      }
+    }
+    return;
+  }
   load_tile(A, in + pl * a.in_plane);
   if (in2) load_tile(B, in2 + pl * a.in2_plane);
   __syncthreads();
@@ -584,7 +688,10 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     c->err = "k_col: column tile too large";
     return;
   }
-  const size_t sm = (size_t)(a.ny_in + a.ny_out + (IP ? TX * nmax : TX * a.ny_in + TX * a.ny_out)) * sizeof(cx<T>);
+  // LDS tiles: in-place resampling and same-length passes (spectrum kept in
+  // registers) use one, the rest two
+  constexpr bool ONE = NYI > 0 && (IP || NYI == NYO);
+  const size_t sm = (size_t)(a.ny_in + a.ny_out + (ONE ? TX * nmax : TX * a.ny_in + TX * a.ny_out)) * sizeof(cx<T>);
   if (sm > 65536)
     LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO, PLAN, IP>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
