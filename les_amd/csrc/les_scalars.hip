@@ -360,26 +360,48 @@ struct StoreScalarRHS {
   T rdz, dt;
   int do_step, euler;
   static constexpr double writes() { return 7.0; }
-  __device__ void operator()(int plane, int y, int x, T v) const {
+  // load phase: every array value the update of point (plane, y, x) reads
+  struct Pre {
+    T p0, p1, nu0, nu1, d0, d1, rf, s0;
+    double flux;
+  };
+  __device__ Pre pre(int plane, int y, int x) const {
+    const int q = plane / ppf, k = plane - q * ppf;
+    const long pl = (long)ld * ny, i = ((long)k * ny + y) * ld + x;
+    Pre p{};
+    p.s0 = do_step ? s[q][i] : T(0);
+    if (x >= nx) return p;
+    p.p0 = Pr ? Pr[i] : Prc;
+    p.p1 = Pr ? Pr[i + pl] : Prc;
+    if (k == 0) {
+      p.flux = sf[q](x + nx * y);
+    } else {
+      p.nu0 = Nu[i];
+      p.d0 = dsdz[q][i];
+    }
+    p.nu1 = Nu[i + pl];
+    p.d1 = dsdz[q][i + pl];
+    p.rf = (do_step && !euler) ? Rf[q][i] : T(0);
+    return p;
+  }
+  __device__ void fin(int plane, int y, int x, T v, const Pre& p) const {
     const int q = plane / ppf, k = plane - q * ppf;
     const long pl = (long)ld * ny, i = ((long)k * ny + y) * ld + x;
     T* Rq = R[q];
     if (k == nz - 2) Rq[i + pl] = T(0);  // level nz: not defined by the reference, 0 here
     if (x >= nx) {
       Rq[i] = T(0);
-      if (do_step && k == nz - 2) s[q][i + pl] = s[q][i] + gz[q];
+      if (do_step && k == nz - 2) s[q][i + pl] = p.s0 + gz[q];
       return;
     }
-    const T p0 = Pr ? Pr[i] : Prc, p1 = Pr ? Pr[i + pl] : Prc;
     T t0;
     if (k == 0) {
-      const double f = sf[q](x + nx * y);
-      sv1[q][x + (long)ld * y] = f;
-      t0 = (T)(-1.0 * f);
+      sv1[q][x + (long)ld * y] = p.flux;
+      t0 = (T)(-1.0 * p.flux);
     } else {
-      t0 = (T(1) / p0) * Nu[i] * dsdz[q][i];
+      t0 = (T(1) / p.p0) * p.nu0 * p.d0;
     }
-    const T t1 = (T(1) / p1) * Nu[i + pl] * dsdz[q][i + pl];
+    const T t1 = (T(1) / p.p1) * p.nu1 * p.d1;
     const T rhs = v + rdz * (t1 - t0);
     Rq[i] = rhs;
     if (!do_step) return;
@@ -388,12 +410,13 @@ struct StoreScalarRHS {
       fo = rhs;
       Rf[q][i] = rhs;
     } else {
-      fo = Rf[q][i];
+      fo = p.rf;
     }
-    const T sn = s[q][i] + dt * (T(1.5) * rhs - T(0.5) * fo);
+    const T sn = p.s0 + dt * (T(1.5) * rhs - T(0.5) * fo);
     s[q][i] = sn;
     if (k == nz - 2) s[q][i + pl] = sn + gz[q];
   }
+  __device__ void operator()(int plane, int y, int x, T v) const { fin(plane, y, x, v, pre(plane, y, x)); }
 };
 
 // horizontal SGS flux divergence of the scalars (spectral), plus -advection

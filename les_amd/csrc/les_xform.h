@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "les_internal.h"
 
@@ -138,6 +139,17 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   }
 }
 
+// a storer with a load phase: Pre pre(plane, y, x) reads what the update
+// needs, fin(plane, y, x, v, pre) computes and writes
+template <class S, class = void>
+struct StorerHasPre {
+  static constexpr bool value = false;
+};
+template <class S>
+struct StorerHasPre<S, std::void_t<typename S::Pre>> {
+  static constexpr bool value = true;
+};
+
 template <typename T, int N, class Storer>
 __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
@@ -214,6 +226,38 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, +1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
+  if constexpr (N > 0 && StorerHasPre<Storer>::value) {
+    // storers that read the arrays they update: every read of the block's
+    // points is issued before the first write (full rows, ldo = N + 2)
+    if (ldo == N + 2) {
+      constexpr int LDO = N + 2, NS = RowGeom<N>::P * LDO, BL = RowGeom<N>::BLOCK, MS = (NS + BL - 1) / BL;
+      typename Storer::Pre pre[MS][2];
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int e = threadIdx.x + i * BL;
+        const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
+        if ((NS % BL == 0 || e < NS) && pp < npp) {
+          pre[i][0] = S.pre(plane, 2 * pp, x);
+          pre[i][1] = S.pre(plane, 2 * pp + 1, x);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int e = threadIdx.x + i * BL;
+        const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
+        if ((NS % BL == 0 || e < NS) && pp < npp) {
+          T v0 = T(0), v1 = T(0);
+          if (x < N) {
+            v0 = buf[ix(t, x)].x;
+            v1 = buf[ix(t, x)].y;
+          }
+          S.fin(plane, 2 * pp, x, v0, pre[i][0]);
+          S.fin(plane, 2 * pp + 1, x, v1, pre[i][1]);
+        }
+      }
+      return;
+    }
+  }
   const float inv_ldo = 1.0f / (float)ldo;
   for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
     const int t = div_small(e, inv_ldo), x = e - t * ldo, pp = pair0 + t;
@@ -898,25 +942,39 @@ struct StoreConvecAB2 {
   int euler;
   const T* beta;  // buoyancy added to RHSz (main.f90:312-317), or null
   static constexpr double writes() { return 5.0; }
-  __device__ void operator()(int plane, int y, int x, T v) const {
+  struct Pre {
+    T dv, beta, rf, u;
+  };
+  __device__ Pre pre(int plane, int y, int x) const {
+    const int f = plane / ppf, r = plane - f * ppf;
+    const long i = ((long)r * ny + y) * ld + x;
+    Pre p;
+    p.dv = dv[f][i];
+    p.beta = (f == 2 && beta) ? beta[i] : T(0);
+    p.rf = euler ? T(0) : Rf[f][i];
+    p.u = U[f][i];
+    return p;
+  }
+  __device__ void fin(int plane, int y, int x, T v, const Pre& p) const {
     const int f = plane / ppf, r = plane - f * ppf;
     const long i = ((long)r * ny + y) * ld + x;
     const T conv = x < nx ? v : T(0);
-    T rhs = -conv - dv[f][i];
-    if (f == 2 && beta) rhs = rhs + beta[i];
+    T rhs = -conv - p.dv;
+    if (f == 2 && beta) rhs = rhs + p.beta;
     R[f][i] = rhs;
     T fo;
     if (euler) {
       fo = rhs;
       Rf[f][i] = rhs;
     } else {
-      fo = Rf[f][i];
+      fo = p.rf;
     }
     if (f == 0)
-      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo + force);
+      U[f][i] = p.u + dt * (T(TADV1) * rhs + T(TADV2) * fo + force);
     else
-      U[f][i] = U[f][i] + dt * (T(TADV1) * rhs + T(TADV2) * fo);
+      U[f][i] = p.u + dt * (T(TADV1) * rhs + T(TADV2) * fo);
   }
+  __device__ void operator()(int plane, int y, int x, T v) const { fin(plane, y, x, v, pre(plane, y, x)); }
 };
 
 
