@@ -301,22 +301,38 @@ struct StoreDiv {
   int ny, ld, nx, nz, wnode;
   T rdz;
   static constexpr double writes() { return 2.0; }  // divt + tz read
-  __device__ void operator()(int plane, int y, int x, T v) const {
+  struct Pre {
+    T a, b;  // the two tz values of the vertical difference
+  };
+  __device__ Pre pre(int plane, int y, int x) const {
     const int k = plane;
     const long i = ((long)plane * ny + y) * ld + x;
     const long pl = (long)ld * ny;
+    Pre p{T(0), T(0)};
+    if (x >= nx || (!wnode && k == nz - 1)) return p;
+    if (wnode) {
+      if (k > 0 && k < nz - 1) p = {tz[i], tz[i - pl]};
+    } else {
+      p = {tz[i + pl], tz[i]};
+    }
+    return p;
+  }
+  __device__ void fin(int plane, int y, int x, T v, const Pre& p) const {
+    const int k = plane;
+    const long i = ((long)plane * ny + y) * ld + x;
     if (!wnode && k == nz - 1) { divt[i] = T(BOGUS); return; }
     if (x >= nx) { divt[i] = T(0); return; }
     T d;
     if (wnode) {
       if (k == 0) d = v;                                            // divstress_w.f90:107-115
-      else if (k < nz - 1) d = v + rdz * (tz[i] - tz[i - pl]);      // ddz_uv
+      else if (k < nz - 1) d = v + rdz * (p.a - p.b);               // ddz_uv
       else d = v + T(0);                                            // ddz_uv: dfdz(nz) = 0
     } else {
-      d = v + rdz * (tz[i + pl] - tz[i]);                           // ddz_w
+      d = v + rdz * (p.a - p.b);                                    // ddz_w
     }
     divt[i] = d;
   }
+  __device__ void operator()(int plane, int y, int x, T v) const { fin(plane, y, x, v, pre(plane, y, x)); }
 };
 
 template <typename T>
