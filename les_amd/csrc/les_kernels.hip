@@ -698,6 +698,33 @@ struct LoadLag {  // field order: ub vb wb | uu uv uw vv vw ww | S11 S12 S13 S22
 };
 
 
+// all LoadLag fields F0 .. 20 of a point at once (k_row_fwd_mf), in LoadLag's
+// evaluation order
+template <typename T, int F0>
+struct LoadLagAll {
+  static constexpr int NF = 21 - F0;
+  LoadLag<T> L;
+  __device__ void all(int k, int y, int x, T (&v)[NF]) const {
+    const long i = ((long)k * L.ny + y) * L.ld + x, pl = (long)L.ld * L.ny;
+    if constexpr (F0 == 0) {
+      T ub, vb, wb;
+      if (k == 0) { ub = L.u[i]; vb = L.v[i]; wb = T(0.25) * L.w[i + pl]; }
+      else { ub = T(0.5) * (L.u[i] + L.u[i - pl]); vb = T(0.5) * (L.v[i] + L.v[i - pl]); wb = L.w[i]; }
+      v[0] = ub; v[1] = vb; v[2] = wb;
+      v[3] = ub * ub; v[4] = ub * vb; v[5] = ub * wb;
+      v[6] = vb * vb; v[7] = vb * wb; v[8] = wb * wb;
+    }
+    const T s11 = L.S11[i], s12 = L.S12[i], s13 = L.S13[i], s22 = L.S22[i], s23 = L.S23[i], s33 = L.S33[i];
+    const T Sv[6] = {s11, s12, s13, s22, s23, s33};
+    const T sm = smag(s11, s12, s13, s22, s23, s33);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      v[9 - F0 + q] = Sv[q];
+      v[15 - F0 + q] = sm * Sv[q];
+    }
+  }
+};
+
 template <typename T>
 __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F4, const T* __restrict__ S11,
                                const T* __restrict__ S12, const T* __restrict__ S13, const T* __restrict__ S22,
@@ -808,7 +835,9 @@ T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const
   const long rows = (long)nf * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
   LoadLag<T> L{u, v, w, S[0], S[1], S[2], S[3], S[4], S[5], g.ny, nz, g.ld, f0};
-  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
+  const bool mf = f0 == 0 ? launch_row_fwd_mf<T>(c, LoadLagAll<T, 0>{L}, X0, kf, 9.0)
+                          : (f0 == 9 ? launch_row_fwd_mf<T>(c, LoadLagAll<T, 9>{L}, X0, kf, 6.0) : false);
+  if (!mf) launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
   cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
   cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
   ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);

@@ -310,6 +310,93 @@ static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long 
   }
 }
 
+// Row transforms of NF fields formed together: one workgroup per (level,
+// row pair); the loader ML forms all NF field values of a point at once
+// (ML::all), so the arrays it combines are read once for all fields, where the
+// plane-batched k_row_fwd reads them once per field.  Two rows of a field
+// share one complex transform as in k_row_fwd (same values, same transform,
+// same output rows ((f nz + k) ny + y)).
+template <typename T, int N, class ML>
+__global__ void __launch_bounds__(256)
+    k_row_fwd_mf(ML L, cx<T>* __restrict__ out, long orow, int ny, int nz, const cx<T>* __restrict__ twg,
+                 int kxo) {
+  constexpr int NF = ML::NF, RS = lds_row(N), BLOCK = 256;
+  constexpr int NP = 2 * N, MP = (NP + BLOCK - 1) / BLOCK;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* buf = tw + N;
+  for (int i = threadIdx.x; i < N - 1; i += BLOCK) tw[i] = twg[N + i];
+  const int k = blockIdx.y, y0 = 2 * blockIdx.x;
+  T v[MP][NF];
+#pragma unroll
+  for (int i = 0; i < MP; ++i) {
+    const int e = threadIdx.x + i * BLOCK;
+    if (NP % BLOCK == 0 || e < NP) {
+      const int r = e / N, x = e - r * N;
+      L.all(k, y0 + r, x, v[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MP; ++i) {
+    const int e = threadIdx.x + i * BLOCK;
+    if (NP % BLOCK == 0 || e < NP) {
+      const int r = e / N, x = e - r * N;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        cx<T>& z = buf[f * RS + lds_pad(x)];
+        if (r == 0) z.x = v[i][f];
+        else z.y = v[i][f];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds_c<T, N, 1, RS, NF, BLOCK, false, true>(buf, tw, -1);
+  const float inv_kxo = 1.0f / (float)kxo;
+  for (int e = threadIdx.x; e < NF * kxo; e += BLOCK) {
+    const int f = div_small(e, inv_kxo), kk = e - f * kxo;
+    const cx<T> zk = buf[f * RS + lds_pad(kk)], zn = buf[f * RS + lds_pad(kk ? N - kk : 0)];
+    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+    const long row = ((long)f * nz + k) * ny + y0;
+    out[row * orow + kk] = A;
+    out[(row + 1) * orow + kk] = B;
+  }
+}
+
+// false when there is no compile-time variant for this row length (or its
+// LDS tile does not fit): the caller then runs the plane-batched kernel
+template <typename T, class ML>
+static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, double read_fields) {
+  const Grid& g = c->g;
+  const int n = g.nx;
+  auto go = [&](auto nn) -> bool {
+    constexpr int N = decltype(nn)::value;
+    constexpr size_t sm = (size_t)(N + ML::NF * lds_row(N)) * sizeof(cx<T>);
+    if constexpr (sm > 160 * 1024) {
+      return false;
+    } else {
+      if (sm > 65536)
+        LCHK(c, hipFuncSetAttribute((const void*)k_row_fwd_mf<T, N, ML>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)sm));
+      ProfScope ps(c, "row_fwd",
+                   (double)g.nz * g.ny * (n * sizeof(T) * read_fields + 2.0 * ML::NF * kxo * sizeof(cx<T>) / 2));
+      hipLaunchKernelGGL((k_row_fwd_mf<T, N, ML>), dim3((unsigned)(g.ny / 2), (unsigned)g.nz), dim3(256), sm,
+                         c->stream, L, out, (long)kxo, g.ny, g.nz, tw_for<T>(c, N), kxo);
+      LCHK(c, hipGetLastError());
+      return true;
+    }
+  };
+  switch (n) {
+#define LES_CASE(NN) \
+  case NN:           \
+    return go(std::integral_constant<int, NN>{});
+    LES_FFT_SIZES(LES_CASE)
+#undef LES_CASE
+    default:
+      return false;
+  }
+}
+
 template <typename T, int N, class S>
 static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
                              int n, int ldo, const cx<T>* in2, T c2) {
