@@ -625,6 +625,41 @@ struct LoadLagScalar {  // theta_b | u_b th_b, v_b th_b, w_b th_b | |S| dT (x,y,
   }
 };
 
+// all ten LoadLagScalar fields of a point at once (k_row_fwd_mf), in
+// LoadLagScalar's evaluation order
+template <typename T>
+struct LoadLagScalarAll {
+  static constexpr int NF = 10;
+  LoadLagScalar<T> L;
+  __device__ void all(int k, int y, int x, T (&v)[NF]) const {
+    const long i = ((long)k * L.ny + y) * L.ld + x, pl = (long)L.ld * L.ny;
+    const T tb = k == 0 ? L.th[i] : T(0.5) * (L.th[i] + L.th[i - pl]);
+    const T ub = k == 0 ? L.u[i] : T(0.5) * (L.u[i] + L.u[i - pl]);
+    const T vb = k == 0 ? L.v[i] : T(0.5) * (L.v[i] + L.v[i - pl]);
+    const T wb = k == 0 ? T(0.25) * L.w[i + pl] : L.w[i];
+    v[0] = tb;
+    v[1] = ub * tb;
+    v[2] = vb * tb;
+    v[3] = wb * tb;
+    const T S = smag(L.S11[i], L.S12[i], L.S13[i], L.S22[i], L.S23[i], L.S33[i]);
+    if (k == 0) {
+      v[4] = S * L.dx[i];
+      v[5] = S * L.dy[i];
+      v[6] = S * (L.dz[i] + L.dz[i + pl]) / T(2);
+      v[7] = L.dx[i];
+      v[8] = L.dy[i];
+      v[9] = (L.dz[i] + L.dz[i + pl]) / T(2);
+    } else {
+      v[4] = S * (L.dx[i] + L.dx[i - pl]) / T(2);
+      v[5] = S * (L.dy[i] + L.dy[i - pl]) / T(2);
+      v[6] = S * L.dz[i];
+      v[7] = (L.dx[i] + L.dx[i - pl]) / T(2);
+      v[8] = (L.dy[i] + L.dy[i - pl]) / T(2);
+      v[9] = L.dz[i];
+    }
+  }
+};
+
 template <typename T>
 __global__ void k_lag_scalar_epilogue(const T* __restrict__ F, const T* __restrict__ Fs, const T* __restrict__ TnLM,
                                       const T* __restrict__ TnQN, const T* __restrict__ Cs, T* GLM, T* GMM, T* GQN,
@@ -738,7 +773,8 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   const long rows = 10L * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
   LoadLagScalar<T> L{u, v, w, tht, S[0], S[1], S[2], S[3], S[4], S[5], dTx, dTy, dTz, g.ny, nz, g.ld};
-  launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
+  if (!launch_row_fwd_mf<T>(c, LoadLagScalarAll<T>{L}, X0, kf, 13.0))
+    launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
   cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
   cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
   ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);
