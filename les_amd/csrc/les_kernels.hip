@@ -1035,21 +1035,37 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
   U(0) = u0;
   double cprev = 1.0;
   double uprev = u0;
+  // level L's (Hx, Hy, Hz) components (zero above the last H plane, nz - 2),
+  // loaded four levels ahead of the recurrence through a shift register
+  auto fetch = [&](int L, T& a, T& b, T& cz) {
+    a = b = cz = T(0);
+    if (L <= nz - 2) {
+      a = Hx[at(L) + oc];
+      b = Hy[at(L) + oc];
+      cz = Hz[at(L) + comp];
+    }
+  };
+  T x1, y1, z1, x2, y2, z2, x3, y3, z3, x4, y4, z4;
+  fetch(1, x1, y1, z1);
+  fetch(2, x2, y2, z2);
+  fetch(3, x3, y3, z3);
+  fetch(4, x4, y4, z4);
   double hxm = (double)Hx[at(0) + oc], hym = (double)Hy[at(0) + oc], hzm = (double)Hz[at(0) + comp];
   for (int j = 1; j <= nz; ++j) {
+    const double hxj = (double)x1, hyj = (double)y1, hzj = (double)z1;  // level j
+    x1 = x2; y1 = y2; z1 = z2;
+    x2 = x3; y2 = y3; z2 = z3;
+    x3 = x4; y3 = y4; z3 = z4;
+    fetch(j + 4, x4, y4, z4);
     double aj, bj, rj;
     if (j < nz) {
       aj = a_in;
       bj = b_in;
-      double hz = 0.0;
-      if (j < nz - 1) hz = (double)Hz[at(j) + comp];
       const double sv = kx * hxm + ky * hym;
-      rj = sg * sv + (hz - hzm) / dz;
-      if (j < nz - 1) {
-        hxm = (double)Hx[at(j) + oc];
-        hym = (double)Hy[at(j) + oc];
-        hzm = hz;
-      }
+      rj = sg * sv + (hzj - hzm) / dz;
+      hxm = hxj;
+      hym = hyj;
+      hzm = hzj;
     } else {
       aj = -1.0;
       bj = 1.0;
