@@ -1036,7 +1036,7 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
   double cprev = 1.0;
   double uprev = u0;
   // level L's (Hx, Hy, Hz) components (zero above the last H plane, nz - 2),
-  // loaded four levels ahead of the recurrence through a shift register
+  // loaded eight levels ahead of the recurrence through a shift register
   auto fetch = [&](int L, T& a, T& b, T& cz) {
     a = b = cz = T(0);
     if (L <= nz - 2) {
@@ -1045,18 +1045,26 @@ __global__ void k_tridiag(const cx<T>* __restrict__ H, const cx<T>* __restrict__
       cz = Hz[at(L) + comp];
     }
   };
-  T x1, y1, z1, x2, y2, z2, x3, y3, z3, x4, y4, z4;
+  T x1, y1, z1, x2, y2, z2, x3, y3, z3, x4, y4, z4, x5, y5, z5, x6, y6, z6, x7, y7, z7, x8, y8, z8;
   fetch(1, x1, y1, z1);
   fetch(2, x2, y2, z2);
   fetch(3, x3, y3, z3);
   fetch(4, x4, y4, z4);
+  fetch(5, x5, y5, z5);
+  fetch(6, x6, y6, z6);
+  fetch(7, x7, y7, z7);
+  fetch(8, x8, y8, z8);
   double hxm = (double)Hx[at(0) + oc], hym = (double)Hy[at(0) + oc], hzm = (double)Hz[at(0) + comp];
   for (int j = 1; j <= nz; ++j) {
     const double hxj = (double)x1, hyj = (double)y1, hzj = (double)z1;  // level j
     x1 = x2; y1 = y2; z1 = z2;
     x2 = x3; y2 = y3; z2 = z3;
     x3 = x4; y3 = y4; z3 = z4;
-    fetch(j + 4, x4, y4, z4);
+    x4 = x5; y4 = y5; z4 = z5;
+    x5 = x6; y5 = y6; z5 = z6;
+    x6 = x7; y6 = y7; z6 = z7;
+    x7 = x8; y7 = y8; z7 = z8;
+    fetch(j + 8, x8, y8, z8);
     double aj, bj, rj;
     if (j < nz) {
       aj = a_in;
