@@ -675,8 +675,16 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
       }
       return;
     }
+    // an i kx output beside an identity output is i kx times the identity
+    // output's inverse (kx is constant along a column): no transform of its own
+    int jid = -1, jkx = -1;
     for (int j = 0; j < a.nout; ++j) {
-      if (kx0 >= a.nko[j]) continue;  // uniform over the block
+      if (a.kind[j] == K_ID && jid < 0) jid = j;
+      if (a.kind[j] == K_IKX && jkx < 0) jkx = j;
+    }
+    if (jid < 0 || !a.do_inv) jkx = -1;
+    for (int j = 0; j < a.nout; ++j) {
+      if (kx0 >= a.nko[j] || j == jkx) continue;  // uniform over the block
       const int kind = a.kind[j];
       const T* G = j == 0 ? G0 : (j == 1 ? G1 : G2);
 #pragma unroll
@@ -707,6 +715,16 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024
       for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
         const int y = e / TX, c = e - y * TX, kk = kx0 + c;
         if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = R[e];
+      }
+      if (j == jid && jkx >= 0 && kx0 < a.nko[jkx]) {
+        cx<T>* ox = jkx == 0 ? o0 : (jkx == 1 ? o1 : o2);
+        const int c0 = threadIdx.x % TX;
+        const T k = kx0 + c0 < a.nki ? kxt[kx0 + c0] : T(0);
+        for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
+          const int y = e / TX, kk = kx0 + c0;
+          const cx<T> v = R[e];
+          if (kk < a.nko[jkx]) ox[pl * a.out_plane[jkx] + (long)y * a.out_row[jkx] + kk] = cx<T>{-k * v.y, k * v.x};
+        }
       }
     }
     return;
