@@ -62,29 +62,36 @@ static const FftPlan& plan_for(const les_ctx* c, int n) {
 // blockIdx.x = group of P row pairs inside it, so loaders and storers get a
 // block-uniform plane index (no per-element integer division).  N > 0 is the
 // compile-time transform length; N == 0 runs the same code with run-time n.
-// TPF threads per transform (n/4: plan 0 passes, radix-2 passes take two
-// butterflies per thread), P transforms per block
-template <int N>
+// PLAN: the FP32 forward (r2c) kernels use radix-8 passes (plan 2) from 64
+// points, the others radix-4 passes (plan 0; inverse epilogues would hold
+// twice the elements per thread with plan 2's thread count, and FP64 forward
+// rows measured faster with plan 0); TPF threads per
+// transform (n/8 with plan 2, whose last radix-4 or -2 pass takes two
+// butterflies per thread; n/4 with plan 0), P transforms per block
+template <int N, bool FWD = false>
 struct RowGeom {
-  static constexpr int TPF = N > 0 ? N / 4 : 64;
+  static constexpr int PLAN = FWD && N >= 64 ? 2 : 0;
+  static constexpr int TPF = N > 0 ? (PLAN == 2 ? N / 8 : N / 4) : 64;
   static constexpr int P = (256 / TPF) > 0 ? 256 / TPF : 1;
   static constexpr int BLOCK = P * TPF;
 };
+template <typename T, int N>
+using RowGeomF = RowGeom<N, sizeof(T) == 4>;  // forward (r2c) row kernels: radix-8 in FP32 (measured)
 
 template <typename T, int N, class Loader>
-__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+__global__ void __launch_bounds__((N > 0 ? RowGeomF<T, N>::BLOCK : 1024))
     k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
               int kxo) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
   const int n = N > 0 ? N : p.n;
-  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
-  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  const int P = N > 0 ? RowGeomF<T, N>::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RowGeomF<T, N>::BLOCK : (int)blockDim.x;
   constexpr int RS = N > 0 ? lds_row(N) : 0;  // padded row stride (compile-time lengths)
   auto ix = [&](int t, int e) { return N > 0 ? t * RS + lds_pad(e) : t * n + e; };
   cx<T>* buf = tw + n;
   if (N > 0)
-    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + i];
+    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + RowGeomF<T, N>::PLAN * n + i];
   else
     for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
   const int plane = blockIdx.y;
@@ -92,7 +99,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int pair0 = blockIdx.x * P;
   if constexpr (N > 0) {
     // every row load of the block is issued before the first LDS store
-    constexpr int NE = RowGeom<N>::P * (N > 0 ? N : 1), BL = RowGeom<N>::BLOCK, MAXE = (NE + BL - 1) / BL;
+    constexpr int NE = RowGeomF<T, N>::P * (N > 0 ? N : 1), BL = RowGeomF<T, N>::BLOCK, MAXE = (NE + BL - 1) / BL;
     cx<T> z[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
@@ -123,7 +130,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, -1);
+    fft_lds_c<T, N, 1, RS, RowGeomF<T, N>::P, RowGeomF<T, N>::BLOCK, false, true, RowGeomF<T, N>::PLAN>(buf, tw, -1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
   const float inv_kxo = 1.0f / (float)kxo;
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   auto ix = [&](int t, int e) { return N > 0 ? t * RS + lds_pad(e) : t * n + e; };
   cx<T>* buf = tw + n;
   if (N > 0)
-    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + i];
+    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + RowGeom<N>::PLAN * n + i];
   else
     for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
   const int plane = blockIdx.y;
@@ -223,7 +230,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true>(buf, tw, +1);
+    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true, RowGeom<N>::PLAN>(buf, tw, +1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
   if constexpr (N > 0 && StorerHasPre<Storer>::value) {
@@ -281,8 +288,8 @@ static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, lon
   const FftPlan& p = plan_for(c, n);
   int P = 1, threads = 64;
   if constexpr (N > 0) {
-    P = RowGeom<N>::P;
-    threads = RowGeom<N>::BLOCK;
+    P = RowGeomF<T, N>::P;
+    threads = RowGeomF<T, N>::BLOCK;
   } else {  // run-time length: radix-4/2/3 passes, n/4 threads per transform
     const int tpf = n / 4;
     P = 256 / tpf > 0 ? 256 / tpf : 1;
@@ -325,7 +332,7 @@ __global__ void __launch_bounds__(256)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* buf = tw + N;
-  for (int i = threadIdx.x; i < N - 1; i += BLOCK) tw[i] = twg[N + i];
+  for (int i = threadIdx.x; i < N - 1; i += BLOCK) tw[i] = twg[N + RowGeomF<T, N>::PLAN * N + i];
   const int k = blockIdx.y, y0 = 2 * blockIdx.x;
   T v[MP][NF];
 #pragma unroll
@@ -350,7 +357,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   __syncthreads();
-  fft_lds_c<T, N, 1, RS, NF, BLOCK, false, true>(buf, tw, -1);
+  fft_lds_c<T, N, 1, RS, NF, BLOCK, false, true, RowGeomF<T, N>::PLAN>(buf, tw, -1);
   const float inv_kxo = 1.0f / (float)kxo;
   for (int e = threadIdx.x; e < NF * kxo; e += BLOCK) {
     const int f = div_small(e, inv_kxo), kk = e - f * kxo;
