@@ -484,11 +484,10 @@ struct FeatLag {
   long n3, pl;
   int nx, ny, ld;
   float dz, idelta;
-  __device__ long index(long p) const {
-    const long nxy = (long)nx * ny;
-    const int k = (int)(p / nxy);
-    const long r = p - k * nxy;
-    const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+  __device__ long index(long p) const {  // 32-bit: point counts stay below 2^31
+    const unsigned nxy = (unsigned)nx * ny, pp = (unsigned)p;
+    const unsigned k = pp / nxy, r = pp - k * nxy;
+    const unsigned y = r / (unsigned)nx, x = r - y * (unsigned)nx;
     return (long)k * pl + (long)y * ld + x;
   }
   __device__ void operator()(long p, float* f) const {
