@@ -11,10 +11,11 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "RHSx", "Cs_opt2", "Nu_t", "txz")
 
 
-def run(prec, tune=None, steps=12, conv=False, autotune=False, n=32):
+def run(prec, tune=None, steps=12, conv=False, autotune=False, n=32, ny=None):
     from les_amd.api import LesSim, checkerboard_flux
-    kw = dict(scalars=dict(wt_s=checkerboard_flux(n, n), dTdz_top=0.01)) if conv else {}
-    s = LesSim(n, n, 17, precision=prec, dt=2e-4, **kw)
+    ny = ny or n
+    kw = dict(scalars=dict(wt_s=checkerboard_flux(n, ny), dTdz_top=0.01)) if conv else {}
+    s = LesSim(n, ny, 17, precision=prec, dt=2e-4, **kw)
     s.init_ic(seed=3)
     if conv:
         s.init_scalar_ic(seed=3)
@@ -78,18 +79,20 @@ def test_radix8_column_plan(torch_mod, prec, tx):
         assert err < tol, (n, err)
 
 
-@pytest.mark.parametrize("prec,conv,n,steps", [("fp64", True, 64, 8), ("fp32", True, 64, 8), ("fp64", False, 64, 8),
-                                                ("fp32", False, 64, 8), ("fp64", True, 256, 3),
-                                                ("fp32", False, 256, 3)])
-def test_streaming_padded_products(torch_mod, prec, conv, n, steps):
+@pytest.mark.parametrize("prec,conv,n,ny,steps", [("fp64", True, 64, 64, 8), ("fp32", True, 64, 64, 8),
+                                                   ("fp64", False, 64, 64, 8), ("fp32", False, 64, 64, 8),
+                                                   ("fp64", True, 256, 256, 3), ("fp32", False, 256, 256, 3),
+                                                   ("fp64", True, 96, 64, 6), ("fp32", True, 96, 64, 6)])
+def test_streaming_padded_products(torch_mod, prec, conv, n, ny, steps):
     """Streaming level runs (radix-8 plan) against level groups (two-pass
-    plan) for the padded products, on 64^2 and 256^2 grids (padded lengths 96
-    and 384; the 384 FP64 ring is the largest LDS tile): momentum with both
-    scalars, and momentum alone.  Tolerances as for the radix-8 column plan."""
-    print("streaming padded products", prec, conv, n, flush=True)
+    plan) for the padded products, on 64^2, 256^2 and 96x64 grids (padded
+    lengths 96, 384 and 144; the 384 FP64 ring is the largest LDS tile):
+    momentum with both scalars, and momentum alone.  Tolerances as for the
+    radix-8 column plan."""
+    print("streaming padded products", prec, conv, n, ny, flush=True)
     key = "pad_mode_" + prec
-    base, _ = run(prec, conv=conv, n=n, steps=steps)
-    s, _ = run(prec, tune={key: 1}, conv=conv, n=n, steps=steps)
+    base, _ = run(prec, conv=conv, n=n, ny=ny, steps=steps)
+    s, _ = run(prec, tune={key: 1}, conv=conv, n=n, ny=ny, steps=steps)
     assert s.get_tune()[key] == 1
     tol = 1e-12 if prec == "fp64" else 2e-4
     names = ("u", "v", "w", "p") + (("theta",) if conv else ())
