@@ -57,7 +57,8 @@ struct Arena {
 struct Tune {
   int col_tx[2] = {16, 8};  // column-FFT tile width (kx columns per block): FP32, FP64
   int col_plan[2] = {0, 0}; // column-FFT plan (les_fft.h): 0 radix-4 passes, 2 radix-8 passes
-  int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS
+  int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS;
+                            // 3 parallel cyclic reduction, one wavefront per mode
   int tri_block = 128;      // horizontal modes per block of the tridiagonal solve
   int pad_mode[2] = {0, 0}; // padded products: 0 level groups per workgroup, 1 streaming level runs
 };

@@ -1113,9 +1113,123 @@ struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: 
   }
 };
 
+// Parallel cyclic reduction: one wavefront per mode (kx, ky), four modes per
+// workgroup; the (nz+1)-row system of press_stag_array.f90:142-165 (the rows
+// and right-hand side k_tridiag forms) sits in LDS, FP64, and each step
+// eliminates the neighbours at distance s = 1, 2, 4, ... until every row
+// stands alone.  Nyquist row and zero mode as k_tridiag.  Another elimination
+// order than the Thomas sweep: the same solution to rounding.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_tridiag_pcr(const cx<T>* __restrict__ H, const cx<T>* __restrict__ BT, cx<T>* __restrict__ P,
+                  const double* __restrict__ kxd, const double* __restrict__ kyd, int nxh, int ny, int nz,
+                  double dz) {
+  extern __shared__ __align__(16) unsigned char tri_smem[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = nz + 1, nm = nxh * ny;
+  const int m = blockIdx.x * 4 + wv;
+  const bool act = m < nm;
+  const int y = act ? m / nxh : 0, x = act ? m - y * nxh : 0;
+  const long ps = (long)nm, fs = ps * (nz - 1);
+  const bool solve = act && y != ny / 2 && m != 0;
+  // per wave two buffers of the rows: a, b, c, Re r, Im r (n values each)
+  double* wbase = reinterpret_cast<double*>(tri_smem) + (size_t)wv * 10 * n;
+  if (act && y == ny / 2)
+    for (int j = lane; j < nz; j += 64) P[j * ps + m] = {T(0), T(0)};
+  if (act && m == 0 && lane == 0) {  // zero mode: integration (press_stag_array.f90:238-247)
+    const cx<double> bot = {(double)BT[m].x, (double)BT[m].y};
+    cx<double> pv = {0.0, 0.0};
+    P[0] = {T(0), T(0)};
+    pv = {pv.x - dz * bot.x, pv.y - dz * bot.y};
+    P[ps] = {(T)pv.x, (T)pv.y};
+    for (int j = 2; j < nz; ++j) {
+      const cx<T> h = H[2 * fs + (long)(j - 1) * ps];
+      pv = {pv.x + (double)h.x * dz, pv.y + (double)h.y * dz};
+      P[j * ps + m] = {(T)pv.x, (T)pv.y};
+    }
+  }
+  {
+    double* o = wbase;
+    const double kx = solve ? kxd[x] : 0.0, ky = solve ? kyd[y] : 0.0;
+    const double a_in = 1.0 / (dz * dz), c_in = 1.0 / (dz * dz);
+    const double b_in = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+    for (int j = lane; j < n; j += 64) {
+      double a = 0.0, b = 1.0, c = 0.0, rx = 0.0, ry = 0.0;
+      if (solve) {
+        if (j == 0) {  // b=-1, c=1, r=-dz*bottom
+          b = -1.0;
+          c = 1.0;
+          rx = -dz * (double)BT[m].x;
+          ry = -dz * (double)BT[m].y;
+        } else if (j < nz) {
+          a = a_in;
+          b = b_in;
+          c = c_in;
+          const cx<T> hx = H[(long)(j - 1) * ps + m], hy = H[fs + (long)(j - 1) * ps + m];
+          const cx<T> hzm = H[2 * fs + (long)(j - 1) * ps + m];
+          const cx<T> hz = j < nz - 1 ? H[2 * fs + (long)j * ps + m] : cx<T>{T(0), T(0)};
+          const double sr = kx * (double)hx.x + ky * (double)hy.x, si = kx * (double)hx.y + ky * (double)hy.y;
+          rx = -si + ((double)hz.x - (double)hzm.x) / dz;
+          ry = sr + ((double)hz.y - (double)hzm.y) / dz;
+        } else {  // top row: a=-1, b=1, r=-dz*top
+          a = -1.0;
+          rx = -(double)BT[ps + m].x * dz;
+          ry = -(double)BT[ps + m].y * dz;
+        }
+      }
+      o[j] = a;
+      o[n + j] = b;
+      o[2 * n + j] = c;
+      o[3 * n + j] = rx;
+      o[4 * n + j] = ry;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int st = 1; st < n; st *= 2) {
+    const double* ia = wbase + cur * 5 * n;
+    double* oa = wbase + (cur ^ 1) * 5 * n;
+    for (int j = lane; j < n; j += 64) {
+      const double aj = ia[j], bj = ia[n + j], cj = ia[2 * n + j];
+      double a2 = 0.0, b2 = bj, c2 = 0.0, rx = ia[3 * n + j], ry = ia[4 * n + j];
+      if (j - st >= 0) {
+        const int q = j - st;
+        const double k1 = aj / ia[n + q];
+        a2 = -ia[q] * k1;
+        b2 = b2 - ia[2 * n + q] * k1;
+        rx = rx - ia[3 * n + q] * k1;
+        ry = ry - ia[4 * n + q] * k1;
+      }
+      if (j + st < n) {
+        const int q = j + st;
+        const double k2 = cj / ia[n + q];
+        c2 = -ia[2 * n + q] * k2;
+        b2 = b2 - ia[q] * k2;
+        rx = rx - ia[3 * n + q] * k2;
+        ry = ry - ia[4 * n + q] * k2;
+      }
+      oa[j] = a2;
+      oa[n + j] = b2;
+      oa[2 * n + j] = c2;
+      oa[3 * n + j] = rx;
+      oa[4 * n + j] = ry;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (solve) {
+    const double* ia = wbase + cur * 5 * n;
+    for (int j = lane; j < nz; j += 64) {
+      const double b = ia[n + j];
+      P[j * ps + m] = {(T)(ia[3 * n + j] / b), (T)(ia[4 * n + j] / b)};
+    }
+  }
+}
+
 // LDS bytes of a tridiagonal schedule with `block` modes per workgroup (0:
 // the mode does not use LDS): gamma per mode, U per mode and component
 size_t tridiag_lds(int mode, int block, int nz) {
+  if (mode == 3) return (size_t)4 * 10 * (nz + 1) * sizeof(double);  // four waves, two row buffers each
   return (size_t)(nz + 1) * block * (mode >= 1 ? sizeof(double) : 0) +
          (size_t)(nz + 1) * block * (mode >= 2 ? sizeof(cx<double>) : 0);
 }
@@ -1132,7 +1246,12 @@ void launch_tridiag(les_ctx* c, const cx<T>* H, const cx<T>* BT, cx<T>* PH, doub
   ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) +
                                   (nz + 1) * ps * 2.0 * ((mode >= 1 ? 0 : sizeof(double)) +
                                                          (mode >= 2 ? 0 : sizeof(cx<double>))));
-  if (mode == 0) {
+  if (mode == 3) {
+    if (sm > 65536)
+      LCHK(c, hipFuncSetAttribute((const void*)k_tridiag_pcr<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL((k_tridiag_pcr<T>), dim3((unsigned)((ps + 3) / 4)), dim3(256), sm, c->stream, H, BT, PH,
+                       c->td.kx, c->td.ky, nxh, g.ny, nz, g.dz);
+  } else if (mode == 0) {
     hipLaunchKernelGGL((k_tridiag<T, 0>), grid, block, 0, c->stream, H, BT, PH, gam, Ub, t.kx, t.ky, c->td.kx,
                        c->td.ky, nxh, g.ny, nz, g.dz, err);
   } else if (mode == 1) {

@@ -56,7 +56,7 @@ def test_autotune_chooses_a_measured_candidate(torch_mod):
     assert ch["tri_mode"] == t["tri_mode"] and ch["tri_block"] == t["tri_block"]
     assert str(t["pad_mode_fp32"]) in rep["pad"] and ch["pad_mode"][0] == t["pad_mode_fp32"]
     assert all(v > 0 for k in ("col", "tridiag", "pad") for v in rep[k].values())
-    if t["col_plan_fp32"] == 0 and t["pad_mode_fp32"] == 0:
+    if t["col_plan_fp32"] == 0 and t["pad_mode_fp32"] == 0 and t["tri_mode"] != 3:
         for n in FIELDS:
             assert np.array_equal(s.get(n), base.get(n)), n
 
@@ -104,6 +104,26 @@ def test_streaming_padded_products(torch_mod, prec, conv, n, ny, steps):
     assert any(not np.array_equal(s.get(nm), base.get(nm)) for nm in names)  # the streaming kernel ran
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cyclic_reduction_pressure_solve(torch_mod, prec):
+    """Tridiagonal schedule 3 (parallel cyclic reduction, one wavefront per
+    mode) against the Thomas sweep: after 12 convective steps the FP64 state
+    agrees to 1e-12 relative (the z systems are diagonally dominant except for
+    the boundary rows), the FP32-bulk state to FP32 round-off (2e-4, as for the
+    transform plans); also on a 64^2 x 16 grid."""
+    print("cyclic reduction", prec, flush=True)
+    tol = 1e-12 if prec == "fp64" else 2e-4
+    for n in (32, 64):
+        base, _ = run(prec, conv=True, n=n)
+        s, _ = run(prec, tune=dict(tri_mode=3, tri_block=64), conv=True, n=n)
+        assert s.get_tune()["tri_mode"] == 3
+        for nm in ("u", "v", "w", "p", "theta"):
+            a, b = s.get(nm)[..., :n], base.get(nm)[..., :n]
+            err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+            print(n, nm, err)
+            assert err < tol, (n, nm, err)
+
+
 def test_set_tune_rejects_bad_shapes(torch_mod):
     from les_amd.api import Les, LesError
     g = Les(16, 16, 9)
@@ -111,5 +131,7 @@ def test_set_tune_rejects_bad_shapes(torch_mod):
         g.set_tune(col_tx_fp32=12)
     with pytest.raises(LesError):
         g.set_tune(tri_mode=2, tri_block=1024)   # (nz+1) x 1024 x 24 B does not fit in LDS
+    with pytest.raises(LesError):
+        g.set_tune(tri_mode=4, tri_block=64)
     with pytest.raises(LesError):
         g.set_tune(pad_mode_fp32=2)
