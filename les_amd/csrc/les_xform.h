@@ -1212,27 +1212,33 @@ static void launch_pad_products_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const 
 
 // ------------------------------------------------------------------------
 // Streaming variant of the padded products.  One workgroup owns padded row yb
-// and a run of output levels [k0, k1).  The physical fields of the levels the
-// products need (k - HB .. k + HA) stay in an LDS ring; each level step
+// and a run of output levels [k0, k1).  The physical fields of level k and of
+// level k + 1 stay in two LDS slots, and the few field pairs the products read
+// below level k (Pol::NPM) in a third, small one.  Each level step
 // inverse-transforms only the one new level, whose spectral rows were loaded
-// into registers during the previous step, so the HBM reads of level k + 1 + HA
-// are in flight while level k is transformed, multiplied and transformed back.
-// Each level is inverse-transformed once per run (the batched kernel redoes
-// the HB + HA halo levels of every CH-level group), and the transforms use the
-// radix-8 plan, whose passes give a level's NP rows more work-items than the
-// two-pass plan.  The physical values equal the batched kernel's up to the
-// rounding of the transform plan.
+// into registers during the previous step, so the HBM reads of level k + 2 are
+// in flight while level k is transformed, multiplied and transformed back.
+// The product of a column x reads only column x, so the thread that owns x
+// keeps level k's lower pairs for the next step and writes the products over
+// level k's first pairs, which no later step reads: the ring holds
+// 2 NP + NPM rows, which lets three 512-thread workgroups share a CU at
+// nx2 = 384 in FP32.  Each level is inverse-transformed once per run (the
+// batched kernel redoes the halo levels of every CH-level group), and the
+// transforms use the radix-8 plan, whose passes give a level's NP rows more
+// work-items than the two-pass plan.  The physical values equal the batched
+// kernel's up to the rounding of the transform plan.
 // ------------------------------------------------------------------------
 template <typename T, int N, class Pol>
 struct PadStream {
   static constexpr int NF = Pol::NF, NP = (NF + 1) / 2, NO = Pol::NOUT, NOP = (NO + 1) / 2;
-  static constexpr int HB = Pol::HB, HA = Pol::HA, NR = HB + HA + 1;  // ring slots
-  static constexpr int RS = lds_row(N), NXH = N / 3, H2 = NXH / 2;    // nx2 = 3 nxh
+  static constexpr int HB = Pol::HB, HA = Pol::HA, NPM = HB > 0 ? Pol::NPM : 0;
+  static_assert(HA == 1 && HB <= 1 && NPM <= NP && NOP <= NP, "streaming products read levels k-1..k+1");
+  static constexpr int RS = lds_row(N), NXH = N / 3, H2 = NXH / 2;  // nx2 = 3 nxh
   static constexpr int BLOCK = N >= 384 ? 512 : 256;
   static constexpr int ITEMS = NP * H2;  // one item: two coefficients of a field pair
   static constexpr int MAXI = (ITEMS + BLOCK - 1) / BLOCK;
   static constexpr int PLAN = 2;
-  static constexpr size_t LDS = sizeof(cx<T>) * ((size_t)N + (size_t)RS * (NR * NP + NOP));
+  static constexpr size_t LDS = sizeof(cx<T>) * ((size_t)N + (size_t)RS * (2 * NP + NPM));
 };
 
 template <typename T, int N, class Pol>
@@ -1240,7 +1246,7 @@ __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
     k_pad_stream(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int lc,
                  const cx<T>* __restrict__ twg, Pol pol) {
   using G = PadStream<T, N, Pol>;
-  constexpr int NF = G::NF, NP = G::NP, NO = G::NO, NOP = G::NOP, HB = G::HB, HA = G::HA, NR = G::NR;
+  constexpr int NF = G::NF, NP = G::NP, NO = G::NO, NOP = G::NOP, NPM = G::NPM;
   constexpr int RS = G::RS, NXH = G::NXH, H2 = G::H2, BLOCK = G::BLOCK, MAXI = G::MAXI, PLAN = G::PLAN;
   constexpr int ITEMS = G::ITEMS;
   struct alignas(2 * sizeof(cx<T>)) cx2 {
@@ -1248,11 +1254,11 @@ __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
   };
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* ring = tw + N;              // [NR][NP][RS]: field pairs of levels k-HB .. k+HA, physical
-  cx<T>* out = ring + NR * NP * RS;  // [NOP][RS]: product pairs of level k
+  cx<T>* full = tw + N;                // [2][NP][RS]: field pairs of levels k and k + 1, physical
+  cx<T>* minus = full + 2 * NP * RS;   // [NPM][RS]: the pairs of level k - 1 the products read
   const int yb = blockIdx.x, nl = nz - 1;
   const int k0 = blockIdx.y * lc, k1 = min(k0 + lc, nl);
-  auto slot = [&](int L) { return ring + ((L - k0 + HB) % NR) * (NP * RS); };
+  auto slot = [&](int L) { return full + ((L - k0 + 2) & 1) * (NP * RS); };
   cx2 ra[MAXI], rb[MAXI];
   auto fetch = [&](int L) {  // spectral rows of level L -> registers (zero outside 0..nz-1)
 #pragma unroll
@@ -1278,7 +1284,7 @@ __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
       row[lds_pad(N - j)] = {A.x + B.y, B.x - A.y};
     }
   };
-  auto stage = [&](int L) {  // registers -> ring slot of level L, zero band [NXH, N - NXH]
+  auto stage = [&](int L) {  // registers -> slot of level L, zero band [NXH, N - NXH]
     cx<T>* s = slot(L);
 #pragma unroll
     for (int i = 0; i < MAXI; ++i) {
@@ -1296,35 +1302,52 @@ __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
     }
   };
   load_twiddles_for(tw, twg, N, true, PLAN);
-  for (int L = k0 - HB; L < k0 + HA; ++L) {  // the levels below the first new one
-    fetch(L);
-    stage(L);
+  if constexpr (NPM > 0) {  // level k0 - 1: only its lower pairs are read
+    fetch(k0 - 1);
+    stage(k0 - 1);
     __syncthreads();
-    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(L), tw, +1);
+    cx<T>* s = slot(k0 - 1);
+    fft_lds_c<T, N, 1, RS, NPM, BLOCK, false, true, PLAN>(s, tw, +1);
+    for (int e = threadIdx.x; e < NPM * RS; e += BLOCK) minus[e] = s[e];
   }
-  fetch(k0 + HA);
+  fetch(k0);
+  stage(k0);
+  __syncthreads();
+  fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(k0), tw, +1);
+  fetch(k0 + 1);
   for (int k = k0; k < k1; ++k) {
-    stage(k + HA);
-    if (k + 1 < k1) fetch(k + 1 + HA);  // in flight during this level's transforms
+    cx<T>* cur = slot(k);
+    cx<T>* nxt = slot(k + 1);
+    if (k > k0) __syncthreads();  // the previous step's products in nxt have been stored
+    stage(k + 1);
+    if (k + 1 < k1) fetch(k + 2);  // in flight during this level's transforms
     __syncthreads();
-    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(k + HA), tw, +1);
+    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(nxt, tw, +1);
     for (int x = threadIdx.x; x < N; x += BLOCK) {
       T o[2 * NOP];
 #pragma unroll
       for (int q = 0; q < 2 * NOP; ++q) o[q] = T(0);
       auto F = [&](int dk, int f) -> T {
-        const cx<T> z = slot(k + dk)[(f >> 1) * RS + lds_pad(x)];
+        const cx<T>* b = dk < 0 ? minus : (dk > 0 ? nxt : cur);
+        const cx<T> z = b[(f >> 1) * RS + lds_pad(x)];
         return (f & 1) ? z.y : z.x;
       };
       pol.product(k, F, o);
+      // column x of level k's lower pairs moves to the small slot before the
+      // products overwrite it (only this thread touches column x)
+      cx<T> keep[NPM > 0 ? NPM : 1];
 #pragma unroll
-      for (int q = 0; q < NOP; ++q) out[q * RS + lds_pad(x)] = {o[2 * q], o[2 * q + 1]};
+      for (int p = 0; p < NPM; ++p) keep[p] = cur[p * RS + lds_pad(x)];
+#pragma unroll
+      for (int p = 0; p < NPM; ++p) minus[p * RS + lds_pad(x)] = keep[p];
+#pragma unroll
+      for (int q = 0; q < NOP; ++q) cur[q * RS + lds_pad(x)] = {o[2 * q], o[2 * q + 1]};
     }
     __syncthreads();
-    fft_lds_c<T, N, 1, RS, NOP, BLOCK, false, true, PLAN>(out, tw, -1);
+    fft_lds_c<T, N, 1, RS, NOP, BLOCK, false, true, PLAN>(cur, tw, -1);
     for (int e = threadIdx.x; e < NO * NXH; e += BLOCK) {
       const int o = e / NXH, kx = e - o * NXH;
-      const cx<T>* z = out + (o >> 1) * RS;
+      const cx<T>* z = cur + (o >> 1) * RS;
       const cx<T> zk = z[lds_pad(kx)], zn = z[lds_pad(kx ? N - kx : 0)];
       const cx<T> v = (o & 1) ? cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)}
                               : cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
@@ -1386,6 +1409,7 @@ static void launch_pad_products(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Po
 template <typename T>
 struct PolConvec {
   static constexpr int NF = 6, NOUT = 3, HB = 1, HA = 1;
+  static constexpr int NPM = 1;  // field pairs read at level k - 1: (u1, u2)
   __device__ static int slot(int o) { return o; }  // output o -> plane block of XP
   T cst;
   template <class Fn>
@@ -1415,6 +1439,7 @@ __device__ __forceinline__ T scalar_adv(int k, T u, T v, T w, T wp, T sx, T sy, 
 template <typename T, int NS>
 struct PolScalar {
   static constexpr int NF = 3 + 3 * NS, NOUT = NS, HB = 0, HA = 1;
+  static constexpr int NPM = 0;
   __device__ static int slot(int o) { return o; }
   T cst;
   template <class Fn>
@@ -1433,6 +1458,7 @@ struct PolScalar {
 template <typename T>
 struct PolConvecScalar {
   static constexpr int NF = 12, NOUT = 5, HB = 1, HA = 1;
+  static constexpr int NPM = 1;  // field pairs read at level k - 1: (u1, u2)
   __device__ static int slot(int o) { return o < 2 ? o : (o == 4 ? 2 : o + 1); }
   T cst;
   template <class Fn>
