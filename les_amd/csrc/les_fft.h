@@ -368,11 +368,28 @@ __host__ __device__ constexpr bool pad_offsets_const(int n, int R, int NS, bool 
   return true;
 }
 
+// Twiddle r, k of the stage whose entries start at OFF of the stage-major
+// table: from the table itself, or from a kernel's own placement of it (a
+// class with a member template at<OFF, NS>(r, k)).
+template <int OFF, int NS, typename T>
+__device__ __forceinline__ cx<T> tw_at(const cx<T>* tw, int r, int k) {
+  return tw[OFF + (r - 1) * NS + k];
+}
+template <int OFF, int NS, typename T>
+__device__ __forceinline__ cx<T> tw_at(cx<T>* tw, int r, int k) {
+  return tw[OFF + (r - 1) * NS + k];
+}
+template <int OFF, int NS, class TW>
+__device__ __forceinline__ auto tw_at(const TW& tw, int r, int k) -> decltype(tw.template at<OFF, NS>(r, k)) {
+  return tw.template at<OFF, NS>(r, k);
+}
+
 // One Stockham stage of NTR transforms of length N in LDS (element e of
 // transform t at buf[t*TS + e*ES], or buf[t*TS + lds_pad(e)] with PAD)
 // executed by BLOCK threads.
-template <typename T, int N, int R, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int OFF>
-__device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sign) {
+template <typename T, int N, int R, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int OFF,
+          class TW>
+__device__ __forceinline__ void fft_stage_c(cx<T>* buf, const TW& tw, int sign) {
   constexpr int NB = N / R;
   constexpr int ITEMS = NTR * NB;
   constexpr int MAXB = (ITEMS + BLOCK - 1) / BLOCK;
@@ -398,10 +415,12 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
       for (int r = 0; r < R; ++r) {
         cx<T> a = base[at_rd(b, r * NB)];
         // k = 0 multiplies by the table's exact (1, -0): no branch needed
-        if (r > 0 && NS > 1) {
-          cx<T> wv = tw[OFF + (r - 1) * NS + k];
-          if (sign > 0) wv.y = -wv.y;
-          a = cmul(a, wv);
+        if constexpr (NS > 1) {
+          if (r > 0) {
+            cx<T> wv = tw_at<OFF, NS>(tw, r, k);
+            if (sign > 0) wv.y = -wv.y;
+            a = cmul(a, wv);
+          }
         }
         v[i][r] = a;
       }
@@ -422,8 +441,9 @@ __device__ __forceinline__ void fft_stage_c(cx<T>* buf, const cx<T>* tw, int sig
   __syncthreads();
 }
 
-template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int PLAN>
-__device__ __forceinline__ void fft_stages_c(cx<T>* buf, const cx<T>* tw, int sign) {
+template <typename T, int N, int S, int NS, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD, int PLAN,
+          class TW>
+__device__ __forceinline__ void fft_stages_c(cx<T>* buf, const TW& tw, int sign) {
   if constexpr (S < fft_nstages(N, PLAN)) {
     constexpr int R = fft_radix(N, S, PLAN);
     fft_stage_c<T, N, R, NS, ES, TS, NTR, BLOCK, LANE_T, PAD, fft_tw_off(N, S, PLAN)>(buf, tw, sign);
@@ -432,9 +452,11 @@ __device__ __forceinline__ void fft_stages_c(cx<T>* buf, const cx<T>* tw, int si
 }
 
 // PAD (with ES = 1, TS = lds_row(N)): padded contiguous transforms
-// tw: the stage-major table of N for PLAN (load_twiddles_for)
-template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD = false, int PLAN = 0>
-__device__ __forceinline__ void fft_lds_c(cx<T>* buf, const cx<T>* tw, int sign) {
+// tw: the stage-major table of N for PLAN (load_twiddles_for), or an object
+// that places that table elsewhere (tw_at)
+template <typename T, int N, int ES, int TS, int NTR, int BLOCK, bool LANE_T, bool PAD = false, int PLAN = 0,
+          class TW = const cx<T>*>
+__device__ __forceinline__ void fft_lds_c(cx<T>* buf, const TW& tw, int sign) {
   static_assert(fft_nstages(N, PLAN) > 0, "FFT length must be 2^a 3^b");
   static_assert(!PAD || (ES == 1 && TS >= lds_row(N)), "padded transforms are contiguous");
   fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T, PAD, PLAN>(buf, tw, sign);

@@ -1211,22 +1211,80 @@ static void launch_pad_products_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const 
 }
 
 // ------------------------------------------------------------------------
+// Twiddles of a compile-time transform kept in the padding words of 16 padded
+// LDS rows: word 17 m + 16 of a padded row (m < N/16) is never an element,
+// since lds_pad skips it.  Stage s >= 1 of the stage-major table takes hole
+// numbers h = H(s) + (r - 1) NS + k, H(s) rounded up to a multiple of 16, and
+// hole h is word 17 (h / 16) + 16 of row h % 16: for a given stage the row and
+// word split into a part that depends on k and a compile-time part of r.  The
+// first stage's twiddles are all 1 and are not stored.
+// ------------------------------------------------------------------------
+template <int N, int PLAN>
+constexpr int tw_hole_off(int s) {
+  int h = 0, ns = fft_radix(N, 0, PLAN);
+  for (int i = 1; i < s; ++i) {
+    const int r = fft_radix(N, i, PLAN);
+    h = (h + (r - 1) * ns + 15) / 16 * 16;
+    ns *= r;
+  }
+  return h;
+}
+template <int N, int PLAN>
+constexpr int tw_stage_at(int off) {
+  for (int s = 0; s < fft_nstages(N, PLAN); ++s)
+    if (fft_tw_off(N, s, PLAN) == off) return s;
+  return -1;
+}
+
+template <typename T, int N, int PLAN>
+struct TwHoles {
+  static constexpr int RS = lds_row(N);
+  static_assert(N % 16 == 0 && tw_hole_off<N, PLAN>(fft_nstages(N, PLAN)) <= N, "twiddles exceed the padding words");
+  cx<T>* rows;  // 16 padded rows
+  __device__ static int word(int h) { return (h & 15) * RS + 17 * (h >> 4) + 16; }
+  template <int OFF, int NS>
+  __device__ cx<T> at(int r, int k) const {
+    constexpr int S = tw_stage_at<N, PLAN>(OFF), H = tw_hole_off<N, PLAN>(S);
+    static_assert(S >= 1 && H % 16 == 0 && (NS % 16 == 0 || 16 % NS == 0), "twiddle hole layout");
+    if constexpr (NS >= 16) {
+      return rows[(k & 15) * RS + 17 * (k >> 4) + (17 * (H / 16 + (r - 1) * (NS / 16)) + 16)];
+    } else {
+      constexpr int GR = 16 / NS;  // stage rows r that share one word column
+      return rows[((r - 1) % GR * NS + k) * RS + (17 * (H / 16 + (r - 1) / GR) + 16)];
+    }
+  }
+  // copy the stage-major table of N for PLAN (from load_twiddles_for's source)
+  __device__ void fill(const cx<T>* twg) const {
+    const cx<T>* tab = twg + N + PLAN * N;
+    int ns = fft_radix(N, 0, PLAN), h0 = 0;
+    for (int s = 1; s < fft_nstages(N, PLAN); ++s) {
+      const int r = fft_radix(N, s, PLAN), cnt = (r - 1) * ns, off = fft_tw_off(N, s, PLAN);
+      for (int e = threadIdx.x; e < cnt; e += blockDim.x) rows[word(h0 + e)] = tab[off + e];
+      h0 = (h0 + cnt + 15) / 16 * 16;
+      ns *= r;
+    }
+  }
+};
+
+// ------------------------------------------------------------------------
 // Streaming variant of the padded products.  One workgroup owns padded row yb
-// and a run of output levels [k0, k1).  The physical fields of level k and of
-// level k + 1 stay in two LDS slots, and the few field pairs the products read
-// below level k (Pol::NPM) in a third, small one.  Each level step
-// inverse-transforms only the one new level, whose spectral rows were loaded
-// into registers during the previous step, so the HBM reads of level k + 2 are
-// in flight while level k is transformed, multiplied and transformed back.
-// The product of a column x reads only column x, so the thread that owns x
-// keeps level k's lower pairs for the next step and writes the products over
-// level k's first pairs, which no later step reads: the ring holds
-// 2 NP + NPM rows, which lets three 512-thread workgroups share a CU at
-// nx2 = 384 in FP32.  Each level is inverse-transformed once per run (the
-// batched kernel redoes the halo levels of every CH-level group), and the
-// transforms use the radix-8 plan, whose passes give a level's NP rows more
-// work-items than the two-pass plan.  The physical values equal the batched
-// kernel's up to the rounding of the transform plan.
+// and a run of output levels [k0, k1).  LDS rows: [X (NP)][O (NOP)][Y (NP)]
+// [M (NPM)]; level L's physical field pairs live in X when L - k0 is even, in
+// Y when odd, and M keeps the pairs the products read below level k.  A level
+// step (1) forms level k's products from X/Y/M into O, moving level k's lower
+// pairs to M; (2) stages level k + 2, whose spectral rows were loaded into
+// registers during the previous step, into level k's slot and starts the HBM
+// reads of level k + 3; (3) transforms level k + 2 and level k's products in
+// one pass over 9 contiguous rows (X O or O Y).  The products go through the
+// backward-signed pass conjugated, which gives the conjugate of their forward
+// transform exactly (every butterfly and twiddle product is symmetric under
+// conjugation), so one pass of one sign serves both.  The product of a column
+// x reads only column x, so the thread that owns x may overwrite it.  With 16
+// rows the twiddles sit in the rows' padding words (TwHoles), which keeps
+// three 512-thread workgroups on a CU at nx2 = 384 in FP32.  Each level is
+// inverse-transformed once per run (the batched kernel redoes the halo levels
+// of every CH-level group), with the radix-8 plan.  The physical values equal
+// the batched kernel's up to the rounding of the transform plan.
 // ------------------------------------------------------------------------
 template <typename T, int N, class Pol>
 struct PadStream {
@@ -1234,31 +1292,49 @@ struct PadStream {
   static constexpr int HB = Pol::HB, HA = Pol::HA, NPM = HB > 0 ? Pol::NPM : 0;
   static_assert(HA == 1 && HB <= 1 && NPM <= NP && NOP <= NP, "streaming products read levels k-1..k+1");
   static constexpr int RS = lds_row(N), NXH = N / 3, H2 = NXH / 2;  // nx2 = 3 nxh
+  static constexpr int NROWS = 2 * NP + NOP + NPM;
+  static constexpr int PLAN = 2;
+  static constexpr bool HOLES = NROWS >= 16 && N % 16 == 0;  // twiddles in the padding words
   static constexpr int BLOCK = N >= 384 ? 512 : 256;
   static constexpr int ITEMS = NP * H2;  // one item: two coefficients of a field pair
   static constexpr int MAXI = (ITEMS + BLOCK - 1) / BLOCK;
-  static constexpr int PLAN = 2;
-  static constexpr size_t LDS = sizeof(cx<T>) * ((size_t)N + (size_t)RS * (2 * NP + NPM));
+  static constexpr size_t LDS = sizeof(cx<T>) * ((size_t)RS * NROWS + (HOLES ? 0 : (size_t)N));
+  // waves per SIMD asked of the register allocator: at nx2 = 384 in FP32 the
+  // 16-row ring fits three workgroups (24 waves) per CU, which holds at 80
+  // registers; elsewhere no bound
+  static constexpr int WPE = (HOLES && N == 384 && sizeof(T) == 4) ? 6 : 1;
 };
 
 template <typename T, int N, class Pol>
 __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
-    k_pad_stream(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int lc,
+    __attribute__((amdgpu_waves_per_eu(PadStream<T, N, Pol>::WPE))) k_pad_stream(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int lc,
                  const cx<T>* __restrict__ twg, Pol pol) {
   using G = PadStream<T, N, Pol>;
-  constexpr int NF = G::NF, NP = G::NP, NO = G::NO, NOP = G::NOP, NPM = G::NPM;
+  constexpr int NF = G::NF, NP = G::NP, NO = G::NO, NOP = G::NOP, NPM = G::NPM, NROWS = G::NROWS;
   constexpr int RS = G::RS, NXH = G::NXH, H2 = G::H2, BLOCK = G::BLOCK, MAXI = G::MAXI, PLAN = G::PLAN;
   constexpr int ITEMS = G::ITEMS;
   struct alignas(2 * sizeof(cx<T>)) cx2 {
     cx<T> a, b;
   };
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* full = tw + N;                // [2][NP][RS]: field pairs of levels k and k + 1, physical
-  cx<T>* minus = full + 2 * NP * RS;   // [NPM][RS]: the pairs of level k - 1 the products read
+  cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* O = X + NP * RS;
+  cx<T>* Y = O + NOP * RS;
+  cx<T>* M = Y + NP * RS;
+  const auto tw = [&] {
+    if constexpr (G::HOLES) {
+      const TwHoles<T, N, PLAN> h{X};
+      h.fill(twg);
+      return h;
+    } else {
+      cx<T>* t = X + NROWS * RS;
+      load_twiddles_for(t, twg, N, true, PLAN);
+      return static_cast<const cx<T>*>(t);
+    }
+  }();
   const int yb = blockIdx.x, nl = nz - 1;
   const int k0 = blockIdx.y * lc, k1 = min(k0 + lc, nl);
-  auto slot = [&](int L) { return full + ((L - k0 + 2) & 1) * (NP * RS); };
+  auto slot = [&](int L) { return ((L - k0) & 1) ? Y : X; };
   cx2 ra[MAXI], rb[MAXI];
   auto fetch = [&](int L) {  // spectral rows of level L -> registers (zero outside 0..nz-1)
 #pragma unroll
@@ -1301,54 +1377,60 @@ __global__ void __launch_bounds__((PadStream<T, N, Pol>::BLOCK))
       s[t * RS + lds_pad(j)] = {T(0), T(0)};
     }
   };
-  load_twiddles_for(tw, twg, N, true, PLAN);
-  if constexpr (NPM > 0) {  // level k0 - 1: only its lower pairs are read
+  if constexpr (NPM > 0) {  // level k0 - 1 (slot Y): only its lower pairs are read
     fetch(k0 - 1);
     stage(k0 - 1);
     __syncthreads();
-    cx<T>* s = slot(k0 - 1);
-    fft_lds_c<T, N, 1, RS, NPM, BLOCK, false, true, PLAN>(s, tw, +1);
-    for (int e = threadIdx.x; e < NPM * RS; e += BLOCK) minus[e] = s[e];
+    fft_lds_c<T, N, 1, RS, NPM, BLOCK, false, true, PLAN>(Y, tw, +1);
+    for (int e = threadIdx.x; e < NPM * N; e += BLOCK) {
+      const int p = e / N, x = e - p * N;
+      M[p * RS + lds_pad(x)] = Y[p * RS + lds_pad(x)];
+    }
   }
   fetch(k0);
   stage(k0);
-  __syncthreads();
-  fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(slot(k0), tw, +1);
   fetch(k0 + 1);
+  __syncthreads();  // level k0 - 1 has left Y
+  stage(k0 + 1);
+  if (k0 + 2 <= k1) fetch(k0 + 2);
+  __syncthreads();
+  fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(X, tw, +1);
+  fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(Y, tw, +1);
   for (int k = k0; k < k1; ++k) {
     cx<T>* cur = slot(k);
     cx<T>* nxt = slot(k + 1);
-    if (k > k0) __syncthreads();  // the previous step's products in nxt have been stored
-    stage(k + 1);
-    if (k + 1 < k1) fetch(k + 2);  // in flight during this level's transforms
-    __syncthreads();
-    fft_lds_c<T, N, 1, RS, NP, BLOCK, false, true, PLAN>(nxt, tw, +1);
+    if (k > k0) __syncthreads();  // the previous level's products in O have been stored
     for (int x = threadIdx.x; x < N; x += BLOCK) {
       T o[2 * NOP];
 #pragma unroll
       for (int q = 0; q < 2 * NOP; ++q) o[q] = T(0);
       auto F = [&](int dk, int f) -> T {
-        const cx<T>* b = dk < 0 ? minus : (dk > 0 ? nxt : cur);
+        const cx<T>* b = dk < 0 ? M : (dk > 0 ? nxt : cur);
         const cx<T> z = b[(f >> 1) * RS + lds_pad(x)];
         return (f & 1) ? z.y : z.x;
       };
       pol.product(k, F, o);
-      // column x of level k's lower pairs moves to the small slot before the
-      // products overwrite it (only this thread touches column x)
-      cx<T> keep[NPM > 0 ? NPM : 1];
 #pragma unroll
-      for (int p = 0; p < NPM; ++p) keep[p] = cur[p * RS + lds_pad(x)];
+      for (int p = 0; p < NPM; ++p) M[p * RS + lds_pad(x)] = cur[p * RS + lds_pad(x)];
 #pragma unroll
-      for (int p = 0; p < NPM; ++p) minus[p * RS + lds_pad(x)] = keep[p];
-#pragma unroll
-      for (int q = 0; q < NOP; ++q) cur[q * RS + lds_pad(x)] = {o[2 * q], o[2 * q + 1]};
+      for (int q = 0; q < NOP; ++q) O[q * RS + lds_pad(x)] = {o[2 * q], -o[2 * q + 1]};
     }
     __syncthreads();
-    fft_lds_c<T, N, 1, RS, NOP, BLOCK, false, true, PLAN>(cur, tw, -1);
+    const bool more = k + 2 <= k1;  // level k + 2 is read by the products of level k + 1
+    if (more) {
+      stage(k + 2);  // into level k's slot
+      if (k + 3 <= k1) fetch(k + 3);  // in flight during the next level's work
+    }
+    __syncthreads();
+    if (more)
+      fft_lds_c<T, N, 1, RS, NP + NOP, BLOCK, false, true, PLAN>(cur == X ? X : O, tw, +1);
+    else
+      fft_lds_c<T, N, 1, RS, NOP, BLOCK, false, true, PLAN>(O, tw, +1);
     for (int e = threadIdx.x; e < NO * NXH; e += BLOCK) {
       const int o = e / NXH, kx = e - o * NXH;
-      const cx<T>* z = cur + (o >> 1) * RS;
-      const cx<T> zk = z[lds_pad(kx)], zn = z[lds_pad(kx ? N - kx : 0)];
+      const cx<T>* z = O + (o >> 1) * RS;
+      const cx<T> a = z[lds_pad(kx)], b = z[lds_pad(kx ? N - kx : 0)];
+      const cx<T> zk = {a.x, -a.y}, zn = {b.x, -b.y};  // forward transform = conj of the pass
       const cx<T> v = (o & 1) ? cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)}
                               : cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
       XP[(((long)Pol::slot(o) * nl + k) * ny2 + yb) * NXH + kx] = v;
