@@ -238,28 +238,35 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
     // points is issued before the first write (full rows, ldo = N + 2)
     if (ldo == N + 2) {
       constexpr int LDO = N + 2, NS = RowGeom<N>::P * LDO, BL = RowGeom<N>::BLOCK, MS = (NS + BL - 1) / BL;
-      typename Storer::Pre pre[MS][2];
+      // load phases of at most PB elements at a time (about 48 registers of
+      // loaded values), so that a large Pre does not cut the waves per SIMD
+      constexpr int W = 2 * (int)(sizeof(typename Storer::Pre) / 4);
+      constexpr int PB = 48 / W < 1 ? 1 : (48 / W > MS ? MS : 48 / W);
 #pragma unroll
-      for (int i = 0; i < MS; ++i) {
-        const int e = threadIdx.x + i * BL;
-        const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
-        if ((NS % BL == 0 || e < NS) && pp < npp) {
-          pre[i][0] = S.pre(plane, 2 * pp, x);
-          pre[i][1] = S.pre(plane, 2 * pp + 1, x);
-        }
-      }
+      for (int i0 = 0; i0 < MS; i0 += PB) {
+        typename Storer::Pre pre[PB][2];
 #pragma unroll
-      for (int i = 0; i < MS; ++i) {
-        const int e = threadIdx.x + i * BL;
-        const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
-        if ((NS % BL == 0 || e < NS) && pp < npp) {
-          T v0 = T(0), v1 = T(0);
-          if (x < N) {
-            v0 = buf[ix(t, x)].x;
-            v1 = buf[ix(t, x)].y;
+        for (int ii = 0; ii < PB; ++ii) {
+          const int i = i0 + ii, e = threadIdx.x + i * BL;
+          const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
+          if (i < MS && (NS % BL == 0 || e < NS) && pp < npp) {
+            pre[ii][0] = S.pre(plane, 2 * pp, x);
+            pre[ii][1] = S.pre(plane, 2 * pp + 1, x);
           }
-          S.fin(plane, 2 * pp, x, v0, pre[i][0]);
-          S.fin(plane, 2 * pp + 1, x, v1, pre[i][1]);
+        }
+#pragma unroll
+        for (int ii = 0; ii < PB; ++ii) {
+          const int i = i0 + ii, e = threadIdx.x + i * BL;
+          const int t = e / LDO, x = e - t * LDO, pp = pair0 + t;
+          if (i < MS && (NS % BL == 0 || e < NS) && pp < npp) {
+            T v0 = T(0), v1 = T(0);
+            if (x < N) {
+              v0 = buf[ix(t, x)].x;
+              v1 = buf[ix(t, x)].y;
+            }
+            S.fin(plane, 2 * pp, x, v0, pre[ii][0]);
+            S.fin(plane, 2 * pp + 1, x, v1, pre[ii][1]);
+          }
         }
       }
       return;
