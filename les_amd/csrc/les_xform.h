@@ -176,19 +176,20 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
+  // the block's plane (uniform), then 32-bit offsets inside it
+  const cx<T>* inp = in + (long)plane * ny * irow;
+  const cx<T>* in2p = in2 ? in2 + (long)plane * ny * irow : nullptr;
   // element j of row pair t: z_j = a_j + i b_j (conjugate partners above n/2)
   auto element = [&](int t, int j) {
     const int pp = pair0 + t;
     cx<T> z = {T(0), T(0)};
     if (pp < npp) {
-      const long r0 = ((long)plane * ny + 2 * pp) * irow;
-      const cx<T>* a = in + r0;
-      const cx<T>* b = a + irow;
+      const unsigned a = (unsigned)(2 * pp) * (unsigned)irow, b = a + (unsigned)irow;
       // optional second spectral input added with weight c2 (same layout)
-      auto get = [&](const cx<T>* q, int jj) {
-        cx<T> v = q[jj];
+      auto get = [&](unsigned r, int jj) {
+        cx<T> v = inp[r + (unsigned)jj];
         if (in2) {
-          const cx<T> w = in2[(q - in) + jj];
+          const cx<T> w = in2p[r + (unsigned)jj];
           v = {v.x + c2 * w.x, v.y + c2 * w.y};
         }
         return v;
@@ -272,17 +273,36 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
       return;
     }
   }
-  const float inv_ldo = 1.0f / (float)ldo;
-  for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
-    const int t = div_small(e, inv_ldo), x = e - t * ldo, pp = pair0 + t;
-    if (pp >= npp) continue;
-    T v0 = T(0), v1 = T(0);
-    if (x < n) {
-      v0 = buf[ix(t, x)].x;
-      v1 = buf[ix(t, x)].y;
+  if constexpr (N > 0) {
+    // the transform's N columns (a whole number of passes over the block),
+    // then the ldo - N pad columns, which the storers set to 0
+    for (int e = threadIdx.x; e < RowGeom<N>::P * N; e += BLOCK) {
+      const int t = e / N, x = e - t * N, pp = pair0 + t;
+      if (pp >= npp) continue;
+      const cx<T> v = buf[ix(t, x)];
+      S(plane, 2 * pp, x, v.x);
+      S(plane, 2 * pp + 1, x, v.y);
     }
-    S(plane, 2 * pp, x, v0);
-    S(plane, 2 * pp + 1, x, v1);
+    const int npad = ldo - N;
+    for (int e = threadIdx.x; e < RowGeom<N>::P * npad; e += BLOCK) {
+      const int t = e / npad, x = N + (e - t * npad), pp = pair0 + t;
+      if (pp >= npp) continue;
+      S(plane, 2 * pp, x, T(0));
+      S(plane, 2 * pp + 1, x, T(0));
+    }
+  } else {
+    const float inv_ldo = 1.0f / (float)ldo;
+    for (int e = threadIdx.x; e < P * ldo; e += BLOCK) {
+      const int t = div_small(e, inv_ldo), x = e - t * ldo, pp = pair0 + t;
+      if (pp >= npp) continue;
+      T v0 = T(0), v1 = T(0);
+      if (x < n) {
+        v0 = buf[ix(t, x)].x;
+        v1 = buf[ix(t, x)].y;
+      }
+      S(plane, 2 * pp, x, v0);
+      S(plane, 2 * pp + 1, x, v1);
+    }
   }
 }
 
@@ -424,6 +444,10 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
     P = 256 / tpf > 0 ? 256 / tpf : 1;
     threads = P * tpf;
   }
+  if (ldo < n) {  // the output rows hold the n physical values (plus pads)
+    c->err = "row inverse: output row shorter than the transform";
+    return;
+  }
   const long blocks = (ny / 2 + P - 1) / P;
   const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_inv",
@@ -461,7 +485,7 @@ struct LoadFields {
   static constexpr double reads() { return 1.0; }
   __device__ T operator()(int plane, int y, int x) const {
     const int f = plane / ppf, r = plane - f * ppf;
-    return scale * p[f][((long)r * ny + y) * ld + x];
+    return scale * (p[f] + (long)r * ny * ld)[(unsigned)(y * ld + x)];
   }
 };
 template <typename T, int NF>
@@ -472,7 +496,7 @@ struct LoadFieldsN {  // up to NF fields, plane = field * ppf + level
   static constexpr double reads() { return 1.0; }
   __device__ T operator()(int plane, int y, int x) const {
     const int f = plane / ppf, r = plane - f * ppf;
-    return scale * p[f][((long)r * ny + y) * ld + x];
+    return scale * (p[f] + (long)r * ny * ld)[(unsigned)(y * ld + x)];
   }
 };
 template <typename T>
@@ -483,7 +507,7 @@ struct StoreFields {
   static constexpr double writes() { return 1.0; }
   __device__ void operator()(int plane, int y, int x, T v) const {
     const int f = plane / ppf, r = plane - f * ppf;
-    p[f][((long)r * ny + y) * ld + x] = x < nx ? v : T(0);
+    (p[f] + (long)r * ny * ld)[(unsigned)(y * ld + x)] = x < nx ? v : T(0);
   }
 };
 
