@@ -540,13 +540,20 @@ __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
 }
 
 // block: TX * n/8 threads (plans 0 and 2: at most two butterflies per thread
-// in any pass)
-template <int TX, int NYI, int NYO>
+// in any pass).  WIDE (FP32): same-length passes, which keep the spectrum in
+// registers, take TX * n/4 threads where that fits 1024, so each thread holds
+// half as many tile elements (65 registers instead of 102 at TX 8, n 256:
+// filt_da's columns 0.231 -> 0.219 ms at 256^2); in FP64 the wider block was
+// slower and is not used
+template <int TX, int NYI, int NYO, bool WIDE = false>
 struct ColGeom {
   static constexpr int NMAX = NYI > NYO ? NYI : NYO;
-  static constexpr int B0 = (TX * NMAX / 8 + 63) / 64 * 64;
+  static constexpr int B8 = (TX * NMAX / 8 + 63) / 64 * 64, B4 = (TX * NMAX / 4 + 63) / 64 * 64;
+  static constexpr int B0 = WIDE && NYI == NYO && B4 <= 1024 ? B4 : B8;
   static constexpr int BLOCK = B0 < 64 ? 64 : B0;
 };
+template <typename T, int TX, int NYI, int NYO>
+using ColGeomT = ColGeom<TX, NYI, NYO, sizeof(T) == 4>;
 
 // NYI/NYO > 0: compile-time y lengths (block size ColGeom::BLOCK); 0: run time.
 // IP: the single identity output of a zero-pad or truncation (padd.f90 /
@@ -554,13 +561,13 @@ struct ColGeom {
 // overlap their destinations, so the tile needs max(NYI, NYO) rows and no
 // second buffer (more workgroups per CU, the same values).
 template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false>
-__global__ void __launch_bounds__((NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : 1024))
+__global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 1024))
     k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
           FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
           const T* __restrict__ kxt, const T* __restrict__ kyt, const T* __restrict__ G0, const T* __restrict__ G1,
           const T* __restrict__ G2) {
   constexpr bool CT = NYI > 0;
-  constexpr int BLK = CT ? ColGeom<TX, NYI, NYO>::BLOCK : 0;
+  constexpr int BLK = CT ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nyi = CT ? NYI : a.ny_in;
   const int nyo = CT ? NYO : a.ny_out;
@@ -860,7 +867,7 @@ template <typename T>
 constexpr int col_tx_default() { return sizeof(T) == 8 ? 8 : 16; }
 template <typename T, int TX, int NYI, int NYO>
 constexpr bool col_tile_fits() {
-  return ColGeom<TX, NYI, NYO>::BLOCK <= 1024 &&
+  return ColGeomT<T, TX, NYI, NYO>::BLOCK <= 1024 &&
          (size_t)(NYI + NYO + TX * NYI + TX * NYO) * sizeof(cx<T>) <= 160 * 1024;
 }
 
@@ -869,7 +876,7 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
                          const ColArgs& a, long nplanes, int ncols, const T* G0, const T* G1, const T* G2) {
   const int nmax = a.ny_in > a.ny_out ? a.ny_in : a.ny_out;
   // the FFT stages need blockDim >= TX*n/8 (at most 4 butterflies per thread)
-  int threads = NYI > 0 ? ColGeom<TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
+  int threads = NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : (TX * nmax / 8 + 63) / 64 * 64;
   if (threads < 64) threads = 64;
   if (threads > 1024) {
     c->err = "k_col: column tile too large";
