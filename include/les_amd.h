@@ -262,13 +262,14 @@ int les_profile_report(les_ctx *c, char *buf, int buflen);
    variant (what & 1; kx columns per block -- FP32 8/16/32, FP64 4/8/16 -- and
    the FFT plan, radix-4 or radix-8 passes), the tridiagonal schedule
    (what & 2; where the Thomas sweep keeps gamma and U -- 0 HBM, 1 gamma in
-   LDS, 2 both in LDS -- and modes per block, or 3: parallel cyclic reduction,
-   one wavefront per mode) and the padded-product pass of the
+   LDS, 2 both in LDS -- and modes per block, 3: parallel cyclic reduction,
+   one wavefront per mode, or 4: the sweep from both ends, four threads per
+   mode, at most 256 modes per block) and the padded-product pass of the
    convective step (what & 4; groups of levels per workgroup, or streaming
    runs of levels with the radix-8 plan) on this context's state, and keeps
    the fastest.  Widths and Thomas schedules give identical results; the
-   plans, the streaming products and cyclic reduction change only the
-   rounding.  report: JSON with the measured ms per
+   plans, the streaming products, cyclic reduction and the two-ended sweep
+   change only the rounding.  report: JSON with the measured ms per
    candidate (may be NULL).  les_get_tune / les_set_tune: {col_tx FP32,
    col_tx FP64, tri_mode, tri_block, col_plan FP32, col_plan FP64,
    pad_mode FP32, pad_mode FP64}. ---- */

@@ -1191,7 +1191,8 @@ static void tune_run(les_ctx* c, int what, std::string& rep) {
     cx<T>* PH = (cx<T>*)scratch(c, S_PHAT, (size_t)nz * ps * sizeof(cx<T>));
     char* tri = (char*)scratch(c, S_TRI, (size_t)(nz + 1) * ps * (sizeof(double) + sizeof(cx<double>)) + 256);
     int* err = (int*)scratch(c, S_TMP2, 256);
-    const int cand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {2, 32}, {2, 64}, {3, 64}};
+    const int cand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {2, 32},
+                           {2, 64}, {3, 64}, {4, 32}, {4, 64}, {4, 128}};
     const Tune keep = c->tune;
     Tune best = keep;
     double tbest = 1e30;
@@ -1287,7 +1288,8 @@ int les_set_tune(les_ctx* c, const int* in8) {
   if (!in8) return -1;
   const int* in6 = in8;
   const bool ok32 = in6[0] == 8 || in6[0] == 16 || in6[0] == 32, ok64 = in6[1] == 4 || in6[1] == 8 || in6[1] == 16;
-  const bool okt = in6[2] >= 0 && in6[2] <= 3 && in6[3] >= 32 && in6[3] <= 512 && in6[3] % 32 == 0 &&
+  const bool okt = in6[2] >= 0 && in6[2] <= 4 && in6[3] >= 32 && in6[3] <= (in6[2] == 4 ? 256 : 512) &&
+                   in6[3] % 32 == 0 &&
                    tridiag_lds(in6[2], in6[3], c->g.nz) <= 160 * 1024;
   const bool okp = (in6[4] == 0 || (in6[4] == 2 && (in6[0] == 8 || in6[0] == 16))) &&
                    (in6[5] == 0 || (in6[5] == 2 && (in6[1] == 4 || in6[1] == 8)));

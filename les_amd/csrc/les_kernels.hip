@@ -1113,6 +1113,166 @@ struct StorePress {  // field 0: p levels 1..nz-1 ; 1: dpdx levels 1..nz-1 ; 2: 
   }
 };
 
+// Two-ended Thomas sweep (schedule 4): four threads per mode, (side, component).
+// Side 0 eliminates rows 0 .. mid-1 downward as k_tridiag does; side 1
+// eliminates rows nz .. mid+1 upward (x_j = v_j - delta_j x_{j-1}, delta_j =
+// a_j / beta'_j, beta'_j = b_j - c_j delta_{j+1}, v_j = (r_j - c_j v_{j+1}) /
+// beta'_j).  The two sides meet at row mid, exchange their partial results
+// through a lane swizzle, and both form x_mid from the same four numbers;
+// each side then substitutes back over its own half.  Half the sequential
+// chain and twice the waves of the one-ended sweep; another elimination
+// order, so the same solution to rounding.  gamma / delta in LDS (one column
+// of nz + 1 values per mode), u / v in HBM scratch; Nyquist row and zero mode
+// as k_tridiag.
+template <typename T>
+__global__ void k_tridiag_tw(const cx<T>* __restrict__ H, const cx<T>* __restrict__ BT, cx<T>* __restrict__ P,
+                             cx<double>* __restrict__ U_g, const double* __restrict__ kxd,
+                             const double* __restrict__ kyd, int nxh, int ny, int nz, double dz, int* err) {
+  extern __shared__ __align__(16) unsigned char tri_smem[];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = t >> 2, side = (t >> 1) & 1, comp = t & 1;
+  const int nm = nxh * ny;
+  if (m >= nm) return;  // blockDim is a multiple of 4: the four threads of a mode leave together
+  const int tm = threadIdx.x >> 2, bm = blockDim.x >> 2;
+  double* gl = reinterpret_cast<double*>(tri_smem);
+  auto gam = [&](int j) -> double& { return gl[j * bm + tm]; };
+  double* Ug = reinterpret_cast<double*>(U_g);
+  auto U = [&](int j) -> double& { return Ug[2 * ((long)j * nm + m) + comp]; };
+  const int y = m / nxh, x = m - y * nxh;
+  const long ps = (long)nm, fs = ps * (nz - 1);
+  const T* Hx = reinterpret_cast<const T*>(H);
+  const T* Hy = reinterpret_cast<const T*>(H + fs);
+  const T* Hz = reinterpret_cast<const T*>(H + 2 * fs);
+  const T* B2 = reinterpret_cast<const T*>(BT);
+  T* P2 = reinterpret_cast<T*>(P);
+  auto at = [&](long j) { return 2 * (j * ps + m); };
+  if (y == ny / 2) {
+    if (side == 0)
+      for (int j = 0; j < nz; ++j) P2[at(j) + comp] = T(0);
+    return;
+  }
+  if (m == 0) {  // zero mode: the integration of k_tridiag, on side 0
+    if (side) return;
+    const double bot = (double)B2[at(0) + comp];
+    double pv = 0.0;
+    P2[at(0) + comp] = T(0);
+    pv = pv - dz * bot;
+    P2[at(1) + comp] = (T)pv;
+    for (int j = 2; j < nz; ++j) {
+      pv = pv + (double)Hz[at(j - 1) + comp] * dz;
+      P2[at(j) + comp] = (T)pv;
+    }
+    return;
+  }
+  const double kx = kxd[x], ky = kyd[y];
+  const double a_in = 1.0 / (dz * dz), c_in = 1.0 / (dz * dz);
+  const double b_in = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+  const int oc = 1 - comp;
+  const double sg = comp == 0 ? -1.0 : 1.0;
+  const int mid = nz / 2;  // 1 <= mid <= nz - 1
+  // level L's (Hx, Hy, Hz) components, zero above the last H plane (nz - 2)
+  auto fetch = [&](int L, T& a, T& b, T& cz) {
+    a = b = cz = T(0);
+    if (L >= 0 && L <= nz - 2) {
+      a = Hx[at(L) + oc];
+      b = Hy[at(L) + oc];
+      cz = Hz[at(L) + comp];
+    }
+  };
+  // interior row j: r_j from levels j - 1 and j (press_stag_array.f90:142-165)
+  auto rhs = [&](double hxm, double hym, double hzm, double hzj) { return sg * (kx * hxm + ky * hym) + (hzj - hzm) / dz; };
+  constexpr int D = 8;  // levels loaded ahead of the recurrence
+  T xs[D], ys[D], zs[D];
+  double pnum, pden;  // this side's share of row mid's equation
+  if (side == 0) {
+    const double bot = (double)B2[at(0) + comp];
+    double bet = -1.0, u = (-dz * bot) / bet, cprev = 1.0;
+    U(0) = u;
+#pragma unroll
+    for (int q = 0; q < D; ++q) fetch(q + 1, xs[q], ys[q], zs[q]);
+    double hxm = (double)Hx[at(0) + oc], hym = (double)Hy[at(0) + oc], hzm = (double)Hz[at(0) + comp];
+    for (int j = 1; j <= mid; ++j) {
+      const double hxj = (double)xs[0], hyj = (double)ys[0], hzj = (double)zs[0];  // level j
+#pragma unroll
+      for (int q = 0; q < D - 1; ++q) {
+        xs[q] = xs[q + 1];
+        ys[q] = ys[q + 1];
+        zs[q] = zs[q + 1];
+      }
+      fetch(j + D, xs[D - 1], ys[D - 1], zs[D - 1]);
+      const double rj = rhs(hxm, hym, hzm, hzj);
+      hxm = hxj;
+      hym = hyj;
+      hzm = hzj;
+      const double gj = cprev / bet;
+      if (j == mid) {  // row mid: a (u_{mid-1} - gamma_mid x_mid) + b x_mid + c x_{mid+1} = r
+        pnum = rj - a_in * u;
+        pden = b_in - a_in * gj;
+        gam(j) = gj;
+        break;
+      }
+      bet = b_in - a_in * gj;
+      if (bet == 0.0) atomicExch(err, 2);
+      gam(j) = gj;
+      u = (rj - a_in * u) / bet;
+      U(j) = u;
+      cprev = c_in;
+    }
+  } else {
+    const double top = (double)B2[at(1) + comp];
+    double bet = 1.0, v = (-top * dz) / bet, aprev = -1.0;  // row nz: a = -1, b = 1
+    U(nz) = v;
+    gam(nz) = aprev / bet;
+#pragma unroll
+    for (int q = 0; q < D; ++q) fetch(nz - 2 - q, xs[q], ys[q], zs[q]);
+    double hzj = 0.0;  // Hz at level nz - 1 (above the last H plane)
+    for (int j = nz - 1; j >= mid + 1; --j) {
+      const double hxm = (double)xs[0], hym = (double)ys[0], hzm = (double)zs[0];  // level j - 1
+#pragma unroll
+      for (int q = 0; q < D - 1; ++q) {
+        xs[q] = xs[q + 1];
+        ys[q] = ys[q + 1];
+        zs[q] = zs[q + 1];
+      }
+      fetch(j - 1 - D, xs[D - 1], ys[D - 1], zs[D - 1]);
+      const double rj = rhs(hxm, hym, hzm, hzj);
+      hzj = hzm;
+      const double dj = aprev / bet;  // delta_{j+1}
+      bet = b_in - c_in * dj;
+      if (bet == 0.0) atomicExch(err, 2);
+      v = (rj - c_in * v) / bet;
+      U(j) = v;
+      aprev = a_in;
+      gam(j) = aprev / bet;  // delta_j
+    }
+    pnum = c_in * v;             // c x_{mid+1} = c v_{mid+1} - c delta_{mid+1} x_mid
+    pden = c_in * gam(mid + 1);
+  }
+  // both sides form x_mid from the same four numbers
+  const double onum = __shfl_xor(pnum, 2), oden = __shfl_xor(pden, 2);
+  const double num0 = side == 0 ? pnum : onum, den0 = side == 0 ? pden : oden;
+  const double num1 = side == 0 ? onum : pnum, den1 = side == 0 ? oden : pden;
+  const double den = den0 - den1;
+  if (den == 0.0) atomicExch(err, 2);
+  const double xm = (num0 - num1) / den;
+  if (side == 0) {
+    P2[at(mid) + comp] = (T)xm;
+    double un = xm;
+    for (int j = mid - 1; j >= 0; --j) {
+      const double uj = U(j) - gam(j + 1) * un;
+      P2[at(j) + comp] = (T)uj;
+      un = uj;
+    }
+  } else {
+    double xn = xm;
+    for (int j = mid + 1; j <= nz - 1; ++j) {
+      const double xj = U(j) - gam(j) * xn;
+      P2[at(j) + comp] = (T)xj;
+      xn = xj;
+    }
+  }
+}
+
 // Parallel cyclic reduction: one wavefront per mode (kx, ky), four modes per
 // workgroup; the (nz+1)-row system of press_stag_array.f90:142-165 (the rows
 // and right-hand side k_tridiag forms) sits in LDS, FP64, and each step
@@ -1230,6 +1390,7 @@ __global__ void __launch_bounds__(256)
 // the mode does not use LDS): gamma per mode, U per mode and component
 size_t tridiag_lds(int mode, int block, int nz) {
   if (mode == 3) return (size_t)4 * 10 * (nz + 1) * sizeof(double);  // four waves, two row buffers each
+  if (mode == 4) return (size_t)(nz + 1) * block * sizeof(double);    // gamma / delta per mode
   return (size_t)(nz + 1) * block * (mode >= 1 ? sizeof(double) : 0) +
          (size_t)(nz + 1) * block * (mode >= 2 ? sizeof(cx<double>) : 0);
 }
@@ -1243,10 +1404,16 @@ void launch_tridiag(les_ctx* c, const cx<T>* H, const cx<T>* BT, cx<T>* PH, doub
   const size_t sm = tridiag_lds(mode, bs, nz);
   // bs modes per block, two threads per mode
   const dim3 grid((unsigned)((ps + bs - 1) / bs)), block((unsigned)(2 * bs));
+  // gamma in HBM for schedule 0, u in HBM for 0, 1 and 4
   ProfScope _ps(c, "tridiag", (3.0 * nl + 2 + nz) * ps * sizeof(cx<T>) +
                                   (nz + 1) * ps * 2.0 * ((mode >= 1 ? 0 : sizeof(double)) +
-                                                         (mode >= 2 ? 0 : sizeof(cx<double>))));
-  if (mode == 3) {
+                                                         (mode == 2 || mode == 3 ? 0 : sizeof(cx<double>))));
+  if (mode == 4) {  // bs modes per block, four threads per mode
+    if (sm > 65536)
+      LCHK(c, hipFuncSetAttribute((const void*)k_tridiag_tw<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL((k_tridiag_tw<T>), dim3((unsigned)((ps + bs - 1) / bs)), dim3((unsigned)(4 * bs)), sm,
+                       c->stream, H, BT, PH, Ub, c->td.kx, c->td.ky, nxh, g.ny, nz, g.dz, err);
+  } else if (mode == 3) {
     if (sm > 65536)
       LCHK(c, hipFuncSetAttribute((const void*)k_tridiag_pcr<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     hipLaunchKernelGGL((k_tridiag_pcr<T>), dim3((unsigned)((ps + 3) / 4)), dim3(256), sm, c->stream, H, BT, PH,

@@ -56,7 +56,7 @@ def test_autotune_chooses_a_measured_candidate(torch_mod):
     assert ch["tri_mode"] == t["tri_mode"] and ch["tri_block"] == t["tri_block"]
     assert str(t["pad_mode_fp32"]) in rep["pad"] and ch["pad_mode"][0] == t["pad_mode_fp32"]
     assert all(v > 0 for k in ("col", "tridiag", "pad") for v in rep[k].values())
-    if t["col_plan_fp32"] == 0 and t["pad_mode_fp32"] == 0 and t["tri_mode"] != 3:
+    if t["col_plan_fp32"] == 0 and t["pad_mode_fp32"] == 0 and t["tri_mode"] not in (3, 4):
         for n in FIELDS:
             assert np.array_equal(s.get(n), base.get(n)), n
 
@@ -105,18 +105,20 @@ def test_streaming_padded_products(torch_mod, prec, conv, n, ny, steps):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_cyclic_reduction_pressure_solve(torch_mod, prec):
-    """Tridiagonal schedule 3 (parallel cyclic reduction, one wavefront per
-    mode) against the Thomas sweep: after 12 convective steps the FP64 state
-    agrees to 1e-12 relative (the z systems are diagonally dominant except for
-    the boundary rows), the FP32-bulk state to FP32 round-off (2e-4, as for the
-    transform plans); also on a 64^2 x 16 grid."""
-    print("cyclic reduction", prec, flush=True)
+@pytest.mark.parametrize("mode,block", [(3, 64), (4, 32), (4, 64)])
+def test_other_elimination_orders_pressure_solve(torch_mod, prec, mode, block):
+    """Tridiagonal schedules 3 (parallel cyclic reduction, one wavefront per
+    mode) and 4 (two-ended Thomas sweep, four threads per mode) against the
+    one-ended sweep: after 12 convective steps the FP64 state agrees to 1e-12
+    relative (the z systems are diagonally dominant except for the boundary
+    rows), the FP32-bulk state to FP32 round-off (2e-4, as for the transform
+    plans); on 32^2 and 64^2 grids (nz = 17: the two ends meet at row 8)."""
+    print("elimination order", mode, block, prec, flush=True)
     tol = 1e-12 if prec == "fp64" else 2e-4
     for n in (32, 64):
         base, _ = run(prec, conv=True, n=n)
-        s, _ = run(prec, tune=dict(tri_mode=3, tri_block=64), conv=True, n=n)
-        assert s.get_tune()["tri_mode"] == 3
+        s, _ = run(prec, tune=dict(tri_mode=mode, tri_block=block), conv=True, n=n)
+        assert s.get_tune()["tri_mode"] == mode
         for nm in ("u", "v", "w", "p", "theta"):
             a, b = s.get(nm)[..., :n], base.get(nm)[..., :n]
             err = np.max(np.abs(a - b)) / np.max(np.abs(b))
@@ -132,6 +134,8 @@ def test_set_tune_rejects_bad_shapes(torch_mod):
     with pytest.raises(LesError):
         g.set_tune(tri_mode=2, tri_block=1024)   # (nz+1) x 1024 x 24 B does not fit in LDS
     with pytest.raises(LesError):
-        g.set_tune(tri_mode=4, tri_block=64)
+        g.set_tune(tri_mode=5, tri_block=64)
+    with pytest.raises(LesError):
+        g.set_tune(tri_mode=4, tri_block=512)   # four threads per mode: at most 256 modes per block
     with pytest.raises(LesError):
         g.set_tune(pad_mode_fp32=2)
