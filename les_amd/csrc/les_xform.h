@@ -62,15 +62,16 @@ static const FftPlan& plan_for(const les_ctx* c, int n) {
 // blockIdx.x = group of P row pairs inside it, so loaders and storers get a
 // block-uniform plane index (no per-element integer division).  N > 0 is the
 // compile-time transform length; N == 0 runs the same code with run-time n.
-// PLAN: the FP32 forward (r2c) kernels use radix-8 passes (plan 2) from 64
-// points, the others radix-4 passes (plan 0; inverse epilogues would hold
-// twice the elements per thread with plan 2's thread count, and FP64 forward
-// rows measured faster with plan 0); TPF threads per
-// transform (n/8 with plan 2, whose last radix-4 or -2 pass takes two
-// butterflies per thread; n/4 with plan 0), P transforms per block
-template <int N, bool FWD = false>
+// PLAN: R8 selects radix-8 passes (plan 2) from 64 points, else radix-4
+// passes (plan 0).  FP32 forward (r2c) kernels and FP32 inverse kernels with
+// plain storers use plan 2; inverse epilogues with a load phase would hold
+// twice the points per thread at plan 2's thread count, and FP64 rows measured
+// faster with plan 0.  TPF threads per transform (n/8 with plan 2, whose last
+// radix-4 or -2 pass takes two butterflies per thread; n/4 with plan 0), P
+// transforms per block
+template <int N, bool R8 = false>
 struct RowGeom {
-  static constexpr int PLAN = FWD && N >= 64 ? 2 : 0;
+  static constexpr int PLAN = R8 && N >= 64 ? 2 : 0;
   static constexpr int TPF = N > 0 ? (PLAN == 2 ? N / 8 : N / 4) : 64;
   static constexpr int P = (256 / TPF) > 0 ? 256 / TPF : 1;
   static constexpr int BLOCK = P * TPF;
@@ -156,21 +157,27 @@ template <class S>
 struct StorerHasPre<S, std::void_t<typename S::Pre>> {
   static constexpr bool value = true;
 };
+// inverse rows: the radix-8 plan (eight transforms per block) in FP32 for
+// plain storers; storers with a load phase keep the radix-4 plan, whose
+// block of four transforms holds half the points per thread
+template <typename T, int N, class S>
+using RowGeomI = RowGeom<N, sizeof(T) == 4 && !StorerHasPre<S>::value>;
 
 template <typename T, int N, class Storer>
-__global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
+__global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024))
     k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
               const cx<T>* __restrict__ twg, int ldo, const cx<T>* __restrict__ in2, T c2) {
+  using RG = RowGeomI<T, N, Storer>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
   const int n = N > 0 ? N : p.n;
-  const int P = N > 0 ? RowGeom<N>::P : (int)(blockDim.x * 4) / n;
-  const int BLOCK = N > 0 ? RowGeom<N>::BLOCK : (int)blockDim.x;
+  const int P = N > 0 ? RG::P : (int)(blockDim.x * 4) / n;
+  const int BLOCK = N > 0 ? RG::BLOCK : (int)blockDim.x;
   constexpr int RS = N > 0 ? lds_row(N) : 0;  // padded row stride (compile-time lengths)
   auto ix = [&](int t, int e) { return N > 0 ? t * RS + lds_pad(e) : t * n + e; };
   cx<T>* buf = tw + n;
   if (N > 0)
-    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + RowGeom<N>::PLAN * n + i];
+    for (int i = threadIdx.x; i < n - 1; i += BLOCK) tw[i] = twg[n + RG::PLAN * n + i];
   else
     for (int i = threadIdx.x; i < n; i += BLOCK) tw[i] = twg[i];
   const int plane = blockIdx.y;
@@ -209,7 +216,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   };
   if constexpr (N > 0) {
     // every row load of the block is issued before the first LDS store
-    constexpr int NE = RowGeom<N>::P * (N > 0 ? N : 1), BL = RowGeom<N>::BLOCK, MAXE = (NE + BL - 1) / BL;
+    constexpr int NE = RG::P * (N > 0 ? N : 1), BL = RG::BLOCK, MAXE = (NE + BL - 1) / BL;
     cx<T> z[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
@@ -231,14 +238,14 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   }
   __syncthreads();
   if constexpr (N > 0)
-    fft_lds_c<T, N, 1, RS, RowGeom<N>::P, RowGeom<N>::BLOCK, false, true, RowGeom<N>::PLAN>(buf, tw, +1);
+    fft_lds_c<T, N, 1, RS, RG::P, RG::BLOCK, false, true, RG::PLAN>(buf, tw, +1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, +1);
   if constexpr (N > 0 && StorerHasPre<Storer>::value) {
     // storers that read the arrays they update: every read of the block's
     // points is issued before the first write (full rows, ldo = N + 2)
     if (ldo == N + 2) {
-      constexpr int LDO = N + 2, NS = RowGeom<N>::P * LDO, BL = RowGeom<N>::BLOCK, MS = (NS + BL - 1) / BL;
+      constexpr int LDO = N + 2, NS = RG::P * LDO, BL = RG::BLOCK, MS = (NS + BL - 1) / BL;
       // load phases of at most PB elements at a time (about 48 registers of
       // loaded values), so that a large Pre does not cut the waves per SIMD
       constexpr int W = 2 * (int)(sizeof(typename Storer::Pre) / 4);
@@ -276,7 +283,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
   if constexpr (N > 0) {
     // the transform's N columns (a whole number of passes over the block),
     // then the ldo - N pad columns, which the storers set to 0
-    for (int e = threadIdx.x; e < RowGeom<N>::P * N; e += BLOCK) {
+    for (int e = threadIdx.x; e < RG::P * N; e += BLOCK) {
       const int t = e / N, x = e - t * N, pp = pair0 + t;
       if (pp >= npp) continue;
       const cx<T> v = buf[ix(t, x)];
@@ -284,7 +291,7 @@ __global__ void __launch_bounds__(N > 0 ? RowGeom<N>::BLOCK : 1024)
       S(plane, 2 * pp + 1, x, v.y);
     }
     const int npad = ldo - N;
-    for (int e = threadIdx.x; e < RowGeom<N>::P * npad; e += BLOCK) {
+    for (int e = threadIdx.x; e < RG::P * npad; e += BLOCK) {
       const int t = e / npad, x = N + (e - t * npad), pp = pair0 + t;
       if (pp >= npp) continue;
       S(plane, 2 * pp, x, T(0));
@@ -437,8 +444,8 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
   const FftPlan& p = plan_for(c, n);
   int P = 1, threads = 64;
   if constexpr (N > 0) {
-    P = RowGeom<N>::P;
-    threads = RowGeom<N>::BLOCK;
+    P = RowGeomI<T, N, S>::P;
+    threads = RowGeomI<T, N, S>::BLOCK;
   } else {  // run-time length: radix-4/2/3 passes, n/4 threads per transform
     const int tpf = n / 4;
     P = 256 / tpf > 0 ? 256 / tpf : 1;
