@@ -1183,7 +1183,7 @@ __global__ void k_tridiag_tw(const cx<T>* __restrict__ H, const cx<T>* __restric
   auto rhs = [&](double hxm, double hym, double hzm, double hzj) { return sg * (kx * hxm + ky * hym) + (hzj - hzm) / dz; };
   constexpr int D = 8;  // levels loaded ahead of the recurrence
   T xs[D], ys[D], zs[D];
-  double pnum, pden;  // this side's share of row mid's equation
+  double pnum = 0.0, pden = 0.0;  // this side's share of row mid's equation
   if (side == 0) {
     const double bot = (double)B2[at(0) + comp];
     double bet = -1.0, u = (-dz * bot) / bet, cprev = 1.0;
