@@ -546,6 +546,22 @@ __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
   return yo + nyi - nyo;
 }
 
+// XCD-aware tile order.  Workgroups go to the eight XCDs round-robin by linear
+// id, and each XCD has its own L2.  Tiles whose rows are 64 bytes come in
+// pairs that share every 128-byte line (kx tiles 2j, 2j + 1 of a plane); this
+// order gives a pair to ids p and p + 8, which run on the same XCD one after
+// the other, so the second tile finds its lines in that L2.  A bijection of
+// the grid when it has a multiple of 16 workgroups and an even width; the
+// identity otherwise.
+__device__ __forceinline__ void xcd_pair_tiles(unsigned& bx, unsigned& by) {
+  const unsigned gx = gridDim.x, n = gx * gridDim.y;
+  if ((n & 15u) != 0 || (gx & 1u) != 0) return;
+  const unsigned p = blockIdx.x + gx * blockIdx.y, x = p & 7u, q = p >> 3;
+  const unsigned tile = 2u * (x + 8u * (q >> 1)) + (q & 1u);
+  bx = tile % gx;
+  by = tile / gx;
+}
+
 // block: TX * n/8 threads (plans 0 and 2: at most two butterflies per thread
 // in any pass).  WIDE (FP32): same-length passes, which keep the spectrum in
 // registers, take TX * n/4 threads where that fits 1024, so each thread holds
@@ -599,8 +615,10 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     else
       fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
   };
-  const int kx0 = blockIdx.x * TX;
-  const long pl = blockIdx.y;
+  unsigned bx = blockIdx.x, by = blockIdx.y;
+  if constexpr (TX * sizeof(cx<T>) == 64) xcd_pair_tiles(bx, by);  // two tiles share 128-byte rows
+  const int kx0 = bx * TX;
+  const long pl = by;
   // input tile -> LDS; with compile-time lengths every load of the tile is
   // issued before the first LDS store, so the HBM latency is paid once
   auto load_tile = [&](cx<T>* dst, const cx<T>* src) {
