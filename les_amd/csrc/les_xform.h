@@ -551,12 +551,13 @@ __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
 // pairs that share every 128-byte line (kx tiles 2j, 2j + 1 of a plane); this
 // order gives a pair to ids p and p + 8, which run on the same XCD one after
 // the other, so the second tile finds its lines in that L2.  A bijection of
-// the grid when it has a multiple of 16 workgroups and an even width; the
-// identity otherwise.
+// the first 16 floor(n/16) workgroups of a grid of even width (the rest, and
+// grids of odd width, keep their own tile).
 __device__ __forceinline__ void xcd_pair_tiles(unsigned& bx, unsigned& by) {
-  const unsigned gx = gridDim.x, n = gx * gridDim.y;
-  if ((n & 15u) != 0 || (gx & 1u) != 0) return;
-  const unsigned p = blockIdx.x + gx * blockIdx.y, x = p & 7u, q = p >> 3;
+  const unsigned gx = gridDim.x, n16 = (gx * gridDim.y) & ~15u;
+  const unsigned p = blockIdx.x + gx * blockIdx.y;
+  if ((gx & 1u) != 0 || p >= n16) return;
+  const unsigned x = p & 7u, q = p >> 3;
   const unsigned tile = 2u * (x + 8u * (q >> 1)) + (q & 1u);
   bx = tile % gx;
   by = tile / gx;
