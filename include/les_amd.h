@@ -260,7 +260,8 @@ int les_profile_report(les_ctx *c, char *buf, int buflen);
 
 /* ---- autotuner (no counterpart in the reference code): times the column-FFT
    variant (what & 1; kx columns per block -- FP32 8/16/32, FP64 4/8/16 -- and
-   the FFT plan, radix-4 or radix-8 passes), the tridiagonal schedule
+   the FFT plan, radix-4 or radix-8 passes; in FP32 the zero-pad and
+   truncation passes always take radix-8), the tridiagonal schedule
    (what & 2; where the Thomas sweep keeps gamma and U -- 0 HBM, 1 gamma in
    LDS, 2 both in LDS -- and modes per block, 3: parallel cyclic reduction,
    one wavefront per mode, or 4: the sweep from both ends, four threads per
