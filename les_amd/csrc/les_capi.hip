@@ -1164,7 +1164,7 @@ static void tune_run(les_ctx* c, int what, std::string& rep) {
         op_ddxy<T>(c, A, B, u, nz);
         op_dealias1<T>(c, u, Bg, nz);
         op_dealias2<T>(c, A, Bg, nz);
-      });
+      }, 6);  // the candidates lie within a few percent of each other
       std::snprintf(buf, sizeof buf, "%s\"%d/%d\": %.4f", i ? ", " : "", c->tune.col_tx[p], c->tune.col_plan[p], ms);
       rep += buf;
       if (ms < tbest) {
