@@ -21,14 +21,26 @@ class LesError(RuntimeError):
     pass
 
 
-def _ptr(t):
+def _ptr(t, dtype=None, numel=None, what="array"):
+    """Device address of `t` after checking what the C ABI assumes: a
+    contiguous GPU tensor, of `dtype` when given, holding at least `numel`
+    elements when given (the kernels index the full reference layout)."""
     if t is None:
         return None
+    if not isinstance(t, torch.Tensor):
+        raise LesError(f"{what}: expected a torch tensor on the GPU, got {type(t).__name__}")
     if not t.is_cuda:
-        raise LesError("les_amd works on GPU tensors only (no CPU fallback)")
+        raise LesError(f"{what}: les_amd works on GPU tensors only (no CPU fallback)")
     if not t.is_contiguous():
-        raise LesError("les_amd needs contiguous tensors in the reference layout")
+        raise LesError(f"{what}: les_amd needs contiguous tensors in the reference layout")
+    if dtype is not None and t.dtype != dtype:
+        raise LesError(f"{what}: expected {dtype}, got {t.dtype}")
+    if numel is not None and t.numel() < numel:
+        raise LesError(f"{what}: needs at least {numel} elements, got {t.numel()} (shape {tuple(t.shape)})")
     return C.c_void_p(t.data_ptr())
+
+
+F64, F32 = torch.float64, torch.float32
 
 
 class Les:
@@ -70,7 +82,34 @@ class Les:
 
     @staticmethod
     def _sfx(t):
-        return "_f32" if t.dtype == torch.float32 else ""
+        if not isinstance(t, torch.Tensor):
+            raise LesError(f"expected a torch tensor on the GPU, got {type(t).__name__}")
+        if t.dtype not in (F64, F32):
+            raise LesError(f"les_amd routines take float64 (or float32 for the FP32-bulk variants), got {t.dtype}")
+        return "_f32" if t.dtype == F32 else ""
+
+    # ---- argument checks: sizes of the reference layout ----
+    def _f(self, t, what, nlev=None, dtype=F64):
+        """A (nlev, ny, ld) field (nlev defaults to nz)."""
+        nlev = self.nz if nlev is None else nlev
+        return _ptr(t, dtype, nlev * self.ny * self.ld, what)
+
+    def _fbig(self, t, what, nlev, dtype=F64):
+        """A (nlev, ny2, ld_big) padded field."""
+        return _ptr(t, dtype, nlev * self.ny2 * self.ld_big, what)
+
+    def _pl(self, t, what, dtype=F64):
+        """One (ny, ld) plane."""
+        return _ptr(t, dtype, self.ny * self.ld, what)
+
+    def _p2(self, t, what):
+        """One (ny, nx) surface array (always float64)."""
+        return _ptr(t, F64, self.ny * self.nx, what)
+
+    def _nlev(self, f, what="f"):
+        if not isinstance(f, torch.Tensor) or f.dim() < 1:
+            raise LesError(f"{what}: expected a (nlev, ny, ld) tensor")
+        return f.shape[0]
 
     # ---- allocation helpers (reference layout) ----
     def field(self, nlev=None, dtype=torch.float64, big=False):
@@ -96,16 +135,24 @@ class Les:
 
     # ---- routines (reference argument order) ----
     def filt_da(self, f, dfdx, dfdy):
-        self._chk(getattr(self.L, "les_filt_da" + self._sfx(f))(self.ctx, _ptr(f), _ptr(dfdx), _ptr(dfdy), f.shape[0]))
+        sfx, n = self._sfx(f), self._nlev(f)
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_filt_da" + sfx)(self.ctx, self._f(f, "f", n, dt), self._f(dfdx, "dfdx", n, dt),
+                                                       self._f(dfdy, "dfdy", n, dt), n))
 
     def ddx(self, dfdx, f):
-        self._chk(getattr(self.L, "les_ddx" + self._sfx(f))(self.ctx, _ptr(dfdx), _ptr(f), f.shape[0]))
+        sfx, n = self._sfx(f), self._nlev(f)
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_ddx" + sfx)(self.ctx, self._f(dfdx, "dfdx", n, dt), self._f(f, "f", n, dt), n))
 
     def ddy(self, dfdy, f):
-        self._chk(getattr(self.L, "les_ddy" + self._sfx(f))(self.ctx, _ptr(dfdy), _ptr(f), f.shape[0]))
+        sfx, n = self._sfx(f), self._nlev(f)
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_ddy" + sfx)(self.ctx, self._f(dfdy, "dfdy", n, dt), self._f(f, "f", n, dt), n))
 
     def ddxy(self, dfdx, dfdy, f):
-        self._chk(self.L.les_ddxy(self.ctx, _ptr(dfdx), _ptr(dfdy), _ptr(f), f.shape[0]))
+        n = self._nlev(f)
+        self._chk(self.L.les_ddxy(self.ctx, self._f(dfdx, "dfdx", n), self._f(dfdy, "dfdy", n), self._f(f, "f", n), n))
 
     def test_filter_init(self, alpha):
         G = torch.zeros((self.ny, self.lh), dtype=torch.float64, device="cuda")
@@ -113,29 +160,42 @@ class Les:
         return G
 
     def test_filter(self, plane, G):
-        self._chk(getattr(self.L, "les_test_filter" + self._sfx(plane))(self.ctx, _ptr(plane), _ptr(G)))
+        """G must have the plane's dtype (the FP32 variant reads float G)."""
+        sfx = self._sfx(plane)
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_test_filter" + sfx)(self.ctx, self._pl(plane, "plane", dt),
+                                                           _ptr(G, dt, self.ny * self.lh, "G")))
 
     def padd(self, u_big, u):
-        self._chk(self.L.les_padd(self.ctx, _ptr(u_big), _ptr(u)))
+        self._chk(self.L.les_padd(self.ctx, self._fbig(u_big, "u_big", 1), self._pl(u, "u")))
 
     def unpadd(self, cc, cc_big):
-        self._chk(self.L.les_unpadd(self.ctx, _ptr(cc), _ptr(cc_big)))
+        self._chk(self.L.les_unpadd(self.ctx, self._pl(cc, "cc"), self._fbig(cc_big, "cc_big", 1)))
 
     def dealias1(self, u, u_big):
-        self._chk(getattr(self.L, "les_dealias1" + self._sfx(u))(self.ctx, _ptr(u), _ptr(u_big), u.shape[0]))
+        sfx, n = self._sfx(u), self._nlev(u, "u")
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_dealias1" + sfx)(self.ctx, self._f(u, "u", n, dt),
+                                                        self._fbig(u_big, "u_big", n, dt), n))
 
     def dealias2(self, u, u_big):
-        self._chk(getattr(self.L, "les_dealias2" + self._sfx(u))(self.ctx, _ptr(u), _ptr(u_big), u.shape[0]))
+        sfx, n = self._sfx(u), self._nlev(u, "u")
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, "les_dealias2" + sfx)(self.ctx, self._f(u, "u", n, dt),
+                                                        self._fbig(u_big, "u_big", n, dt), n))
 
     def ddz_uv(self, dfdz, f):
-        self._chk(self.L.les_ddz_uv(self.ctx, _ptr(dfdz), _ptr(f)))
+        self._chk(self.L.les_ddz_uv(self.ctx, self._f(dfdz, "dfdz"), self._f(f, "f")))
 
     def ddz_w(self, dfdz, f):
-        self._chk(self.L.les_ddz_w(self.ctx, _ptr(dfdz), _ptr(f)))
+        self._chk(self.L.les_ddz_w(self.ctx, self._f(dfdz, "dfdz"), self._f(f, "f")))
 
     def convec(self, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz):
-        fn = getattr(self.L, "les_convec" + self._sfx(u))
-        self._chk(fn(self.ctx, *[_ptr(t) for t in (u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz)]))
+        sfx = self._sfx(u)
+        dt = F32 if sfx else F64
+        names = ("u", "v", "w", "dudy", "dudz", "dvdx", "dvdz", "dwdx", "dwdy", "cx", "cy", "cz")
+        args = (u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz)
+        self._chk(getattr(self.L, "les_convec" + sfx)(self.ctx, *[self._f(t, n, None, dt) for t, n in zip(args, names)]))
 
     def wallstress(self, u, v, dudz, dvdz, txz, tyz, psi_m=None, psi_m0=None, phi_m=None):
         n2 = (self.ny, self.nx)
@@ -144,38 +204,51 @@ class Les:
         psi_m0 = torch.zeros(n2, **dev) if psi_m0 is None else psi_m0
         phi_m = torch.ones(n2, **dev) if phi_m is None else phi_m
         ustar = torch.zeros(n2, **dev)
-        self._chk(self.L.les_wallstress(self.ctx, *[_ptr(t) for t in (u, v, dudz, dvdz, txz, tyz, psi_m, psi_m0,
-                                                                      phi_m, ustar)]))
+        f = [self._f(t, n) for t, n in ((u, "u"), (v, "v"), (dudz, "dudz"), (dvdz, "dvdz"), (txz, "txz"),
+                                         (tyz, "tyz"))]
+        p = [self._p2(t, n) for t, n in ((psi_m, "psi_m"), (psi_m0, "psi_m0"), (phi_m, "phi_m"), (ustar, "ustar"))]
+        self._chk(self.L.les_wallstress(self.ctx, *f, *p))
         return ustar
 
-    def sgs_fields(self, dtype=torch.float64):
-        d = {n: self.field(dtype=dtype) for n in SGS_NAMES}
+    def sgs_fields(self):
+        """The Lagrangian SGS arrays of sgsmodule.f90 for the routine-level
+        entry points, which are FP64 (the FP32-bulk state keeps its own)."""
+        d = {n: self.field() for n in SGS_NAMES}
         d["Beta_avg"] = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
         d["Betaclip_avg"] = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
         return d
 
-    @staticmethod
-    def _sgs_struct(d):
-        return SgsFields(*[C.c_void_p(d[n].data_ptr()) for n in SGS_NAMES],
-                         C.c_void_p(d["Beta_avg"].data_ptr()), C.c_void_p(d["Betaclip_avg"].data_ptr()))
+    def _sgs_struct(self, d):
+        return SgsFields(*[self._f(d[n], n) for n in SGS_NAMES],
+                         _ptr(d["Beta_avg"], F64, self.nz, "Beta_avg"),
+                         _ptr(d["Betaclip_avg"], F64, self.nz, "Betaclip_avg"))
+
+    def _arr(self, ts, names):
+        if len(ts) != len(names):
+            raise LesError(f"expected {len(names)} arrays ({', '.join(names)}), got {len(ts)}")
+        return (C.c_void_p * len(names))(*[self._f(t, n).value for t, n in zip(ts, names)])
+
+    GRAD_NAMES = ("dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz")
+    TAU_NAMES = ("txx", "txy", "txz", "tyy", "tyz", "tzz")
+    S_NAMES = ("S11", "S12", "S13", "S22", "S23", "S33")
 
     def calc_sij(self, grads):
         S = [self.field() for _ in range(6)]
-        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
-        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
-        self._chk(self.L.les_calc_sij(self.ctx, g, s))
+        self._chk(self.L.les_calc_sij(self.ctx, self._arr(grads, self.GRAD_NAMES), self._arr(S, self.S_NAMES)))
         return S
 
     def sgs_stag(self, grads, u, v, w, taus, sgs):
-        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
-        t = (C.c_void_p * 6)(*[x.data_ptr() for x in taus])
+        g = self._arr(grads, self.GRAD_NAMES)
+        t = self._arr(taus, self.TAU_NAMES)
         st = self._sgs_struct(sgs)
-        self._chk(self.L.les_sgs_stag(self.ctx, g, _ptr(u), _ptr(v), _ptr(w), t, C.byref(st)))
+        self._chk(self.L.les_sgs_stag(self.ctx, g, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"), t,
+                                      C.byref(st)))
 
     def lagrange_sdep(self, S, u, v, w, sgs):
-        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        s = self._arr(S, self.S_NAMES)
         st = self._sgs_struct(sgs)
-        self._chk(self.L.les_lagrange_sdep(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), C.byref(st)))
+        self._chk(self.L.les_lagrange_sdep(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                           C.byref(st)))
 
     def interpolag_sdep(self, sgs):
         st = self._sgs_struct(sgs)
@@ -188,10 +261,11 @@ class Les:
     def scaledep_dynamic(self, S, u, v, w):
         """scaledep_dynamic.f90 on the dummies S (S11..S33 in that order):
         returns the per-level Cs_opt2 and beta as (nz,) float64 GPU tensors."""
-        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
-        cs = torch.zeros(self.nz, dtype=torch.float64, device=u.device)
-        beta = torch.zeros(self.nz, dtype=torch.float64, device=u.device)
-        self._chk(self.L.les_scaledep_dynamic(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(cs), _ptr(beta)))
+        s = self._arr(S, self.S_NAMES)
+        cs = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        beta = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        self._chk(self.L.les_scaledep_dynamic(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                              _ptr(cs), _ptr(beta)))
         return cs, beta
 
     # ---- autotuner (launch shapes only; results do not depend on them) ----
@@ -257,26 +331,39 @@ class Les:
         self._chk(self.L.les_mlp_get_norm(self.ctx, *[x.ctypes.data_as(C.c_void_p) for x in a]))
         return a
 
+    @staticmethod
+    def _xy(X, T, what="T"):
+        P = X.shape[0] if isinstance(X, torch.Tensor) and X.dim() == 2 else -1
+        if P < 0:
+            raise LesError("X: expected a (P, 8) float32 tensor")
+        px = _ptr(X, F32, P * 8, "X")
+        pt = None if T is None else _ptr(T, F32, P * 2, what)
+        return P, px, pt
+
     def mlp_fit_norm(self, X, Y):
-        self._chk(self.L.les_mlp_fit_norm(self.ctx, _ptr(X), _ptr(Y), X.shape[0]))
+        P, px, py = self._xy(X, Y, "Y")
+        self._chk(self.L.les_mlp_fit_norm(self.ctx, px, py, P))
 
     def mlp_forward(self, X):
         """(P, 8) float32 raw features on the GPU -> (P, 2) (Cs_opt2, Beta)."""
-        Y = torch.empty((X.shape[0], 2), dtype=torch.float32, device=X.device)
-        self._chk(self.L.les_mlp_forward(self.ctx, _ptr(X), X.shape[0], _ptr(Y)))
+        P, px, _ = self._xy(X, None)
+        Y = torch.empty((P, 2), dtype=torch.float32, device=X.device)
+        self._chk(self.L.les_mlp_forward(self.ctx, px, P, _ptr(Y)))
         self.sync()
         return Y
 
     def mlp_grad(self, X, T):
+        P, px, pt = self._xy(X, T)
         g = torch.empty(self.L.les_mlp_nparam(), dtype=torch.float32, device=X.device)
         loss = torch.empty(1, dtype=torch.float32, device=X.device)
-        self._chk(self.L.les_mlp_grad(self.ctx, _ptr(X), _ptr(T), X.shape[0], _ptr(g), _ptr(loss)))
+        self._chk(self.L.les_mlp_grad(self.ctx, px, pt, P, _ptr(g), _ptr(loss)))
         self.sync()
         return float(loss.item()), g
 
     def mlp_train(self, X, T, batch=4096, steps=1000, lr=1e-3, seed=0):
+        P, px, pt = self._xy(X, T)
         hist = torch.empty(max(steps, 1), dtype=torch.float32, device=X.device)
-        self._chk(self.L.les_mlp_train(self.ctx, _ptr(X), _ptr(T), X.shape[0], int(batch), int(steps),
+        self._chk(self.L.les_mlp_train(self.ctx, px, pt, P, int(batch), int(steps),
                                        float(lr), int(seed), _ptr(hist)))
         self.sync()
         return hist[:steps]
@@ -308,73 +395,98 @@ class Les:
         self._chk(self.L.les_set_scalar_lag_init(self.ctx, int(glm_init), int(gqn_init)))
 
     def obukhov(self, u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, ustar_avg):
-        self._chk(self.L.les_obukhov(self.ctx, *[_ptr(t) for t in (u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0,
-                                                                   psi_h, psi_h0, phi_m, phi_h, ustar_avg)]))
+        f = [self._f(t, n) for t, n in ((u, "u"), (v, "v"), (theta, "theta"), (qmix, "qmix"))]
+        pl = [self._pl(sgs_t3, "sgs_t3"), self._pl(sgs_q3, "sgs_q3")]
+        p2 = [self._p2(t, n) for t, n in ((psi_m, "psi_m"), (psi_m0, "psi_m0"), (psi_h, "psi_h"), (psi_h0, "psi_h0"),
+                                          (phi_m, "phi_m"), (phi_h, "phi_h"), (ustar_avg, "ustar_avg"))]
+        self._chk(self.L.les_obukhov(self.ctx, *f, *pl, *p2))
 
     def calcbeta(self, theta, qmix, beta):
-        self._chk(self.L.les_calcbeta(self.ctx, _ptr(theta), _ptr(qmix), _ptr(beta)))
+        self._chk(self.L.les_calcbeta(self.ctx, self._f(theta, "theta"), self._f(qmix, "qmix"),
+                                      self._f(beta, "beta_scal")))
 
     def scalar_rhs_calc(self, which, dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg, phi_h, rhs, sgs_vert1):
-        self._chk(self.L.les_scalar_rhs_calc(self.ctx, int(which),
-                                             *[_ptr(t) for t in (dsdx, dsdy, dsdz, u, v, w, nu_t, pr, ustar_avg,
-                                                                 phi_h, rhs, sgs_vert1)]))
+        f = [self._f(t, n) for t, n in ((dsdx, "dsdx"), (dsdy, "dsdy"), (dsdz, "dsdz"), (u, "u"), (v, "v"),
+                                         (w, "w"), (nu_t, "Nu_t"), (pr, "Pr_"))]
+        self._chk(self.L.les_scalar_rhs_calc(self.ctx, int(which), *f, self._p2(ustar_avg, "ustar_avg"),
+                                             self._p2(phi_h, "phi_h"), self._f(rhs, "RHS"),
+                                             self._pl(sgs_vert1, "sgs_vert1")))
 
     def step_scalar(self, which, scalar, rhs, rhs_f):
-        self._chk(self.L.les_step_scalar(self.ctx, int(which), _ptr(scalar), _ptr(rhs), _ptr(rhs_f)))
+        self._chk(self.L.les_step_scalar(self.ctx, int(which), self._f(scalar, "scalar"), self._f(rhs, "RHS"),
+                                         self._f(rhs_f, "RHS_f")))
 
-    def scalar_sgs_fields(self, phi_h=None, psi_m=None, dtype=torch.float64):
-        d = {n: self.field(dtype=dtype) for n in SCALAR_SGS_NAMES}
+    def scalar_sgs_fields(self, phi_h=None, psi_m=None):
+        """The scalar SGS arrays of sgsmodule.f90 (FP64, as the routine-level
+        entry points take them)."""
+        d = {n: self.field() for n in SCALAR_SGS_NAMES}
         n2 = (self.ny, self.nx)
         d["phi_h"] = torch.ones(n2, dtype=torch.float64, device="cuda") if phi_h is None else phi_h
         d["psi_m"] = torch.zeros(n2, dtype=torch.float64, device="cuda") if psi_m is None else psi_m
         return d
 
-    @staticmethod
-    def _ssgs_struct(d):
-        return ScalarSgsFields(*[C.c_void_p(d[n].data_ptr()) for n in SCALAR_SGS_NAMES],
-                               C.c_void_p(d["phi_h"].data_ptr()), C.c_void_p(d["psi_m"].data_ptr()))
+    def _ssgs_struct(self, d):
+        return ScalarSgsFields(*[self._f(d[n], n) for n in SCALAR_SGS_NAMES],
+                               self._p2(d["phi_h"], "phi_h"), self._p2(d["psi_m"], "psi_m"))
 
     def interpolag_scalar_sdep(self, ssgs):
         st = self._ssgs_struct(ssgs)
         self._chk(self.L.les_interpolag_scalar_sdep(self.ctx, C.byref(st)))
 
     def lagrange_scalar_sdep(self, S, u, v, w, theta, sgs, ssgs):
-        s = (C.c_void_p * 6)(*[t.data_ptr() for t in S])
+        s = self._arr(S, self.S_NAMES)
         m, st = self._sgs_struct(sgs), self._ssgs_struct(ssgs)
-        self._chk(self.L.les_lagrange_scalar_sdep(self.ctx, s, _ptr(u), _ptr(v), _ptr(w), _ptr(theta), C.byref(m),
-                                                  C.byref(st)))
+        self._chk(self.L.les_lagrange_scalar_sdep(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                                  self._f(theta, "theta"), C.byref(m), C.byref(st)))
 
     def sgs_stag_scalar(self, grads, u, v, w, theta, taus, sgs, ssgs):
-        g = (C.c_void_p * 9)(*[t.data_ptr() for t in grads])
-        t = (C.c_void_p * 6)(*[x.data_ptr() for x in taus])
+        g = self._arr(grads, self.GRAD_NAMES)
+        t = self._arr(taus, self.TAU_NAMES)
         m, st = self._sgs_struct(sgs), self._ssgs_struct(ssgs)
-        self._chk(self.L.les_sgs_stag_scalar(self.ctx, g, _ptr(u), _ptr(v), _ptr(w), _ptr(theta), t, C.byref(m),
-                                             C.byref(st)))
+        self._chk(self.L.les_sgs_stag_scalar(self.ctx, g, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                             self._f(theta, "theta"), t, C.byref(m), C.byref(st)))
+
+    def _divstress(self, name, divt, tx, ty, tz):
+        sfx = self._sfx(divt)
+        dt = F32 if sfx else F64
+        self._chk(getattr(self.L, name + sfx)(self.ctx, self._f(divt, "divt", None, dt), self._f(tx, "tx", None, dt),
+                                              self._f(ty, "ty", None, dt), self._f(tz, "tz", None, dt)))
 
     def divstress_uv(self, divt, tx, ty, tz):
-        self._chk(getattr(self.L, "les_divstress_uv" + self._sfx(divt))(self.ctx, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)))
+        self._divstress("les_divstress_uv", divt, tx, ty, tz)
 
     def divstress_w(self, divt, tx, ty, tz):
-        self._chk(getattr(self.L, "les_divstress_w" + self._sfx(divt))(self.ctx, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)))
+        self._divstress("les_divstress_w", divt, tx, ty, tz)
 
     def press_stag_array(self, u, v, w, divtz, p, dpdx, dpdy):
-        fn = getattr(self.L, "les_press_stag_array" + self._sfx(u))
-        self._chk(fn(self.ctx, *[_ptr(t) for t in (u, v, w, divtz, p, dpdx, dpdy)]))
+        sfx = self._sfx(u)
+        dt = F32 if sfx else F64
+        f = [self._f(t, n, None, dt) for t, n in ((u, "u"), (v, "v"), (w, "w"), (divtz, "divtz"))]
+        self._chk(getattr(self.L, "les_press_stag_array" + sfx)(
+            self.ctx, *f, self._f(p, "p", self.nz + 1, dt), self._f(dpdx, "dpdx", None, dt),
+            self._f(dpdy, "dpdy", None, dt)))
 
     def tridag_array(self, a, b, c, r, u):
-        self._chk(self.L.les_tridag_array(self.ctx, *[_ptr(t) for t in (a, b, c, r, u)]))
+        """a, b, c real (nz+1, ny, lh); r, u complex as (nz+1, ny, 2 lh) float64."""
+        nr = (self.nz + 1) * self.ny * self.lh
+        co = [_ptr(t, F64, nr, n) for t, n in ((a, "a"), (b, "b"), (c, "c"))]
+        self._chk(self.L.les_tridag_array(self.ctx, *co, _ptr(r, F64, 2 * nr, "r"), _ptr(u, F64, 2 * nr, "u")))
 
     def project(self, u, v, w, dpdx, dpdy, dpdz):
-        self._chk(self.L.les_project(self.ctx, *[_ptr(t) for t in (u, v, w, dpdx, dpdy, dpdz)]))
+        self._chk(self.L.les_project(self.ctx, *[self._f(t, n) for t, n in ((u, "u"), (v, "v"), (w, "w"),
+                                                                            (dpdx, "dpdx"), (dpdy, "dpdy"),
+                                                                            (dpdz, "dpdz"))]))
 
     def rmsdiv(self, dudx, dvdy, dwdz):
         r = C.c_double()
-        self._chk(self.L.les_rmsdiv(self.ctx, _ptr(dudx), _ptr(dvdy), _ptr(dwdz), C.byref(r)))
+        self._chk(self.L.les_rmsdiv(self.ctx, self._f(dudx, "dudx"), self._f(dvdy, "dvdy"), self._f(dwdz, "dwdz"),
+                                    C.byref(r)))
         return r.value
 
     def timestep_conditions(self, u, v, w, nu_t):
         out = (C.c_double * 4)()
-        self._chk(self.L.les_timestep_conditions(self.ctx, _ptr(u), _ptr(v), _ptr(w), _ptr(nu_t), out))
+        self._chk(self.L.les_timestep_conditions(self.ctx, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                                 self._f(nu_t, "Nu_t"), out))
         return list(out)
 
 
@@ -406,7 +518,7 @@ class LesSim(Les):
         P = self.nx * self.ny * (self.nz - 1)
         X = torch.empty((P, 8), dtype=torch.float32, device="cuda")
         Y = torch.empty((P, 2), dtype=torch.float32, device="cuda") if labels else None
-        self._chk(self.L.les_mlp_harvest(self.ctx, _ptr(X), _ptr(Y)))
+        self._chk(self.L.les_mlp_harvest(self.ctx, _ptr(X, F32, P * 8, "X"), _ptr(Y, F32, P * 2, "Y")))
         self.sync()
         return X, Y
 

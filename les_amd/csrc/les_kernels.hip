@@ -814,14 +814,25 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
       for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 3) atomicAdd(red + 3 * k + threadIdx.x, sh[threadIdx.x][0]);
+  // one partial per block; k_lag_avg adds them in block order, so the plane
+  // sums have the same bits on every run (no atomics)
+  if (threadIdx.x < 3) red[(3 * (long)k + threadIdx.x) * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_avg, int nz) {
+// lagrange_Sdep.f90:425-426: Beta_avg = sum Cs2_2d / sum Cs_opt2,
+// Betaclip_avg = sum Cs2_4d / sum Cs2_2d over the nx*ny points of each level
+__global__ void k_lag_avg(const double* red, double* Beta_avg, double* Betaclip_avg, int nz, int nb) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nz) {
-    Beta_avg[k] = red[3 * k] / red[3 * k + 2];
-    Betaclip_avg[k] = red[3 * k + 1] / red[3 * k];
+    double s[3];
+    for (int j = 0; j < 3; ++j) {
+      const double* p = red + (3 * (long)k + j) * nb;
+      double a = 0.0;
+      for (int b = 0; b < nb; ++b) a += p[b];
+      s[j] = a;
+    }
+    Beta_avg[k] = s[0] / s[2];
+    Betaclip_avg[k] = s[1] / s[0];
   }
 }
 
@@ -865,14 +876,13 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   op_interpolag<T>(c, s);
   T* F = lag_filter_mom<T>(c, S, u, v, w);
   const size_t n3 = (size_t)g.ld * g.ny * nz;
-  double* red = (double*)scratch(c, S_RED, 3 * nz * sizeof(double));
-  op_fill<double>(c, red, 3 * (size_t)nz, 0.0);
+  const unsigned bpl = (unsigned)((g.ld * g.ny + 255) / 256 < 64 ? (g.ld * g.ny + 255) / 256 : 64);
+  double* red = (double*)scratch(c, S_RED, 3 * (size_t)nz * bpl * sizeof(double));
   const int init_lm = g.inilag && !c->flm_init && (c->jt == lag_count(c) || c->jt == 1);
   const int init_qn = g.inilag && !c->fqn_init && (c->jt == lag_count(c) || c->jt == 1);
   const double delta = g.delta;
   {
     ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
-    const unsigned bpl = (unsigned)((g.ld * g.ny + 255) / 256 < 64 ? (g.ld * g.ny + 255) / 256 : 64);
     hipLaunchKernelGGL((k_lag_epilogue<T>), dim3(bpl, nz), dim3(256), 0, c->stream, F, F + 21 * n3,
                        S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
                        (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
@@ -882,7 +892,7 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   LCHK(c, hipGetLastError());
   if (s.Beta_avg && s.Betaclip_avg) {
     hipLaunchKernelGGL(k_lag_avg, dim3((nz + 63) / 64), dim3(64), 0, c->stream, red, (double*)s.Beta_avg,
-                       (double*)s.Betaclip_avg, nz);
+                       (double*)s.Betaclip_avg, nz, (int)bpl);
     LCHK(c, hipGetLastError());
   }
   if (init_lm) c->flm_init = 1;

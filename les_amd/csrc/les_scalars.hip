@@ -17,39 +17,65 @@
 namespace les {
 
 // ------------------------------------------------------------------------
-// deterministic plane sums in double: blockIdx.y = level, NPART blocks per
-// level write partial sums, a second kernel adds them in a fixed order
+// deterministic, compensated plane sums in double: blockIdx.y = level, NPART
+// blocks per level write partial sums, a second kernel adds them in a fixed
+// order.  Every addition is an error-free TwoSum whose rounding error is
+// carried in a second word, so a plane sum is the exact sum rounded once
+// (up to a term of order n eps^2) whatever the order of the additions: plane
+// means of nearly constant fields (theta ~ 283 K, whose deviations calcbeta
+// scales by g z_i / u*^2 ~ 1e4) do not depend on the summation order.
 // ------------------------------------------------------------------------
 constexpr int NPART = 32;
+
+struct CSum {  // s + c is the running sum
+  double s, c;
+};
+__device__ __forceinline__ CSum csum_add(CSum a, double x) {
+  const double t = a.s + x, bb = t - a.s;
+  return CSum{t, a.c + ((a.s - (t - bb)) + (x - bb))};
+}
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+  const double t = a.s + b.s, bb = t - a.s;
+  return CSum{t, (a.c + b.c) + ((a.s - (t - bb)) + (b.s - bb))};
+}
 
 template <class Fn>
 __global__ void k_level_partial(Fn fn, int nx, int ny, double* part) {
   const int k = blockIdx.y, n = nx * ny;
-  double a = 0.0;
+  CSum a{0.0, 0.0};
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const int y = j / nx, x = j - y * nx;
-    a += fn(k, y, x);
+    a = csum_add(a, fn(k, y, x));
   }
-  __shared__ double sh[256];
-  sh[threadIdx.x] = a;
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = a.s;
+  sh[1][threadIdx.x] = a.c;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    if (threadIdx.x < s) {
+      const CSum m = csum_merge(CSum{sh[0][threadIdx.x], sh[1][threadIdx.x]},
+                                CSum{sh[0][threadIdx.x + s], sh[1][threadIdx.x + s]});
+      sh[0][threadIdx.x] = m.s;
+      sh[1][threadIdx.x] = m.c;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) {
+    part[2 * (k * gridDim.x + blockIdx.x)] = sh[0][0];
+    part[2 * (k * gridDim.x + blockIdx.x) + 1] = sh[1][0];
+  }
 }
 __global__ void k_level_final(const double* part, int nparts, int nlev, double* out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nlev) return;
-  double a = 0.0;
-  for (int p = 0; p < nparts; ++p) a += part[k * nparts + p];
-  out[k] = a;
+  CSum a{0.0, 0.0};
+  for (int p = 0; p < nparts; ++p) a = csum_merge(a, CSum{part[2 * (k * nparts + p)], part[2 * (k * nparts + p) + 1]});
+  out[k] = a.s + a.c;
 }
 template <class Fn>
 static void level_sums(les_ctx* c, const Fn& fn, int nlev, double* out, double bytes) {
   const Grid& g = c->g;
-  double* part = (double*)scratch(c, S_RED, (size_t)NPART * nlev * sizeof(double));
+  double* part = (double*)scratch(c, S_RED, (size_t)2 * NPART * nlev * sizeof(double));
   {
     ProfScope _ps(c, "level_sum", bytes);
     hipLaunchKernelGGL((k_level_partial<Fn>), dim3(NPART, nlev), dim3(256), 0, c->stream, fn, g.nx, g.ny, part);

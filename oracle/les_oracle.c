@@ -27,6 +27,21 @@ static const double VONK = 0.41;   /* param.f90:98 */
 static const double TADV1 = 1.5;   /* param.f90:151 */
 static const double TADV2 = -0.5;  /* 1 - tadv1 */
 
+/* Planted mistakes, for the tests that prove the closed-form pins catch them
+   (tests/test_oracle_mutations.py builds the oracle with -DOR_MUTANT=n into a
+   temporary directory).  The oracle's own build leaves OR_MUTANT at 0.
+     1  lagrange_Sdep: the 2-Delta / 4-Delta factors tf1^2 = 4 and tf2^2 = 16 swapped
+     2  lagrange_Sdep: Beta = Cs2_2d / Cs2_4d (inverted)
+     3  step: AB2 weights tadv1 / tadv2 swapped
+     4  step: mean pressure forcing applied to v instead of u
+     5  press_stag_array: sign of the divtz boundary rows flipped
+     6  lagrange_scalar_Sdep: the factors 4 and 16 swapped
+     7  lagrange_scalar_Sdep: Beta inverted
+     8  step: no Euler first step when initu is false */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+
 typedef struct { double re, im; } cpx;
 
 /* ------------------------------------------------------------------ */
@@ -680,6 +695,20 @@ void or_interpolag_scalar_sdep(or_ctx *c, or_scalar_sgs *g) {
 /* lagrange_Sdep.f90:4-478 (non-MPI, no buildings, no inflow)           */
 /* ------------------------------------------------------------------ */
 static inline double fq(double x) { return (double)(float)x; }   /* Fortran real(x) */
+
+/* Plane sums (Fortran sum() over a level, whose order the compiler chooses):
+   compensated with an error-free TwoSum per addition, so the result is the
+   exact sum rounded once up to a term of order n eps^2, independent of the
+   order.  The plane means of theta (~283 K) that calcbeta subtracts and
+   scales by g z_i/u*^2 (scalars_module.f90:199-220) would otherwise carry the
+   rounding of a sequential sum. */
+typedef struct { double s, c; } csum_t;
+static inline void csum_add(csum_t *a, double x) {
+  const double t = a->s + x, bb = t - a->s;
+  a->c += (a->s - (t - bb)) + (x - bb);
+  a->s = t;
+}
+static inline double csum_val(const csum_t *a) { return a->s + a->c; }
 static const float F1EM24 = 1e-24f;                               /* 1E-24 literal */
 
 void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
@@ -760,8 +789,8 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
                                 2.0 * (T[SH12][i] * T[SH12][i] + T[SH13][i] * T[SH13][i] + T[SH23][i] * T[SH23][i])));
         double M[6], N[6], Lv[6], Qv[6];
         for (int q = 0; q < 6; q++) {
-          M[q] = cst * (T[SSB11 + q][i] - tf1_2 * Sb * T[SB11 + q][i]);
-          N[q] = cst * (T[SSH11 + q][i] - tf2_2 * Sh * T[SH11 + q][i]);
+          M[q] = cst * (T[SSB11 + q][i] - (OR_MUTANT == 1 ? tf2_2 : tf1_2) * Sb * T[SB11 + q][i]);
+          N[q] = cst * (T[SSH11 + q][i] - (OR_MUTANT == 1 ? tf1_2 : tf2_2) * Sh * T[SH11 + q][i]);
         }
         Lv[0] = T[L11][i]; Lv[1] = T[L12][i]; Lv[2] = T[L13][i]; Lv[3] = T[L22][i]; Lv[4] = T[L23][i]; Lv[5] = T[L33][i];
         Qv[0] = T[Q11][i]; Qv[1] = T[Q12][i]; Qv[2] = T[Q13][i]; Qv[3] = T[Q22][i]; Qv[4] = T[Q23][i]; Qv[5] = T[Q33][i];
@@ -802,7 +831,7 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
         double cs4d = FQN[i] / FNN[i];
         if (x >= ld - 2) cs4d = e24;
         cs4d = fmax(e24, fq(cs4d));
-        double beta = pow(cs4d / cs2d, expo);
+        double beta = OR_MUTANT == 2 ? pow(cs2d / cs4d, expo) : pow(cs4d / cs2d, expo);
         if (k == nz - 1) beta = 1.0;
         s->Beta[o + i] = beta;
         double bclip = fmax(fq(beta), fq(1.0 / (tf1 * tf2)));
@@ -1044,7 +1073,8 @@ void or_press_stag_array(or_ctx *c, const double *u, const double *v, const doub
     CRE(Hy + pl * (nz - 1), i) = BOGUS; CIM(Hy + pl * (nz - 1), i) = 0.0;
     CRE(Hz + pl * (nz - 1), i) = 0.0; CIM(Hz + pl * (nz - 1), i) = 0.0;
   }
-  for (size_t i = 0; i < pl; i++) { botw[i] = cst * divtz[i]; topw[i] = cst * divtz[pl * (nz - 1) + i]; }
+  const double wsgn = OR_MUTANT == 5 ? -1.0 : 1.0;
+  for (size_t i = 0; i < pl; i++) { botw[i] = wsgn * cst * divtz[i]; topw[i] = wsgn * cst * divtz[pl * (nz - 1) + i]; }
   rfft2_fwd(&c->sm, botw); rfft2_fwd(&c->sm, topw);
   for (int k = 0; k < nz - 1; k++) { kill_oddballs(c, Hx + pl * k); kill_oddballs(c, Hy + pl * k); kill_oddballs(c, Hz + pl * k); }
   kill_oddballs(c, topw); kill_oddballs(c, botw);
@@ -1339,16 +1369,18 @@ void or_step(or_ctx *c, or_state *s) {
   if (scal && !c->S.passive)           /* main.f90:312-317 */
     for (size_t i = 0; i < nv; i++) s->RHSz[i] = s->RHSz[i] + s->beta_scal[i];
   const double force = c->P.force;
-  if (s->jt == 1 && !c->P.initu) {
+  if (s->jt == 1 && !c->P.initu && OR_MUTANT != 8) {
     memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
     memcpy(s->RHSy_f, s->RHSy, n3 * sizeof(double));
     memcpy(s->RHSz_f, s->RHSz, n3 * sizeof(double));
   }
   const double dt = c->P.dt;
   for (size_t i = 0; i < nv; i++) {
-    s->u[i] = s->u[i] + dt * (TADV1 * s->RHSx[i] + TADV2 * s->RHSx_f[i] + force);
-    s->v[i] = s->v[i] + dt * (TADV1 * s->RHSy[i] + TADV2 * s->RHSy_f[i]);
-    s->w[i] = s->w[i] + dt * (TADV1 * s->RHSz[i] + TADV2 * s->RHSz_f[i]);
+    const double ta1 = OR_MUTANT == 3 ? TADV2 : TADV1, ta2 = OR_MUTANT == 3 ? TADV1 : TADV2;
+    const double fu = OR_MUTANT == 4 ? 0.0 : force, fv = OR_MUTANT == 4 ? force : 0.0;
+    s->u[i] = s->u[i] + dt * (ta1 * s->RHSx[i] + ta2 * s->RHSx_f[i] + fu);
+    s->v[i] = s->v[i] + dt * (ta1 * s->RHSy[i] + ta2 * s->RHSy_f[i] + fv);
+    s->w[i] = s->w[i] + dt * (ta1 * s->RHSz[i] + ta2 * s->RHSz_f[i]);
   }
   for (size_t i = 0; i < pl; i++) { s->u[nv + i] = BOGUS; s->v[nv + i] = BOGUS; s->w[nv + i] = BOGUS; }
   or_press_stag_array(c, s->u, s->v, s->w, s->divtz, s->p, s->dpdx, s->dpdy);
@@ -1408,10 +1440,10 @@ void or_obukhov(or_ctx *c, const double *u, const double *v, const double *theta
         }
   }
   const double g_ = (S->g * S->z_i) / (S->u_star * S->u_star);                 /* :541 */
-  double tsum = 0.0;
+  csum_t tsum = { 0.0, 0.0 };
   for (int y = 0; y < ny; y++)
-    for (int x = 0; x < nx; x++) tsum += theta[x + (size_t)ld * y];
-  const double theta_ = tsum / (double)(float)(nx * ny);                        /* :556 */
+    for (int x = 0; x < nx; x++) csum_add(&tsum, theta[x + (size_t)ld * y]);
+  const double theta_ = csum_val(&tsum) / (double)(float)(nx * ny);             /* :556 */
   double *u1 = (double *)malloc(sizeof(double) * pl), *v1 = (double *)malloc(sizeof(double) * pl);
   memcpy(u1, u, sizeof(double) * pl); memcpy(v1, v, sizeof(double) * pl);
   or_test_filter(c, u1, c->Gtt);                                                /* :560-563 */
@@ -1469,13 +1501,13 @@ void or_calcbeta(or_ctx *c, const double *theta, const double *qmix, double *bet
   memset(beta, 0, sizeof(double) * pl * nz);
   double *bar = (double *)calloc(nz, sizeof(double));
   for (int k = 0; k < nz; k++) {
-    double a = 0.0;
+    csum_t a = { 0.0, 0.0 };
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < nx; x++) {
         const size_t i = pl * k + x + (size_t)ld * y;
-        a = a + theta[i] * (1.0 + 0.61 * 0.001 * qmix[i] * S->q_scale);
+        csum_add(&a, theta[i] * (1.0 + 0.61 * 0.001 * qmix[i] * S->q_scale));
       }
-    bar[k] = a / (double)(nx * ny);
+    bar[k] = csum_val(&a) / (double)(nx * ny);
   }
   for (int k = 1; k < nz - 1; k++)
     for (int y = 0; y < ny; y++)
@@ -1650,13 +1682,15 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
     double slog = 0.0;
     for (int i = 0; i < nx * ny; i++) slog += log(c->P.z0);
     const double zo_avg = exp(slog / (double)(float)(nx * ny));
-    double su = 0.0, spsi = 0.0, swt = 0.0;
+    csum_t su = { 0.0, 0.0 }, spsi = { 0.0, 0.0 };
+    double swt = 0.0;
     for (int y = 0; y < ny; y++)
-      for (int x = 0; x < nx; x++) su += sqrt(u[x + (size_t)ld * y] * u[x + (size_t)ld * y] + v[x + (size_t)ld * y] * v[x + (size_t)ld * y]);
-    for (int j = 0; j < nx * ny; j++) spsi += g->psi_m[j];
+      for (int x = 0; x < nx; x++)
+        csum_add(&su, sqrt(u[x + (size_t)ld * y] * u[x + (size_t)ld * y] + v[x + (size_t)ld * y] * v[x + (size_t)ld * y]));
+    for (int j = 0; j < nx * ny; j++) csum_add(&spsi, g->psi_m[j]);
     for (int j = 0; j < nx * ny; j++) swt += c->wt_s[j];
-    const double ustar_ = su / (double)(float)(nx * ny) * VONK /
-                          (log(0.5 * c->dz / zo_avg) - spsi / (double)(float)(nx * ny));
+    const double ustar_ = csum_val(&su) / (double)(float)(nx * ny) * VONK /
+                          (log(0.5 * c->dz / zo_avg) - csum_val(&spsi) / (double)(float)(nx * ny));
     const double sflux = (swt / (double)(nx * ny)) / P->T_scale / P->u_star;
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < nx; x++)
@@ -1731,8 +1765,8 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
                                       2.0 * (T[SH12][i] * T[SH12][i] + T[SH13][i] * T[SH13][i] + T[SH23][i] * T[SH23][i])));
         double M[3], N[3];
         for (int q = 0; q < 3; q++) {                                             /* :268-274 */
-          M[q] = cst * (T[SXB + q][i] - 4.0 * Sb * T[DXB + q][i]);
-          N[q] = cst * (T[SXH + q][i] - 16.0 * Sh * T[DXH + q][i]);
+          M[q] = cst * (T[SXB + q][i] - (OR_MUTANT == 6 ? 16.0 : 4.0) * Sb * T[DXB + q][i]);
+          N[q] = cst * (T[SXH + q][i] - (OR_MUTANT == 6 ? 4.0 : 16.0) * Sh * T[DXH + q][i]);
         }
         const double LM = T[L1][i] * M[0] + T[L2][i] * M[1] + T[L3][i] * M[2];  /* :284-287 */
         const double MM = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
@@ -1768,7 +1802,7 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
         double c4 = GQN[i] / GNN[i];
         if (x >= ld - 2) c4 = e24;
         c4 = fmax(e24, fq(c4));
-        double beta = pow(c4 / c2, expo);                                          /* :370 */
+        double beta = OR_MUTANT == 7 ? pow(c2 / c4, expo) : pow(c4 / c2, expo);  /* :370 */
         if (k == nz - 1) beta = 1.0;                                              /* :379-382 */
         m->Beta[o + i] = beta;
         const double bclip = fmax(fq(beta), fq(1.0 / (tf1 * tf2)));               /* :384-389 */

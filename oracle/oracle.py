@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# LES_ORACLE_LIB: another build of the same source (the mutation tests load
+# oracles with planted mistakes this way)
+_LIB = os.environ.get("LES_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 D = C.POINTER(C.c_double)
@@ -389,6 +391,13 @@ class OracleSim:
     @property
     def jt(self):
         return self.s.contents.jt
+
+    def set_time(self, jt, jt_total):
+        """Step counters of the state (param.f90:147-148); the next step is
+        jt + 1, jt_total + 1."""
+        st = self.s.contents
+        st.jt = int(jt)
+        st.jt_total = int(jt_total)
 
     def step(self, n=1):
         for _ in range(n):

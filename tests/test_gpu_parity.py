@@ -4,7 +4,7 @@ oracle on the same seeded inputs.  FP64 bar: relative max error <= 1e-12
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import FLIP_ULPS, flip_check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -96,6 +96,113 @@ def test_test_filter(torch_mod, grid):
         pg = G(pl)
         g.test_filter(pg, Gd)
         assert relerr(interior(H(pg), o.nx), interior(po, o.nx)) < TOL
+
+
+def f32(a):
+    """FP32-representable copy of an FP64 oracle input."""
+    return a.astype(np.float32).astype(np.float64)
+
+
+def G32(t):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(t, dtype=np.float32)).cuda()
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_fp32_routines(torch_mod, grid):
+    """The _f32 entry points (FP32-bulk mode) against the FP64 oracle on
+    FP32-representable inputs: single-precision transforms of these lengths
+    keep the relative max error near 1e-6, the bar is 1e-5."""
+    print("fp32 routines", grid, flush=True)
+    o, g = mk(*grid)
+    tol = 1e-5
+    nx = o.nx
+    f = f32(rnd(o, 201))
+    fo, dxo, dyo = f.copy(), o.field(), o.field()
+    o.filt_da(fo, dxo, dyo)
+    fg, dxg, dyg = G32(f), G32(o.field()), G32(o.field())
+    g.filt_da(fg, dxg, dyg)
+    for nm, a, b in (("f", fg, fo), ("dfdx", dxg, dxo), ("dfdy", dyg, dyo)):
+        assert relerr(interior(H(a), nx), interior(b, nx)) < tol, ("filt_da", nm)
+    for nm in ("ddx", "ddy"):
+        ao = o.field()
+        getattr(o, nm)(ao, f)
+        ag = G32(o.field())
+        getattr(g, nm)(ag, G32(f))
+        assert relerr(interior(H(ag), nx), interior(ao, nx)) < tol, nm
+    t = o.tables()
+    pl = f[0].copy()
+    o.test_filter(pl, t["G_test"])
+    pg = G32(f[0])
+    g.test_filter(pg, G32(t["G_test"]))
+    assert relerr(interior(H(pg), nx), interior(pl, nx)) < tol, "test_filter"
+    bo = o.field(big=True)
+    o.dealias1(f, bo)
+    bg = G32(o.field(big=True))
+    g.dealias1(G32(f), bg)
+    assert relerr(H(bg)[..., :o.nx2], bo[..., :o.nx2]) < tol, "dealias1"
+    b2 = f32(rnd(o, 202, big=True))
+    uo = o.field()
+    o.dealias2(uo, b2.copy())
+    ug = G32(o.field())
+    g.dealias2(ug, G32(b2))
+    assert relerr(interior(H(ug), nx), interior(uo, nx)) < tol, "dealias2"
+    ins = [f32(rnd(o, 210 + i)) for i in range(9)]
+    outs_o = [o.field() for _ in range(3)]
+    o.convec(*ins, *outs_o)
+    outs_g = [G32(o.field()) for _ in range(3)]
+    g.convec(*[G32(a) for a in ins], *outs_g)
+    for a, b in zip(outs_g, outs_o):
+        assert relerr(interior(H(a), nx)[:-1], interior(b, nx)[:-1]) < tol, "convec"
+    tx, ty, tz = f32(rnd(o, 220)), f32(rnd(o, 221)), f32(rnd(o, 222))
+    for name in ("divstress_uv", "divstress_w"):
+        do = o.field()
+        getattr(o, name)(do, tx, ty, tz)
+        dg = G32(o.field())
+        getattr(g, name)(dg, G32(tx), G32(ty), G32(tz))
+        nv = o.nz - 1 if name == "divstress_uv" else o.nz
+        assert relerr(H(dg)[:nv, :, :nx], do[:nv, :, :nx]) < tol, name
+    u, v, w, dz = (f32(rnd(o, 230 + i)) for i in range(4))
+    po, pxo, pyo = o.field(o.nz + 1), o.field(), o.field()
+    o.press_stag_array(u, v, w, dz, po, pxo, pyo)
+    pg, pxg, pyg = G32(o.field(o.nz + 1)), G32(o.field()), G32(o.field())
+    g.press_stag_array(G32(u), G32(v), G32(w), G32(dz), pg, pxg, pyg)
+    for nm, a, b in (("p", pg, po), ("dpdx", pxg, pxo), ("dpdy", pyg, pyo)):
+        assert relerr(interior(H(a), nx)[:-1], interior(b, nx)[:-1]) < tol, ("press_stag_array", nm)
+
+
+def test_argument_checks(torch_mod):
+    """The Python mirror refuses arrays the C ABI would index out of bounds:
+    the wrong precision for an FP64-only routine, too few levels, a host
+    tensor, a non-contiguous view, a float G for the FP64 filter."""
+    import torch
+    from les_amd.api import Les, LesError
+    g = Les(32, 32, 9)
+    ok = g.field()
+    with pytest.raises(LesError):
+        g.ddz_uv(g.field(dtype=torch.float32), g.field(dtype=torch.float32))   # FP64 only
+    with pytest.raises(LesError):
+        g.ddz_w(ok, g.field(nlev=5))                                            # nz levels
+    with pytest.raises(LesError):
+        g.filt_da(ok, ok.cpu(), ok)                                             # host tensor
+    with pytest.raises(LesError):
+        g.filt_da(ok, g.field(dtype=torch.float32), ok)                         # mixed precision
+    with pytest.raises(LesError):
+        g.ddx(g.field()[:, :, ::2], ok)                                         # not contiguous
+    with pytest.raises(LesError):
+        g.test_filter(ok[0].clone(), g.test_filter_init(2.0).float())           # G of another dtype
+    with pytest.raises(LesError):
+        g.press_stag_array(ok, ok, ok, ok, g.field(), ok, ok)                   # p needs nz+1 levels
+    with pytest.raises(LesError):
+        g.calc_sij([ok] * 8)                                                    # nine gradients
+    sg = g.sgs_fields()
+    sg["F_LM"] = g.field(dtype=torch.float32)
+    with pytest.raises(LesError):
+        g.interpolag_sdep(sg)
+    n = (g.nz + 1) * g.ny * g.lh
+    with pytest.raises(LesError):
+        a = torch.zeros(n, dtype=torch.float64, device="cuda")
+        g.tridag_array(a, a, a, a, a)                                           # r, u are complex
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -241,17 +348,17 @@ def test_lagrange_sdep(torch_mod, grid, init):
     g.set_time(5, 600, flm_init=not init, fqn_init=not init)
     o.lagrange_sdep(S, u, v, w, st)
     g.lagrange_sdep([G(a) for a in S], G(u), G(v), G(w), gd)
-    for n in ("F_LM", "F_MM", "F_QN", "F_NN", "TnLMMM", "TnQNNN", "Beta", "Cs_opt2",
-              "L11t", "L22t", "L33t", "Q11t", "Q22t", "Q33t"):
-        a, b = interior(H(gd[n]), o.nx), interior(arr[n], o.nx)
-        # FP32 quantisation points (lagrange_Sdep.f90:321-421) can flip one
-        # float ulp where the FP64 inputs differ in the last bits
-        bad = np.abs(a - b) > 1e-12 * np.max(np.abs(b)) + 2e-7 * np.abs(b)
-        assert bad.mean() < 1e-3, (n, bad.sum())
+    # the Leonard-type stresses have no single-precision rounding point
+    for n in ("L11t", "L22t", "L33t", "Q11t", "Q22t", "Q33t"):
+        assert relerr(interior(H(gd[n]), o.nx), interior(arr[n], o.nx)) < TOL, n
+    # FP32 quantisation points (lagrange_Sdep.f90:321-421) can flip one float
+    # ulp where the FP64 inputs of a rounding differ in their last bits
+    for n in ("F_LM", "F_MM", "F_QN", "F_NN", "TnLMMM", "TnQNNN", "Beta", "Cs_opt2"):
+        flip_check(interior(H(gd[n]), o.nx), interior(arr[n], o.nx), n, max_ulps=FLIP_ULPS[n])
     for n in ("Beta", "Cs_opt2"):
         assert np.all(H(gd[n])[..., o.nx:] == arr[n][..., o.nx:]), n
-    assert relerr(H(gd["Beta_avg"]), arr["Beta_avg"]) < 1e-6
-    assert relerr(H(gd["Betaclip_avg"]), arr["Betaclip_avg"]) < 1e-6
+    assert relerr(H(gd["Beta_avg"]), arr["Beta_avg"]) < 1e-10
+    assert relerr(H(gd["Betaclip_avg"]), arr["Betaclip_avg"]) < 1e-10
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -268,13 +375,21 @@ def test_sgs_stag(torch_mod, grid, jt_total):
     g.set_time(5, jt_total, flm_init=True, fqn_init=True)
     o.sgs_stag(grads, u, v, w, taus_o, st)
     g.sgs_stag([G(a) for a in grads], G(u), G(v), G(w), taus_g, gd)
+    # before jt_total = 500 Nu_t takes the constant Cs^2 (sgs_stag.f90:166-173):
+    # no single-precision rounding point reaches tau, so the bar is strict.
+    # After it, Nu_t = Cs_opt2 |S| Delta^2 inherits the float flips of Cs_opt2.
     for n, a, b in zip(("txx", "txy", "txz", "tyy", "tyz", "tzz"), taus_g, taus_o):
         x, y = H(a), b
-        bad = np.abs(x - y) > 1e-12 * np.max(np.abs(y[:-1, :, :o.nx])) + 2e-7 * np.abs(y)
-        assert bad[..., :o.nx].mean() < 1e-3, n
+        if jt_total < 500:
+            assert relerr(x[:-1, :, :o.nx], y[:-1, :, :o.nx]) < TOL, n
+        else:
+            flip_check(x[:-1, :, :o.nx], y[:-1, :, :o.nx], n, max_ulps=FLIP_ULPS["tau"])
         assert np.array_equal(x[-1], y[-1]), n
     a, b = interior(H(gd["Nu_t"]), o.nx), interior(arr["Nu_t"], o.nx)
-    assert (np.abs(a - b) > 1e-12 * np.max(np.abs(b)) + 2e-7 * np.abs(b)).mean() < 1e-3
+    if jt_total < 500:
+        assert relerr(a, b) < TOL
+    else:
+        flip_check(a, b, "Nu_t", max_ulps=FLIP_ULPS["tau"])
 
 
 @pytest.mark.parametrize("grid", GRIDS)
@@ -354,8 +469,51 @@ def test_full_step_vs_oracle(torch_mod, prec):
         assert relerr(a[:nl, :, :nx], b[:nl, :, :nx]) < tol10, name
     for name in ("F_LM", "F_MM", "Cs_opt2", "Beta"):
         a, b = sg.get(name)[..., :nx], so[name][..., :nx]
-        bad = np.abs(a - b) > (1e-9 if prec == "fp64" else 1e-3) * np.max(np.abs(b)) + 2e-7 * np.abs(b)
-        assert bad.mean() < 1e-3, name
+        if prec == "fp64":
+            flip_check(a, b, name, rel=1e-9, max_ulps=FLIP_ULPS[name], l2=1e-9)
+        else:
+            assert (np.abs(a - b) > 1e-3 * np.max(np.abs(b))).mean() < 1e-3, name
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_full_step_crossing_dynamic_switch(torch_mod, prec):
+    """From jt_total = 490 the run crosses jt_total = 500, where Nu_t stops
+    using the constant Cs^2 and takes the dynamic Cs_opt2 of lagrange_Sdep
+    (sgs_stag.f90:166-173): Lagrangian initialisation at jt = 5
+    (jt_total 495), the first dynamic Nu_t at jt = 10 (jt_total 500), another
+    update at jt = 15.  Whole-step state against the oracle after 10 and 16
+    steps, so the float flips of Cs_opt2 now reach the velocity field."""
+    print("full step across jt_total = 500", prec, flush=True)
+    from oracle.oracle import Oracle, OracleSim
+    from les_amd.api import LesSim
+    nx, ny, nz = 32, 32, 17
+    o = Oracle(nx, ny, nz)
+    so = OracleSim(o, seed=7)
+    so.set_time(0, 490)
+    sg = LesSim(nx, ny, nz, precision=prec)
+    sg.init_ic(seed=7)
+    sg.set_time(0, 490)
+    for n, tol in ((10, 1e-11 if prec == "fp64" else 1e-4), (6, 1e-10 if prec == "fp64" else 2e-4)):
+        so.step(n)
+        sg.step(n)
+        assert sg.jt == so.jt
+        for name in ("u", "v", "w", "p", "RHSx", "txz"):
+            a, b = sg.get(name), so[name]
+            nl = b.shape[0] - 1
+            err = relerr(a[:nl, :, :nx], b[:nl, :, :nx])
+            print(f"  step {so.jt} {name}: {err:.2e}", flush=True)
+            # FP32-bulk: RHS and stress are differences of O(u^2) terms
+            t = tol if prec == "fp64" or name not in ("RHSx", "txz") else 5 * tol
+            assert err < t, (name, err)
+        nu_a, nu_b = sg.get("Nu_t")[:nz - 1, :, :nx], so["Nu_t"][:nz - 1, :, :nx]
+        cs_a, cs_b = sg.get("Cs_opt2")[:nz - 1, :, :nx], so["Cs_opt2"][:nz - 1, :, :nx]
+        if prec == "fp64":
+            flip_check(cs_a, cs_b, "Cs_opt2", rel=1e-10, max_ulps=FLIP_ULPS["Cs_opt2"], l2=1e-9)
+            flip_check(nu_a, nu_b, "Nu_t", rel=1e-10, max_ulps=FLIP_ULPS["tau"], l2=1e-9)
+        else:
+            assert relerr(nu_a, nu_b) < 5e-3 and relerr(cs_a, cs_b) < 5e-3
+    # the dynamic coefficient is in use: Nu_t / (|S| Delta^2) is no longer the constant
+    assert not np.allclose(cs_b, float(np.float32(0.16)) ** 2)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
