@@ -98,6 +98,9 @@ struct SumThetaV {  // theta (1 + 0.61e-3 q q_scale) per level, scalars_module.f
   int ld, ny;
   double qs, tref, qref;
   __device__ double operator()(int k, int y, int x) const {
+    // no fused multiply-add: calcbeta scales theta_v - mean(theta_v) by
+    // g z_i / u*^2 ~ 1e4, so one rounding of theta_v shows at 1e-12 of beta
+#pragma clang fp contract(off)
     const long i = ((long)k * ny + y) * ld + x;
     return ((double)th[i] + tref) * (1.0 + 0.61 * 0.001 * ((double)q[i] + qref) * qs);
   }
@@ -209,6 +212,7 @@ void op_obukhov(les_ctx* c, const T* u, const T* v, const T* theta, double* sgs_
 template <typename T>
 __global__ void k_beta(const T* __restrict__ th, const T* __restrict__ q, const double* __restrict__ sums, T* beta,
                        int nx, int ld, int ny, int nz, double g_hat, double qs, double tref, double qref) {
+#pragma clang fp contract(off)  // the rounding of theta_v as in SumThetaV
   const long pl = (long)ld * ny, n = pl * nz;
   const double nxy = (double)(nx * ny);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {

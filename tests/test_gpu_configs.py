@@ -28,7 +28,7 @@ count as the (chaotic) flows separate from round-off.
 import numpy as np
 import pytest
 
-from conftest import FLIP_ULPS, flip_check, relerr
+from conftest import FLIP_ULPS, MULTI_STEP_L2, buoyancy_tol, flip_check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +44,8 @@ TUNE_256 = dict(col_tx_fp32=8, col_tx_fp64=8, tri_mode=1, tri_block=64, col_plan
 # streaming products in FP64 (the largest LDS ring) and the two-ended sweep
 TUNE_256_ALT = dict(col_tx_fp32=16, col_tx_fp64=8, tri_mode=4, tri_block=64, col_plan_fp32=0, col_plan_fp64=2,
                     pad_mode_fp32=0, pad_mode_fp64=1)
+
+
 
 
 def _field_errs(sg, so, names, nx, label, tol, lev=None):
@@ -187,7 +189,7 @@ def test_neutral128_step_and_dynamic_update(torch_mod, prec):
     for name in ("F_LM", "F_MM", "F_QN", "F_NN", "Cs_opt2", "Beta"):
         a, b = sg.get(name)[:64, :, :128], so[name][:64, :, :128]
         if prec == "fp64":
-            flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS[name], l2=1e-9)
+            flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS[name], l2=MULTI_STEP_L2)
         else:
             e = rel_l2(a, b)
             print(f"  {name} rel L2 {e:.1e}", flush=True)
@@ -263,12 +265,18 @@ def conv256_oracle(torch_mod):
 def test_convective256_full_size_step(conv256_oracle, prec, tune):
     """One step of the benchmarked workload at full size (256x256x128) with
     the launch shapes the bench runs and the alternatives the tuner weighs,
-    against the oracle: FP64 at 1e-12, FP32-bulk at 5e-5."""
+    against the oracle: FP64 at 1e-12, FP32-bulk at 5e-5.  The buoyancy
+    beta_scal amplifies the rounding of theta by g z_i / u*^2 ~ 1e4; its
+    FP64 bound is four ulps of theta_v carried through that factor."""
     print("convective 256^2x128", prec, tune, flush=True)
     sg = _conv_gpu(256, 256, 129, prec, TUNE_256 if tune == "bench" else TUNE_256_ALT)
     sg.step(1)
     so = _Snap(conv256_oracle)
-    tol = 1e-12 if prec == "fp64" else 5e-5
+    tol = {n: 1e-12 if prec == "fp64" else 5e-5 for n in CONV_NAMES}
+    if prec == "fp64":
+        g_hat = CONV_SCALARS["g"] * CONV_SCALARS["z_i"] / CONV_SCALARS["u_star"] ** 2
+        tol["beta_scal"] = buoyancy_tol(so["beta_scal"][:128, :, :256], g_hat)
+        print(f"  beta_scal bound {tol['beta_scal']:.2e}", flush=True)
     _field_errs(sg, so, CONV_NAMES, 256, f"{prec}/{tune} step 1", tol, lev=128)
     for name in ("ustar_avg", "psi_m", "phi_h"):
         err = relerr(sg.get(name), so[name])
@@ -291,7 +299,7 @@ def test_convective256_dynamic_update(torch_mod, prec):
     for name in ("F_LM", "F_MM", "Cs_opt2", "Beta", "G_LM", "G_NN", "Cs2_Pr", "Pr_t"):
         a, b = sg.get(name)[:16, :, :256], so[name][:16, :, :256]
         if prec == "fp64":
-            flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS[name], l2=1e-9)
+            flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS[name], l2=MULTI_STEP_L2)
         else:
             e = rel_l2(a, b)
             print(f"  {name} rel L2 {e:.1e}", flush=True)

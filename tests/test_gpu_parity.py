@@ -4,7 +4,7 @@ oracle on the same seeded inputs.  FP64 bar: relative max error <= 1e-12
 import numpy as np
 import pytest
 
-from conftest import FLIP_ULPS, flip_check, relerr
+from conftest import FLIP_ULPS, MULTI_STEP_L2, flip_check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -470,7 +470,7 @@ def test_full_step_vs_oracle(torch_mod, prec):
     for name in ("F_LM", "F_MM", "Cs_opt2", "Beta"):
         a, b = sg.get(name)[..., :nx], so[name][..., :nx]
         if prec == "fp64":
-            flip_check(a, b, name, rel=1e-9, max_ulps=FLIP_ULPS[name], l2=1e-9)
+            flip_check(a, b, name, rel=1e-9, max_ulps=FLIP_ULPS[name], l2=MULTI_STEP_L2)
         else:
             assert (np.abs(a - b) > 1e-3 * np.max(np.abs(b))).mean() < 1e-3, name
 
@@ -508,8 +508,8 @@ def test_full_step_crossing_dynamic_switch(torch_mod, prec):
         nu_a, nu_b = sg.get("Nu_t")[:nz - 1, :, :nx], so["Nu_t"][:nz - 1, :, :nx]
         cs_a, cs_b = sg.get("Cs_opt2")[:nz - 1, :, :nx], so["Cs_opt2"][:nz - 1, :, :nx]
         if prec == "fp64":
-            flip_check(cs_a, cs_b, "Cs_opt2", rel=1e-10, max_ulps=FLIP_ULPS["Cs_opt2"], l2=1e-9)
-            flip_check(nu_a, nu_b, "Nu_t", rel=1e-10, max_ulps=FLIP_ULPS["tau"], l2=1e-9)
+            flip_check(cs_a, cs_b, "Cs_opt2", rel=1e-10, max_ulps=FLIP_ULPS["Cs_opt2"], l2=MULTI_STEP_L2)
+            flip_check(nu_a, nu_b, "Nu_t", rel=1e-10, max_ulps=FLIP_ULPS["tau"], l2=MULTI_STEP_L2)
         else:
             assert relerr(nu_a, nu_b) < 5e-3 and relerr(cs_a, cs_b) < 5e-3
     # the dynamic coefficient is in use: Nu_t / (|S| Delta^2) is no longer the constant

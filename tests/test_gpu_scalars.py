@@ -5,7 +5,7 @@ FP32 rounding points of lagrange_scalar_Sdep may flip isolated float ulps."""
 import numpy as np
 import pytest
 
-from conftest import FLIP_ULPS, flip_check, relerr
+from conftest import FLIP_ULPS, MULTI_STEP_L2, flip_check, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -274,7 +274,7 @@ def test_convective_step_vs_oracle(torch_mod, prec):
     for name in ("G_LM", "G_MM", "Pr_t", "Cs2_Pr", "F_LM"):
         a, b = sg.get(name)[..., :nx], so[name][..., :nx]
         if prec == "fp64":
-            flip_check(a, b, name, rel=1e-9, max_ulps=FLIP_ULPS.get(name, 1), l2=1e-9)
+            flip_check(a, b, name, rel=1e-9, max_ulps=FLIP_ULPS.get(name, 1), l2=MULTI_STEP_L2)
         else:
             assert (np.abs(a - b) > 1e-3 * np.max(np.abs(b))).mean() < 1e-3, name
 
@@ -300,17 +300,20 @@ def test_convective_step_crossing_dynamic_switch(torch_mod, prec):
             print(f"  step {so.jt} {name}: {err:.2e}", flush=True)
             assert err < tol, (name, err)
         # the scalar right-hand side takes Nu_t / Pr_t pointwise: where Pr_t
-        # carries a float flip (a few ulps, 1e-7 relative) its SGS flux term
-        # moves by that fraction, at isolated points
+        # carries a float flip (a few ulps, ~1e-7 relative) its SGS flux term
+        # moves by that fraction, and the spectral derivatives of the next
+        # steps spread the difference over the plane at a decaying level
         a, b = sg.get("RHS_T")[:nz - 1, :, :nx], so["RHS_T"][:nz - 1, :, :nx]
+        err, l2 = relerr(a, b), float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+        print(f"  step {so.jt} RHS_T: max {err:.2e} rel L2 {l2:.2e}", flush=True)
         if prec == "fp64":
-            flip_check(a, b, "RHS_T", rel=tol, max_ulps=None, l2=1e-9)
+            assert err < 1e-8 and l2 < 1e-9, ("RHS_T", err, l2)
         else:
-            assert relerr(a, b) < 5 * tol
+            assert err < 5 * tol
         for name in ("Pr_t", "Nu_t", "Cs_opt2"):
             a, b = sg.get(name)[:nz - 1, :, :nx], so[name][:nz - 1, :, :nx]
             if prec == "fp64":
-                flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS.get(name, 1), l2=1e-9)
+                flip_check(a, b, name, rel=1e-10, max_ulps=FLIP_ULPS.get(name, 1), l2=MULTI_STEP_L2)
             else:
                 assert relerr(a, b) < 5e-3, name
     assert not np.all(so["Pr_t"][:nz - 1, :, :nx] == 0.55)   # the dynamic Pr_t is in use
