@@ -183,6 +183,17 @@ int les_press_stag_array(les_ctx *c, const double *u, const double *v, const dou
 int les_tridag_array(les_ctx *c, const double *a, const double *b, const double *cc, const double *r,
                      double *u);
 
+/* Allocation-free variant of les_tridag_array (no counterpart in the
+   reference code), for callers that cannot allocate device memory: code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+   Same arguments, results and errors as les_tridag_array; the Thomas factors
+   ((nz+1)*lh*ny doubles) and the error flag live in one file-scope device
+   array of 64 MiB per GPU, held until the process ends (grids up to e.g.
+   256 x 256 x 128; larger ones return 2 with a message).  Every call on a GPU
+   shares that array: calls that overlap on one GPU race. */
+int les_tridag_array_noalloc(les_ctx *c, const double *a, const double *b, const double *cc, const double *r,
+                             double *u);
+
 /* ---- forcing.f90:128-219 project; rmsdiv.f90:1-113; scalars_module2.f90:22-70 ---- */
 int les_project(les_ctx *c, double *u, double *v, double *w, const double *dpdx, const double *dpdy,
                 const double *dpdz);

@@ -472,6 +472,14 @@ class Les:
         co = [_ptr(t, F64, nr, n) for t, n in ((a, "a"), (b, "b"), (c, "c"))]
         self._chk(self.L.les_tridag_array(self.ctx, *co, _ptr(r, F64, 2 * nr, "r"), _ptr(u, F64, 2 * nr, "u")))
 
+    def tridag_array_noalloc(self, a, b, c, r, u):
+        """tridag_array without device allocation (the Thomas factors in the
+        library's per-GPU scratch array); results identical to tridag_array."""
+        nr = (self.nz + 1) * self.ny * self.lh
+        co = [_ptr(t, F64, nr, n) for t, n in ((a, "a"), (b, "b"), (c, "c"))]
+        self._chk(self.L.les_tridag_array_noalloc(self.ctx, *co, _ptr(r, F64, 2 * nr, "r"),
+                                                  _ptr(u, F64, 2 * nr, "u")))
+
     def project(self, u, v, w, dpdx, dpdy, dpdz):
         self._chk(self.L.les_project(self.ctx, *[self._f(t, n) for t, n in ((u, "u"), (v, "v"), (w, "w"),
                                                                             (dpdx, "dpdx"), (dpdy, "dpdy"),

@@ -361,6 +361,7 @@ int les_create(const les_grid* gi, les_ctx** out) {
     return 2;
   }
   check(c, hipGetDevice(&c->device), "hipGetDevice");
+  if (c->err.empty()) noalloc_init(c->device, &c->err);
   HostTables h;
   host_tables(g, h);
   c->keep2 = keep_cols(g, h.G2);

@@ -269,6 +269,14 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
 template <typename T>
 void op_press_stag(les_ctx* c, const T* u, const T* v, const T* w, const T* divtz, T* p, T* dpdx, T* dpdy);
 int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u);
+// the same on caller-provided device scratch: gam of tridag_array_gamma_bytes,
+// err one int; returns 0, or 1 / 2 for tridag_array.f90's two stops
+size_t tridag_array_gamma_bytes(const les_ctx* c);
+int tridag_array_with(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u,
+                      double* gam, int* err);
+// les_noalloc.hip: device address of the allocation-free entry points'
+// scratch array on `device` (the current device), looked up once per device
+int noalloc_init(int device, std::string* err);
 // the pressure solve's tridiagonal sweep under the schedule in c->tune
 size_t tridiag_lds(int mode, int block, int nz);
 template <typename T>

@@ -1525,12 +1525,15 @@ __global__ void k_tridag_array(const double* a, const double* b, const double* c
   }
 }
 
-int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u) {
+size_t tridag_array_gamma_bytes(const les_ctx* c) {
+  return (size_t)(c->g.nz + 1) * c->g.lh * c->g.ny * sizeof(double);
+}
+
+int tridag_array_with(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u,
+                      double* gam, int* err) {
   const Grid& g = c->g;
   const int n = g.nz + 1, nm = g.lh * g.ny;
-  double* gam = (double*)scratch(c, S_TMP0, (size_t)n * nm * sizeof(double));
-  int* err = (int*)scratch(c, S_TMP2, 256);
-  LCHK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+  op_fill<int>(c, err, 1, 0);  // a fill kernel, not a memset node, so the call can sit in a captured graph
   hipLaunchKernelGGL(k_tridag_array, dim3((nm + 127) / 128), dim3(128), 0, c->stream, a, b, cc,
                      (const cx<double>*)r, (cx<double>*)u, gam, g.lh, g.ny, n, err);
   LCHK(c, hipGetLastError());
@@ -1538,6 +1541,12 @@ int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* 
   LCHK(c, hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   LCHK(c, hipStreamSynchronize(c->stream));
   return h;
+}
+
+int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u) {
+  double* gam = (double*)scratch(c, S_TMP0, tridag_array_gamma_bytes(c));
+  int* err = (int*)scratch(c, S_TMP2, 256);
+  return tridag_array_with(c, a, b, cc, r, u, gam, err);
 }
 
 // ------------------------------------------------------------------------
