@@ -250,6 +250,32 @@ int les_graph_prepare(les_ctx *c);
 /* rmsdiv of the state's current derivatives and timestep_conditions */
 int les_state_diag(les_ctx *c, double *rms, double out4[4]);
 
+/* ---- statistics, FP64 reductions on the device (plane sums compensated,
+   so independent of the summation order).  stats layout, nl = nz-1 levels:
+   ubar, vbar, wbar, <u'^2>, <v'^2>, <w'^2>, <u'w'>, <txz>, <u'w'>+<txz> (nl
+   doubles each; fluctuations on w-nodes, zero on the first level), then
+   E11(k1)/z (nl rows of nx/2): 9 nl + nl nx/2 doubles.  surface: ustar_,
+   ustar_avg_time, ustar_time, theta_, thetav_, q_time, wt_, L_ (the
+   scalar entries and L_ are 0 without scalars). ---- */
+/* spectrum, io.f90:1613-1646: spec (nlev, nx/2) device doubles */
+int les_spectrum(les_ctx *c, const double *f, int nlev, double *spec);
+/* the current statistics of avg_stats, io.f90:1722-1755 (stats: device) */
+int les_avg_stats(les_ctx *c, const double *u, const double *v, const double *w, const double *txz,
+                  double *stats);
+/* obukhov's plane averages, scalars_module.f90:545-561,655-661: first-level
+   u, v, txz, tyz (ld,ny), psi_m, psi_m0, ustar_avg (nx,ny); theta, qmix
+   (ld,ny) and sgs_t3, sgs_q3 (ld,ny) or all four NULL; out8: device */
+int les_surface_stats(les_ctx *c, const double *u, const double *v, const double *txz, const double *tyz,
+                      const double *psi_m, const double *psi_m0, const double *ustar_avg, const double *theta,
+                      const double *qmix, const double *sgs_t3, const double *sgs_q3, double *out8);
+/* the same on the whole-step state (either precision) into host arrays
+   (either may be NULL); _accumulate folds them into running means held on
+   the device (avg_stats' cumulative averages, io.f90:1800-1813) and
+   _average reads those back (n: samples; reset != 0 restarts them) */
+int les_state_stats(les_ctx *c, double *stats, double *surface);
+int les_state_stats_accumulate(les_ctx *c);
+int les_state_stats_average(les_ctx *c, double *stats, double *surface, int *n, int reset);
+
 /* Device memory and host mirrors (no reference counterpart): the Fortran
    drop-in shims keep one device mirror per host array, keyed by its address,
    and push inputs / pull outputs around each routine-level call. */

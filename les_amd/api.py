@@ -491,6 +491,41 @@ class Les:
                                     C.byref(r)))
         return r.value
 
+    # ---- statistics (io.f90 spectrum / avg_stats, obukhov's plane averages) ----
+    def stats_len(self):
+        return stats_len(self.nz, self.nx)
+
+    def spectrum(self, f):
+        """spectrum (io.f90:1613-1646) of every level of f: (nlev, nx/2) float64 GPU tensor."""
+        n = self._nlev(f)
+        spec = torch.zeros((n, self.nx // 2), dtype=F64, device="cuda")
+        self._chk(self.L.les_spectrum(self.ctx, self._f(f, "f", n), n, _ptr(spec)))
+        return spec
+
+    def avg_stats(self, u, v, w, txz):
+        """Current statistics of avg_stats (io.f90:1722-1755) as a float64 GPU
+        tensor in the layout of stats_fields."""
+        out = torch.zeros(self.stats_len(), dtype=F64, device="cuda")
+        self._chk(self.L.les_avg_stats(self.ctx, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                       self._f(txz, "txz"), _ptr(out)))
+        return out
+
+    def surface_stats(self, u, v, txz, tyz, psi_m, psi_m0, ustar_avg, theta=None, qmix=None, sgs_t3=None,
+                      sgs_q3=None):
+        """obukhov's plane averages (scalars_module.f90:545-561,655-661) from
+        first-level fields: float64 GPU tensor (8,) in the order of SURFACE_NAMES."""
+        out = torch.zeros(8, dtype=F64, device="cuda")
+        sc = [theta, qmix, sgs_t3, sgs_q3]
+        if any(x is None for x in sc) and not all(x is None for x in sc):
+            raise LesError("surface_stats: theta, qmix, sgs_t3, sgs_q3 go together")
+        scp = [None] * 4 if sc[0] is None else [self._pl(theta, "theta"), self._pl(qmix, "qmix"),
+                                                 self._pl(sgs_t3, "sgs_t3"), self._pl(sgs_q3, "sgs_q3")]
+        self._chk(self.L.les_surface_stats(self.ctx, self._pl(u, "u"), self._pl(v, "v"), self._pl(txz, "txz"),
+                                           self._pl(tyz, "tyz"), self._p2(psi_m, "psi_m"),
+                                           self._p2(psi_m0, "psi_m0"), self._p2(ustar_avg, "ustar_avg"), *scp,
+                                           _ptr(out)))
+        return out
+
     def timestep_conditions(self, u, v, w, nu_t):
         out = (C.c_double * 4)()
         self._chk(self.L.les_timestep_conditions(self.ctx, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
@@ -584,6 +619,46 @@ class LesSim(Les):
         a, b = C.c_int(), C.c_int()
         self._chk(self.L.les_get_time(self.ctx, C.byref(a), C.byref(b)))
         return a.value
+
+    # ---- statistics of the state (FP64 reductions on the device) ----
+    def stats(self):
+        """Current statistics of the state: (stats, surface) host float64
+        arrays; stats_fields() names the parts."""
+        import numpy as np
+        st, sf = np.zeros(self.stats_len()), np.zeros(8)
+        self._chk(self.L.les_state_stats(self.ctx, st.ctypes.data_as(C.c_void_p), sf.ctypes.data_as(C.c_void_p)))
+        return st, sf
+
+    def stats_accumulate(self):
+        """Fold the current statistics into running means kept on the device."""
+        self._chk(self.L.les_state_stats_accumulate(self.ctx))
+
+    def stats_average(self, reset=False):
+        """Running means since the last reset: (stats, surface, samples)."""
+        import numpy as np
+        st, sf, n = np.zeros(self.stats_len()), np.zeros(8), C.c_int()
+        self._chk(self.L.les_state_stats_average(self.ctx, st.ctypes.data_as(C.c_void_p),
+                                                 sf.ctypes.data_as(C.c_void_p), C.byref(n), int(bool(reset))))
+        return st, sf, n.value
+
+
+SURFACE_NAMES = ("ustar_", "ustar_avg_time", "ustar_time", "theta_", "thetav_", "q_time", "wt_", "L_")
+STATS_NAMES = ("ubar", "vbar", "wbar", "u2", "v2", "w2", "uw", "txz", "stress")
+
+
+def stats_len(nz, nx):
+    """Length of the statistics vector of a grid: 9 (nz-1) + (nz-1) nx/2."""
+    return 9 * (nz - 1) + (nz - 1) * (nx // 2)
+
+
+def stats_fields(a, nz, nx):
+    """Named views of a statistics vector: the nine plane profiles (nz-1
+    levels each; u2, v2, w2, uw on w-nodes, zero on the first level; stress =
+    uw + txz) and Eoz, E11(k1)/z as (nz-1, nx/2)."""
+    nl = nz - 1
+    d = {n: a[i * nl:(i + 1) * nl] for i, n in enumerate(STATS_NAMES)}
+    d["Eoz"] = a[9 * nl:].reshape(nl, nx // 2)
+    return d
 
 
 def checkerboard_flux(nx, ny, hi=0.1, lo=0.02, patches=4):

@@ -1769,6 +1769,95 @@ int les_state_init_scalar_ic(les_ctx* c, uint64_t seed, double T_init, double z_
   DONE(c);
 }
 
+// ---- statistics: spectrum (io.f90:1613-1646), the current statistics of
+// avg_stats (io.f90:1722-1755) and obukhov's plane averages
+// (scalars_module.f90:545-561,655-661), FP64 reductions on the device ----
+int les_spectrum(les_ctx* c, const double* f, int nlev, double* spec) {
+  GUARD(c);
+  if (nlev < 1 || nlev > c->g.nz + 1) {
+    c->err = "spectrum: nlev must be 1 .. nz+1";
+    return 2;
+  }
+  op_spectrum<double>(c, f, nlev, spec, 0);
+  DONE(c);
+}
+int les_avg_stats(les_ctx* c, const double* u, const double* v, const double* w, const double* txz, double* stats) {
+  GUARD(c);
+  op_avg_stats<double>(c, u, v, w, txz, stats);
+  DONE(c);
+}
+int les_surface_stats(les_ctx* c, const double* u, const double* v, const double* txz, const double* tyz,
+                      const double* psi_m, const double* psi_m0, const double* ustar_avg, const double* theta,
+                      const double* qmix, const double* sgs_t3, const double* sgs_q3, double* out8) {
+  GUARD(c);
+  op_surface_stats<double>(c, u, v, txz, tyz, psi_m, psi_m0, ustar_avg, theta, qmix, sgs_t3, sgs_q3, 0.0, 0.0,
+                           out8);
+  DONE(c);
+}
+
+}  // extern "C"
+
+// statistics of the state into the device buffer d (stats_len + 8 doubles)
+template <typename T>
+static void state_stats_dev(les_ctx* c, double* d) {
+  op_avg_stats<T>(c, F<T>(c, "u"), F<T>(c, "v"), F<T>(c, "w"), F<T>(c, "txz"), d);
+  const bool sc = c->st->scal != 0;
+  op_surface_stats<T>(c, F<T>(c, "u"), F<T>(c, "v"), F<T>(c, "txz"), F<T>(c, "tyz"),
+                      (const double*)state_ptr(c, "psi_m"), (const double*)state_ptr(c, "psi_m0"),
+                      (const double*)state_ptr(c, "ustar_avg"), sc ? F<T>(c, "theta") : nullptr,
+                      sc ? F<T>(c, "qmix") : nullptr, sc ? (const double*)state_ptr(c, "sgs_t3") : nullptr,
+                      sc ? (const double*)state_ptr(c, "sgs_q3") : nullptr, c->scal.theta_ref, c->scal.q_ref,
+                      d + stats_len(c));
+}
+static double* state_stats_now(les_ctx* c) {
+  const size_t len = stats_len(c) + 8;
+  double* d = (double*)scratch(c, S_STAT, len * sizeof(double));
+  if (c->st->prec)
+    state_stats_dev<float>(c, d);
+  else
+    state_stats_dev<double>(c, d);
+  return d;
+}
+static void stats_to_host(les_ctx* c, const double* d, double* stats, double* surface) {
+  const size_t len = stats_len(c);
+  if (stats) check(c, hipMemcpyAsync(stats, d, len * sizeof(double), hipMemcpyDeviceToHost, c->stream), "stats d2h");
+  if (surface)
+    check(c, hipMemcpyAsync(surface, d + len, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "stats d2h");
+  check(c, hipStreamSynchronize(c->stream), "stats sync");
+}
+
+extern "C" {
+
+int les_state_stats(les_ctx* c, double* stats, double* surface) {
+  GUARD(c);
+  if (!c->st) return -1;
+  const double* d = state_stats_now(c);
+  stats_to_host(c, d, stats, surface);
+  DONE(c);
+}
+int les_state_stats_accumulate(les_ctx* c) {
+  GUARD(c);
+  if (!c->st) return -1;
+  const size_t len = stats_len(c) + 8;
+  const double* d = state_stats_now(c);
+  double* avg = (double*)scratch(c, S_STATAVG, len * sizeof(double));
+  if (c->stats_n == 0) op_fill<double>(c, avg, len, 0.0);
+  op_running_mean(c, avg, d, len, c->stats_n);
+  c->stats_n += 1;
+  DONE(c);
+}
+int les_state_stats_average(les_ctx* c, double* stats, double* surface, int* n, int reset) {
+  GUARD(c);
+  if (!c->st) return -1;
+  if (n) *n = (int)c->stats_n;
+  if (c->stats_n > 0) {
+    const double* avg = (const double*)scratch(c, S_STATAVG, (stats_len(c) + 8) * sizeof(double));
+    stats_to_host(c, avg, stats, surface);
+  }
+  if (reset) c->stats_n = 0;
+  DONE(c);
+}
+
 int les_state_diag(les_ctx* c, double* rms, double out4[4]) {
   GUARD(c);
   if (!c->st) return -1;

@@ -24,6 +24,26 @@ __device__ __forceinline__ LevelCol level_col(long i, long pl, int ld) {
   return {(int)k, (int)((ii - k * p) % (unsigned)ld)};
 }
 
+// Compensated summation: s + c is the running sum, every addition an
+// error-free TwoSum whose rounding error is carried in c.  A sum formed this
+// way is the exact sum rounded once, up to a term of order n eps^2, in any
+// order of the additions (plane means of the statistics and of calcbeta).
+// No contraction: a product fused into the TwoSum's first addition would
+// break the error term.
+struct CSum {
+  double s, c;
+};
+__device__ __forceinline__ CSum csum_add(CSum a, double x) {
+#pragma clang fp contract(off)
+  const double t = a.s + x, bb = t - a.s;
+  return CSum{t, a.c + ((a.s - (t - bb)) + (x - bb))};
+}
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+#pragma clang fp contract(off)
+  const double t = a.s + b.s, bb = t - a.s;
+  return CSum{t, (a.c + b.c) + ((a.s - (t - bb)) + (b.s - bb))};
+}
+
 // Grid and run constants (param.f90:56-65,98,100,144-151 of the reference),
 // plus the derived sizes used by the kernels.
 struct Grid {
@@ -127,6 +147,7 @@ struct les_ctx {
   void* mlp = nullptr;             // les::MlpModel
   int closure_mode = 0, closure_period = 10;
   double closure_thresh = 0.2;
+  long stats_n = 0;                // samples in the running statistics (les_state_stats_accumulate)
   // host-address -> device mirror registry (les_mirror)
   std::vector<std::pair<const void*, std::pair<void*, size_t>>> mirrors;
 };
@@ -137,7 +158,8 @@ namespace les {
 enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
-  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_TUNE0, S_TUNE1, S_TUNE2, S_NSLOTS
+  S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_TUNE0, S_TUNE1, S_TUNE2,
+  S_STATP, S_STATX, S_STAT, S_STATAVG, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 void scratch_release(les_ctx* c, int slot);  // give a slot's memory back (stream drained first)
@@ -274,6 +296,22 @@ int op_tridag_array(les_ctx* c, const double* a, const double* b, const double* 
 size_t tridag_array_gamma_bytes(const les_ctx* c);
 int tridag_array_with(les_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u,
                       double* gam, int* err);
+// les_stats.hip: statistics as FP64 reductions (all outputs device doubles).
+// stats layout over nl = nz - 1 levels: ubar vbar wbar <u'^2> <v'^2> <w'^2>
+// <u'w'> <txz> <u'w'>+<txz> (nl each), then E11(k1)/z (nl rows of nx/2)
+size_t stats_len(const les_ctx* c);
+template <typename T>
+void op_spectrum(les_ctx* c, const T* f, int nlev, double* spec, int eoz);
+template <typename T>
+void op_avg_stats(les_ctx* c, const T* u, const T* v, const T* w, const T* txz, double* stats);
+// surface: ustar_, ustar_avg_time, ustar_time, theta_, thetav_, q_time, wt_,
+// L_; theta..sgs_q3 may be null; th_ref, q_ref are added to stored theta, q
+template <typename T>
+void op_surface_stats(les_ctx* c, const T* u, const T* v, const T* txz, const T* tyz, const double* psi_m,
+                      const double* psi_m0, const double* ustar_avg, const T* theta, const T* qmix,
+                      const double* sgs_t3, const double* sgs_q3, double th_ref, double q_ref, double* out8);
+// running means (n avg + cur) / (n + 1) over len doubles
+void op_running_mean(les_ctx* c, double* avg, const double* cur, size_t len, long n);
 // les_noalloc.hip: device address of the allocation-free entry points'
 // scratch array on `device` (the current device), looked up once per device
 int noalloc_init(int device, std::string* err);

@@ -27,18 +27,6 @@ namespace les {
 // ------------------------------------------------------------------------
 constexpr int NPART = 32;
 
-struct CSum {  // s + c is the running sum
-  double s, c;
-};
-__device__ __forceinline__ CSum csum_add(CSum a, double x) {
-  const double t = a.s + x, bb = t - a.s;
-  return CSum{t, a.c + ((a.s - (t - bb)) + (x - bb))};
-}
-__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
-  const double t = a.s + b.s, bb = t - a.s;
-  return CSum{t, (a.c + b.c) + ((a.s - (t - bb)) + (b.s - bb))};
-}
-
 template <class Fn>
 __global__ void k_level_partial(Fn fn, int nx, int ny, double* part) {
   const int k = blockIdx.y, n = nx * ny;

@@ -2004,3 +2004,110 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
   }
   for (int t = 0; t < NT; t++) free(T[t]);
 }
+
+/* ------------------------------------------------------------------ */
+/* statistics: io.f90 spectrum / avg_stats, obukhov's plane averages    */
+/* ------------------------------------------------------------------ */
+
+/* spectrum, io.f90:1613-1646: per row vel_r = f/nx, its halfcomplex
+   transform, spec(1) += 0.5 c_0^2, spec(k) += |c_(k-1)|^2 (k = 2..nx/2,
+   Nyquist taken as 0), then spec / ny */
+void or_spectrum(or_ctx *c, const double *f, int nlev, double *spec) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nk = nx / 2;
+  const size_t pl = (size_t)ld * ny;
+  cpx *row = (cpx *)malloc(sizeof(cpx) * (size_t)nx), *t = (cpx *)malloc(sizeof(cpx) * (size_t)nx);
+  csum_t *acc = (csum_t *)malloc(sizeof(csum_t) * (size_t)nk);
+  for (int k = 0; k < nlev; k++) {
+    for (int j = 0; j < nk; j++) { acc[j].s = 0.0; acc[j].c = 0.0; }
+    for (int y = 0; y < ny; y++) {
+      const double *r = f + pl * k + (size_t)ld * y;
+      for (int x = 0; x < nx; x++) { row[x].re = r[x] / (double)nx; row[x].im = 0.0; }
+      fft_exec(&c->sm.px, row, t, -1);
+      csum_add(&acc[0], 0.5 * row[0].re * row[0].re);
+      for (int j = 1; j < nk; j++) csum_add(&acc[j], row[j].re * row[j].re + row[j].im * row[j].im);
+    }
+    for (int j = 0; j < nk; j++) spec[(size_t)k * nk + j] = csum_val(&acc[j]) / (double)ny;
+  }
+  free(row); free(t); free(acc);
+}
+
+static double plane_mean(const or_ctx *c, const double *a, int stride) {
+  csum_t s = { 0.0, 0.0 };
+  for (int y = 0; y < c->ny; y++)
+    for (int x = 0; x < c->nx; x++) csum_add(&s, a[x + (size_t)stride * y]);
+  return csum_val(&s) / (double)(c->nx * c->ny);
+}
+
+void or_avg_stats(or_ctx *c, const double *u, const double *v, const double *w, const double *txz, double *stats) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nl = c->nz - 1, nk = nx / 2;
+  const size_t pl = (size_t)ld * ny;
+  double *ub = stats, *vb = stats + nl, *wb = stats + 2 * nl;
+  double *u2 = stats + 3 * nl, *v2 = stats + 4 * nl, *w2 = stats + 5 * nl;
+  double *uw = stats + 6 * nl, *tb = stats + 7 * nl, *st = stats + 8 * nl, *eoz = stats + 9 * nl;
+  for (int k = 0; k < nl; k++) {                                             /* :1725-1727 */
+    ub[k] = plane_mean(c, u + pl * k, ld);
+    vb[k] = plane_mean(c, v + pl * k, ld);
+    wb[k] = plane_mean(c, w + pl * k, ld);
+    tb[k] = plane_mean(c, txz + pl * k, ld);                                 /* :1749 */
+  }
+  for (int k = 0; k < nl; k++) {
+    csum_t s2u = { 0, 0 }, s2v = { 0, 0 }, s2w = { 0, 0 }, suw = { 0, 0 };
+    if (k > 0)                                                               /* :1729-1742, zero at jz = 1 */
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const size_t i = pl * k + x + (size_t)ld * y;
+          const double up = 0.5 * (u[i] - ub[k] + u[i - pl] - ub[k - 1]);
+          const double vp = 0.5 * (v[i] - vb[k] + v[i - pl] - vb[k - 1]);
+          const double wp = w[i] - wb[k];
+          csum_add(&s2u, up * up); csum_add(&s2v, vp * vp); csum_add(&s2w, wp * wp); csum_add(&suw, up * wp);
+        }
+    const double n = (double)(nx * ny);
+    u2[k] = csum_val(&s2u) / n; v2[k] = csum_val(&s2v) / n; w2[k] = csum_val(&s2w) / n;  /* :1744-1746 */
+    uw[k] = csum_val(&suw) / n;                                              /* :1748 */
+    st[k] = uw[k] + tb[k];                                                   /* :1750 */
+  }
+  or_spectrum(c, u, nl, eoz);                                                /* :1753 */
+  for (int k = 0; k < nl; k++) {
+    const double z = ((double)(k + 1) - 0.5) * c->dz;                        /* :1754 */
+    for (int j = 0; j < nk; j++) eoz[(size_t)k * nk + j] = eoz[(size_t)k * nk + j] / z;
+  }
+}
+
+void or_surface_stats(or_ctx *c, const double *u, const double *v, const double *txz, const double *tyz,
+                      const double *psi_m, const double *psi_m0, const double *ustar_avg, const double *theta,
+                      const double *qmix, const double *sgs_t3, const double *sgs_q3, double out[8]) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld;
+  const double n = (double)(float)(nx * ny);
+  csum_t su = { 0, 0 };
+  for (int y = 0; y < ny; y++)
+    for (int x = 0; x < nx; x++) {
+      const size_t i = x + (size_t)ld * y;
+      csum_add(&su, sqrt(u[i] * u[i] + v[i] * v[i]));
+    }
+  const double mpsi = plane_mean(c, psi_m, nx), mpsi0 = plane_mean(c, psi_m0, nx);
+  /* zo_ = exp(mean(log(zo))) = zo for the uniform roughness (bottombc.f90:53) */
+  const double ustar_ = csum_val(&su) / n * VONK / (log(0.5 * c->dz / c->P.z0) - mpsi + mpsi0);   /* :557-558 */
+  const double mtx = plane_mean(c, txz, ld), mty = plane_mean(c, tyz, ld);
+  out[0] = ustar_;
+  out[1] = plane_mean(c, ustar_avg, nx);                                     /* :655 */
+  out[2] = pow(mtx * mtx + mty * mty, 0.25);                                 /* :656-657 */
+  out[3] = out[4] = out[5] = out[6] = out[7] = 0.0;
+  if (theta && qmix && sgs_t3 && sgs_q3) {
+    const or_scalar_params *S = &c->S;
+    csum_t stv = { 0, 0 }, swtv = { 0, 0 };
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = x + (size_t)ld * y;
+        csum_add(&stv, theta[i] * (1 + 0.61 * 0.001 * qmix[i] * S->q_scale));               /* :547 */
+        csum_add(&swtv, sgs_t3[i] + 0.61 * theta[i] * (0.001 * S->q_scale * sgs_q3[i]));    /* :549-550 */
+      }
+    const double theta_ = plane_mean(c, theta, ld), thetav_ = csum_val(&stv) / n;         /* :560-561 */
+    const double wt_ = plane_mean(c, sgs_t3, ld), wtv_ = csum_val(&swtv) / n;             /* :554-555 */
+    const double g_ = (S->g * S->z_i) / (S->u_star * S->u_star);                          /* :545 */
+    out[3] = theta_;
+    out[4] = thetav_;
+    out[5] = plane_mean(c, qmix, ld);                                                      /* :660 */
+    out[6] = wt_;
+    out[7] = wtv_ != 0.0 ? -(ustar_ * ustar_ * ustar_) / (VONK * (g_ / theta_) * wtv_) : 0.0;  /* :661 */
+  }
+}

@@ -230,6 +230,21 @@ void or_set_time(or_ctx *c, int jt, int jt_total);
 void or_set_sgs_model(or_ctx *c, int model);
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init);
 
+/* ---- statistics (all plane sums compensated) ----
+   spectrum, io.f90:1613-1646: spec (nlev, nx/2), spec(k) = <|c_k|^2>_y with
+   c_k the x coefficients of f/nx, the mean (k = 0) counted with 1/2 */
+void or_spectrum(or_ctx *c, const double *f, int nlev, double *spec);
+/* avg_stats, io.f90:1722-1755 (current statistics of levels 1..nz-1, nl =
+   nz-1): stats = [ubar | vbar | wbar | <u'^2> | <v'^2> | <w'^2> | <u'w'> |
+   <txz> | <u'w'> + <txz>] (nl each) then Eozbar (nl, nx/2) = spectrum(u)/z */
+void or_avg_stats(or_ctx *c, const double *u, const double *v, const double *w, const double *txz, double *stats);
+/* obukhov's plane averages, scalars_module.f90:545-561,655-661: out = [ustar_,
+   ustar_avg_time, ustar_time, theta_, thetav_, q_time, wt_, L_]; theta, qmix,
+   sgs_t3, sgs_q3 may be NULL (neutral: the scalar entries and L_ are 0) */
+void or_surface_stats(or_ctx *c, const double *u, const double *v, const double *txz, const double *tyz,
+                      const double *psi_m, const double *psi_m0, const double *ustar_avg, const double *theta,
+                      const double *qmix, const double *sgs_t3, const double *sgs_q3, double out[8]);
+
 /* counter-based uniform in [0,1): splitmix64 of (seed, index) */
 double or_uniform(uint64_t seed, uint64_t index);
 

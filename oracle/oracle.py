@@ -320,6 +320,26 @@ class Oracle:
     def rmsdiv(self, dudx, dvdy, dwdz):
         return self.L.or_rmsdiv(C.c_void_p(self.ctx), _p(dudx), _p(dvdy), _p(dwdz))
 
+    def spectrum(self, f):
+        spec = np.zeros((f.shape[0], self.nx // 2))
+        self.L.or_spectrum(C.c_void_p(self.ctx), _p(f), f.shape[0], _p(spec))
+        return spec
+
+    def avg_stats(self, u, v, w, txz):
+        out = np.zeros(9 * (self.nz - 1) + (self.nz - 1) * (self.nx // 2))
+        self.L.or_avg_stats(C.c_void_p(self.ctx), _p(u), _p(v), _p(w), _p(txz), _p(out))
+        return out
+
+    def surface_stats(self, u, v, txz, tyz, psi_m, psi_m0, ustar_avg, theta=None, qmix=None, sgs_t3=None,
+                      sgs_q3=None):
+        """First-level planes (ny, ld) of u, v, txz, tyz (and theta, qmix,
+        sgs_t3, sgs_q3), (ny, nx) psi_m, psi_m0, ustar_avg."""
+        out = np.zeros(8)
+        sc = [None if a is None else _p(np.ascontiguousarray(a)) for a in (theta, qmix, sgs_t3, sgs_q3)]
+        self.L.or_surface_stats(C.c_void_p(self.ctx), *[_p(np.ascontiguousarray(a)) for a in
+                                                       (u, v, txz, tyz, psi_m, psi_m0, ustar_avg)], *sc, _p(out))
+        return out
+
     def timestep_conditions(self, u, v, w, nu_t):
         out = np.zeros(4)
         self.L.or_timestep_conditions(C.c_void_p(self.ctx), _p(u), _p(v), _p(w), _p(nu_t), _p(out))

@@ -101,9 +101,41 @@ def neutral32(torch_mod):
             s.step(n - done)
         done = n
         snaps[n] = {"oracle": {k: so[k].copy() for k in names}}
+        snaps[n]["oracle"]["stats"] = oracle_state_stats(o, so)
         for p, s in sims.items():
             snaps[n][p] = {k: s.get(k) for k in names}
+            snaps[n][p]["stats"] = s.stats()
     return nx, ny, nz, snaps
+
+
+def oracle_state_stats(o, so, scal=False):
+    """The oracle's statistics of its state (avg_stats and obukhov's plane
+    averages), the quantities les_state_stats reduces on the GPU."""
+    st = o.avg_stats(so["u"], so["v"], so["w"], so["txz"])
+    extra = (so["theta"][0], so["qmix"][0], so["sgs_t3"], so["sgs_q3"]) if scal else ()
+    sf = o.surface_stats(so["u"][0], so["v"][0], so["txz"][0], so["tyz"][0], so["psi_m"], so["psi_m0"],
+                         so["ustar_avg"], *extra)
+    return st, sf
+
+
+def compare_stats(g, o, nz, nx, label, tol_prof, tol_spec, tol_us):
+    """GPU statistics (les_state_stats) against the oracle's of its own state:
+    relative L2 of each plane profile and of each level's E11(k1)/z, relative
+    error of the three u* values."""
+    from les_amd.api import stats_fields
+    fg, fo = stats_fields(g[0], nz, nx), stats_fields(o[0], nz, nx)
+    rows = {k: rel_l2(fg[k], fo[k]) for k in ("ubar", "vbar", "u2", "v2", "w2", "uw", "txz", "stress")}
+    # the plane-mean w is zero up to round-off (divergence-free flow): its
+    # difference is measured against the mean streamwise velocity
+    rows["wbar"] = float(np.linalg.norm(fg["wbar"] - fo["wbar"]) / np.linalg.norm(fo["ubar"]))
+    rows["Eoz"] = max(rel_l2(fg["Eoz"][k], fo["Eoz"][k]) for k in range(nz - 1))
+    for i, n in enumerate(("ustar_", "ustar_avg_time", "ustar_time")):
+        rows[n] = abs(g[1][i] - o[1][i]) / abs(o[1][i])
+    print(f"  {label}: " + " ".join(f"{k}={v:.1e}" for k, v in rows.items()), flush=True)
+    for k, v in rows.items():
+        t = tol_spec if k == "Eoz" else (tol_us if k.startswith("ustar") else tol_prof)
+        assert v < t, (label, k, v, t)
+    return rows
 
 
 class _Snap:
@@ -130,25 +162,32 @@ def test_neutral32_fields(neutral32, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_neutral32_statistics(neutral32, prec):
-    """After 100 steps: plane-mean profiles of u, v, txz (relative L2), the
-    plane mean of u* (obukhov's ustar_), and the streamwise spectra of u at
-    the first level and mid-height.  The north-star bar for FP32-bulk is
-    1e-4 relative L2 drift of the 100-step mean profiles."""
+@pytest.mark.parametrize("nsteps", [10, 100])
+def test_neutral32_statistics(neutral32, prec, nsteps):
+    """After 10 and 100 steps, the GPU's FP64 reductions of its state
+    (les_state_stats: plane-mean profiles of u, v, w and txz, the w-node
+    variances, <u'w'> and the total stress, E11(k1)/z on every level, and
+    obukhov's three u* values) against the same statistics of the oracle's
+    state.  The north-star bar for FP32-bulk is 1e-4 relative L2 drift of the
+    100-step mean profiles; the variances and <u'w'> are differences of
+    O(u^2) terms and are held to 1e-3."""
     nx, ny, nz, snaps = neutral32
-    g, o = snaps[100][prec], snaps[100]["oracle"]
+    g, o = snaps[nsteps][prec]["stats"], snaps[nsteps]["oracle"]["stats"]
+    if prec == "fp64":
+        rows = compare_stats(g, o, nz, nx, f"fp64 {nsteps}-step statistics", 1e-10, 1e-9, 1e-11)
+    else:
+        rows = compare_stats(g, o, nz, nx, f"fp32 {nsteps}-step statistics", 1e-3, 1e-3, 1e-4)
+        for k in ("ubar", "vbar", "txz"):
+            assert rows[k] < 1e-4, (k, rows[k])
+    # the oracle's reductions agree with numpy's on the oracle fields: the
+    # mean profile, and io.f90's spectrum (mean counted with 1/2, one-sided
+    # |c_k|^2) = half of the two-sided spectrum_x, divided by z = dz/2
     nl = nz - 1
-    tol = 1e-11 if prec == "fp64" else 1e-4
-    rows = {}
-    for name in ("u", "v", "txz"):
-        rows[name] = rel_l2(mean_profile(g[name], nx, nl), mean_profile(o[name], nx, nl))
-    rows["ustar"] = abs(g["ustar_avg"].mean() - o["ustar_avg"].mean()) / abs(o["ustar_avg"].mean())
-    for k in (0, nl // 2):
-        rows[f"E_u(k={k + 1})"] = rel_l2(spectrum_x(g["u"], nx, k), spectrum_x(o["u"], nx, k))
-    print(f"  {prec} 100-step statistics: " + " ".join(f"{k}={v:.1e}" for k, v in rows.items()), flush=True)
-    for k, v in rows.items():
-        bound = tol if not k.startswith("E_u") else (1e-9 if prec == "fp64" else 1e-3)
-        assert v < bound, (k, v, bound)
+    of = snaps[nsteps]["oracle"]
+    from les_amd.api import stats_fields
+    fo = stats_fields(o[0], nz, nx)
+    assert rel_l2(fo["ubar"], mean_profile(of["u"], nx, nl)) < 1e-14
+    assert rel_l2(fo["Eoz"][0] * (0.5 / nl), 0.5 * spectrum_x(of["u"], nx, 0)[:nx // 2]) < 1e-12
 
 
 # ---------------------------------------------------------- 128^2 x 64 ----
@@ -186,6 +225,10 @@ def test_neutral128_step_and_dynamic_update(torch_mod, prec):
     so.step(4)
     sg.step(4)
     _field_errs(sg, so, ("u", "v", "w", "p"), 128, f"{prec} step 5", 1e-11 if prec == "fp64" else 5e-5)
+    if prec == "fp64":
+        compare_stats(sg.stats(), oracle_state_stats(o, so), 65, 128, "fp64 5-step statistics", 1e-10, 1e-9, 1e-11)
+    else:
+        compare_stats(sg.stats(), oracle_state_stats(o, so), 65, 128, "fp32 5-step statistics", 1e-3, 1e-3, 1e-4)
     for name in ("F_LM", "F_MM", "F_QN", "F_NN", "Cs_opt2", "Beta"):
         a, b = sg.get(name)[:64, :, :128], so[name][:64, :, :128]
         if prec == "fp64":
