@@ -276,6 +276,51 @@ int les_state_stats(les_ctx *c, double *stats, double *surface);
 int les_state_stats_accumulate(les_ctx *c);
 int les_state_stats_average(les_ctx *c, double *stats, double *surface, int *n, int reset);
 
+/* ---- run files (host side; the state entry points copy through host
+   buffers).  Fortran sequential unformatted records: 4-byte little-endian
+   byte counts around each record; records over 2^31-9 bytes are split into
+   subrecords (leading count negative while more follow, trailing count
+   negative after the first), as the gfortran / flang runtimes do. ---- */
+/* one record of the concatenated segments; append != 0 adds it at the end */
+int les_record_write(const char *path, int append, int nseg, const void *const *seg, const uint64_t *bytes);
+/* record `index` (0 = first): its length, and up to buf_bytes of it into buf
+   (buf may be NULL) */
+int les_record_read(const char *path, int index, void *buf, uint64_t buf_bytes, uint64_t *record_bytes);
+/* checkpoint, io.f90:1314-1338 (output_final's vel_sc.out / vel.out): the
+   state as one record, real(8) throughout (FP32-bulk fields widened, theta
+   and q in the reference convention); the layout with scalars when the state
+   carries them */
+int les_state_checkpoint(les_ctx *c, const char *path);
+/* initial.f90:46-175 (initu): the first record of path read with the list
+   for model (1..7), initsc and inilag, into the state fields it names (the
+   model-7 list with initsc and without inilag does not match checkpoint's
+   layout: the reference reads such a file with its fields shifted, and so
+   does this reader).  Returns 2 for an unknown model, 3 for a record shorter
+   than the list. */
+int les_state_read_initial(les_ctx *c, const char *path, int model, int initsc, int inilag);
+/* total_time.dat: param.f90:200-210 reads jt_total (returns 1 if the file is
+   missing: the reference then takes 0); io.f90:1375-1379 writes it */
+int les_read_total_time(const char *path, int *jt_total);
+int les_write_total_time(const char *path, int jt_total);
+/* patches, bottombc.f90:23-57: wts.dat, wqs.dat, zom.dat in dir (ny lines of
+   nx free-format values, W/m^2 and m) -> wt_s (K m/s), wq_s ((g/kg) m/s), zo
+   (units of z_i), each (nx,ny) x fastest */
+int les_read_surface(const char *dir, int nx, int ny, double z_i, double *wt_s, double *wq_s, double *zo);
+/* param_nml of input.nml (param.f90:174-230): defaults of param.f90:63-85,
+   the namelist's values, jt_total from total_time.dat (NULL or missing: 0),
+   and for jt_total > 0 initu = initsc = 1, inilag = 0; dt = dt_dim u_star /
+   z_i.  Returns 3 at an unknown or malformed item (the values before it are
+   kept, as a Fortran namelist READ with iostat leaves them). */
+typedef struct les_param_nml {
+  double q_scale, T_scale, u_star, ug0, vg0, inv_strength, inv_strength_q, dTdz_top, dqdz_top, T_init, q_init,
+      dt_dim;
+  double z_i, dt;
+  int coriolis_forcing, nsteps, c_count, p_count, ubc, damping_method, initu, initsc, inilag, jt_total;
+} les_param_nml;
+int les_read_param_nml(const char *nml_path, const char *total_time_path, les_param_nml *p);
+/* one numeric or logical item of any namelist group (case-insensitive) */
+int les_read_nml_value(const char *path, const char *group, const char *key, double *value);
+
 /* Device memory and host mirrors (no reference counterpart): the Fortran
    drop-in shims keep one device mirror per host array, keyed by its address,
    and push inputs / pull outputs around each routine-level call. */

@@ -44,6 +44,15 @@ class ScalarSgsFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in SCALAR_SGS_NAMES] + [("phi_h", C.c_void_p), ("psi_m", C.c_void_p)]
 
 
+class ParamNml(C.Structure):
+    """les_param_nml (param.f90:63-85,174-230)."""
+    _fields_ = [(n, C.c_double) for n in ("q_scale", "T_scale", "u_star", "ug0", "vg0", "inv_strength",
+                                          "inv_strength_q", "dTdz_top", "dqdz_top", "T_init", "q_init", "dt_dim",
+                                          "z_i", "dt")] + \
+               [(n, C.c_int) for n in ("coriolis_forcing", "nsteps", "c_count", "p_count", "ubc", "damping_method",
+                                       "initu", "initsc", "inilag", "jt_total")]
+
+
 def declared_symbols():
     """Names of the functions include/les_amd.h declares."""
     txt = open(HEADER).read()
@@ -88,6 +97,16 @@ def load():
     L.les_ensemble_last_error.restype = C.c_char_p
     for nm in ("les_ensemble_window", "les_ensemble_step", "les_ensemble_sync"):
         getattr(L, nm).argtypes = [vp] + ([C.c_int] if nm == "les_ensemble_step" else [])
+    u64p = C.POINTER(C.c_uint64)
+    L.les_record_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp), u64p]
+    L.les_record_read.argtypes = [C.c_char_p, C.c_int, vp, C.c_uint64, u64p]
+    L.les_state_checkpoint.argtypes = [vp, C.c_char_p]
+    L.les_state_read_initial.argtypes = [vp, C.c_char_p, C.c_int, C.c_int, C.c_int]
+    L.les_read_total_time.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+    L.les_write_total_time.argtypes = [C.c_char_p, C.c_int]
+    L.les_read_surface.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, vp, vp, vp]
+    L.les_read_param_nml.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(ParamNml)]
+    L.les_read_nml_value.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
     for name in declared_symbols():
         if name not in ("les_destroy", "les_last_error", "les_state_field", "les_ensemble_destroy",
                         "les_ensemble_last_error"):

@@ -629,6 +629,16 @@ class LesSim(Les):
         self._chk(self.L.les_state_stats(self.ctx, st.ctypes.data_as(C.c_void_p), sf.ctypes.data_as(C.c_void_p)))
         return st, sf
 
+    def checkpoint(self, path):
+        """Write the state as io.f90's checkpoint record (vel_sc.out layout
+        with scalars, vel.out without)."""
+        self._chk(self.L.les_state_checkpoint(self.ctx, _b(path)))
+
+    def read_initial(self, path, model=5, initsc=0, inilag=0):
+        """Read an initial-condition record with initial.f90's list for
+        model / initsc / inilag into the state."""
+        self._chk(self.L.les_state_read_initial(self.ctx, _b(path), int(model), int(initsc), int(inilag)))
+
     def stats_accumulate(self):
         """Fold the current statistics into running means kept on the device."""
         self._chk(self.L.les_state_stats_accumulate(self.ctx))
@@ -640,6 +650,82 @@ class LesSim(Les):
         self._chk(self.L.les_state_stats_average(self.ctx, st.ctypes.data_as(C.c_void_p),
                                                  sf.ctypes.data_as(C.c_void_p), C.byref(n), int(bool(reset))))
         return st, sf, n.value
+
+
+# ---- run files (host only: no GPU needed) ----
+def _b(path):
+    return str(path).encode()
+
+
+def record_write(path, arrays, append=False):
+    """One Fortran sequential unformatted record holding the arrays (numpy,
+    in order), as a Fortran WRITE of that list would."""
+    import numpy as np
+    arrs = [np.ascontiguousarray(a) for a in arrays]
+    n = len(arrs)
+    ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+    sizes = (C.c_uint64 * n)(*[a.nbytes for a in arrs])
+    if _lib.load().les_record_write(_b(path), int(append), n, ptrs, sizes) != 0:
+        raise LesError(f"record_write failed: {path}")
+
+
+def record_read(path, index=0):
+    """Bytes of record `index` of a Fortran sequential unformatted file."""
+    L = _lib.load()
+    nb = C.c_uint64()
+    if L.les_record_read(_b(path), int(index), None, 0, C.byref(nb)) != 0:
+        raise LesError(f"record_read: no record {index} in {path}")
+    buf = C.create_string_buffer(max(nb.value, 1))
+    L.les_record_read(_b(path), int(index), buf, nb.value, C.byref(nb))
+    return buf.raw[:nb.value]
+
+
+def read_total_time(path):
+    """jt_total of total_time.dat, or 0 if the file is missing (param.f90:200-210)."""
+    v = C.c_int()
+    rc = _lib.load().les_read_total_time(_b(path), C.byref(v))
+    if rc == 1:
+        return 0
+    if rc != 0:
+        raise LesError(f"read_total_time: malformed {path}")
+    return v.value
+
+
+def write_total_time(path, jt_total):
+    if _lib.load().les_write_total_time(_b(path), int(jt_total)) != 0:
+        raise LesError(f"write_total_time failed: {path}")
+
+
+def read_surface(directory, nx, ny, z_i=1000.0):
+    """wts.dat, wqs.dat, zom.dat of a run directory (bottombc.f90:23-57):
+    (wt_s K m/s, wq_s (g/kg) m/s, zo in units of z_i), each (ny, nx)."""
+    import numpy as np
+    out = [np.zeros((ny, nx)) for _ in range(3)]
+    rc = _lib.load().les_read_surface(_b(directory), nx, ny, float(z_i), *[a.ctypes.data for a in out])
+    if rc != 0:
+        raise LesError(f"read_surface failed ({rc}) in {directory}")
+    return tuple(out)
+
+
+def read_param_nml(nml_path, total_time_path=None):
+    """param_nml of input.nml with the reference's defaults and restart rule
+    (param.f90:174-230) as a dict; status 3 (an unknown item) is returned
+    beside it."""
+    from ._lib import ParamNml
+    p = ParamNml()
+    rc = _lib.load().les_read_param_nml(None if nml_path is None else _b(nml_path),
+                                        None if total_time_path is None else _b(total_time_path), C.byref(p))
+    if rc not in (0, 3):
+        raise LesError(f"read_param_nml failed ({rc}): {nml_path}")
+    return {f[0]: getattr(p, f[0]) for f in p._fields_}, rc
+
+
+def read_nml_value(path, group, key):
+    v = C.c_double()
+    rc = _lib.load().les_read_nml_value(_b(path), group.encode(), key.encode(), C.byref(v))
+    if rc != 0:
+        raise LesError(f"read_nml_value: {group}/{key} not found in {path} ({rc})")
+    return v.value
 
 
 SURFACE_NAMES = ("ustar_", "ustar_avg_time", "ustar_time", "theta_", "thetav_", "q_time", "wt_", "L_")

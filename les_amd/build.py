@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libles_amd.so")
 SOURCES = ["les_kernels.hip", "les_scalars.hip", "les_dynamic.hip", "les_mlp.hip", "les_coltx.hip", "les_capi.hip",
-           "les_noalloc.hip", "les_stats.hip"]
+           "les_noalloc.hip", "les_stats.hip", "les_io.hip"]
 ARCH = os.environ.get("LES_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
