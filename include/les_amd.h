@@ -103,12 +103,48 @@ int les_lagrange_sdep(les_ctx *c, double *const S[6], const double *u, const dou
                       const les_sgs_fields *s);
 int les_interpolag_sdep(les_ctx *c, const les_sgs_fields *s);
 
-/* ---- SGS model selection, param.f90 'model' (sgs_stag.f90:128-160):
-   5 = Lagrangian scale-dependent (default), 3 = scale-dependent dynamic with
-   plane-averaged coefficients.  Model 3 keeps no Lagrangian averages, fills
-   Cs_opt2 level by level and has no jt_total < 500 constant-Cs period; it
-   has no scalar SGS model, so it cannot be combined with les_set_scalars. ---- */
+/* ---- SGS model selection, param.f90 'model' (sgs_stag.f90:56-175):
+   1 = Smagorinsky, Cs_opt2 = Co^2 = 0.16^2 every step with Mason's
+       wall-damped length l(jz) (sgs_stag.f90:73-104);
+   2 = standard dynamic, plane-averaged (std_dynamic.f90);
+   3 = scale-dependent dynamic with plane-averaged coefficients
+       (scaledep_dynamic.f90);
+   4 = Lagrangian scale-invariant (lagrange_Ssim.f90, interpolag_Ssim.f90);
+   5 = Lagrangian scale-dependent (default).
+   u_lag accumulates under models 4 and 5 only; the jt_total < 500 constant-Cs
+   period applies to model 5 only (:165-174).  The scalar Lagrangian model
+   (scalar model 7) needs SGS model 5; with the others the scalars use the
+   constant Pr (scalars_module.f90:77-84).  Returns 1 for another model. ---- */
 int les_set_sgs_model(les_ctx *c, int model);
+/* ---- std_dynamic.f90:4-140: Cs_opt2 (nz device doubles) of the standard
+   dynamic model from S11..S33 and u, v, w (ld,ny,nz), Delta = (dx dy dz)^(1/3) ---- */
+int les_std_dynamic(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                    double *Cs_opt2);
+/* ---- lagrange_Ssim.f90:4-300 (calls interpolag_Ssim first) and
+   interpolag_Ssim.f90 on the F_LM, F_MM, Beta, Cs_opt2, u_lag.. members of s;
+   Beta_avg and Betaclip_avg receive nx*ny, the plane sums of Beta = 1 ---- */
+int les_lagrange_ssim(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                      const les_sgs_fields *s);
+int les_interpolag_ssim(les_ctx *c, const les_sgs_fields *s);
+
+/* ---- Coriolis forcing and the top sponge layer (param.f90:68-75,220,
+   main.f90:131-138,172-185,319-348, topbc.f90:14-95, scalars_module.f90:143-148,
+   scalars_module_q.f90:116-121).  With coriolis_forcing the step adds
+   coriol (v - vg) to RHSx and -coriol (u - ug) to RHSy, coriol =
+   1.3962634e-4 z_i/u_star, ug = ug0/u_star, vg = vg0/u_star.  With ubc = 1
+   the sponge of damping_method 1 relaxes w in the top quarter, that of
+   damping_method 2 relaxes u, v, w (and theta, q when scalars run) to their
+   plane means above 0.75 L_z, L_z = Lz z_i.  Returns 1 on an invalid switch. ---- */
+typedef struct les_forcing {
+  int coriolis_forcing;
+  double ug0, vg0;      /* geostrophic wind, m/s */
+  double u_star, z_i;   /* m/s, m */
+  int ubc;              /* 0: no sponge, 1: sponge */
+  int damping_method;   /* 1 or 2 */
+} les_forcing;
+int les_set_forcing(les_ctx *c, const les_forcing *f);
+/* setsponge's profile sponge(1..nz) (host doubles, nz) */
+int les_get_sponge(les_ctx *c, double *sponge);
 /* ---- scaledep_dynamic.f90:4-255 with rtnewt (sgsmodule.f90:37-59).
    S: the routine's dummy arguments S11, S12, S13, S22, S23, S33 (ld,ny,nz);
    sgs_stag.f90:132 binds them to u, v, w, S11, S12, S13.  u, v, w are

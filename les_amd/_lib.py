@@ -53,6 +53,12 @@ class ParamNml(C.Structure):
                                        "initu", "initsc", "inilag", "jt_total")]
 
 
+class Forcing(C.Structure):
+    """les_forcing (param.f90:68-75, topbc.f90)."""
+    _fields_ = [("coriolis_forcing", C.c_int), ("ug0", C.c_double), ("vg0", C.c_double), ("u_star", C.c_double),
+                ("z_i", C.c_double), ("ubc", C.c_int), ("damping_method", C.c_int)]
+
+
 def declared_symbols():
     """Names of the functions include/les_amd.h declares."""
     txt = open(HEADER).read()
@@ -107,6 +113,8 @@ def load():
     L.les_read_surface.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, vp, vp, vp]
     L.les_read_param_nml.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(ParamNml)]
     L.les_read_nml_value.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
+    L.les_set_forcing.argtypes = [vp, C.POINTER(Forcing)]
+    L.les_get_sponge.argtypes = [vp, vp]
     for name in declared_symbols():
         if name not in ("les_destroy", "les_last_error", "les_state_field", "les_ensemble_destroy",
                         "les_ensemble_last_error"):

@@ -11,6 +11,8 @@ gradients, F_LM ...) are passed explicitly.  Errors the reference turned into
 import ctypes as C
 import math
 
+import numpy as np
+
 import torch
 
 from . import _lib
@@ -255,8 +257,40 @@ class Les:
         self._chk(self.L.les_interpolag_sdep(self.ctx, C.byref(st)))
 
     def set_sgs_model(self, model):
-        """param.f90 'model': 5 (Lagrangian scale-dependent) or 3 (plane-averaged)."""
+        """param.f90 'model': 1 Smagorinsky with Mason's wall damping, 2 standard
+        dynamic, 3 plane-averaged scale-dependent, 4 Lagrangian scale-invariant,
+        5 Lagrangian scale-dependent."""
         self._chk(self.L.les_set_sgs_model(self.ctx, int(model)))
+
+    def std_dynamic(self, S, u, v, w):
+        """std_dynamic.f90: the per-level Cs_opt2 as an (nz,) float64 GPU tensor."""
+        s = self._arr(S, self.S_NAMES)
+        cs = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        self._chk(self.L.les_std_dynamic(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"), _ptr(cs)))
+        return cs
+
+    def lagrange_ssim(self, S, u, v, w, sgs):
+        s = self._arr(S, self.S_NAMES)
+        st = self._sgs_struct(sgs)
+        self._chk(self.L.les_lagrange_ssim(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                           C.byref(st)))
+
+    def interpolag_ssim(self, sgs):
+        st = self._sgs_struct(sgs)
+        self._chk(self.L.les_interpolag_ssim(self.ctx, C.byref(st)))
+
+    def set_forcing(self, coriolis=0, ug0=0.0, vg0=0.0, u_star=0.45, z_i=1000.0, ubc=0, damping_method=1):
+        """Coriolis forcing toward the geostrophic wind (ug0, vg0) m/s and the
+        top sponge (ubc=1, damping_method 1 or 2), main.f90:319-348."""
+        self._forcing = _lib.Forcing(int(coriolis), float(ug0), float(vg0), float(u_star), float(z_i), int(ubc),
+                                     int(damping_method))
+        self._chk(self.L.les_set_forcing(self.ctx, C.byref(self._forcing)))
+
+    def sponge(self):
+        """setsponge's sponge(1..nz) as an (nz,) numpy array."""
+        sp = np.zeros(self.nz)
+        self._chk(self.L.les_get_sponge(self.ctx, sp.ctypes.data_as(C.c_void_p)))
+        return sp
 
     def scaledep_dynamic(self, S, u, v, w):
         """scaledep_dynamic.f90 on the dummies S (S11..S33 in that order):
@@ -536,10 +570,11 @@ class Les:
 class LesSim(Les):
     """Whole-step driver on context-owned device state (main.f90:202-415)."""
 
-    def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, sgs_model=5, **kw):
+    def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, sgs_model=5, forcing=None, **kw):
         """scalars: None (neutral) or a dict of set_scalars arguments (the
         convective step with theta, q, buoyancy and the scalar SGS model).
-        sgs_model: 5 (Lagrangian scale-dependent) or 3 (plane-averaged)."""
+        sgs_model: param.f90 'model', 1..5 (set_sgs_model).  forcing: None or a
+        dict of set_forcing arguments (Coriolis, sponge)."""
         # a stream of its own (default priority): graph capture needs a
         # non-default stream, and the state is only touched through the library
         self.stream = torch.cuda.Stream()
@@ -548,6 +583,8 @@ class LesSim(Les):
             self.set_sgs_model(sgs_model)
         if scalars is not None:
             self.set_scalars(**scalars)
+        if forcing is not None:
+            self.set_forcing(**forcing)
         self.precision = precision
         self._chk(self.L.les_state_alloc(self.ctx, 1 if precision == "fp32" else 0))
         self._chk(self.L.les_set_graph(self.ctx, 1 if graph else 0))

@@ -303,6 +303,11 @@ static void step_once(les_ctx* c) {
   op_divstress<T>(c, dz, tau[2], tau[4], tau[5], 1);
   T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
   const int euler = (c->jt == 1 && !g.initu) ? 1 : 0;
+  // Coriolis and sponge (main.f90:319-348, scalars_module.f90:143-148): the
+  // products and plane means of the fields before the update
+  const ForcingArgs<T> fo = op_forcing_pre<T>(c, u, v, w, scal ? (const T*)F<T>(c, "theta") : (const T*)nullptr,
+                                              scal ? (const T*)F<T>(c, "qmix") : (const T*)nullptr);
+  const ForcingArgs<T>* fop = (fo.cor[0] || fo.damp) ? &fo : nullptr;
   if (scal) {
     // convec + RHS assembly + AB2 (main.f90:304-390) and the scalars'
     // scalar_RHS_calc + step_scalar in one padded-grid pass
@@ -314,15 +319,72 @@ static void step_once(les_ctx* c) {
     op_convec_scalar_ab2<T>(c, u, v, w, grc2, R3, Rf3, dv3, euler,
                             P.passive ? (const T*)nullptr : (const T*)F<T>(c, "beta_scal"), jobs, F<T>(c, "Nu_t"),
                             P.model == 7 ? (const T*)F<T>(c, "Pr_t") : (const T*)nullptr,
-                            (c->jt == 1 && !P.initsc) ? 1 : 0);
+                            (c->jt == 1 && !P.initsc) ? 1 : 0, fop);
   } else {
     // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
     op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
-                     F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, euler);
+                     F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, euler, nullptr, fop);
   }
   T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
   op_press_stag<T>(c, u, v, w, dz, p, px, py);
   op_pressure_update<T>(c, u, v, w, Rx, Ry, Rz, p, px, py, pz);
+}
+
+// Mason's wall-damped mixing length of model 1, sgs_stag.f90:84-94
+// (lbc_mom 'wall', Co = 0.16, nnn = 2, vonk = 0.41): l(jz)^2 per level, z on
+// the first uv-node at jz = 1 and on w-nodes above.  Integer powers as
+// products, the -1/nnn power by pow.
+std::vector<double> mason_l2(const Grid& g) {
+#pragma clang fp contract(off)
+  const double Co = 0.16, vonk = 0.41;
+  std::vector<double> l2(g.nz);
+  for (int k = 0; k < g.nz; ++k) {
+    const double zz = k == 0 ? (1.0 - 0.5) * g.dz : (double)k * g.dz;
+    const double a = vonk * zz;
+    const double l = pow(Co * Co * (1.0 / (a * a)) + 1.0 / (g.delta * g.delta), -1.0 / 2.0);
+    l2[k] = l * l;
+  }
+  return l2;
+}
+
+// coriol (param.f90:220, a default-real literal), ug, vg (main.f90:131-138)
+// and setsponge (topbc.f90:14-74, nproc = 1) with L_z = Lz z_i
+void set_forcing_constants(const Grid& g, Forcing& F) {
+#pragma clang fp contract(off)
+  const int nz = g.nz;
+  F.coriol = (double)1.3962634E-04f * F.z_i / F.u_star;
+  F.cug = F.coriol * (F.ug0 / F.u_star);
+  F.cvg = F.coriol * (F.vg0 / F.u_star);
+  F.sponge.assign(nz + 1, 0.0);
+  if (F.ubc != 1) return;
+  double* sp = F.sponge.data();
+  const double z_i = F.z_i, L_z = g.Lz * z_i, dz = g.dz;
+  if (F.damping_method == 2) {
+    const double z_d = 0.75 * (L_z * 1.0), cfrdmp = 3.9;
+    for (int k = 1; k <= nz - 1; ++k) {
+      const double z_local = ((double)k - 0.5) * dz * z_i;
+      sp[k] = (z_local >= z_d && z_local <= L_z * 1.0)
+                  ? cfrdmp * 0.5 * (1.0 - cos(PI * (z_local - z_d) / (L_z - z_d)))
+                  : 0.0;
+    }
+    sp[nz] = sp[nz - 1];
+  } else {
+    // nz - 0.75*nz + 1 and the loop bound 0.75*nz + 1 are default-real
+    // expressions; the bound is truncated to an integer
+    const float den = (float)nz - 0.75f * (float)nz + 1.0f;
+    const double factor = 9.0 / (double)den;
+    sp[nz] = z_i / (50.0 * F.u_star);
+    const int kend = (int)(0.75f * (float)nz + 1.0f);
+    for (int k = nz - 1; k >= kend; --k) sp[k] = sp[k + 1] / pow(5.0, factor);
+  }
+}
+
+// a captured step window bakes the model and forcing switches in
+static void drop_graph(les_ctx* c) {
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
 }
 
 }  // namespace les
@@ -397,6 +459,8 @@ void les_destroy(les_ctx* c) {
   les::mlp_free(c);
   if (c->wt_s) (void)hipFree(c->wt_s);
   if (c->wq_s) (void)hipFree(c->wq_s);
+  if (c->l2z) (void)hipFree(c->l2z);
+  if (c->frc.dcoef) (void)hipFree(c->frc.dcoef);
   delete c;
 }
 
@@ -608,15 +672,92 @@ int les_lagrange_sdep(les_ctx* c, double* const S[6], const double* u, const dou
 }
 int les_set_sgs_model(les_ctx* c, int model) {
   GUARD(c);
-  if (model != 3 && model != 5) {
-    c->err = "les_set_sgs_model: model must be 3 (plane-averaged scale-dependent) or 5 (Lagrangian)";
+  if (model < 1 || model > 5) {
+    c->err = "les_set_sgs_model: model must be 1 (Smagorinsky-Mason), 2 (standard dynamic), 3 (plane-averaged "
+             "scale-dependent), 4 (Lagrangian scale-invariant) or 5 (Lagrangian scale-dependent)";
     return 1;
   }
-  if (model == 3 && c->scal.enable) {
-    c->err = "les_set_sgs_model: model 3 has no scalar SGS model (sgs_stag.f90:150-153 runs under model 5 only)";
+  if (model != 5 && c->scal.enable && c->scal.model == 7) {
+    c->err = "les_set_sgs_model: the scalar Lagrangian model (model 7) runs with lagrange_Sdep, SGS model 5 "
+             "(sgs_stag.f90:150-153)";
     return 1;
+  }
+  if (model != 5 && c->closure_mode) {
+    c->err = "les_set_sgs_model: the learned closure replaces the model-5 update; switch it off first";
+    return 1;
+  }
+  if (model == 1) {
+    const Grid& g = c->g;
+    std::vector<double> l2 = mason_l2(g);
+    if (!c->l2z) check(c, hipMalloc((void**)&c->l2z, g.nz * sizeof(double)), "hipMalloc l2z");
+    if (c->err.empty()) check(c, hipMemcpy(c->l2z, l2.data(), g.nz * sizeof(double), hipMemcpyHostToDevice), "l2z");
+    if (!c->err.empty()) return 1;
   }
   c->sgs_model = model;
+  drop_graph(c);
+  DONE(c);
+}
+int les_std_dynamic(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                    double* Cs_opt2) {
+  GUARD(c);
+  if (!S || !Cs_opt2) return -1;
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_std_dynamic<double>(c, Sc, u, v, w, Cs_opt2, (double*)nullptr);
+  DONE(c);
+}
+int les_lagrange_ssim(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                      const les_sgs_fields* s) {
+  GUARD(c);
+  if (!S || !s) return -1;
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_lagrange_ssim<double>(c, Sc, u, v, w, to_ptrs(s));
+  DONE(c);
+}
+int les_interpolag_ssim(les_ctx* c, const les_sgs_fields* s) {
+  GUARD(c);
+  if (!s) return -1;
+  op_interpolag_ssim<double>(c, to_ptrs(s));
+  DONE(c);
+}
+
+// Coriolis forcing and the top sponge
+int les_set_forcing(les_ctx* c, const les_forcing* f) {
+  GUARD(c);
+  if (!f) return -1;
+  if (f->ubc != 0 && f->ubc != 1) { c->err = "les_set_forcing: ubc must be 0 or 1"; return 1; }
+  if (f->ubc == 1 && f->damping_method != 1 && f->damping_method != 2) {
+    c->err = "les_set_forcing: damping_method must be 1 or 2";
+    return 1;
+  }
+  if ((f->coriolis_forcing || f->ubc == 1) && !(f->u_star > 0.0 && f->z_i > 0.0)) {
+    c->err = "les_set_forcing: u_star and z_i must be positive";
+    return 1;
+  }
+  const Grid& g = c->g;
+  const int nl = g.nz - 1;
+  Forcing& F = c->frc;
+  F.coriolis = f->coriolis_forcing ? 1 : 0;
+  F.ubc = f->ubc;
+  F.damping_method = f->damping_method;
+  F.ug0 = f->ug0; F.vg0 = f->vg0; F.u_star = f->u_star; F.z_i = f->z_i;
+  set_forcing_constants(g, F);
+  std::vector<double> coef(2 * (size_t)nl, 0.0);
+  for (int k = 0; k < nl; ++k) {
+    coef[k] = 0.5 * (F.sponge[k + 1] + F.sponge[k + 2]);
+    coef[nl + k] = F.damping_method == 2 ? 0.5 * F.sponge[k + 1] : F.sponge[k + 1];
+  }
+  if (!F.dcoef) check(c, hipMalloc((void**)&F.dcoef, coef.size() * sizeof(double)), "hipMalloc forcing");
+  if (c->err.empty())
+    check(c, hipMemcpy(F.dcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice), "forcing");
+  if (!c->err.empty()) return 1;
+  drop_graph(c);
+  DONE(c);
+}
+int les_get_sponge(les_ctx* c, double* sponge) {
+  GUARD(c);
+  if (!sponge) return -1;
+  const int nz = c->g.nz;
+  for (int k = 0; k < nz; ++k) sponge[k] = c->frc.sponge.empty() ? 0.0 : c->frc.sponge[k + 1];
   DONE(c);
 }
 int les_scaledep_dynamic(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
@@ -1589,8 +1730,8 @@ int les_profile_report(les_ctx* c, char* buf, int buflen) {
 int les_set_scalars(les_ctx* c, const les_scalar_params* p, const double* wt_s, const double* wq_s) {
   GUARD(c);
   if (!p) return -1;
-  if (p->enable && c->sgs_model == 3) {
-    c->err = "les_set_scalars: SGS model 3 has no scalar SGS model; select model 5 first";
+  if (p->enable && p->model == 7 && c->sgs_model != 5) {
+    c->err = "les_set_scalars: the scalar Lagrangian model (model 7) runs with lagrange_Sdep; select SGS model 5";
     return 1;
   }
   ScalarParams& P = c->scal;
@@ -1697,7 +1838,7 @@ int les_sgs_stag_scalar(les_ctx* c, double* const grad[9], const double* u, cons
   GUARD(c);
   if (!m || !s) return -1;
   if (!c->wt_s) { c->err = "les_sgs_stag_scalar: call les_set_scalars first"; return 1; }
-  if (c->sgs_model == 3) { c->err = "les_sgs_stag_scalar: the scalar SGS model needs SGS model 5"; return 1; }
+  if (c->sgs_model != 5) { c->err = "les_sgs_stag_scalar: the scalar SGS model needs SGS model 5"; return 1; }
   const double* g[9];
   for (int i = 0; i < 9; ++i) g[i] = grad[i];
   const les::ScalarSgsPtrs ss = to_sptrs(s);

@@ -250,9 +250,45 @@ void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v
   LCHK(c, hipGetLastError());
 }
 
+// SGS model 2, std_dynamic.f90:4-140: the L.M / M.M sums of the
+// scale-dependent model with beta = 1 (M_ij = 2 Delta^2 (|S|S_ij bar - 4
+// |S_bar| S_ij bar), :124-132), Delta = (dx dy dz)^(1/3) (:28), over the
+// same 2-Delta filter set
+template <typename T>
+void op_std_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                    T* Cs_planes) {
+  OpTag _tag(c, "std_dynamic");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  const T* F2 = lag_filter_mom<T>(c, S, u, v, w);
+  double* part = (double*)scratch(c, S_RED, (size_t)SD_NPART * nz * 2 * sizeof(double));
+  double* lev = (double*)scratch(c, S_SDLEV, 2 * (size_t)nz * sizeof(double));
+  double* beta = lev;
+  double* cs = cs_lev ? cs_lev : lev + nz;
+  op_fill<double>(c, beta, nz, 1.0);
+  const double delta = pow(g.dx * g.dy * g.dz, 1.0 / 3.0);
+  {
+    ProfScope _ps(c, "sd_lm", 21.0 * (double)g.nx * g.ny * nz * sizeof(T));
+    hipLaunchKernelGGL((k_sd_lm<T>), dim3(SD_NPART, nz), dim3(256), 0, c->stream, F2, (const T*)nullptr,
+                       (const T*)nullptr, (long)n3, g.nx, g.ld, g.ny, (const double*)beta, 2.0 * (delta * delta),
+                       part);
+  }
+  LCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_sd_cs, dim3((nz + 63) / 64), dim3(64), 0, c->stream, (const double*)part, SD_NPART, nz, cs);
+  LCHK(c, hipGetLastError());
+  if (Cs_planes) {
+    ProfScope _ps(c, "sd_fill", (double)n3 * sizeof(T));
+    hipLaunchKernelGGL((k_sd_fill<T>), grid1(n3), dim3(256), 0, c->stream, (const double*)cs, Cs_planes,
+                       (long)g.ld * g.ny, (long)n3);
+  }
+  LCHK(c, hipGetLastError());
+}
+
 #define LES_SD_INST(T)                                                                                      \
   template void op_scaledep_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*, \
-                                       double*, T*);
+                                       double*, T*);                                                        \
+  template void op_std_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*, T*);
 LES_SD_INST(double)
 LES_SD_INST(float)
 #undef LES_SD_INST

@@ -99,6 +99,30 @@ struct ScalarParams {
   double theta_ref = 0.0, q_ref = 0.0;
 };
 
+// Coriolis forcing and the top sponge (param.f90:68-75,220, main.f90:131-138,
+// 319-348, topbc.f90:14-95), set by les_set_forcing
+struct Forcing {
+  int coriolis = 0, ubc = 0, damping_method = 1;
+  double ug0 = 0.0, vg0 = 0.0, u_star = 1.0, z_i = 1000.0;
+  double coriol = 0.0, cug = 0.0, cvg = 0.0;  // coriol, coriol*ug, coriol*vg
+  std::vector<double> sponge;                 // sponge(1..nz) at [1..nz]
+  double* dcoef = nullptr;  // device, 2 (nz-1): 0.5 (sponge(jz) + sponge(jz+1)) | the RHSz factor
+};
+
+// What the AB2 epilogues add after the buoyancy term, per step: the
+// Coriolis products of the velocities before the update (formed by
+// op_forcing_pre, because the epilogue updates u and v in place) and the
+// sponge terms about the plane means of the same fields.
+template <typename T>
+struct ForcingArgs {
+  const T* cor[2] = {nullptr, nullptr};  // coriol*v (for RHSx), coriol*u (for RHSy); levels < nz-1
+  T cvg = T(0), cug = T(0);
+  int damp = 0;                  // damping_method with ubc = 1, else 0
+  const double* coef = nullptr;  // Forcing::dcoef
+  const double* sums = nullptr;  // plane sums of u, v, w, theta, q on levels < nz-1 ([5][nz-1])
+  double nxy = 1.0;
+};
+
 struct State;  // whole-step device state (les_capi.hip)
 
 struct ProfRec {
@@ -139,7 +163,11 @@ struct les_ctx {
   double zo_avg = 0.0;             // exp(mean log zo), lagrange_scalar_Sdep.f90:110
   double wt_mean = 0.0;            // sum(wt_s)/(nx*ny), lagrange_scalar_Sdep.f90:114
   int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
-  int sgs_model = 5;               // param.f90 'model': 5 Lagrangian scale-dependent, 3 plane-averaged
+  int sgs_model = 5;               // param.f90 'model': 1 Smagorinsky-Mason, 2 standard dynamic,
+                                   // 3 plane-averaged scale-dependent, 4 Lagrangian scale-invariant,
+                                   // 5 Lagrangian scale-dependent
+  double* l2z = nullptr;           // model 1: l(jz)^2 of sgs_stag.f90:84-94, device (nz)
+  les::Forcing frc;
   les::Tune tune;
   int lag_cs = 0;                  // steps between Lagrangian updates when it differs from cs_count (closure)
   int last_lag_jt = -1;            // jt of the last lagrange_Sdep update
@@ -159,7 +187,7 @@ enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
   S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_TUNE0, S_TUNE1, S_TUNE2,
-  S_STATP, S_STATX, S_STAT, S_STATAVG, S_NSLOTS
+  S_STATP, S_STATX, S_STAT, S_STATAVG, S_FORCE, S_FSUM, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 void scratch_release(les_ctx* c, int slot);  // give a slot's memory back (stream drained first)
@@ -189,7 +217,12 @@ void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, co
 template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
-                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta = nullptr);
+                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta = nullptr,
+                   const ForcingArgs<T>* fo = nullptr);
+// the per-step inputs of the forcing terms (les_scalars.hip): Coriolis
+// products and plane sums of the fields before the AB2 update; theta/q may be null
+template <typename T>
+ForcingArgs<T> op_forcing_pre(les_ctx* c, const T* u, const T* v, const T* w, const T* theta, const T* q);
 template <typename T>
 void op_wallstress(les_ctx* c, const T* u, const T* v, T* dudz, T* dvdz, T* txz, T* tyz,
                    const double* psi_m, const double* psi_m0, const double* phi_m, double* ustar_avg);
@@ -208,6 +241,11 @@ struct ScalarSgsPtrs {
 template <typename T> void op_calc_sij(les_ctx* c, const T* const g[9], T* const S[6]);
 template <typename T> void op_interpolag(les_ctx* c, const SgsPtrs& s);
 template <typename T> void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl);
+// interpolag_Ssim.f90 on F_LM, F_MM (model 4)
+template <typename T> void op_interpolag_ssim(les_ctx* c, const SgsPtrs& s);
+// lagrange_Ssim.f90:4-300 (model 4)
+template <typename T>
+void op_lagrange_ssim(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s);
 // fields f0..20 of the lagrange_Sdep filter set (f0 = 9: strain rates only)
 template <typename T>
 T* lag_filter_mom(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, int f0 = 0);
@@ -228,6 +266,11 @@ void op_sgs_stag(les_ctx* c, const T* const g[9], const T* u, const T* v, const 
 template <typename T>
 void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
                          double* beta_lev, T* Cs_planes);
+// SGS model 2 (les_dynamic.hip): std_dynamic.f90:4-140, per-level Cs_opt2 to
+// cs_lev (device, nz; null: scratch) and, with Cs_planes, to every point
+template <typename T>
+void op_std_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                    T* Cs_planes);
 
 // ---- learned closure (les_mlp.hip) ----
 int mlp_nparam();
@@ -281,7 +324,7 @@ void op_pad_products_probe(les_ctx* c, int scal);
 template <typename T>
 void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
                           const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
-                          const T* Nu_t, const T* Pr_t, int scal_euler);
+                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo = nullptr);
 // lagrange_scalar_Sdep.f90:1-467 (theta Nyquist-filtered in place); with
 // reuse_mom the momentum model's filtered fields of this step are reused
 template <typename T>

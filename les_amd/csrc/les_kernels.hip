@@ -421,13 +421,15 @@ void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, co
 template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
-                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta) {
+                   const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta,
+                   const ForcingArgs<T>* fo) {
   OpTag _tag(c, "convec");
   const Grid& g = c->g;
   const int nz = g.nz;
   cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy);
   StoreConvecAB2<T> SC{{RHSx, RHSy, RHSz}, {RHSx_f, RHSy_f, RHSz_f}, {divtx, divty, divtz}, {u, v, w},
                        nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
+  if (fo) SC.fo = *fo;
   launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * (nz - 1) * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
   T* tops[9] = {RHSx + top, RHSy + top, RHSz + top, u + top, v + top, w + top, RHSx_f + top, RHSy_f + top,
@@ -546,7 +548,8 @@ template <typename T>
 __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, const T* __restrict__ S13,
                           const T* __restrict__ S22, const T* __restrict__ S23, const T* __restrict__ S33,
                           const T* __restrict__ Cs, T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy,
-                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2, T* Pr_t, T Pr) {
+                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2_const,
+                          const double* __restrict__ l2z, T* Pr_t, T Pr) {
   const long pl = (long)ld * ny, n = pl * nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const LevelCol kx_ = level_col(i, pl, ld);
@@ -558,6 +561,7 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
       txz[i] = T(0); tyz[i] = T(0);
     }
     if (x >= nx) continue;
+    const T l2 = l2z ? (T)l2z[k] : l2_const;  // model 1: l(jz)^2 (sgs_stag.f90:84-94), else delta^2
     const T c0 = use_const ? cs_const : Cs[i];
     const T nu0 = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]) * c0 * l2;
     Nu_t[i] = nu0;
@@ -571,7 +575,8 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
     } else {
       const long p = i + pl;
       const T c1 = use_const ? cs_const : Cs[p];
-      const T nu1 = smag(S11[p], S12[p], S13[p], S22[p], S23[p], S33[p]) * c1 * l2;
+      const T l2p = l2z ? (T)l2z[k + 1] : l2_const;
+      const T nu1 = smag(S11[p], S12[p], S13[p], S22[p], S23[p], S33[p]) * c1 * l2p;
       const T cst = T(0.5) * (nu0 + nu1);
       txx[i] = T(-2) * (cst + T(0)) * T(0.5) * (S11[i] + S11[p]);
       txy[i] = T(-2) * (cst + T(0)) * T(0.5) * (S12[i] + S12[p]);
@@ -585,8 +590,12 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
 
 // ------------------------------------------------------------------------
 // interpolag_Sdep.f90: Lagrangian back-trajectory + trilinear interpolation
+// of NF fields.  SIM: interpolag_Ssim.f90, whose two lowest levels keep the
+// doubled displacement without the clamp to [-1, 1] (and level 1 its own
+// 0.25 w_lag(2) value) where interpolag_Sdep.f90:264-269 clamps both to
+// the clamped level-2 value.
 // ------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NF = 4, bool SIM = false>
 __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl,
                              const T* __restrict__ FF, T* FLM, T* FMM, T* FQN, T* FNN, int nx, int ld, int ny,
                              int nz, T cs, T dt, T dx, T dy, T dz) {
@@ -605,7 +614,16 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
     const T xp = -uu * dt * cs / dx;
     const T yp = -vv * dt * cs / dy;
     T zp;
-    if (k <= 1) {  // :243-295: level 1 = 0.25 w_lag(2); then wall doubling + clamps
+    if (SIM && k <= 1) {  // interpolag_Ssim.f90: doubling only
+      const long i2 = (long)pl + (long)y * ld + x;
+      if (k == 1) {
+        const T z2 = -(wl[i2] / cs) * dt * cs / dz;
+        zp = z2 + (z2 < T(0) ? z2 : T(0));
+      } else {
+        const T z1 = -(T(0.25) * (wl[i2] / cs)) * dt * cs / dz;
+        zp = z1 + (z1 > T(0) ? z1 : T(0));
+      }
+    } else if (k <= 1) {  // :243-295: level 1 = 0.25 w_lag(2); then wall doubling + clamps
       const long i2 = (long)pl + (long)y * ld + x;
       T z2 = -(wl[i2] / cs) * dt * cs / dz;
       z2 = z2 + (z2 < T(0) ? z2 : T(0));
@@ -627,7 +645,7 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
     const long ixy0 = (long)k * pl + (long)ya * ld + xa, ixyz = (long)za * pl + (long)ya * ld + xa;
     T* outs[4] = {FLM, FMM, FQN, FNN};
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < NF; ++f) {
       const T* G = FF + f * n3;
       outs[f][i] = fx * fy * (G[i000] * fz + G[i00z] * czv) + fx * cyv * (G[i0y0] * fz + G[i0yz] * czv) +
                    cxv * fy * (G[ix00] * fz + G[ix0z] * czv) + cxv * cyv * (G[ixy0] * fz + G[ixyz] * czv);
@@ -660,6 +678,28 @@ void op_interpolag(les_ctx* c, const SgsPtrs& s) {
   OpTag _tag(c, "interpolag");
   T* F[4] = {(T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN};
   op_interpolag4<T>(c, F, (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag);
+}
+
+// interpolag_Ssim.f90 on (F_LM, F_MM)
+template <typename T>
+void op_interpolag_ssim(les_ctx* c, const SgsPtrs& s) {
+  OpTag _tag(c, "interpolag_ssim");
+  const Grid& g = c->g;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  T* FF = (T*)scratch(c, S_FF, 2 * n3 * sizeof(T));
+  op_copy<T>(c, FF, (T*)s.F_LM, n3);
+  op_copy<T>(c, FF + n3, (T*)s.F_MM, n3);
+  {
+    ProfScope _ps(c, "interpolag", (2.0 + 3 + 2) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_interpolag<T, 2, true>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
+                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, FF, (T*)s.F_LM, (T*)s.F_MM,
+                       (T*)nullptr, (T*)nullptr, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy),
+                       T(g.dz));
+  }
+  LCHK(c, hipGetLastError());
+  op_fill<T>(c, (T*)s.u_lag, n3, T(0));
+  op_fill<T>(c, (T*)s.v_lag, n3, T(0));
+  op_fill<T>(c, (T*)s.w_lag, n3, T(0));
 }
 
 // ------------------------------------------------------------------------
@@ -899,28 +939,105 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   if (init_qn) c->fqn_init = 1;
 }
 
+// lagrange_Ssim.f90:142-271 from the 2-Delta filter set: Beta = 1, no
+// single-precision rounding, eps = 1e-32, F_LM initialised to 0.025 MM.
+// Every point of the (ld, ny) planes, pads included, like the reference.
+template <typename T>
+__global__ void k_lag_ssim_epilogue(const T* __restrict__ F2, T* FLM, T* FMM, T* Beta, T* Cs, int nx, int ld, int ny,
+                                    int nz, double cst, double opftdelta, double lagran_dt, int init) {
+#pragma clang fp contract(off)
+  const long pl = (long)ld * ny, n3 = pl * nz;
+  const double eps = 1.e-32;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const bool pad = kx_.x >= nx;
+    auto f2 = [&](int f) { return (double)F2[f * n3 + i]; };
+    const double ub = f2(0), vb = f2(1), wb = f2(2);
+    double Lv[6];
+    Lv[0] = f2(3) - ub * ub; Lv[1] = f2(4) - ub * vb; Lv[2] = f2(5) - ub * wb;
+    Lv[3] = f2(6) - vb * vb; Lv[4] = f2(7) - vb * wb; Lv[5] = f2(8) - wb * wb;
+    double sb[6], ssb[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { sb[m] = f2(9 + m); ssb[m] = f2(15 + m); }
+    const double Sb = sqrt(2.0 * (sb[0] * sb[0] + sb[3] * sb[3] + sb[5] * sb[5] +
+                                  2.0 * (sb[1] * sb[1] + sb[2] * sb[2] + sb[4] * sb[4])));
+    double M[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) M[m] = cst * (ssb[m] - 4.0 * Sb * sb[m]);  // fourbeta = 4 Beta = 4
+    const double LM = Lv[0] * M[0] + Lv[3] * M[3] + Lv[5] * M[5] + 2.0 * (Lv[1] * M[1] + Lv[2] * M[2] + Lv[4] * M[4]);
+    const double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+    double flm = FLM[i], fmm = FMM[i];
+    if (init) { fmm = pad ? 1.0 : MM; flm = pad ? 1.0 : 0.025 * MM; }  // :235-244
+    double Tn = fmax(flm * fmm, eps);                                     // :256-261
+    Tn = opftdelta * pow(Tn, -0.125);
+    const double dumfac = lagran_dt / Tn;
+    const double epsi = dumfac / (1.0 + dumfac);
+    flm = (epsi * LM + (1.0 - epsi) * flm);
+    fmm = (epsi * MM + (1.0 - epsi) * fmm);
+    flm = fmax(eps, flm);
+    const double cs = fmax(eps, pad ? 0.0 : flm / fmm);                   // :267-269
+    FLM[i] = (T)flm;
+    FMM[i] = (T)fmm;
+    Beta[i] = T(1);
+    Cs[i] = (T)cs;
+  }
+}
+
+template <typename T>
+void op_lagrange_ssim(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, const SgsPtrs& s) {
+  OpTag _tag(c, "lagrange_ssim");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  c->last_lag_jt = c->jt;
+  op_interpolag_ssim<T>(c, s);
+  T* F = lag_filter_mom<T>(c, S, u, v, w);
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  const int init = g.inilag && !c->flm_init && (c->jt == lag_count(c) || c->jt == 1);
+  const double delta = g.delta;
+  {
+    ProfScope _ps(c, "lag_epilogue", (21.0 + 2 + 4) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_lag_ssim_epilogue<T>), grid1(n3), dim3(256), 0, c->stream, (const T*)F, (T*)s.F_LM,
+                       (T*)s.F_MM, (T*)s.Beta, (T*)s.Cs_opt2, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
+                       1.5 * delta, g.dt * (double)lag_count(c), init);
+  }
+  LCHK(c, hipGetLastError());
+  // :270-271: plane sums of Beta = 1
+  if (s.Beta_avg) op_fill<double>(c, (double*)s.Beta_avg, nz, (double)g.nx * g.ny);
+  if (s.Betaclip_avg) op_fill<double>(c, (double*)s.Betaclip_avg, nz, (double)g.nx * g.ny);
+  if (init) c->flm_init = 1;
+}
+
 template <typename T>
 void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const T* w, T* const tau[6],
                  const SgsPtrs& s, T* theta, const ScalarSgsPtrs* ss) {
   OpTag _tag(c, "sgs_stag");
   const Grid& g = c->g;
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
-  const bool m3 = c->sgs_model == 3;  // no Lagrangian averages, plane-averaged Cs (sgs_stag.f90:128-140)
+  const int model = c->sgs_model;
+  const bool lagr = model == 4 || model == 5;  // u_lag accumulates under models 4, 5, 7 (sgs_stag.f90:59-69)
   T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
   T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
   {
     ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + (ss ? 6 : 3)) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_calc_sij<T>), grid1(n3), dim3(256), 0, c->stream, gr[0], gr[1], gr[2], gr[3], gr[4], gr[5],
                        gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx, g.ld, g.ny, g.nz, u, v, w,
-                       m3 ? (T*)nullptr : (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
+                       lagr ? (T*)s.u_lag : (T*)nullptr, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
                        ss ? (T*)ss->v_lag_s : (T*)nullptr, ss ? (T*)ss->w_lag_s : (T*)nullptr);
   }
   LCHK(c, hipGetLastError());
   const int jt = c->jt;
-  if (jt == 1 && g.inilag) {
+  if (model == 1) {
+    // Cs_opt2 = Co^2 (sgs_stag.f90:75-76), written by k_sgs_tau below
+  } else if (jt == 1 && g.inilag) {
     op_fill<T>(c, (T*)s.Cs_opt2, n3, T(0.03));
   } else if ((jt >= 1 || g.initu) && (jt % g.cs_count == 0)) {
-    if (m3) {
+    if (model == 2) {
+      const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+      op_std_dynamic<T>(c, Sc, u, v, w, nullptr, (T*)s.Cs_opt2);
+    } else if (model == 4) {
+      const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+      op_lagrange_ssim<T>(c, Sc, u, v, w, s);
+    } else if (model == 3) {
       // sgs_stag.f90:132 passes ten arguments to the seven-argument routine,
       // so its S11..S33 are bound to u, v, w, S11, S12, S13
       const T* Sd[6] = {u, v, w, S[0], S[1], S[2]};
@@ -941,15 +1058,17 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
       }
     }
   }
-  const int use_const = !m3 && c->jt_total < 500;  // :165-174 applies to models 5-7
-  const T cs0 = (T)(double)(0.16f * 0.16f);
+  // :165-174 sets Cs_opt2 = 0.16**2 (default real) under models 5-7 while
+  // jt_total < 500; model 1 sets Co**2 = 0.16_rprec**2 every step
+  const int use_const = model == 1 || (model == 5 && c->jt_total < 500);
+  const T cs0 = model == 1 ? (T)(0.16 * 0.16) : (T)(double)(0.16f * 0.16f);
   const T l2 = (T)(g.delta * g.delta);
   {
     ProfScope _ps2(c, "sgs_tau", (6.0 + 1 + 1 + 6) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_sgs_tau<T>), grid1(n3), dim3(256), 0, c->stream, S[0], S[1], S[2], S[3], S[4], S[5],
                        (const T*)s.Cs_opt2, cs0, use_const, (T*)s.Cs_opt2, (T*)s.Nu_t, tau[0], tau[1], tau[2], tau[3],
-                       tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2, ss ? (T*)ss->Pr_t : (T*)nullptr,
-                       (T)c->scal.Pr);
+                       tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2, model == 1 ? (const double*)c->l2z : nullptr,
+                       ss ? (T*)ss->Pr_t : (T*)nullptr, (T)c->scal.Pr);
   }
   LCHK(c, hipGetLastError());
 }
@@ -1774,7 +1893,8 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
                              const T*, const T*, T*, T*, T*);                                                    \
   template void op_convec_ab2<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,         \
-                                 const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int, const T*); \
+                                 const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int, const T*,  \
+                                 const ForcingArgs<T>*);                                                         \
   template void op_wallstress<T>(les_ctx*, const T*, const T*, T*, T*, T*, T*, const double*, const double*,     \
                                  const double*, double*);                                                        \
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
@@ -1783,6 +1903,8 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, int);                 \
   template void launch_tridiag<T>(les_ctx*, const cx<T>*, const cx<T>*, cx<T>*, double*, cx<double>*, int*);   \
   template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \
+  template void op_lagrange_ssim<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);  \
+  template void op_interpolag_ssim<T>(les_ctx*, const SgsPtrs&);                                                 \
   template void op_sgs_stag<T>(les_ctx*, const T* const[9], const T*, const T*, const T*, T* const[6],           \
                                const SgsPtrs&, T*, const ScalarSgsPtrs*);                                        \
   template void op_press_stag<T>(les_ctx*, const T*, const T*, const T*, const T*, T*, T*, T*);                  \

@@ -361,8 +361,18 @@ struct LoadScalarTemp {
   }
 };
 
+// the sponge term of the scalar equations (scalars_module.f90:143-148,
+// scalars_module_q.f90:116-121): field 3 = theta, 4 = q of the plane sums
+template <typename T>
+__device__ __forceinline__ T scalar_sponge(const ForcingArgs<T>& fo, T rhs, int field, int r, int nl, T s0) {
+#pragma clang fp contract(off)
+  const T m = (T)(fo.sums[field * nl + r] / fo.nxy);
+  return rhs - (T)fo.coef[r] * (s0 - m);
+}
+
 // c2r epilogue: RHS = -advection + d/dx + d/dy (spectral) + d/dz of the
-// vertical SGS flux (scalars_module.f90:436-452), then step_scalar (:457-504)
+// vertical SGS flux (scalars_module.f90:436-452), the sponge term, then
+// step_scalar (:457-504)
 template <typename T>
 struct StoreScalarRHS {
   T* s[2];
@@ -377,6 +387,8 @@ struct StoreScalarRHS {
   int ppf, ny, ld, nx, nz;
   T rdz, dt;
   int do_step, euler;
+  ForcingArgs<T> fo;  // damp == 2: sponge term about the plane mean
+  int sfield[2];      // plane-sum index of each scalar (3 theta, 4 q)
   static constexpr double writes() { return 7.0; }
   // load phase: every array value the update of point (plane, y, x) reads
   struct Pre {
@@ -420,7 +432,8 @@ struct StoreScalarRHS {
       t0 = (T(1) / p.p0) * p.nu0 * p.d0;
     }
     const T t1 = (T(1) / p.p1) * p.nu1 * p.d1;
-    const T rhs = v + rdz * (t1 - t0);
+    T rhs = v + rdz * (t1 - t0);
+    if (fo.damp == 2) rhs = scalar_sponge(fo, rhs, sfield[q], k, ppf, p.s0);
     Rq[i] = rhs;
     if (!do_step) return;
     T fo;
@@ -442,7 +455,7 @@ struct StoreScalarRHS {
 // step in the c2r epilogue
 template <typename T>
 static void scalar_sgs_finish(les_ctx* c, const ScalarJob* jobs, int ns, const T* Nu_t, const T* Pr_t,
-                              const cx<T>* X3, int do_step, int euler) {
+                              const cx<T>* X3, int do_step, int euler, const ForcingArgs<T>* fo = nullptr) {
   const Grid& g = c->g;
   const int nz = g.nz, nl = nz - 1;
   const long rowsT = 2L * ns * nl * g.ny;
@@ -469,7 +482,9 @@ static void scalar_sgs_finish(les_ctx* c, const ScalarJob* jobs, int ns, const T
     S.sv1[q] = jobs[q].sgs_vert1;
     S.sf[q] = surface_flux(c, jobs[q].which);
     S.gz[q] = (T)(g.Lz > 1.0 ? lid_gradient(c, jobs[q].which) * g.dz : 0.0);
+    S.sfield[q] = 3 + jobs[q].which;
   }
+  if (fo && fo->damp == 2 && do_step) S.fo = *fo;
   S.Nu = Nu_t; S.Pr = Pr_t; S.Prc = (T)c->scal.Pr;
   S.ppf = nl; S.ny = g.ny; S.ld = g.ld; S.nx = g.nx; S.nz = nz;
   S.rdz = T(1.0 / g.dz); S.dt = T(g.dt);
@@ -534,7 +549,7 @@ struct LoadConvecScalar {
 template <typename T>
 void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
                           const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
-                          const T* Nu_t, const T* Pr_t, int scal_euler) {
+                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo) {
   OpTag _tag(c, "convec_scalar");
   const Grid& g = c->g;
   const int nz = g.nz, nl = nz - 1;
@@ -562,14 +577,75 @@ void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T*
   col_out(b, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
   launch_col<T>(c, XP, nullptr, X3, nullptr, nullptr, b, 5L * nl, g.nxh);
   // scalars first: their epilogue reads no velocity, the momentum one updates it
-  scalar_sgs_finish<T>(c, jobs, 2, Nu_t, Pr_t, X3 + (size_t)3 * nl * g.ny * g.nxh, 1, scal_euler);
+  scalar_sgs_finish<T>(c, jobs, 2, Nu_t, Pr_t, X3 + (size_t)3 * nl * g.ny * g.nxh, 1, scal_euler, fo);
   StoreConvecAB2<T> SC{{R[0], R[1], R[2]}, {Rf[0], Rf[1], Rf[2]}, {divt[0], divt[1], divt[2]}, {u, v, w},
                        nl, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
+  if (fo) SC.fo = *fo;
   launch_row_inv<T>(c, X3, g.nxh, g.nxh, SC, 3L * nl * g.ny, g.nx, g.ld, g.ny);
   const size_t pl = (size_t)g.ld * g.ny, top = pl * nl;
   T* tops[9] = {R[0] + top, R[1] + top, R[2] + top, u + top, v + top, w + top, Rf[0] + top, Rf[1] + top,
                 Rf[2] + top};
   op_fill_n<T>(c, tops, euler_first ? 9 : 6, pl, T(BOGUS));
+}
+
+// ------------------------------------------------------------------------
+// Coriolis and sponge inputs of one step (main.f90:319-348): the products
+// coriol*v, coriol*u of the velocities before the AB2 update, and the plane
+// sums of u, v, w, theta, q on levels 1..nz-1 (compensated, as every plane
+// sum here) for damping_method 2
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_coriolis_pre(const T* __restrict__ u, const T* __restrict__ v, T* cu, T* cv, T coriol, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    cv[i] = coriol * v[i];
+    cu[i] = coriol * u[i];
+  }
+}
+
+template <typename T>
+struct SumLevels5 {  // level lev = field * nl + k of (u, v, w, theta, q); absent fields sum to 0
+  const T* f[5];
+  int nl, ld, ny;
+  __device__ double operator()(int lev, int y, int x) const {
+    const int q = lev / nl, k = lev - q * nl;
+    return f[q] ? (double)f[q][((long)k * ny + y) * ld + x] : 0.0;
+  }
+};
+
+template <typename T>
+ForcingArgs<T> op_forcing_pre(les_ctx* c, const T* u, const T* v, const T* w, const T* theta, const T* q) {
+  ForcingArgs<T> fa;
+  const Forcing& F = c->frc;
+  if (!F.coriolis && F.ubc != 1) return fa;
+  OpTag _tag(c, "forcing");
+  const Grid& g = c->g;
+  const int nl = g.nz - 1;
+  const long nv = (long)g.ld * g.ny * nl;
+  if (F.coriolis) {
+    T* cc = (T*)scratch(c, S_FORCE, 2 * (size_t)nv * sizeof(T));
+    {
+      ProfScope _ps(c, "coriolis_pre", 4.0 * nv * sizeof(T));
+      hipLaunchKernelGGL((k_coriolis_pre<T>), grid1((size_t)nv), dim3(256), 0, c->stream, u, v, cc, cc + nv,
+                         (T)F.coriol, nv);
+    }
+    LCHK(c, hipGetLastError());
+    fa.cor[0] = cc + nv;  // coriol * v feeds RHSx
+    fa.cor[1] = cc;       // coriol * u feeds RHSy
+    fa.cvg = (T)F.cvg;
+    fa.cug = (T)F.cug;
+  }
+  if (F.ubc == 1) {
+    fa.damp = F.damping_method;
+    fa.coef = F.dcoef;
+    fa.nxy = (double)g.nx * g.ny;
+    if (F.damping_method == 2) {
+      double* sums = (double*)scratch(c, S_FSUM, 5 * (size_t)nl * sizeof(double));
+      level_sums(c, SumLevels5<T>{{u, v, w, theta, q}, nl, g.ld, g.ny}, 5 * nl, sums,
+                 (theta ? 5.0 : 3.0) * g.nx * g.ny * nl * sizeof(T));
+      fa.sums = sums;
+    }
+  }
+  return fa;
 }
 
 // step_scalar, scalars_module.f90:457-504 (routine-level entry point)
@@ -848,7 +924,8 @@ void op_pad_products_probe(les_ctx* c, int scal) {
   template void op_step_scalar<T>(les_ctx*, int, T*, const T*, const T*);                                        \
   template void op_convec_scalar_ab2<T>(les_ctx*, T*, T*, T*, const T* const[9], T* const[3], T* const[3],        \
                                         const T* const[3], int, const T*, const ScalarJob*, const T*, const T*,   \
-                                        int);                                                                     \
+                                        int, const ForcingArgs<T>*);                                              \
+  template ForcingArgs<T> op_forcing_pre<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*);          \
   template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
                                       const SgsPtrs&, const ScalarSgsPtrs&, bool);                               \
   template void op_pad_products_probe<T>(les_ctx*, int);

@@ -1106,6 +1106,29 @@ struct LoadConvec {  // fields: u1,u2,u3, vort1, vort2, vort3 ; levels 0..nz-1
   }
 };
 
+// main.f90:319-348 after the buoyancy term: Coriolis on whole rows, then
+// the sponge on the nx*ny points of level r, each term rounded as written.
+// f: velocity component, u0: its value before the update, cor: the Coriolis
+// product of the other component.
+template <typename T>
+__device__ __forceinline__ T forcing_terms(const ForcingArgs<T>& fo, T rhs, int f, int r, int x, int nx, int nl,
+                                           T u0, T cor) {
+#pragma clang fp contract(off)
+  if (fo.cor[0]) {
+    if (f == 0) rhs = rhs + cor - fo.cvg;
+    else if (f == 1) rhs = rhs - cor + fo.cug;
+  }
+  if (x < nx) {
+    if (fo.damp == 2) {
+      const T m = (T)(fo.sums[f * nl + r] / fo.nxy);
+      rhs = rhs - (T)fo.coef[(f == 2 ? nl : 0) + r] * (u0 - m);
+    } else if (fo.damp == 1 && f == 2) {
+      rhs = rhs - (T)fo.coef[nl + r] * u0;
+    }
+  }
+  return rhs;
+}
+
 // main.f90:309-378 fused into the c2r epilogue of convec: RHS = -conv - divt,
 // u* = u + dt*(1.5 RHS - 0.5 RHS_f + force) (force on u only).  Arithmetic
 // and evaluation order as k_ab2.
@@ -1119,9 +1142,10 @@ struct StoreConvecAB2 {
   T dt, force;
   int euler;
   const T* beta;  // buoyancy added to RHSz (main.f90:312-317), or null
+  ForcingArgs<T> fo;  // Coriolis and sponge terms (none by default)
   static constexpr double writes() { return 5.0; }
   struct Pre {
-    T dv, beta, rf, u;
+    T dv, beta, rf, u, cor;
   };
   __device__ Pre pre(int plane, int y, int x) const {
     const int f = plane / ppf, r = plane - f * ppf;
@@ -1131,6 +1155,7 @@ struct StoreConvecAB2 {
     p.beta = (f == 2 && beta) ? beta[i] : T(0);
     p.rf = euler ? T(0) : Rf[f][i];
     p.u = U[f][i];
+    p.cor = (fo.cor[0] && f < 2) ? fo.cor[f][i] : T(0);
     return p;
   }
   __device__ void fin(int plane, int y, int x, T v, const Pre& p) const {
@@ -1139,6 +1164,7 @@ struct StoreConvecAB2 {
     const T conv = x < nx ? v : T(0);
     T rhs = -conv - p.dv;
     if (f == 2 && beta) rhs = rhs + p.beta;
+    if (fo.cor[0] || fo.damp) rhs = forcing_terms(fo, rhs, f, r, x, nx, ppf, p.u, p.cor);
     R[f][i] = rhs;
     T fo;
     if (euler) {

@@ -37,7 +37,12 @@ static const double TADV2 = -0.5;  /* 1 - tadv1 */
      5  press_stag_array: sign of the divtz boundary rows flipped
      6  lagrange_scalar_Sdep: the factors 4 and 16 swapped
      7  lagrange_scalar_Sdep: Beta inverted
-     8  step: no Euler first step when initu is false */
+     8  step: no Euler first step when initu is false
+     9  step: Coriolis term of RHSy with the wrong sign
+    10  step: sponge of RHSx, RHSy taken at the w-node below (sponge(jz)) instead
+        of the average of sponge(jz) and sponge(jz+1)
+    11  interpolag_Ssim: the w_lag clamp of interpolag_Sdep applied
+    12  model 1: the wall level's mixing length taken at z = dz instead of dz/2 */
 #ifndef OR_MUTANT
 #define OR_MUTANT 0
 #endif
@@ -161,6 +166,9 @@ struct or_ctx {
   or_scalar_params S;       /* scalar switches and constants (zero: no scalars) */
   double *wt_s, *wq_s;      /* (nx,ny) kinematic surface fluxes */
   int glm_init, gqn_init;   /* SAVE flags, lagrange_scalar_Sdep.f90:65-66 */
+  or_forcing F;             /* Coriolis and sponge switches (or_set_forcing) */
+  double coriol, cug, cvg;  /* coriol, coriol*ug, coriol*vg */
+  double *sponge;           /* topbc sponge(1..nz) at [1..nz]; [0] unused */
 };
 
 static void plan2_init(plan2 *p, int nx, int ny) {
@@ -208,13 +216,15 @@ or_ctx *or_create(const or_params *p) {
   init_wavenumber(c);
   or_test_filter_init(c, 2.0, c->Gt);    /* main.f90:166 */
   or_test_filter_init(c, 4.0, c->Gtt);   /* main.f90:169 */
+  c->sponge = (double *)calloc((size_t)c->nz + 1, sizeof(double));
+  c->sgs_model = 5;
   return c;
 }
 
 void or_destroy(or_ctx *c) {
   if (!c) return;
   free(c->kx); free(c->ky); free(c->k2); free(c->Gt); free(c->Gtt);
-  free(c->wt_s); free(c->wq_s);
+  free(c->wt_s); free(c->wq_s); free(c->sponge);
   fplan_free(&c->sm.px); fplan_free(&c->sm.py); fplan_free(&c->bg.px); fplan_free(&c->bg.py);
   free(c);
 }
@@ -624,13 +634,15 @@ void or_calc_sij(or_ctx *c, const double *const g[9], double *const S[6]) {
 /* interpolag_Sdep.f90:2-380 (non-MPI, read_inflow_file=.false.)        */
 /* ------------------------------------------------------------------ */
 /* the same interpolation serves interpolag_scalar_Sdep.f90:30-316 on
-   G_LM..G_NN with u_lag_s..w_lag_s */
-static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *vl, double *wl) {
+   G_LM..G_NN with u_lag_s..w_lag_s, and interpolag_Ssim.f90 on F_LM, F_MM
+   (nf = 2), which lacks the clamp of w_lag to [-1, 1] at the two lowest
+   levels (interpolag_Sdep.f90:264-269) */
+static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *vl, double *wl, int nf, int clamp) {
   const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
   const size_t pl = (size_t)ld * ny, n3 = pl * nz;
   const double cs = (double)c->P.cs_count, dt = c->P.dt;
   double *FF[4];
-  for (int q = 0; q < 4; q++) { FF[q] = (double *)malloc(sizeof(double) * n3); memcpy(FF[q], F[q], sizeof(double) * n3); }
+  for (int q = 0; q < nf; q++) { FF[q] = (double *)malloc(sizeof(double) * n3); memcpy(FF[q], F[q], sizeof(double) * n3); }
   for (size_t i = 0; i < n3; i++) { ul[i] = ul[i] / cs; vl[i] = vl[i] / cs; }
   for (int k = nz - 1; k >= 1; k--)
     for (size_t i = 0; i < pl; i++) {
@@ -650,7 +662,7 @@ static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *v
       wl[pl + i] = wl[pl + i] + fmin(wl[pl + i], 0.0);
       wl[i] = wl[i] + fmax(wl[i], 0.0);
     }
-  for (size_t i = 0; i < pl; i++) {
+  for (size_t i = 0; clamp && i < pl; i++) {
     wl[pl + i] = fmin(1.0, wl[pl + i]);
     wl[i] = fmin(1.0, wl[pl + i]);
     wl[pl + i] = fmax(-1.0, wl[pl + i]);
@@ -669,7 +681,7 @@ static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *v
         size_t i0y0 = pl * k + x + (size_t)ld * ya, i0yz = pl * za + x + (size_t)ld * ya;
         size_t ix00 = pl * k + xa + (size_t)ld * y, ix0z = pl * za + xa + (size_t)ld * y;
         size_t ixy0 = pl * k + xa + (size_t)ld * ya, ixyz = pl * za + xa + (size_t)ld * ya;
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < nf; q++) {
           const double *G = FF[q];
           F[q][i] = fx * fy * (G[i000] * fz + G[i00z] * cz)
                   + fx * cy * (G[i0y0] * fz + G[i0yz] * cz)
@@ -678,17 +690,22 @@ static void interpolag_core(or_ctx *c, double *const F[4], double *ul, double *v
         }
       }
   memset(ul, 0, sizeof(double) * n3); memset(vl, 0, sizeof(double) * n3); memset(wl, 0, sizeof(double) * n3);
-  for (int q = 0; q < 4; q++) free(FF[q]);
+  for (int q = 0; q < nf; q++) free(FF[q]);
 }
 
 void or_interpolag_sdep(or_ctx *c, or_sgs_state *s) {
   double *F[4] = { s->F_LM, s->F_MM, s->F_QN, s->F_NN };
-  interpolag_core(c, F, s->u_lag, s->v_lag, s->w_lag);
+  interpolag_core(c, F, s->u_lag, s->v_lag, s->w_lag, 4, 1);
+}
+
+void or_interpolag_ssim(or_ctx *c, or_sgs_state *s) {
+  double *F[4] = { s->F_LM, s->F_MM, NULL, NULL };
+  interpolag_core(c, F, s->u_lag, s->v_lag, s->w_lag, 2, OR_MUTANT == 11);
 }
 
 void or_interpolag_scalar_sdep(or_ctx *c, or_scalar_sgs *g) {
   double *F[4] = { g->G_LM, g->G_MM, g->G_QN, g->G_NN };
-  interpolag_core(c, F, g->u_lag_s, g->v_lag_s, g->w_lag_s);
+  interpolag_core(c, F, g->u_lag_s, g->v_lag_s, g->w_lag_s, 4, 1);
 }
 
 /* ------------------------------------------------------------------ */
@@ -849,6 +866,131 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
 }
 
 /* ------------------------------------------------------------------ */
+/* the 2-Delta filtered quantities of one level shared by lagrange_Ssim  */
+/* (lagrange_Ssim.f90:142-215) and std_dynamic (std_dynamic.f90:30-122):  */
+/* L_ij = bar(u_i u_j) - bar(u_i) bar(u_j), bar(S_ij), bar(|S| S_ij).    */
+/* Both routines put u, v on the first uv-node and 0.25 w(2) there, and  */
+/* w-node averages above; the products are formed before filtering.       */
+/* ------------------------------------------------------------------ */
+enum { LV_UB, LV_VB, LV_WB, LV_L11, LV_L12, LV_L13, LV_L22, LV_L23, LV_L33,
+       LV_SB11, LV_SB12, LV_SB13, LV_SB22, LV_SB23, LV_SB33,
+       LV_SS11, LV_SS12, LV_SS13, LV_SS22, LV_SS23, LV_SS33, LV_N };
+
+static void level_2delta(or_ctx *c, const double *const S[6], const double *u, const double *v,
+                         const double *w, int k, double *T[LV_N]) {
+  const size_t pl = (size_t)c->ld * c->ny, o = pl * k;
+  for (size_t i = 0; i < pl; i++) {
+    double ub, vb, wb;
+    if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; }
+    else { ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i]; }
+    T[LV_UB][i] = ub; T[LV_VB][i] = vb; T[LV_WB][i] = wb;
+    T[LV_L11][i] = ub * ub; T[LV_L12][i] = ub * vb; T[LV_L13][i] = ub * wb;
+    T[LV_L22][i] = vb * vb; T[LV_L23][i] = vb * wb; T[LV_L33][i] = wb * wb;
+  }
+  for (int q = 0; q < 3; q++) or_test_filter(c, T[LV_UB + q], c->Gt);
+  const int Lp[6][3] = { { LV_L11, LV_UB, LV_UB }, { LV_L12, LV_UB, LV_VB }, { LV_L13, LV_UB, LV_WB },
+                         { LV_L22, LV_VB, LV_VB }, { LV_L23, LV_VB, LV_WB }, { LV_L33, LV_WB, LV_WB } };
+  for (int q = 0; q < 6; q++) {
+    or_test_filter(c, T[Lp[q][0]], c->Gt);
+    for (size_t i = 0; i < pl; i++) T[Lp[q][0]][i] = T[Lp[q][0]][i] - T[Lp[q][1]][i] * T[Lp[q][2]][i];
+  }
+  for (size_t i = 0; i < pl; i++) {
+    const double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+    const double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+    const double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+    for (int q = 0; q < 6; q++) { T[LV_SB11 + q][i] = S[q][o + i]; T[LV_SS11 + q][i] = Sm * S[q][o + i]; }
+  }
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[LV_SB11 + q], c->Gt);
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[LV_SS11 + q], c->Gt);
+}
+
+/* L:M and M:M at plane point i with M_ij = cst (bar(|S|S_ij) - 4 |S_bar| bar(S_ij))
+   (beta = 1: lagrange_Ssim.f90:221-231, std_dynamic.f90:124-132) */
+static void level_lm_mm(double *const T[LV_N], size_t i, double cst, double *LM, double *MM) {
+  const double *sb[6] = { T[LV_SB11], T[LV_SB12], T[LV_SB13], T[LV_SB22], T[LV_SB23], T[LV_SB33] };
+  const double Sb = sqrt(2.0 * (sb[0][i] * sb[0][i] + sb[3][i] * sb[3][i] + sb[5][i] * sb[5][i] +
+                                2.0 * (sb[1][i] * sb[1][i] + sb[2][i] * sb[2][i] + sb[4][i] * sb[4][i])));
+  double M[6], L[6];
+  for (int q = 0; q < 6; q++) {
+    M[q] = cst * (T[LV_SS11 + q][i] - 4.0 * Sb * sb[q][i]);
+    L[q] = T[LV_L11 + q][i];
+  }
+  /* q: 0=11 1=12 2=13 3=22 4=23 5=33 */
+  *LM = L[0] * M[0] + L[3] * M[3] + L[5] * M[5] + 2.0 * (L[1] * M[1] + L[2] * M[2] + L[4] * M[4]);
+  *MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+}
+
+/* lagrange_Ssim.f90:4-300 (non-MPI, no buildings, no inflow, Beta = 1) */
+void or_lagrange_ssim(or_ctx *c, const double *const S[6], const double *u,
+                      const double *v, const double *w, or_sgs_state *s) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double delta = c->delta, eps = 1.e-32;
+  const double opftdelta = 1.5 * delta, powcoeff = -1.0 / 8.0;       /* :71-75 */
+  or_interpolag_ssim(c, s);                                          /* :123 */
+  const double lagran_dt = c->P.dt * (double)c->P.cs_count;
+  const double cst = 2.0 * (delta * delta);
+  double *T[LV_N];
+  for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
+  const int jt = c->jt;
+  const int init = c->P.inilag && !c->flm_init && (jt == c->P.cs_count || jt == 1);   /* :235-244 */
+  for (int k = 0; k < nz; k++) {
+    const size_t o = pl * k;
+    level_2delta(c, S, u, v, w, k, T);
+    double *FLM = s->F_LM + o, *FMM = s->F_MM + o;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < ld; x++) {
+        const size_t i = x + (size_t)ld * y;
+        double LM, MM;
+        level_lm_mm(T, i, cst, &LM, &MM);
+        if (init) {
+          FMM[i] = MM; FLM[i] = 0.025 * MM;
+          if (x >= ld - 2) { FMM[i] = 1.0; FLM[i] = 1.0; }
+        }
+        double Tn = fmax(FLM[i] * FMM[i], eps);                      /* :256-260 */
+        Tn = opftdelta * pow(Tn, powcoeff);
+        const double dumfac = lagran_dt / Tn;
+        const double epsi = dumfac / (1.0 + dumfac);
+        FLM[i] = (epsi * LM + (1.0 - epsi) * FLM[i]);                /* :263-265 */
+        FMM[i] = (epsi * MM + (1.0 - epsi) * FMM[i]);
+        FLM[i] = fmax(eps, FLM[i]);
+        double cs = FLM[i] / FMM[i];                                 /* :267-269 */
+        if (x >= ld - 2) cs = 0.0;
+        s->Cs_opt2[o + i] = fmax(eps, cs);
+        s->Beta[o + i] = 1.0;                                        /* :117 */
+      }
+    s->Beta_avg[k] = (double)nx * ny;                                /* :270-271, sums of Beta = 1 */
+    s->Betaclip_avg[k] = (double)nx * ny;
+  }
+  if (init) c->flm_init = 1;
+  for (int t = 0; t < LV_N; t++) free(T[t]);
+}
+
+/* std_dynamic.f90:4-140: one coefficient per level from plane sums of L:M
+   and M:M over the (ld, ny) arrays, Delta = (dx dy dz)^(1/3) */
+void or_std_dynamic(or_ctx *c, const double *const S[6], const double *u, const double *v,
+                    const double *w, double *Cs_opt2) {
+  const int ld = c->ld, ny = c->ny, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double delta = pow(c->dx * c->dy * c->dz, 1.0 / 3.0);        /* :28 */
+  const double cst = 2.0 * (delta * delta);                          /* :124 */
+  double *T[LV_N];
+  for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
+  for (int k = 0; k < nz; k++) {
+    level_2delta(c, S, u, v, w, k, T);
+    csum_t num = { 0.0, 0.0 }, den = { 0.0, 0.0 };
+    for (size_t i = 0; i < pl; i++) {
+      double LM, MM;
+      level_lm_mm(T, i, cst, &LM, &MM);
+      csum_add(&num, LM);
+      csum_add(&den, MM);
+    }
+    Cs_opt2[k] = fmax(0.0, csum_val(&num) / csum_val(&den));         /* :135-138 */
+  }
+  for (int t = 0; t < LV_N; t++) free(T[t]);
+}
+
+/* ------------------------------------------------------------------ */
 /* sgs_stag.f90:3-366 (model=5, sgs=.true., molec=.false., 'wall')      */
 /* ------------------------------------------------------------------ */
 static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
@@ -877,6 +1019,12 @@ void or_sgs_stag_scalar(or_ctx *c, const double *const g[9], const double *u,
   sgs_stag_impl(c, g, u, v, w, theta, tau, s, ss, 7);
 }
 
+void or_sgs_stag_model(or_ctx *c, const double *const g[9], const double *u,
+                       const double *v, const double *w, double *const tau[6],
+                       or_sgs_state *s, int model) {
+  sgs_stag_impl(c, g, u, v, w, NULL, tau, s, NULL, model);
+}
+
 static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
                           const double *v, const double *w, double *theta,
                           double *const tau[6], or_sgs_state *s, or_scalar_sgs *ss, int model) {
@@ -888,13 +1036,22 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
   double *txx = tau[0], *txy = tau[1], *txz = tau[2], *tyy = tau[3], *tyz = tau[4], *tzz = tau[5];
   double *txzp = (double *)malloc(sizeof(double) * pl), *tyzp = (double *)malloc(sizeof(double) * pl);
   memcpy(txzp, txz, sizeof(double) * pl); memcpy(tyzp, tyz, sizeof(double) * pl);
-  if (model != 3)                                               /* sgs_stag.f90:59-69 */
+  if (model == 4 || model == 5 || model == 7)                   /* sgs_stag.f90:59-69 */
     for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
   if (ss)
     for (size_t i = 0; i < n3; i++) { ss->u_lag_s[i] += u[i]; ss->v_lag_s[i] += v[i]; ss->w_lag_s[i] += w[i]; }
   const int jt = c->jt;
-  const double l = c->delta;
-  if (jt == 1 && c->P.inilag) {
+  double *lz = (double *)malloc(sizeof(double) * nz);
+  for (int k = 0; k < nz; k++) lz[k] = c->delta;                  /* :110 dynamic models: l = delta */
+  if (model == 1) {                                               /* :73-104, lbc_mom = 'wall' */
+    const double Co = 0.16, vonk = 0.41;                          /* param.f90:98,123; nnn = 2 */
+    for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = Co * Co;
+    for (int k = 0; k < nz; k++) {
+      const double zz = k == 0 ? (OR_MUTANT == 12 ? 1.0 : 1.0 - 0.5) * c->dz : (double)k * c->dz;
+      const double a = vonk * zz;
+      lz[k] = pow(Co * Co * (1.0 / (a * a)) + 1.0 / (c->delta * c->delta), -1.0 / 2.0);
+    }
+  } else if (jt == 1 && c->P.inilag) {
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = 0.03;
   } else if ((jt >= 1 || c->P.initu) && (jt % c->P.cs_count == 0)) {
     if (model == 3) {
@@ -906,12 +1063,20 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
       for (int k = 0; k < nz; k++)
         for (size_t i = 0; i < pl; i++) s->Cs_opt2[pl * k + i] = ziko[k];
       free(ziko);
+    } else if (model == 2) {                                      /* :123-126 */
+      double *ziko = (double *)calloc(nz, sizeof(double));
+      or_std_dynamic(c, (const double *const *)S, u, v, w, ziko);
+      for (int k = 0; k < nz; k++)
+        for (size_t i = 0; i < pl; i++) s->Cs_opt2[pl * k + i] = ziko[k];
+      free(ziko);
+    } else if (model == 4) {                                      /* :137-138 */
+      or_lagrange_ssim(c, (const double *const *)S, u, v, w, s);
     } else {
       or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
       if (ss) or_lagrange_scalar_sdep(c, (const double *const *)S, u, v, w, theta, s, ss);
     }
   }
-  if (model != 3 && c->jt_total < 500) {                        /* :165-174 (models 5, 6, 7) */
+  if ((model == 5 || model == 7) && c->jt_total < 500) {       /* :165-174 (models 5, 6, 7) */
     const double cs0 = (double)(0.16f * 0.16f);
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = cs0;
     if (ss) for (size_t i = 0; i < n3; i++) ss->Pr_t[i] = c->S.Pr;
@@ -922,7 +1087,7 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
         size_t i = pl * k + x + (size_t)ld * y;
         double Sm = sqrt(2.0 * (S[0][i] * S[0][i] + S[3][i] * S[3][i] + S[5][i] * S[5][i] +
                                 2.0 * (S[1][i] * S[1][i] + S[2][i] * S[2][i] + S[4][i] * S[4][i])));
-        s->Nu_t[i] = Sm * s->Cs_opt2[i] * (l * l);
+        s->Nu_t[i] = Sm * s->Cs_opt2[i] * (lz[k] * lz[k]);
       }
   for (int y = 0; y < ny; y++)
     for (int x = 0; x < nx; x++) {
@@ -957,7 +1122,7 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
     txx[top + i] = BOGUS; txy[top + i] = BOGUS; tyy[top + i] = BOGUS; tzz[top + i] = BOGUS;
     txz[top + i] = 0.0; tyz[top + i] = 0.0;
   }
-  free(txzp); free(tyzp);
+  free(txzp); free(tyzp); free(lz);
   for (int q = 0; q < 6; q++) free(S[q]);
 }
 
@@ -1319,6 +1484,7 @@ void or_state_init_ic(or_ctx *c, or_state *s, uint64_t seed, double amp, double 
 }
 
 static void theta_q_all_in_one(or_ctx *c, or_state *s, int which);
+static void momentum_forcing(or_ctx *c, or_state *s);
 
 void or_step(or_ctx *c, or_state *s) {
   const int nz = c->nz, nx = c->nx, ny = c->ny, ld = c->ld;
@@ -1347,10 +1513,8 @@ void or_step(or_ctx *c, or_state *s) {
   or_wallstress(c, s->u, s->v, g[2], g[5], s->tau[2], s->tau[4], s->psi_m, s->psi_m0, s->phi_m, s->ustar_avg);
   if (scal && c->S.model == 7)
     or_sgs_stag_scalar(c, (const double *const *)g, s->u, s->v, s->w, s->theta, s->tau, &s->sgs, &s->ssgs);
-  else if (c->sgs_model == 3)
-    or_sgs_stag_model3(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
   else
-    or_sgs_stag(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs);
+    or_sgs_stag_model(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs, c->sgs_model);
   if (scal) {                          /* main.f90:273-276 */
     theta_q_all_in_one(c, s, 0);
     theta_q_all_in_one(c, s, 1);
@@ -1368,6 +1532,7 @@ void or_step(or_ctx *c, or_state *s) {
   }
   if (scal && !c->S.passive)           /* main.f90:312-317 */
     for (size_t i = 0; i < nv; i++) s->RHSz[i] = s->RHSz[i] + s->beta_scal[i];
+  momentum_forcing(c, s);
   const double force = c->P.force;
   if (s->jt == 1 && !c->P.initu && OR_MUTANT != 8) {
     memcpy(s->RHSx_f, s->RHSx, n3 * sizeof(double));
@@ -1400,6 +1565,84 @@ void or_step(or_ctx *c, or_state *s) {
 }
 
 void or_set_time(or_ctx *c, int jt, int jt_total) { c->jt = jt; c->jt_total = jt_total; }
+
+/* param.f90:220 coriol, main.f90:131-138 ug, vg; topbc.f90:14-74 setsponge
+   (non-MPI, nproc = 1), called when ubc == 1 (main.f90:172-185) */
+void or_set_forcing(or_ctx *c, const or_forcing *f) {
+  const int nz = c->nz;
+  c->F = *f;
+  c->coriol = (double)1.3962634E-04f * f->z_i / f->u_star;   /* a default-real literal */
+  c->cug = c->coriol * (f->ug0 / f->u_star);
+  c->cvg = c->coriol * (f->vg0 / f->u_star);
+  double *sp = c->sponge;
+  for (int k = 0; k <= nz; k++) sp[k] = 0.0;
+  if (f->ubc != 1) return;
+  const double z_i = f->z_i, L_z = c->P.Lz * z_i, dz = c->dz;
+  if (f->damping_method == 2) {
+    const double z_d = 0.75 * (L_z * 1.0), cfrdmp = 3.9;
+    for (int k = 1; k <= nz - 1; k++) {
+      const double z_local = ((double)k - 0.5) * dz * z_i;
+      if (z_local >= z_d && z_local <= L_z * 1.0)
+        sp[k] = cfrdmp * 0.5 * (1.0 - cos(PI * (z_local - z_d) / (L_z - z_d)));
+      else
+        sp[k] = 0.0;
+    }
+    sp[nz] = sp[nz - 1];
+  } else if (f->damping_method == 1) {
+    /* nz - 0.75*nz + 1 and 0.75*nz + 1 are default-real expressions; the
+       loop bound is truncated to an integer */
+    const float den = (float)nz - 0.75f * (float)nz + 1.0f;
+    const double factor = 9.0 / (double)den;
+    sp[nz] = z_i / (50.0 * f->u_star);
+    const int kend = (int)(0.75f * (float)nz + 1.0f);
+    for (int k = nz - 1; k >= kend; k--) sp[k] = sp[k + 1] / pow(5.0, factor);
+  }
+}
+
+void or_get_sponge(const or_ctx *c, double *sp) {
+  for (int k = 0; k < c->nz; k++) sp[k] = c->sponge[k + 1];
+}
+
+/* plane mean of level k over the nx*ny points (compensated sum) */
+static double level_mean(const or_ctx *c, const double *f, int k) {
+  csum_t a = { 0.0, 0.0 };
+  const size_t pl = (size_t)c->ld * c->ny;
+  for (int y = 0; y < c->ny; y++)
+    for (int x = 0; x < c->nx; x++) csum_add(&a, f[pl * k + x + (size_t)c->ld * y]);
+  return csum_val(&a) / (double)(c->nx * c->ny);
+}
+
+/* main.f90:319-348: Coriolis on levels 1..nz-1 (whole rows), then the
+   sponge on the nx*ny points of each level */
+static void momentum_forcing(or_ctx *c, or_state *s) {
+  const int nz = c->nz, nx = c->nx, ny = c->ny, ld = c->ld;
+  const size_t pl = (size_t)ld * ny, nv = pl * (nz - 1);
+  if (c->F.coriolis) {
+    const double sy = OR_MUTANT == 9 ? -1.0 : 1.0;
+    for (size_t i = 0; i < nv; i++) {
+      s->RHSx[i] = s->RHSx[i] + c->coriol * s->v[i] - c->cvg;
+      s->RHSy[i] = s->RHSy[i] - sy * c->coriol * s->u[i] + sy * c->cug;
+    }
+  }
+  if (c->F.ubc != 1) return;
+  const double *sp = c->sponge;
+  for (int k = 0; k < nz - 1; k++) {
+    const int jz = k + 1;
+    const double cxy = 0.5 * (sp[jz] + sp[OR_MUTANT == 10 ? jz : jz + 1]);
+    const double mu = level_mean(c, s->u, k), mv = level_mean(c, s->v, k), mw = level_mean(c, s->w, k);
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const size_t i = pl * k + x + (size_t)ld * y;
+        if (c->F.damping_method == 2) {
+          s->RHSx[i] = s->RHSx[i] - cxy * (s->u[i] - mu);
+          s->RHSy[i] = s->RHSy[i] - cxy * (s->v[i] - mv);
+          s->RHSz[i] = s->RHSz[i] - 0.5 * sp[jz] * (s->w[i] - mw);
+        } else if (c->F.damping_method == 1) {
+          s->RHSz[i] = s->RHSz[i] - sp[jz] * s->w[i];
+        }
+      }
+  }
+}
 void or_set_sgs_model(or_ctx *c, int model) { c->sgs_model = model; }
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init) { c->flm_init = flm_init; c->fqn_init = fqn_init; }
 
@@ -1650,6 +1893,15 @@ static void theta_q_all_in_one(or_ctx *c, or_state *s, int which) {
   memcpy(Rf, R, sizeof(double) * n3);                                             /* :121 */
   or_scalar_rhs_calc(c, which, dx, dy, dzf, s->u, s->v, s->w, s->sgs.Nu_t, Pr_, s->ustar_avg, s->phi_h,
                      R, which ? s->sgs_q3 : s->sgs_t3);
+  if (c->F.ubc == 1 && c->F.damping_method == 2)                                  /* :143-148 */
+    for (int k = 0; k < nz - 1; k++) {
+      const double cxy = 0.5 * (c->sponge[k + 1] + c->sponge[k + 2]), m = level_mean(c, sc, k);
+      for (int y = 0; y < c->ny; y++)
+        for (int x = 0; x < c->nx; x++) {
+          const size_t i = pl * k + x + (size_t)c->ld * y;
+          R[i] = R[i] - cxy * (sc[i] - m);
+        }
+    }
   if (which == 0) or_calcbeta(c, s->theta, s->qmix, s->beta_scal);               /* :154 */
   if (c->jt == 1 && !c->S.initsc) memcpy(Rf, R, sizeof(double) * n3);           /* :157-159 */
   or_step_scalar(c, which, sc, R, Rf);

@@ -226,9 +226,37 @@ void or_state_init_scalar_ic(or_ctx *c, or_state *s, uint64_t seed, double T_ini
 /* jt / jt_total seen by routine-level calls (param.f90:147-148) and the
    lagrange_Sdep SAVE init flags (lagrange_Sdep.f90:67-68) */
 void or_set_time(or_ctx *c, int jt, int jt_total);
-/* SGS model of or_step: 5 (Lagrangian scale-dependent, default) or 3 */
+/* SGS model of or_step (param.f90 'model'): 1 Smagorinsky with Mason's wall
+   damping, 2 standard dynamic, 3 plane-averaged scale-dependent, 4 Lagrangian
+   scale-invariant, 5 Lagrangian scale-dependent (default) */
 void or_set_sgs_model(or_ctx *c, int model);
 void or_set_lag_init(or_ctx *c, int flm_init, int fqn_init);
+
+/* Coriolis forcing and the top sponge (param.f90:68-75,220, main.f90:131-138,
+   319-348, topbc.f90:14-95).  ug0/vg0 in m/s, u_star in m/s, z_i in m; the
+   domain height is Lz z_i. */
+typedef struct or_forcing {
+  int coriolis;          /* coriolis_forcing */
+  double ug0, vg0;       /* geostrophic wind (m/s) */
+  double u_star, z_i;
+  int ubc;               /* 1: sponge layer */
+  int damping_method;    /* 1: relax w in the top quarter; 2: cosine profile above 0.75 L_z */
+} or_forcing;
+void or_set_forcing(or_ctx *c, const or_forcing *f);
+/* setsponge's profile sponge(1..nz) into sp[0..nz-1] (zeros when ubc != 1) */
+void or_get_sponge(const or_ctx *c, double *sp);
+
+/* std_dynamic.f90:4-140: Cs_opt2(nz) of the standard dynamic model */
+void or_std_dynamic(or_ctx *c, const double *const S[6], const double *u, const double *v,
+                    const double *w, double *Cs_opt2);
+/* interpolag_Ssim.f90 and lagrange_Ssim.f90:4-300 (model 4, F_LM and F_MM) */
+void or_interpolag_ssim(or_ctx *c, or_sgs_state *s);
+void or_lagrange_ssim(or_ctx *c, const double *const S[6], const double *u,
+                      const double *v, const double *w, or_sgs_state *s);
+/* sgs_stag.f90:3-366 with any model 1..5 (neutral) */
+void or_sgs_stag_model(or_ctx *c, const double *const grad[9], const double *u,
+                       const double *v, const double *w, double *const tau[6],
+                       or_sgs_state *s, int model);
 
 /* ---- statistics (all plane sums compensated) ----
    spectrum, io.f90:1613-1646: spec (nlev, nx/2), spec(k) = <|c_k|^2>_y with

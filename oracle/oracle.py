@@ -49,6 +49,11 @@ SCALAR_SGS_NAMES = ("G_LM", "G_MM", "G_QN", "G_NN", "u_lag_s", "v_lag_s", "w_lag
                     "phi_h", "psi_m")
 
 
+class Forcing(C.Structure):
+    _fields_ = [("coriolis", C.c_int), ("ug0", C.c_double), ("vg0", C.c_double), ("u_star", C.c_double),
+                ("z_i", C.c_double), ("ubc", C.c_int), ("damping_method", C.c_int)]
+
+
 class ScalarSgs(C.Structure):
     _fields_ = [(n, D) for n in SCALAR_SGS_NAMES]
 
@@ -96,6 +101,7 @@ def lib():
         L.or_state_init_scalar_ic.argtypes = [vp, C.POINTER(State), C.c_uint64, C.c_double, C.c_double,
                                               C.c_double, C.c_double, C.c_double]
         L.or_set_scalars.argtypes = [vp, C.POINTER(ScalarParams), D, D]
+        L.or_set_forcing.argtypes = [vp, C.POINTER(Forcing)]
         L.or_step.argtypes = [vp, C.POINTER(State)]
         L.or_uniform.restype = C.c_double
         L.or_uniform.argtypes = [C.c_uint64, C.c_uint64]
@@ -236,6 +242,33 @@ class Oracle:
         g = (D * 9)(*[_p(a) for a in grads])
         t = (D * 6)(*[_p(a) for a in taus])
         self.L.or_sgs_stag(C.c_void_p(self.ctx), g, _p(u), _p(v), _p(w), t, C.byref(st))
+
+    def sgs_stag_model(self, grads, u, v, w, taus, st, model):
+        g = (D * 9)(*[_p(a) for a in grads])
+        t = (D * 6)(*[_p(a) for a in taus])
+        self.L.or_sgs_stag_model(C.c_void_p(self.ctx), g, _p(u), _p(v), _p(w), t, C.byref(st), C.c_int(model))
+
+    def std_dynamic(self, S, u, v, w):
+        cs = np.zeros(self.nz)
+        s6 = (D * 6)(*[_p(a) for a in S])
+        self.L.or_std_dynamic(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(cs))
+        return cs
+
+    def interpolag_ssim(self, st):
+        self.L.or_interpolag_ssim(C.c_void_p(self.ctx), C.byref(st))
+
+    def lagrange_ssim(self, S, u, v, w, st):
+        s6 = (D * 6)(*[_p(a) for a in S])
+        self.L.or_lagrange_ssim(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), C.byref(st))
+
+    def set_forcing(self, coriolis=0, ug0=0.0, vg0=0.0, u_star=0.45, z_i=1000.0, ubc=0, damping_method=1):
+        self.forcing = Forcing(coriolis, ug0, vg0, u_star, z_i, ubc, damping_method)
+        self.L.or_set_forcing(C.c_void_p(self.ctx), C.byref(self.forcing))
+
+    def sponge(self):
+        sp = np.zeros(self.nz)
+        self.L.or_get_sponge(C.c_void_p(self.ctx), _p(sp))
+        return sp
 
     # ---- transforms ----
     def rfft2_fwd(self, plane, big=False):

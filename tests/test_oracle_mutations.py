@@ -5,7 +5,7 @@ mistake (les_oracle.c, OR_MUTANT = n) into a temporary directory, runs the
 pin that should catch it against that build in a child process
 (LES_ORACLE_LIB points oracle/oracle.py at it), and requires the pin to fail.
 The unmodified oracle passes the same pins in the main suite
-(tests/test_oracle_closed_forms.py).
+(tests/test_oracle_closed_forms.py, tests/test_oracle_physics_pins.py).
 """
 import os
 import shlex
@@ -17,6 +17,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PINS = "tests/test_oracle_closed_forms.py::"
+PHYS = "tests/test_oracle_physics_pins.py::"
 MUTANTS = {
     1: ("lagrange_Sdep: 4 <-> 16 in M_ij / N_ij", PINS + "test_lagrange_constant_norm_strain_closed_form"),
     2: ("lagrange_Sdep: Beta inverted", PINS + "test_lagrange_constant_norm_strain_closed_form"),
@@ -26,6 +27,10 @@ MUTANTS = {
     6: ("lagrange_scalar_Sdep: 4 <-> 16", PINS + "test_lagrange_scalar_constant_norm_strain_closed_form"),
     7: ("lagrange_scalar_Sdep: Beta inverted", PINS + "test_lagrange_scalar_constant_norm_strain_closed_form"),
     8: ("step: no Euler first step without initu", PINS + "test_step_laminar_column_closed_form[0]"),
+    9: ("step: Coriolis term of RHSy with the wrong sign", PHYS + "test_coriolis_term"),
+    10: ("step: sponge of RHSx at sponge(jz) only", PHYS + "test_sponge_terms[2]"),
+    11: ("interpolag_Ssim: interpolag_Sdep's w_lag clamp", PHYS + "test_interpolag_ssim_vertical_shift"),
+    12: ("model 1: wall-level length at z = dz", PHYS + "test_mason_length_wall_limit"),
 }
 
 
