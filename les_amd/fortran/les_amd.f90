@@ -1,210 +1,22 @@
-! les_amd.f90 -- Fortran (iso_c_binding) side of the C ABI in include/les_amd.h.
+! les_amd.f90 -- Fortran side of the C ABI in include/les_amd.h.
 !
-! module les_amd      : bind(C) interfaces, one per C entry point (device
-!                        pointers as type(c_ptr)), plus les_check.
+! module les_amd_capi : generated iso_c_binding interfaces of every C entry
+!                        point and struct (les_amd_capi.f90, gen_capi.py).
+! module les_amd      : les_amd_capi plus the global context and les_check.
 ! module les_amd_host : host-array wrappers with the reference routines'
-!                        argument lists (filt_da.f90, ddx.f90, convec.f90 ...).
+!                        argument lists (filt_da.f90, ddx.f90, convec.f90,
+!                        sgs_stag.f90 ...), one per routine-level entry point.
 !                        Each keeps a device mirror per host array (les_mirror),
 !                        pushes the inputs, runs the HIP routine and pulls the
-!                        outputs, so a reference routine body can be replaced by
-!                        one call.  The whole-step path (les_step) keeps the
-!                        state on the GPU instead and is the fast path.
+!                        outputs, so a reference routine body can be replaced
+!                        by one call.  Fields are (ld,ny,nlev) real(8), planes
+!                        (ld,ny), surface arrays (nx,ny).  The whole-step path
+!                        (les_step) keeps the state on the GPU instead and is
+!                        the fast path.
 module les_amd
   use iso_c_binding
+  use les_amd_capi
   implicit none
-
-  type, bind(C) :: les_grid
-    integer(c_int) :: nx, ny, nz
-    real(c_double) :: Lx, Ly, Lz, dt, z0, force
-    integer(c_int) :: inilag, initu, cs_count
-  end type les_grid
-
-  ! les_scalar_params: param.f90:53,72-90,139-146,174 and bottombc.f90:26-63
-  type, bind(C) :: les_scalar_params
-    integer(c_int) :: enable, model, initsc, passive
-    real(c_double) :: Pr, z_i, u_star, T_scale, q_scale, g, dTdz_top, dqdz_top
-  end type les_scalar_params
-
-  interface
-    integer(c_int) function les_create(g, ctx) bind(C, name='les_create')
-      import :: c_int, c_ptr, les_grid
-      type(les_grid), intent(in) :: g
-      type(c_ptr), intent(out) :: ctx
-    end function
-    subroutine les_destroy(ctx) bind(C, name='les_destroy')
-      import :: c_ptr
-      type(c_ptr), value :: ctx
-    end subroutine
-    type(c_ptr) function les_last_error(ctx) bind(C, name='les_last_error')
-      import :: c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function les_get_grid(ctx, g) bind(C, name='les_get_grid')
-      import :: c_int, c_ptr, les_grid
-      type(c_ptr), value :: ctx
-      type(les_grid), intent(out) :: g
-    end function
-    integer(c_int) function les_sync(ctx) bind(C, name='les_sync')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-    end function
-    integer(c_int) function les_set_time(ctx, jt, jt_total) bind(C, name='les_set_time')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      integer(c_int), value :: jt, jt_total
-    end function
-    ! --- routine-level (device pointers) ---
-    integer(c_int) function les_filt_da(ctx, f, dfdx, dfdy, nlev) bind(C, name='les_filt_da')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, f, dfdx, dfdy
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_ddx(ctx, dfdx, f, nlev) bind(C, name='les_ddx')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, dfdx, f
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_ddy(ctx, dfdy, f, nlev) bind(C, name='les_ddy')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, dfdy, f
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_ddxy(ctx, dfdx, dfdy, f, nlev) bind(C, name='les_ddxy')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, dfdx, dfdy, f
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_test_filter(ctx, plane, g) bind(C, name='les_test_filter')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, plane, g
-    end function
-    integer(c_int) function les_dealias1(ctx, u, u_big, nlev) bind(C, name='les_dealias1')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, u, u_big
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_dealias2(ctx, u, u_big, nlev) bind(C, name='les_dealias2')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, u, u_big
-      integer(c_int), value :: nlev
-    end function
-    integer(c_int) function les_ddz_uv(ctx, dfdz, f) bind(C, name='les_ddz_uv')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, dfdz, f
-    end function
-    integer(c_int) function les_ddz_w(ctx, dfdz, f) bind(C, name='les_ddz_w')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, dfdz, f
-    end function
-    integer(c_int) function les_convec(ctx, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz) &
-        bind(C, name='les_convec')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz
-    end function
-    integer(c_int) function les_divstress_uv(ctx, divt, tx, ty, tz) bind(C, name='les_divstress_uv')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, divt, tx, ty, tz
-    end function
-    integer(c_int) function les_divstress_w(ctx, divt, tx, ty, tz) bind(C, name='les_divstress_w')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, divt, tx, ty, tz
-    end function
-    integer(c_int) function les_press_stag_array(ctx, u, v, w, divtz, p, dpdx, dpdy) &
-        bind(C, name='les_press_stag_array')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, u, v, w, divtz, p, dpdx, dpdy
-    end function
-    integer(c_int) function les_project(ctx, u, v, w, dpdx, dpdy, dpdz) bind(C, name='les_project')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx, u, v, w, dpdx, dpdy, dpdz
-    end function
-    integer(c_int) function les_rmsdiv(ctx, dudx, dvdy, dwdz, rms) bind(C, name='les_rmsdiv')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx, dudx, dvdy, dwdz
-      real(c_double), intent(out) :: rms
-    end function
-    ! --- mirrors ---
-    type(c_ptr) function les_mirror(ctx, host, bytes) bind(C, name='les_mirror')
-      import :: c_ptr, c_int64_t
-      type(c_ptr), value :: ctx, host
-      integer(c_int64_t), value :: bytes
-    end function
-    integer(c_int) function les_mirror_push(ctx, host, bytes) bind(C, name='les_mirror_push')
-      import :: c_int, c_ptr, c_int64_t
-      type(c_ptr), value :: ctx, host
-      integer(c_int64_t), value :: bytes
-    end function
-    integer(c_int) function les_mirror_pull(ctx, host, bytes) bind(C, name='les_mirror_pull')
-      import :: c_int, c_ptr, c_int64_t
-      type(c_ptr), value :: ctx, host
-      integer(c_int64_t), value :: bytes
-    end function
-    integer(c_int) function les_memcpy_d2h(ctx, dst, src, bytes) bind(C, name='les_memcpy_d2h')
-      import :: c_int, c_ptr, c_int64_t
-      type(c_ptr), value :: ctx, dst, src
-      integer(c_int64_t), value :: bytes
-    end function
-    ! --- whole-step state ---
-    integer(c_int) function les_state_alloc(ctx, precision) bind(C, name='les_state_alloc')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      integer(c_int), value :: precision
-    end function
-    type(c_ptr) function les_state_field(ctx, name, nlev, elsize) bind(C, name='les_state_field')
-      import :: c_ptr, c_char, c_int
-      type(c_ptr), value :: ctx
-      character(kind=c_char), dimension(*), intent(in) :: name
-      integer(c_int), intent(out) :: nlev, elsize
-    end function
-    integer(c_int) function les_state_init_ic(ctx, seed, amp, ustar, kappa_ic) bind(C, name='les_state_init_ic')
-      import :: c_int, c_ptr, c_int64_t, c_double
-      type(c_ptr), value :: ctx
-      integer(c_int64_t), value :: seed
-      real(c_double), value :: amp, ustar, kappa_ic
-    end function
-    integer(c_int) function les_step(ctx, nsteps) bind(C, name='les_step')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      integer(c_int), value :: nsteps
-    end function
-    integer(c_int) function les_set_graph(ctx, enable) bind(C, name='les_set_graph')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      integer(c_int), value :: enable
-    end function
-    ! --- SGS model selection (param.f90 'model': 5 or 3) and scaledep_dynamic.f90 ---
-    integer(c_int) function les_set_sgs_model(ctx, model) bind(C, name='les_set_sgs_model')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      integer(c_int), value :: model
-    end function
-    integer(c_int) function les_scaledep_dynamic(ctx, S, u, v, w, Cs_opt2, beta) &
-         bind(C, name='les_scaledep_dynamic')
-      import :: c_int, c_ptr
-      type(c_ptr), value :: ctx
-      type(c_ptr), intent(in) :: S(6)
-      type(c_ptr), value :: u, v, w, Cs_opt2, beta
-    end function
-    ! --- scalars: theta and q (S_FLAG), convective surface fluxes ---
-    integer(c_int) function les_set_scalars(ctx, p, wt_s, wq_s) bind(C, name='les_set_scalars')
-      import :: c_int, c_ptr, les_scalar_params
-      type(c_ptr), value :: ctx
-      type(les_scalar_params), intent(in) :: p
-      type(c_ptr), value :: wt_s, wq_s
-    end function
-    integer(c_int) function les_state_init_scalar_ic(ctx, seed, T_init, z_inv, inv_strength, amp_K, q_init) &
-         bind(C, name='les_state_init_scalar_ic')
-      import :: c_int, c_ptr, c_int64_t, c_double
-      type(c_ptr), value :: ctx
-      integer(c_int64_t), value :: seed
-      real(c_double), value :: T_init, z_inv, inv_strength, amp_K, q_init
-    end function
-    integer(c_int) function les_scalar_ref(ctx, theta_ref, q_ref) bind(C, name='les_scalar_ref')
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: theta_ref, q_ref
-    end function
-  end interface
 
   ! context used by the host wrappers (set by les_init_global)
   type(c_ptr), save :: les_ctx_global = c_null_ptr
@@ -241,15 +53,58 @@ module les_amd_host
   use iso_c_binding
   use les_amd
   implicit none
+
+  ! sgsmodule.f90 arrays of the Lagrangian models: (ld,ny,nz) fields and the
+  ! per-level averages (nz)
+  type :: les_h_sgs
+    real(c_double), allocatable :: F_LM(:, :, :), F_MM(:, :, :), F_QN(:, :, :), F_NN(:, :, :), &
+         Beta(:, :, :), Cs_opt2(:, :, :), Nu_t(:, :, :), u_lag(:, :, :), v_lag(:, :, :), w_lag(:, :, :), &
+         TnLMMM(:, :, :), TnQNNN(:, :, :), L11t(:, :, :), L22t(:, :, :), L33t(:, :, :), Q11t(:, :, :), &
+         Q22t(:, :, :), Q33t(:, :, :)
+    real(c_double), allocatable :: Beta_avg(:), Betaclip_avg(:)
+  end type les_h_sgs
+
+  ! the scalar model's arrays: G_LM .. Pr_t, Cs2_Pr (ld,ny,nz); phi_h, psi_m (nx,ny)
+  type :: les_h_scalar_sgs
+    real(c_double), allocatable :: G_LM(:, :, :), G_MM(:, :, :), G_QN(:, :, :), G_NN(:, :, :), &
+         u_lag_s(:, :, :), v_lag_s(:, :, :), w_lag_s(:, :, :), Pr_t(:, :, :), Cs2_Pr(:, :, :)
+    real(c_double), allocatable :: phi_h(:, :), psi_m(:, :)
+  end type les_h_scalar_sgs
+
 contains
+
+  ! ---------------------------------------------------------------- sizes
+  type(les_grid) function grid()
+    call les_check(les_get_grid(les_ctx_global, grid), 'les_get_grid')
+  end function grid
 
   integer(c_int64_t) function field_bytes(nlev)
     integer, intent(in) :: nlev
     type(les_grid) :: g
-    call les_check(les_get_grid(les_ctx_global, g), 'les_get_grid')
+    g = grid()
     field_bytes = int(2 * (g%nx / 2 + 1), c_int64_t) * g%ny * nlev * 8_c_int64_t
   end function field_bytes
 
+  integer(c_int64_t) function surf_bytes()
+    type(les_grid) :: g
+    g = grid()
+    surf_bytes = int(g%nx, c_int64_t) * g%ny * 8_c_int64_t
+  end function surf_bytes
+
+  integer(c_int64_t) function big_bytes(nlev)   ! (ld_big, ny2, nlev) of the 3/2 grid
+    integer, intent(in) :: nlev
+    type(les_grid) :: g
+    g = grid()
+    big_bytes = int(2 * ((3 * g%nx / 2) / 2 + 1), c_int64_t) * (3 * g%ny / 2) * nlev * 8_c_int64_t
+  end function big_bytes
+
+  integer(c_int64_t) function lhny_bytes()
+    type(les_grid) :: g
+    g = grid()
+    lhny_bytes = int(g%nx / 2 + 1, c_int64_t) * g%ny * 8_c_int64_t
+  end function lhny_bytes
+
+  ! ---------------------------------------------------------------- mirrors
   type(c_ptr) function pushed(a, nb)
     real(c_double), target, intent(in) :: a(*)
     integer(c_int64_t), intent(in) :: nb
@@ -269,7 +124,81 @@ contains
     call les_check(les_mirror_pull(les_ctx_global, c_loc(a(1)), nb), 'mirror pull')
   end subroutine pull
 
-  ! filt_da.f90:3-42 : filt_da(f_c, dfdx_c, dfdy_c), levels lbz:nz
+  ! ---------------------------------------------------------------- SGS state
+  subroutine les_h_sgs_alloc(s)
+    type(les_h_sgs), intent(inout) :: s
+    type(les_grid) :: g
+    integer :: ld
+    g = grid()
+    ld = 2 * (g%nx / 2 + 1)
+    allocate (s%F_LM(ld, g%ny, g%nz), s%F_MM(ld, g%ny, g%nz), s%F_QN(ld, g%ny, g%nz), s%F_NN(ld, g%ny, g%nz), &
+         s%Beta(ld, g%ny, g%nz), s%Cs_opt2(ld, g%ny, g%nz), s%Nu_t(ld, g%ny, g%nz), s%u_lag(ld, g%ny, g%nz), &
+         s%v_lag(ld, g%ny, g%nz), s%w_lag(ld, g%ny, g%nz), s%TnLMMM(ld, g%ny, g%nz), s%TnQNNN(ld, g%ny, g%nz), &
+         s%L11t(ld, g%ny, g%nz), s%L22t(ld, g%ny, g%nz), s%L33t(ld, g%ny, g%nz), s%Q11t(ld, g%ny, g%nz), &
+         s%Q22t(ld, g%ny, g%nz), s%Q33t(ld, g%ny, g%nz), s%Beta_avg(g%nz), s%Betaclip_avg(g%nz))
+    s%F_LM = 0; s%F_MM = 0; s%F_QN = 0; s%F_NN = 0; s%Beta = 0; s%Cs_opt2 = 0; s%Nu_t = 0
+    s%u_lag = 0; s%v_lag = 0; s%w_lag = 0; s%TnLMMM = 0; s%TnQNNN = 0; s%L11t = 0; s%L22t = 0
+    s%L33t = 0; s%Q11t = 0; s%Q22t = 0; s%Q33t = 0; s%Beta_avg = 0; s%Betaclip_avg = 0
+  end subroutine les_h_sgs_alloc
+
+  type(les_sgs_fields) function sgs_push(s)
+    type(les_h_sgs), target, intent(in) :: s
+    integer(c_int64_t) :: nb, nl
+    nb = field_bytes(size(s%F_LM, 3))
+    nl = 8_c_int64_t * size(s%Beta_avg)
+    sgs_push = les_sgs_fields(pushed(s%F_LM, nb), pushed(s%F_MM, nb), pushed(s%F_QN, nb), pushed(s%F_NN, nb), &
+         pushed(s%Beta, nb), pushed(s%Cs_opt2, nb), pushed(s%Nu_t, nb), pushed(s%u_lag, nb), pushed(s%v_lag, nb), &
+         pushed(s%w_lag, nb), pushed(s%TnLMMM, nb), pushed(s%TnQNNN, nb), pushed(s%L11t, nb), pushed(s%L22t, nb), &
+         pushed(s%L33t, nb), pushed(s%Q11t, nb), pushed(s%Q22t, nb), pushed(s%Q33t, nb), &
+         pushed(s%Beta_avg, nl), pushed(s%Betaclip_avg, nl))
+  end function sgs_push
+
+  subroutine sgs_pull(s)
+    type(les_h_sgs), target, intent(inout) :: s
+    integer(c_int64_t) :: nb, nl
+    nb = field_bytes(size(s%F_LM, 3))
+    nl = 8_c_int64_t * size(s%Beta_avg)
+    call pull(s%F_LM, nb); call pull(s%F_MM, nb); call pull(s%F_QN, nb); call pull(s%F_NN, nb)
+    call pull(s%Beta, nb); call pull(s%Cs_opt2, nb); call pull(s%Nu_t, nb); call pull(s%u_lag, nb)
+    call pull(s%v_lag, nb); call pull(s%w_lag, nb); call pull(s%TnLMMM, nb); call pull(s%TnQNNN, nb)
+    call pull(s%L11t, nb); call pull(s%L22t, nb); call pull(s%L33t, nb); call pull(s%Q11t, nb)
+    call pull(s%Q22t, nb); call pull(s%Q33t, nb); call pull(s%Beta_avg, nl); call pull(s%Betaclip_avg, nl)
+  end subroutine sgs_pull
+
+  subroutine les_h_scalar_sgs_alloc(s)
+    type(les_h_scalar_sgs), intent(inout) :: s
+    type(les_grid) :: g
+    integer :: ld
+    g = grid()
+    ld = 2 * (g%nx / 2 + 1)
+    allocate (s%G_LM(ld, g%ny, g%nz), s%G_MM(ld, g%ny, g%nz), s%G_QN(ld, g%ny, g%nz), s%G_NN(ld, g%ny, g%nz), &
+         s%u_lag_s(ld, g%ny, g%nz), s%v_lag_s(ld, g%ny, g%nz), s%w_lag_s(ld, g%ny, g%nz), &
+         s%Pr_t(ld, g%ny, g%nz), s%Cs2_Pr(ld, g%ny, g%nz), s%phi_h(g%nx, g%ny), s%psi_m(g%nx, g%ny))
+    s%G_LM = 0; s%G_MM = 0; s%G_QN = 0; s%G_NN = 0; s%u_lag_s = 0; s%v_lag_s = 0; s%w_lag_s = 0
+    s%Pr_t = 0; s%Cs2_Pr = 0; s%phi_h = 1; s%psi_m = 0
+  end subroutine les_h_scalar_sgs_alloc
+
+  type(les_scalar_sgs_fields) function ssgs_push(s)
+    type(les_h_scalar_sgs), target, intent(in) :: s
+    integer(c_int64_t) :: nb, n2
+    nb = field_bytes(size(s%G_LM, 3))
+    n2 = surf_bytes()
+    ssgs_push = les_scalar_sgs_fields(pushed(s%G_LM, nb), pushed(s%G_MM, nb), pushed(s%G_QN, nb), &
+         pushed(s%G_NN, nb), pushed(s%u_lag_s, nb), pushed(s%v_lag_s, nb), pushed(s%w_lag_s, nb), &
+         pushed(s%Pr_t, nb), pushed(s%Cs2_Pr, nb), pushed(s%phi_h, n2), pushed(s%psi_m, n2))
+  end function ssgs_push
+
+  subroutine ssgs_pull(s)
+    type(les_h_scalar_sgs), target, intent(inout) :: s
+    integer(c_int64_t) :: nb
+    nb = field_bytes(size(s%G_LM, 3))
+    call pull(s%G_LM, nb); call pull(s%G_MM, nb); call pull(s%G_QN, nb); call pull(s%G_NN, nb)
+    call pull(s%u_lag_s, nb); call pull(s%v_lag_s, nb); call pull(s%w_lag_s, nb); call pull(s%Pr_t, nb)
+    call pull(s%Cs2_Pr, nb)
+  end subroutine ssgs_pull
+
+  ! ---------------------------------------------------------------- spectral
+  ! filt_da.f90:3-42 : filt_da(f_c, dfdx_c, dfdy_c), f Nyquist-filtered in place
   subroutine les_h_filt_da(f, dfdx, dfdy, nlev)
     real(c_double), target :: f(*), dfdx(*), dfdy(*)
     integer, intent(in) :: nlev
@@ -301,6 +230,68 @@ contains
     call pull(dfdy, nb)
   end subroutine les_h_ddy
 
+  ! ddxy.f90:1-29 : ddxy(dfdx, dfdy, f)
+  subroutine les_h_ddxy(dfdx, dfdy, f, nlev)
+    real(c_double), target :: dfdx(*), dfdy(*), f(*)
+    integer, intent(in) :: nlev
+    integer(c_int64_t) :: nb
+    nb = field_bytes(nlev)
+    call les_check(les_ddxy(les_ctx_global, dev(dfdx, nb), dev(dfdy, nb), pushed(f, nb), nlev), 'ddxy')
+    call pull(dfdx, nb)
+    call pull(dfdy, nb)
+  end subroutine les_h_ddxy
+
+  ! test_filtermodule.f90:32-63 : test_filter_init(alpha, G) with G (lh,ny)
+  subroutine les_h_test_filter_init(alpha, G)
+    real(c_double), intent(in) :: alpha
+    real(c_double), target :: G(*)
+    integer(c_int64_t) :: nb
+    nb = lhny_bytes()
+    call les_check(les_test_filter_init(les_ctx_global, alpha, dev(G, nb)), 'test_filter_init')
+    call pull(G, nb)
+  end subroutine les_h_test_filter_init
+
+  ! test_filtermodule.f90:13-30 : test_filter(plane, G), (ld,ny) in place
+  subroutine les_h_test_filter(plane, G)
+    real(c_double), target :: plane(*), G(*)
+    integer(c_int64_t) :: nb
+    nb = field_bytes(1)
+    call les_check(les_test_filter(les_ctx_global, pushed(plane, nb), pushed(G, lhny_bytes())), 'test_filter')
+    call pull(plane, nb)
+  end subroutine les_h_test_filter
+
+  ! padd.f90:1-26 / unpadd.f90:1-22 : complex (lh,ny) <-> (lh_big,ny2) planes
+  subroutine les_h_padd(u_big, u)
+    real(c_double), target :: u_big(*), u(*)
+    call les_check(les_padd(les_ctx_global, dev(u_big, big_bytes(1)), pushed(u, field_bytes(1))), 'padd')
+    call pull(u_big, big_bytes(1))
+  end subroutine les_h_padd
+
+  subroutine les_h_unpadd(cc, cc_big)
+    real(c_double), target :: cc(*), cc_big(*)
+    call les_check(les_unpadd(les_ctx_global, dev(cc, field_bytes(1)), pushed(cc_big, big_bytes(1))), 'unpadd')
+    call pull(cc, field_bytes(1))
+  end subroutine les_h_unpadd
+
+  ! dealias1.f90:1-31 (u untouched) / dealias2.f90:1-33
+  subroutine les_h_dealias1(u, u_big, nlev)
+    real(c_double), target :: u(*), u_big(*)
+    integer, intent(in) :: nlev
+    call les_check(les_dealias1(les_ctx_global, pushed(u, field_bytes(nlev)), dev(u_big, big_bytes(nlev)), nlev), &
+         'dealias1')
+    call pull(u_big, big_bytes(nlev))
+  end subroutine les_h_dealias1
+
+  subroutine les_h_dealias2(u, u_big, nlev)
+    real(c_double), target :: u(*), u_big(*)
+    integer, intent(in) :: nlev
+    call les_check(les_dealias2(les_ctx_global, dev(u, field_bytes(nlev)), pushed(u_big, big_bytes(nlev)), nlev), &
+         'dealias2')
+    call pull(u, field_bytes(nlev))
+    call pull(u_big, big_bytes(nlev))
+  end subroutine les_h_dealias2
+
+  ! ---------------------------------------------------------------- vertical FD
   ! ddz_uv.f90:5-57 / ddz_w.f90:6-51 : (ld,ny,1:nz)
   subroutine les_h_ddz_uv(dfdz, f, nz)
     real(c_double), target :: dfdz(*), f(*)
@@ -320,6 +311,7 @@ contains
     call pull(dfdz, nb)
   end subroutine les_h_ddz_w
 
+  ! ---------------------------------------------------------------- momentum
   ! convec.f90:1-248 : convec(cx, cy, cz) with its sim_param inputs explicit
   subroutine les_h_convec(u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, cx, cy, cz, nz)
     real(c_double), target :: u(*), v(*), w(*), dudy(*), dudz(*), dvdx(*), dvdz(*), dwdx(*), dwdy(*)
@@ -334,6 +326,20 @@ contains
     call pull(cy, nb)
     call pull(cz, nb)
   end subroutine les_h_convec
+
+  ! wallstress.f90:5-69 ('wall') with the surface arrays explicit
+  subroutine les_h_wallstress(u, v, dudz, dvdz, txz, tyz, psi_m, psi_m0, phi_m, ustar_avg, nz)
+    real(c_double), target :: u(*), v(*), dudz(*), dvdz(*), txz(*), tyz(*)
+    real(c_double), target :: psi_m(*), psi_m0(*), phi_m(*), ustar_avg(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb, n2
+    nb = field_bytes(nz)
+    n2 = surf_bytes()
+    call les_check(les_wallstress(les_ctx_global, pushed(u, nb), pushed(v, nb), pushed(dudz, nb), &
+         pushed(dvdz, nb), pushed(txz, nb), pushed(tyz, nb), pushed(psi_m, n2), pushed(psi_m0, n2), &
+         pushed(phi_m, n2), dev(ustar_avg, n2)), 'wallstress')
+    call pull(dudz, nb); call pull(dvdz, nb); call pull(txz, nb); call pull(tyz, nb); call pull(ustar_avg, n2)
+  end subroutine les_h_wallstress
 
   ! divstress_uv.f90:1-63 / divstress_w.f90:4-86
   subroutine les_h_divstress_uv(divt, tx, ty, tz, nz)
@@ -371,6 +377,18 @@ contains
     call pull(dpdy, nb)
   end subroutine les_h_press_stag_array
 
+  ! tridag_array.f90:1-109 : a, b, c real (lh,ny,nz+1), r, u complex (lh,ny,nz+1)
+  subroutine les_h_tridag_array(a, b, c, r, u)
+    real(c_double), target :: a(*), b(*), c(*), r(*), u(*)
+    type(les_grid) :: g
+    integer(c_int64_t) :: nr
+    g = grid()
+    nr = int(g%nx / 2 + 1, c_int64_t) * g%ny * (g%nz + 1) * 8_c_int64_t
+    call les_check(les_tridag_array(les_ctx_global, pushed(a, nr), pushed(b, nr), pushed(c, nr), &
+         pushed(r, 2 * nr), dev(u, 2 * nr)), 'tridag_array')
+    call pull(u, 2 * nr)
+  end subroutine les_h_tridag_array
+
   ! forcing.f90:128-219 : project() with sim_param arrays explicit
   subroutine les_h_project(u, v, w, dpdx, dpdy, dpdz, nz)
     real(c_double), target :: u(*), v(*), w(*), dpdx(*), dpdy(*), dpdz(*)
@@ -383,5 +401,267 @@ contains
     call pull(v, nb)
     call pull(w, nb)
   end subroutine les_h_project
+
+  ! rmsdiv.f90:1-113 : rmsdiv(rms) with the derivatives explicit
+  subroutine les_h_rmsdiv(dudx, dvdy, dwdz, rms, nz)
+    real(c_double), target :: dudx(*), dvdy(*), dwdz(*)
+    real(c_double), target, intent(out) :: rms
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    nb = field_bytes(nz)
+    call les_check(les_rmsdiv(les_ctx_global, pushed(dudx, nb), pushed(dvdy, nb), pushed(dwdz, nb), c_loc(rms)), &
+         'rmsdiv')
+  end subroutine les_h_rmsdiv
+
+  ! scalars_module2.f90:22-70 timestep_conditions: out = the four CFL and
+  ! viscous stability numbers
+  subroutine les_h_timestep_conditions(u, v, w, Nu_t, out, nz)
+    real(c_double), target :: u(*), v(*), w(*), Nu_t(*)
+    real(c_double), intent(out) :: out(4)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    nb = field_bytes(nz)
+    call les_check(les_timestep_conditions(les_ctx_global, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+         pushed(Nu_t, nb), out), 'timestep_conditions')
+  end subroutine les_h_timestep_conditions
+
+  ! ---------------------------------------------------------------- SGS
+  ! sgs_stag.f90:44-53 calc_Sij(dudx, ..., dwdz) -> S11 .. S33
+  subroutine les_h_calc_sij(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, &
+       S11, S12, S13, S22, S23, S33, nz)
+    real(c_double), target :: dudx(*), dudy(*), dudz(*), dvdx(*), dvdy(*), dvdz(*), dwdx(*), dwdy(*), dwdz(*)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: gr(9), S(6)
+    nb = field_bytes(nz)
+    gr = [pushed(dudx, nb), pushed(dudy, nb), pushed(dudz, nb), pushed(dvdx, nb), pushed(dvdy, nb), &
+         pushed(dvdz, nb), pushed(dwdx, nb), pushed(dwdy, nb), pushed(dwdz, nb)]
+    S = [dev(S11, nb), dev(S12, nb), dev(S13, nb), dev(S22, nb), dev(S23, nb), dev(S33, nb)]
+    call les_check(les_calc_sij(les_ctx_global, gr, S), 'calc_sij')
+    call pull(S11, nb); call pull(S12, nb); call pull(S13, nb)
+    call pull(S22, nb); call pull(S23, nb); call pull(S33, nb)
+  end subroutine les_h_calc_sij
+
+  ! sgs_stag.f90:3-366 with the selected model (les_set_sgs_model)
+  subroutine les_h_sgs_stag(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, u, v, w, &
+       txx, txy, txz, tyy, tyz, tzz, s, nz)
+    real(c_double), target :: dudx(*), dudy(*), dudz(*), dvdx(*), dvdy(*), dvdz(*), dwdx(*), dwdy(*), dwdz(*)
+    real(c_double), target :: u(*), v(*), w(*), txx(*), txy(*), txz(*), tyy(*), tyz(*), tzz(*)
+    type(les_h_sgs), target, intent(inout) :: s
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: gr(9), tau(6)
+    nb = field_bytes(nz)
+    gr = [pushed(dudx, nb), pushed(dudy, nb), pushed(dudz, nb), pushed(dvdx, nb), pushed(dvdy, nb), &
+         pushed(dvdz, nb), pushed(dwdx, nb), pushed(dwdy, nb), pushed(dwdz, nb)]
+    tau = [pushed(txx, nb), pushed(txy, nb), pushed(txz, nb), pushed(tyy, nb), pushed(tyz, nb), pushed(tzz, nb)]
+    call les_check(les_sgs_stag(les_ctx_global, gr, pushed(u, nb), pushed(v, nb), pushed(w, nb), tau, &
+         sgs_push(s)), 'sgs_stag')
+    call pull(txx, nb); call pull(txy, nb); call pull(txz, nb)
+    call pull(tyy, nb); call pull(tyz, nb); call pull(tzz, nb)
+    call sgs_pull(s)
+  end subroutine les_h_sgs_stag
+
+  ! lagrange_Sdep.f90:4-478 / lagrange_Ssim.f90:4-300 (model 4, ssim = .true.)
+  subroutine les_h_lagrange(S11, S12, S13, S22, S23, S33, u, v, w, s, nz, ssim)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*), u(*), v(*), w(*)
+    type(les_h_sgs), target, intent(inout) :: s
+    integer, intent(in) :: nz
+    logical, intent(in) :: ssim
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: Sp(6)
+    nb = field_bytes(nz)
+    Sp = [pushed(S11, nb), pushed(S12, nb), pushed(S13, nb), pushed(S22, nb), pushed(S23, nb), pushed(S33, nb)]
+    if (ssim) then
+      call les_check(les_lagrange_ssim(les_ctx_global, Sp, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+           sgs_push(s)), 'lagrange_ssim')
+    else
+      call les_check(les_lagrange_sdep(les_ctx_global, Sp, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+           sgs_push(s)), 'lagrange_sdep')
+    end if
+    call sgs_pull(s)
+  end subroutine les_h_lagrange
+
+  ! interpolag_Sdep.f90:2-380 / interpolag_Ssim.f90 (ssim = .true.)
+  subroutine les_h_interpolag(s, ssim)
+    type(les_h_sgs), target, intent(inout) :: s
+    logical, intent(in) :: ssim
+    if (ssim) then
+      call les_check(les_interpolag_ssim(les_ctx_global, sgs_push(s)), 'interpolag_ssim')
+    else
+      call les_check(les_interpolag_sdep(les_ctx_global, sgs_push(s)), 'interpolag_sdep')
+    end if
+    call sgs_pull(s)
+  end subroutine les_h_interpolag
+
+  ! std_dynamic.f90:4-140 / scaledep_dynamic.f90:4-255 (beta present): the
+  ! per-level Cs_opt2 (and beta) of the plane-averaged dynamic models
+  subroutine les_h_plane_dynamic(S11, S12, S13, S22, S23, S33, u, v, w, cs, nz, beta)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*), u(*), v(*), w(*)
+    real(c_double), target :: cs(*)
+    integer, intent(in) :: nz
+    real(c_double), target, optional :: beta(*)
+    integer(c_int64_t) :: nb, nl
+    type(c_ptr) :: S(6)
+    nb = field_bytes(nz)
+    nl = 8_c_int64_t * nz
+    S = [pushed(S11, nb), pushed(S12, nb), pushed(S13, nb), pushed(S22, nb), pushed(S23, nb), pushed(S33, nb)]
+    if (present(beta)) then
+      call les_check(les_scaledep_dynamic(les_ctx_global, S, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+           dev(cs, nl), dev(beta, nl)), 'scaledep_dynamic')
+      call pull(beta, nl)
+    else
+      call les_check(les_std_dynamic(les_ctx_global, S, pushed(u, nb), pushed(v, nb), pushed(w, nb), dev(cs, nl)), &
+           'std_dynamic')
+    end if
+    call pull(cs, nl)
+  end subroutine les_h_plane_dynamic
+
+  ! ---------------------------------------------------------------- scalars
+  ! obukhov, scalars_module.f90:508-690
+  subroutine les_h_obukhov(u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, &
+       ustar_avg, nz)
+    real(c_double), target :: u(*), v(*), theta(*), qmix(*), sgs_t3(*), sgs_q3(*)
+    real(c_double), target :: psi_m(*), psi_m0(*), psi_h(*), psi_h0(*), phi_m(*), phi_h(*), ustar_avg(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb, np, n2
+    nb = field_bytes(nz)
+    np = field_bytes(1)
+    n2 = surf_bytes()
+    call les_check(les_obukhov(les_ctx_global, pushed(u, nb), pushed(v, nb), pushed(theta, nb), pushed(qmix, nb), &
+         pushed(sgs_t3, np), pushed(sgs_q3, np), pushed(psi_m, n2), pushed(psi_m0, n2), pushed(psi_h, n2), &
+         pushed(psi_h0, n2), pushed(phi_m, n2), pushed(phi_h, n2), pushed(ustar_avg, n2)), 'obukhov')
+    call pull(sgs_t3, np); call pull(sgs_q3, np); call pull(psi_m, n2); call pull(psi_m0, n2)
+    call pull(psi_h, n2); call pull(psi_h0, n2); call pull(phi_m, n2); call pull(phi_h, n2)
+    call pull(ustar_avg, n2)
+  end subroutine les_h_obukhov
+
+  ! calcbeta, scalars_module.f90:175-228
+  subroutine les_h_calcbeta(theta, qmix, beta_scal, nz)
+    real(c_double), target :: theta(*), qmix(*), beta_scal(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    nb = field_bytes(nz)
+    call les_check(les_calcbeta(les_ctx_global, pushed(theta, nb), pushed(qmix, nb), dev(beta_scal, nb)), &
+         'calcbeta')
+    call pull(beta_scal, nb)
+  end subroutine les_h_calcbeta
+
+  ! scalar_RHS_calc, scalars_module.f90:232-454 (which = 0) and
+  ! scalars_module_q.f90:114-330 (which = 1)
+  subroutine les_h_scalar_rhs_calc(which, dsdx, dsdy, dsdz, u, v, w, Nu_t, Pr_, ustar_avg, phi_h, RHS, &
+       sgs_vert1, nz)
+    integer, intent(in) :: which, nz
+    real(c_double), target :: dsdx(*), dsdy(*), dsdz(*), u(*), v(*), w(*), Nu_t(*), Pr_(*)
+    real(c_double), target :: ustar_avg(*), phi_h(*), RHS(*), sgs_vert1(*)
+    integer(c_int64_t) :: nb, n2
+    nb = field_bytes(nz)
+    n2 = surf_bytes()
+    call les_check(les_scalar_rhs_calc(les_ctx_global, which, pushed(dsdx, nb), pushed(dsdy, nb), &
+         pushed(dsdz, nb), pushed(u, nb), pushed(v, nb), pushed(w, nb), pushed(Nu_t, nb), pushed(Pr_, nb), &
+         pushed(ustar_avg, n2), pushed(phi_h, n2), dev(RHS, nb), dev(sgs_vert1, field_bytes(1))), &
+         'scalar_rhs_calc')
+    call pull(dsdz, nb)
+    call pull(RHS, nb)
+    call pull(sgs_vert1, field_bytes(1))
+  end subroutine les_h_scalar_rhs_calc
+
+  ! step_scalar, scalars_module.f90:457-504 / step_scalar_q, scalars_module_q.f90:357-397
+  subroutine les_h_step_scalar(which, scalar, RHS, RHS_f, nz)
+    integer, intent(in) :: which, nz
+    real(c_double), target :: scalar(*), RHS(*), RHS_f(*)
+    integer(c_int64_t) :: nb
+    nb = field_bytes(nz)
+    call les_check(les_step_scalar(les_ctx_global, which, pushed(scalar, nb), pushed(RHS, nb), pushed(RHS_f, nb)), &
+         'step_scalar')
+    call pull(scalar, nb)
+  end subroutine les_h_step_scalar
+
+  ! interpolag_scalar_Sdep.f90:1-316
+  subroutine les_h_interpolag_scalar_sdep(ss)
+    type(les_h_scalar_sgs), target, intent(inout) :: ss
+    call les_check(les_interpolag_scalar_sdep(les_ctx_global, ssgs_push(ss)), 'interpolag_scalar_sdep')
+    call ssgs_pull(ss)
+  end subroutine les_h_interpolag_scalar_sdep
+
+  ! lagrange_scalar_Sdep.f90:1-467 (theta Nyquist-filtered in place)
+  subroutine les_h_lagrange_scalar_sdep(S11, S12, S13, S22, S23, S33, u, v, w, theta, s, ss, nz)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*), u(*), v(*), w(*), theta(*)
+    type(les_h_sgs), target, intent(inout) :: s
+    type(les_h_scalar_sgs), target, intent(inout) :: ss
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: Sp(6)
+    nb = field_bytes(nz)
+    Sp = [pushed(S11, nb), pushed(S12, nb), pushed(S13, nb), pushed(S22, nb), pushed(S23, nb), pushed(S33, nb)]
+    call les_check(les_lagrange_scalar_sdep(les_ctx_global, Sp, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+         pushed(theta, nb), sgs_push(s), ssgs_push(ss)), 'lagrange_scalar_sdep')
+    call pull(theta, nb)
+    call sgs_pull(s)
+    call ssgs_pull(ss)
+  end subroutine les_h_lagrange_scalar_sdep
+
+  ! sgs_stag.f90:3-366 with model = 7
+  subroutine les_h_sgs_stag_scalar(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, u, v, w, theta, &
+       txx, txy, txz, tyy, tyz, tzz, s, ss, nz)
+    real(c_double), target :: dudx(*), dudy(*), dudz(*), dvdx(*), dvdy(*), dvdz(*), dwdx(*), dwdy(*), dwdz(*)
+    real(c_double), target :: u(*), v(*), w(*), theta(*), txx(*), txy(*), txz(*), tyy(*), tyz(*), tzz(*)
+    type(les_h_sgs), target, intent(inout) :: s
+    type(les_h_scalar_sgs), target, intent(inout) :: ss
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: gr(9), tau(6)
+    nb = field_bytes(nz)
+    gr = [pushed(dudx, nb), pushed(dudy, nb), pushed(dudz, nb), pushed(dvdx, nb), pushed(dvdy, nb), &
+         pushed(dvdz, nb), pushed(dwdx, nb), pushed(dwdy, nb), pushed(dwdz, nb)]
+    tau = [pushed(txx, nb), pushed(txy, nb), pushed(txz, nb), pushed(tyy, nb), pushed(tyz, nb), pushed(tzz, nb)]
+    call les_check(les_sgs_stag_scalar(les_ctx_global, gr, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+         pushed(theta, nb), tau, sgs_push(s), ssgs_push(ss)), 'sgs_stag_scalar')
+    call pull(theta, nb)
+    call pull(txx, nb); call pull(txy, nb); call pull(txz, nb)
+    call pull(tyy, nb); call pull(tyz, nb); call pull(tzz, nb)
+    call sgs_pull(s)
+    call ssgs_pull(ss)
+  end subroutine les_h_sgs_stag_scalar
+
+  ! ---------------------------------------------------------------- statistics
+  ! spectrum, io.f90:1613-1646 : spec (nx/2, nlev)
+  subroutine les_h_spectrum(f, nlev, spec)
+    real(c_double), target :: f(*), spec(*)
+    integer, intent(in) :: nlev
+    integer(c_int64_t) :: ns
+    type(les_grid) :: g
+    g = grid()
+    ns = 8_c_int64_t * (g%nx / 2) * nlev
+    call les_check(les_spectrum(les_ctx_global, pushed(f, field_bytes(nlev)), nlev, dev(spec, ns)), 'spectrum')
+    call pull(spec, ns)
+  end subroutine les_h_spectrum
+
+  ! avg_stats, io.f90:1722-1755 : stats (9 nl + nl nx/2), nl = nz - 1
+  subroutine les_h_avg_stats(u, v, w, txz, stats, nz)
+    real(c_double), target :: u(*), v(*), w(*), txz(*), stats(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb, ns
+    type(les_grid) :: g
+    g = grid()
+    nb = field_bytes(nz)
+    ns = 8_c_int64_t * (nz - 1) * (9 + g%nx / 2)
+    call les_check(les_avg_stats(les_ctx_global, pushed(u, nb), pushed(v, nb), pushed(w, nb), pushed(txz, nb), &
+         dev(stats, ns)), 'avg_stats')
+    call pull(stats, ns)
+  end subroutine les_h_avg_stats
+
+  ! obukhov's plane averages, scalars_module.f90:545-561,655-661 (neutral)
+  subroutine les_h_surface_stats(u, v, txz, tyz, psi_m, psi_m0, ustar_avg, out8)
+    real(c_double), target :: u(*), v(*), txz(*), tyz(*), psi_m(*), psi_m0(*), ustar_avg(*), out8(8)
+    integer(c_int64_t) :: np, n2
+    np = field_bytes(1)
+    n2 = surf_bytes()
+    call les_check(les_surface_stats(les_ctx_global, pushed(u, np), pushed(v, np), pushed(txz, np), &
+         pushed(tyz, np), pushed(psi_m, n2), pushed(psi_m0, n2), pushed(ustar_avg, n2), c_null_ptr, c_null_ptr, &
+         c_null_ptr, c_null_ptr, dev(out8, 64_c_int64_t)), 'surface_stats')
+    call pull(out8, 64_c_int64_t)
+  end subroutine les_h_surface_stats
 
 end module les_amd_host
