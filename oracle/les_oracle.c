@@ -20,6 +20,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define BOGUS (-1234567890.0)
 static const double PI = 3.1415926535897932384626433;
@@ -241,58 +244,76 @@ void or_get_tables(const or_ctx *c, double *kx, double *ky, double *k2, double *
 /* ------------------------------------------------------------------ */
 /* 2-D real transforms, FFTW-2 rfftwnd in-place semantics              */
 /* ------------------------------------------------------------------ */
+/* The rows of a plane, then its columns, are independent 1-D transforms:
+   with OpenMP they are shared among the threads (each with its own work
+   vectors); every transform is computed the same way whatever the thread
+   count, so the results do not depend on it. */
 static void rfft2_fwd(const plan2 *P, double *a) {
   const int nx = P->nx, ny = P->ny, lh = P->lh, ld = P->ld;
-  cpx *z = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
-  cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
+  const size_t nmax = (size_t)(nx > ny ? nx : ny);
   cpx *out = (cpx *)malloc(sizeof(cpx) * (size_t)lh * ny);
-  for (int y = 0; y < ny; y += 2) {          /* two real rows per complex FFT */
-    for (int x = 0; x < nx; x++) { z[x].re = a[x + (size_t)ld * y]; z[x].im = a[x + (size_t)ld * (y + 1)]; }
-    fft_exec(&P->px, z, t, -1);
-    for (int k = 0; k < lh; k++) {
-      cpx zk = z[k % nx], zn = z[(nx - k) % nx];
-      /* A = (zk + conj zn)/2, B = (zk - conj zn)/(2i) */
-      cpx A = { 0.5 * (zk.re + zn.re), 0.5 * (zk.im - zn.im) };
-      double dr = zk.re - zn.re, di = zk.im + zn.im;
-      cpx B = { 0.5 * di, -0.5 * dr };
-      out[k + (size_t)lh * y] = A;
-      out[k + (size_t)lh * (y + 1)] = B;
+#pragma omp parallel
+  {
+    cpx *z = (cpx *)malloc(sizeof(cpx) * nmax);
+    cpx *t = (cpx *)malloc(sizeof(cpx) * nmax);
+#pragma omp for schedule(static)
+    for (int y = 0; y < ny; y += 2) {          /* two real rows per complex FFT */
+      for (int x = 0; x < nx; x++) { z[x].re = a[x + (size_t)ld * y]; z[x].im = a[x + (size_t)ld * (y + 1)]; }
+      fft_exec(&P->px, z, t, -1);
+      for (int k = 0; k < lh; k++) {
+        cpx zk = z[k % nx], zn = z[(nx - k) % nx];
+        /* A = (zk + conj zn)/2, B = (zk - conj zn)/(2i) */
+        cpx A = { 0.5 * (zk.re + zn.re), 0.5 * (zk.im - zn.im) };
+        double dr = zk.re - zn.re, di = zk.im + zn.im;
+        cpx B = { 0.5 * di, -0.5 * dr };
+        out[k + (size_t)lh * y] = A;
+        out[k + (size_t)lh * (y + 1)] = B;
+      }
     }
-  }
-  for (int k = 0; k < lh; k++) {
-    for (int y = 0; y < ny; y++) z[y] = out[k + (size_t)lh * y];
-    fft_exec(&P->py, z, t, -1);
-    for (int y = 0; y < ny; y++) out[k + (size_t)lh * y] = z[y];
+#pragma omp for schedule(static)
+    for (int k = 0; k < lh; k++) {
+      for (int y = 0; y < ny; y++) z[y] = out[k + (size_t)lh * y];
+      fft_exec(&P->py, z, t, -1);
+      for (int y = 0; y < ny; y++) out[k + (size_t)lh * y] = z[y];
+    }
+    free(z); free(t);
   }
   memcpy(a, out, sizeof(cpx) * (size_t)lh * ny);
-  free(z); free(t); free(out);
+  free(out);
 }
 
 static void rfft2_inv(const plan2 *P, double *a) {
   const int nx = P->nx, ny = P->ny, lh = P->lh, ld = P->ld;
+  const size_t nmax = (size_t)(nx > ny ? nx : ny);
   cpx *X = (cpx *)malloc(sizeof(cpx) * (size_t)lh * ny);
   memcpy(X, a, sizeof(cpx) * (size_t)lh * ny);
-  cpx *z = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
-  cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)(nx > ny ? nx : ny));
-  for (int k = 0; k < lh; k++) {
-    for (int y = 0; y < ny; y++) z[y] = X[k + (size_t)lh * y];
-    fft_exec(&P->py, z, t, +1);
-    for (int y = 0; y < ny; y++) X[k + (size_t)lh * y] = z[y];
-  }
-  for (int y = 0; y < ny; y += 2) {
-    cpx *A = X + (size_t)lh * y, *B = X + (size_t)lh * (y + 1);
-    A[0].im = 0.0; B[0].im = 0.0; A[nx / 2].im = 0.0; B[nx / 2].im = 0.0;
-    for (int k = 0; k <= nx / 2; k++) {
-      z[k].re = A[k].re - B[k].im; z[k].im = A[k].im + B[k].re;          /* A + iB */
-      if (k > 0 && k < nx / 2) {
-        z[nx - k].re = A[k].re + B[k].im; z[nx - k].im = B[k].re - A[k].im; /* conj A + i conj B */
-      }
+#pragma omp parallel
+  {
+    cpx *z = (cpx *)malloc(sizeof(cpx) * nmax);
+    cpx *t = (cpx *)malloc(sizeof(cpx) * nmax);
+#pragma omp for schedule(static)
+    for (int k = 0; k < lh; k++) {
+      for (int y = 0; y < ny; y++) z[y] = X[k + (size_t)lh * y];
+      fft_exec(&P->py, z, t, +1);
+      for (int y = 0; y < ny; y++) X[k + (size_t)lh * y] = z[y];
     }
-    fft_exec(&P->px, z, t, +1);
-    for (int x = 0; x < nx; x++) { a[x + (size_t)ld * y] = z[x].re; a[x + (size_t)ld * (y + 1)] = z[x].im; }
-    for (int x = nx; x < ld; x++) { a[x + (size_t)ld * y] = 0.0; a[x + (size_t)ld * (y + 1)] = 0.0; }
+#pragma omp for schedule(static)
+    for (int y = 0; y < ny; y += 2) {
+      cpx *A = X + (size_t)lh * y, *B = X + (size_t)lh * (y + 1);
+      A[0].im = 0.0; B[0].im = 0.0; A[nx / 2].im = 0.0; B[nx / 2].im = 0.0;
+      for (int k = 0; k <= nx / 2; k++) {
+        z[k].re = A[k].re - B[k].im; z[k].im = A[k].im + B[k].re;          /* A + iB */
+        if (k > 0 && k < nx / 2) {
+          z[nx - k].re = A[k].re + B[k].im; z[nx - k].im = B[k].re - A[k].im; /* conj A + i conj B */
+        }
+      }
+      fft_exec(&P->px, z, t, +1);
+      for (int x = 0; x < nx; x++) { a[x + (size_t)ld * y] = z[x].re; a[x + (size_t)ld * (y + 1)] = z[x].im; }
+      for (int x = nx; x < ld; x++) { a[x + (size_t)ld * y] = 0.0; a[x + (size_t)ld * (y + 1)] = 0.0; }
+    }
+    free(z); free(t);
   }
-  free(X); free(z); free(t);
+  free(X);
 }
 
 void or_rfft2_fwd(or_ctx *c, double *plane, int big) { rfft2_fwd(big ? &c->bg : &c->sm, plane); }
@@ -324,6 +345,7 @@ static void mul_ik(const or_ctx *c, const double *k, const double *in, double *o
 void or_filt_da(or_ctx *c, double *f, double *dfdx, double *dfdy, int nlev) {
   const size_t pl = (size_t)c->ld * c->ny;
   const double cst = 1.0 / (c->nx * c->ny);
+#pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nlev; k++) {
     double *F = f + pl * k, *DX = dfdx + pl * k, *DY = dfdy + pl * k;
     for (size_t i = 0; i < pl; i++) F[i] = cst * F[i];
@@ -340,6 +362,7 @@ void or_filt_da(or_ctx *c, double *f, double *dfdx, double *dfdy, int nlev) {
 static void ddxy_one(or_ctx *c, double *out, const double *f, const double *k, int nlev) {
   const size_t pl = (size_t)c->ld * c->ny;
   const double cst = 1.0 / (c->nx * c->ny);
+#pragma omp parallel for schedule(dynamic)
   for (int l = 0; l < nlev; l++) {
     double *O = out + pl * l; const double *F = f + pl * l;
     for (size_t i = 0; i < pl; i++) O[i] = cst * F[i];
@@ -355,6 +378,7 @@ void or_ddy(or_ctx *c, double *dfdy, const double *f, int nlev) { ddxy_one(c, df
 void or_ddxy(or_ctx *c, double *dfdx, double *dfdy, const double *f, int nlev) {
   const size_t pl = (size_t)c->ld * c->ny;
   const double cst = 1.0 / (c->nx * c->ny);
+#pragma omp parallel for schedule(dynamic)
   for (int l = 0; l < nlev; l++) {
     double *DX = dfdx + pl * l, *DY = dfdy + pl * l; const double *F = f + pl * l;
     for (size_t i = 0; i < pl; i++) DX[i] = cst * F[i];
@@ -423,14 +447,18 @@ void or_unpadd(or_ctx *c, double *cc, const double *big) {
 void or_dealias1(or_ctx *c, const double *u, double *u_big, int nlev) {
   const size_t pl = (size_t)c->ld * c->ny, plb = (size_t)c->ld_big * c->ny2;
   const double cst = 1.0 / (c->nx * c->ny);
-  double *tmp = (double *)malloc(sizeof(double) * pl);
-  for (int k = 0; k < nlev; k++) {
-    for (size_t i = 0; i < pl; i++) tmp[i] = cst * u[pl * k + i];
-    rfft2_fwd(&c->sm, tmp);
-    or_padd(c, u_big + plb * k, tmp);
-    rfft2_inv(&c->bg, u_big + plb * k);
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc(sizeof(double) * pl);
+#pragma omp for schedule(dynamic)
+    for (int k = 0; k < nlev; k++) {
+      for (size_t i = 0; i < pl; i++) tmp[i] = cst * u[pl * k + i];
+      rfft2_fwd(&c->sm, tmp);
+      or_padd(c, u_big + plb * k, tmp);
+      rfft2_inv(&c->bg, u_big + plb * k);
+    }
+    free(tmp);
   }
-  free(tmp);
 }
 
 /* dealias2.f90:1-33 */
@@ -438,6 +466,7 @@ void or_dealias2(or_ctx *c, double *u, double *u_big, int nlev) {
   const size_t pl = (size_t)c->ld * c->ny, plb = (size_t)c->ld_big * c->ny2;
   const double cst = 1.0 / (c->nx2 * c->ny2);
   for (size_t i = 0; i < plb * nlev; i++) u_big[i] = cst * u_big[i];
+#pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nlev; k++) {
     rfft2_fwd(&c->bg, u_big + plb * k);
     or_unpadd(c, u + pl * k, u_big + plb * k);
@@ -489,9 +518,9 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
   double *b_v1 = (double *)calloc(plb * nz, sizeof(double));
   double *b_v2 = (double *)calloc(plb * nz, sizeof(double));
   double *b_v3 = (double *)calloc(plb * nz, sizeof(double));
-  double *cc = (double *)calloc(plb, sizeof(double));
   double cst = 1.0 / (c->nx * c->ny);
   /* convec.f90:45-65 */
+#pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     double *X = cx + pl * k, *Y = cy + pl * k, *Z = cz + pl * k;
     for (size_t i = 0; i < pl; i++) { X[i] = cst * u1[pl * k + i]; Y[i] = cst * u2[pl * k + i]; Z[i] = cst * u3[pl * k + i]; }
@@ -500,6 +529,7 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
     rfft2_inv(&c->bg, b_u1 + plb * k); rfft2_inv(&c->bg, b_u2 + plb * k); rfft2_inv(&c->bg, b_u3 + plb * k);
   }
   /* convec.f90:67-114 vorticity */
+#pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     double *X = cx + pl * k, *Y = cy + pl * k, *Z = cz + pl * k;
     for (size_t i = 0; i < pl; i++) {
@@ -519,6 +549,10 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
   }
   cst = 1.0 / (c->nx2 * c->ny2);
   /* CX: convec.f90:121-152 */
+#pragma omp parallel
+  {
+    double *cc = (double *)malloc(sizeof(double) * plb);
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz - 1; k++) {
     for (size_t i = 0; i < plb; i++) {
       if (k == 0)
@@ -532,7 +566,13 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
     or_unpadd(c, cx + pl * k, cc);
     rfft2_inv(&c->sm, cx + pl * k);
   }
+    free(cc);
+  }
   /* CY: convec.f90:157-188 */
+#pragma omp parallel
+  {
+    double *cc = (double *)malloc(sizeof(double) * plb);
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz - 1; k++) {
     for (size_t i = 0; i < plb; i++) {
       if (k == 0)
@@ -546,7 +586,13 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
     or_unpadd(c, cy + pl * k, cc);
     rfft2_inv(&c->sm, cy + pl * k);
   }
+    free(cc);
+  }
   /* CZ: convec.f90:192-233 */
+#pragma omp parallel
+  {
+    double *cc = (double *)malloc(sizeof(double) * plb);
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz - 1; k++) {
     for (size_t i = 0; i < plb; i++) {
       if (k == 0)
@@ -559,8 +605,10 @@ void or_convec(or_ctx *c, const double *u1, const double *u2, const double *u3,
     or_unpadd(c, cz + pl * k, cc);
     rfft2_inv(&c->sm, cz + pl * k);
   }
+    free(cc);
+  }
   for (size_t i = 0; i < pl; i++) { cx[pl * (nz - 1) + i] = BOGUS; cy[pl * (nz - 1) + i] = BOGUS; cz[pl * (nz - 1) + i] = BOGUS; }
-  free(b_u1); free(b_u2); free(b_u3); free(b_v1); free(b_v2); free(b_v3); free(cc);
+  free(b_u1); free(b_u2); free(b_u3); free(b_v1); free(b_v2); free(b_v3);
 }
 
 /* ------------------------------------------------------------------ */
@@ -743,9 +791,14 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
          SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
          SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
          SMAG, NT };
-  double *T[NT];
-  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
   const int jt = c->jt;
+  const int init_lm = c->P.inilag && !c->flm_init && (jt == c->P.cs_count || jt == 1);
+  const int init_qn = c->P.inilag && !c->fqn_init && (jt == c->P.cs_count || jt == 1);
+#pragma omp parallel
+  {
+    double *T[NT];
+    for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     const size_t o = pl * k;
     for (size_t i = 0; i < pl; i++) {
@@ -794,8 +847,6 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
     for (int q = 0; q < 6; q++) or_test_filter(c, T[SSB11 + q], c->Gt);
     for (int q = 0; q < 6; q++) or_test_filter(c, T[SSH11 + q], c->Gtt);
     double *FLM = s->F_LM + o, *FMM = s->F_MM + o, *FQN = s->F_QN + o, *FNN = s->F_NN + o;
-    int init_lm = c->P.inilag && !c->flm_init && (jt == c->P.cs_count || jt == 1);
-    int init_qn = c->P.inilag && !c->fqn_init && (jt == c->P.cs_count || jt == 1);
     double sum2d = 0.0, sum4d = 0.0, sumcs = 0.0;
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < ld; x++) {
@@ -857,12 +908,13 @@ void or_lagrange_sdep(or_ctx *c, const double *const S[6], const double *u,
         s->Cs_opt2[o + i] = fmax(e24, fq(cso));
         if (x < nx) { sum2d += cs2d; sum4d += cs4d; sumcs += s->Cs_opt2[o + i]; }
       }
-    if (init_lm && k == nz - 1) c->flm_init = 1;
-    if (init_qn && k == nz - 1) c->fqn_init = 1;
     s->Beta_avg[k] = sum2d / sumcs;
     s->Betaclip_avg[k] = sum4d / sum2d;
   }
-  for (int t = 0; t < NT; t++) free(T[t]);
+    for (int t = 0; t < NT; t++) free(T[t]);
+  }
+  if (init_lm) c->flm_init = 1;
+  if (init_qn) c->fqn_init = 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -930,10 +982,13 @@ void or_lagrange_ssim(or_ctx *c, const double *const S[6], const double *u,
   or_interpolag_ssim(c, s);                                          /* :123 */
   const double lagran_dt = c->P.dt * (double)c->P.cs_count;
   const double cst = 2.0 * (delta * delta);
-  double *T[LV_N];
-  for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
   const int jt = c->jt;
   const int init = c->P.inilag && !c->flm_init && (jt == c->P.cs_count || jt == 1);   /* :235-244 */
+#pragma omp parallel
+  {
+    double *T[LV_N];
+    for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     const size_t o = pl * k;
     level_2delta(c, S, u, v, w, k, T);
@@ -962,8 +1017,9 @@ void or_lagrange_ssim(or_ctx *c, const double *const S[6], const double *u,
     s->Beta_avg[k] = (double)nx * ny;                                /* :270-271, sums of Beta = 1 */
     s->Betaclip_avg[k] = (double)nx * ny;
   }
+    for (int t = 0; t < LV_N; t++) free(T[t]);
+  }
   if (init) c->flm_init = 1;
-  for (int t = 0; t < LV_N; t++) free(T[t]);
 }
 
 /* std_dynamic.f90:4-140: one coefficient per level from plane sums of L:M
@@ -974,8 +1030,11 @@ void or_std_dynamic(or_ctx *c, const double *const S[6], const double *u, const 
   const size_t pl = (size_t)ld * ny;
   const double delta = pow(c->dx * c->dy * c->dz, 1.0 / 3.0);        /* :28 */
   const double cst = 2.0 * (delta * delta);                          /* :124 */
-  double *T[LV_N];
-  for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp parallel
+  {
+    double *T[LV_N];
+    for (int t = 0; t < LV_N; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     level_2delta(c, S, u, v, w, k, T);
     csum_t num = { 0.0, 0.0 }, den = { 0.0, 0.0 };
@@ -987,7 +1046,8 @@ void or_std_dynamic(or_ctx *c, const double *const S[6], const double *u, const 
     }
     Cs_opt2[k] = fmax(0.0, csum_val(&num) / csum_val(&den));         /* :135-138 */
   }
-  for (int t = 0; t < LV_N; t++) free(T[t]);
+    for (int t = 0; t < LV_N; t++) free(T[t]);
+  }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1224,6 +1284,7 @@ void or_press_stag_array(or_ctx *c, const double *u, const double *v, const doub
   /* p_hat(:,:,0)=0 */
   for (size_t i = 0; i < pl; i++) p[i] = 0.0;
   /* jz = 1..nz-1 -> array level k = jz-1 */
+#pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nz - 1; k++) {
     for (size_t i = 0; i < pl; i++) {
       Hx[pl * k + i] = cst / TADV1 * (u[pl * k + i] / dt);
@@ -1291,6 +1352,7 @@ void or_press_stag_array(or_ctx *c, const double *u, const double *v, const doub
   }
   for (int j = 0; j < n; j++) kill_oddballs(c, p + pl * j);
   rfft2_inv(&c->sm, p);
+#pragma omp parallel for schedule(dynamic)
   for (int k = 1; k <= nz - 1; k++) {
     double *P = p + pl * k;
     mul_ik(c, c->kx, P, dpdx + pl * (k - 1));
@@ -1444,6 +1506,22 @@ void or_state_destroy(or_state *s) {
   free(h->G_LM); free(h->G_MM); free(h->G_QN); free(h->G_NN);
   free(h->u_lag_s); free(h->v_lag_s); free(h->w_lag_s); free(h->Pr_t); free(h->Cs2_Pr);
   free(s);
+}
+
+/* OpenMP threads the oracle's 2-D transforms use (1 without OpenMP) */
+int or_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }
 
 double or_uniform(uint64_t seed, uint64_t index) {
@@ -1951,9 +2029,14 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
   enum { UB, VB, WB, TB, UH, VH, WH, TH, L1, L2, L3, Q1, Q2, Q3,
          SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
          SXB, SYB, SZB, SXH, SYH, SZH, DXB, DYB, DZB, DXH, DYH, DZH, SMAG, NT };
-  double *T[NT];
-  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
   const int jt = c->jt;
+  const int init_lm = c->P.inilag && !c->glm_init && (jt == c->P.cs_count || jt == 1);
+  const int init_qn = c->P.inilag && !c->gqn_init && (jt == c->P.cs_count || jt == 1);
+#pragma omp parallel
+  {
+    double *T[NT];
+    for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {                                                  /* :138-398 */
     const size_t o = pl * k;
     for (size_t i = 0; i < pl; i++) {
@@ -2006,8 +2089,6 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
     for (int q = 0; q < 3; q++) { or_test_filter(c, T[SXB + q], c->Gt); or_test_filter(c, T[SXH + q], c->Gtt); }
     for (int q = 0; q < 3; q++) { or_test_filter(c, T[DXB + q], c->Gt); or_test_filter(c, T[DXH + q], c->Gtt); }
     double *GLM = g->G_LM + o, *GMM = g->G_MM + o, *GQN = g->G_QN + o, *GNN = g->G_NN + o;
-    const int init_lm = c->P.inilag && !c->glm_init && (jt == c->P.cs_count || jt == 1);
-    const int init_qn = c->P.inilag && !c->gqn_init && (jt == c->P.cs_count || jt == 1);
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < ld; x++) {
         const size_t i = x + (size_t)ld * y;
@@ -2064,10 +2145,11 @@ void or_lagrange_scalar_sdep(or_ctx *c, const double *const S[6], const double *
         g->Cs2_Pr[o + i] = cpr;
         g->Pr_t[o + i] = m->Cs_opt2[o + i] / cpr;                                  /* :395 */
       }
-    if (init_lm && k == nz - 1) c->glm_init = 1;
-    if (init_qn && k == nz - 1) c->gqn_init = 1;
   }
-  for (int t = 0; t < NT; t++) free(T[t]);
+    for (int t = 0; t < NT; t++) free(T[t]);
+  }
+  if (init_lm) c->glm_init = 1;
+  if (init_qn) c->gqn_init = 1;
   free(theta_t); free(dTdx); free(dTdy); free(dTdz);
 }
 
@@ -2151,9 +2233,12 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
          SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
          SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
          SMAG, SBAR, SHAT, NT };
-  double *T[NT];
-  for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
   const double nxy = (double)(nx * ny);
+#pragma omp parallel
+  {
+    double *T[NT];
+    for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
     const size_t o = pl * k;
     for (size_t i = 0; i < pl; i++) {                                      /* :44-73 */
@@ -2254,7 +2339,9 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
       }
     Cs_opt2[k] = fmax(0.0, num / den);
   }
-  for (int t = 0; t < NT; t++) free(T[t]);
+    for (int t = 0; t < NT; t++) free(T[t]);
+  }
+
 }
 
 /* ------------------------------------------------------------------ */

@@ -273,6 +273,10 @@ void or_surface_stats(or_ctx *c, const double *u, const double *v, const double 
                       const double *psi_m, const double *psi_m0, const double *ustar_avg, const double *theta,
                       const double *qmix, const double *sgs_t3, const double *sgs_q3, double out[8]);
 
+/* OpenMP threads of the 2-D transforms (results do not depend on them) */
+int  or_threads(void);
+void or_set_threads(int n);
+
 /* counter-based uniform in [0,1): splitmix64 of (seed, index) */
 double or_uniform(uint64_t seed, uint64_t index);
 
