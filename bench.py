@@ -48,7 +48,9 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
-    tuned = sim.autotune(7)["chosen"] if tune else None  # launch shapes only, results unchanged
+    # tile widths, tridiagonal schedules and padded-product variants; the FFT
+    # plan and the cyclic-reduction schedule change the rounding, not the method
+    tuned = sim.autotune(7)["chosen"] if tune else None
     sim.set_graph(False)
     sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
     sim.set_graph(graph)
@@ -105,7 +107,12 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     """Learned-closure run: the FP32-bulk step with lagrange_Sdep every `period`
     steps and the network at the other cs_count multiples.  The network is
     first trained on the GPU on `samples` (features, Cs_opt2/Beta) pairs
-    harvested from the dynamic updates of the same workload."""
+    harvested from six dynamic updates of the same workload past jt_total =
+    500, where Cs_opt2 drives Nu_t.  `held_out`: on a run from another seed,
+    the relative L2 error against the dynamic Cs_opt2 of the network, of the
+    constant 0.16^2 used without a dynamic model, of the best constant, of the
+    mean Cs^2(z) profile of the training data and of the previous update's
+    field (5 steps stale)."""
     import torch
     from les_amd.api import LesSim, checkerboard_flux
     scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
@@ -113,13 +120,14 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     sim.init_ic(seed=42)
     if conv:
         sim.init_scalar_ic(seed=42, **SCALAR_IC)
+    sim.set_time(0, 1000)   # past the constant-Cs period of sgs_stag.f90:166-173
     sim.autotune(7)
     sim.mlp_init(1)
-    nharv = 4
+    nharv = 6
     gen = torch.Generator(device="cuda").manual_seed(0)
     Xs, Ys = [], []
     sim.step(10)
-    for _ in range(nharv):          # dynamic updates at jt = 15, 20, 25, 30
+    for _ in range(nharv):          # dynamic updates at jt = 15, 20, ..., 40
         sim.step(5)
         X, Y = sim.mlp_harvest()
         idx = torch.randint(0, X.shape[0], (samples // nharv,), device="cuda", generator=gen)
@@ -135,7 +143,10 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     torch.cuda.synchronize()
     train_ms = (time.perf_counter() - t0) * 1e3
     loss0, loss1 = float(hist[:20].mean()), float(hist[-100:].mean())
+    prof_cs = Y[:, 0].reshape(-1, nz - 1, nx * ny).mean(dim=(0, 2))   # training mean Cs^2(z)
+    best_cs = float(Y[:, 0].mean())
     del X, Y
+    held = held_out_gain(sim, nx, ny, nz, conv, prof_cs, best_cs)
     sim.set_closure(1, period=period, threshold=threshold)
     sim.step(max(warmup, 2 * period))  # eager: the closure's windows run once before capture
     judge = sim.closure_judge()
@@ -163,7 +174,7 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     out = {"ms_per_step": ms, "period": period, "threshold": threshold, "train_samples": samples,
            "train_steps": train_steps, "train_batch": 8192, "train_ms": train_ms, "loss_first": loss0,
            "loss_last": loss1, "judge_rel_err": float(judge2[2]), "judge_rel_err_first": float(judge[2]),
-           "network_in_use": bool(judge2[3] == 1.0)}
+           "network_in_use": bool(judge2[3] == 1.0), "held_out": held}
     if prof:
         P = nx * ny * (nz - 1)
         flop = 2.0 * (8 * 64 + 64 * 64 * 2 + 64 * 2)  # per point, forward
@@ -173,6 +184,44 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
                 out[tag] = {"launches": n, "avg_ms": t_ms / n, "GB/s": b / n / (t_ms / n * 1e-3) / 1e9,
                             "TFLOP/s": P * flop / (t_ms / n * 1e-3) / 1e12}
     return out
+
+
+def held_out_gain(trained, nx, ny, nz, conv, prof_cs, best_cs, updates=4):
+    """The trained network on a run from another seed (seed 7): at each of
+    `updates` dynamic updates past jt_total = 500, relative L2 errors against
+    the dynamic Cs_opt2 (lagrange_Sdep's output)."""
+    import numpy as np
+    import torch
+    from les_amd.api import LesSim, checkerboard_flux
+    scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
+    b = LesSim(nx, ny, nz, precision="fp32", dt=1e-4, graph=False, scalars=scal)
+    b.init_ic(seed=7)
+    if conv:
+        b.init_scalar_ic(seed=7, **SCALAR_IC)
+    b.set_time(0, 1000)
+    b.mlp_init(0)
+    b.mlp_set_params(trained.mlp_params())
+    b.mlp_set_norm(*trained.mlp_norm())
+    b.step(10)
+    prev, rows = None, []
+    prof = prof_cs.repeat_interleave(nx * ny)
+    for _ in range(updates):
+        b.step(5)
+        X, Y = b.mlp_harvest()
+        cs = Y[:, 0]
+        net = b.mlp_forward(X)[:, 0].clamp(min=0)
+        rel = lambda p: float(torch.linalg.norm(p - cs) / torch.linalg.norm(cs))
+        rows.append({"net": rel(net), "const_ref": rel(torch.full_like(cs, float(np.float32(0.16) ** 2))),
+                     "const_best": rel(torch.full_like(cs, best_cs)), "profile": rel(prof),
+                     "previous": rel(prev) if prev is not None else None})
+        prev = cs.clone()
+        del X, Y, net
+    del b
+    torch.cuda.empty_cache()
+    mean = lambda k: float(np.mean([r[k] for r in rows if r[k] is not None]))
+    return {"updates": rows, "net": mean("net"), "const_ref": mean("const_ref"), "const_best": mean("const_best"),
+            "profile": mean("profile"), "previous": mean("previous"),
+            "gain_vs_const_ref": mean("const_ref") / mean("net"), "gain_vs_profile": mean("profile") / mean("net")}
 
 
 def run_ensemble(steps, warmup, n_each=8, grids=((64, 64, 49), (96, 96, 65), (128, 128, 65))):
@@ -238,27 +287,33 @@ def run_ensemble(steps, warmup, n_each=8, grids=((64, 64, 49), (96, 96, 65), (12
 
 
 def fp32_error(f32, f64, nx, nz):
-    """Per-field relative L2 error of the FP32-bulk state against the FP64
-    state after the same steps, and the worst plane-mean profile error."""
+    """Relative L2 error of the FP32-bulk state against the FP64 state after
+    the same steps, per field, and of the plane-mean profiles of u, v and
+    theta - T_init/T_scale (the v profile, whose mean is near zero, measured
+    against the norm of the u profile)."""
     out = {}
+    ref = {"theta": SCALAR_IC["T_init"] / SCALARS["T_scale"]}
+    l2 = lambda x: float((x ** 2).sum() ** 0.5)
+    prof = {}
     for n in f64:
         a, b = f32[n], f64[n]
         lev = slice(1, nz) if n == "p" else (slice(1, nz - 1) if n == "w" else slice(0, nz - 1))
         a, b = a[lev, :, :nx], b[lev, :, :nx]
-        out[n] = float(((a - b) ** 2).sum() ** 0.5 / max((b ** 2).sum() ** 0.5, 1e-300))
+        out[n] = l2(a - b) / max(l2(b), 1e-300)
         if n in ("u", "v", "theta"):
-            pa, pb = a.mean(axis=(1, 2)), b.mean(axis=(1, 2))
-            out[n + "_mean_profile"] = float(abs(pa - pb).max() / max(abs(pb).max(), 1e-300))
+            prof[n] = (a.mean(axis=(1, 2)) - ref.get(n, 0.0), b.mean(axis=(1, 2)) - ref.get(n, 0.0))
+    for n, (pa, pb) in prof.items():
+        den = l2(prof["u"][1]) if n == "v" else l2(pb)
+        out[n + "_mean_profile_rel_l2"] = l2(pa - pb) / max(den, 1e-300)
     return out
 
 
-def cpu_baseline(nx, ny, nz, nz_sample, conv=True):
-    """Oracle (single-threaded C restatement) on the full x-y grid and
-    nz_sample w-levels: one plain step and one step with the dynamic update;
-    per-step time scaled linearly to the full level count."""
+def _cpu_step_times(nx, ny, nz, conv):
+    """One plain step and one dynamic-update step of the oracle on the given
+    grid (seconds)."""
     from oracle.oracle import Oracle, OracleSim, ScalarParams
     from les_amd.api import checkerboard_flux
-    o = Oracle(nx, ny, nz_sample, dt=1e-4)
+    o = Oracle(nx, ny, nz, dt=1e-4)
     if conv:
         o.set_scalars(ScalarParams(*SCALARS.values()), checkerboard_flux(nx, ny), None)
         so = OracleSim(o, seed=42, scalar_ic=SCALAR_IC)
@@ -270,10 +325,39 @@ def cpu_baseline(nx, ny, nz, nz_sample, conv=True):
     so.s.contents.jt = 4   # next step is jt = 5: lagrange_Sdep runs
     so.step(1)
     t2 = time.perf_counter()
-    t_plain, t_lag = t1 - t0, t2 - t1
-    scale = (nz - 1) / (nz_sample - 1)
-    t_step = (4 * t_plain + t_lag) / 5 * scale
-    return nx * ny * (nz - 1) / t_step, t_plain, t_lag
+    del so, o
+    return t1 - t0, t2 - t1
+
+
+def cpu_baseline(nx, ny, nz, nz_samples, conv=True):
+    """The oracle (C FP64 restatement, OpenMP over the rows, columns and
+    levels of its transforms) on this host: on all the threads OpenMP gives it
+    on the full grid, and on one thread at two sample depths whose per-level
+    cost shows the linear scaling used to extrapolate to the full depth.  A
+    step is weighted 4 plain : 1 dynamic update (cs_count = 5)."""
+    import oracle.oracle as O
+    nthr = O.threads()
+    tp, tl = _cpu_step_times(nx, ny, nz, conv)
+    t_all = (4 * tp + tl) / 5
+    O.set_threads(1)
+    one = []
+    for nzs in nz_samples:
+        a, b = _cpu_step_times(nx, ny, nzs, conv)
+        one.append((nzs, a, b, (4 * a + b) / 5))
+    O.set_threads(nthr)
+    (z1, _, _, s1), (z2, _, _, s2) = one
+    per_level = (s2 - s1) / (z2 - z1)            # seconds per level and step, one thread
+    t_one = s2 + per_level * (nz - z2)
+    pts = nx * ny * (nz - 1)
+    return {"value": pts / t_all, "unit": "grid-point-steps/s", "cores": nthr, "kind": "port",
+            "host_cpu_count": os.cpu_count(),
+            "sample": (f"oracle (C FP64 restatement) on the full {nx}x{ny}x{nz - 1} grid with {nthr} OpenMP "
+                       f"threads: plain step {tp:.2f} s, dynamic-update step {tl:.2f} s, weighted 4:1"),
+            "one_core": {"value": pts / t_one, "cores": 1,
+                         "sample": [{"levels": z - 1, "plain_s": a, "dynamic_s": b} for z, a, b, _ in one],
+                         "per_level_s": per_level,
+                         "extrapolated_step_s": t_one},
+            "speedup_all_vs_one": t_one / t_all}
 
 
 def main():
@@ -286,7 +370,10 @@ def main():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=129)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--cpu-nz", type=int, default=17, help="w-levels in the CPU baseline sample")
+    ap.add_argument("--cpu-nz", type=int, nargs=2, default=(9, 17),
+                    help="w-levels of the two one-thread CPU samples")
+    ap.add_argument("--drift-steps", type=int, default=100,
+                    help="steps after which the FP32-bulk state is compared with the FP64 one")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", choices=("convective", "neutral"), default="convective")
@@ -303,18 +390,20 @@ def main():
 
     graph = not args.no_graph
     conv = args.workload == "convective"
+    D = max(args.drift_steps, W + K + args.profile_steps)
     ms32, prof32, ok32, rms32, cfl32, f32, tune32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
+                                                            extra_steps=D - W - K - args.profile_steps,
                                                             keep=not args.no_fp64, conv=conv, tune=not args.no_tune)
     print(f"fp32-bulk: {ms32:.3f} ms/step finite={ok32} rmsdiv={rms32:.3e} cfl={cfl32[0]:.3e}",
           file=sys.stderr, flush=True)
     ms64, err = None, None
     if not args.no_fp64:
         ms64, prof64, ok64, rms64, cfl64, f64, tune64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
-                                                                extra_steps=args.profile_steps, keep=True, conv=conv,
+                                                                extra_steps=D - W - K, keep=True, conv=conv,
                                                                 tune=not args.no_tune)
         print(f"fp64: {ms64:.3f} ms/step finite={ok64} rmsdiv={rms64:.3e}", file=sys.stderr, flush=True)
         err = fp32_error(f32, f64, nx, nz)
-        err["steps"] = W + K + args.profile_steps
+        err["steps"] = D
         del f32, f64
 
     closure = None
@@ -324,7 +413,10 @@ def main():
         closure["speedup_vs_dynamic"] = ms32 / closure["ms_per_step"]
         print(f"learned closure: {closure['ms_per_step']:.3f} ms/step judge err {closure['judge_rel_err']:.4f} "
               f"in use {closure['network_in_use']} (train {closure['train_ms']:.0f} ms, loss "
-              f"{closure['loss_first']:.3g} -> {closure['loss_last']:.3g})", file=sys.stderr, flush=True)
+              f"{closure['loss_first']:.3g} -> {closure['loss_last']:.3g}); held-out net "
+              f"{closure['held_out']['net']:.4f} vs const {closure['held_out']['const_ref']:.4f} profile "
+              f"{closure['held_out']['profile']:.4f} previous {closure['held_out']['previous']:.4f}",
+              file=sys.stderr, flush=True)
 
     ensemble = None
     if not args.no_ensemble:
@@ -350,13 +442,22 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc, "kernel": dom,
                 "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_s * 1e3,
                 "profiled_ms_per_step": tot}
+    # step-level roofline: algorithmic field passes of the whole step (130 for
+    # the neutral model-5 step, 203 with theta, q and the scalar model) at the
+    # precision the mode stores the bulk fields
+    passes = 203 if conv else 130
+
+    def step_roof(ms, bpe):
+        B = passes * bpe * pts
+        a = B / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "passes": passes, "bytes_per_elem": bpe, "B_alg": B, "achieved": round(a, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4), "ms_per_step": ms}
+    step_roofline = {"fp32": step_roof(ms32, 4), "fp64": step_roof(ms64, 8) if ms64 else None}
     cpu = None
     if not args.no_cpu:
-        rate, tp, tl = cpu_baseline(nx, ny, nz, args.cpu_nz, conv=conv)
-        cpu = {"value": rate, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (C FP64 restatement, 1 thread) on {nx}x{ny}x{args.cpu_nz - 1}: one plain step "
-                         f"({tp:.2f} s) and one dynamic-update step ({tl:.2f} s), weighted 4:1 and scaled to "
-                         f"{nz - 1} levels"}
+        cpu = cpu_baseline(nx, ny, nz, args.cpu_nz, conv=conv)
+        print(f"cpu baseline: {cpu['value']:.3e} pts/s on {cpu['cores']} threads, {cpu['one_core']['value']:.3e} "
+              f"on 1", file=sys.stderr, flush=True)
     line = {
         "metric": "grid-point-steps/s at 256^2x128, FP32-bulk vs FP64 GPU; HBM roofline fraction",
         "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -376,6 +477,7 @@ def main():
         "fp32_over_fp64": (ms64 / ms32) if ms64 else None,
         "fp32_rel_error": err,
         "roofline": roof,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "learned_closure": closure,
         "ensemble": ensemble,

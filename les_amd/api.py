@@ -302,12 +302,16 @@ class Les:
                                               _ptr(cs), _ptr(beta)))
         return cs, beta
 
-    # ---- autotuner (launch shapes only; results do not depend on them) ----
+    # ---- autotuner (launch shapes and FFT plans) ----
     def autotune(self, what=7):
         """Time column-FFT tile widths and plans (what & 1), tridiagonal
         schedules (what & 2) and the padded-product variant (what & 4) on this
         context's state and keep the fastest; returns the measured ms per
-        candidate and the choice."""
+        candidate and the choice.  Tile widths, the Thomas-sweep storage and the
+        padded-product variant leave the results unchanged; the FFT plan
+        (radix-4 or radix-8 passes) and the parallel-cyclic-reduction schedule
+        of the tridiagonal solve change the order of operations, so results
+        after tuning agree with the untuned ones to rounding, not bit for bit."""
         import json
         buf = C.create_string_buffer(4096)
         self._chk(self.L.les_autotune(self.ctx, int(what), buf, len(buf)))

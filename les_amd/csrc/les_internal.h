@@ -73,7 +73,8 @@ struct Arena {
   std::vector<size_t> cap;
 };
 
-// Launch shapes chosen by les_autotune (only the FFT plan changes rounding).
+// Launch shapes chosen by les_autotune; the FFT plan and the PCR tridiagonal
+// schedule (tri_mode 3) change the rounding, the other choices do not.
 struct Tune {
   int col_tx[2] = {16, 8};  // column-FFT tile width (kx columns per block): FP32, FP64
   int col_plan[2] = {0, 0}; // column-FFT plan (les_fft.h): 0 radix-4 passes, 2 radix-8 passes

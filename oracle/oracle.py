@@ -500,3 +500,12 @@ def mlp_slices():
 
 def uniform(seed, index):
     return lib().or_uniform(seed, index)
+
+
+def threads():
+    """OpenMP threads of the oracle's transforms (its results do not depend on them)."""
+    return lib().or_threads()
+
+
+def set_threads(n):
+    lib().or_set_threads(int(n))
