@@ -125,7 +125,7 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     sim.mlp_init(1)
     nharv = 6
     gen = torch.Generator(device="cuda").manual_seed(0)
-    Xs, Ys = [], []
+    Xs, Ys, Ls = [], [], []
     sim.step(10)
     for _ in range(nharv):          # dynamic updates at jt = 15, 20, ..., 40
         sim.step(5)
@@ -133,6 +133,7 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
         idx = torch.randint(0, X.shape[0], (samples // nharv,), device="cuda", generator=gen)
         Xs.append(X[idx].contiguous())
         Ys.append(Y[idx].contiguous())
+        Ls.append(idx // (nx * ny))   # harvest rows are level-major
         del X, Y
     X, Y = torch.cat(Xs), torch.cat(Ys)
     del Xs, Ys
@@ -143,7 +144,9 @@ def run_closure(nx, ny, nz, steps, warmup, graph, conv, samples=2_000_000, train
     torch.cuda.synchronize()
     train_ms = (time.perf_counter() - t0) * 1e3
     loss0, loss1 = float(hist[:20].mean()), float(hist[-100:].mean())
-    prof_cs = Y[:, 0].reshape(-1, nz - 1, nx * ny).mean(dim=(0, 2))   # training mean Cs^2(z)
+    lev = torch.cat(Ls)              # training mean Cs^2(z)
+    prof_cs = (torch.bincount(lev, weights=Y[:, 0].double(), minlength=nz - 1) /
+               torch.bincount(lev, minlength=nz - 1).clamp(min=1)).float()
     best_cs = float(Y[:, 0].mean())
     del X, Y
     held = held_out_gain(sim, nx, ny, nz, conv, prof_cs, best_cs)

@@ -251,12 +251,12 @@ static void scalar_jobs(les_ctx* c, ScalarJob jobs[2]) {
 // its filt_da and q before its own, as in the reference.  The advection, SGS
 // flux divergence and step_scalar follow inside op_convec_scalar_ab2.
 template <typename T>
-static void scalars_prep(les_ctx* c, const ScalarJob jobs[2]) {
+static void scalars_prep(les_ctx* c, const ScalarJob jobs[2], SpecIn<T>* si) {
   const Grid& g = c->g;
   const ScalarParams& P = c->scal;
-  op_filt_da<T>(c, (T*)jobs[0].s, (T*)jobs[0].dsdx, (T*)jobs[0].dsdy, g.nz);
+  op_filt_da<T>(c, (T*)jobs[0].s, (T*)jobs[0].dsdx, (T*)jobs[0].dsdy, g.nz, S_FSPEC1, &si->s[0]);
   op_calcbeta<T>(c, (T*)jobs[0].s, (T*)jobs[1].s, F<T>(c, "beta_scal"), P.theta_ref, P.q_ref);
-  op_filt_da<T>(c, (T*)jobs[1].s, (T*)jobs[1].dsdx, (T*)jobs[1].dsdy, g.nz);
+  op_filt_da<T>(c, (T*)jobs[1].s, (T*)jobs[1].dsdx, (T*)jobs[1].dsdy, g.nz, S_FSPEC2, &si->s[1]);
   op_scalar_dz<T>(c, jobs, 2, (const double*)state_ptr(c, "phi_h"), (const double*)state_ptr(c, "ustar_avg"), 0);
 }
 
@@ -280,7 +280,11 @@ static void step_once(les_ctx* c) {
   }
   // without scalars obukhov's neutral branch leaves psi = 0, phi = 1: the
   // state's constant contents
-  op_filt_da3<T>(c, u, v, w, gr, g.nz);
+  // the x-spectral rows of the filtered u, v, w and their x, y derivatives
+  // stay in S_FSPEC0 for the convective inputs (nothing changes u, v, w
+  // before convec)
+  SpecIn<T> si;
+  op_filt_da3<T>(c, u, v, w, gr, g.nz, S_FSPEC0, &si.uvw);
   op_ddz3<T>(c, gr[2], gr[5], gr[8], u, v, w);
   op_wallstress<T>(c, u, v, gr[2], gr[5], tau[2], tau[4], (const double*)state_ptr(c, "psi_m"),
                    (const double*)state_ptr(c, "psi_m0"), (const double*)state_ptr(c, "phi_m"),
@@ -295,7 +299,7 @@ static void step_once(les_ctx* c) {
   ScalarJob jobs[2];
   if (scal) {  // main.f90:273-276
     scalar_jobs<T>(c, jobs);
-    scalars_prep<T>(c, jobs);
+    scalars_prep<T>(c, jobs, &si);
   }
   T *dx = F<T>(c, "divtx"), *dy = F<T>(c, "divty"), *dz = F<T>(c, "divtz");
   op_divstress<T>(c, dx, tau[0], tau[1], tau[2], 0);
@@ -319,11 +323,11 @@ static void step_once(les_ctx* c) {
     op_convec_scalar_ab2<T>(c, u, v, w, grc2, R3, Rf3, dv3, euler,
                             P.passive ? (const T*)nullptr : (const T*)F<T>(c, "beta_scal"), jobs, F<T>(c, "Nu_t"),
                             P.model == 7 ? (const T*)F<T>(c, "Pr_t") : (const T*)nullptr,
-                            (c->jt == 1 && !P.initsc) ? 1 : 0, fop);
+                            (c->jt == 1 && !P.initsc) ? 1 : 0, fop, &si);
   } else {
     // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
     op_convec_ab2<T>(c, u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], Rx, Ry, Rz, F<T>(c, "RHSx_f"),
-                     F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, euler, nullptr, fop);
+                     F<T>(c, "RHSy_f"), F<T>(c, "RHSz_f"), dx, dy, dz, euler, nullptr, fop, &si);
   }
   T *p = (T*)state_ptr(c, "p"), *px = F<T>(c, "dpdx"), *py = F<T>(c, "dpdy"), *pz = F<T>(c, "dpdz");
   op_press_stag<T>(c, u, v, w, dz, p, px, py);

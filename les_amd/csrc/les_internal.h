@@ -188,7 +188,7 @@ enum Slot : int {
   S_XS0 = 0, S_XS1, S_XS2, S_XS3, S_BIG0, S_BIG1, S_BIG2, S_SIJ, S_FF, S_LAGF, S_LAGX,
   S_PSPEC, S_PHAT, S_TRI, S_TMP0, S_TMP1, S_TMP2, S_RED, S_PLANE,
   S_SC0, S_SC1, S_SC2, S_SC3, S_LAGF2, S_RED2, S_SDQ, S_SDLEV, S_MLPP, S_TUNE0, S_TUNE1, S_TUNE2,
-  S_STATP, S_STATX, S_STAT, S_STATAVG, S_FORCE, S_FSUM, S_NSLOTS
+  S_STATP, S_STATX, S_STAT, S_STATAVG, S_FORCE, S_FSUM, S_FSPEC0, S_FSPEC1, S_FSPEC2, S_FWALL, S_NSLOTS
 };
 void* scratch(les_ctx* c, int slot, size_t bytes);
 void scratch_release(les_ctx* c, int slot);  // give a slot's memory back (stream drained first)
@@ -199,8 +199,19 @@ template <> const Tables<float>& tables<float>(const les_ctx* c);
 
 // ---- launchers (les_kernels.hip); all enqueue on c->stream ----
 // horizontal spectral operators, fields (ld,ny,nlev)
-template <typename T> void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev);
-template <typename T> void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev);
+// with keep_slot >= 0 the x-spectral rows of (f, df/dx, df/dy) stay in that
+// scratch slot (kind-major, field, level, y; scale 1/nx) and *spec points there
+template <typename T>
+void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev, int keep_slot = -1, const cx<T>** spec = nullptr);
+template <typename T>
+void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev, int keep_slot = -1,
+                 const cx<T>** spec = nullptr);
+// filt_da's kept spectra of u, v, w and of theta, q for the convective inputs
+template <typename T>
+struct SpecIn {
+  const cx<T>* uvw = nullptr;
+  const cx<T>* s[2] = {nullptr, nullptr};
+};
 template <typename T> void op_ddxy(les_ctx* c, T* dfdx, T* dfdy, const T* f, int nlev);  // null outputs skipped
 template <typename T> void op_test_filter(les_ctx* c, T* planes, int nplanes, const T* G, int keep);
 template <typename T> void op_test_filter_level0(les_ctx* c, const T* a, const T* b, T* dst, const T* G, int keep);
@@ -219,7 +230,7 @@ template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
                    const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta = nullptr,
-                   const ForcingArgs<T>* fo = nullptr);
+                   const ForcingArgs<T>* fo = nullptr, const SpecIn<T>* si = nullptr);
 // the per-step inputs of the forcing terms (les_scalars.hip): Coriolis
 // products and plane sums of the fields before the AB2 update; theta/q may be null
 template <typename T>
@@ -325,7 +336,8 @@ void op_pad_products_probe(les_ctx* c, int scal);
 template <typename T>
 void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
                           const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
-                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo = nullptr);
+                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo = nullptr,
+                          const SpecIn<T>* si = nullptr);
 // lagrange_scalar_Sdep.f90:1-467 (theta Nyquist-filtered in place); with
 // reuse_mom the momentum model's filtered fields of this step are reused
 template <typename T>

@@ -35,13 +35,13 @@ const Tables<float>& tables<float>(const les_ctx* c) { return c->tf; }
 // nf fields (up to 3) -> for each field the outputs kinds[0..nout): one
 // row_fwd, one column pass and one row_inv over all fields.
 template <typename T>
-static void spectral_deriv(les_ctx* c, const T* const* fs, int nf, int nlev, T* const* outs, const int kinds[3],
-                           int nout) {
+static const cx<T>* spectral_deriv(les_ctx* c, const T* const* fs, int nf, int nlev, T* const* outs,
+                                   const int kinds[3], int nout, int keep_slot = -1) {
   const Grid& g = c->g;
   const long rows = (long)nlev * g.ny;
   const size_t fo = (size_t)nf * rows * g.nxh;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, fo * sizeof(cx<T>));
-  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, fo * nout * sizeof(cx<T>));
+  cx<T>* X1 = (cx<T>*)scratch(c, keep_slot >= 0 ? keep_slot : S_XS1, fo * nout * sizeof(cx<T>));
   LoadFields<T> L{{fs[0], nf > 1 ? fs[1] : nullptr, nf > 2 ? fs[2] : nullptr, nullptr}, nlev, g.ny, g.ld,
                   T(1.0 / ((double)g.nx * g.ny))};
   launch_row_fwd<T>(c, L, X0, g.nxh, rows * nf, g.nx, g.nxh, g.ny);
@@ -56,25 +56,28 @@ static void spectral_deriv(les_ctx* c, const T* const* fs, int nf, int nlev, T* 
   S.ld = g.ld;
   S.nx = g.nx;
   launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nf * nout, g.nx, g.ld, g.ny);
+  return X1;
 }
 
 template <typename T>
-void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev) {
+void op_filt_da(les_ctx* c, T* f, T* dfdx, T* dfdy, int nlev, int keep_slot, const cx<T>** spec) {
   OpTag _tag(c, "filt_da");
   const T* fs[1] = {f};
   T* outs[3] = {f, dfdx, dfdy};
   const int kinds[3] = {K_ID, K_IKX, K_IKY};
-  spectral_deriv<T>(c, fs, 1, nlev, outs, kinds, 3);
+  const cx<T>* X1 = spectral_deriv<T>(c, fs, 1, nlev, outs, kinds, 3, keep_slot);
+  if (spec) *spec = keep_slot >= 0 ? X1 : nullptr;
 }
 
 // filt_da on u, v, w in one pipeline (main.f90:231-233)
 template <typename T>
-void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev) {
+void op_filt_da3(les_ctx* c, T* u, T* v, T* w, T* const gr[9], int nlev, int keep_slot, const cx<T>** spec) {
   OpTag _tag(c, "filt_da");
   const T* fs[3] = {u, v, w};
   T* outs[9] = {u, v, w, gr[0], gr[3], gr[6], gr[1], gr[4], gr[7]};
   const int kinds[3] = {K_ID, K_IKX, K_IKY};
-  spectral_deriv<T>(c, fs, 3, nlev, outs, kinds, 3);
+  const cx<T>* X1 = spectral_deriv<T>(c, fs, 3, nlev, outs, kinds, 3, keep_slot);
+  if (spec) *spec = keep_slot >= 0 ? X1 : nullptr;
 }
 
 template <typename T>
@@ -376,19 +379,32 @@ struct StoreConvec {  // 3 outputs, levels 0..nz-2 ; level nz-1 BOGUS handled ap
 // levels 1..nz-1); returns the scratch holding them.
 template <typename T>
 static cx<T>* convec_spectral(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,
-                              const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy) {
+                              const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy,
+                              const SpecIn<T>* si = nullptr) {
   const Grid& g = c->g;
   const int nz = g.nz;
-  const long rows6 = 6L * nz * g.ny;
-  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
-  LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh, g.ny);
   // pad in y: 6*nz planes
   const long rowsB = 6L * nz * g.ny2;
   cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
-  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
-  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
+  bool padded = false;
+  if (si && si->uvw) {
+    // the inputs from filt_da's kept spectra; the wall planes of dudz, dvdz
+    // (wallstress) by one small forward transform
+    cx<T>* W = (cx<T>*)scratch(c, S_FWALL, (size_t)6 * g.ny * g.nxh * sizeof(cx<T>));
+    LoadPlanes6<T> LW{{dudz, dvdz, dudz, dudz, dudz, dudz}, g.ld, T(1.0 / g.nx)};
+    launch_row_fwd<T>(c, LW, W, g.nxh, 2L * g.ny, g.nx, g.nxh, g.ny);
+    ConvecSpecIn<T, 0> fn{si->uvw, {nullptr, nullptr}, W, nz, g.ny, g.nxh, T(1.0 / g.ny), T(1.0 / g.dz)};
+    padded = launch_col_pad_fn<T>(c, fn, XB, 6L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.5);
+  }
+  if (!padded) {
+    const long rows6 = 6L * nz * g.ny;
+    cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
+    LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
+    launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh, g.ny);
+    ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+    col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+    launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
+  }
   // padded-grid products without materialising the padded fields
   const long rowsP = 3L * (nz - 1) * g.ny2;
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)rowsP * g.nxh * sizeof(cx<T>));
@@ -422,11 +438,11 @@ template <typename T>
 void op_convec_ab2(les_ctx* c, T* u, T* v, T* w, const T* dudy, const T* dudz, const T* dvdx, const T* dvdz,
                    const T* dwdx, const T* dwdy, T* RHSx, T* RHSy, T* RHSz, T* RHSx_f, T* RHSy_f, T* RHSz_f,
                    const T* divtx, const T* divty, const T* divtz, int euler_first, const T* beta,
-                   const ForcingArgs<T>* fo) {
+                   const ForcingArgs<T>* fo, const SpecIn<T>* si) {
   OpTag _tag(c, "convec");
   const Grid& g = c->g;
   const int nz = g.nz;
-  cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy);
+  cx<T>* X3 = convec_spectral<T>(c, u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, si);
   StoreConvecAB2<T> SC{{RHSx, RHSy, RHSz}, {RHSx_f, RHSy_f, RHSz_f}, {divtx, divty, divtz}, {u, v, w},
                        nz - 1, g.ny, g.ld, g.nx, (T)g.dt, (T)g.force, euler_first, beta};
   if (fo) SC.fo = *fo;
@@ -602,8 +618,9 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
   const long pl = (long)ld * ny, n3 = pl * nz;
   const long n = (long)nx * ny * nz;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(q / ((long)nx * ny));
-    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const unsigned pn = (unsigned)(nx * ny), qq = (unsigned)q;  // nx*ny*nz < 2^31
+    const int k = (int)(qq / pn);
+    const int r = (int)(qq - (unsigned)k * pn), y = r / nx, x = r - y * nx;
     const long i = (long)k * pl + (long)y * ld + x;
     // u_lag, v_lag to w-nodes (interpolag_Sdep.f90:228-241)
     T uu = ul[i] / cs, vv = vl[i] / cs;
@@ -1798,8 +1815,9 @@ __global__ void k_rmsdiv(const T* du, const T* dv, const T* dw, int nx, int ld, 
   const long n = (long)nx * ny * nl;
   double acc = 0.0;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(q / ((long)nx * ny));
-    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const unsigned pn = (unsigned)(nx * ny), qq = (unsigned)q;  // nx*ny*nz < 2^31
+    const int k = (int)(qq / pn);
+    const int r = (int)(qq - (unsigned)k * pn), y = r / nx, x = r - y * nx;
     const long i = ((long)k * ny + y) * ld + x;
     acc += fabs((double)du[i] + (double)dv[i] + (double)dw[i]);
   }
@@ -1834,8 +1852,9 @@ __global__ void k_tsc(const T* u, const T* v, const T* w, const T* nu, int nx, i
   const long n = (long)nx * ny * nl;
   double m0 = -1.0, m1 = 0.0, m2 = 0.0, m3 = -1e300;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(q / ((long)nx * ny));
-    const int r = (int)(q - (long)k * nx * ny), y = r / nx, x = r - y * nx;
+    const unsigned pn = (unsigned)(nx * ny), qq = (unsigned)q;  // nx*ny*nz < 2^31
+    const int k = (int)(qq / pn);
+    const int r = (int)(qq - (unsigned)k * pn), y = r / nx, x = r - y * nx;
     const long i = ((long)k * ny + y) * ld + x;
     const double a = u[i], b = v[i], d = w[i];
     m0 = fmax(m0, a * a + b * b + d * d);
@@ -1877,8 +1896,8 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
 // explicit instantiations
 // ------------------------------------------------------------------------
 #define LES_INST(T)                                                                                              \
-  template void op_filt_da<T>(les_ctx*, T*, T*, T*, int);                                                        \
-  template void op_filt_da3<T>(les_ctx*, T*, T*, T*, T* const[9], int);                                          \
+  template void op_filt_da<T>(les_ctx*, T*, T*, T*, int, int, const cx<T>**);                                   \
+  template void op_filt_da3<T>(les_ctx*, T*, T*, T*, T* const[9], int, int, const cx<T>**);                     \
   template void op_ddxy<T>(les_ctx*, T*, T*, const T*, int);                                                     \
   template void op_test_filter<T>(les_ctx*, T*, int, const T*, int);                                             \
   template void op_test_filter_level0<T>(les_ctx*, const T*, const T*, T*, const T*, int);                       \
@@ -1894,7 +1913,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
                              const T*, const T*, T*, T*, T*);                                                    \
   template void op_convec_ab2<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,         \
                                  const T*, T*, T*, T*, T*, T*, T*, const T*, const T*, const T*, int, const T*,  \
-                                 const ForcingArgs<T>*);                                                         \
+                                 const ForcingArgs<T>*, const SpecIn<T>*);                                       \
   template void op_wallstress<T>(les_ctx*, const T*, const T*, T*, T*, T*, T*, const double*, const double*,     \
                                  const double*, double*);                                                        \
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \

@@ -549,26 +549,41 @@ struct LoadConvecScalar {
 template <typename T>
 void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T* const R[3], T* const Rf[3],
                           const T* const divt[3], int euler_first, const T* beta, const ScalarJob* jobs,
-                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo) {
+                          const T* Nu_t, const T* Pr_t, int scal_euler, const ForcingArgs<T>* fo,
+                          const SpecIn<T>* si) {
   OpTag _tag(c, "convec_scalar");
   const Grid& g = c->g;
   const int nz = g.nz, nl = nz - 1;
-  const long rows = 12L * nz * g.ny;
-  cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
-  LoadConvecScalar<T> L{};
-  L.lc = LoadConvec<T>{u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], g.ny, nz, g.ld,
-                       T(1.0 / ((double)g.nx * g.ny))};
-  for (int q = 0; q < 2; ++q) {
-    L.gs[3 * q] = (const T*)jobs[q].dsdx;
-    L.gs[3 * q + 1] = (const T*)jobs[q].dsdy;
-    L.gs[3 * q + 2] = (const T*)jobs[q].dsdz;
-  }
-  launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
   const long rowsB = 12L * nz * g.ny2;
   cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
-  ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
-  launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 12L * nz, g.nxh);
+  bool padded = false;
+  if (si && si->uvw && si->s[0] && si->s[1]) {
+    // the inputs from filt_da's kept spectra; wall dudz, dvdz (wallstress) and
+    // the scalars' surface and lid gradients by one small forward transform
+    const size_t pl = (size_t)g.ld * g.ny, top = pl * (nz - 1);
+    cx<T>* W = (cx<T>*)scratch(c, S_FWALL, (size_t)6 * g.ny * g.nxh * sizeof(cx<T>));
+    const T *d0 = (const T*)jobs[0].dsdz, *d1 = (const T*)jobs[1].dsdz;
+    LoadPlanes6<T> LW{{gr[2], gr[5], d0, d0 + top, d1, d1 + top}, g.ld, T(1.0 / g.nx)};
+    launch_row_fwd<T>(c, LW, W, g.nxh, 6L * g.ny, g.nx, g.nxh, g.ny);
+    ConvecSpecIn<T, 2> fn{si->uvw, {si->s[0], si->s[1]}, W, nz, g.ny, g.nxh, T(1.0 / g.ny), T(1.0 / g.dz)};
+    padded = launch_col_pad_fn<T>(c, fn, XB, 12L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.6);
+  }
+  if (!padded) {
+    const long rows = 12L * nz * g.ny;
+    cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows * g.nxh * sizeof(cx<T>));
+    LoadConvecScalar<T> L{};
+    L.lc = LoadConvec<T>{u, v, w, gr[1], gr[2], gr[3], gr[5], gr[6], gr[7], g.ny, nz, g.ld,
+                         T(1.0 / ((double)g.nx * g.ny))};
+    for (int q = 0; q < 2; ++q) {
+      L.gs[3 * q] = (const T*)jobs[q].dsdx;
+      L.gs[3 * q + 1] = (const T*)jobs[q].dsdy;
+      L.gs[3 * q + 2] = (const T*)jobs[q].dsdz;
+    }
+    launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
+    ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
+    col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+    launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 12L * nz, g.nxh);
+  }
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)5 * nl * g.ny2 * g.nxh * sizeof(cx<T>));
   launch_pad_products<T, PolConvecScalar<T>, 4, 1>(c, XB, XP, PolConvecScalar<T>{T(1.0 / ((double)g.nx2 * g.ny2))},
                                                    "convec_scalar_big");
@@ -924,7 +939,7 @@ void op_pad_products_probe(les_ctx* c, int scal) {
   template void op_step_scalar<T>(les_ctx*, int, T*, const T*, const T*);                                        \
   template void op_convec_scalar_ab2<T>(les_ctx*, T*, T*, T*, const T* const[9], T* const[3], T* const[3],        \
                                         const T* const[3], int, const T*, const ScalarJob*, const T*, const T*,   \
-                                        int, const ForcingArgs<T>*);                                              \
+                                        int, const ForcingArgs<T>*, const SpecIn<T>*);                            \
   template ForcingArgs<T> op_forcing_pre<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*);          \
   template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
                                       const SgsPtrs&, const ScalarSgsPtrs&, bool);                               \

@@ -986,6 +986,196 @@ static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0,
   launch_col_n<T, TXD, 0, 0>(c, in, in2, o0, o1, o2, a, nplanes, ncols, G0, G1, G2);
 }
 
+// ------------------------------------------------------------------------
+// Zero-pad column pass (padd.f90 on the y direction) whose input planes a
+// functor forms: fn.comb(plane) gives the weighted x-spectral planes (rows of
+// nki coefficients) whose sum is the input.  Same
+// tile, transforms and output layout as the in-place zero-pad pass of k_col
+// (the FP32 radix-8 plan, FP64 the tuned one), so the convective inputs can
+// be formed from filt_da's x-spectral rows instead of forward-transforming
+// physical fields again.
+// ------------------------------------------------------------------------
+template <typename T, int TX, int NYI, int NYO, int PLAN, class Fn>
+__global__ void __launch_bounds__((ColGeomT<T, TX, NYI, NYO>::BLOCK))
+    k_col_pad_fn(Fn fn, cx<T>* __restrict__ out, int nki, long out_row, long out_plane,
+                 const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout) {
+  static_assert(NYO > NYI && NYO < 2 * NYI, "3/2 zero-pad pass");
+  constexpr int BLK = ColGeomT<T, TX, NYI, NYO>::BLOCK;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* twi = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* two = twi + NYI;
+  cx<T>* R = two + NYO;  // TX * NYO
+  for (int i = threadIdx.x; i < NYI; i += BLK) twi[i] = i < NYI - 1 ? twin[NYI + PLAN * NYI + i] : cx<T>{};
+  for (int i = threadIdx.x; i < NYO; i += BLK) two[i] = i < NYO - 1 ? twout[NYO + PLAN * NYO + i] : cx<T>{};
+  unsigned bx = blockIdx.x, by = blockIdx.y;
+  if constexpr (TX * sizeof(cx<T>) == 64) xcd_pair_tiles(bx, by);
+  const int kx0 = bx * TX;
+  const long pl = by;
+  {
+    constexpr int NI = NYI * TX, MAXI = (NI + BLK - 1) / BLK;
+    const auto lc = fn.comb(pl);
+    // every load of the tile issued before the first combination
+    cx<T> raw[MAXI][4];
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      const int e = threadIdx.x + i * BLK;
+      const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+      const bool ok = (NI % BLK == 0 || e < NI) && kk < nki;
+      const long off = (long)y * nki + kk;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) raw[i][q] = (ok && q < lc.n) ? lc.p[q][off] : cx<T>{T(0), T(0)};
+    }
+    cx<T> rg[MAXI];
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      rg[i] = cx<T>{T(0), T(0)};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        rg[i].x += lc.c[q] * raw[i][q].x;
+        rg[i].y += lc.c[q] * raw[i][q].y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      const int e = threadIdx.x + i * BLK;
+      if (NI % BLK == 0 || e < NI) R[e] = rg[i];
+    }
+  }
+  __syncthreads();
+  fft_lds_c<T, NYI, TX, 1, TX, BLK, true, false, PLAN>(R, twi, -1);
+  // rows NYI/2+1 .. NYI-1 move up by NYO-NYI (never onto a source row), then
+  // rows NYI/2 .. NYO-NYI/2 are zero
+  constexpr int NMV = (NYI / 2 - 1) * TX, NZR = (NYO - NYI + 1) * TX;
+  for (int e = threadIdx.x; e < NMV; e += BLK) {
+    const int yi = NYI / 2 + 1 + e / TX, c = e % TX;
+    R[(yi + NYO - NYI) * TX + c] = R[yi * TX + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NZR; e += BLK) R[(NYI / 2) * TX + e] = cx<T>{T(0), T(0)};
+  __syncthreads();
+  fft_lds_c<T, NYO, TX, 1, TX, BLK, true, false, PLAN>(R, two, +1);
+  for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
+    const int y = e / TX, c = e - y * TX, kk = kx0 + c;
+    if (kk < nki) out[pl * out_plane + (long)y * out_row + kk] = R[e];
+  }
+}
+
+template <typename T, int TX, int NYI, int NYO, int PLAN, class Fn>
+static bool launch_col_pad_fn_n(les_ctx* c, const Fn& fn, cx<T>* out, long nplanes, int nki, long out_row,
+                                long out_plane, double reads_per_elem) {
+  using G = ColGeomT<T, TX, NYI, NYO>;
+  const size_t sm = (size_t)(NYI + NYO + TX * NYO) * sizeof(cx<T>);
+  if (sm > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_col_pad_fn<T, TX, NYI, NYO, PLAN, Fn>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const unsigned gx = (unsigned)((nki + TX - 1) / TX);
+  const double es = 2.0 * sizeof(T);
+  ProfScope ps(c, "col_pad", (double)nplanes * nki * es * (NYI * reads_per_elem + NYO));
+  hipLaunchKernelGGL((k_col_pad_fn<T, TX, NYI, NYO, PLAN, Fn>), dim3(gx, (unsigned)nplanes), dim3(G::BLOCK), sm,
+                     c->stream, fn, out, nki, out_row, out_plane, tw_for<T>(c, NYI), tw_for<T>(c, NYO));
+  LCHK(c, hipGetLastError());
+  return true;
+}
+
+// false where no compile-time variant exists for (ny, 3 ny / 2)
+template <typename T, class Fn>
+static bool launch_col_pad_fn(les_ctx* c, const Fn& fn, cx<T>* out, long nplanes, int nki, long out_row,
+                              long out_plane, double reads_per_elem) {
+  constexpr int TX = sizeof(T) == 8 ? 8 : 16;
+  const int plan = sizeof(T) == 4 ? 2 : c->tune.col_plan[1];
+#define LES_CASE(I, O)                                                                                        \
+  if (c->g.ny == I && c->g.ny2 == O) {                                                                         \
+    if (plan == 2)                                                                                             \
+      return launch_col_pad_fn_n<T, TX, I, O, 2>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem); \
+    return launch_col_pad_fn_n<T, TX, I, O, 0>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem);   \
+  }
+  LES_CASE(16, 24) LES_CASE(32, 48) LES_CASE(64, 96) LES_CASE(96, 144) LES_CASE(128, 192) LES_CASE(256, 384)
+#undef LES_CASE
+  return false;
+}
+
+// The convective inputs of convec.f90:45-114 (u, v, w, the three vorticity
+// components; with NS = 2 also dtheta/dx, dy, dz and dq/dx, dy, dz), as the
+// x-spectral rows the zero-pad pass reads (scale 1/(nx ny) of the physical
+// fields), formed from filt_da's x-spectral rows of the filtered fields
+// (kinds f, d/dx, d/dy; scale 1/nx) instead of a forward transform of the
+// physical fields.  The vertical differences of ddz_uv.f90 and scalar_dz are
+// linear, so they are taken on the x-spectral rows; the planes those
+// differences do not give (the wall values of dudz, dvdz from wallstress, the
+// surface and lid gradients of the scalars) come from one small forward
+// transform (W).  Plane order: field-major, (f * nz + k).
+// up to four x-spectral planes (rows of nxh coefficients, ny rows each) and
+// real weights: one input plane of the zero-pad pass as a linear combination
+template <typename T>
+struct LinComb4 {
+  const cx<T>* p[4];
+  T c[4];
+  int n;
+  __device__ cx<T> at(long off) const {
+    cx<T> v{T(0), T(0)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < n) {
+        const cx<T> a = p[q][off];
+        v.x += c[q] * a.x;
+        v.y += c[q] * a.y;
+      }
+    return v;
+  }
+};
+
+template <typename T, int NS>
+struct ConvecSpecIn {
+  const cx<T>* U;      // filt_da of u, v, w: rows (((j * 3 + f) * nz + k) * ny + y), j = f, d/dx, d/dy
+  const cx<T>* S[2];   // filt_da of theta, q: rows ((j * nz + k) * ny + y)
+  const cx<T>* W;      // rows (p * ny + y): dudz(0), dvdz(0), then per scalar its dz at levels 0 and nz-1
+  int nz, ny, nxh;
+  T s, rdz;            // 1 / ny, 1 / dz
+  // the weights of input plane `plane` (uniform over a workgroup)
+  __device__ LinComb4<T> comb(long plane) const {
+    const int f = (int)(plane / nz), k = (int)(plane - (long)f * nz);
+    const long pl = (long)ny * nxh;
+    auto A = [&](int j, int fi, int kk) { return U + (((long)j * 3 + fi) * nz + kk) * pl; };
+    auto Wp = [&](int p) { return W + (long)p * pl; };
+    const T h = T(0.5) * s, sd = s * rdz;
+    LinComb4<T> L{{nullptr, nullptr, nullptr, nullptr}, {T(0), T(0), T(0), T(0)}, 0};
+    auto add = [&](const cx<T>* p, T c) { L.p[L.n] = p; L.c[L.n] = c; ++L.n; };
+    if (f < 3) {
+      add(A(0, f, k), s);
+    } else if (f == 3) {  // dw/dy - dv/dz (dw/dy averaged onto the first uv-node; dv/dz = 0 at the lid)
+      if (k == 0) { add(A(2, 2, 0), h); add(A(2, 2, 1), h); add(Wp(1), -s); }
+      else if (k == nz - 1) add(A(2, 2, k), s);
+      else { add(A(2, 2, k), s); add(A(0, 1, k), -sd); add(A(0, 1, k - 1), sd); }
+    } else if (f == 4) {  // du/dz - dw/dx
+      if (k == 0) { add(Wp(0), s); add(A(1, 2, 0), -h); add(A(1, 2, 1), -h); }
+      else if (k == nz - 1) add(A(1, 2, k), -s);
+      else { add(A(0, 0, k), sd); add(A(0, 0, k - 1), -sd); add(A(1, 2, k), -s); }
+    } else if (f == 5) {  // dv/dx - du/dy
+      add(A(1, 1, k), s);
+      add(A(2, 0, k), -s);
+    } else if constexpr (NS > 0) {
+      const int q = (f - 6) / 3, d = (f - 6) - 3 * q;
+      auto Sj = [&](int j, int kk) { return S[q] + ((long)j * nz + kk) * pl; };
+      if (d < 2) add(Sj(d + 1, k), s);
+      else if (k == 0) add(Wp(2 + 2 * q), s);
+      else if (k == nz - 1) add(Wp(3 + 2 * q), s);
+      else { add(Sj(0, k), sd); add(Sj(0, k - 1), -sd); }
+    }
+    return L;
+  }
+};
+
+// the physical planes ConvecSpecIn takes from W, forward-transformed with the
+// filt_da scale 1/nx
+template <typename T>
+struct LoadPlanes6 {
+  const T* p[6];
+  int ld;
+  T scale;
+  static constexpr double reads() { return 1.0; }
+  __device__ T operator()(int plane, int y, int x) const { return scale * p[plane][(unsigned)(y * ld + x)]; }
+};
+
 static ColArgs col_args(int ny_in, int ny_out, int nki, long in_row, long in_plane) {
   ColArgs a;
   std::memset(&a, 0, sizeof(a));
