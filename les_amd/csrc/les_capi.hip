@@ -302,9 +302,8 @@ static void step_once(les_ctx* c) {
     scalars_prep<T>(c, jobs, &si);
   }
   T *dx = F<T>(c, "divtx"), *dy = F<T>(c, "divty"), *dz = F<T>(c, "divtz");
-  op_divstress<T>(c, dx, tau[0], tau[1], tau[2], 0);
-  op_divstress<T>(c, dy, tau[1], tau[3], tau[4], 0);
-  op_divstress<T>(c, dz, tau[2], tau[4], tau[5], 1);
+  T* const dv3[3] = {dx, dy, dz};
+  op_divstress3<T>(c, dv3, tau);
   T *Rx = F<T>(c, "RHSx"), *Ry = F<T>(c, "RHSy"), *Rz = F<T>(c, "RHSz");
   const int euler = (c->jt == 1 && !g.initu) ? 1 : 0;
   // Coriolis and sponge (main.f90:319-348, scalars_module.f90:143-148): the

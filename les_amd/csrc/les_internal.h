@@ -223,6 +223,8 @@ template <typename T> void op_ddz_uv(les_ctx* c, T* dfdz, const T* f);
 template <typename T> void op_ddz_w(les_ctx* c, T* dfdz, const T* f);
 template <typename T> void op_ddz3(les_ctx* c, T* dudz, T* dvdz, T* dwdz, const T* u, const T* v, const T* w);
 template <typename T> void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, int wnode);
+// divtx, divty, divtz of the step in one pipeline; tau: txx, txy, txz, tyy, tyz, tzz
+template <typename T> void op_divstress3(les_ctx* c, T* const divt[3], T* const tau[6]);
 template <typename T>
 void op_convec(les_ctx* c, const T* u, const T* v, const T* w, const T* dudy, const T* dudz,
                const T* dvdx, const T* dvdz, const T* dwdx, const T* dwdy, T* cx, T* cy, T* cz);

@@ -338,6 +338,54 @@ struct StoreDiv {
   __device__ void operator()(int plane, int y, int x, T v) const { fin(plane, y, x, v, pre(plane, y, x)); }
 };
 
+// the three divergences of the step in one pipeline (main.f90:294-297):
+// plane f*nz + k of output f, with f's vertical stress and node type
+template <typename T>
+struct StoreDiv3 {
+  StoreDiv<T> d[3];
+  int nz;
+  static constexpr double writes() { return 2.0; }
+  using Pre = typename StoreDiv<T>::Pre;
+  __device__ Pre pre(int plane, int y, int x) const {
+    const int f = plane / nz;
+    return d[f].pre(plane - f * nz, y, x);
+  }
+  __device__ void fin(int plane, int y, int x, T v, const Pre& p) const {
+    const int f = plane / nz;
+    d[f].fin(plane - f * nz, y, x, v, p);
+  }
+  __device__ void operator()(int plane, int y, int x, T v) const { fin(plane, y, x, v, pre(plane, y, x)); }
+};
+
+// divstress_uv (divtx, divty) and divstress_w (divtz) of the step: one
+// forward row pass over the six horizontal-derivative inputs, one column
+// pass and one inverse row pass for the three outputs
+template <typename T>
+void op_divstress3(les_ctx* c, T* const divt[3], T* const tau[6]) {
+  OpTag _tag(c, "divstress");
+  const Grid& g = c->g;
+  const int nlev = g.nz;
+  const long rows = (long)nlev * g.ny;
+  const size_t xs = (size_t)rows * g.nxh * sizeof(cx<T>);
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, 6 * xs);
+  cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, 3 * xs);
+  // tau: txx, txy, txz, tyy, tyz, tzz; d/dx of (txx, txy, txz), d/dy of (txy, tyy, tyz)
+  LoadFieldsN<T, 9> L{};
+  const T* in[6] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4]};
+  for (int q = 0; q < 6; ++q) L.p[q] = in[q];
+  L.ppf = nlev; L.ny = g.ny; L.ld = g.ld; L.scale = T(1.0 / ((double)g.nx * g.ny));
+  launch_row_fwd<T>(c, L, X0, g.nxh, 6 * rows, g.nx, g.nxh, g.ny);
+  ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  a.in2_plane = a.in_plane;
+  col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  launch_col<T>(c, X0, X0 + (size_t)3 * rows * g.nxh, X1, nullptr, nullptr, a, 3L * nlev, g.nxh);
+  StoreDiv3<T> S{{StoreDiv<T>{divt[0], tau[2], g.ny, g.ld, g.nx, g.nz, 0, T(1.0 / g.dz)},
+                  StoreDiv<T>{divt[1], tau[4], g.ny, g.ld, g.nx, g.nz, 0, T(1.0 / g.dz)},
+                  StoreDiv<T>{divt[2], tau[5], g.ny, g.ld, g.nx, g.nz, 1, T(1.0 / g.dz)}},
+                 nlev};
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, 3 * rows, g.nx, g.ld, g.ny);
+}
+
 template <typename T>
 void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, int wnode) {
   OpTag _tag(c, "divstress");
@@ -1909,6 +1957,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_ddz_w<T>(les_ctx*, T*, const T*);                                                             \
   template void op_ddz3<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*);                                  \
   template void op_divstress<T>(les_ctx*, T*, const T*, const T*, const T*, int);                                \
+  template void op_divstress3<T>(les_ctx*, T* const[3], T* const[6]);                                            \
   template void op_convec<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*, const T*, const T*,     \
                              const T*, const T*, T*, T*, T*);                                                    \
   template void op_convec_ab2<T>(les_ctx*, T*, T*, T*, const T*, const T*, const T*, const T*, const T*,         \
