@@ -8,7 +8,10 @@
 ! fields, rmsdiv of a divergence-free pair, timestep_conditions, the
 ! spectrum of one mode, the plane mean of avg_stats and the cosine sponge --
 ! then whole steps on the device state through les_step with every SGS model,
-! the forcing and the convective step.  Prints "LES_FORTRAN_OK" on success.
+! the forcing and the convective step.  With a directory as its argument it
+! writes each whole-step state there as a checkpoint record (model_<m>.out,
+! convective.out), which tests/test_fortran_gpu.py compares with the oracle.
+! Prints "LES_FORTRAN_OK" on success.
 program les_fortran_check
   use iso_c_binding
   use les_amd
@@ -20,7 +23,10 @@ program les_fortran_check
   type(les_grid) :: g
   real(c_double) :: dx, dz, worst
   integer :: nchecks
+  character(len=1024) :: outdir
 
+  outdir = ''
+  if (command_argument_count() >= 1) call get_command_argument(1, outdir)
   g = les_grid(nx, ny, nz, 2 * pi, 2 * pi, 1.0_c_double, dt, z0, 1.0_c_double, 1, 1, 5)
   dx = 2 * pi / nx
   dz = 1.0_c_double / (nz - 1)
@@ -402,6 +408,7 @@ contains
     real(c_double), target :: tref, qref
     real(c_double) :: mean
     integer :: model
+    character(len=1100) :: name
     allocate (uh(ld, ny, nz))
     do model = 1, 5
       call les_init_global(g)
@@ -419,6 +426,10 @@ contains
       mean = sum(uh(1:nx, 1:ny, 1:nz - 1)) / (nx * ny * (nz - 1))
       call expect(merge(0.0_c_double, 1.0_c_double, mean > 1 .and. mean < 100), 0.0_c_double, &
            'plausible mean u after 6 steps')
+      if (len_trim(outdir) > 0) then
+        write (name, '(a,a,i0,a)') trim(outdir), '/model_', model, '.out'
+        call les_check(les_state_checkpoint(les_ctx_global, trim(name)//c_null_char), 'checkpoint')
+      end if
       call les_destroy(les_ctx_global)
     end do
     ! convective step: theta and q with a uniform surface heat flux
@@ -439,6 +450,10 @@ contains
     mean = (sum(uh(1:nx, 1:ny, 1)) / (nx * ny) + tref) * 400.0_c_double
     call expect(merge(0.0_c_double, 1.0_c_double, mean > 282 .and. mean < 284), 0.0_c_double, &
          'plausible surface theta after 5 convective steps')
+    if (len_trim(outdir) > 0) then
+      name = trim(outdir)//'/convective.out'
+      call les_check(les_state_checkpoint(les_ctx_global, trim(name)//c_null_char), 'checkpoint convective')
+    end if
     call les_destroy(les_ctx_global)
   end subroutine check_whole_steps
 
