@@ -110,16 +110,25 @@ int les_interpolag_sdep(les_ctx *c, const les_sgs_fields *s);
    3 = scale-dependent dynamic with plane-averaged coefficients
        (scaledep_dynamic.f90);
    4 = Lagrangian scale-invariant (lagrange_Ssim.f90, interpolag_Ssim.f90);
-   5 = Lagrangian scale-dependent (default).
+   5 = Lagrangian scale-dependent (default);
+   6 = scale-dependent dynamic with plane-averaged coefficients and no
+       quintic (new_scaledep_dynamic.f90), with scalars also Pr_t of
+       scalar_scaledep_dynamic.f90 (:143-149).
    u_lag accumulates under models 4 and 5 only; the jt_total < 500 constant-Cs
-   period applies to model 5 only (:165-174).  The scalar Lagrangian model
-   (scalar model 7) needs SGS model 5; with the others the scalars use the
-   constant Pr (scalars_module.f90:77-84).  Returns 1 for another model. ---- */
+   period applies to models 5 and 6 (:165-174).  The scalar Lagrangian model
+   (scalar model 7) needs SGS model 5, and scalar model 6 goes with SGS model
+   6 and only with it; with models 1-4 the scalars use the constant Pr
+   (scalars_module.f90:77-84).  Returns 1 for another model. ---- */
 int les_set_sgs_model(les_ctx *c, int model);
 /* ---- std_dynamic.f90:4-140: Cs_opt2 (nz device doubles) of the standard
    dynamic model from S11..S33 and u, v, w (ld,ny,nz), Delta = (dx dy dz)^(1/3) ---- */
 int les_std_dynamic(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
                     double *Cs_opt2);
+/* ---- new_scaledep_dynamic.f90:1-223 (model 6): Cs_opt2 (nz device doubles)
+   from per-level plane sums of L:M, M:M, Q:N, N:N with the single-precision
+   clips of :208-220; S: calc_Sij's S11..S33, u, v, w (ld,ny,nz) ---- */
+int les_new_scaledep_dynamic(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                             double *Cs_opt2);
 /* ---- lagrange_Ssim.f90:4-300 (calls interpolag_Ssim first) and
    interpolag_Ssim.f90 on the F_LM, F_MM, Beta, Cs_opt2, u_lag.. members of s;
    Beta_avg and Betaclip_avg receive nx*ny, the plane sums of Beta = 1 ---- */
@@ -159,7 +168,8 @@ int les_scaledep_dynamic(les_ctx *c, double *const S[6], const double *u, const 
    Must precede les_state_alloc for the state to carry the scalars. */
 typedef struct les_scalar_params {
   int enable;      /* S_FLAG: theta_all_in_one + q_all_in_one every step */
-  int model;       /* 7: Pr_ = Pr_t of lagrange_scalar_Sdep; otherwise Pr */
+  int model;       /* 7: Pr_ = Pr_t of lagrange_scalar_Sdep (SGS model 5); 6: Pr_t of
+                      scalar_scaledep_dynamic (SGS model 6); otherwise Pr */
   int initsc;      /* scalar history given (no Euler scalar step at jt==1) */
   int passive;     /* passive_scalar: no buoyancy in RHSz */
   double Pr, z_i, u_star, T_scale, q_scale, g;
@@ -203,7 +213,15 @@ int les_interpolag_scalar_sdep(les_ctx *c, const les_scalar_sgs_fields *s);
    reads TnLMMM, TnQNNN, Cs_opt2 of m and overwrites m->Beta */
 int les_lagrange_scalar_sdep(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
                              double *theta, const les_sgs_fields *m, const les_scalar_sgs_fields *s);
-/* sgs_stag.f90:3-500 with model=7 (:59-69,150-153,166-173) */
+/* scalar_scaledep_dynamic.f90:1-307 (model 6 with scalars, lbc = 1): theta
+   is Nyquist-filtered in place (:60); Cs_opt2 is the (ld,ny,nz) field; writes
+   s->Pr_t on levels 1..nz-1 and the routine's Cs2_Pr(1..nz-1) into the first
+   nz-1 doubles of s->Cs2_Pr (sgs_stag.f90:149 binds its (ld,ny,nz) array);
+   reads s->phi_h, s->psi_m ---- */
+int les_scalar_scaledep_dynamic(les_ctx *c, double *const S[6], const double *u, const double *v, const double *w,
+                                double *theta, const double *Cs_opt2, const les_scalar_sgs_fields *s);
+/* sgs_stag.f90:3-500 with model=7 (:59-69,150-153,166-173), or with model 6
+   (:143-149, scalar_scaledep_dynamic) when SGS model 6 is selected */
 int les_sgs_stag_scalar(les_ctx *c, double *const grad[9], const double *u, const double *v, const double *w,
                         double *theta, double *const tau[6], const les_sgs_fields *m,
                         const les_scalar_sgs_fields *s);

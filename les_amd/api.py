@@ -258,8 +258,10 @@ class Les:
 
     def set_sgs_model(self, model):
         """param.f90 'model': 1 Smagorinsky with Mason's wall damping, 2 standard
-        dynamic, 3 plane-averaged scale-dependent, 4 Lagrangian scale-invariant,
-        5 Lagrangian scale-dependent."""
+        dynamic, 3 plane-averaged scale-dependent (quintic for beta), 4
+        Lagrangian scale-invariant, 5 Lagrangian scale-dependent, 6
+        plane-averaged scale-dependent (new_scaledep_dynamic; with scalars,
+        scalar model 6: Pr_t of scalar_scaledep_dynamic)."""
         self._chk(self.L.les_set_sgs_model(self.ctx, int(model)))
 
     def std_dynamic(self, S, u, v, w):
@@ -267,6 +269,15 @@ class Les:
         s = self._arr(S, self.S_NAMES)
         cs = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
         self._chk(self.L.les_std_dynamic(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"), _ptr(cs)))
+        return cs
+
+    def new_scaledep_dynamic(self, S, u, v, w):
+        """new_scaledep_dynamic.f90 (model 6): the per-level Cs_opt2 as an (nz,)
+        float64 GPU tensor."""
+        s = self._arr(S, self.S_NAMES)
+        cs = torch.zeros(self.nz, dtype=torch.float64, device="cuda")
+        self._chk(self.L.les_new_scaledep_dynamic(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
+                                                  _ptr(cs)))
         return cs
 
     def lagrange_ssim(self, S, u, v, w, sgs):
@@ -477,6 +488,16 @@ class Les:
         self._chk(self.L.les_lagrange_scalar_sdep(self.ctx, s, self._f(u, "u"), self._f(v, "v"), self._f(w, "w"),
                                                   self._f(theta, "theta"), C.byref(m), C.byref(st)))
 
+    def scalar_scaledep_dynamic(self, S, u, v, w, theta, Cs_opt2, ssgs):
+        """scalar_scaledep_dynamic.f90 (model 6 with scalars): filters theta in
+        place, writes ssgs["Pr_t"] on levels 1..nz-1 and the per-level Cs2_Pr
+        into the first nz-1 values of ssgs["Cs2_Pr"]."""
+        s = self._arr(S, self.S_NAMES)
+        st = self._ssgs_struct(ssgs)
+        self._chk(self.L.les_scalar_scaledep_dynamic(self.ctx, s, self._f(u, "u"), self._f(v, "v"),
+                                                     self._f(w, "w"), self._f(theta, "theta"),
+                                                     self._f(Cs_opt2, "Cs_opt2"), C.byref(st)))
+
     def sgs_stag_scalar(self, grads, u, v, w, theta, taus, sgs, ssgs):
         g = self._arr(grads, self.GRAD_NAMES)
         t = self._arr(taus, self.TAU_NAMES)
@@ -577,7 +598,7 @@ class LesSim(Les):
     def __init__(self, nx, ny, nz, precision="fp64", graph=False, scalars=None, sgs_model=5, forcing=None, **kw):
         """scalars: None (neutral) or a dict of set_scalars arguments (the
         convective step with theta, q, buoyancy and the scalar SGS model).
-        sgs_model: param.f90 'model', 1..5 (set_sgs_model).  forcing: None or a
+        sgs_model: param.f90 'model', 1..6 (set_sgs_model).  forcing: None or a
         dict of set_forcing arguments (Coriolis, sponge)."""
         # a stream of its own (default priority): graph capture needs a
         # non-default stream, and the state is only touched through the library

@@ -290,7 +290,7 @@ static void step_once(les_ctx* c) {
                    (const double*)state_ptr(c, "psi_m0"), (const double*)state_ptr(c, "phi_m"),
                    (double*)state_ptr(c, "ustar_avg"));
   const T* grc[9] = {gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]};
-  if (scal && c->scal.model == 7) {
+  if (scal && (c->scal.model == 7 || c->sgs_model == 6)) {
     const ScalarSgsPtrs ss = scalar_sgs_ptrs<T>(c);
     op_sgs_stag<T>(c, grc, u, v, w, tau, sgs_ptrs<T>(c), F<T>(c, "theta"), &ss);
   } else {
@@ -321,7 +321,7 @@ static void step_once(les_ctx* c) {
     const T* grc2[9] = {gr[0], gr[1], gr[2], gr[3], gr[4], gr[5], gr[6], gr[7], gr[8]};
     op_convec_scalar_ab2<T>(c, u, v, w, grc2, R3, Rf3, dv3, euler,
                             P.passive ? (const T*)nullptr : (const T*)F<T>(c, "beta_scal"), jobs, F<T>(c, "Nu_t"),
-                            P.model == 7 ? (const T*)F<T>(c, "Pr_t") : (const T*)nullptr,
+                            (P.model == 7 || c->sgs_model == 6) ? (const T*)F<T>(c, "Pr_t") : (const T*)nullptr,
                             (c->jt == 1 && !P.initsc) ? 1 : 0, fop, &si);
   } else {
     // convec + RHS assembly + AB2 predictor in one epilogue (main.f90:304-390)
@@ -675,14 +675,20 @@ int les_lagrange_sdep(les_ctx* c, double* const S[6], const double* u, const dou
 }
 int les_set_sgs_model(les_ctx* c, int model) {
   GUARD(c);
-  if (model < 1 || model > 5) {
+  if (model < 1 || model > 6) {
     c->err = "les_set_sgs_model: model must be 1 (Smagorinsky-Mason), 2 (standard dynamic), 3 (plane-averaged "
-             "scale-dependent), 4 (Lagrangian scale-invariant) or 5 (Lagrangian scale-dependent)";
+             "scale-dependent), 4 (Lagrangian scale-invariant), 5 (Lagrangian scale-dependent) or 6 "
+             "(plane-averaged scale-dependent, new_scaledep_dynamic)";
     return 1;
   }
   if (model != 5 && c->scal.enable && c->scal.model == 7) {
     c->err = "les_set_sgs_model: the scalar Lagrangian model (model 7) runs with lagrange_Sdep, SGS model 5 "
              "(sgs_stag.f90:150-153)";
+    return 1;
+  }
+  if (model == 6 && c->scal.enable && c->scal.model != 6) {
+    c->err = "les_set_sgs_model: SGS model 6 takes Pr_t from scalar_scaledep_dynamic (scalars_module.f90:77-84); "
+             "set the scalar model to 6";
     return 1;
   }
   if (model != 5 && c->closure_mode) {
@@ -706,6 +712,14 @@ int les_std_dynamic(les_ctx* c, double* const S[6], const double* u, const doubl
   if (!S || !Cs_opt2) return -1;
   const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
   op_std_dynamic<double>(c, Sc, u, v, w, Cs_opt2, (double*)nullptr);
+  DONE(c);
+}
+int les_new_scaledep_dynamic(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                             double* Cs_opt2) {
+  GUARD(c);
+  if (!S || !Cs_opt2) return -1;
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  op_new_scaledep_dynamic<double>(c, Sc, u, v, w, Cs_opt2, (double*)nullptr);
   DONE(c);
 }
 int les_lagrange_ssim(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
@@ -1737,6 +1751,11 @@ int les_set_scalars(les_ctx* c, const les_scalar_params* p, const double* wt_s, 
     c->err = "les_set_scalars: the scalar Lagrangian model (model 7) runs with lagrange_Sdep; select SGS model 5";
     return 1;
   }
+  if (p->enable && (p->model == 6) != (c->sgs_model == 6)) {
+    c->err = "les_set_scalars: scalar model 6 (Pr_t of scalar_scaledep_dynamic) goes with SGS model 6 and only "
+             "with it (sgs_stag.f90:143-149, scalars_module.f90:77-84)";
+    return 1;
+  }
   ScalarParams& P = c->scal;
   P.enable = p->enable; P.model = p->model; P.initsc = p->initsc; P.passive = p->passive;
   P.Pr = p->Pr; P.z_i = p->z_i; P.u_star = p->u_star; P.T_scale = p->T_scale; P.q_scale = p->q_scale;
@@ -1841,11 +1860,25 @@ int les_sgs_stag_scalar(les_ctx* c, double* const grad[9], const double* u, cons
   GUARD(c);
   if (!m || !s) return -1;
   if (!c->wt_s) { c->err = "les_sgs_stag_scalar: call les_set_scalars first"; return 1; }
-  if (c->sgs_model != 5) { c->err = "les_sgs_stag_scalar: the scalar SGS model needs SGS model 5"; return 1; }
+  if (c->sgs_model != 5 && c->sgs_model != 6) {
+    c->err = "les_sgs_stag_scalar: the scalar SGS models need SGS model 5 (scalar model 7) or 6";
+    return 1;
+  }
   const double* g[9];
   for (int i = 0; i < 9; ++i) g[i] = grad[i];
   const les::ScalarSgsPtrs ss = to_sptrs(s);
   op_sgs_stag<double>(c, g, u, v, w, tau, to_ptrs(m), theta, &ss);
+  DONE(c);
+}
+
+int les_scalar_scaledep_dynamic(les_ctx* c, double* const S[6], const double* u, const double* v, const double* w,
+                                double* theta, const double* Cs_opt2, const les_scalar_sgs_fields* s) {
+  GUARD(c);
+  if (!S || !s || !theta || !Cs_opt2) return -1;
+  if (!c->wt_s) { c->err = "les_scalar_scaledep_dynamic: call les_set_scalars first"; return 1; }
+  const double* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+  const les::ScalarSgsPtrs ss = to_sptrs(s);
+  op_scalar_scaledep<double>(c, Sc, u, v, w, theta, Cs_opt2, ss, false);
   DONE(c);
 }
 

@@ -1,15 +1,17 @@
-// les_dynamic.hip -- SGS model 3 on gfx950: the scale-dependent dynamic
-// Smagorinsky model with plane-averaged coefficients (scaledep_dynamic.f90,
-// rtnewt of sgsmodule.f90:37-59), as sgs_stag.f90:128-140 calls it.
+// les_dynamic.hip -- the plane-averaged dynamic SGS models on gfx950: model 2
+// (std_dynamic.f90), model 3 (scaledep_dynamic.f90 with rtnewt of
+// sgsmodule.f90:37-59, as sgs_stag.f90:128-140 calls it) and model 6
+// (new_scaledep_dynamic.f90, sgs_stag.f90:143-148).
 //
-// Per level the routine needs 21 fields filtered at 2 Delta and 4 Delta --
-// exactly the lagrange_Sdep filter set, so lag_filter_mom serves it -- plus
+// Per level they need the 21 fields that lagrange_Sdep filters at 2 Delta
+// and 4 Delta -- lag_filter_mom serves all three -- and models 3 and 6 also
 // Q_ij, whose products are of the already 2-Delta-filtered velocities
 // (scaledep_dynamic.f90:94-115): one more batched transform of 6 fields.
-// The ten plane sums of the quintic (:183-215) and then L.M, M.M (:233-244)
-// are deterministic two-pass reductions in double (NPART blocks per level,
-// fixed-order final add); the quintic's Newton solve runs on the device, one
-// thread per level, so the whole model stays inside a captured step.
+// Model 3's ten plane sums of the quintic (:183-215), then L.M, M.M
+// (:233-244), and model 6's four sums are deterministic two-pass reductions
+// in double (NPART blocks per level, fixed-order final add); the quintic's
+// Newton solve runs on the device, one thread per level, so every model
+// stays inside a captured step.
 #include "les_xform.h"
 
 // Evaluated as written (no multiply-add contraction), like the CPU
@@ -199,17 +201,14 @@ __global__ void k_sd_fill(const double* __restrict__ cs, T* Cs, long pl, long n3
     Cs[i] = (T)cs[(unsigned)i / (unsigned)pl];
 }
 
+// Q_ij input of models 3 and 6: the 4-Delta filter of products of the
+// 2-Delta filtered velocities (scaledep_dynamic.f90:94-115,
+// new_scaledep_dynamic.f90:94-115) into scratch, 6 fields (ld,ny,nz)
 template <typename T>
-void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
-                         double* beta_lev, T* Cs_planes) {
-  OpTag _tag(c, "scaledep");
+static const T* sd_q6(les_ctx* c, const T* F2) {
   const Grid& g = c->g;
   const int nz = g.nz, k4 = c->keep4;
   const size_t n3 = (size_t)g.ld * g.ny * nz;
-  const T* F = lag_filter_mom<T>(c, S, u, v, w);  // F2 = F, F4 = F + 21 n3
-  const T* F2 = F;
-  const T* F4 = F + 21 * n3;
-  // Q_ij input: Gtt(u_bar_i u_bar_j), :94-115
   const long rows = 6L * nz * g.ny;
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * k4 * sizeof(cx<T>));
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k4 * sizeof(cx<T>));
@@ -221,6 +220,67 @@ void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v
   T* Q6 = (T*)scratch(c, S_SDQ, 6 * n3 * sizeof(T));
   StoreFields<T> SQ{{Q6, nullptr, nullptr, nullptr}, 6 * nz, g.ny, g.ld, g.nx};
   launch_row_inv<T>(c, X1, k4, k4, SQ, rows, g.nx, g.ld, g.ny);
+  return Q6;
+}
+
+// SGS model 6, new_scaledep_dynamic.f90:183-205: per level L.M, M.M, Q.N and
+// N.N with M_ij = 2 Delta^2 (|S|S_ij bar - 4 |S_bar| S_ij bar) and N_ij the
+// same at 4 Delta with 16
+template <typename T>
+__global__ void __launch_bounds__(256) k_nsd_sums(const T* __restrict__ F2, const T* __restrict__ F4,
+                                                  const T* __restrict__ Q6, long n3, int nx, int ld, int ny,
+                                                  double cst, double* part) {
+  const int k = blockIdx.y, n = nx * ny;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    SdPoint<T> p;
+    p.load(F2, F4, Q6, n3, ((long)k * ny + y) * ld + x);
+    double M[6], N[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      M[q] = cst * (p.ssb[q] - 4.0 * p.Sb * p.sb[q]);
+      N[q] = cst * (p.ssh[q] - 16.0 * p.Sh * p.sh[q]);
+    }
+    a[0] += sym_dot(p.L, M);
+    a[1] += sym_dot(M, M);
+    a[2] += sym_dot(p.Q, N);
+    a[3] += sym_dot(N, N);
+  }
+  block_sum_out<4>(a, part + ((long)k * gridDim.x + blockIdx.x) * 4);
+}
+
+// :210-220: LM/MM and QN/NN clipped at 1E-24 in single precision, Beta their
+// ratio clipped below at 1/8 in single precision, Cs_opt2 = max(0, LM/MM /
+// betaclip); the same rule gives Cs2_Pr of scalar_scaledep_dynamic.f90:230-240
+__device__ __forceinline__ double sd_clipped_ratio(double lm, double mm, double qn, double nn) {
+  const double c2 = (double)fmaxf(1e-24f, (float)(lm / mm));
+  const double c4 = (double)fmaxf(1e-24f, (float)(qn / nn));
+  const double beta = c4 / c2;
+  const double betaclip = (double)fmaxf((float)beta, (float)(1.0 / 8.0));
+  return fmax(0.0, c2 / betaclip);
+}
+
+__global__ void k_nsd_cs(const double* __restrict__ part, int nparts, int nlev, double* cs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nlev) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < nparts; ++p)
+    for (int q = 0; q < 4; ++q) s[q] += part[((long)k * nparts + p) * 4 + q];
+  cs[k] = sd_clipped_ratio(s[0], s[1], s[2], s[3]);
+}
+
+template <typename T>
+void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                         double* beta_lev, T* Cs_planes) {
+  OpTag _tag(c, "scaledep");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  const T* F = lag_filter_mom<T>(c, S, u, v, w);  // F2 = F, F4 = F + 21 n3
+  const T* F2 = F;
+  const T* F4 = F + 21 * n3;
+  const T* Q6 = sd_q6<T>(c, F2);  // :94-115
   double* part = (double*)scratch(c, S_RED, (size_t)SD_NPART * nz * SD_NS * sizeof(double));
   double* lev = (double*)scratch(c, S_SDLEV, 2 * (size_t)nz * sizeof(double));
   double* beta = beta_lev ? beta_lev : lev;
@@ -285,10 +345,45 @@ void op_std_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, con
   LCHK(c, hipGetLastError());
 }
 
+// SGS model 6, new_scaledep_dynamic.f90 (sgs_stag.f90:143-148): the model-3
+// filter set (lag_filter_mom on calc_Sij's S_ij, then Q_ij), four plane sums
+// per level and the clipped ratios, one Cs_opt2 per level
+template <typename T>
+void op_new_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                             T* Cs_planes) {
+  OpTag _tag(c, "new_scaledep");
+  const Grid& g = c->g;
+  const int nz = g.nz;
+  const size_t n3 = (size_t)g.ld * g.ny * nz;
+  const T* F = lag_filter_mom<T>(c, S, u, v, w);
+  const T* F2 = F;
+  const T* F4 = F + 21 * n3;
+  const T* Q6 = sd_q6<T>(c, F2);
+  double* part = (double*)scratch(c, S_RED, (size_t)SD_NPART * nz * 4 * sizeof(double));
+  double* lev = (double*)scratch(c, S_SDLEV, 2 * (size_t)nz * sizeof(double));
+  double* cs = cs_lev ? cs_lev : lev + nz;
+  {
+    ProfScope _ps(c, "sd_sums", (42.0 + 6) * (double)g.nx * g.ny * nz * sizeof(T));
+    hipLaunchKernelGGL((k_nsd_sums<T>), dim3(SD_NPART, nz), dim3(256), 0, c->stream, F2, F4, Q6, (long)n3, g.nx,
+                       g.ld, g.ny, 2.0 * (g.delta * g.delta), part);
+  }
+  LCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_nsd_cs, dim3((nz + 63) / 64), dim3(64), 0, c->stream, (const double*)part, SD_NPART, nz, cs);
+  LCHK(c, hipGetLastError());
+  if (Cs_planes) {
+    ProfScope _ps(c, "sd_fill", (double)n3 * sizeof(T));
+    hipLaunchKernelGGL((k_sd_fill<T>), grid1(n3), dim3(256), 0, c->stream, (const double*)cs, Cs_planes,
+                       (long)g.ld * g.ny, (long)n3);
+  }
+  LCHK(c, hipGetLastError());
+}
+
 #define LES_SD_INST(T)                                                                                      \
   template void op_scaledep_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*, \
                                        double*, T*);                                                        \
-  template void op_std_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*, T*);
+  template void op_std_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*, T*);      \
+  template void op_new_scaledep_dynamic<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, double*,  \
+                                           T*);
 LES_SD_INST(double)
 LES_SD_INST(float)
 #undef LES_SD_INST

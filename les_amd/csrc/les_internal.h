@@ -285,6 +285,18 @@ void op_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v
 template <typename T>
 void op_std_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
                     T* Cs_planes);
+// SGS model 6 (les_dynamic.hip): new_scaledep_dynamic.f90, the same outputs
+template <typename T>
+void op_new_scaledep_dynamic(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, double* cs_lev,
+                             T* Cs_planes);
+// its scalar part (les_scalars.hip): scalar_scaledep_dynamic.f90 filters theta
+// in place, writes Cs2_Pr(1..nz-1) into the first nz-1 elements of ss.Cs2_Pr
+// (as the reference's call binds sgs_stag's (ld,ny,nz) array) and
+// Pr_t = Cs_opt2 / Cs2_Pr on levels 1..nz-1; reuse_mom: the momentum filter
+// set of this update is already in its scratch slot
+template <typename T>
+void op_scalar_scaledep(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, T* theta,
+                        const T* Cs_opt2, const ScalarSgsPtrs& ss, bool reuse_mom);
 
 // ---- learned closure (les_mlp.hip) ----
 int mlp_nparam();

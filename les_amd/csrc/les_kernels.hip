@@ -1080,14 +1080,16 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   const size_t n3 = (size_t)g.ld * g.ny * g.nz;
   const int model = c->sgs_model;
   const bool lagr = model == 4 || model == 5;  // u_lag accumulates under models 4, 5, 7 (sgs_stag.f90:59-69)
+  const bool lag_s = ss && model == 5;          // u_lag_s too, with the scalar Lagrangian model
   T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
   T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
   {
-    ProfScope _ps(c, "calc_sij", (9.0 + 6 + 6 + (ss ? 6 : 3)) * n3 * sizeof(T));
+    ProfScope _ps(c, "calc_sij", (9.0 + 6 + (lagr ? 6 : 0) + (lag_s ? 6 : 0) + 3) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_calc_sij<T>), grid1(n3), dim3(256), 0, c->stream, gr[0], gr[1], gr[2], gr[3], gr[4], gr[5],
                        gr[6], gr[7], gr[8], S[0], S[1], S[2], S[3], S[4], S[5], g.nx, g.ld, g.ny, g.nz, u, v, w,
-                       lagr ? (T*)s.u_lag : (T*)nullptr, (T*)s.v_lag, (T*)s.w_lag, ss ? (T*)ss->u_lag_s : (T*)nullptr,
-                       ss ? (T*)ss->v_lag_s : (T*)nullptr, ss ? (T*)ss->w_lag_s : (T*)nullptr);
+                       lagr ? (T*)s.u_lag : (T*)nullptr, (T*)s.v_lag, (T*)s.w_lag,
+                       lag_s ? (T*)ss->u_lag_s : (T*)nullptr, lag_s ? (T*)ss->v_lag_s : (T*)nullptr,
+                       lag_s ? (T*)ss->w_lag_s : (T*)nullptr);
   }
   LCHK(c, hipGetLastError());
   const int jt = c->jt;
@@ -1107,6 +1109,10 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
       // so its S11..S33 are bound to u, v, w, S11, S12, S13
       const T* Sd[6] = {u, v, w, S[0], S[1], S[2]};
       op_scaledep_dynamic<T>(c, Sd, u, v, w, nullptr, nullptr, (T*)s.Cs_opt2);
+    } else if (model == 6) {  // :143-149
+      const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
+      op_new_scaledep_dynamic<T>(c, Sc, u, v, w, nullptr, (T*)s.Cs_opt2);
+      if (ss) op_scalar_scaledep<T>(c, Sc, u, v, w, theta, (const T*)s.Cs_opt2, *ss, true);
     } else if (c->closure_mode && c->mlp && jt % c->closure_period != 0) {
       // learned closure between dynamic updates: only the filtered strain
       // rates are needed (fields 9..20 of the lagrange_Sdep filter set)
@@ -1125,7 +1131,7 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   }
   // :165-174 sets Cs_opt2 = 0.16**2 (default real) under models 5-7 while
   // jt_total < 500; model 1 sets Co**2 = 0.16_rprec**2 every step
-  const int use_const = model == 1 || (model == 5 && c->jt_total < 500);
+  const int use_const = model == 1 || ((model == 5 || model == 6) && c->jt_total < 500);
   const T cs0 = model == 1 ? (T)(0.16 * 0.16) : (T)(double)(0.16f * 0.16f);
   const T l2 = (T)(g.delta * g.delta);
   {

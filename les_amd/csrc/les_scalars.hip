@@ -258,8 +258,10 @@ struct DzArgs {
   T top[2];
   int ns, surface_only, lag;
   const double *phi_h, *ustar;
-  const double* lagsums;  // lag: {sum |u(1)|, sum psi_m}
+  const double* lagsums;  // lag: {sum |u(1)|, sum psi_m, sum phi_h (lag 2)}
   double zlog, sflux;     // lag: ln(dz/2/zo_avg), mean surface flux
+  // lag 1: lagrange_scalar_Sdep's first-level dTdz (local phi_h); lag 2:
+  // scalar_scaledep_dynamic's (plane-mean phi_h, opposite sign)
 };
 
 // TPT consecutive points per thread, both scalars per point; surface_only
@@ -285,9 +287,13 @@ __global__ void k_scalar_dz(DzArgs<T> a, int nx, int ld, int ny, int nz, double 
       if (k == 0) {
         const int j = x + nx * y;
         double v;
-        if (a.lag) {
+        if (a.lag == 1) {
           const double ustar_ = a.lagsums[0] / fn * VONK / (a.zlog - a.lagsums[1] / fn);
           v = -a.phi_h[j] * a.sflux / (ustar_ * VONK * dz / 2.0);
+        } else if (a.lag == 2) {  // scalar_scaledep_dynamic.f90:75-79: plane-mean phi_h, no sign change
+          const double ustar_ = a.lagsums[0] / fn * VONK / (a.zlog - a.lagsums[1] / fn);
+          const double phi_h_avg = a.lagsums[2] / fn;
+          v = phi_h_avg * a.sflux / (ustar_ * VONK * dz / 2.0);
         } else {
           v = -a.phi_h[j] * a.sf[q](j) / (a.ustar[j] * VONK * dz / 2.0);
         }
@@ -912,6 +918,228 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   if (init_qn) c->gqn_init = 1;
 }
 
+// ------------------------------------------------------------------------
+// scalar_scaledep_dynamic.f90 (SGS model 6 with scalars): the plane-averaged
+// scale-dependent Cs^2/Pr_sgs.  Ten fields through both test filters as in
+// lagrange_scalar_Sdep, but theta is read after its in-place filt_da (:60)
+// and |S| dT_i, dT_i are taken on level jz as they are (:179-200, no
+// staggering average); Q_i filters products of the 2-Delta bars (:123-137),
+// one more 4-Delta pass of 3 fields.  Plane sums over levels 1..nz-1.
+// ------------------------------------------------------------------------
+constexpr int SSD_NPART = 32;  // partial-sum blocks per level
+
+// four block sums of 256 threads in a fixed order
+__device__ void block_sums4(const double (&acc)[4], double* out) {
+  __shared__ double sh[4][256];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) sh[s][threadIdx.x] = acc[s];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <typename T>
+struct LoadSdScalarAll {  // theta_b | u_b th_b, v_b th_b, w_b th_b | |S| dT (x,y,z) | dT (x,y,z)
+  static constexpr int NF = 10;
+  const T *u, *v, *w, *th, *S11, *S12, *S13, *S22, *S23, *S33, *dx, *dy, *dz;
+  int ny, nz, ld;
+  static constexpr double reads() { return 13.0 / 10.0; }
+  __device__ void all(int k, int y, int x, T (&r)[NF]) const {
+    const long i = ((long)k * ny + y) * ld + x, pl = (long)ld * ny;
+    const T tb = k == 0 ? th[i] : T(0.5) * (th[i] + th[i - pl]);  // :84-105
+    const T ub = k == 0 ? u[i] : T(0.5) * (u[i] + u[i - pl]);
+    const T vb = k == 0 ? v[i] : T(0.5) * (v[i] + v[i - pl]);
+    const T wb = k == 0 ? T(0.25) * w[i + pl] : w[i];
+    r[0] = tb;
+    r[1] = ub * tb;
+    r[2] = vb * tb;
+    r[3] = wb * tb;
+    const T S = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);  // :140-142
+    r[4] = S * dx[i];
+    r[5] = S * dy[i];
+    r[6] = S * dz[i];
+    r[7] = dx[i];
+    r[8] = dy[i];
+    r[9] = dz[i];
+  }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int f = plane / nz, k = plane - f * nz;
+    T r[NF];
+    all(k, y, x, r);
+    return r[f];
+  }
+};
+
+// u_bar_i theta_bar of the 2-Delta bars (:123-125): the input of Q_i's filter
+template <typename T>
+struct LoadBarTheta {
+  const T *F, *Fs;  // momentum filter set (u, v, w bars: fields 0..2); scalar set (theta bar: field 0)
+  long n3;
+  int ny, nz, ld;
+  static constexpr double reads() { return 2.0; }
+  __device__ T operator()(int plane, int y, int x) const {
+    const int q = plane / nz, k = plane - q * nz;
+    const long i = ((long)k * ny + y) * ld + x;
+    return F[q * n3 + i] * Fs[i];
+  }
+};
+
+// :211-231: L_i.M_i, M_i.M_i, Q_i.N_i, N_i.N_i per level, const = Delta^2,
+// M_i = const (|S| dT_i bar - 4 |S_bar| dT_i bar), N_i with 16 at 4 Delta
+template <typename T>
+__global__ void __launch_bounds__(256) k_ssd_sums(const T* __restrict__ F, const T* __restrict__ Fs,
+                                                  const T* __restrict__ Q3, long n3, int nx, int ld, int ny,
+                                                  double cst, double* part) {
+  const int k = blockIdx.y, n = nx * ny;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int y = j / nx, x = j - y * nx;
+    const long i = ((long)k * ny + y) * ld + x;
+    auto f2m = [&](int f) { return (double)F[f * n3 + i]; };
+    auto f4m = [&](int f) { return (double)F[(21 + f) * n3 + i]; };
+    auto f2s = [&](int f) { return (double)Fs[f * n3 + i]; };
+    auto f4s = [&](int f) { return (double)Fs[(10 + f) * n3 + i]; };
+    const double tb = f2s(0), th = f4s(0);
+    double sb[6], sh[6], L[3], Q[3], M[3], N[3];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) { sb[m] = f2m(9 + m); sh[m] = f4m(9 + m); }
+    const double Sb = sqrt(2.0 * (sb[0] * sb[0] + sb[3] * sb[3] + sb[5] * sb[5] +
+                                  2.0 * (sb[1] * sb[1] + sb[2] * sb[2] + sb[4] * sb[4])));
+    const double Sh = sqrt(2.0 * (sh[0] * sh[0] + sh[3] * sh[3] + sh[5] * sh[5] +
+                                  2.0 * (sh[1] * sh[1] + sh[2] * sh[2] + sh[4] * sh[4])));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      L[q] = f2s(1 + q) - f2m(q) * tb;                    // :116-121
+      Q[q] = (double)Q3[q * n3 + i] - f4m(q) * th;        // :132-137
+      M[q] = cst * (f2s(4 + q) - 4.0 * Sb * f2s(7 + q));  // :212-214
+      N[q] = cst * (f4s(4 + q) - 16.0 * Sh * f4s(7 + q));
+    }
+    a[0] += L[0] * M[0] + L[1] * M[1] + L[2] * M[2];
+    a[1] += M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
+    a[2] += Q[0] * N[0] + Q[1] * N[1] + Q[2] * N[2];
+    a[3] += N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+  }
+  block_sums4(a, part + ((long)k * gridDim.x + blockIdx.x) * 4);
+}
+
+// per level :230-240 (the clips of the momentum routine) and Pr_t = Cs_opt2 /
+// Cs2_Pr (:242) over the whole (ld, ny) plane; Cs2_Pr(jz) into element jz-1
+template <typename T>
+__global__ void k_ssd_cs(const double* __restrict__ part, int nparts, int nl, double* cpr, T* Cs2_Pr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nl) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < nparts; ++p)
+    for (int q = 0; q < 4; ++q) s[q] += part[((long)k * nparts + p) * 4 + q];
+  const double c2 = (double)fmaxf(1e-24f, (float)(s[0] / s[1]));
+  const double c4 = (double)fmaxf(1e-24f, (float)(s[2] / s[3]));
+  const double beta = c4 / c2;
+  const double betaclip = (double)fmaxf((float)beta, (float)(1.0 / 8.0));
+  const double v = fmax(0.0, c2 / betaclip);
+  cpr[k] = v;
+  Cs2_Pr[k] = (T)v;
+}
+// Model 6 leaves Cs_opt2 unclipped below (Cs2_2d^2 / Cs2_4d can reach
+// 1e-48), so in FP32-bulk a Cs_opt2 that underflows to 0 would make Pr_t 0
+// and the scalar flux Nu_t / Pr_t 0/0; there Pr_t is kept at or above the
+// smallest normal float (the flux is then 0, as the FP64 ratio nearly is)
+template <typename T>
+__global__ void k_ssd_prt(const T* __restrict__ Cs, const double* __restrict__ cpr, T* Pr_t, unsigned pl,
+                          unsigned n) {
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double r = (double)Cs[i] / cpr[i / pl];
+    if constexpr (sizeof(T) == 4) r = fmax(r, 1.1754943508222875e-38);
+    Pr_t[i] = (T)r;
+  }
+}
+
+template <typename T>
+void op_scalar_scaledep(les_ctx* c, const T* const S[6], const T* u, const T* v, const T* w, T* theta,
+                        const T* Cs_opt2, const ScalarSgsPtrs& ss, bool reuse_mom) {
+  OpTag _tag(c, "scalar_scaledep");
+  const Grid& g = c->g;
+  const ScalarParams& P = c->scal;
+  const int nz = g.nz, nl = nz - 1;
+  const size_t n3 = (size_t)g.ld * g.ny * nz, pl = (size_t)g.ld * g.ny;
+  T* dTx = (T*)scratch(c, S_SC1, n3 * sizeof(T));
+  T* dTy = (T*)scratch(c, S_SC2, n3 * sizeof(T));
+  T* dTz = (T*)scratch(c, S_SC3, n3 * sizeof(T));
+  op_filt_da<T>(c, theta, dTx, dTy, nz);  // :60, theta filtered in place
+  double* red = (double*)scratch(c, S_RED2, (R2_BETA + nz) * sizeof(double));
+  level_sums(c, SumSpeed<T>{u, v, g.ld}, 1, red + R2_LAG, 2.0 * g.nx * g.ny * sizeof(T));
+  level_sums(c, SumPlane<double>{ss.psi_m, g.nx, 0.0}, 1, red + R2_LAG + 1, (double)g.nx * g.ny * 8);
+  level_sums(c, SumPlane<double>{ss.phi_h, g.nx, 0.0}, 1, red + R2_LAG + 2, (double)g.nx * g.ny * 8);
+  {  // ddz_uv with the lid gradient and the first-level value of :61-79
+    DzArgs<T> a{};
+    a.ns = 1;
+    a.s[0] = theta;
+    a.dz[0] = dTz;
+    a.top[0] = (T)lid_gradient(c, 0);
+    a.lag = 2;
+    a.lagsums = red + R2_LAG;
+    a.zlog = log(0.5 * g.dz / c->zo_avg);
+    a.sflux = c->wt_mean / P.T_scale / P.u_star;  // :78
+    ProfScope _ps(c, "scalar_dz", 2.0 * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_scalar_dz<T, 4>), dim3((unsigned)((n3 + 1023) / 1024)), dim3(256), 0, c->stream, a, g.nx,
+                       g.ld, g.ny, nz, g.dz, T(1.0 / g.dz), (unsigned)n3);
+  }
+  LCHK(c, hipGetLastError());
+  const T* F = reuse_mom ? (const T*)scratch(c, S_LAGF, 42 * n3 * sizeof(T)) : lag_filter_mom<T>(c, S, u, v, w);
+  const int k2 = c->keep2, k4 = c->keep4, kf = k2 > k4 ? k2 : k4;
+  const Tables<T>& t = tables<T>(c);
+  const long rows = 10L * nz * g.ny;
+  cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, (size_t)rows * kf * sizeof(cx<T>));
+  LoadSdScalarAll<T> L{u, v, w, theta, S[0], S[1], S[2], S[3], S[4], S[5], dTx, dTy, dTz, g.ny, nz, g.ld};
+  if (!launch_row_fwd_mf<T>(c, L, X0, kf, 13.0)) launch_row_fwd<T>(c, L, X0, kf, rows, g.nx, kf, g.ny);
+  cx<T>* X2 = (cx<T>*)scratch(c, S_XS1, (size_t)rows * k2 * sizeof(cx<T>));
+  cx<T>* X4 = (cx<T>*)scratch(c, S_XS2, (size_t)rows * k4 * sizeof(cx<T>));
+  ColArgs a = col_args(g.ny, g.ny, kf, kf, (long)g.ny * kf);
+  a.lhG = g.lh;
+  col_out(a, 0, K_G, k2, k2, (long)g.ny * k2);
+  col_out(a, 1, K_G, k4, k4, (long)g.ny * k4);
+  launch_col<T>(c, X0, nullptr, X2, X4, nullptr, a, 10L * nz, kf, t.G2, t.G4);
+  T* Fs = (T*)scratch(c, S_LAGF2, 20 * n3 * sizeof(T));
+  StoreFields<T> S2{{Fs, nullptr, nullptr, nullptr}, 10 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X2, k2, k2, S2, rows, g.nx, g.ld, g.ny);
+  StoreFields<T> S4{{Fs + 10 * n3, nullptr, nullptr, nullptr}, 10 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, X4, k4, k4, S4, rows, g.nx, g.ld, g.ny);
+  // Q_i: Gtt(u_bar_i theta_bar)
+  const long rq = 3L * nz * g.ny;
+  cx<T>* Y0 = (cx<T>*)scratch(c, S_XS0, (size_t)rq * k4 * sizeof(cx<T>));
+  cx<T>* Y1 = (cx<T>*)scratch(c, S_XS1, (size_t)rq * k4 * sizeof(cx<T>));
+  launch_row_fwd<T>(c, LoadBarTheta<T>{F, Fs, (long)n3, g.ny, nz, g.ld}, Y0, k4, rq, g.nx, k4, g.ny);
+  ColArgs aq = col_args(g.ny, g.ny, k4, k4, (long)g.ny * k4);
+  aq.lhG = g.lh;
+  col_out(aq, 0, K_G, k4, k4, (long)g.ny * k4);
+  launch_col<T>(c, Y0, nullptr, Y1, nullptr, nullptr, aq, 3L * nz, k4, t.G4);
+  T* Q3 = (T*)scratch(c, S_SDQ, 3 * n3 * sizeof(T));
+  StoreFields<T> SQ{{Q3, nullptr, nullptr, nullptr}, 3 * nz, g.ny, g.ld, g.nx};
+  launch_row_inv<T>(c, Y1, k4, k4, SQ, rq, g.nx, g.ld, g.ny);
+  double* part = (double*)scratch(c, S_RED, (size_t)SSD_NPART * nl * 4 * sizeof(double));
+  double* cpr = (double*)scratch(c, S_SDLEV, 2 * (size_t)nz * sizeof(double));
+  {
+    ProfScope _ps(c, "ssd_sums", (42.0 + 20 + 3) * (double)g.nx * g.ny * nl * sizeof(T));
+    hipLaunchKernelGGL((k_ssd_sums<T>), dim3(SSD_NPART, nl), dim3(256), 0, c->stream, F, (const T*)Fs,
+                       (const T*)Q3, (long)n3, g.nx, g.ld, g.ny, g.delta * g.delta, part);
+  }
+  LCHK(c, hipGetLastError());
+  hipLaunchKernelGGL((k_ssd_cs<T>), dim3((nl + 63) / 64), dim3(64), 0, c->stream, (const double*)part, SSD_NPART,
+                     nl, cpr, (T*)ss.Cs2_Pr);
+  LCHK(c, hipGetLastError());
+  {
+    const size_t n = pl * nl;
+    ProfScope _ps(c, "ssd_prt", 2.0 * n * sizeof(T));
+    hipLaunchKernelGGL((k_ssd_prt<T>), grid1(n), dim3(256), 0, c->stream, Cs_opt2, (const double*)cpr,
+                       (T*)ss.Pr_t, (unsigned)pl, (unsigned)n);
+  }
+  LCHK(c, hipGetLastError());
+}
+
 // the padded-product pass of a convective step alone, on the scratch buffers
 // the step uses (the autotuner times its variants): momentum and both scalars
 // (scal = 1) or momentum only
@@ -943,6 +1171,8 @@ void op_pad_products_probe(les_ctx* c, int scal) {
   template ForcingArgs<T> op_forcing_pre<T>(les_ctx*, const T*, const T*, const T*, const T*, const T*);          \
   template void op_lagrange_scalar<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*,             \
                                       const SgsPtrs&, const ScalarSgsPtrs&, bool);                               \
+  template void op_scalar_scaledep<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, T*, const T*,  \
+                                      const ScalarSgsPtrs&, bool);                                               \
   template void op_pad_products_probe<T>(les_ctx*, int);
 
 LES_SCAL_INST(double)

@@ -518,6 +518,21 @@ contains
     call pull(cs, nl)
   end subroutine les_h_plane_dynamic
 
+  ! new_scaledep_dynamic.f90:1-223 (model 6): the per-level Cs_opt2
+  subroutine les_h_new_scaledep_dynamic(S11, S12, S13, S22, S23, S33, u, v, w, cs, nz)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*), u(*), v(*), w(*)
+    real(c_double), target :: cs(*)
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb, nl
+    type(c_ptr) :: S(6)
+    nb = field_bytes(nz)
+    nl = 8_c_int64_t * nz
+    S = [pushed(S11, nb), pushed(S12, nb), pushed(S13, nb), pushed(S22, nb), pushed(S23, nb), pushed(S33, nb)]
+    call les_check(les_new_scaledep_dynamic(les_ctx_global, S, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+         dev(cs, nl)), 'new_scaledep_dynamic')
+    call pull(cs, nl)
+  end subroutine les_h_new_scaledep_dynamic
+
   ! ---------------------------------------------------------------- scalars
   ! obukhov, scalars_module.f90:508-690
   subroutine les_h_obukhov(u, v, theta, qmix, sgs_t3, sgs_q3, psi_m, psi_m0, psi_h, psi_h0, phi_m, phi_h, &
@@ -601,6 +616,23 @@ contains
     call sgs_pull(s)
     call ssgs_pull(ss)
   end subroutine les_h_lagrange_scalar_sdep
+
+  ! scalar_scaledep_dynamic.f90:1-307 (model 6): theta filtered in place,
+  ! ss%Pr_t on levels 1..nz-1, the per-level Cs2_Pr in ss%Cs2_Pr(1:nz-1)
+  subroutine les_h_scalar_scaledep_dynamic(S11, S12, S13, S22, S23, S33, u, v, w, theta, cs_opt2, ss, nz)
+    real(c_double), target :: S11(*), S12(*), S13(*), S22(*), S23(*), S33(*), u(*), v(*), w(*)
+    real(c_double), target :: theta(*), cs_opt2(*)
+    type(les_h_scalar_sgs), target, intent(inout) :: ss
+    integer, intent(in) :: nz
+    integer(c_int64_t) :: nb
+    type(c_ptr) :: S(6)
+    nb = field_bytes(nz)
+    S = [pushed(S11, nb), pushed(S12, nb), pushed(S13, nb), pushed(S22, nb), pushed(S23, nb), pushed(S33, nb)]
+    call les_check(les_scalar_scaledep_dynamic(les_ctx_global, S, pushed(u, nb), pushed(v, nb), pushed(w, nb), &
+         pushed(theta, nb), pushed(cs_opt2, nb), ssgs_push(ss)), 'scalar_scaledep_dynamic')
+    call pull(theta, nb)
+    call ssgs_pull(ss)
+  end subroutine les_h_scalar_scaledep_dynamic
 
   ! sgs_stag.f90:3-366 with model = 7
   subroutine les_h_sgs_stag_scalar(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, u, v, w, theta, &

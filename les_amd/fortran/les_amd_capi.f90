@@ -350,6 +350,17 @@ module les_amd_capi
       type(c_ptr), value :: w
       type(c_ptr), value :: Cs_opt2
     end function
+    integer(c_int) function les_new_scaledep_dynamic(c, S, u, v, w, Cs_opt2) bind(C, &
+        name='les_new_scaledep_dynamic')
+      import
+      implicit none
+      type(c_ptr), value :: c
+      type(c_ptr) :: S(*)
+      type(c_ptr), value :: u
+      type(c_ptr), value :: v
+      type(c_ptr), value :: w
+      type(c_ptr), value :: Cs_opt2
+    end function
     integer(c_int) function les_lagrange_ssim(c, S, u, v, w, s_) bind(C, name='les_lagrange_ssim')
       import
       implicit none
@@ -487,6 +498,19 @@ module les_amd_capi
       type(c_ptr), value :: w
       type(c_ptr), value :: theta
       type(les_sgs_fields), intent(in) :: m
+      type(les_scalar_sgs_fields), intent(in) :: s_
+    end function
+    integer(c_int) function les_scalar_scaledep_dynamic(c, S, u, v, w, theta, Cs_opt2, s_) bind(C, &
+        name='les_scalar_scaledep_dynamic')
+      import
+      implicit none
+      type(c_ptr), value :: c
+      type(c_ptr) :: S(*)
+      type(c_ptr), value :: u
+      type(c_ptr), value :: v
+      type(c_ptr), value :: w
+      type(c_ptr), value :: theta
+      type(c_ptr), value :: Cs_opt2
       type(les_scalar_sgs_fields), intent(in) :: s_
     end function
     integer(c_int) function les_sgs_stag_scalar(c, grad, u, v, w, theta, tau, m, s) bind(C, &

@@ -10,7 +10,7 @@
 ! then whole steps on the device state through les_step with every SGS model,
 ! the forcing and the convective step.  With a directory as its argument it
 ! writes each whole-step state there as a checkpoint record (model_<m>.out,
-! convective.out), which tests/test_fortran_gpu.py compares with the oracle.
+! convective.out) for tests/test_fortran_gpu.py to compare.
 ! Prints "LES_FORTRAN_OK" on success.
 program les_fortran_check
   use iso_c_binding
@@ -410,7 +410,7 @@ contains
     integer :: model
     character(len=1100) :: name
     allocate (uh(ld, ny, nz))
-    do model = 1, 5
+    do model = 1, 6
       call les_init_global(g)
       call les_check(les_set_sgs_model(les_ctx_global, model), 'set_sgs_model')
       if (model == 1) then

@@ -165,7 +165,7 @@ struct or_ctx {
   plan2 sm, bg;
   int flm_init, fqn_init;   /* SAVE flags, lagrange_Sdep.f90:67-68 */
   int jt, jt_total;         /* mirrors param jt/jt_total for routine calls */
-  int sgs_model;            /* param.f90 'model' of the whole step: 5 (default) or 3 */
+  int sgs_model;            /* param.f90 'model' of the whole step: 1..6 (5 default; 7 = 5 + scalar model 7) */
   or_scalar_params S;       /* scalar switches and constants (zero: no scalars) */
   double *wt_s, *wq_s;      /* (nx,ny) kinematic surface fluxes */
   int glm_init, gqn_init;   /* SAVE flags, lagrange_scalar_Sdep.f90:65-66 */
@@ -1098,7 +1098,7 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
   memcpy(txzp, txz, sizeof(double) * pl); memcpy(tyzp, tyz, sizeof(double) * pl);
   if (model == 4 || model == 5 || model == 7)                   /* sgs_stag.f90:59-69 */
     for (size_t i = 0; i < n3; i++) { s->u_lag[i] += u[i]; s->v_lag[i] += v[i]; s->w_lag[i] += w[i]; }
-  if (ss)
+  if (ss && model == 7)
     for (size_t i = 0; i < n3; i++) { ss->u_lag_s[i] += u[i]; ss->v_lag_s[i] += v[i]; ss->w_lag_s[i] += w[i]; }
   const int jt = c->jt;
   double *lz = (double *)malloc(sizeof(double) * nz);
@@ -1131,12 +1131,22 @@ static void sgs_stag_impl(or_ctx *c, const double *const g[9], const double *u,
       free(ziko);
     } else if (model == 4) {                                      /* :137-138 */
       or_lagrange_ssim(c, (const double *const *)S, u, v, w, s);
+    } else if (model == 6) {                                      /* :143-149 */
+      double *ziko = (double *)calloc(nz, sizeof(double));
+      or_new_scaledep_dynamic(c, (const double *const *)S, u, v, w, ziko);
+      for (int k = 0; k < nz; k++)
+        for (size_t i = 0; i < pl; i++) s->Cs_opt2[pl * k + i] = ziko[k];
+      free(ziko);
+      /* Cs2_Pr(nz-1) of the routine lands in the first nz-1 elements of
+         sgs_stag's (ld, ny, nz) Cs2_Pr (sequence association) */
+      if (ss) or_scalar_scaledep_dynamic(c, (const double *const *)S, u, v, w, theta, s->Cs_opt2, ss->psi_m,
+                                         ss->phi_h, ss->Cs2_Pr, ss->Pr_t);
     } else {
       or_lagrange_sdep(c, (const double *const *)S, u, v, w, s);
       if (ss) or_lagrange_scalar_sdep(c, (const double *const *)S, u, v, w, theta, s, ss);
     }
   }
-  if ((model == 5 || model == 7) && c->jt_total < 500) {       /* :165-174 (models 5, 6, 7) */
+  if ((model == 5 || model == 6 || model == 7) && c->jt_total < 500) {   /* :165-174 */
     const double cs0 = (double)(0.16f * 0.16f);
     for (size_t i = 0; i < n3; i++) s->Cs_opt2[i] = cs0;
     if (ss) for (size_t i = 0; i < n3; i++) ss->Pr_t[i] = c->S.Pr;
@@ -1591,6 +1601,8 @@ void or_step(or_ctx *c, or_state *s) {
   or_wallstress(c, s->u, s->v, g[2], g[5], s->tau[2], s->tau[4], s->psi_m, s->psi_m0, s->phi_m, s->ustar_avg);
   if (scal && c->S.model == 7)
     or_sgs_stag_scalar(c, (const double *const *)g, s->u, s->v, s->w, s->theta, s->tau, &s->sgs, &s->ssgs);
+  else if (scal && c->sgs_model == 6)  /* model = 6: scalar_scaledep_dynamic gives Pr_t (sgs_stag.f90:149) */
+    sgs_stag_impl(c, (const double *const *)g, s->u, s->v, s->w, s->theta, s->tau, &s->sgs, &s->ssgs, 6);
   else
     or_sgs_stag_model(c, (const double *const *)g, s->u, s->v, s->w, s->tau, &s->sgs, c->sgs_model);
   if (scal) {                          /* main.f90:273-276 */
@@ -1963,7 +1975,8 @@ static void theta_q_all_in_one(or_ctx *c, or_state *s, int which) {
   double *dx = which ? s->dqdx : s->dTdx, *dy = which ? s->dqdy : s->dTdy, *dzf = which ? s->dqdz : s->dTdz;
   double *R = which ? s->RHS_q : s->RHS_T, *Rf = which ? s->RHS_qf : s->RHS_Tf;
   double *Pr_ = (double *)malloc(sizeof(double) * n3);
-  for (size_t i = 0; i < n3; i++) Pr_[i] = c->S.model == 7 ? s->ssgs.Pr_t[i] : c->S.Pr;   /* :73-81 */
+  const int dyn_pr = c->S.model == 7 || c->sgs_model == 6;                      /* :73-81: models 6, 7 */
+  for (size_t i = 0; i < n3; i++) Pr_[i] = dyn_pr ? s->ssgs.Pr_t[i] : c->S.Pr;
   or_filt_da(c, sc, dx, dy, nz);                                                  /* :87-88 */
   or_ddz_uv(c, dzf, sc);
   const double top = lid_gradient(c, which);
@@ -2216,6 +2229,60 @@ double or_rtnewt(const double A[6]) {
   return 1.0;
 }
 
+/* The filtered fields of one level that scaledep_dynamic.f90:44-181 and
+   new_scaledep_dynamic.f90:52-181 share: the 2-Delta bars of the w-node
+   velocities and their 4-Delta hats, L_ij, Q_ij (4-Delta filter of products
+   of the 2-Delta bars), the 2- and 4-Delta filtered S_ij and |S| S_ij, and
+   |S| itself.  S[0..5] are the routine's S11..S33 dummies. */
+enum { UB, VB, WB, UH, VH, WH, L11, L12, L13, L22, L23, L33, Q11, Q12, Q13, Q22, Q23, Q33,
+       SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
+       SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
+       SMAG, SBAR, SHAT, NT };
+
+static void sd_level_fields(or_ctx *c, const double *const S[6], const double *u, const double *v,
+                            const double *w, int k, double *T[NT]) {
+  const size_t pl = (size_t)c->ld * c->ny, o = pl * k;
+  for (size_t i = 0; i < pl; i++) {                                      /* :44-73 */
+    double ub, vb, wb;
+    if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; }
+    else { ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i]; }
+    T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb;
+    T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb;
+    T[L11][i] = ub * ub; T[L12][i] = ub * vb; T[L13][i] = ub * wb;
+    T[L22][i] = vb * vb; T[L23][i] = vb * wb; T[L33][i] = wb * wb;
+  }
+  for (int q = 0; q < 3; q++) or_test_filter(c, T[UB + q], c->Gt);        /* :78-80 */
+  const int Lp[6][3] = { { L11, UB, UB }, { L12, UB, VB }, { L13, UB, WB }, { L22, VB, VB }, { L23, VB, WB }, { L33, WB, WB } };
+  for (int q = 0; q < 6; q++) {                                           /* :81-92 */
+    or_test_filter(c, T[Lp[q][0]], c->Gt);
+    for (size_t i = 0; i < pl; i++) T[Lp[q][0]][i] = T[Lp[q][0]][i] - T[Lp[q][1]][i] * T[Lp[q][2]][i];
+  }
+  for (size_t i = 0; i < pl; i++) {                                       /* :94-99 products of the 2-Delta bars */
+    T[Q11][i] = T[UB][i] * T[UB][i]; T[Q12][i] = T[UB][i] * T[VB][i]; T[Q13][i] = T[UB][i] * T[WB][i];
+    T[Q22][i] = T[VB][i] * T[VB][i]; T[Q23][i] = T[VB][i] * T[WB][i]; T[Q33][i] = T[WB][i] * T[WB][i];
+  }
+  for (int q = 0; q < 3; q++) or_test_filter(c, T[UH + q], c->Gtt);       /* :101-103 */
+  const int Qp[6][3] = { { Q11, UH, UH }, { Q12, UH, VH }, { Q13, UH, WH }, { Q22, VH, VH }, { Q23, VH, WH }, { Q33, WH, WH } };
+  for (int q = 0; q < 6; q++) {                                           /* :104-115 */
+    or_test_filter(c, T[Qp[q][0]], c->Gtt);
+    for (size_t i = 0; i < pl; i++) T[Qp[q][0]][i] = T[Qp[q][0]][i] - T[Qp[q][1]][i] * T[Qp[q][2]][i];
+  }
+  for (size_t i = 0; i < pl; i++) {                                       /* :117-139 */
+    double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+    double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+    double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+    T[SMAG][i] = Sm;
+    for (int q = 0; q < 6; q++) {
+      T[SB11 + q][i] = S[q][o + i]; T[SH11 + q][i] = S[q][o + i];
+      T[SSB11 + q][i] = Sm * S[q][o + i]; T[SSH11 + q][i] = Sm * S[q][o + i];
+    }
+  }
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);    /* :140-151 */
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[SSB11 + q], c->Gt);   /* :170-181 */
+  for (int q = 0; q < 6; q++) or_test_filter(c, T[SSH11 + q], c->Gtt);
+}
+
 /* ------------------------------------------------------------------ */
 /* scaledep_dynamic.f90:4-255 (non-MPI).  S[0..5] are the routine's dummy
    arguments S11, S12, S13, S22, S23, S33 in that order: the caller passes
@@ -2229,10 +2296,6 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
   const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
   const size_t pl = (size_t)ld * ny;
   const double delta = c->delta;
-  enum { UB, VB, WB, UH, VH, WH, L11, L12, L13, L22, L23, L33, Q11, Q12, Q13, Q22, Q23, Q33,
-         SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
-         SSB11, SSB12, SSB13, SSB22, SSB23, SSB33, SSH11, SSH12, SSH13, SSH22, SSH23, SSH33,
-         SMAG, SBAR, SHAT, NT };
   const double nxy = (double)(nx * ny);
 #pragma omp parallel
   {
@@ -2240,46 +2303,7 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
     for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
 #pragma omp for schedule(dynamic)
   for (int k = 0; k < nz; k++) {
-    const size_t o = pl * k;
-    for (size_t i = 0; i < pl; i++) {                                      /* :44-73 */
-      double ub, vb, wb;
-      if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; }
-      else { ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i]; }
-      T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb;
-      T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb;
-      T[L11][i] = ub * ub; T[L12][i] = ub * vb; T[L13][i] = ub * wb;
-      T[L22][i] = vb * vb; T[L23][i] = vb * wb; T[L33][i] = wb * wb;
-    }
-    for (int q = 0; q < 3; q++) or_test_filter(c, T[UB + q], c->Gt);        /* :78-80 */
-    const int Lp[6][3] = { { L11, UB, UB }, { L12, UB, VB }, { L13, UB, WB }, { L22, VB, VB }, { L23, VB, WB }, { L33, WB, WB } };
-    for (int q = 0; q < 6; q++) {                                           /* :81-92 */
-      or_test_filter(c, T[Lp[q][0]], c->Gt);
-      for (size_t i = 0; i < pl; i++) T[Lp[q][0]][i] = T[Lp[q][0]][i] - T[Lp[q][1]][i] * T[Lp[q][2]][i];
-    }
-    for (size_t i = 0; i < pl; i++) {                                       /* :94-99 products of the 2-Delta bars */
-      T[Q11][i] = T[UB][i] * T[UB][i]; T[Q12][i] = T[UB][i] * T[VB][i]; T[Q13][i] = T[UB][i] * T[WB][i];
-      T[Q22][i] = T[VB][i] * T[VB][i]; T[Q23][i] = T[VB][i] * T[WB][i]; T[Q33][i] = T[WB][i] * T[WB][i];
-    }
-    for (int q = 0; q < 3; q++) or_test_filter(c, T[UH + q], c->Gtt);       /* :101-103 */
-    const int Qp[6][3] = { { Q11, UH, UH }, { Q12, UH, VH }, { Q13, UH, WH }, { Q22, VH, VH }, { Q23, VH, WH }, { Q33, WH, WH } };
-    for (int q = 0; q < 6; q++) {                                           /* :104-115 */
-      or_test_filter(c, T[Qp[q][0]], c->Gtt);
-      for (size_t i = 0; i < pl; i++) T[Qp[q][0]][i] = T[Qp[q][0]][i] - T[Qp[q][1]][i] * T[Qp[q][2]][i];
-    }
-    for (size_t i = 0; i < pl; i++) {                                       /* :117-139 */
-      double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
-      double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
-      double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
-      T[SMAG][i] = Sm;
-      for (int q = 0; q < 6; q++) {
-        T[SB11 + q][i] = S[q][o + i]; T[SH11 + q][i] = S[q][o + i];
-        T[SSB11 + q][i] = Sm * S[q][o + i]; T[SSH11 + q][i] = Sm * S[q][o + i];
-      }
-    }
-    for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);    /* :140-151 */
-    for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
-    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSB11 + q], c->Gt);   /* :170-181 */
-    for (int q = 0; q < 6; q++) or_test_filter(c, T[SSH11 + q], c->Gtt);
+    sd_level_fields(c, S, u, v, w, k, T);
     double sa1 = 0, sb1 = 0, sc1 = 0, sd1 = 0, se1 = 0, sa2 = 0, sb2 = 0, sc2 = 0, sd2 = 0, se2 = 0;
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < ld; x++) {
@@ -2342,6 +2366,188 @@ void or_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
     for (int t = 0; t < NT; t++) free(T[t]);
   }
 
+}
+
+/* ------------------------------------------------------------------ */
+/* new_scaledep_dynamic.f90:1-223 (non-MPI; model = 6, called at
+   sgs_stag.f90:143-148 with calc_Sij's S11..S33).  Per level the plane sums
+   L:M, M:M, Q:N and N:N over the (ld, ny) arrays (compensated here), with
+   M_ij = 2 Delta^2 (bar(|S|S_ij) - 4 |S_bar| S_bar_ij) and
+   N_ij = 2 Delta^2 (hat(|S|S_ij) - 16 |S_hat| S_hat_ij); the two
+   coefficients LM/MM, QN/NN clipped at 1E-24 in single precision, Beta their
+   ratio clipped below at 1/8 in single precision, Cs_opt2 = max(0, LM/MM /
+   betaclip).  Output: Cs_opt2(nz).                                       */
+/* ------------------------------------------------------------------ */
+void or_new_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                             const double *v, const double *w, double *Cs_opt2) {
+  const int ld = c->ld, ny = c->ny, nz = c->nz;
+  const size_t pl = (size_t)ld * ny;
+  const double delta = c->delta;                                          /* :43 */
+  const double tf1 = 2.0, tf2 = 4.0, tf1_2 = tf1 * tf1, tf2_2 = tf2 * tf2; /* :44-47 */
+  const double cst = 2.0 * (delta * delta);                               /* :49 */
+#pragma omp parallel
+  {
+    double *T[NT];
+    for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
+  for (int k = 0; k < nz; k++) {
+    sd_level_fields(c, S, u, v, w, k, T);                                 /* :51-186 */
+    csum_t lm = { 0.0, 0.0 }, mm = { 0.0, 0.0 }, qn = { 0.0, 0.0 }, nn = { 0.0, 0.0 };
+    for (size_t i = 0; i < pl; i++) {
+      double sb[6], sh[6], L[6], Q[6], M[6], N[6];
+      for (int q = 0; q < 6; q++) {
+        sb[q] = T[SB11 + q][i]; sh[q] = T[SH11 + q][i]; L[q] = T[L11 + q][i]; Q[q] = T[Q11 + q][i];
+      }
+      /* q: 0=11 1=12 2=13 3=22 4=23 5=33 */
+      const double Sb = sqrt(2.0 * (sb[0] * sb[0] + sb[3] * sb[3] + sb[5] * sb[5] +
+                                    2.0 * (sb[1] * sb[1] + sb[2] * sb[2] + sb[4] * sb[4])));   /* :154-155 */
+      const double Sh = sqrt(2.0 * (sh[0] * sh[0] + sh[3] * sh[3] + sh[5] * sh[5] +
+                                    2.0 * (sh[1] * sh[1] + sh[2] * sh[2] + sh[4] * sh[4])));   /* :157-158 */
+      for (int q = 0; q < 6; q++) {                                       /* :188-200 */
+        M[q] = cst * (T[SSB11 + q][i] - (OR_MUTANT == 13 ? tf2_2 : tf1_2) * Sb * sb[q]);
+        N[q] = cst * (T[SSH11 + q][i] - (OR_MUTANT == 13 ? tf1_2 : tf2_2) * Sh * sh[q]);
+      }
+      csum_add(&lm, L[0] * M[0] + L[3] * M[3] + L[5] * M[5] + 2.0 * (L[1] * M[1] + L[2] * M[2] + L[4] * M[4]));
+      csum_add(&mm, M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]));
+      csum_add(&qn, Q[0] * N[0] + Q[3] * N[3] + Q[5] * N[5] + 2.0 * (Q[1] * N[1] + Q[2] * N[2] + Q[4] * N[4]));
+      csum_add(&nn, N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]));
+    }
+    double c2 = csum_val(&lm) / csum_val(&mm);                            /* :210-211 */
+    double c4 = csum_val(&qn) / csum_val(&nn);
+    c2 = (double)fmaxf(F1EM24, (float)c2);                                /* :213-214 */
+    c4 = (double)fmaxf(F1EM24, (float)c4);
+    const double beta = c4 / c2;                                          /* :216 */
+    const double betaclip = (double)fmaxf((float)beta, (float)(1.0 / 8.0));   /* :217 */
+    Cs_opt2[k] = fmax(0.0, c2 / betaclip);                                /* :219-220 */
+  }
+    for (int t = 0; t < NT; t++) free(T[t]);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* scalar_scaledep_dynamic.f90:1-307 (non-MPI, lbc = 1; model = 6 after
+   new_scaledep_dynamic, sgs_stag.f90:149).  theta is filtered in place by
+   its filt_da (:60) and read filtered afterwards; dTdz from ddz_uv with the
+   lid gradient (:61-64) and, on the first level (the whole (ld, ny) plane),
+   the uniform phi_h_avg * surface_flux / (ustar_ vonk dz/2) with ustar_ from
+   the mean first-level speed and psi_m (:73-79).  Per level 1..nz-1: L_i,
+   Q_i (4-Delta filter of products of the 2-Delta bars), M_i and N_i from
+   |S| dT_i and dT_i without staggering averages, const = Delta^2; the plane
+   sums give Cs2_Pr2, Cs2_Pr4, Beta and Cs2_Pr as in the momentum routine
+   (:227-240), and Pr_t(:,:,jz) = Cs_opt2(:,:,jz) / Cs2_Pr(jz) (:242).
+   Outputs Cs2_Pr(nz-1) and Pr_t levels 1..nz-1.                          */
+/* ------------------------------------------------------------------ */
+void or_scalar_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                                const double *v, const double *w, double *theta,
+                                const double *Cs_opt2, const double *psi_m, const double *phi_h,
+                                double *Cs2_Pr, double *Pr_t) {
+  const int nx = c->nx, ny = c->ny, ld = c->ld, nz = c->nz;
+  const size_t pl = (size_t)ld * ny, n3 = pl * nz;
+  const or_scalar_params *P = &c->S;
+  const double delta = c->delta, cst = delta * delta;                     /* :58, :219 */
+  double *dTdx = (double *)calloc(n3, sizeof(double)), *dTdy = (double *)calloc(n3, sizeof(double));
+  double *dTdz = (double *)calloc(n3, sizeof(double));
+  or_filt_da(c, theta, dTdx, dTdy, nz);                                   /* :60 */
+  or_ddz_uv(c, dTdz, theta);                                              /* :61 */
+  const double top = lid_gradient(c, 0);
+  for (size_t i = 0; i < pl; i++) dTdz[pl * (nz - 1) + i] = top;          /* :62-64 */
+  {                                                                       /* :73-79 */
+    double slog = 0.0, swt = 0.0;
+    for (int i = 0; i < nx * ny; i++) slog += log(c->P.z0);
+    const double fn = (double)(float)(nx * ny);
+    const double zo_avg = exp(slog / fn);
+    csum_t su = { 0.0, 0.0 }, spsi = { 0.0, 0.0 }, sphi = { 0.0, 0.0 };
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++)
+        csum_add(&su, sqrt(u[x + (size_t)ld * y] * u[x + (size_t)ld * y] + v[x + (size_t)ld * y] * v[x + (size_t)ld * y]));
+    for (int j = 0; j < nx * ny; j++) csum_add(&spsi, psi_m[j]);
+    for (int j = 0; j < nx * ny; j++) csum_add(&sphi, phi_h[j]);
+    for (int j = 0; j < nx * ny; j++) swt += c->wt_s[j];
+    const double ustar_ = csum_val(&su) / fn * VONK / (log(0.5 * c->dz / zo_avg) - csum_val(&spsi) / fn);
+    const double phi_h_avg = csum_val(&sphi) / fn;
+    const double sflux = (swt / (double)(nx * ny)) / P->T_scale / P->u_star;
+    const double d1 = phi_h_avg * sflux / (ustar_ * VONK * c->dz / 2.0);
+    for (size_t i = 0; i < pl; i++) dTdz[i] = d1;
+  }
+  enum { UB, VB, WB, TB, UH, VH, WH, TH, L1, L2, L3, Q1, Q2, Q3,
+         SB11, SB12, SB13, SB22, SB23, SB33, SH11, SH12, SH13, SH22, SH23, SH33,
+         SXB, SYB, SZB, SXH, SYH, SZH, DXB, DYB, DZB, DXH, DYH, DZH, NT };
+#pragma omp parallel
+  {
+    double *T[NT];
+    for (int t = 0; t < NT; t++) T[t] = (double *)calloc(pl, sizeof(double));
+#pragma omp for schedule(dynamic)
+  for (int k = 0; k < nz - 1; k++) {                                      /* :82-244 */
+    const size_t o = pl * k;
+    for (size_t i = 0; i < pl; i++) {                                     /* :84-105 */
+      double ub, vb, wb, tb;
+      if (k == 0) { ub = u[i]; vb = v[i]; wb = 0.25 * w[pl + i]; tb = theta[i]; }
+      else {
+        ub = 0.5 * (u[o + i] + u[o - pl + i]); vb = 0.5 * (v[o + i] + v[o - pl + i]); wb = w[o + i];
+        tb = 0.5 * (theta[o + i] + theta[o - pl + i]);
+      }
+      T[UB][i] = ub; T[VB][i] = vb; T[WB][i] = wb; T[TB][i] = tb;
+      T[UH][i] = ub; T[VH][i] = vb; T[WH][i] = wb; T[TH][i] = tb;
+      T[L1][i] = ub * tb; T[L2][i] = vb * tb; T[L3][i] = wb * tb;
+    }
+    for (int q = 0; q < 4; q++) or_test_filter(c, T[UB + q], c->Gt);     /* :111-114 */
+    for (int q = 0; q < 3; q++) {                                         /* :116-121 */
+      or_test_filter(c, T[L1 + q], c->Gt);
+      for (size_t i = 0; i < pl; i++) T[L1 + q][i] = T[L1 + q][i] - T[UB + q][i] * T[TB][i];
+    }
+    for (size_t i = 0; i < pl; i++)                                       /* :123-125 */
+      for (int q = 0; q < 3; q++) T[Q1 + q][i] = T[UB + q][i] * T[TB][i];
+    for (int q = 0; q < 4; q++) or_test_filter(c, T[UH + q], c->Gtt);    /* :127-130 */
+    for (int q = 0; q < 3; q++) {                                         /* :132-137 */
+      or_test_filter(c, T[Q1 + q], c->Gtt);
+      for (size_t i = 0; i < pl; i++) T[Q1 + q][i] = T[Q1 + q][i] - T[UH + q][i] * T[TH][i];
+    }
+    for (size_t i = 0; i < pl; i++) {                                     /* :140-200 */
+      const double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+      const double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+      const double Sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+      for (int q = 0; q < 6; q++) { T[SB11 + q][i] = S[q][o + i]; T[SH11 + q][i] = S[q][o + i]; }
+      const double dt3[3] = { dTdx[o + i], dTdy[o + i], dTdz[o + i] };
+      for (int q = 0; q < 3; q++) {
+        T[SXB + q][i] = Sm * dt3[q]; T[SXH + q][i] = Sm * dt3[q];
+        T[DXB + q][i] = dt3[q]; T[DXH + q][i] = dt3[q];
+      }
+    }
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SB11 + q], c->Gt);   /* :159-171 */
+    for (int q = 0; q < 6; q++) or_test_filter(c, T[SH11 + q], c->Gtt);
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[SXB + q], c->Gt);    /* :187-193 */
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[SXH + q], c->Gtt);
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[DXB + q], c->Gt);    /* :203-209 */
+    for (int q = 0; q < 3; q++) or_test_filter(c, T[DXH + q], c->Gtt);
+    csum_t lm = { 0.0, 0.0 }, mm = { 0.0, 0.0 }, qn = { 0.0, 0.0 }, nn = { 0.0, 0.0 };
+    for (size_t i = 0; i < pl; i++) {
+      const double sb11 = T[SB11][i], sb12 = T[SB12][i], sb13 = T[SB13][i], sb22 = T[SB22][i], sb23 = T[SB23][i], sb33 = T[SB33][i];
+      const double sh11 = T[SH11][i], sh12 = T[SH12][i], sh13 = T[SH13][i], sh22 = T[SH22][i], sh23 = T[SH23][i], sh33 = T[SH33][i];
+      const double Sb = sqrt(2.0 * (sb11 * sb11 + sb22 * sb22 + sb33 * sb33 + 2.0 * (sb12 * sb12 + sb13 * sb13 + sb23 * sb23)));  /* :173-174 */
+      const double Sh = sqrt(2.0 * (sh11 * sh11 + sh22 * sh22 + sh33 * sh33 + 2.0 * (sh12 * sh12 + sh13 * sh13 + sh23 * sh23)));  /* :176-177 */
+      double M[3], N[3];
+      for (int q = 0; q < 3; q++) {                                       /* :211-218 */
+        M[q] = cst * (T[SXB + q][i] - 4.0 * Sb * T[DXB + q][i]);
+        N[q] = cst * (T[SXH + q][i] - 16.0 * Sh * T[DXH + q][i]);
+      }
+      csum_add(&lm, T[L1][i] * M[0] + T[L2][i] * M[1] + T[L3][i] * M[2]);  /* :230-231 */
+      csum_add(&mm, M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+      csum_add(&qn, T[Q1][i] * N[0] + T[Q2][i] * N[1] + T[Q3][i] * N[2]);
+      csum_add(&nn, N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    }
+    double c2 = csum_val(&lm) / csum_val(&mm), c4 = csum_val(&qn) / csum_val(&nn);
+    c2 = (double)fmaxf(F1EM24, (float)c2);                                /* :233-234 */
+    c4 = (double)fmaxf(F1EM24, (float)c4);
+    const double beta = OR_MUTANT == 14 ? c2 / c4 : c4 / c2;             /* :236 */
+    const double betaclip = (double)fmaxf((float)beta, (float)(1.0 / 8.0));
+    double cpr = c2 / betaclip;                                           /* :239-240 */
+    cpr = fmax(0.0, cpr);
+    Cs2_Pr[k] = cpr;
+    for (size_t i = 0; i < pl; i++) Pr_t[o + i] = Cs_opt2[o + i] / cpr;   /* :242 */
+  }
+    for (int t = 0; t < NT; t++) free(T[t]);
+  }
+  free(dTdx); free(dTdy); free(dTdz);
 }
 
 /* ------------------------------------------------------------------ */

@@ -253,7 +253,18 @@ void or_std_dynamic(or_ctx *c, const double *const S[6], const double *u, const 
 void or_interpolag_ssim(or_ctx *c, or_sgs_state *s);
 void or_lagrange_ssim(or_ctx *c, const double *const S[6], const double *u,
                       const double *v, const double *w, or_sgs_state *s);
-/* sgs_stag.f90:3-366 with any model 1..5 (neutral) */
+/* new_scaledep_dynamic.f90:1-223 (model 6): Cs_opt2(nz) from the plane sums
+   of L:M, M:M, Q:N, N:N with single-precision clips; S = calc_Sij's S11..S33 */
+void or_new_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                             const double *v, const double *w, double *Cs_opt2);
+/* scalar_scaledep_dynamic.f90:1-307 (model 6 with scalars): filters theta in
+   place (its filt_da), writes Cs2_Pr(nz-1) and Pr_t (ld,ny,nz) levels
+   1..nz-1 = Cs_opt2 / Cs2_Pr; psi_m, phi_h: (nx,ny) */
+void or_scalar_scaledep_dynamic(or_ctx *c, const double *const S[6], const double *u,
+                                const double *v, const double *w, double *theta,
+                                const double *Cs_opt2, const double *psi_m, const double *phi_h,
+                                double *Cs2_Pr, double *Pr_t);
+/* sgs_stag.f90:3-366 with any model 1..6 (neutral) */
 void or_sgs_stag_model(or_ctx *c, const double *const grad[9], const double *u,
                        const double *v, const double *w, double *const tau[6],
                        or_sgs_state *s, int model);

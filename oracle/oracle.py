@@ -254,6 +254,24 @@ class Oracle:
         self.L.or_std_dynamic(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(cs))
         return cs
 
+    def new_scaledep_dynamic(self, S, u, v, w):
+        """new_scaledep_dynamic.f90 (model 6): Cs_opt2 per level."""
+        cs = np.zeros(self.nz)
+        s6 = (D * 6)(*[_p(a) for a in S])
+        self.L.or_new_scaledep_dynamic(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(cs))
+        return cs
+
+    def scalar_scaledep_dynamic(self, S, u, v, w, theta, Cs_opt2, psi_m, phi_h, Pr_t):
+        """scalar_scaledep_dynamic.f90 (model 6 with scalars): filters theta in
+        place, writes Pr_t levels 1..nz-1; returns Cs2_Pr per level (nz-1)."""
+        cpr = np.zeros(self.nz - 1)
+        s6 = (D * 6)(*[_p(a) for a in S])
+        pm = np.ascontiguousarray(psi_m, dtype=np.float64)
+        ph = np.ascontiguousarray(phi_h, dtype=np.float64)
+        self.L.or_scalar_scaledep_dynamic(C.c_void_p(self.ctx), s6, _p(u), _p(v), _p(w), _p(theta), _p(Cs_opt2),
+                                          _p(pm), _p(ph), _p(cpr), _p(Pr_t))
+        return cpr
+
     def interpolag_ssim(self, st):
         self.L.or_interpolag_ssim(C.c_void_p(self.ctx), C.byref(st))
 

@@ -39,10 +39,10 @@ def _record_fields(path, n3, nfields):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model", [1, 2, 3, 4, 5, "convective"])
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5, 6, "convective"])
 def test_fortran_whole_steps_vs_oracle(torch_mod, fortran_run, model):
     """The states the Fortran program reaches through les_step (32x32x9, seed
-    42, 6 steps with SGS model 1..5, model 1 with the Coriolis force and the
+    42, 6 steps with SGS model 1..6, model 1 with the Coriolis force and the
     sponge of damping_method 1; 5 convective steps with theta, q and a uniform
     surface heat flux) against the oracle run with the same set-up: u, v, w
     and RHSx..RHSz (and theta) to 1e-9 relative, the bound of the 10-step

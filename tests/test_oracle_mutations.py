@@ -31,6 +31,8 @@ MUTANTS = {
     10: ("step: sponge of RHSx at sponge(jz) only", PHYS + "test_sponge_terms[2]"),
     11: ("interpolag_Ssim: interpolag_Sdep's w_lag clamp", PHYS + "test_interpolag_ssim_vertical_shift"),
     12: ("model 1: wall-level length at z = dz", PHYS + "test_mason_length_wall_limit"),
+    13: ("new_scaledep_dynamic: 4 <-> 16 in M_ij / N_ij", PHYS + "test_new_scaledep_closed_form"),
+    14: ("scalar_scaledep_dynamic: Beta inverted", PHYS + "test_scalar_scaledep_closed_form"),
 }
 
 

@@ -311,3 +311,103 @@ def test_lagrange_ssim_limits(O):
         assert np.allclose(flm, ex, rtol=1e-9, atol=1e-12 * np.abs(ex).max())
         assert np.allclose(arr["Cs_opt2"][k, :, :nx], np.maximum(1e-32, flm / fmm), rtol=1e-14)
         assert np.all(arr["Cs_opt2"][k, :, nx:] == 1e-32)
+
+
+# ------------------------------------------------ model 6 (new_scaledep)
+F24 = float(np.float32(1e-24))
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _clipped(c2, c4):
+    """new_scaledep_dynamic.f90:213-220 / scalar_scaledep_dynamic.f90:233-240"""
+    c2, c4 = max(F24, _f32(c2)), max(F24, _f32(c4))
+    beta = c4 / c2
+    return max(0.0, c2 / max(_f32(beta), _f32(0.125))), beta
+
+
+def _strain(o, a):
+    """S11 = a cos x, S12 = a sin x, S22 = -S11: trace-free, |S| = 2a at every
+    point, and the single x mode passes both test filters unchanged."""
+    x = np.arange(o.nx) * o.dx
+    S = [o.field() for _ in range(6)]
+    for k in range(o.nz):
+        S[0][k, :, :o.nx] = a * np.cos(x)[None, :]
+        S[1][k, :, :o.nx] = a * np.sin(x)[None, :]
+        S[3][k, :, :o.nx] = -a * np.cos(x)[None, :]
+    return S
+
+
+def _xfield(o, modes):
+    """sum of amp sin(m x) on every level"""
+    x = np.arange(o.nx) * o.dx
+    f = o.field()
+    f[:, :, :o.nx] = sum(amp * np.sin(m * x) for m, amp in modes)[None, None, :]
+    return f
+
+
+def test_new_scaledep_closed_form(O):
+    """Model 6 on 32^2 (2-Delta filter keeps kx < 8, 4-Delta kx < 4) with the
+    constant-norm strain and u = A sin 10x + C sin 11x + D sin 5x + E sin 6x,
+    v = w = 0.  The 10/11 pair lies beyond both filters, the 5/6 pair between
+    them, so the only cos x parts are L11 = ... + AC cos x and Q11 = ... + DE
+    cos x; with M_ij = 2 Delta^2 (2a - 4 2a) S_ij and N_ij = 2 Delta^2 (2a -
+    16 2a) S_ij the plane sums are LM = -6 a^2 Delta^2 AC nxy, MM = 288 a^4
+    Delta^4 nxy, QN = -30 a^2 Delta^2 DE nxy, NN = 7200 a^4 Delta^4 nxy:
+    Cs2_2d = -AC / (48 a^2 Delta^2), Cs2_4d = -DE / (240 a^2 Delta^2), Beta =
+    DE / (5 AC) = 0.5, and Cs_opt2 = Cs2_2d / Beta on every level."""
+    o = O.Oracle(32, 32, 6)
+    a, A, C, D, E = 1.0, 0.3, -0.2, 0.25, -0.6
+    S = _strain(o, a)
+    u = _xfield(o, [(10, A), (11, C), (5, D), (6, E)])
+    v, w = o.field(), o.field()
+    cs = o.new_scaledep_dynamic(S, u, v, w)
+    d2 = o.dx * o.dy
+    want, beta = _clipped(-A * C / (48 * a * a * d2), -D * E / (240 * a * a * d2))
+    assert abs(beta - 0.5) < 1e-9
+    print("Cs_opt2", cs, "closed form", want)
+    assert np.allclose(cs, want, rtol=2.5e-7, atol=0)
+
+
+def test_scalar_scaledep_closed_form(O):
+    """scalar_scaledep_dynamic on the same strain, zero surface flux (so dT/dz
+    = 0 on the first level), u = A sin 11x + D sin 5x, v = w = 0 and theta =
+    T0 + Gamma z + b sin x + g sin 12x + h sin 6x.  Only the pair (11, 12)
+    puts a cos x part into L1 = Ag/2 cos x (the (5, 6) pair's passes both
+    filtered sides and cancels); Q1 = Dh/2 cos x (the 2-Delta bars' (5, 6)
+    product through the 4-Delta filter).  dT/dx bar = b cos x + 6h cos 6x,
+    dT/dx hat = b cos x, dT/dz = Gamma above the first level, so with const
+    = Delta^2: M1 = -6 a Delta^2 (b cos x + 6h cos 6x), M3 = -6 a Gamma
+    Delta^2, N1 = -30 a b Delta^2 cos x, N3 = -30 a Gamma Delta^2, and per
+    level LM = -1.5 A g a b Delta^2 nxy, MM = 36 a^2 Delta^4 ((b^2 + 36 h^2)/2
+    + Gamma^2) nxy, QN = -7.5 D h a b Delta^2 nxy, NN = 900 a^2 Delta^4 (b^2/2
+    + Gamma^2) nxy (Gamma -> 0 on the first level); Cs2_Pr from the clipped
+    ratios and Pr_t = Cs_opt2 / Cs2_Pr."""
+    o = O.Oracle(32, 32, 6)
+    o.set_scalars(O.scalar_params(), np.zeros((o.ny, o.nx)), None)
+    a, A, D = 1.0, 0.3, 0.25
+    b, g, h, G, T0 = 0.5, -0.2, -0.4, 0.3, 0.7
+    S = _strain(o, a)
+    u = _xfield(o, [(11, A), (5, D)])
+    v, w = o.field(), o.field()
+    th = _xfield(o, [(1, b), (12, g), (6, h)])
+    for k in range(o.nz):
+        th[k, :, :o.nx] += T0 + G * (k + 0.5) * o.dz
+    cs = o.field()
+    for k in range(o.nz):
+        cs[k] = 0.02 + 0.001 * k
+    pr = o.field()
+    cpr = o.scalar_scaledep_dynamic(S, u, v, w, th, cs, np.zeros((o.ny, o.nx)), np.ones((o.ny, o.nx)), pr)
+    d2 = o.dx * o.dy
+    for k in range(o.nz - 1):
+        Gk = 0.0 if k == 0 else G
+        mm = 36 * a * a * d2 * d2 * ((b * b + 36 * h * h) / 2 + Gk * Gk)
+        nn = 900 * a * a * d2 * d2 * (b * b / 2 + Gk * Gk)
+        want, beta = _clipped(-1.5 * A * g * a * b * d2 / mm, -7.5 * D * h * a * b * d2 / nn)
+        assert beta > 1.5                     # unclipped and far from 1: an inverted ratio cannot pass
+        print(k, "Cs2_Pr", cpr[k], "closed form", want)
+        assert abs(cpr[k] - want) <= 2.5e-7 * want
+        assert np.allclose(pr[k, :, :], cs[k] / cpr[k], rtol=1e-15)
+    assert np.all(pr[o.nz - 1] == 0.0)        # the lid level is not written
