@@ -215,20 +215,42 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024)
     return z;
   };
   if constexpr (N > 0) {
-    // every row load of the block is issued before the first LDS store
-    constexpr int NE = RG::P * (N > 0 ? N : 1), BL = RG::BLOCK, MAXE = (NE + BL - 1) / BL;
-    cx<T> z[MAXE];
+    // every row load of the block is issued before the first LDS store; one
+    // item is coefficient j < N/2 of both rows of a pair, loaded once and
+    // stored as element j and as its conjugate partner N - j (kxi <= N/2:
+    // the Nyquist element N/2 is zero and goes with item 0)
+    constexpr int H = (N > 0 ? N : 2) / 2, NI = RG::P * H, BL = RG::BLOCK, MAXI = (NI + BL - 1) / BL;
+    cx<T> za[MAXI], zb[MAXI];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
+    for (int i = 0; i < MAXI; ++i) {
       const int e = threadIdx.x + i * BL;
-      const int t = e / N, j = e - t * N;
-      z[i] = (NE % BL == 0 || e < NE) ? element(t, j) : cx<T>{T(0), T(0)};
+      const int t = e / H, j = e - t * H, pp = pair0 + t;
+      za[i] = zb[i] = cx<T>{T(0), T(0)};
+      if ((NI % BL == 0 || e < NI) && pp < npp && j < kxi) {
+        const unsigned a = (unsigned)(2 * pp) * (unsigned)irow, b = a + (unsigned)irow;
+        za[i] = inp[a + (unsigned)j];
+        zb[i] = inp[b + (unsigned)j];
+        if (in2) {
+          const cx<T> wa = in2p[a + (unsigned)j], wb = in2p[b + (unsigned)j];
+          za[i] = {za[i].x + c2 * wa.x, za[i].y + c2 * wa.y};
+          zb[i] = {zb[i].x + c2 * wb.x, zb[i].y + c2 * wb.y};
+        }
+      }
     }
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
+    for (int i = 0; i < MAXI; ++i) {
       const int e = threadIdx.x + i * BL;
-      const int t = e / N, j = e - t * N;
-      if (NE % BL == 0 || e < NE) buf[ix(t, j)] = z[i];
+      const int t = e / H, j = e - t * H;
+      if (NI % BL == 0 || e < NI) {
+        const cx<T> A = za[i], B = zb[i];
+        if (j == 0) {
+          buf[ix(t, 0)] = {A.x, B.x};  // Im of the DC entries is ignored (halfcomplex c2r)
+          buf[ix(t, H)] = {T(0), T(0)};
+        } else {
+          buf[ix(t, j)] = {A.x - B.y, A.y + B.x};
+          buf[ix(t, N - j)] = {A.x + B.y, B.x - A.y};
+        }
+      }
     }
   } else {
     for (int e = threadIdx.x; e < P * n; e += BLOCK) {
@@ -453,6 +475,10 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
   }
   if (ldo < n) {  // the output rows hold the n physical values (plus pads)
     c->err = "row inverse: output row shorter than the transform";
+    return;
+  }
+  if (2 * kxi > n) {  // the Nyquist coefficient is never stored
+    c->err = "row inverse: more input coefficients than n/2";
     return;
   }
   const long blocks = (ny / 2 + P - 1) / P;
