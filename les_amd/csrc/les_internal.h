@@ -88,7 +88,7 @@ struct Tune {
 // the lbc=1 surface of bottombc.f90:26-63.
 struct ScalarParams {
   int enable = 0;   // S_FLAG: theta and q transported
-  int model = 5;    // 7: Pr_ = Pr_t from lagrange_scalar_Sdep; else constant Pr
+  int model = 5;    // 7: Pr_ = Pr_t from lagrange_scalar_Sdep, 6: from scalar_scaledep_dynamic; else Pr
   int initsc = 1;   // scalar history given: no Euler scalar step at jt == 1
   int passive = 0;  // passive_scalar: no buoyancy in RHSz
   double Pr = 0.55, z_i = 1000.0, u_star = 1.0, T_scale = 400.0, q_scale = 100.0, g = 9.81;
@@ -166,7 +166,8 @@ struct les_ctx {
   int glm_init = 0, gqn_init = 0;  // lagrange_scalar_Sdep.f90:65-66 SAVE flags
   int sgs_model = 5;               // param.f90 'model': 1 Smagorinsky-Mason, 2 standard dynamic,
                                    // 3 plane-averaged scale-dependent, 4 Lagrangian scale-invariant,
-                                   // 5 Lagrangian scale-dependent
+                                   // 5 Lagrangian scale-dependent, 6 plane-averaged scale-dependent
+                                   // without the quintic (new_scaledep_dynamic)
   double* l2z = nullptr;           // model 1: l(jz)^2 of sgs_stag.f90:84-94, device (nz)
   les::Forcing frc;
   les::Tune tune;
@@ -255,6 +256,9 @@ struct ScalarSgsPtrs {
 template <typename T> void op_calc_sij(les_ctx* c, const T* const g[9], T* const S[6]);
 template <typename T> void op_interpolag(les_ctx* c, const SgsPtrs& s);
 template <typename T> void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl);
+// the interpolation of F into scratch (returned, 4 fields) for an epilogue
+// that reads it and writes F
+template <typename T> T* op_interpolag4_into(les_ctx* c, const T* const F[4], T* ul, T* vl, T* wl);
 // interpolag_Ssim.f90 on F_LM, F_MM (model 4)
 template <typename T> void op_interpolag_ssim(les_ctx* c, const SgsPtrs& s);
 // lagrange_Ssim.f90:4-300 (model 4)

@@ -659,10 +659,22 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
 // 0.25 w_lag(2) value) where interpolag_Sdep.f90:264-269 clamps both to
 // the clamped level-2 value.
 // ------------------------------------------------------------------------
+template <typename T>
+struct In4 {
+  const T* p[4];
+};
+template <typename T>
+struct Out4 {
+  T* p[4];
+};
+
+// in: the fields before the step (never the arrays written); out: the
+// interpolated fields on the nx interior columns and, with pad_copy, the
+// input's pad columns as they are (the reference leaves them untouched)
 template <typename T, int NF = 4, bool SIM = false>
-__global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl,
-                             const T* __restrict__ FF, T* FLM, T* FMM, T* FQN, T* FNN, int nx, int ld, int ny,
-                             int nz, T cs, T dt, T dx, T dy, T dz) {
+__global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl, In4<T> in,
+                             Out4<T> out, int pad_copy, int nx, int ld, int ny, int nz, T cs, T dt, T dx, T dy,
+                             T dz) {
   const long pl = (long)ld * ny, n3 = pl * nz;
   const long n = (long)nx * ny * nz;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
@@ -708,14 +720,23 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
     const long i0y0 = (long)k * pl + (long)ya * ld + x, i0yz = (long)za * pl + (long)ya * ld + x;
     const long ix00 = (long)k * pl + (long)y * ld + xa, ix0z = (long)za * pl + (long)y * ld + xa;
     const long ixy0 = (long)k * pl + (long)ya * ld + xa, ixyz = (long)za * pl + (long)ya * ld + xa;
-    T* outs[4] = {FLM, FMM, FQN, FNN};
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const T* G = FF + f * n3;
-      outs[f][i] = fx * fy * (G[i000] * fz + G[i00z] * czv) + fx * cyv * (G[i0y0] * fz + G[i0yz] * czv) +
-                   cxv * fy * (G[ix00] * fz + G[ix0z] * czv) + cxv * cyv * (G[ixy0] * fz + G[ixyz] * czv);
+      const T* G = in.p[f];
+      out.p[f][i] = fx * fy * (G[i000] * fz + G[i00z] * czv) + fx * cyv * (G[i0y0] * fz + G[i0yz] * czv) +
+                    cxv * fy * (G[ix00] * fz + G[ix0z] * czv) + cxv * cyv * (G[ixy0] * fz + G[ixyz] * czv);
     }
   }
+  if (pad_copy) {
+    const unsigned npad = (unsigned)(ld - nx), m = npad * (unsigned)ny * (unsigned)nz;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+      const unsigned row = q / npad, xo = q - row * npad;  // row = k ny + y
+      const long i = (long)row * ld + nx + xo;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) out.p[f][i] = in.p[f][i];
+    }
+  }
+  (void)n3;
 }
 
 // interpolag_Sdep.f90:2-380 on (F_LM, F_MM, F_QN, F_NN) with u_lag; the same
@@ -728,14 +749,39 @@ void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl) {
   for (int q = 0; q < 4; ++q) op_copy<T>(c, FF + q * n3, F[q], n3);
   {
     ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
+    const In4<T> in{{FF, FF + n3, FF + 2 * n3, FF + 3 * n3}};
+    const Out4<T> out{{F[0], F[1], F[2], F[3]}};
     hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)ul, (const T*)vl, (const T*)wl, FF, F[0], F[1], F[2], F[3], g.nx, g.ld, g.ny, g.nz,
-                       T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+                       (const T*)ul, (const T*)vl, (const T*)wl, in, out, 0, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)),
+                       T(g.dt), T(g.dx), T(g.dy), T(g.dz));
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, ul, n3, T(0));
   op_fill<T>(c, vl, n3, T(0));
   op_fill<T>(c, wl, n3, T(0));
+}
+
+// the same interpolation from the fields F into four fields of scratch (pads
+// copied), for an update whose epilogue reads the interpolated values and
+// writes F: no copy of F beforehand.  Returns the scratch fields (4 n3).
+template <typename T>
+T* op_interpolag4_into(les_ctx* c, const T* const F[4], T* ul, T* vl, T* wl) {
+  const Grid& g = c->g;
+  const size_t n3 = (size_t)g.ld * g.ny * g.nz;
+  T* Fo = (T*)scratch(c, S_FF, 4 * n3 * sizeof(T));
+  {
+    ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
+    const In4<T> in{{F[0], F[1], F[2], F[3]}};
+    const Out4<T> out{{Fo, Fo + n3, Fo + 2 * n3, Fo + 3 * n3}};
+    hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
+                       (const T*)ul, (const T*)vl, (const T*)wl, in, out, 1, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)),
+                       T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+  }
+  LCHK(c, hipGetLastError());
+  op_fill<T>(c, ul, n3, T(0));
+  op_fill<T>(c, vl, n3, T(0));
+  op_fill<T>(c, wl, n3, T(0));
+  return Fo;
 }
 
 template <typename T>
@@ -756,10 +802,11 @@ void op_interpolag_ssim(les_ctx* c, const SgsPtrs& s) {
   op_copy<T>(c, FF + n3, (T*)s.F_MM, n3);
   {
     ProfScope _ps(c, "interpolag", (2.0 + 3 + 2) * n3 * sizeof(T));
+    const In4<T> in{{FF, FF + n3, nullptr, nullptr}};
+    const Out4<T> out{{(T*)s.F_LM, (T*)s.F_MM, nullptr, nullptr}};
     hipLaunchKernelGGL((k_interpolag<T, 2, true>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, FF, (T*)s.F_LM, (T*)s.F_MM,
-                       (T*)nullptr, (T*)nullptr, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy),
-                       T(g.dz));
+                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, in, out, 0, g.nx, g.ld, g.ny, g.nz,
+                       T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, (T*)s.u_lag, n3, T(0));
@@ -833,10 +880,11 @@ struct LoadLagAll {
 template <typename T>
 __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F4, const T* __restrict__ S11,
                                const T* __restrict__ S12, const T* __restrict__ S13, const T* __restrict__ S22,
-                               const T* __restrict__ S23, const T* __restrict__ S33, T* FLM, T* FMM, T* FQN, T* FNN,
-                               T* TnLM, T* TnQN, T* Beta, T* Cs, T* L11t, T* L22t, T* L33t, T* Q11t, T* Q22t,
-                               T* Q33t, double* red, int nx, int ld, int ny, int nz, double cst, double opftdelta,
-                               double lagran_dt, double expo, int init_lm, int init_qn) {
+                               const T* __restrict__ S23, const T* __restrict__ S33, const T* __restrict__ Fi,
+                               T* FLM, T* FMM, T* FQN, T* FNN, T* TnLM, T* TnQN, T* Beta, T* Cs, T* L11t, T* L22t,
+                               T* L33t, T* Q11t, T* Q22t, T* Q33t, double* red, int nx, int ld, int ny, int nz,
+                               double cst, double opftdelta, double lagran_dt, double expo, int init_lm,
+                               int init_qn) {
   const long pl = (long)ld * ny, n3 = pl * nz;
   const int n2 = ld * ny;  // pad columns too: the reference loops over ld there
   const double e24 = (double)1e-24f;
@@ -873,7 +921,8 @@ __global__ void k_lag_epilogue(const T* __restrict__ F2, const T* __restrict__ F
     const double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
     const double QN = Qv[0] * N[0] + Qv[3] * N[3] + Qv[5] * N[5] + 2.0 * (Qv[1] * N[1] + Qv[2] * N[2] + Qv[4] * N[4]);
     const double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
-    double flm = FLM[i], fmm = FMM[i], fqn = FQN[i], fnn = FNN[i];
+    // the interpolated F (interpolag_Sdep's output, four fields of Fi)
+    double flm = Fi[i], fmm = Fi[n3 + i], fqn = Fi[2 * n3 + i], fnn = Fi[3 * n3 + i];
     if (init_lm) { fmm = pad ? 1.0 : MM; flm = pad ? 1.0 : 0.03 * MM; }
     double Tn = fmax(fq(flm * fmm), e24);
     TnLM[i] = (T)Tn;
@@ -978,7 +1027,12 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   const Grid& g = c->g;
   const int nz = g.nz;
   c->last_lag_jt = c->jt;
-  op_interpolag<T>(c, s);
+  const T* Fi;
+  {  // interpolag_Sdep (lagrange_Sdep.f90:128) into scratch; the epilogue writes F
+    OpTag _t2(c, "interpolag");
+    const T* F4s[4] = {(const T*)s.F_LM, (const T*)s.F_MM, (const T*)s.F_QN, (const T*)s.F_NN};
+    Fi = op_interpolag4_into<T>(c, F4s, (T*)s.u_lag, (T*)s.v_lag, (T*)s.w_lag);
+  }
   T* F = lag_filter_mom<T>(c, S, u, v, w);
   const size_t n3 = (size_t)g.ld * g.ny * nz;
   const unsigned bpl = (unsigned)((g.ld * g.ny + 255) / 256 < 64 ? (g.ld * g.ny + 255) / 256 : 64);
@@ -989,7 +1043,7 @@ void op_lagrange(les_ctx* c, const T* const S[6], const T* u, const T* v, const 
   {
     ProfScope _ps(c, "lag_epilogue", (42.0 + 4 + 14) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_lag_epilogue<T>), dim3(bpl, nz), dim3(256), 0, c->stream, F, F + 21 * n3,
-                       S[0], S[1], S[2], S[3], S[4], S[5], (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
+                       S[0], S[1], S[2], S[3], S[4], S[5], Fi, (T*)s.F_LM, (T*)s.F_MM, (T*)s.F_QN, (T*)s.F_NN,
                        (T*)s.TnLMMM, (T*)s.TnQNNN, (T*)s.Beta, (T*)s.Cs_opt2, (T*)s.L11t, (T*)s.L22t, (T*)s.L33t,
                        (T*)s.Q11t, (T*)s.Q22t, (T*)s.Q33t, red, g.nx, g.ld, g.ny, nz, 2.0 * (delta * delta),
                        1.5 * delta, g.dt * (double)lag_count(c), log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
@@ -1974,6 +2028,7 @@ void op_timestep_conditions(les_ctx* c, const T* u, const T* v, const T* w, cons
   template void op_calc_sij<T>(les_ctx*, const T* const[9], T* const[6]);                                        \
   template void op_interpolag<T>(les_ctx*, const SgsPtrs&);                                                      \
   template void op_interpolag4<T>(les_ctx*, T* const[4], T*, T*, T*);                                            \
+  template T* op_interpolag4_into<T>(les_ctx*, const T* const[4], T*, T*, T*);                                   \
   template T* lag_filter_mom<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, int);                 \
   template void launch_tridiag<T>(les_ctx*, const cx<T>*, const cx<T>*, cx<T>*, double*, cx<double>*, int*);   \
   template void op_lagrange<T>(les_ctx*, const T* const[6], const T*, const T*, const T*, const SgsPtrs&);       \

@@ -777,9 +777,10 @@ struct LoadLagScalarAll {
 
 template <typename T>
 __global__ void k_lag_scalar_epilogue(const T* __restrict__ F, const T* __restrict__ Fs, const T* __restrict__ TnLM,
-                                      const T* __restrict__ TnQN, const T* __restrict__ Cs, T* GLM, T* GMM, T* GQN,
-                                      T* GNN, T* Beta, T* Cs2_Pr, T* Pr_t, int nx, int ld, int ny, int nz, double cst,
-                                      double opftdelta, double lagran_dt, double expo, int init_lm, int init_qn) {
+                                      const T* __restrict__ TnQN, const T* __restrict__ Cs, const T* __restrict__ Gi,
+                                      T* GLM, T* GMM, T* GQN, T* GNN, T* Beta, T* Cs2_Pr, T* Pr_t, int nx, int ld,
+                                      int ny, int nz, double cst, double opftdelta, double lagran_dt, double expo,
+                                      int init_lm, int init_qn) {
   const long pl = (long)ld * ny, n3 = pl * nz;
   const double e24 = (double)1e-24f;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
@@ -811,7 +812,8 @@ __global__ void k_lag_scalar_epilogue(const T* __restrict__ F, const T* __restri
     const double MM = M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
     const double QN = Q1 * N[0] + Q2 * N[1] + Q3 * N[2];
     const double NN = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
-    double glm = GLM[i], gmm = GMM[i], gqn = GQN[i], gnn = GNN[i];
+    // the interpolated G (interpolag_scalar_Sdep's output, four fields of Gi)
+    double glm = Gi[i], gmm = Gi[n3 + i], gqn = Gi[2 * n3 + i], gnn = Gi[3 * n3 + i];
     if (init_lm) { gmm = pad ? 1.0 : MM; glm = pad ? 1.0 : 0.03 * MM; }  // :289-299
     double Tn = (double)TnLM[i];                                          // :318-331
     Tn = opftdelta * pow(Tn, -0.125);
@@ -851,10 +853,11 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   const ScalarParams& P = c->scal;
   const int nz = g.nz;
   const size_t n3 = (size_t)g.ld * g.ny * nz;
-  {  // interpolag_scalar_Sdep (:88)
+  const T* Gi;
+  {  // interpolag_scalar_Sdep (:88) into scratch; the epilogue writes G
     OpTag _t2(c, "interpolag_scalar");
-    T* G[4] = {(T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN, (T*)ss.G_NN};
-    op_interpolag4<T>(c, G, (T*)ss.u_lag_s, (T*)ss.v_lag_s, (T*)ss.w_lag_s);
+    const T* G[4] = {(const T*)ss.G_LM, (const T*)ss.G_MM, (const T*)ss.G_QN, (const T*)ss.G_NN};
+    Gi = op_interpolag4_into<T>(c, G, (T*)ss.u_lag_s, (T*)ss.v_lag_s, (T*)ss.w_lag_s);
   }
   // theta_temp = theta (:84-86), then filt_da(theta) in place and ddz_uv (:95-99)
   T* tht = (T*)scratch(c, S_SC0, n3 * sizeof(T));
@@ -909,7 +912,8 @@ void op_lagrange_scalar(les_ctx* c, const T* const S[6], const T* u, const T* v,
   {
     ProfScope _ps(c, "lag_scalar_epilogue", (6.0 + 20 + 2 + 1 + 4 + 4 + 3) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_lag_scalar_epilogue<T>), grid1(n3), dim3(256), 0, c->stream, F, Fs, (const T*)m.TnLMMM,
-                       (const T*)m.TnQNNN, (const T*)m.Cs_opt2, (T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN, (T*)ss.G_NN,
+                       (const T*)m.TnQNNN, (const T*)m.Cs_opt2, Gi, (T*)ss.G_LM, (T*)ss.G_MM, (T*)ss.G_QN,
+                       (T*)ss.G_NN,
                        (T*)m.Beta, (T*)ss.Cs2_Pr, (T*)ss.Pr_t, g.nx, g.ld, g.ny, nz, delta * delta, 1.5 * delta,
                        g.dt * (double)lag_count(c), log(2.0) / (log(4.0) - log(2.0)), init_lm, init_qn);
   }
