@@ -170,6 +170,6 @@ def test_model_selection_errors(torch_mod):
     from les_amd.api import Les, LesError, LesSim
     g = Les(16, 16, 5)
     with pytest.raises(LesError):
-        g.set_sgs_model(6)
+        g.set_sgs_model(7)   # model 7 is SGS model 5 with scalar model 7
     with pytest.raises(LesError):   # the scalar Lagrangian model (scalar model 7) needs SGS model 5
         LesSim(16, 16, 5, sgs_model=3, scalars=dict(wt_s=np.full((16, 16), 0.05)))
