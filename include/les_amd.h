@@ -402,8 +402,10 @@ int les_profile_report(les_ctx *c, char *buf, int buflen);
    LDS, 2 both in LDS -- and modes per block, 3: parallel cyclic reduction,
    one wavefront per mode, or 4: the sweep from both ends, four threads per
    mode, at most 256 modes per block) and the padded-product pass of the
-   convective step (what & 4; groups of levels per workgroup, or streaming
-   runs of levels with the radix-8 plan) on this context's state, and keeps
+   convective step (what & 4; 0: groups of levels per workgroup, 1:
+   streaming runs of levels with the radix-8 plan, 2: streaming runs with one
+   transform per wavefront, for the convective step with both scalars at
+   nx2 = 384, elsewhere the same as 1) on this context's state, and keeps
    the fastest.  Widths and Thomas schedules give identical results; the
    plans, the streaming products, cyclic reduction and the two-ended sweep
    change only the rounding.  report: JSON with the measured ms per

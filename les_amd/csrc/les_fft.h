@@ -462,6 +462,94 @@ __device__ __forceinline__ void fft_lds_c(cx<T>* buf, const TW& tw, int sign) {
   fft_stages_c<T, N, 0, 1, ES, TS, NTR, BLOCK, LANE_T, PAD, PLAN>(buf, tw, sign);
 }
 
+// ---- one transform per wavefront ----
+// A wavefront's LDS accesses execute in program order, so a transform that
+// one wavefront owns needs no workgroup barrier between its passes, only a
+// fence that keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham stage (radix R after NS points) of one padded N-point row,
+// run by the 64 lanes of one wavefront (lane b owns butterflies b, b + 64 ..).
+// The same arithmetic as fft_stage_c, so the results are identical.
+template <typename T, int N, int R, int NS, int OFF, class TW>
+__device__ __forceinline__ void wave_stage(cx<T>* row, const TW& tw, int lane, int sign) {
+  constexpr int NB = N / R, MAXB = (NB + 63) / 64;
+  constexpr bool RD_C = pad_offsets_const(N, R, NS, true);
+  constexpr bool WR_C = pad_offsets_const(N, R, NS, false);
+  auto at_rd = [](int e0, int d) { return RD_C ? lds_pad(e0) + d + (d >> 4) : lds_pad(e0 + d); };
+  auto at_wr = [](int e0, int d) { return WR_C ? lds_pad(e0) + d + (d >> 4) : lds_pad(e0 + d); };
+  cx<T> v[MAXB][R];
+  int dd[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = lane + 64 * i;
+    dd[i] = 0;
+    if (NB % 64 == 0 || b < NB) {
+      const int k = b % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> a = row[at_rd(b, r * NB)];
+        if constexpr (NS > 1) {
+          if (r > 0) {
+            cx<T> wv = tw_at<OFF, NS>(tw, r, k);
+            if (sign > 0) wv.y = -wv.y;
+            a = cmul(a, wv);
+          }
+        }
+        v[i][r] = a;
+      }
+      reg_dft<T, R>(v[i], sign);
+      dd[i] = (b / NS) * NS * R + k;
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = lane + 64 * i;
+    if (NB % 64 == 0 || b < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) row[at_wr(dd[i], r * NS)] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
+// Stages S .. SEND-1 of the compile-time plan, by one wavefront.
+template <typename T, int N, int PLAN, int S, int SEND, int NS = 1, class TW>
+__device__ __forceinline__ void wave_fft(cx<T>* row, const TW& tw, int lane, int sign) {
+  if constexpr (S < SEND) {
+    constexpr int R = fft_radix(N, S, PLAN);
+    wave_stage<T, N, R, NS, fft_tw_off(N, S, PLAN)>(row, tw, lane, sign);
+    wave_fft<T, N, PLAN, S + 1, SEND, NS * R>(row, tw, lane, sign);
+  }
+}
+
+// The last stage of the plan when it has exactly 64 butterflies: lane k
+// returns its outputs k + 64 r (r < R) in v instead of writing them.
+template <typename T, int N, int PLAN, class TW>
+__device__ __forceinline__ void wave_last_stage(const cx<T>* row, const TW& tw, int lane, int sign,
+                                                cx<T> (&v)[fft_radix(N, fft_nstages(N, PLAN) - 1, PLAN)]) {
+  constexpr int S = fft_nstages(N, PLAN) - 1, R = fft_radix(N, S, PLAN), NS = N / R, OFF = fft_tw_off(N, S, PLAN);
+  static_assert(NS == 64, "one butterfly per lane in the last stage");
+  constexpr bool RD_C = pad_offsets_const(N, R, NS, true);
+  auto at_rd = [](int e0, int d) { return RD_C ? lds_pad(e0) + d + (d >> 4) : lds_pad(e0 + d); };
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    cx<T> a = row[at_rd(lane, r * NS)];
+    if (r > 0) {
+      cx<T> wv = tw_at<OFF, NS>(tw, r, lane);
+      if (sign > 0) wv.y = -wv.y;
+      a = cmul(a, wv);
+    }
+    v[r] = a;
+  }
+  reg_dft<T, R>(v, sign);
+}
+
 }  // namespace les
 
 #pragma clang fp contract(fast)

@@ -81,7 +81,9 @@ struct Tune {
   int tri_mode = 0;         // Thomas sweep storage: 0 gamma/U in HBM, 1 gamma in LDS, 2 both in LDS;
                             // 3 parallel cyclic reduction, one wavefront per mode
   int tri_block = 128;      // horizontal modes per block of the tridiagonal solve
-  int pad_mode[2] = {0, 0}; // padded products: 0 level groups per workgroup, 1 streaming level runs
+  int pad_mode[2] = {0, 0}; // padded products: 0 level groups per workgroup, 1 streaming level runs,
+                            // 2 streaming with one transform per wavefront (convective step with
+                            // both scalars at nx2 = 384; elsewhere as 1)
 };
 
 // Scalar transport switches and constants (param.f90:53,72-90,139-146,174),

@@ -1774,10 +1774,21 @@ static void launch_pad_stream_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Po
 // CH32 / CH64: levels per workgroup of the batched kernel in FP32 / FP64
 // (sized to the 160 KB LDS); c->tune.pad_mode 1 selects the streaming kernel
 // where it is instantiated (nx2 >= 96)
+template <typename T>
+struct PolConvecScalar;
+template <typename T>
+static bool launch_pad_wave(les_ctx* c, const cx<T>* XB, cx<T>* XP, const PolConvecScalar<T>& pol, const char* tag);
+
+// pad_mode 2: the one-transform-per-wavefront kernel where it is
+// instantiated (the convective step with both scalars at nx2 = 384), the
+// streaming kernel elsewhere
 template <typename T, class Pol, int CH32, int CH64>
 static void launch_pad_products(les_ctx* c, const cx<T>* XB, cx<T>* XP, const Pol& pol, const char* tag) {
   constexpr int CH = sizeof(T) == 4 ? CH32 : CH64;
-  const bool stream = c->tune.pad_mode[sizeof(T) == 8 ? 1 : 0] == 1;
+  const int mode = c->tune.pad_mode[sizeof(T) == 8 ? 1 : 0];
+  if constexpr (std::is_same_v<Pol, PolConvecScalar<T>>)
+    if (mode == 2 && launch_pad_wave<T>(c, XB, XP, pol, tag)) return;
+  const bool stream = mode >= 1;
   switch (c->g.nx2) {
 #define LES_CASE(NN) \
   case NN:           \
@@ -1802,20 +1813,27 @@ struct PolConvec {
   static constexpr int NPM = 1;  // field pairs read at level k - 1: (u1, u2)
   __device__ static int slot(int o) { return o; }  // output o -> plane block of XP
   T cst;
+  // cx, cy at level k (reads levels k and k + 1)
   template <class Fn>
-  __device__ void product(int k, Fn F, T* o) const {
+  __device__ void product_xy(int k, Fn F, T* o) const {
     const T u1 = F(0, 0), u2 = F(0, 1), u3 = F(0, 2), v1 = F(0, 3), v2 = F(0, 4), v3 = F(0, 5);
     const T u3p = F(1, 2), v1p = F(1, 3), v2p = F(1, 4);
     if (k == 0) {
       o[0] = cst * (u2 * (-v3) + T(0.5) * u3p * (v2p));
       o[1] = cst * (u1 * (v3) + T(0.5) * u3p * (-v1p));
-      o[2] = T(0);
     } else {
-      const T u1m = F(-1, 0), u2m = F(-1, 1);
       o[0] = cst * (u2 * (-v3) + T(0.5) * (u3p * (v2p) + u3 * (v2)));
       o[1] = cst * (u1 * (v3) + T(0.5) * (u3p * (-v1p) + u3 * (-v1)));
-      o[2] = cst * T(0.5) * ((u1 + u1m) * (-v2) + (u2 + u2m) * (v1));
     }
+  }
+  // cz at a level k >= 1 from u1, u2, omega1, omega2 there and u1, u2 at k - 1
+  __device__ T cz(T u1, T u2, T v1, T v2, T u1m, T u2m) const {
+    return cst * T(0.5) * ((u1 + u1m) * (-v2) + (u2 + u2m) * (v1));
+  }
+  template <class Fn>
+  __device__ void product(int k, Fn F, T* o) const {
+    product_xy(k, F, o);
+    o[2] = k == 0 ? T(0) : cz(F(0, 0), F(0, 1), F(0, 3), F(0, 4), F(-1, 0), F(-1, 1));
   }
 };
 
@@ -1851,19 +1869,228 @@ struct PolConvecScalar {
   static constexpr int NPM = 1;  // field pairs read at level k - 1: (u1, u2)
   __device__ static int slot(int o) { return o < 2 ? o : (o == 4 ? 2 : o + 1); }
   T cst;
+  // the outputs at level k that read levels k and k + 1 only: cx, cy, theta, q
   template <class Fn>
-  __device__ void product(int k, Fn F, T* o) const {
-    T m[3];
-    PolConvec<T>{cst}.product(k, F, m);
-    o[0] = m[0];
-    o[1] = m[1];
-    o[4] = m[2];
+  __device__ void product_same(int k, Fn F, T* o) const {
+    PolConvec<T>{cst}.product_xy(k, F, o);
     const T u = F(0, 0), v = F(0, 1), w = F(0, 2), wp = F(1, 2);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
       o[2 + s] = scalar_adv<T>(k, u, v, w, wp, F(0, 6 + 3 * s), F(0, 7 + 3 * s), F(0, 8 + 3 * s), F(1, 8 + 3 * s),
                                cst);
   }
+  template <class Fn>
+  __device__ void product(int k, Fn F, T* o) const {
+    product_same(k, F, o);
+    o[4] = k == 0 ? T(0) : PolConvec<T>{cst}.cz(F(0, 0), F(0, 1), F(0, 3), F(0, 4), F(-1, 0), F(-1, 1));
+  }
 };
+
+
+// ------------------------------------------------------------------------
+// Streaming padded products of the convective step with both scalars, one
+// transform per wavefront (tune.pad_mode 2, nx2 = 384).  A workgroup of eight
+// wavefronts owns padded row yb and a run of output levels [k0, k1).  LDS
+// rows: X and Y (six field pairs each, levels k and k + 1 alternately),
+// O (two rows: (cx, cy) and (theta, q) advection at level k) and C (two
+// rows, each pairing cz of two consecutive levels).  cz(k + 1) reads u1, u2
+// at levels k and k + 1, so it is formed while both are resident and no
+// level k - 1 row is kept.  A level step is
+//   (1) the products: cx, cy, theta, q at level k into O, cz(k + 1) into C;
+//   (2) wavefronts 0-5 stage field pair w of level k + 2 into level k's slot
+//       and transform it, issuing the HBM reads of level k + 3; wavefront 6
+//       transforms O's first row and stores it, and at even steps runs the
+//       first two passes of the completed C row; wavefront 7 transforms O's
+//       second row and stores it, and at odd steps finishes the C row,
+// with two workgroup barriers per level: every transform belongs to one
+// wavefront, whose passes need only wave_sync.  The last pass of each
+// product row stays in registers and stores the kx < nxh coefficients
+// directly (conjugate partners exchanged across lanes).  The twiddles sit in
+// the rows' padding words.  Products enter conjugated, as in k_pad_stream;
+// the cz pairs round differently from k_pad_stream's (cz, 0) rows.
+// ------------------------------------------------------------------------
+template <typename T, int N>
+struct PadWave {
+  static constexpr int NP = 6, NXH = N / 3, RS = lds_row(N), PLAN = 2, BLOCK = 512, NROWS = 16;
+  static constexpr int NST = fft_nstages(N, PLAN);
+  static constexpr bool OK = N % 16 == 0 && NXH == 128 && N / fft_radix(N, NST - 1, PLAN) == 64;
+  static constexpr size_t LDS = sizeof(cx<T>) * (size_t)RS * NROWS;
+  // at 52 KB (FP32) three workgroups share a CU: 24 wavefronts, 80 registers
+  static constexpr int WPE = sizeof(T) == 4 ? 6 : 1;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWave<T, N>::WPE)))
+    k_pad_wave(const cx<T>* __restrict__ XB, cx<T>* __restrict__ XP, int ny2, int nz, int lc,
+               const cx<T>* __restrict__ twg, PolConvecScalar<T> pol) {
+  using G = PadWave<T, N>;
+  constexpr int NP = G::NP, RS = G::RS, NXH = G::NXH, PLAN = G::PLAN, NST = G::NST;
+  constexpr int RL = fft_radix(N, NST - 1, PLAN);
+  static_assert(G::OK && RL == 6, "one field pair (128 coefficients) per wavefront, 64 last-stage butterflies");
+  struct alignas(2 * sizeof(cx<T>)) cx2 {
+    cx<T> a, b;
+  };
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* Y = X + NP * RS;
+  cx<T>* O = Y + NP * RS;
+  cx<T>* C = O + 2 * RS;
+  const TwHoles<T, N, PLAN> tw{X};
+  tw.fill(twg);
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int yb = blockIdx.x, nl = nz - 1;
+  const int k0 = blockIdx.y * lc, k1 = min(k0 + lc, nl), n = k1 - k0;
+  auto slot = [&](int L) { return ((L - k0) & 1) ? Y : X; };
+  cx2 ra, rb;  // coefficients 2 lane, 2 lane + 1 of fields 2w, 2w + 1
+  auto fetch = [&](int L) {
+    ra = cx2{{T(0), T(0)}, {T(0), T(0)}};
+    rb = ra;
+    if (L >= 0 && L < nz) {
+      ra = reinterpret_cast<const cx2*>(XB + (((long)(2 * w) * nz + L) * ny2 + yb) * NXH)[lane];
+      rb = reinterpret_cast<const cx2*>(XB + (((long)(2 * w + 1) * nz + L) * ny2 + yb) * NXH)[lane];
+    }
+  };
+  // field pair (a, b) as one complex row (z_j = a_j + i b_j, z_{N-j} its
+  // conjugate partner), zero band [NXH, N - NXH]
+  auto put = [](cx<T>* row, int j, cx<T> A, cx<T> B) {
+    if (j == 0) {
+      row[lds_pad(0)] = {A.x, B.x};
+    } else {
+      row[lds_pad(j)] = {A.x - B.y, A.y + B.x};
+      row[lds_pad(N - j)] = {A.x + B.y, B.x - A.y};
+    }
+  };
+  auto stage_row = [&](cx<T>* row) {
+    put(row, 2 * lane, ra.a, rb.a);
+    put(row, 2 * lane + 1, ra.b, rb.b);
+    constexpr int NZB = N - 2 * NXH + 1;
+#pragma unroll
+    for (int e = 0; e < NZB; e += 64)
+      if (e + lane < NZB) row[lds_pad(NXH + e + lane)] = {T(0), T(0)};
+  };
+  auto transform = [&](cx<T>* row) { wave_fft<T, N, PLAN, 0, NST>(row, tw, lane, +1); };
+  // last pass in registers, then the kx < NXH coefficients of the row's two
+  // real outputs to XP planes sa, sb at levels la, lb (none where < 0)
+  auto finish_store = [&](const cx<T>* row, int sa, int la, int sb, int lb) {
+    cx<T> v[RL];
+    wave_last_stage<T, N, PLAN>(row, tw, lane, +1, v);
+    const int src = (64 - lane) & 63;  // holds the partners N - kx
+    const cx<T> p4 = {__shfl(v[4].x, src), __shfl(v[4].y, src)};
+    const cx<T> p5 = {__shfl(v[5].x, src), __shfl(v[5].y, src)};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const cx<T> a = v[r];
+      const cx<T> b = r == 0 ? (lane ? p5 : v[0]) : (lane ? p4 : p5);
+      const int kx = lane + 64 * r;
+      const cx<T> zk = {a.x, -a.y}, zn = {b.x, -b.y};  // forward transform = conj of the pass
+      if (la >= 0)
+        XP[(((long)sa * nl + la) * ny2 + yb) * NXH + kx] = cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
+      if (lb >= 0)
+        XP[(((long)sb * nl + lb) * ny2 + yb) * NXH + kx] = cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+    }
+  };
+  // cz at level L >= 1 from the pairs of level L (a) and pair 0 of level L - 1 (m)
+  auto cz_at = [&](const cx<T>* a, const cx<T>* m, int x) -> T {
+    const cx<T> p0 = a[lds_pad(x)], p1 = a[RS + lds_pad(x)], p2 = a[2 * RS + lds_pad(x)], q0 = m[lds_pad(x)];
+    return PolConvec<T>{pol.cst}.cz(p0.x, p0.y, p1.y, p2.x, q0.x, q0.y);
+  };
+  constexpr int S_CX = 0, S_CY = 1, S_CZ = 2, S_T = 3, S_Q = 4;  // XP planes (PolConvecScalar::slot)
+  __syncthreads();  // twiddles in place
+  // level k0 into X; wavefront 0 also puts pair 0 of level k0 - 1 into Y for cz(k0)
+  if (w < NP) {
+    fetch(k0);
+    stage_row(X + w * RS);
+    if (w == 0) {
+      fetch(k0 - 1);
+      stage_row(Y);
+    }
+    wave_sync();
+    transform(X + w * RS);
+    if (w == 0) transform(Y);
+    fetch(k0 + 1);
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < N; x += G::BLOCK) C[lds_pad(x)] = {k0 == 0 ? T(0) : cz_at(X, Y, x), T(0)};
+  __syncthreads();
+  if (w < NP) {
+    stage_row(Y + w * RS);
+    if (k0 + 2 <= k1) fetch(k0 + 2);
+    wave_sync();
+    transform(Y + w * RS);
+  }
+  __syncthreads();
+  for (int k = k0; k < k1; ++k) {
+    const int c = k - k0;
+    cx<T>* cur = slot(k);
+    cx<T>* nxt = slot(k + 1);
+    for (int x = threadIdx.x; x < N; x += G::BLOCK) {
+      T o[4];
+      auto F = [&](int dk, int f) -> T {
+        const cx<T> z = (dk > 0 ? nxt : cur)[(f >> 1) * RS + lds_pad(x)];
+        return (f & 1) ? z.y : z.x;
+      };
+      pol.product_same(k, F, o);
+      O[lds_pad(x)] = {o[0], -o[1]};
+      O[RS + lds_pad(x)] = {o[2], -o[3]};
+      // cz(k + 1): part (c + 1) & 1 of C row ((c + 1) >> 1) & 1, conjugated
+      const T z = cz_at(nxt, cur, x);
+      T* cp = reinterpret_cast<T*>(C + (((c + 1) >> 1) & 1) * RS + lds_pad(x));
+      if ((c + 1) & 1)
+        cp[1] = -z;
+      else
+        cp[0] = z;
+    }
+    __syncthreads();
+    if (w < NP) {
+      if (k + 2 <= k1) {  // level k + 2 is read by the products of level k + 1
+        cx<T>* row = cur + w * RS;
+        stage_row(row);
+        if (k + 3 <= k1) fetch(k + 3);  // in flight during the next level's work
+        wave_sync();
+        transform(row);
+      }
+    } else if (w == NP) {
+      wave_fft<T, N, PLAN, 0, NST - 1>(O, tw, lane, +1);
+      finish_store(O, S_CX, k, S_CY, k);
+      if ((c & 1) == 0) wave_fft<T, N, PLAN, 0, NST - 1>(C + ((c >> 1) & 1) * RS, tw, lane, +1);
+    } else {
+      wave_fft<T, N, PLAN, 0, NST - 1>(O + RS, tw, lane, +1);
+      finish_store(O + RS, S_T, k, S_Q, k);
+      if (c & 1) finish_store(C + ((c >> 1) & 1) * RS, S_CZ, k - 1, S_CZ, k);
+    }
+    __syncthreads();
+  }
+  // a run of odd length leaves cz(k1 - 1) paired with cz(k1), which another run stores
+  if ((n & 1) && w == NP + 1) finish_store(C + (((n - 1) >> 1) & 1) * RS, S_CZ, k1 - 1, S_CZ, -1);
+}
+
+template <typename T, int N>
+static void launch_pad_wave_n(les_ctx* c, const cx<T>* XB, cx<T>* XP, const PolConvecScalar<T>& pol,
+                              const char* tag) {
+  using G = PadWave<T, N>;
+  static_assert(G::LDS <= 163840, "padded-product ring exceeds LDS");
+  const Grid& g = c->g;
+  const int nz = g.nz, nl = nz - 1;
+  if (g.nx2 != N || 3 * g.nxh != N) {
+    c->err = "padded products (one transform per wavefront): grid does not match the kernel";
+    return;
+  }
+  if (G::LDS > 65536)
+    LCHK(c, hipFuncSetAttribute((const void*)k_pad_wave<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)G::LDS));
+  const int lc = (nl + PAD_STREAM_RUNS - 1) / PAD_STREAM_RUNS;
+  const unsigned gy = (unsigned)((nl + lc - 1) / lc);
+  ProfScope ps(c, tag, (12.0 * nz + 5.0 * (nz - 1)) * g.ny2 * g.nxh * sizeof(cx<T>));
+  hipLaunchKernelGGL((k_pad_wave<T, N>), dim3(g.ny2, gy), dim3(G::BLOCK), G::LDS, c->stream, XB, XP, g.ny2, nz, lc,
+                     tw_for<T>(c, N), pol);
+  LCHK(c, hipGetLastError());
+}
+template <typename T>
+static bool launch_pad_wave(les_ctx* c, const cx<T>* XB, cx<T>* XP, const PolConvecScalar<T>& pol, const char* tag) {
+  if (c->g.nx2 != 384) return false;
+  launch_pad_wave_n<T, 384>(c, XB, XP, pol, tag);
+  return true;
+}
 
 }  // namespace les

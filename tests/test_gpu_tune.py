@@ -11,11 +11,11 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "RHSx", "Cs_opt2", "Nu_t", "txz")
 
 
-def run(prec, tune=None, steps=12, conv=False, autotune=False, n=32, ny=None):
+def run(prec, tune=None, steps=12, conv=False, autotune=False, n=32, ny=None, nz=17):
     from les_amd.api import LesSim, checkerboard_flux
     ny = ny or n
     kw = dict(scalars=dict(wt_s=checkerboard_flux(n, ny), dTdz_top=0.01)) if conv else {}
-    s = LesSim(n, ny, 17, precision=prec, dt=2e-4, **kw)
+    s = LesSim(n, ny, nz, precision=prec, dt=2e-4, **kw)
     s.init_ic(seed=3)
     if conv:
         s.init_scalar_ic(seed=3)
@@ -79,21 +79,25 @@ def test_radix8_column_plan(torch_mod, prec, tx):
         assert err < tol, (n, err)
 
 
-@pytest.mark.parametrize("prec,conv,n,ny,steps", [("fp64", True, 64, 64, 8), ("fp32", True, 64, 64, 8),
-                                                   ("fp64", False, 64, 64, 8), ("fp32", False, 64, 64, 8),
-                                                   ("fp64", True, 256, 256, 3), ("fp32", False, 256, 256, 3),
-                                                   ("fp64", True, 96, 64, 6), ("fp32", True, 96, 64, 6)])
-def test_streaming_padded_products(torch_mod, prec, conv, n, ny, steps):
-    """Streaming level runs (radix-8 plan) against level groups (two-pass
-    plan) for the padded products, on 64^2, 256^2 and 96x64 grids (padded
-    lengths 96, 384 and 144; the 384 FP64 ring is the largest LDS tile):
-    momentum with both scalars, and momentum alone.  Tolerances as for the
-    radix-8 column plan."""
-    print("streaming padded products", prec, conv, n, ny, flush=True)
+@pytest.mark.parametrize("prec,conv,n,ny,nz,steps,mode", [
+    ("fp64", True, 64, 64, 17, 8, 1), ("fp32", True, 64, 64, 17, 8, 1), ("fp64", False, 64, 64, 17, 8, 1),
+    ("fp32", False, 64, 64, 17, 8, 1), ("fp64", True, 256, 256, 17, 3, 1), ("fp32", False, 256, 256, 17, 3, 1),
+    ("fp64", True, 96, 64, 17, 6, 1), ("fp32", True, 96, 64, 17, 6, 1),
+    ("fp32", True, 256, 256, 17, 3, 2), ("fp64", True, 256, 256, 17, 3, 2), ("fp32", True, 256, 128, 13, 3, 2),
+    ("fp64", True, 256, 128, 14, 2, 2), ("fp32", True, 256, 128, 14, 2, 2)])
+def test_streaming_padded_products(torch_mod, prec, conv, n, ny, nz, steps, mode):
+    """Streaming level runs (radix-8 plan; mode 2: one transform per
+    wavefront, cz of two levels in one transform) against level groups
+    (two-pass plan) for the padded products, on 64^2, 256^2, 256x128 and
+    96x64 grids (padded lengths 96, 384 and 144; the 384 FP64 ring is the
+    largest LDS tile): momentum with both scalars, and momentum alone.
+    nz = 13 and 14 give level runs of odd length (3) and of one level.
+    Tolerances as for the radix-8 column plan."""
+    print("streaming padded products", prec, conv, n, ny, nz, mode, flush=True)
     key = "pad_mode_" + prec
-    base, _ = run(prec, conv=conv, n=n, ny=ny, steps=steps)
-    s, _ = run(prec, tune={key: 1}, conv=conv, n=n, ny=ny, steps=steps)
-    assert s.get_tune()[key] == 1
+    base, _ = run(prec, conv=conv, n=n, ny=ny, nz=nz, steps=steps)
+    s, _ = run(prec, tune={key: mode}, conv=conv, n=n, ny=ny, nz=nz, steps=steps)
+    assert s.get_tune()[key] == mode
     tol = 1e-12 if prec == "fp64" else 2e-4
     names = ("u", "v", "w", "p") + (("theta",) if conv else ())
     for nm in names:
@@ -102,6 +106,9 @@ def test_streaming_padded_products(torch_mod, prec, conv, n, ny, steps):
         print(nm, err)
         assert err < tol, (nm, err)
     assert any(not np.array_equal(s.get(nm), base.get(nm)) for nm in names)  # the streaming kernel ran
+    if mode == 2:  # the wavefront kernel ran, not the streaming kernel of mode 1
+        s1, _ = run(prec, tune={key: 1}, conv=conv, n=n, ny=ny, nz=nz, steps=steps)
+        assert any(not np.array_equal(s.get(nm), s1.get(nm)) for nm in names)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -138,4 +145,4 @@ def test_set_tune_rejects_bad_shapes(torch_mod):
     with pytest.raises(LesError):
         g.set_tune(tri_mode=4, tri_block=512)   # four threads per mode: at most 256 modes per block
     with pytest.raises(LesError):
-        g.set_tune(pad_mode_fp32=2)
+        g.set_tune(pad_mode_fp32=3)
