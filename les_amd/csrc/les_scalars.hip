@@ -367,6 +367,34 @@ struct LoadScalarTemp {
   }
 };
 
+// the same 2 NS fields formed at one point (k_row_fwd_mf): Nu_t and Pr_t are
+// read once for all of them; field f = dir NS + s, the same values as
+// LoadScalarTemp
+template <typename T, int NS>
+struct LoadScalarTempAll {
+  static constexpr int NF = 2 * NS;
+  const T* ds[2][2];  // [dir][scalar]
+  const T *Nu, *Pr;
+  T Prc;
+  int ny, ld;
+  T cst;
+  __device__ void all(int k, int y, int x, T* v) const {
+    const long pl = (long)ld * ny, i = ((long)k * ny + y) * ld + x;
+    const T p0 = Pr ? Pr[i] : Prc;
+    T a;
+    if (k == 0) {
+      a = (T(1) / p0) * Nu[i];
+    } else {
+      const T p1 = Pr ? Pr[i + pl] : Prc;
+      a = (T(1) / (T(0.5) * (p0 + p1))) * T(0.5) * (Nu[i] + Nu[i + pl]);
+    }
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) v[dir * NS + s] = cst * (a * ds[dir][s][i]);
+  }
+};
+
 // the sponge term of the scalar equations (scalars_module.f90:143-148,
 // scalars_module_q.f90:116-121): field 3 = theta, 4 = q of the plane sums
 template <typename T>
@@ -473,7 +501,16 @@ static void scalar_sgs_finish(les_ctx* c, const ScalarJob* jobs, int ns, const T
   }
   LT.Nu = Nu_t; LT.Pr = Pr_t; LT.Prc = (T)c->scal.Pr;
   LT.ns = ns; LT.ppf = nl; LT.ny = g.ny; LT.ld = g.ld; LT.cst = T(1.0 / ((double)g.nx * g.ny));
-  launch_row_fwd<T>(c, LT, XT, g.nxh, rowsT, g.nx, g.nxh, g.ny);
+  auto all_fields = [&](auto nsc) {
+    constexpr int NS = decltype(nsc)::value;
+    LoadScalarTempAll<T, NS> LA{};
+    for (int d = 0; d < 2; ++d)
+      for (int q = 0; q < NS; ++q) LA.ds[d][q] = LT.ds[d][q];
+    LA.Nu = LT.Nu; LA.Pr = LT.Pr; LA.Prc = LT.Prc; LA.ny = LT.ny; LA.ld = LT.ld; LA.cst = LT.cst;
+    return launch_row_fwd_mf<T>(c, LA, XT, g.nxh, (Pr_t ? 2.0 : 1.0) + 2.0 * NS, nl);
+  };
+  const bool mf = ns == 2 ? all_fields(std::integral_constant<int, 2>{}) : all_fields(std::integral_constant<int, 1>{});
+  if (!mf) launch_row_fwd<T>(c, LT, XT, g.nxh, rowsT, g.nx, g.nxh, g.ny);
   cx<T>* XD = (cx<T>*)scratch(c, S_XS1, (size_t)ns * nl * g.ny * g.nxh * sizeof(cx<T>));
   ColArgs d = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
   d.in2_plane = d.in_plane;

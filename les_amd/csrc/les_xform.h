@@ -428,10 +428,12 @@ __global__ void __launch_bounds__(256)
 
 // false when there is no compile-time variant for this row length (or its
 // LDS tile does not fit): the caller then runs the plane-batched kernel
+// nlev levels (the grid's nz when < 0), output rows ((f nlev + k) ny + y)
 template <typename T, class ML>
-static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, double read_fields) {
+static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, double read_fields, int nlev = -1) {
   const Grid& g = c->g;
   const int n = g.nx;
+  const int nl = nlev < 0 ? g.nz : nlev;
   auto go = [&](auto nn) -> bool {
     constexpr int N = decltype(nn)::value;
     constexpr size_t sm = (size_t)(N + ML::NF * lds_row(N)) * sizeof(cx<T>);
@@ -442,9 +444,9 @@ static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, doub
         LCHK(c, hipFuncSetAttribute((const void*)k_row_fwd_mf<T, N, ML>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)sm));
       ProfScope ps(c, "row_fwd",
-                   (double)g.nz * g.ny * (n * sizeof(T) * read_fields + 2.0 * ML::NF * kxo * sizeof(cx<T>) / 2));
-      hipLaunchKernelGGL((k_row_fwd_mf<T, N, ML>), dim3((unsigned)(g.ny / 2), (unsigned)g.nz), dim3(256), sm,
-                         c->stream, L, out, (long)kxo, g.ny, g.nz, tw_for<T>(c, N), kxo);
+                   (double)nl * g.ny * (n * sizeof(T) * read_fields + 2.0 * ML::NF * kxo * sizeof(cx<T>) / 2));
+      hipLaunchKernelGGL((k_row_fwd_mf<T, N, ML>), dim3((unsigned)(g.ny / 2), (unsigned)nl), dim3(256), sm,
+                         c->stream, L, out, (long)kxo, g.ny, nl, tw_for<T>(c, N), kxo);
       LCHK(c, hipGetLastError());
       return true;
     }
