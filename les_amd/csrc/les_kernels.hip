@@ -652,6 +652,100 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
   }
 }
 
+// k_calc_sij and k_sgs_tau in one pass, for steps in which no dynamic model
+// reads S_ij: a point forms S_ij at its level and the level above from the
+// gradients (the expressions of k_calc_sij), then Nu_t and tau as k_sgs_tau,
+// so S_ij never goes to memory
+template <typename T>
+__device__ __forceinline__ void sij_at(const T* const* gr, long i, long pl, int k, T (&s)[6]) {
+  T ux, uy, uz, vx, vy, vz, wx, wy, wz;
+  if (k == 0) {
+    ux = gr[0][i]; uy = gr[1][i]; uz = gr[2][i];
+    vx = gr[3][i]; vy = gr[4][i]; vz = gr[5][i];
+    wx = T(0.5) * (gr[6][i] + gr[6][i + pl]);
+    wy = T(0.5) * (gr[7][i] + gr[7][i + pl]);
+    wz = gr[8][i];
+  } else {
+    const long m = i - pl;
+    ux = T(0.5) * (gr[0][i] + gr[0][m]);
+    uy = T(0.5) * (gr[1][i] + gr[1][m]);
+    uz = gr[2][i];
+    vx = T(0.5) * (gr[3][i] + gr[3][m]);
+    vy = T(0.5) * (gr[4][i] + gr[4][m]);
+    vz = gr[5][i];
+    wx = gr[6][i];
+    wy = gr[7][i];
+    wz = T(0.5) * (gr[8][i] + gr[8][m]);
+  }
+  s[0] = ux; s[1] = T(0.5) * (uy + vx); s[2] = T(0.5) * (uz + wx);
+  s[3] = vy; s[4] = T(0.5) * (vz + wy); s[5] = wz;
+  // opaque from here on, as if read back from memory: the Nu_t and tau
+  // arithmetic then contracts exactly as in k_sgs_tau (identical bits)
+#pragma unroll
+  for (int q = 0; q < 6; ++q) asm volatile("" : "+v"(s[q]));
+}
+template <typename T>
+struct Grads9 {
+  const T* p[9];
+};
+template <typename T>
+__global__ void k_sij_tau(Grads9<T> G, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                          T* u_lag, T* v_lag, T* w_lag, T* u_lag_s, T* v_lag_s, T* w_lag_s,
+                          const T* __restrict__ Cs, T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy,
+                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2_const,
+                          const double* __restrict__ l2z, T* Pr_t, T Pr) {
+  const long pl = (long)ld * ny, n = pl * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const LevelCol kx_ = level_col(i, pl, ld);
+    const int k = kx_.k, x = kx_.x;
+    if (u_lag) {  // sgs_stag.f90:62-64, whole arrays
+      u_lag[i] = u_lag[i] + u[i];
+      v_lag[i] = v_lag[i] + v[i];
+      w_lag[i] = w_lag[i] + w[i];
+    }
+    if (u_lag_s) {  // sgs_stag.f90:66-68
+      u_lag_s[i] = u_lag_s[i] + u[i];
+      v_lag_s[i] = v_lag_s[i] + v[i];
+      w_lag_s[i] = w_lag_s[i] + w[i];
+    }
+    if (use_const) Cs_out[i] = cs_const;  // sgs_stag.f90:166-173, whole arrays
+    if (use_const && Pr_t) Pr_t[i] = Pr;
+    if (k == nz - 1) {  // sgs_stag.f90:356-364
+      txx[i] = T(BOGUS); txy[i] = T(BOGUS); tyy[i] = T(BOGUS); tzz[i] = T(BOGUS);
+      txz[i] = T(0); tyz[i] = T(0);
+    }
+    if (x >= nx) continue;
+    T s0[6];
+    sij_at<T>(G.p, i, pl, k, s0);
+    const T l2 = l2z ? (T)l2z[k] : l2_const;
+    const T c0 = use_const ? cs_const : Cs[i];
+    const T nu0 = smag(s0[0], s0[1], s0[2], s0[3], s0[4], s0[5]) * c0 * l2;
+    Nu_t[i] = nu0;
+    if (k == nz - 1) continue;
+    if (k == 0) {  // wall level, sgs_stag.f90:223-241 (txz, tyz(1) kept from wallstress)
+      const T cst = nu0;
+      txx[i] = T(-2) * (cst + T(0)) * s0[0];
+      txy[i] = T(-2) * (cst + T(0)) * s0[1];
+      tyy[i] = T(-2) * (cst + T(0)) * s0[3];
+      tzz[i] = T(-2) * (cst + T(0)) * s0[5];
+    } else {
+      const long p = i + pl;
+      T s1[6];
+      sij_at<T>(G.p, p, pl, k + 1, s1);
+      const T c1 = use_const ? cs_const : Cs[p];
+      const T l2p = l2z ? (T)l2z[k + 1] : l2_const;
+      const T nu1 = smag(s1[0], s1[1], s1[2], s1[3], s1[4], s1[5]) * c1 * l2p;
+      const T cst = T(0.5) * (nu0 + nu1);
+      txx[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[0] + s1[0]);
+      txy[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[1] + s1[1]);
+      tyy[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[3] + s1[3]);
+      tzz[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[5] + s1[5]);
+      txz[i] = T(-2) * (nu0 + T(0)) * s0[2];
+      tyz[i] = T(-2) * (nu0 + T(0)) * s0[4];
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // interpolag_Sdep.f90: Lagrangian back-trajectory + trilinear interpolation
 // of NF fields.  SIM: interpolag_Ssim.f90, whose two lowest levels keep the
@@ -1135,6 +1229,28 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
   const int model = c->sgs_model;
   const bool lagr = model == 4 || model == 5;  // u_lag accumulates under models 4, 5, 7 (sgs_stag.f90:59-69)
   const bool lag_s = ss && model == 5;          // u_lag_s too, with the scalar Lagrangian model
+  const int jt = c->jt;
+  // :165-174 sets Cs_opt2 = 0.16**2 (default real) under models 5-7 while
+  // jt_total < 500; model 1 sets Co**2 = 0.16_rprec**2 every step
+  const int use_const = model == 1 || ((model == 5 || model == 6) && c->jt_total < 500);
+  const T cs0 = model == 1 ? (T)(0.16 * 0.16) : (T)(double)(0.16f * 0.16f);
+  const T l2 = (T)(g.delta * g.delta);
+  // a dynamic update this step reads S_ij; otherwise S_ij and tau are one pass
+  const bool dyn = model != 1 && !(jt == 1 && g.inilag) && (jt >= 1 || g.initu) && jt % g.cs_count == 0;
+  if (!dyn) {
+    if (model != 1 && jt == 1 && g.inilag) op_fill<T>(c, (T*)s.Cs_opt2, n3, T(0.03));
+    Grads9<T> G;
+    for (int q = 0; q < 9; ++q) G.p[q] = gr[q];
+    ProfScope _ps(c, "sij_tau", (9.0 + (lagr ? 6 : 0) + (lag_s ? 6 : 0) + 3 + 1 + 1 + 6) * n3 * sizeof(T));
+    hipLaunchKernelGGL((k_sij_tau<T>), grid1(n3), dim3(256), 0, c->stream, G, u, v, w,
+                       lagr ? (T*)s.u_lag : (T*)nullptr, (T*)s.v_lag, (T*)s.w_lag,
+                       lag_s ? (T*)ss->u_lag_s : (T*)nullptr, lag_s ? (T*)ss->v_lag_s : (T*)nullptr,
+                       lag_s ? (T*)ss->w_lag_s : (T*)nullptr, (const T*)s.Cs_opt2, cs0, use_const, (T*)s.Cs_opt2,
+                       (T*)s.Nu_t, tau[0], tau[1], tau[2], tau[3], tau[4], tau[5], g.nx, g.ld, g.ny, g.nz, l2,
+                       model == 1 ? (const double*)c->l2z : nullptr, ss ? (T*)ss->Pr_t : (T*)nullptr, (T)c->scal.Pr);
+    LCHK(c, hipGetLastError());
+    return;
+  }
   T* Sb = (T*)scratch(c, S_SIJ, 6 * n3 * sizeof(T));
   T* S[6] = {Sb, Sb + n3, Sb + 2 * n3, Sb + 3 * n3, Sb + 4 * n3, Sb + 5 * n3};
   {
@@ -1146,12 +1262,7 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
                        lag_s ? (T*)ss->w_lag_s : (T*)nullptr);
   }
   LCHK(c, hipGetLastError());
-  const int jt = c->jt;
-  if (model == 1) {
-    // Cs_opt2 = Co^2 (sgs_stag.f90:75-76), written by k_sgs_tau below
-  } else if (jt == 1 && g.inilag) {
-    op_fill<T>(c, (T*)s.Cs_opt2, n3, T(0.03));
-  } else if ((jt >= 1 || g.initu) && (jt % g.cs_count == 0)) {
+  {  // the dynamic update (:56-163)
     if (model == 2) {
       const T* Sc[6] = {S[0], S[1], S[2], S[3], S[4], S[5]};
       op_std_dynamic<T>(c, Sc, u, v, w, nullptr, (T*)s.Cs_opt2);
@@ -1183,11 +1294,6 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
       }
     }
   }
-  // :165-174 sets Cs_opt2 = 0.16**2 (default real) under models 5-7 while
-  // jt_total < 500; model 1 sets Co**2 = 0.16_rprec**2 every step
-  const int use_const = model == 1 || ((model == 5 || model == 6) && c->jt_total < 500);
-  const T cs0 = model == 1 ? (T)(0.16 * 0.16) : (T)(double)(0.16f * 0.16f);
-  const T l2 = (T)(g.delta * g.delta);
   {
     ProfScope _ps2(c, "sgs_tau", (6.0 + 1 + 1 + 6) * n3 * sizeof(T));
     hipLaunchKernelGGL((k_sgs_tau<T>), grid1(n3), dim3(256), 0, c->stream, S[0], S[1], S[2], S[3], S[4], S[5],
