@@ -652,8 +652,8 @@ __global__ void k_sgs_tau(const T* __restrict__ S11, const T* __restrict__ S12, 
   }
 }
 
-// k_calc_sij and k_sgs_tau in one pass, for steps in which no dynamic model
-// reads S_ij: a point forms S_ij at its level and the level above from the
+// k_calc_sij and k_sgs_tau in one pass (k_sij_tau_col), for steps in which no
+// dynamic model reads S_ij: S_ij at a level and the level above from the
 // gradients (the expressions of k_calc_sij), then Nu_t and tau as k_sgs_tau,
 // so S_ij never goes to memory
 template <typename T>
@@ -688,16 +688,32 @@ template <typename T>
 struct Grads9 {
   const T* p[9];
 };
-template <typename T>
-__global__ void k_sij_tau(Grads9<T> G, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
-                          T* u_lag, T* v_lag, T* w_lag, T* u_lag_s, T* v_lag_s, T* w_lag_s,
-                          const T* __restrict__ Cs, T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy,
-                          T* txz, T* tyy, T* tyz, T* tzz, int nx, int ld, int ny, int nz, T l2_const,
-                          const double* __restrict__ l2z, T* Pr_t, T Pr) {
-  const long pl = (long)ld * ny, n = pl * nz;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const LevelCol kx_ = level_col(i, pl, ld);
-    const int k = kx_.k, x = kx_.x;
+// One thread per column and run of L levels: S_ij of the level above is
+// formed once and carried to the next level, and the five gradients the next
+// level's averages need stay in registers (the expressions of sij_at, so the
+// bits of k_calc_sij followed by k_sgs_tau)
+template <typename T, int L>
+__global__ void __launch_bounds__(256)
+    k_sij_tau_col(Grads9<T> G, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                  T* u_lag, T* v_lag, T* w_lag, T* u_lag_s, T* v_lag_s, T* w_lag_s, const T* __restrict__ Cs,
+                  T cs_const, int use_const, T* Cs_out, T* Nu_t, T* txx, T* txy, T* txz, T* tyy, T* tyz, T* tzz,
+                  int nx, int ld, int ny, int nz, T l2_const, const double* __restrict__ l2z, T* Pr_t, T Pr) {
+  const long pl = (long)ld * ny;
+  const long col = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (col >= pl) return;
+  const int k0 = blockIdx.y * L, k1 = min(k0 + L, nz);
+  const int x = (int)(col % ld);
+  T sc[6], sn[6], g5[5];  // S_ij at k and k + 1; ux, uy, vx, vy, wz at k
+  if (x < nx) {
+    const long i0 = (long)k0 * pl + col;
+    sij_at<T>(G.p, i0, pl, k0, sc);
+    g5[0] = G.p[0][i0]; g5[1] = G.p[1][i0]; g5[2] = G.p[3][i0]; g5[3] = G.p[4][i0]; g5[4] = G.p[8][i0];
+  }
+#pragma unroll
+  for (int kk = 0; kk < L; ++kk) {
+    const int k = k0 + kk;
+    if (k >= k1) break;
+    const long i = (long)k * pl + col;
     if (u_lag) {  // sgs_stag.f90:62-64, whole arrays
       u_lag[i] = u_lag[i] + u[i];
       v_lag[i] = v_lag[i] + v[i];
@@ -715,34 +731,50 @@ __global__ void k_sij_tau(Grads9<T> G, const T* __restrict__ u, const T* __restr
       txz[i] = T(0); tyz[i] = T(0);
     }
     if (x >= nx) continue;
-    T s0[6];
-    sij_at<T>(G.p, i, pl, k, s0);
+    if (k + 1 < nz) {  // S_ij one level up, from level k + 1 and the level-k gradients held in g5
+      const long p = i + pl;
+      const T ux1 = G.p[0][p], uy1 = G.p[1][p], vx1 = G.p[3][p], vy1 = G.p[4][p], wz1 = G.p[8][p];
+      const T ux = T(0.5) * (ux1 + g5[0]);
+      const T uy = T(0.5) * (uy1 + g5[1]);
+      const T uz = G.p[2][p];
+      const T vx = T(0.5) * (vx1 + g5[2]);
+      const T vy = T(0.5) * (vy1 + g5[3]);
+      const T vz = G.p[5][p];
+      const T wx = G.p[6][p];
+      const T wy = G.p[7][p];
+      const T wz = T(0.5) * (wz1 + g5[4]);
+      sn[0] = ux; sn[1] = T(0.5) * (uy + vx); sn[2] = T(0.5) * (uz + wx);
+      sn[3] = vy; sn[4] = T(0.5) * (vz + wy); sn[5] = wz;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) asm volatile("" : "+v"(sn[q]));
+      g5[0] = ux1; g5[1] = uy1; g5[2] = vx1; g5[3] = vy1; g5[4] = wz1;
+    }
     const T l2 = l2z ? (T)l2z[k] : l2_const;
     const T c0 = use_const ? cs_const : Cs[i];
-    const T nu0 = smag(s0[0], s0[1], s0[2], s0[3], s0[4], s0[5]) * c0 * l2;
+    const T nu0 = smag(sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]) * c0 * l2;
     Nu_t[i] = nu0;
     if (k == nz - 1) continue;
     if (k == 0) {  // wall level, sgs_stag.f90:223-241 (txz, tyz(1) kept from wallstress)
       const T cst = nu0;
-      txx[i] = T(-2) * (cst + T(0)) * s0[0];
-      txy[i] = T(-2) * (cst + T(0)) * s0[1];
-      tyy[i] = T(-2) * (cst + T(0)) * s0[3];
-      tzz[i] = T(-2) * (cst + T(0)) * s0[5];
+      txx[i] = T(-2) * (cst + T(0)) * sc[0];
+      txy[i] = T(-2) * (cst + T(0)) * sc[1];
+      tyy[i] = T(-2) * (cst + T(0)) * sc[3];
+      tzz[i] = T(-2) * (cst + T(0)) * sc[5];
     } else {
       const long p = i + pl;
-      T s1[6];
-      sij_at<T>(G.p, p, pl, k + 1, s1);
       const T c1 = use_const ? cs_const : Cs[p];
       const T l2p = l2z ? (T)l2z[k + 1] : l2_const;
-      const T nu1 = smag(s1[0], s1[1], s1[2], s1[3], s1[4], s1[5]) * c1 * l2p;
+      const T nu1 = smag(sn[0], sn[1], sn[2], sn[3], sn[4], sn[5]) * c1 * l2p;
       const T cst = T(0.5) * (nu0 + nu1);
-      txx[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[0] + s1[0]);
-      txy[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[1] + s1[1]);
-      tyy[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[3] + s1[3]);
-      tzz[i] = T(-2) * (cst + T(0)) * T(0.5) * (s0[5] + s1[5]);
-      txz[i] = T(-2) * (nu0 + T(0)) * s0[2];
-      tyz[i] = T(-2) * (nu0 + T(0)) * s0[4];
+      txx[i] = T(-2) * (cst + T(0)) * T(0.5) * (sc[0] + sn[0]);
+      txy[i] = T(-2) * (cst + T(0)) * T(0.5) * (sc[1] + sn[1]);
+      tyy[i] = T(-2) * (cst + T(0)) * T(0.5) * (sc[3] + sn[3]);
+      tzz[i] = T(-2) * (cst + T(0)) * T(0.5) * (sc[5] + sn[5]);
+      txz[i] = T(-2) * (nu0 + T(0)) * sc[2];
+      tyz[i] = T(-2) * (nu0 + T(0)) * sc[4];
     }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sc[q] = sn[q];
   }
 }
 
@@ -1242,7 +1274,9 @@ void op_sgs_stag(les_ctx* c, const T* const gr[9], const T* u, const T* v, const
     Grads9<T> G;
     for (int q = 0; q < 9; ++q) G.p[q] = gr[q];
     ProfScope _ps(c, "sij_tau", (9.0 + (lagr ? 6 : 0) + (lag_s ? 6 : 0) + 3 + 1 + 1 + 6) * n3 * sizeof(T));
-    hipLaunchKernelGGL((k_sij_tau<T>), grid1(n3), dim3(256), 0, c->stream, G, u, v, w,
+    constexpr int LR = 8;  // levels per thread
+    const dim3 gcol((unsigned)(((size_t)g.ld * g.ny + 255) / 256), (unsigned)((g.nz + LR - 1) / LR));
+    hipLaunchKernelGGL((k_sij_tau_col<T, LR>), gcol, dim3(256), 0, c->stream, G, u, v, w,
                        lagr ? (T*)s.u_lag : (T*)nullptr, (T*)s.v_lag, (T*)s.w_lag,
                        lag_s ? (T*)ss->u_lag_s : (T*)nullptr, lag_s ? (T*)ss->v_lag_s : (T*)nullptr,
                        lag_s ? (T*)ss->w_lag_s : (T*)nullptr, (const T*)s.Cs_opt2, cs0, use_const, (T*)s.Cs_opt2,
