@@ -518,6 +518,53 @@ __device__ __forceinline__ void wave_stage(cx<T>* row, const TW& tw, int lane, i
   wave_sync();
 }
 
+// A stage with at most 64 butterflies, one per lane, with two options that
+// leave the arithmetic unchanged: SWAP34 exchanges bits 3 and 4 of the lane
+// index below 32 in the lane -> butterfly map (in the second radix-8 pass of
+// a 384-point transform it puts butterfly groups whose writes fall on
+// disjoint LDS banks in one 16-lane write group), and RTW takes the stage's
+// twiddles, already conjugated for sign +1, from the registers rtw[r - 1].
+template <typename T, int N, int R, int NS, int OFF, bool SWAP34, bool RTW, class TW>
+__device__ __forceinline__ void wave_stage_lane(cx<T>* row, const TW& tw, int lane, int sign, const cx<T>* rtw) {
+  constexpr int NB = N / R;
+  static_assert(NB <= 64, "one butterfly per lane");
+  constexpr bool RD_C = pad_offsets_const(N, R, NS, true);
+  constexpr bool WR_C = pad_offsets_const(N, R, NS, false);
+  auto at_rd = [](int e0, int d) { return RD_C ? lds_pad(e0) + d + (d >> 4) : lds_pad(e0 + d); };
+  auto at_wr = [](int e0, int d) { return WR_C ? lds_pad(e0) + d + (d >> 4) : lds_pad(e0 + d); };
+  const int b = SWAP34 && lane < 32 ? ((lane & ~24) | ((lane & 8) << 1) | ((lane & 16) >> 1)) : lane;
+  cx<T> v[R];
+  int dd = 0;
+  if (NB == 64 || b < NB) {
+    const int k = b % NS;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cx<T> a = row[at_rd(b, r * NB)];
+      if constexpr (NS > 1) {
+        if (r > 0) {
+          cx<T> wv;
+          if constexpr (RTW) {
+            wv = rtw[r - 1];
+          } else {
+            wv = tw_at<OFF, NS>(tw, r, k);
+            if (sign > 0) wv.y = -wv.y;
+          }
+          a = cmul(a, wv);
+        }
+      }
+      v[r] = a;
+    }
+    reg_dft<T, R>(v, sign);
+    dd = (b / NS) * NS * R + k;
+  }
+  wave_sync();
+  if (NB == 64 || b < NB) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) row[at_wr(dd, r * NS)] = v[r];
+  }
+  wave_sync();
+}
+
 // Stages S .. SEND-1 of the compile-time plan, by one wavefront.
 template <typename T, int N, int PLAN, int S, int SEND, int NS = 1, class TW>
 __device__ __forceinline__ void wave_fft(cx<T>* row, const TW& tw, int lane, int sign) {
@@ -529,10 +576,13 @@ __device__ __forceinline__ void wave_fft(cx<T>* row, const TW& tw, int lane, int
 }
 
 // The last stage of the plan when it has exactly 64 butterflies: lane k
-// returns its outputs k + 64 r (r < R) in v instead of writing them.
+// returns its outputs k + 64 r (r < R) in v instead of writing them; with
+// rtw, the twiddles come from registers (conjugated for sign +1, as in
+// wave_stage_lane).
 template <typename T, int N, int PLAN, class TW>
 __device__ __forceinline__ void wave_last_stage(const cx<T>* row, const TW& tw, int lane, int sign,
-                                                cx<T> (&v)[fft_radix(N, fft_nstages(N, PLAN) - 1, PLAN)]) {
+                                                cx<T> (&v)[fft_radix(N, fft_nstages(N, PLAN) - 1, PLAN)],
+                                                const cx<T>* rtw = nullptr) {
   constexpr int S = fft_nstages(N, PLAN) - 1, R = fft_radix(N, S, PLAN), NS = N / R, OFF = fft_tw_off(N, S, PLAN);
   static_assert(NS == 64, "one butterfly per lane in the last stage");
   constexpr bool RD_C = pad_offsets_const(N, R, NS, true);
@@ -541,8 +591,13 @@ __device__ __forceinline__ void wave_last_stage(const cx<T>* row, const TW& tw, 
   for (int r = 0; r < R; ++r) {
     cx<T> a = row[at_rd(lane, r * NS)];
     if (r > 0) {
-      cx<T> wv = tw_at<OFF, NS>(tw, r, lane);
-      if (sign > 0) wv.y = -wv.y;
+      cx<T> wv;
+      if (rtw) {
+        wv = rtw[r - 1];
+      } else {
+        wv = tw_at<OFF, NS>(tw, r, lane);
+        if (sign > 0) wv.y = -wv.y;
+      }
       a = cmul(a, wv);
     }
     v[r] = a;

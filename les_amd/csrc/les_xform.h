@@ -1928,7 +1928,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
   using G = PadWave<T, N>;
   constexpr int NP = G::NP, RS = G::RS, NXH = G::NXH, PLAN = G::PLAN, NST = G::NST;
   constexpr int RL = fft_radix(N, NST - 1, PLAN);
-  static_assert(G::OK && RL == 6, "one field pair (128 coefficients) per wavefront, 64 last-stage butterflies");
+  static_assert(G::OK && RL == 6 && N == 384 && NST == 3 && fft_radix(N, 0, PLAN) == 8 && fft_radix(N, 1, PLAN) == 8,
+                "384 = 8 x 8 x 6: one field pair (128 coefficients) per wavefront, 64 last-stage butterflies");
   struct alignas(2 * sizeof(cx<T>)) cx2 {
     cx<T> a, b;
   };
@@ -1944,6 +1945,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
   const int yb = blockIdx.x, nl = nz - 1;
   const int k0 = blockIdx.y * lc, k1 = min(k0 + lc, nl), n = k1 - k0;
   auto slot = [&](int L) { return ((L - k0) & 1) ? Y : X; };
+  constexpr int OFF1 = fft_tw_off(N, 1, PLAN), OFF2 = fft_tw_off(N, 2, PLAN);
+  cx<T> rtw[RL - 1];  // the last pass's twiddles of this lane (k = lane), filled once the table is in LDS
   cx2 ra, rb;  // coefficients 2 lane, 2 lane + 1 of fields 2w, 2w + 1
   auto fetch = [&](int L) {
     ra = cx2{{T(0), T(0)}, {T(0), T(0)}};
@@ -1971,12 +1974,21 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
     for (int e = 0; e < NZB; e += 64)
       if (e + lane < NZB) row[lds_pad(NXH + e + lane)] = {T(0), T(0)};
   };
-  auto transform = [&](cx<T>* row) { wave_fft<T, N, PLAN, 0, NST>(row, tw, lane, +1); };
+  // the first two passes (radix 8, 8): the second with the bank-disjoint
+  // butterfly order; all three passes; the last pass with register twiddles
+  auto passes12 = [&](cx<T>* row) {
+    wave_stage_lane<T, N, 8, 1, 0, false, false>(row, tw, lane, +1, rtw);
+    wave_stage_lane<T, N, 8, 8, OFF1, true, false>(row, tw, lane, +1, rtw);
+  };
+  auto transform = [&](cx<T>* row) {
+    passes12(row);
+    wave_stage_lane<T, N, RL, 64, OFF2, false, true>(row, tw, lane, +1, rtw);
+  };
   // last pass in registers, then the kx < NXH coefficients of the row's two
   // real outputs to XP planes sa, sb at levels la, lb (none where < 0)
   auto finish_store = [&](const cx<T>* row, int sa, int la, int sb, int lb) {
     cx<T> v[RL];
-    wave_last_stage<T, N, PLAN>(row, tw, lane, +1, v);
+    wave_last_stage<T, N, PLAN>(row, tw, lane, +1, v, rtw);
     const int src = (64 - lane) & 63;  // holds the partners N - kx
     const cx<T> p4 = {__shfl(v[4].x, src), __shfl(v[4].y, src)};
     const cx<T> p5 = {__shfl(v[5].x, src), __shfl(v[5].y, src)};
@@ -1999,6 +2011,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
   };
   constexpr int S_CX = 0, S_CY = 1, S_CZ = 2, S_T = 3, S_Q = 4;  // XP planes (PolConvecScalar::slot)
   __syncthreads();  // twiddles in place
+#pragma unroll
+  for (int r = 1; r < RL; ++r) {
+    rtw[r - 1] = tw_at<OFF2, 64>(tw, r, lane);
+    rtw[r - 1].y = -rtw[r - 1].y;
+  }
   // level k0 into X; wavefront 0 also puts pair 0 of level k0 - 1 into Y for cz(k0)
   if (w < NP) {
     fetch(k0);
@@ -2026,7 +2043,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
     const int c = k - k0;
     cx<T>* cur = slot(k);
     cx<T>* nxt = slot(k + 1);
-    for (int x = threadIdx.x; x < N; x += G::BLOCK) {
+    // column x of wavefront w < 6: the two 16-column blocks a 32-lane read
+    // group takes lie 16 blocks apart where possible (padded, their LDS
+    // banks do not overlap); every column once
+    for (int x0 = threadIdx.x; x0 < N; x0 += G::BLOCK) {
+      const int blk = w < 4 ? 2 * w + (lane >> 5) + 16 * ((lane >> 4) & 1) : 8 + 4 * (w - 4) + (lane >> 4);
+      const int x = 16 * blk + (lane & 15);
       T o[4];
       auto F = [&](int dk, int f) -> T {
         const cx<T> z = (dk > 0 ? nxt : cur)[(f >> 1) * RS + lds_pad(x)];
@@ -2053,11 +2075,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
         transform(row);
       }
     } else if (w == NP) {
-      wave_fft<T, N, PLAN, 0, NST - 1>(O, tw, lane, +1);
+      passes12(O);
       finish_store(O, S_CX, k, S_CY, k);
-      if ((c & 1) == 0) wave_fft<T, N, PLAN, 0, NST - 1>(C + ((c >> 1) & 1) * RS, tw, lane, +1);
+      if ((c & 1) == 0) passes12(C + ((c >> 1) & 1) * RS);
     } else {
-      wave_fft<T, N, PLAN, 0, NST - 1>(O + RS, tw, lane, +1);
+      passes12(O + RS);
       finish_store(O + RS, S_T, k, S_Q, k);
       if (c & 1) finish_store(C + ((c >> 1) & 1) * RS, S_CZ, k - 1, S_CZ, k);
     }
