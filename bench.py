@@ -40,7 +40,7 @@ SCALAR_IC = dict(T_init=283.0, z_inv=0.8, inv_strength=0.01, amp_K=0.1, q_init=1
 
 
 def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_steps=0, keep=False, conv=True,
-            tune=True):
+            tune=True, layout=None):
     import torch
     from les_amd.api import LesSim, checkerboard_flux
     scal = dict(SCALARS, wt_s=checkerboard_flux(nx, ny)) if conv else None
@@ -51,6 +51,8 @@ def run_gpu(nx, ny, nz, prec, steps, warmup, profile_steps, graph=True, extra_st
     # tile widths, tridiagonal schedules and padded-product variants; the FFT
     # plan and the cyclic-reduction schedule change the rounding, not the method
     tuned = sim.autotune(7)["chosen"] if tune else None
+    if layout is not None:
+        sim.set_layout(layout)
     sim.set_graph(False)
     sim.step(warmup)            # eager warm-up: every routine runs once, scratch reaches full size
     sim.set_graph(graph)
@@ -383,6 +385,8 @@ def main():
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure run")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of les_autotune")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-instance ensemble run")
+    ap.add_argument("--xs-layout", type=int, choices=(0, 1), default=None,
+                    help="x-spectral layout of the bench context (0 row-major, 1 tiled); default: the context's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -396,14 +400,15 @@ def main():
     D = max(args.drift_steps, W + K + args.profile_steps)
     ms32, prof32, ok32, rms32, cfl32, f32, tune32 = run_gpu(nx, ny, nz, "fp32", K, W, args.profile_steps, graph,
                                                             extra_steps=D - W - K - args.profile_steps,
-                                                            keep=not args.no_fp64, conv=conv, tune=not args.no_tune)
+                                                            keep=not args.no_fp64, conv=conv, tune=not args.no_tune,
+                                                            layout=args.xs_layout)
     print(f"fp32-bulk: {ms32:.3f} ms/step finite={ok32} rmsdiv={rms32:.3e} cfl={cfl32[0]:.3e}",
           file=sys.stderr, flush=True)
     ms64, err = None, None
     if not args.no_fp64:
         ms64, prof64, ok64, rms64, cfl64, f64, tune64 = run_gpu(nx, ny, nz, "fp64", K, W, 0, graph,
                                                                 extra_steps=D - W - K, keep=True, conv=conv,
-                                                                tune=not args.no_tune)
+                                                                tune=not args.no_tune, layout=args.xs_layout)
         print(f"fp64: {ms64:.3f} ms/step finite={ok64} rmsdiv={rms64:.3e}", file=sys.stderr, flush=True)
         err = fp32_error(f32, f64, nx, nz)
         err["steps"] = D

@@ -415,6 +415,17 @@ int les_profile_report(les_ctx *c, char *buf, int buflen);
 int les_autotune(les_ctx *c, int what, char *report, int report_len);
 int les_get_tune(les_ctx *c, int *out8);
 int les_set_tune(les_ctx *c, const int *in8);
+/* Layout of the x-spectral intermediates that the row passes write for the
+   column passes of filt_da / ddx / ddy (also read by the zero-pad pass of
+   the convective inputs formed from filt_da's spectra) and of divstress; no
+   counterpart in the reference code, whose FFTW plans write row-major
+   spectra.  0 row-major, 1 tiles of 8 coefficient columns, each contiguous
+   over a plane's ny rows (needs nx a multiple of 16; the default there).
+   Results are bit-identical in either layout; the tiles give the column
+   passes whole 128-byte lines.  les_get_layout returns the current one (-1
+   for a null context). */
+int les_set_layout(les_ctx *c, int xs_tiled);
+int les_get_layout(les_ctx *c);
 
 /* ---- ensembles (no counterpart in the reference code): independent
    contexts, each with its own grid, surface parameters, precision and state

@@ -343,6 +343,14 @@ class Les:
                       pad_mode_fp64], dtype=np.int32)
         self._chk(self.L.les_set_tune(self.ctx, a.ctypes.data_as(C.c_void_p)))
 
+    def set_layout(self, xs_tiled):
+        """x-spectral intermediate layout: 0 row-major, 1 8-column tiles
+        (nx a multiple of 16); results are bit-identical in both."""
+        self._chk(self.L.les_set_layout(self.ctx, int(xs_tiled)))
+
+    def get_layout(self):
+        return int(self.L.les_get_layout(self.ctx))
+
     def set_tune_choice(self, chosen):
         """Apply the 'chosen' entry of an autotune report."""
         self.set_tune(col_tx_fp32=chosen["col_tx"][0], col_tx_fp64=chosen["col_tx"][1],

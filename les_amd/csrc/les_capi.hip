@@ -416,6 +416,7 @@ int les_create(const les_grid* gi, les_ctx** out) {
   g.delta = sqrt(g.dx * g.dy);
   g.cs_count = gi->cs_count > 0 ? gi->cs_count : 5;
   g.inilag = gi->inilag; g.initu = gi->initu;
+  c->tune.xs_tiled = g.nxh % 8 == 0;  // tiled x-spectral intermediates wherever rows allow them
   c->p_nx = make_plan(g.nx); c->p_ny = make_plan(g.ny);
   c->p_nx2 = make_plan(g.nx2); c->p_ny2 = make_plan(g.ny2);
   if (c->p_nx.n < 0 || c->p_ny.n < 0 || c->p_nx2.n < 0 || c->p_ny2.n < 0 || g.nx > 1024 || g.ny > 1024 ||
@@ -1444,6 +1445,20 @@ int les_get_tune(les_ctx* c, int* out8) {
   out8[7] = c->tune.pad_mode[1];
   DONE(c);
 }
+int les_set_layout(les_ctx* c, int xs_tiled) {
+  GUARD(c);
+  if (xs_tiled < 0 || xs_tiled > 1 || (xs_tiled && c->g.nxh % 8)) {
+    c->err = "les_set_layout: xs_tiled is 0 or 1, and 1 needs nx a multiple of 16";
+    return 1;
+  }
+  c->tune.xs_tiled = xs_tiled;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  DONE(c);
+}
+int les_get_layout(les_ctx* c) { return c ? c->tune.xs_tiled : -1; }
 int les_set_tune(les_ctx* c, const int* in8) {
   GUARD(c);
   if (!in8) return -1;

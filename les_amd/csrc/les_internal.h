@@ -84,6 +84,8 @@ struct Tune {
   int pad_mode[2] = {0, 0}; // padded products: 0 level groups per workgroup, 1 streaming level runs,
                             // 2 streaming with one transform per wavefront (convective step with
                             // both scalars at nx2 = 384; elsewhere as 1)
+  int xs_tiled = 0;         // x-spectral intermediates between the row and column passes: 0 row-major,
+                            // 1 8-coefficient column tiles (xs_off; set at creation where nx/2 % 8 == 0)
 };
 
 // Scalar transport switches and constants (param.f90:53,72-90,139-146,174),

@@ -44,9 +44,14 @@ static const cx<T>* spectral_deriv(les_ctx* c, const T* const* fs, int nf, int n
   cx<T>* X1 = (cx<T>*)scratch(c, keep_slot >= 0 ? keep_slot : S_XS1, fo * nout * sizeof(cx<T>));
   LoadFields<T> L{{fs[0], nf > 1 ? fs[1] : nullptr, nf > 2 ? fs[2] : nullptr, nullptr}, nlev, g.ny, g.ld,
                   T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, rows * nf, g.nx, g.nxh, g.ny);
+  const int tl = xs_layout(c, g.nxh);
+  launch_row_fwd<T>(c, L, X0, g.nxh, rows * nf, g.nx, g.nxh, g.ny, tl);
   ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
-  for (int j = 0; j < nout; ++j) col_out(a, j, kinds[j], g.nxh, g.nxh, (long)g.ny * g.nxh);
+  a.in_tl = tl;
+  for (int j = 0; j < nout; ++j) {
+    col_out(a, j, kinds[j], g.nxh, g.nxh, (long)g.ny * g.nxh);
+    a.out_tl[j] = tl;
+  }
   launch_col<T>(c, X0, nullptr, X1, X1 + fo, X1 + 2 * fo, a, (long)nlev * nf, g.nxh);
   // output rows are ordered (kind j, field f): outs[j*nf + f]
   StoreFields<T> S{};
@@ -55,7 +60,7 @@ static const cx<T>* spectral_deriv(les_ctx* c, const T* const* fs, int nf, int n
   S.ny = g.ny;
   S.ld = g.ld;
   S.nx = g.nx;
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nf * nout, g.nx, g.ld, g.ny);
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows * nf * nout, g.nx, g.ld, g.ny, nullptr, T(1), tl);
   return X1;
 }
 
@@ -374,16 +379,18 @@ void op_divstress3(les_ctx* c, T* const divt[3], T* const tau[6]) {
   const T* in[6] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4]};
   for (int q = 0; q < 6; ++q) L.p[q] = in[q];
   L.ppf = nlev; L.ny = g.ny; L.ld = g.ld; L.scale = T(1.0 / ((double)g.nx * g.ny));
-  launch_row_fwd<T>(c, L, X0, g.nxh, 6 * rows, g.nx, g.nxh, g.ny);
+  const int tl = xs_layout(c, g.nxh);
+  launch_row_fwd<T>(c, L, X0, g.nxh, 6 * rows, g.nx, g.nxh, g.ny, tl);
   ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
   a.in2_plane = a.in_plane;
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  a.in_tl = a.out_tl[0] = tl;
   launch_col<T>(c, X0, X0 + (size_t)3 * rows * g.nxh, X1, nullptr, nullptr, a, 3L * nlev, g.nxh);
   StoreDiv3<T> S{{StoreDiv<T>{divt[0], tau[2], g.ny, g.ld, g.nx, g.nz, 0, T(1.0 / g.dz)},
                   StoreDiv<T>{divt[1], tau[4], g.ny, g.ld, g.nx, g.nz, 0, T(1.0 / g.dz)},
                   StoreDiv<T>{divt[2], tau[5], g.ny, g.ld, g.nx, g.nz, 1, T(1.0 / g.dz)}},
                  nlev};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, 3 * rows, g.nx, g.ld, g.ny);
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, 3 * rows, g.nx, g.ld, g.ny, nullptr, T(1), tl);
 }
 
 template <typename T>
@@ -396,13 +403,15 @@ void op_divstress(les_ctx* c, T* divt, const T* tx, const T* ty, const T* tz, in
   cx<T>* X0 = (cx<T>*)scratch(c, S_XS0, 2 * xs);
   cx<T>* X1 = (cx<T>*)scratch(c, S_XS1, xs);
   LoadFields<T> L{{tx, ty, nullptr, nullptr}, nlev, g.ny, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-  launch_row_fwd<T>(c, L, X0, g.nxh, 2 * rows, g.nx, g.nxh, g.ny);
+  const int tl = xs_layout(c, g.nxh);
+  launch_row_fwd<T>(c, L, X0, g.nxh, 2 * rows, g.nx, g.nxh, g.ny, tl);
   ColArgs a = col_args(g.ny, g.ny, g.nxh, g.nxh, (long)g.ny * g.nxh);
   a.in2_plane = a.in_plane;
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny * g.nxh);
+  a.in_tl = a.out_tl[0] = tl;
   launch_col<T>(c, X0, X0 + (size_t)rows * g.nxh, X1, nullptr, nullptr, a, nlev, g.nxh);
   StoreDiv<T> S{divt, tz, g.ny, g.ld, g.nx, g.nz, wnode, T(1.0 / g.dz)};
-  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld, g.ny);
+  launch_row_inv<T>(c, X1, g.nxh, g.nxh, S, rows, g.nx, g.ld, g.ny, nullptr, T(1), tl);
 }
 
 // ------------------------------------------------------------------------
@@ -434,23 +443,25 @@ static cx<T>* convec_spectral(les_ctx* c, const T* u, const T* v, const T* w, co
   // pad in y: 6*nz planes
   const long rowsB = 6L * nz * g.ny2;
   cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  const int tl = xs_layout(c, g.nxh);
   bool padded = false;
   if (si && si->uvw) {
     // the inputs from filt_da's kept spectra; the wall planes of dudz, dvdz
     // (wallstress) by one small forward transform
     cx<T>* W = (cx<T>*)scratch(c, S_FWALL, (size_t)6 * g.ny * g.nxh * sizeof(cx<T>));
     LoadPlanes6<T> LW{{dudz, dvdz, dudz, dudz, dudz, dudz}, g.ld, T(1.0 / g.nx)};
-    launch_row_fwd<T>(c, LW, W, g.nxh, 2L * g.ny, g.nx, g.nxh, g.ny);
+    launch_row_fwd<T>(c, LW, W, g.nxh, 2L * g.ny, g.nx, g.nxh, g.ny, tl);
     ConvecSpecIn<T, 0> fn{si->uvw, {nullptr, nullptr}, W, nz, g.ny, g.nxh, T(1.0 / g.ny), T(1.0 / g.dz)};
-    padded = launch_col_pad_fn<T>(c, fn, XB, 6L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.5);
+    padded = launch_col_pad_fn<T>(c, fn, XB, 6L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.5, tl);
   }
   if (!padded) {
     const long rows6 = 6L * nz * g.ny;
     cx<T>* XS = (cx<T>*)scratch(c, S_XS0, (size_t)rows6 * g.nxh * sizeof(cx<T>));
     LoadConvec<T> L{u, v, w, dudy, dudz, dvdx, dvdz, dwdx, dwdy, g.ny, nz, g.ld, T(1.0 / ((double)g.nx * g.ny))};
-    launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh, g.ny);
+    launch_row_fwd<T>(c, L, XS, g.nxh, rows6, g.nx, g.nxh, g.ny, tl);
     ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
     col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+    a.in_tl = tl;
     launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 6L * nz, g.nxh);
   }
   // padded-grid products without materialising the padded fields

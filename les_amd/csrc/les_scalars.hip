@@ -552,11 +552,13 @@ void op_scalar_rhs(les_ctx* c, const ScalarJob* jobs, int ns, const T* u, const 
     L.p[5 + 3 * q] = (const T*)jobs[q].dsdz;
   }
   L.ppf = nz; L.ny = g.ny; L.ld = g.ld; L.scale = T(1.0 / ((double)g.nx * g.ny));
-  launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
+  const int tl = xs_layout(c, g.nxh);
+  launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny, tl);
   const long rowsB = (long)nf * nz * g.ny2;
   cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
   ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
   col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+  a.in_tl = tl;
   launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, (long)nf * nz, g.nxh);
   // products on the padded grid (dealias2 scaling included)
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)ns * nl * g.ny2 * g.nxh * sizeof(cx<T>));
@@ -599,6 +601,7 @@ void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T*
   const int nz = g.nz, nl = nz - 1;
   const long rowsB = 12L * nz * g.ny2;
   cx<T>* XB = (cx<T>*)scratch(c, S_XS1, (size_t)rowsB * g.nxh * sizeof(cx<T>));
+  const int tl = xs_layout(c, g.nxh);
   bool padded = false;
   if (si && si->uvw && si->s[0] && si->s[1]) {
     // the inputs from filt_da's kept spectra; wall dudz, dvdz (wallstress) and
@@ -607,9 +610,9 @@ void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T*
     cx<T>* W = (cx<T>*)scratch(c, S_FWALL, (size_t)6 * g.ny * g.nxh * sizeof(cx<T>));
     const T *d0 = (const T*)jobs[0].dsdz, *d1 = (const T*)jobs[1].dsdz;
     LoadPlanes6<T> LW{{gr[2], gr[5], d0, d0 + top, d1, d1 + top}, g.ld, T(1.0 / g.nx)};
-    launch_row_fwd<T>(c, LW, W, g.nxh, 6L * g.ny, g.nx, g.nxh, g.ny);
+    launch_row_fwd<T>(c, LW, W, g.nxh, 6L * g.ny, g.nx, g.nxh, g.ny, tl);
     ConvecSpecIn<T, 2> fn{si->uvw, {si->s[0], si->s[1]}, W, nz, g.ny, g.nxh, T(1.0 / g.ny), T(1.0 / g.dz)};
-    padded = launch_col_pad_fn<T>(c, fn, XB, 12L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.6);
+    padded = launch_col_pad_fn<T>(c, fn, XB, 12L * nz, g.nxh, g.nxh, (long)g.ny2 * g.nxh, 1.6, tl);
   }
   if (!padded) {
     const long rows = 12L * nz * g.ny;
@@ -622,9 +625,10 @@ void op_convec_scalar_ab2(les_ctx* c, T* u, T* v, T* w, const T* const gr[9], T*
       L.gs[3 * q + 1] = (const T*)jobs[q].dsdy;
       L.gs[3 * q + 2] = (const T*)jobs[q].dsdz;
     }
-    launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny);
+    launch_row_fwd<T>(c, L, XS, g.nxh, rows, g.nx, g.nxh, g.ny, tl);
     ColArgs a = col_args(g.ny, g.ny2, g.nxh, g.nxh, (long)g.ny * g.nxh);
     col_out(a, 0, K_ID, g.nxh, g.nxh, (long)g.ny2 * g.nxh);
+    a.in_tl = tl;
     launch_col<T>(c, XS, nullptr, XB, nullptr, nullptr, a, 12L * nz, g.nxh);
   }
   cx<T>* XP = (cx<T>*)scratch(c, S_XS2, (size_t)5 * nl * g.ny2 * g.nxh * sizeof(cx<T>));

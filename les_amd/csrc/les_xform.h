@@ -39,6 +39,37 @@ __device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
 // it by less than (e/d + 1) 2^-23 < 1/(2d), so the truncation is exact
 __device__ __forceinline__ int div_small(int e, float inv_d) { return (int)(((float)e + 0.5f) * inv_d); }
 
+// Offset of element (y, k) in a plane of ny x-spectral rows of nk
+// coefficients: row-major (y nk + k), or tiled, where each 8-coefficient
+// column tile of the plane is contiguous ((k / 8) 8 ny + 8 y + k % 8, nk a
+// multiple of 8).  In the tiled layout the two rows of a row-kernel pair fill
+// whole 128-byte lines of every tile and a column kernel's 8-column tile is
+// one contiguous block, where the row-major layout gives a column kernel
+// 64-byte row segments (FP32).
+__host__ __device__ __forceinline__ long xs_off(int y, int k, int ny, long nk, int tiled) {
+  return tiled ? (long)(k >> 3) * (8L * ny) + 8 * y + (k & 7) : (long)y * nk + k;
+}
+
+
+// The r2c split of a packed pair of real rows: coefficient k of row 2pp
+// (h = 0) or of row 2pp + 1 (h = 1) from Z_k and Z_{n-k} of their complex FFT.
+template <typename T>
+__device__ __forceinline__ cx<T> r2c_half(cx<T> zk, cx<T> zn, int h) {
+  return h == 0 ? cx<T>{T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)}
+                : cx<T>{T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
+}
+// Tiled output of a row pair: element f of the pair's 2 ceil8(kxo) slots is
+// coefficient k = 8 (f / 16) + f % 8 of row 2pp + (f / 8) % 2, at offset
+// (f / 16) 8 ny + 16 pp + f % 16 of the plane (xs_off), so 16 consecutive
+// threads write one whole 128-byte line (FP32) rather than two halves of it
+// in two store instructions.
+__device__ __forceinline__ int tiled_k(int f) { return 8 * (f >> 4) + (f & 7); }
+__device__ __forceinline__ long tiled_pair_off(int f, int pp, int ny) {
+  return (long)(f >> 4) * (8L * ny) + 16 * pp + (f & 15);
+}
+
+// the x-spectral layout of rows of nk coefficients: the context's, where nk allows tiles
+static inline int xs_layout(const les_ctx* c, long nk) { return c->tune.xs_tiled && nk % 8 == 0 ? 1 : 0; }
 
 template <typename T>
 static const cx<T>* tw_for(const les_ctx* c, int n) {
@@ -82,7 +113,7 @@ using RowGeomF = RowGeom<N, sizeof(T) == 4>;  // forward (r2c) row kernels: radi
 template <typename T, int N, class Loader>
 __global__ void __launch_bounds__((N > 0 ? RowGeomF<T, N>::BLOCK : 1024))
     k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
-              int kxo) {
+              int kxo, int tiled) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
   const int n = N > 0 ? N : p.n;
@@ -134,16 +165,27 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomF<T, N>::BLOCK : 1024))
     fft_lds_c<T, N, 1, RS, RowGeomF<T, N>::P, RowGeomF<T, N>::BLOCK, false, true, RowGeomF<T, N>::PLAN>(buf, tw, -1);
   else
     fft_lds<T, false, 2>(buf, p, 1, n, P, tw, -1);
+  if (tiled) {
+    // 16-thread groups walk the block's P pairs within a tile first, so a
+    // tile's lines from this block (P consecutive pairs) go out together
+    const int per = 2 * ((kxo + 7) & ~7);
+    cx<T>* op = out + (long)plane * ny * orow;
+    for (int e = threadIdx.x; e < P * per; e += BLOCK) {
+      const int grp = e >> 4, t = grp % P, f = 16 * (grp / P) + (e & 15), pp = pair0 + t, k = tiled_k(f);
+      if (pp >= npp || k >= kxo) continue;
+      const cx<T> zk = buf[ix(t, k)], zn = buf[ix(t, k ? n - k : 0)];
+      op[tiled_pair_off(f, pp, ny)] = r2c_half(zk, zn, (f >> 3) & 1);
+    }
+    return;
+  }
   const float inv_kxo = 1.0f / (float)kxo;
   for (int e = threadIdx.x; e < P * kxo; e += BLOCK) {
     const int t = div_small(e, inv_kxo), k = e - t * kxo, pp = pair0 + t;
     if (pp >= npp) continue;
     const cx<T> zk = buf[ix(t, k)], zn = buf[ix(t, k ? n - k : 0)];
-    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
-    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
     const long row = (long)plane * ny + 2 * pp;
-    out[row * orow + k] = A;
-    out[(row + 1) * orow + k] = B;
+    out[row * orow + k] = r2c_half(zk, zn, 0);
+    out[(row + 1) * orow + k] = r2c_half(zk, zn, 1);
   }
 }
 
@@ -166,7 +208,7 @@ using RowGeomI = RowGeom<N, sizeof(T) == 4 && !StorerHasPre<S>::value>;
 template <typename T, int N, class Storer>
 __global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024))
     k_row_inv(const cx<T>* __restrict__ in, long irow, int kxi, Storer S, int ny, FftPlan p,
-              const cx<T>* __restrict__ twg, int ldo, const cx<T>* __restrict__ in2, T c2) {
+              const cx<T>* __restrict__ twg, int ldo, const cx<T>* __restrict__ in2, T c2, int tiled) {
   using RG = RowGeomI<T, N, Storer>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
@@ -191,12 +233,13 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024)
     const int pp = pair0 + t;
     cx<T> z = {T(0), T(0)};
     if (pp < npp) {
-      const unsigned a = (unsigned)(2 * pp) * (unsigned)irow, b = a + (unsigned)irow;
+      const int a = 2 * pp, b = a + 1;
       // optional second spectral input added with weight c2 (same layout)
-      auto get = [&](unsigned r, int jj) {
-        cx<T> v = inp[r + (unsigned)jj];
+      auto get = [&](int r, int jj) {
+        const unsigned o = (unsigned)xs_off(r, jj, ny, irow, tiled);
+        cx<T> v = inp[o];
         if (in2) {
-          const cx<T> w = in2p[r + (unsigned)jj];
+          const cx<T> w = in2p[o];
           v = {v.x + c2 * w.x, v.y + c2 * w.y};
         }
         return v;
@@ -227,11 +270,11 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024)
       const int t = e / H, j = e - t * H, pp = pair0 + t;
       za[i] = zb[i] = cx<T>{T(0), T(0)};
       if ((NI % BL == 0 || e < NI) && pp < npp && j < kxi) {
-        const unsigned a = (unsigned)(2 * pp) * (unsigned)irow, b = a + (unsigned)irow;
-        za[i] = inp[a + (unsigned)j];
-        zb[i] = inp[b + (unsigned)j];
+        const unsigned a = (unsigned)xs_off(2 * pp, j, ny, irow, tiled), b = (unsigned)xs_off(2 * pp + 1, j, ny, irow, tiled);
+        za[i] = inp[a];
+        zb[i] = inp[b];
         if (in2) {
-          const cx<T> wa = in2p[a + (unsigned)j], wb = in2p[b + (unsigned)j];
+          const cx<T> wa = in2p[a], wb = in2p[b];
           za[i] = {za[i].x + c2 * wa.x, za[i].y + c2 * wa.y};
           zb[i] = {zb[i].x + c2 * wb.x, zb[i].y + c2 * wb.y};
         }
@@ -340,7 +383,8 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomI<T, N, Storer>::BLOCK : 1024)
 #define LES_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(384)
 
 template <typename T, int N, class L>
-static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo) {
+static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, long nplanes, int ny, int n, int kxo,
+                             int tiled) {
   const FftPlan& p = plan_for(c, n);
   int P = 1, threads = 64;
   if constexpr (N > 0) {
@@ -355,21 +399,26 @@ static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, lon
   const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
   hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
-                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo);
+                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo, tiled);
   LCHK(c, hipGetLastError());
 }
 
 template <typename T, class L>
-static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo, int ny) {
+static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long nrows, int n, int kxo, int ny,
+                           int tiled = 0) {
   const long nplanes = nrows / ny;
+  if (tiled && orow % 8) {
+    c->err = "row forward: a tiled x-spectral layout needs rows of a multiple of 8 coefficients";
+    return;
+  }
   switch (n) {
 #define LES_CASE(NN) \
   case NN:           \
-    return launch_row_fwd_n<T, NN>(c, ld, out, orow, nplanes, ny, n, kxo);
+    return launch_row_fwd_n<T, NN>(c, ld, out, orow, nplanes, ny, n, kxo, tiled);
     LES_FFT_SIZES(LES_CASE)
 #undef LES_CASE
     default:
-      return launch_row_fwd_n<T, 0>(c, ld, out, orow, nplanes, ny, n, kxo);
+      return launch_row_fwd_n<T, 0>(c, ld, out, orow, nplanes, ny, n, kxo, tiled);
   }
 }
 
@@ -382,7 +431,7 @@ static void launch_row_fwd(les_ctx* c, const L& ld, cx<T>* out, long orow, long 
 template <typename T, int N, class ML>
 __global__ void __launch_bounds__(256)
     k_row_fwd_mf(ML L, cx<T>* __restrict__ out, long orow, int ny, int nz, const cx<T>* __restrict__ twg,
-                 int kxo) {
+                 int kxo, int tiled) {
   constexpr int NF = ML::NF, RS = lds_row(N), BLOCK = 256;
   constexpr int NP = 2 * N, MP = (NP + BLOCK - 1) / BLOCK;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -414,15 +463,24 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   fft_lds_c<T, N, 1, RS, NF, BLOCK, false, true, RowGeomF<T, N>::PLAN>(buf, tw, -1);
+  if (tiled) {
+    const int per = 2 * ((kxo + 7) & ~7);
+    const float inv_per = 1.0f / (float)per;
+    for (int e = threadIdx.x; e < NF * per; e += BLOCK) {
+      const int f = div_small(e, inv_per), q = e - f * per, kk = tiled_k(q);
+      if (kk >= kxo) continue;
+      const cx<T> zk = buf[f * RS + lds_pad(kk)], zn = buf[f * RS + lds_pad(kk ? N - kk : 0)];
+      out[((long)f * nz + k) * ny * orow + tiled_pair_off(q, blockIdx.x, ny)] = r2c_half(zk, zn, (q >> 3) & 1);
+    }
+    return;
+  }
   const float inv_kxo = 1.0f / (float)kxo;
   for (int e = threadIdx.x; e < NF * kxo; e += BLOCK) {
     const int f = div_small(e, inv_kxo), kk = e - f * kxo;
     const cx<T> zk = buf[f * RS + lds_pad(kk)], zn = buf[f * RS + lds_pad(kk ? N - kk : 0)];
-    const cx<T> A = {T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)};
-    const cx<T> B = {T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)};
     const long row = ((long)f * nz + k) * ny + y0;
-    out[row * orow + kk] = A;
-    out[(row + 1) * orow + kk] = B;
+    out[row * orow + kk] = r2c_half(zk, zn, 0);
+    out[(row + 1) * orow + kk] = r2c_half(zk, zn, 1);
   }
 }
 
@@ -430,8 +488,13 @@ __global__ void __launch_bounds__(256)
 // LDS tile does not fit): the caller then runs the plane-batched kernel
 // nlev levels (the grid's nz when < 0), output rows ((f nlev + k) ny + y)
 template <typename T, class ML>
-static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, double read_fields, int nlev = -1) {
+static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, double read_fields, int nlev = -1,
+                              int tiled = 0) {
   const Grid& g = c->g;
+  if (tiled && kxo % 8) {
+    c->err = "row forward: a tiled x-spectral layout needs rows of a multiple of 8 coefficients";
+    return true;
+  }
   const int n = g.nx;
   const int nl = nlev < 0 ? g.nz : nlev;
   auto go = [&](auto nn) -> bool {
@@ -446,7 +509,7 @@ static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, doub
       ProfScope ps(c, "row_fwd",
                    (double)nl * g.ny * (n * sizeof(T) * read_fields + 2.0 * ML::NF * kxo * sizeof(cx<T>) / 2));
       hipLaunchKernelGGL((k_row_fwd_mf<T, N, ML>), dim3((unsigned)(g.ny / 2), (unsigned)nl), dim3(256), sm,
-                         c->stream, L, out, (long)kxo, g.ny, nl, tw_for<T>(c, N), kxo);
+                         c->stream, L, out, (long)kxo, g.ny, nl, tw_for<T>(c, N), kxo, tiled);
       LCHK(c, hipGetLastError());
       return true;
     }
@@ -464,7 +527,7 @@ static bool launch_row_fwd_mf(les_ctx* c, const ML& L, cx<T>* out, int kxo, doub
 
 template <typename T, int N, class S>
 static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nplanes, int ny,
-                             int n, int ldo, const cx<T>* in2, T c2) {
+                             int n, int ldo, const cx<T>* in2, T c2, int tiled) {
   const FftPlan& p = plan_for(c, n);
   int P = 1, threads = 64;
   if constexpr (N > 0) {
@@ -488,7 +551,7 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
   ProfScope ps(c, "row_inv",
                (double)nplanes * ny * (kxi * sizeof(cx<T>) * (in2 ? 2 : 1) + ldo * sizeof(T) * st.writes()));
   hipLaunchKernelGGL((k_row_inv<T, N, S>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
-                     in, irow, kxi, st, ny, p, tw_for<T>(c, n), ldo, in2, c2);
+                     in, irow, kxi, st, ny, p, tw_for<T>(c, n), ldo, in2, c2, tiled);
   LCHK(c, hipGetLastError());
 }
 
@@ -496,16 +559,20 @@ static void launch_row_inv_n(les_ctx* c, const cx<T>* in, long irow, int kxi, co
 // weight c2 before the inverse transform
 template <typename T, class S>
 static void launch_row_inv(les_ctx* c, const cx<T>* in, long irow, int kxi, const S& st, long nrows, int n,
-                           int ldo, int ny, const cx<T>* in2 = nullptr, T c2 = T(1)) {
+                           int ldo, int ny, const cx<T>* in2 = nullptr, T c2 = T(1), int tiled = 0) {
   const long nplanes = nrows / ny;
+  if (tiled && irow % 8) {
+    c->err = "row inverse: a tiled x-spectral layout needs rows of a multiple of 8 coefficients";
+    return;
+  }
   switch (n) {
 #define LES_CASE(NN) \
   case NN:           \
-    return launch_row_inv_n<T, NN>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2);
+    return launch_row_inv_n<T, NN>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2, tiled);
     LES_FFT_SIZES(LES_CASE)
 #undef LES_CASE
     default:
-      return launch_row_inv_n<T, 0>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2);
+      return launch_row_inv_n<T, 0>(c, in, irow, kxi, st, nplanes, ny, n, ldo, in2, c2, tiled);
   }
 }
 
@@ -560,6 +627,7 @@ struct ColArgs {
   long in_row, in_plane, in2_plane;
   long out_row[3], out_plane[3];
   int lhG;               // row length of the G tables (lh)
+  int in_tl, out_tl[3];  // x-spectral layout of the input(s) / each output: 0 row-major, 1 tiled (xs_off)
 };
 
 __device__ __forceinline__ int map_row(int yo, int nyi, int nyo) {
@@ -660,7 +728,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
         rg[i] = cx<T>{T(0), T(0)};
         if (NI % BLK == 0 || e < NI) {
           const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-          if (kk < a.nki) rg[i] = src[(long)y * a.in_row + kk];
+          if (kk < a.nki) rg[i] = src[xs_off(y, kk, nyi, a.in_row, a.in_tl)];
         }
       }
 #pragma unroll
@@ -671,7 +739,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     } else {
       for (int e = threadIdx.x; e < nyi * TX; e += bd) {
         const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-        dst[e] = kk < a.nki ? src[(long)y * a.in_row + kk] : cx<T>{T(0), T(0)};
+        dst[e] = kk < a.nki ? src[xs_off(y, kk, nyi, a.in_row, a.in_tl)] : cx<T>{T(0), T(0)};
       }
     }
   };
@@ -702,7 +770,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     inv(R);
     for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = R[e];
+      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + xs_off(y, kk, nyo, a.out_row[0], a.out_tl[0])] = R[e];
     }
     return;
   }
@@ -719,7 +787,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
         r[i] = cx<T>{T(0), T(0)};
         if (NE % BLK == 0 || e < NE) {
           const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-          if (kk < a.nki) r[i] = src[(long)y * a.in_row + kk];
+          if (kk < a.nki) r[i] = src[xs_off(y, kk, nyi, a.in_row, a.in_tl)];
         }
       }
     };
@@ -770,7 +838,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       if (a.do_inv) inv(R);
       for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
         const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-        if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = R[e];
+        if (kk < a.nko[0]) o0[pl * a.out_plane[0] + xs_off(y, kk, nyo, a.out_row[0], a.out_tl[0])] = R[e];
       }
       return;
     }
@@ -813,7 +881,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
       for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
         const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-        if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = R[e];
+        if (kk < a.nko[j]) o[pl * a.out_plane[j] + xs_off(y, kk, nyo, a.out_row[j], a.out_tl[j])] = R[e];
       }
       if (j == jid && jkx >= 0 && kx0 < a.nko[jkx]) {
         cx<T>* ox = jkx == 0 ? o0 : (jkx == 1 ? o1 : o2);
@@ -822,7 +890,8 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
         for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
           const int y = e / TX, kk = kx0 + c0;
           const cx<T> v = R[e];
-          if (kk < a.nko[jkx]) ox[pl * a.out_plane[jkx] + (long)y * a.out_row[jkx] + kk] = cx<T>{-k * v.y, k * v.x};
+          if (kk < a.nko[jkx])
+            ox[pl * a.out_plane[jkx] + xs_off(y, kk, nyo, a.out_row[jkx], a.out_tl[jkx])] = cx<T>{-k * v.y, k * v.x};
         }
       }
     }
@@ -876,7 +945,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     if (a.do_inv) inv(A);
     for (int e = threadIdx.x; e < nyo * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + (long)y * a.out_row[0] + kk] = A[e];
+      if (kk < a.nko[0]) o0[pl * a.out_plane[0] + xs_off(y, kk, nyo, a.out_row[0], a.out_tl[0])] = A[e];
     }
     return;
   }
@@ -908,7 +977,7 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
     for (int e = threadIdx.x; e < nyo * TX; e += bd) {
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-      if (kk < a.nko[j]) o[pl * a.out_plane[j] + (long)y * a.out_row[j] + kk] = B[e];
+      if (kk < a.nko[j]) o[pl * a.out_plane[j] + xs_off(y, kk, nyo, a.out_row[j], a.out_tl[j])] = B[e];
     }
     __syncthreads();
   }
@@ -1026,7 +1095,7 @@ static void launch_col(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o0,
 template <typename T, int TX, int NYI, int NYO, int PLAN, class Fn>
 __global__ void __launch_bounds__((ColGeomT<T, TX, NYI, NYO>::BLOCK))
     k_col_pad_fn(Fn fn, cx<T>* __restrict__ out, int nki, long out_row, long out_plane,
-                 const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout) {
+                 const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout, int in_tl) {
   static_assert(NYO > NYI && NYO < 2 * NYI, "3/2 zero-pad pass");
   constexpr int BLK = ColGeomT<T, TX, NYI, NYO>::BLOCK;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1049,7 +1118,7 @@ __global__ void __launch_bounds__((ColGeomT<T, TX, NYI, NYO>::BLOCK))
       const int e = threadIdx.x + i * BLK;
       const int y = e / TX, c = e - y * TX, kk = kx0 + c;
       const bool ok = (NI % BLK == 0 || e < NI) && kk < nki;
-      const long off = (long)y * nki + kk;
+      const long off = ok ? xs_off(y, kk, NYI, nki, in_tl) : 0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) raw[i][q] = (ok && q < lc.n) ? lc.p[q][off] : cx<T>{T(0), T(0)};
     }
@@ -1090,7 +1159,7 @@ __global__ void __launch_bounds__((ColGeomT<T, TX, NYI, NYO>::BLOCK))
 
 template <typename T, int TX, int NYI, int NYO, int PLAN, class Fn>
 static bool launch_col_pad_fn_n(les_ctx* c, const Fn& fn, cx<T>* out, long nplanes, int nki, long out_row,
-                                long out_plane, double reads_per_elem) {
+                                long out_plane, double reads_per_elem, int in_tl) {
   using G = ColGeomT<T, TX, NYI, NYO>;
   const size_t sm = (size_t)(NYI + NYO + TX * NYO) * sizeof(cx<T>);
   if (sm > 65536)
@@ -1100,7 +1169,7 @@ static bool launch_col_pad_fn_n(les_ctx* c, const Fn& fn, cx<T>* out, long nplan
   const double es = 2.0 * sizeof(T);
   ProfScope ps(c, "col_pad", (double)nplanes * nki * es * (NYI * reads_per_elem + NYO));
   hipLaunchKernelGGL((k_col_pad_fn<T, TX, NYI, NYO, PLAN, Fn>), dim3(gx, (unsigned)nplanes), dim3(G::BLOCK), sm,
-                     c->stream, fn, out, nki, out_row, out_plane, tw_for<T>(c, NYI), tw_for<T>(c, NYO));
+                     c->stream, fn, out, nki, out_row, out_plane, tw_for<T>(c, NYI), tw_for<T>(c, NYO), in_tl);
   LCHK(c, hipGetLastError());
   return true;
 }
@@ -1108,14 +1177,16 @@ static bool launch_col_pad_fn_n(les_ctx* c, const Fn& fn, cx<T>* out, long nplan
 // false where no compile-time variant exists for (ny, 3 ny / 2)
 template <typename T, class Fn>
 static bool launch_col_pad_fn(les_ctx* c, const Fn& fn, cx<T>* out, long nplanes, int nki, long out_row,
-                              long out_plane, double reads_per_elem) {
+                              long out_plane, double reads_per_elem, int in_tl = 0) {
   constexpr int TX = sizeof(T) == 8 ? 8 : 16;
   const int plan = sizeof(T) == 4 ? 2 : c->tune.col_plan[1];
 #define LES_CASE(I, O)                                                                                        \
   if (c->g.ny == I && c->g.ny2 == O) {                                                                         \
     if (plan == 2)                                                                                             \
-      return launch_col_pad_fn_n<T, TX, I, O, 2>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem); \
-    return launch_col_pad_fn_n<T, TX, I, O, 0>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem);   \
+      return launch_col_pad_fn_n<T, TX, I, O, 2>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem, \
+                                                 in_tl);                                                          \
+    return launch_col_pad_fn_n<T, TX, I, O, 0>(c, fn, out, nplanes, nki, out_row, out_plane, reads_per_elem,   \
+                                               in_tl);                                                            \
   }
   LES_CASE(16, 24) LES_CASE(32, 48) LES_CASE(64, 96) LES_CASE(96, 144) LES_CASE(128, 192) LES_CASE(256, 384)
 #undef LES_CASE

@@ -988,6 +988,17 @@ module les_amd_capi
       type(c_ptr), value :: c
       type(c_ptr), value :: in8
     end function
+    integer(c_int) function les_set_layout(c, xs_tiled) bind(C, name='les_set_layout')
+      import
+      implicit none
+      type(c_ptr), value :: c
+      integer(c_int), value :: xs_tiled
+    end function
+    integer(c_int) function les_get_layout(c) bind(C, name='les_get_layout')
+      import
+      implicit none
+      type(c_ptr), value :: c
+    end function
     integer(c_int) function les_ensemble_create(members, n, out) bind(C, name='les_ensemble_create')
       import
       implicit none

@@ -146,3 +146,49 @@ def test_set_tune_rejects_bad_shapes(torch_mod):
         g.set_tune(tri_mode=4, tri_block=512)   # four threads per mode: at most 256 modes per block
     with pytest.raises(LesError):
         g.set_tune(pad_mode_fp32=3)
+
+
+def run_layout(prec, layout, tune=None, n=32, ny=None, nz=17, steps=12, smodel=5):
+    from les_amd.api import LesSim, checkerboard_flux
+    ny = ny or n
+    s = LesSim(n, ny, nz, precision=prec, dt=2e-4, sgs_model=6 if smodel == 6 else 5,
+               scalars=dict(wt_s=checkerboard_flux(n, ny), dTdz_top=0.01, model=smodel))
+    s.init_ic(seed=3)
+    s.init_scalar_ic(seed=3)
+    if tune:
+        s.set_tune(**tune)
+    s.set_layout(layout)
+    assert s.get_layout() == layout
+    s.step(steps)
+    return s
+
+
+@pytest.mark.parametrize("prec,n,ny,tune,smodel", [
+    ("fp32", 32, 32, None, 7), ("fp64", 32, 32, None, 6),
+    ("fp32", 48, 32, dict(col_tx_fp32=8), 5), ("fp32", 32, 48, dict(col_tx_fp32=32), 7),
+    ("fp32", 32, 48, dict(col_tx_fp32=16, col_plan_fp32=2), 6),
+    ("fp64", 48, 32, dict(col_tx_fp64=4), 7), ("fp64", 32, 32, dict(col_tx_fp64=16), 5),
+    ("fp64", 48, 48, dict(col_tx_fp64=8, col_plan_fp64=2), 7),
+    ("fp32", 256, 256, dict(col_tx_fp32=8, col_plan_fp32=2, pad_mode_fp32=2), 7),
+    ("fp64", 256, 128, dict(col_tx_fp64=8, pad_mode_fp64=1), 7),
+])
+def test_tiled_spectral_layout_is_bit_identical(torch_mod, prec, n, ny, tune, smodel):
+    """The tiled x-spectral layout moves only addresses: twelve convective
+    steps (two dynamic updates; filt_da's kept spectra feeding the zero-pad
+    pass of the convective products, divstress; every column tile width and
+    plan) give bit-identical states."""
+    print("tiled layout", prec, n, ny, tune, smodel, flush=True)
+    a = run_layout(prec, 0, tune, n, ny, smodel=smodel)
+    b = run_layout(prec, 1, tune, n, ny, smodel=smodel)
+    for name in FIELDS + ("theta", "qmix", "RHS_T"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+
+
+def test_tiled_layout_needs_nx_multiple_of_16(torch_mod):
+    from les_amd.api import LesSim
+    s = LesSim(24, 24, 9, precision="fp32", dt=2e-4)
+    with pytest.raises(RuntimeError):
+        s.set_layout(1)
+    with pytest.raises(RuntimeError):
+        s.set_layout(2)
+    assert s.get_layout() == 0
