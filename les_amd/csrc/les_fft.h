@@ -524,7 +524,11 @@ __device__ __forceinline__ void wave_stage(cx<T>* row, const TW& tw, int lane, i
 // a 384-point transform it puts butterfly groups whose writes fall on
 // disjoint LDS banks in one 16-lane write group), and RTW takes the stage's
 // twiddles, already conjugated for sign +1, from the registers rtw[r - 1].
-template <typename T, int N, int R, int NS, int OFF, bool SWAP34, bool RTW, class TW>
+// ZLO <= ZHI: the row is known to hold zeros at indices [ZLO, ZHI] (the
+// first pass of a zero-padded spectrum), which are taken as zero without
+// being read (the same arithmetic on the same values; those words need not
+// be written).
+template <typename T, int N, int R, int NS, int OFF, bool SWAP34, bool RTW, int ZLO = 0, int ZHI = -1, class TW>
 __device__ __forceinline__ void wave_stage_lane(cx<T>* row, const TW& tw, int lane, int sign, const cx<T>* rtw) {
   constexpr int NB = N / R;
   static_assert(NB <= 64, "one butterfly per lane");
@@ -539,7 +543,13 @@ __device__ __forceinline__ void wave_stage_lane(cx<T>* row, const TW& tw, int la
     const int k = b % NS;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      cx<T> a = row[at_rd(b, r * NB)];
+      cx<T> a = {T(0), T(0)};
+      if (ZLO > ZHI || r * NB + NB - 1 < ZLO || r * NB > ZHI) {
+        a = row[at_rd(b, r * NB)];
+      } else if (r * NB < ZLO || r * NB + NB - 1 > ZHI) {  // part of this read lies in the zero band
+        const int e = b + r * NB;
+        if (e < ZLO || e > ZHI) a = row[at_rd(b, r * NB)];
+      }
       if constexpr (NS > 1) {
         if (r > 0) {
           cx<T> wv;

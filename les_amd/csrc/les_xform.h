@@ -2037,22 +2037,22 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
       row[lds_pad(N - j)] = {A.x + B.y, B.x - A.y};
     }
   };
+  // the zero band [NXH, N - NXH] is not written: the first pass of a
+  // staged row takes it as zero without reading it
   auto stage_row = [&](cx<T>* row) {
     put(row, 2 * lane, ra.a, rb.a);
     put(row, 2 * lane + 1, ra.b, rb.b);
-    constexpr int NZB = N - 2 * NXH + 1;
-#pragma unroll
-    for (int e = 0; e < NZB; e += 64)
-      if (e + lane < NZB) row[lds_pad(NXH + e + lane)] = {T(0), T(0)};
   };
   // the first two passes (radix 8, 8): the second with the bank-disjoint
-  // butterfly order; all three passes; the last pass with register twiddles
+  // butterfly order; all three passes of a staged row; the last pass with
+  // register twiddles
   auto passes12 = [&](cx<T>* row) {
     wave_stage_lane<T, N, 8, 1, 0, false, false>(row, tw, lane, +1, rtw);
     wave_stage_lane<T, N, 8, 8, OFF1, true, false>(row, tw, lane, +1, rtw);
   };
   auto transform = [&](cx<T>* row) {
-    passes12(row);
+    wave_stage_lane<T, N, 8, 1, 0, false, false, NXH, N - NXH>(row, tw, lane, +1, rtw);
+    wave_stage_lane<T, N, 8, 8, OFF1, true, false>(row, tw, lane, +1, rtw);
     wave_stage_lane<T, N, RL, 64, OFF2, false, true>(row, tw, lane, +1, rtw);
   };
   // last pass in registers, then the kx < NXH coefficients of the row's two
