@@ -1972,9 +1972,11 @@ struct PolConvecScalar {
 //   (1) the products: cx, cy, theta, q at level k into O, cz(k + 1) into C;
 //   (2) wavefronts 0-5 stage field pair w of level k + 2 into level k's slot
 //       and transform it, issuing the HBM reads of level k + 3; wavefront 6
-//       transforms O's first row and stores it, and at even steps runs the
-//       first two passes of the completed C row; wavefront 7 transforms O's
-//       second row and stores it, and at odd steps finishes the C row,
+//       transforms O's first row and stores it, and runs the first pass of
+//       a C row at the even step that completes it and the second pass at
+//       the next step; wavefront 7 transforms O's second row and stores it,
+//       and finishes that C row two steps after its completion (every
+//       wavefront then has at most four passes per level step),
 // with two workgroup barriers per level: every transform belongs to one
 // wavefront, whose passes need only wave_sync.  The last pass of each
 // product row stays in registers and stores the kx < nxh coefficients
@@ -2046,9 +2048,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
   // the first two passes (radix 8, 8): the second with the bank-disjoint
   // butterfly order; all three passes of a staged row; the last pass with
   // register twiddles
+  auto pass1 = [&](cx<T>* row) { wave_stage_lane<T, N, 8, 1, 0, false, false>(row, tw, lane, +1, rtw); };
+  auto pass2 = [&](cx<T>* row) { wave_stage_lane<T, N, 8, 8, OFF1, true, false>(row, tw, lane, +1, rtw); };
   auto passes12 = [&](cx<T>* row) {
-    wave_stage_lane<T, N, 8, 1, 0, false, false>(row, tw, lane, +1, rtw);
-    wave_stage_lane<T, N, 8, 8, OFF1, true, false>(row, tw, lane, +1, rtw);
+    pass1(row);
+    pass2(row);
   };
   auto transform = [&](cx<T>* row) {
     wave_stage_lane<T, N, 8, 1, 0, false, false, NXH, N - NXH>(row, tw, lane, +1, rtw);
@@ -2146,18 +2150,33 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
         transform(row);
       }
     } else if (w == NP) {
+      // a C row completed at an even step c: its first pass at c, its second at c + 1
       passes12(O);
       finish_store(O, S_CX, k, S_CY, k);
-      if ((c & 1) == 0) passes12(C + ((c >> 1) & 1) * RS);
+      if ((c & 1) == 0)
+        pass1(C + ((c >> 1) & 1) * RS);
+      else
+        pass2(C + ((c >> 1) & 1) * RS);
     } else {
+      // ... and its last pass at c + 2, storing cz(k0 + c), cz(k0 + c + 1)
       passes12(O + RS);
       finish_store(O + RS, S_T, k, S_Q, k);
-      if (c & 1) finish_store(C + ((c >> 1) & 1) * RS, S_CZ, k - 1, S_CZ, k);
+      if ((c & 1) == 0 && c >= 2) finish_store(C + (((c >> 1) + 1) & 1) * RS, S_CZ, k - 2, S_CZ, k - 1);
     }
     __syncthreads();
   }
-  // a run of odd length leaves cz(k1 - 1) paired with cz(k1), which another run stores
-  if ((n & 1) && w == NP + 1) finish_store(C + (((n - 1) >> 1) & 1) * RS, S_CZ, k1 - 1, S_CZ, -1);
+  // the C rows whose passes the run's last steps did not reach: completed at
+  // n - 1 (n odd; first pass done; its second level, cz(k1), another run
+  // stores) or at n - 2 (n even; two passes done)
+  if (w == NP + 1) {
+    if (n & 1) {
+      cx<T>* row = C + (((n - 1) >> 1) & 1) * RS;
+      pass2(row);
+      finish_store(row, S_CZ, k1 - 1, S_CZ, -1);
+    } else if (n >= 2) {
+      finish_store(C + (((n - 2) >> 1) & 1) * RS, S_CZ, k1 - 2, S_CZ, k1 - 1);
+    }
+  }
 }
 
 template <typename T, int N>

@@ -84,14 +84,15 @@ def test_radix8_column_plan(torch_mod, prec, tx):
     ("fp32", False, 64, 64, 17, 8, 1), ("fp64", True, 256, 256, 17, 3, 1), ("fp32", False, 256, 256, 17, 3, 1),
     ("fp64", True, 96, 64, 17, 6, 1), ("fp32", True, 96, 64, 17, 6, 1),
     ("fp32", True, 256, 256, 17, 3, 2), ("fp64", True, 256, 256, 17, 3, 2), ("fp32", True, 256, 128, 13, 3, 2),
-    ("fp64", True, 256, 128, 14, 2, 2), ("fp32", True, 256, 128, 14, 2, 2)])
+    ("fp64", True, 256, 128, 14, 2, 2), ("fp32", True, 256, 128, 14, 2, 2), ("fp32", True, 256, 128, 6, 3, 2)])
 def test_streaming_padded_products(torch_mod, prec, conv, n, ny, nz, steps, mode):
     """Streaming level runs (radix-8 plan; mode 2: one transform per
     wavefront, cz of two levels in one transform) against level groups
     (two-pass plan) for the padded products, on 64^2, 256^2, 256x128 and
     96x64 grids (padded lengths 96, 384 and 144; the 384 FP64 ring is the
     largest LDS tile): momentum with both scalars, and momentum alone.
-    nz = 13 and 14 give level runs of odd length (3) and of one level.
+    nz = 13, 14 and 6 give level runs of odd length (3), of one level and of
+    two levels.
     Tolerances as for the radix-8 column plan."""
     print("streaming padded products", prec, conv, n, ny, nz, mode, flush=True)
     key = "pad_mode_" + prec
