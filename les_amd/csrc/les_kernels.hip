@@ -1357,12 +1357,19 @@ struct LoadPress {  // fields Hx, Hy, Hz on levels 0..nz-2, scaled (const/tadv1)
   const T *u, *v, *w;
   int ny, nl, ld;
   T cst, dt;
+  int v2 = 0;  // rows aligned to two elements (set by the launcher)
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int plane, int y, int x) const {
+  __device__ T operator()(int plane, int y, int x) const { return xf(row_ptr(plane, y)[x]); }
+  __device__ T xf(T a) const { return cst * (a / dt); }
+  __device__ const T* row_ptr(int plane, int y) const {
     const int f = plane / nl, r = plane - f * nl;
-    const long i = ((long)r * ny + y) * ld + x;
     const T* p = f == 0 ? u : (f == 1 ? v : w);
-    return cst * (p[i] / dt);
+    return p + ((long)r * ny + y) * ld;
+  }
+  bool rows_aligned() const {
+    for (const T* q : {u, v, w})
+      if ((uintptr_t)q % (2 * sizeof(T))) return false;
+    return ld % 2 == 0;
   }
 };
 template <typename T>

@@ -110,6 +110,16 @@ struct RowGeom {
 template <typename T, int N>
 using RowGeomF = RowGeom<N, sizeof(T) == 4>;  // forward (r2c) row kernels: radix-8 in FP32 (measured)
 
+// Loaders whose value at (plane, y, x) is xf(row_ptr(plane, y)[x]) over rows
+// of ld contiguous elements (field reads with a pointwise scaling).  When the launcher has
+// found every row aligned to two elements (v2), the forward row kernel loads
+// x, x + 1 of a row with one two-element load: half the load instructions,
+// the same values.
+template <class L, class = void>
+struct HasRows : std::false_type {};
+template <class L>
+struct HasRows<L, std::void_t<decltype(std::declval<const L&>().row_ptr(0, 0))>> : std::true_type {};
+
 template <typename T, int N, class Loader>
 __global__ void __launch_bounds__((N > 0 ? RowGeomF<T, N>::BLOCK : 1024))
     k_row_fwd(Loader L, cx<T>* __restrict__ out, long orow, int ny, FftPlan p, const cx<T>* __restrict__ twg,
@@ -129,7 +139,37 @@ __global__ void __launch_bounds__((N > 0 ? RowGeomF<T, N>::BLOCK : 1024))
   const int plane = blockIdx.y;
   const int npp = ny / 2;
   const int pair0 = blockIdx.x * P;
-  if constexpr (N > 0) {
+  bool staged = false;
+  if constexpr (N > 0 && N % 2 == 0 && HasRows<Loader>::value) {
+    if (L.v2) {
+      constexpr int NE = RowGeomF<T, N>::P * N / 2, BL = RowGeomF<T, N>::BLOCK, MAXE = (NE + BL - 1) / BL;
+      using V2 = HIP_vector_type<T, 2>;
+      V2 a[MAXE], b[MAXE];
+#pragma unroll
+      for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * BL;
+        const int t = e / (N / 2), x = 2 * (e - t * (N / 2)), pp = pair0 + t;
+        a[i] = V2{T(0), T(0)};
+        b[i] = a[i];
+        if ((NE % BL == 0 || e < NE) && pp < npp) {
+          a[i] = *reinterpret_cast<const V2*>(L.row_ptr(plane, 2 * pp) + x);
+          b[i] = *reinterpret_cast<const V2*>(L.row_ptr(plane, 2 * pp + 1) + x);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MAXE; ++i) {
+        const int e = threadIdx.x + i * BL;
+        const int t = e / (N / 2), x = 2 * (e - t * (N / 2));
+        if (NE % BL == 0 || e < NE) {
+          buf[ix(t, x)] = cx<T>{L.xf(a[i].x), L.xf(b[i].x)};
+          buf[ix(t, x + 1)] = cx<T>{L.xf(a[i].y), L.xf(b[i].y)};
+        }
+      }
+      staged = true;
+    }
+  }
+  if (staged) {
+  } else if constexpr (N > 0) {
     // every row load of the block is issued before the first LDS store
     constexpr int NE = RowGeomF<T, N>::P * (N > 0 ? N : 1), BL = RowGeomF<T, N>::BLOCK, MAXE = (NE + BL - 1) / BL;
     cx<T> z[MAXE];
@@ -398,8 +438,10 @@ static void launch_row_fwd_n(les_ctx* c, const L& ld, cx<T>* out, long orow, lon
   const long blocks = (ny / 2 + P - 1) / P;
   const size_t sm = (size_t)(n + P * (N > 0 ? lds_row(n) : n)) * sizeof(cx<T>);
   ProfScope ps(c, "row_fwd", (double)nplanes * ny * (n * sizeof(T) * ld.reads() + kxo * sizeof(cx<T>)));
+  L lv = ld;
+  if constexpr (HasRows<L>::value) lv.v2 = lv.rows_aligned() ? 1 : 0;
   hipLaunchKernelGGL((k_row_fwd<T, N, L>), dim3((unsigned)blocks, (unsigned)nplanes), dim3(threads), sm, c->stream,
-                     ld, out, orow, ny, p, tw_for<T>(c, n), kxo, tiled);
+                     lv, out, orow, ny, p, tw_for<T>(c, n), kxo, tiled);
   LCHK(c, hipGetLastError());
 }
 
@@ -584,10 +626,18 @@ struct LoadFields {
   const T* p[4];
   int ppf, ny, ld;
   T scale;
+  int v2 = 0;  // rows aligned to two elements (set by the launcher)
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int plane, int y, int x) const {
+  __device__ T operator()(int plane, int y, int x) const { return xf(row_ptr(plane, y)[x]); }
+  __device__ T xf(T v) const { return scale * v; }
+  __device__ const T* row_ptr(int plane, int y) const {
     const int f = plane / ppf, r = plane - f * ppf;
-    return scale * (p[f] + (long)r * ny * ld)[(unsigned)(y * ld + x)];
+    return p[f] + (long)r * ny * ld + (unsigned)(y * ld);
+  }
+  bool rows_aligned() const {
+    for (const T* q : p)
+      if (q && (uintptr_t)q % (2 * sizeof(T))) return false;
+    return ld % 2 == 0;
   }
 };
 template <typename T, int NF>
@@ -595,10 +645,18 @@ struct LoadFieldsN {  // up to NF fields, plane = field * ppf + level
   const T* p[NF];
   int ppf, ny, ld;
   T scale;
+  int v2 = 0;  // rows aligned to two elements (set by the launcher)
   static constexpr double reads() { return 1.0; }
-  __device__ T operator()(int plane, int y, int x) const {
+  __device__ T operator()(int plane, int y, int x) const { return xf(row_ptr(plane, y)[x]); }
+  __device__ T xf(T v) const { return scale * v; }
+  __device__ const T* row_ptr(int plane, int y) const {
     const int f = plane / ppf, r = plane - f * ppf;
-    return scale * (p[f] + (long)r * ny * ld)[(unsigned)(y * ld + x)];
+    return p[f] + (long)r * ny * ld + (unsigned)(y * ld);
+  }
+  bool rows_aligned() const {
+    for (const T* q : p)
+      if (q && (uintptr_t)q % (2 * sizeof(T))) return false;
+    return ld % 2 == 0;
   }
 };
 template <typename T>
