@@ -2050,6 +2050,7 @@ struct PadWave {
   static constexpr size_t LDS = sizeof(cx<T>) * (size_t)RS * NROWS;
   // at 52 KB (FP32) three workgroups share a CU: 24 wavefronts, 80 registers
   static constexpr int WPE = sizeof(T) == 4 ? 6 : 1;
+  static constexpr int NCARRY = sizeof(T) == 4 ? 2 : 0;  // product rows kept in registers across levels
 };
 
 template <typename T, int N>
@@ -2172,6 +2173,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
     transform(Y + w * RS);
   }
   __syncthreads();
+  // the first NC rows (u, v, w and the vorticity) of level k + 1 at this
+  // thread's column: level k's products read them and level k + 1's take
+  // them from registers (a thread keeps its column from level to level);
+  // FP32 only, where the registers allow it without spills
+  static_assert(N <= G::BLOCK, "at most one product column per thread");
+  constexpr int NC = G::NCARRY;
+  cx<T> carry[NC > 0 ? NC : 1];
   for (int k = k0; k < k1; ++k) {
     const int c = k - k0;
     cx<T>* cur = slot(k);
@@ -2183,15 +2191,27 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
       const int blk = w < 4 ? 2 * w + (lane >> 5) + 16 * ((lane >> 4) & 1) : 8 + 4 * (w - 4) + (lane >> 4);
       const int x = 16 * blk + (lane & 15);
       T o[4];
+      cx<T> cr[NP], nr[NP];  // the field pairs of levels k and k + 1 at x (nxt row 3 is not used)
+#pragma unroll
+      for (int r = 0; r < NP; ++r)
+        if (r >= NC || c == 0) cr[r] = cur[r * RS + lds_pad(x)];
+      if (c > 0) {
+#pragma unroll
+        for (int r = 0; r < NC; ++r) cr[r] = carry[r];
+      }
+#pragma unroll
+      for (int r = 0; r < NP; ++r) nr[r] = r == 3 ? cx<T>{T(0), T(0)} : nxt[r * RS + lds_pad(x)];
       auto F = [&](int dk, int f) -> T {
-        const cx<T> z = (dk > 0 ? nxt : cur)[(f >> 1) * RS + lds_pad(x)];
+        const cx<T> z = dk > 0 ? nr[f >> 1] : cr[f >> 1];
         return (f & 1) ? z.y : z.x;
       };
       pol.product_same(k, F, o);
       O[lds_pad(x)] = {o[0], -o[1]};
       O[RS + lds_pad(x)] = {o[2], -o[3]};
       // cz(k + 1): part (c + 1) & 1 of C row ((c + 1) >> 1) & 1, conjugated
-      const T z = cz_at(nxt, cur, x);
+      const T z = PolConvec<T>{pol.cst}.cz(nr[0].x, nr[0].y, nr[1].y, nr[2].x, cr[0].x, cr[0].y);
+#pragma unroll
+      for (int r = 0; r < NC; ++r) carry[r] = nr[r];
       T* cp = reinterpret_cast<T*>(C + (((c + 1) >> 1) & 1) * RS + lds_pad(x));
       if ((c + 1) & 1)
         cp[1] = -z;
