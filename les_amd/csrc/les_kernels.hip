@@ -876,6 +876,124 @@ __global__ void k_interpolag(const T* __restrict__ ul, const T* __restrict__ vl,
   (void)n3;
 }
 
+// The same interpolation walking up the columns of a TX x TY tile through a
+// run of KC levels (blockIdx.z): the fields of levels k - 1, k, k + 1 sit in
+// an LDS ring of three level tiles with a one-point periodic halo in x and y,
+// so every neighbour value comes from LDS and each field is read from memory
+// about once (plus the halo and two levels per run) instead of once per
+// neighbour; u_lag, v_lag of the level below come from registers.  The
+// arithmetic and its order are those of k_interpolag (identical results).
+template <typename T, int NF, bool SIM, int TX>
+__global__ void __launch_bounds__(256)
+    k_interpolag_walk(const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl, In4<T> in,
+                      Out4<T> out, int pad_copy, int nx, int ld, int ny, int nz, int kc, T cs, T dt, T dx, T dy,
+                      T dz) {
+  constexpr int TY = 256 / TX, HX = TX + 2, HP = HX * (TY + 2);
+  const int kb = blockIdx.z * kc, ke = min(kb + kc, nz);
+  __shared__ T tile[3][NF][HP];
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, x = x0 + lx, y = y0 + ly;
+  const long pl = (long)ld * ny;
+  auto load_level = [&](int kk) {
+    const int sl = kk % 3;
+    for (int e = threadIdx.x; e < HP; e += 256) {
+      const int hy = e / HX, hx = e - hy * HX;
+      const int gx = (x0 + hx - 1 + nx) % nx, gy = (y0 + hy - 1 + ny) % ny;
+      const long gi = (long)kk * pl + (long)gy * ld + gx;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) tile[sl][f][e] = in.p[f][gi];
+    }
+  };
+  if (kb >= 1) load_level(kb - 1);
+  load_level(kb);
+  if (kb + 1 < nz) load_level(kb + 1);
+  const int c0 = (ly + 1) * HX + lx + 1;
+  T ulm = T(0), vlm = T(0);  // u_lag, v_lag of this column one level down
+  if (kb >= 1) {
+    const long im = (long)(kb - 1) * pl + (long)y * ld + x;
+    ulm = ul[im];
+    vlm = vl[im];
+  }
+  for (int k = kb; k < ke; ++k) {
+    __syncthreads();  // levels k - 1, k and k + 1 are in the ring
+    const long i = (long)k * pl + (long)y * ld + x;
+    const T ulk = ul[i], vlk = vl[i];
+    T uu = ulk / cs, vv = vlk / cs;
+    if (k >= 1) {
+      uu = T(0.5) * (uu + ulm / cs);
+      vv = T(0.5) * (vv + vlm / cs);
+    }
+    ulm = ulk;
+    vlm = vlk;
+    const T xp = -uu * dt * cs / dx;
+    const T yp = -vv * dt * cs / dy;
+    T zp;
+    if (SIM && k <= 1) {
+      const long i2 = (long)pl + (long)y * ld + x;
+      if (k == 1) {
+        const T z2 = -(wl[i2] / cs) * dt * cs / dz;
+        zp = z2 + (z2 < T(0) ? z2 : T(0));
+      } else {
+        const T z1 = -(T(0.25) * (wl[i2] / cs)) * dt * cs / dz;
+        zp = z1 + (z1 > T(0) ? z1 : T(0));
+      }
+    } else if (k <= 1) {
+      const long i2 = (long)pl + (long)y * ld + x;
+      T z2 = -(wl[i2] / cs) * dt * cs / dz;
+      z2 = z2 + (z2 < T(0) ? z2 : T(0));
+      z2 = z2 < T(1) ? z2 : T(1);
+      z2 = z2 > T(-1) ? z2 : T(-1);
+      zp = z2;
+    } else {
+      zp = -(wl[i] / cs) * dt * cs / dz;
+    }
+    const int ax = signbit(xp) ? -1 : 1, ay = signbit(yp) ? -1 : 1, az = signbit(zp) ? -1 : 1;
+    int za = k + az;
+    za = za < 0 ? 0 : (za > nz - 1 ? nz - 1 : za);
+    const T cxv = fabs(xp), cyv = fabs(yp), czv = fabs(zp);
+    const T fx = T(1) - cxv, fy = T(1) - cyv, fz = T(1) - czv;
+    const int cy = c0 + ay * HX, cx = c0 + ax, cxy = cy + ax;
+    const int sk = k % 3, sz = za % 3;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const T* G = tile[sk][f];
+      const T* Z = tile[sz][f];
+      out.p[f][i] = fx * fy * (G[c0] * fz + Z[c0] * czv) + fx * cyv * (G[cy] * fz + Z[cy] * czv) +
+                    cxv * fy * (G[cx] * fz + Z[cx] * czv) + cxv * cyv * (G[cxy] * fz + Z[cxy] * czv);
+    }
+    if (pad_copy && x0 + TX == nx) {  // the pad columns of this tile's rows, as they are
+      const int npad = ld - nx;
+      for (int e = threadIdx.x; e < npad * TY; e += 256) {
+        const int r = e / npad, xo = e - r * npad;
+        const long ip = (long)k * pl + (long)(y0 + r) * ld + nx + xo;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) out.p[f][ip] = in.p[f][ip];
+      }
+    }
+    __syncthreads();  // level k - 1's slot is free
+    if (k + 2 < nz && k + 2 <= ke) load_level(k + 2);
+  }
+}
+
+// k_interpolag over the grid: the column walk where the tile divides the
+// grid, the pointwise kernel elsewhere
+template <typename T, int NF, bool SIM>
+static void launch_interpolag(les_ctx* c, const T* ul, const T* vl, const T* wl, const In4<T>& in,
+                              const Out4<T>& out, int pad_copy) {
+  const Grid& g = c->g;
+  const T cs = T(lag_count(c)), dt = T(g.dt), dx = T(g.dx), dy = T(g.dy), dz = T(g.dz);
+  const int kc = 16, gz = (g.nz + kc - 1) / kc;  // runs of 16 levels: enough workgroups to fill the CUs
+  if (g.nx % 64 == 0 && g.ny % 4 == 0)
+    hipLaunchKernelGGL((k_interpolag_walk<T, NF, SIM, 64>), dim3(g.nx / 64, g.ny / 4, gz), dim3(256), 0, c->stream,
+                       ul, vl, wl, in, out, pad_copy, g.nx, g.ld, g.ny, g.nz, kc, cs, dt, dx, dy, dz);
+  else if (g.nx % 32 == 0 && g.ny % 8 == 0)
+    hipLaunchKernelGGL((k_interpolag_walk<T, NF, SIM, 32>), dim3(g.nx / 32, g.ny / 8, gz), dim3(256), 0, c->stream,
+                       ul, vl, wl, in, out, pad_copy, g.nx, g.ld, g.ny, g.nz, kc, cs, dt, dx, dy, dz);
+  else
+    hipLaunchKernelGGL((k_interpolag<T, NF, SIM>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream, ul,
+                       vl, wl, in, out, pad_copy, g.nx, g.ld, g.ny, g.nz, cs, dt, dx, dy, dz);
+}
+
 // interpolag_Sdep.f90:2-380 on (F_LM, F_MM, F_QN, F_NN) with u_lag; the same
 // scheme serves interpolag_scalar_Sdep.f90:1-316 on G_* with u_lag_s
 template <typename T>
@@ -888,9 +1006,7 @@ void op_interpolag4(les_ctx* c, T* const F[4], T* ul, T* vl, T* wl) {
     ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
     const In4<T> in{{FF, FF + n3, FF + 2 * n3, FF + 3 * n3}};
     const Out4<T> out{{F[0], F[1], F[2], F[3]}};
-    hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)ul, (const T*)vl, (const T*)wl, in, out, 0, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)),
-                       T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+    launch_interpolag<T, 4, false>(c, ul, vl, wl, in, out, 0);
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, ul, n3, T(0));
@@ -910,9 +1026,7 @@ T* op_interpolag4_into(les_ctx* c, const T* const F[4], T* ul, T* vl, T* wl) {
     ProfScope _ps(c, "interpolag", (4.0 + 3 + 4) * n3 * sizeof(T));
     const In4<T> in{{F[0], F[1], F[2], F[3]}};
     const Out4<T> out{{Fo, Fo + n3, Fo + 2 * n3, Fo + 3 * n3}};
-    hipLaunchKernelGGL((k_interpolag<T>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)ul, (const T*)vl, (const T*)wl, in, out, 1, g.nx, g.ld, g.ny, g.nz, T(lag_count(c)),
-                       T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+    launch_interpolag<T, 4, false>(c, ul, vl, wl, in, out, 1);
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, ul, n3, T(0));
@@ -941,9 +1055,7 @@ void op_interpolag_ssim(les_ctx* c, const SgsPtrs& s) {
     ProfScope _ps(c, "interpolag", (2.0 + 3 + 2) * n3 * sizeof(T));
     const In4<T> in{{FF, FF + n3, nullptr, nullptr}};
     const Out4<T> out{{(T*)s.F_LM, (T*)s.F_MM, nullptr, nullptr}};
-    hipLaunchKernelGGL((k_interpolag<T, 2, true>), grid1((size_t)g.nx * g.ny * g.nz), dim3(256), 0, c->stream,
-                       (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, in, out, 0, g.nx, g.ld, g.ny, g.nz,
-                       T(lag_count(c)), T(g.dt), T(g.dx), T(g.dy), T(g.dz));
+    launch_interpolag<T, 2, true>(c, (const T*)s.u_lag, (const T*)s.v_lag, (const T*)s.w_lag, in, out, 0);
   }
   LCHK(c, hipGetLastError());
   op_fill<T>(c, (T*)s.u_lag, n3, T(0));
