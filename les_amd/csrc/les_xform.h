@@ -738,8 +738,17 @@ using ColGeomT = ColGeom<TX, NYI, NYO, sizeof(T) == 4>;
 // unpadd.f90) resampled in place in the input tile: the rows that move never
 // overlap their destinations, so the tile needs max(NYI, NYO) rows and no
 // second buffer (more workgroups per CU, the same values).
-template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false>
+// waves per SIMD the register allocator is asked for: FP32 same-length
+// passes of 512 threads fit four workgroups per CU (eight waves per SIMD) at
+// 64 registers, three at the 65 they take unasked; elsewhere no request
+template <typename T, int TX, int NYI, int NYO, bool IP, bool IN2>
+constexpr int col_wpe() {
+  return sizeof(T) == 4 && NYI > 0 && NYI == NYO && !IP && !IN2 && ColGeomT<T, TX, NYI, NYO>::BLOCK == 512 ? 8 : 1;
+}
+
+template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false, bool IN2 = false>
 __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 1024))
+    __attribute__((amdgpu_waves_per_eu(col_wpe<T, TX, NYI, NYO, IP, IN2>())))
     k_col(const cx<T>* __restrict__ in, const cx<T>* __restrict__ in2, cx<T>* o0, cx<T>* o1, cx<T>* o2, ColArgs a,
           FftPlan pin, FftPlan pout, const cx<T>* __restrict__ twin, const cx<T>* __restrict__ twout,
           const T* __restrict__ kxt, const T* __restrict__ kyt, const T* __restrict__ G0, const T* __restrict__ G1,
@@ -771,7 +780,8 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       fft_lds<T, true, 4>(buf, pout, TX, 1, TX, two, +1);
   };
   unsigned bx = blockIdx.x, by = blockIdx.y;
-  if constexpr (TX * sizeof(cx<T>) == 64) xcd_pair_tiles(bx, by);  // two tiles share 128-byte rows
+  if constexpr (TX * sizeof(cx<T>) == 64)
+    if (!a.in_tl) xcd_pair_tiles(bx, by);  // two row-major tiles share 128-byte rows
   const int kx0 = bx * TX;
   const long pl = by;
   // input tile -> LDS; with compile-time lengths every load of the tile is
@@ -838,15 +848,19 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
     constexpr int NE = (CT ? NYI : 1) * TX, ME = (NE + (CT ? BLK : 64) - 1) / (CT ? BLK : 64);
     cx<T>* R = two + nyo;
     cx<T> ra[ME], rb[ME];
+    // offset of tile element e = y TX + c in a plane of nk-coefficient rows
+    // of layout tl: with TX = 8 and tiles, the tile is contiguous
+    auto eoff = [&](int e, long nk, int ny_, int tl) -> long {
+      if (TX == 8 && tl) return (long)(kx0 >> 3) * (8L * ny_) + e;
+      const int y = e / TX, c = e - y * TX;
+      return xs_off(y, kx0 + c, ny_, nk, tl);
+    };
     auto fetch = [&](cx<T> (&r)[ME], const cx<T>* src) {
 #pragma unroll
       for (int i = 0; i < ME; ++i) {
         const int e = threadIdx.x + i * BLK;
         r[i] = cx<T>{T(0), T(0)};
-        if (NE % BLK == 0 || e < NE) {
-          const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-          if (kk < a.nki) r[i] = src[xs_off(y, kk, nyi, a.in_row, a.in_tl)];
-        }
+        if ((NE % BLK == 0 || e < NE) && kx0 + e % TX < a.nki) r[i] = src[eoff(e, a.in_row, nyi, a.in_tl)];
       }
     };
     auto put = [&](const cx<T> (&r)[ME]) {
@@ -864,14 +878,14 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       }
     };
     fetch(ra, in + pl * a.in_plane);
-    if (in2) fetch(rb, in2 + pl * a.in2_plane);
+    if (IN2 && in2) fetch(rb, in2 + pl * a.in2_plane);
     if (a.do_fwd) {
       put(ra);
       __syncthreads();
       fwd(R);
       get(ra);
     }
-    if (in2) {
+    if (IN2 && in2) {
       // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
       const int c0 = threadIdx.x % TX;
       const T kx = kx0 + c0 < a.nki ? kxt[kx0 + c0] : T(0);
@@ -894,10 +908,8 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       }
       __syncthreads();
       if (a.do_inv) inv(R);
-      for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
-        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-        if (kk < a.nko[0]) o0[pl * a.out_plane[0] + xs_off(y, kk, nyo, a.out_row[0], a.out_tl[0])] = R[e];
-      }
+      for (int e = threadIdx.x; e < NYO * TX; e += BLK)
+        if (kx0 + e % TX < a.nko[0]) o0[pl * a.out_plane[0] + eoff(e, a.out_row[0], nyo, a.out_tl[0])] = R[e];
       return;
     }
     // an i kx output beside an identity output is i kx times the identity
@@ -937,29 +949,26 @@ __global__ void __launch_bounds__((NYI > 0 ? ColGeomT<T, TX, NYI, NYO>::BLOCK : 
       __syncthreads();
       if (a.do_inv) inv(R);
       cx<T>* o = j == 0 ? o0 : (j == 1 ? o1 : o2);
-      for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
-        const int y = e / TX, c = e - y * TX, kk = kx0 + c;
-        if (kk < a.nko[j]) o[pl * a.out_plane[j] + xs_off(y, kk, nyo, a.out_row[j], a.out_tl[j])] = R[e];
-      }
+      for (int e = threadIdx.x; e < NYO * TX; e += BLK)
+        if (kx0 + e % TX < a.nko[j]) o[pl * a.out_plane[j] + eoff(e, a.out_row[j], nyo, a.out_tl[j])] = R[e];
       if (j == jid && jkx >= 0 && kx0 < a.nko[jkx]) {
         cx<T>* ox = jkx == 0 ? o0 : (jkx == 1 ? o1 : o2);
         const int c0 = threadIdx.x % TX;
         const T k = kx0 + c0 < a.nki ? kxt[kx0 + c0] : T(0);
         for (int e = threadIdx.x; e < NYO * TX; e += BLK) {
-          const int y = e / TX, kk = kx0 + c0;
           const cx<T> v = R[e];
-          if (kk < a.nko[jkx])
-            ox[pl * a.out_plane[jkx] + xs_off(y, kk, nyo, a.out_row[jkx], a.out_tl[jkx])] = cx<T>{-k * v.y, k * v.x};
+          if (kx0 + c0 < a.nko[jkx])
+            ox[pl * a.out_plane[jkx] + eoff(e, a.out_row[jkx], nyo, a.out_tl[jkx])] = cx<T>{-k * v.y, k * v.x};
         }
       }
     }
     return;
   }
   load_tile(A, in + pl * a.in_plane);
-  if (in2) load_tile(B, in2 + pl * a.in2_plane);
+  if (IN2 && in2) load_tile(B, in2 + pl * a.in2_plane);
   __syncthreads();
   if (a.do_fwd) fwd(A);
-  if (in2) {
+  if (IN2 && in2) {
     // divergence mode: out = i kx FFT(in) + i ky FFT(in2)
     if constexpr (CT) {
       // wavenumbers first (a thread keeps one column), then the transform
@@ -1067,9 +1076,18 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
   // registers) use one, the rest two
   constexpr bool ONE = NYI > 0 && (IP || NYI == NYO);
   const size_t sm = (size_t)(a.ny_in + a.ny_out + (ONE ? TX * nmax : TX * a.ny_in + TX * a.ny_out)) * sizeof(cx<T>);
-  if (sm > 65536)
-    LCHK(c, hipFuncSetAttribute((const void*)k_col<T, TX, NYI, NYO, PLAN, IP>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  // the divergence mode (a second input) only with equal lengths; its own
+  // instantiation, so that the one-input kernel does not reserve the second
+  // input's registers (65 -> 48 VGPRs at TX 8, n 256: five workgroups per CU
+  // instead of three)
+  constexpr bool CAN2 = NYI == NYO && !IP;
+  if (!CAN2 && in2) {
+    c->err = "k_col: a second input needs equal lengths";
+    return;
+  }
+  const void* kern = CAN2 && in2 ? (const void*)k_col<T, TX, NYI, NYO, PLAN, IP, CAN2>
+                                 : (const void*)k_col<T, TX, NYI, NYO, PLAN, IP, false>;
+  if (sm > 65536) LCHK(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   const Tables<T>& t = tables<T>(c);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   double bytes = 0;
@@ -1079,9 +1097,14 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
     for (int j = 0; j < (in2 ? 1 : a.nout); ++j) bytes += (double)nplanes * a.ny_out * a.nko[j] * es;
   }
   ProfScope ps(c, a.ny_in < a.ny_out ? "col_pad" : (a.ny_in > a.ny_out ? "col_trunc" : "col"), bytes);
-  hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN, IP>), dim3(gx, (unsigned)nplanes), dim3(threads), sm, c->stream, in, in2,
-                     o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out), tw_for<T>(c, a.ny_in),
-                     tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
+  if (CAN2 && in2)
+    hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN, IP, CAN2>), dim3(gx, (unsigned)nplanes), dim3(threads), sm,
+                       c->stream, in, in2, o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out),
+                       tw_for<T>(c, a.ny_in), tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
+  else
+    hipLaunchKernelGGL((k_col<T, TX, NYI, NYO, PLAN, IP, false>), dim3(gx, (unsigned)nplanes), dim3(threads), sm,
+                       c->stream, in, nullptr, o0, o1, o2, a, plan_for(c, a.ny_in), plan_for(c, a.ny_out),
+                       tw_for<T>(c, a.ny_in), tw_for<T>(c, a.ny_out), t.kx, t.ky, G0, G1, G2);
   LCHK(c, hipGetLastError());
 }
 
