@@ -740,10 +740,12 @@ using ColGeomT = ColGeom<TX, NYI, NYO, sizeof(T) == 4>;
 // second buffer (more workgroups per CU, the same values).
 // waves per SIMD the register allocator is asked for: FP32 same-length
 // passes of 512 threads fit four workgroups per CU (eight waves per SIMD) at
-// 64 registers, three at the 65 they take unasked; elsewhere no request
+// 64 registers, three at the 65 they take unasked (a two-register spill is
+// the cheaper side: filt_da's columns 0.198 -> 0.186, divstress's 0.086 ->
+// 0.078 ms per step); elsewhere no request
 template <typename T, int TX, int NYI, int NYO, bool IP, bool IN2>
 constexpr int col_wpe() {
-  return sizeof(T) == 4 && NYI > 0 && NYI == NYO && !IP && !IN2 && ColGeomT<T, TX, NYI, NYO>::BLOCK == 512 ? 8 : 1;
+  return sizeof(T) == 4 && NYI > 0 && NYI == NYO && !IP && ColGeomT<T, TX, NYI, NYO>::BLOCK == 512 ? 8 : 1;
 }
 
 template <typename T, int TX, int NYI, int NYO, int PLAN = 0, bool IP = false, bool IN2 = false>
@@ -1076,10 +1078,8 @@ static void launch_col_n(les_ctx* c, const cx<T>* in, const cx<T>* in2, cx<T>* o
   // registers) use one, the rest two
   constexpr bool ONE = NYI > 0 && (IP || NYI == NYO);
   const size_t sm = (size_t)(a.ny_in + a.ny_out + (ONE ? TX * nmax : TX * a.ny_in + TX * a.ny_out)) * sizeof(cx<T>);
-  // the divergence mode (a second input) only with equal lengths; its own
-  // instantiation, so that the one-input kernel does not reserve the second
-  // input's registers (65 -> 48 VGPRs at TX 8, n 256: five workgroups per CU
-  // instead of three)
+  // the divergence mode (a second input) only with equal lengths, in its own
+  // instantiation
   constexpr bool CAN2 = NYI == NYO && !IP;
   if (!CAN2 && in2) {
     c->err = "k_col: a second input needs equal lengths";
