@@ -2242,6 +2242,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
         cp[0] = z;
     }
     __syncthreads();
+    // wavefronts 6 and 7 carry the longest work of this phase (four passes)
+    if (w >= NP) __builtin_amdgcn_s_setprio(1);
     if (w < NP) {
       if (k + 2 <= k1) {  // level k + 2 is read by the products of level k + 1
         cx<T>* row = cur + w * RS;
@@ -2264,6 +2266,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PadWav
       finish_store(O + RS, S_T, k, S_Q, k);
       if ((c & 1) == 0 && c >= 2) finish_store(C + (((c >> 1) + 1) & 1) * RS, S_CZ, k - 2, S_CZ, k - 1);
     }
+    if (w >= NP) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
   }
   // the C rows whose passes the run's last steps did not reach: completed at
